@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Benchmark of the large-step dynamics hot path on one MI355X.
+
+Workload: the moist supercell case (cases/cfg2_supercell.ini): 120 x 120 x 40
+grid points, dx = 1 km, geometric levels to 20 km, WK82 sounding, quarter-
+circle hodograph, Kessler microphysics, open boundaries, dt = 6 s, 6 acoustic
+substeps. A "step" is one large RK3 step (3 stages, 1 + 3 + 6 acoustic
+substeps, Kessler). Inputs are resident in HBM before the timed region.
+
+Prints one JSON line: grid-point·steps/s of the mixed-precision path (value),
+the plain-FP64 path beside it, the relative error of the mixed path against
+the FP64 path after the same steps, the whole-step and dominant-kernel
+roofline against 8 TB/s HBM3E, and the CPU oracle timed on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
+CASE = ROOT / "cases" / "cfg2_supercell.ini"
+
+# Algorithmic HBM bytes per interior point and large step: every phase that
+# is separated by a global dependency reads each distinct 3-D input once and
+# writes each output once, at its storage width (mass fluxes/divergence,
+# halos, 1-D/2-D arrays not counted). Moist + Kessler, ns = 6 (10 substeps):
+# 339 accesses; mixed: pi', pi'prev 8 B, the rest 4 B.
+ALG_BYTES_STEP = {"mixed": 1520, "fp64": 2712}
+
+# Per-kernel algorithmic bytes per interior point per launch (distinct arrays
+# the kernel must read and write, moist case).
+#  acou_wpi: r u v w thp tw tth thbw dthbdz rtjx rtjy kt3 aw qv qc qr qvb (16)
+#            + pip (8 B); w thp (4 B) + pip pipprev (8 B) written.
+#  acou_uv : r u v thp qv tu tv thb (7) + pip pipprev (8 B); w u v.
+KERNEL_BYTES = {
+    "mixed": {"acou_wpi": 16 * 4 + 8 + 2 * 4 + 2 * 8, "acou_uv": 7 * 4 + 2 * 8 + 2 * 4},
+    "fp64": {"acou_wpi": 21 * 8, "acou_uv": 11 * 8},
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def timed_run(case, precision: str, warmup: int, steps: int):
+    from wfe_amd import Integrator
+
+    gi = Integrator.from_case(case, precision)
+    dt = case.scalar("dt")
+    for s in range(warmup):
+        gi.step(dt, s * dt)
+    gi.sync()
+    t0 = time.perf_counter()
+    for s in range(warmup, warmup + steps):
+        gi.step(dt, s * dt)
+    gi.sync()
+    t1 = time.perf_counter()
+    return gi, t1 - t0
+
+
+def kernel_profile(case, precision: str, warmup: int, steps: int):
+    """HIP-event timing of every launch on the context's stream."""
+    from wfe_amd import Integrator
+
+    gi = Integrator.from_case(case, precision)
+    dt = case.scalar("dt")
+    for s in range(warmup):
+        gi.step(dt, s * dt)
+    gi.kernel_timing(2)
+    for s in range(warmup, warmup + steps):
+        gi.step(dt, s * dt)
+    rep = gi.kernel_report()
+    gi.kernel_timing(0)
+    return rep
+
+
+def cpu_baseline(case, budget_s: float):
+    """The FP64 oracle on the host cores, on a bounded sample of steps."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from oracle_lib import OracleModel, olib
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    olib().orc_set_threads(threads)
+    om = OracleModel.from_case(case)
+    dt = case.scalar("dt")
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        om.step(dt, n * dt)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or n >= 400:
+            break
+    return n, el, threads
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=30)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is described, not measured)"}))
+        return 2
+
+    from wfe_amd import Case
+
+    case = Case(CASE)
+    g = case.grid
+    N = g.n_interior
+    K, W = args.steps, args.warmup
+
+    log(f"[bench] mixed: {W} warmup + {K} timed steps on {g.nx}x{g.ny}x{g.nz}")
+    gi_m, t_m = timed_run(case, "mixed", W, K)
+    log(f"[bench] mixed {t_m:.3f} s -> {1e3 * t_m / K:.3f} ms/step")
+    log(f"[bench] fp64: {W} warmup + {K} timed steps")
+    gi_d, t_d = timed_run(case, "fp64", W, K)
+    log(f"[bench] fp64 {t_d:.3f} s -> {1e3 * t_d / K:.3f} ms/step")
+
+    # Error of the mixed path against the plain FP64 path after W + K steps.
+    errs = {}
+    for f in ("u", "v", "w", "thp", "pip", "qv", "qc", "qr"):
+        nzl = g.nz + 1 if f == "w" else g.nz
+        a = g.interior(gi_m.get(f), nzl)
+        b = g.interior(gi_d.get(f), nzl)
+        nb = float(np.linalg.norm(b))
+        errs[f] = float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a))
+    del gi_m, gi_d
+
+    value = N * K / t_m
+    value_fp64 = N * K / t_d
+    log(f"[bench] kernel profile: {args.profile_steps} steps")
+    rep = kernel_profile(case, "mixed", 5, args.profile_steps)
+    dom = max(rep, key=lambda k: rep[k][1])
+    cnt, tot_ms = rep[dom]
+    avg_s = tot_ms / cnt / 1e3
+    kb = KERNEL_BYTES["mixed"].get(dom)
+    roof = None
+    if kb is not None:
+        ach = kb * N / avg_s
+        roof = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": ach / HBM_PEAK, "traffic": None, "kernel": dom,
+                "avg_launch_us": avg_s * 1e6, "launches": cnt,
+                "alg_bytes_per_launch": kb * N}
+    step_bw = ALG_BYTES_STEP["mixed"] * N * K / t_m
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"[bench] cpu baseline: oracle for ~{args.cpu_budget:.0f} s")
+        n, el, thr = cpu_baseline(case, args.cpu_budget)
+        cpu = {"value": N * n / el, "unit": "grid-pt·steps/s", "cores": thr, "kind": "port",
+               "sample": f"FP64 oracle, cfg2 supercell 120x120x40 from the initial state, "
+                         f"{n} large steps in {el:.1f} s on {thr} OpenMP threads"}
+
+    out = {
+        "metric": "grid-pt·steps/s FP32-mixed vs FP64 plain at stated error; % HBM peak",
+        "value": value,
+        "unit": "grid-pt·steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": 1e3 * t_m / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (pi' and column solve f64)",
+        "data": "synthetic (analytic WK82 sounding, quarter-circle hodograph, +3 K bubble)",
+        "config": {"workload": "moist supercell with Kessler, 120x120x40, dx=1 km, "
+                               "stretched to 20 km, dt=6 s, ns=6",
+                   "grid": [g.nx, g.ny, g.nz], "precision": "mixed"},
+        "fp64_plain": {"value": value_fp64, "ms_per_step": 1e3 * t_d / K,
+                       "speedup_mixed_vs_fp64": value / value_fp64},
+        "rel_err_vs_fp64": errs,
+        "step_roofline": {"alg_bytes_per_pt_step": ALG_BYTES_STEP["mixed"],
+                          "achieved_GBps": step_bw / 1e9, "frac": step_bw / HBM_PEAK},
+        "roofline": roof,
+        "kernels_ms": {k: {"launches": v[0], "total_ms": v[1], "avg_us": 1e3 * v[1] / v[0]}
+                       for k, v in sorted(rep.items(), key=lambda kv: -kv[1][1])},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,205 @@
+/*
+ * wfe_hip.h -- C ABI of the MI355X (gfx950) dynamics hot path.
+ *
+ * Plain C: POD structs, host pointers and sizes, int status codes. Every
+ * function returns WFE_OK (0) on success or a negative WFE_E* code; the text
+ * of the last error on the calling thread is in wfe_last_error().
+ *
+ * All 3-D host arrays use the reference's padded layout
+ *   idx(i,j,k) = ((k+ng)*NY + (j+ng))*NX + (i+ng),
+ *   NX = nx+2ng, NY = ny+2ng, NZ = nz+1+2ng   (src/core/grid.hpp:22-36),
+ * as float64 on the host whatever the device precision. 2-D arrays are
+ * NX*NY (padded), 1-D vertical arrays NZ (index k+ng).
+ *
+ * The reference has no foreign-function interface; each entry point below
+ * names the C++ interface of the reference it replaces (file:line in the
+ * reference tree).
+ */
+#ifndef WFE_HIP_H
+#define WFE_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WFE_HIP_ABI_VERSION 1
+
+enum {
+  WFE_OK = 0,
+  WFE_E_ARG = -1,     /* invalid argument, size or state */
+  WFE_E_HIP = -2,     /* HIP runtime error */
+  WFE_E_CONFIG = -3,  /* case configuration error */
+  WFE_E_IO = -4,      /* file read/write error */
+  WFE_E_NOMEM = -5    /* allocation failure */
+};
+
+/* Device precision of a context. */
+enum {
+  WFE_PREC_FP64 = 0,  /* everything stored and computed in FP64 ("plain FP64") */
+  WFE_PREC_MIXED = 1  /* FP32 storage and bulk arithmetic; pi' stored in FP64;
+                         acoustic column assembly and tridiagonal elimination,
+                         pi = pibar + pi' and the pi' gradient in FP64 */
+};
+
+/* Field selectors for upload/download/absmax. */
+enum {
+  WFE_F_U = 0, WFE_F_V, WFE_F_W, WFE_F_THP, WFE_F_PIP, WFE_F_QV, WFE_F_QC, WFE_F_QR,
+  WFE_F_TU = 8, WFE_F_TV, WFE_F_TW, WFE_F_TTHP, WFE_F_TPIP, WFE_F_TQV, WFE_F_TQC, WFE_F_TQR,
+  WFE_F_RAIN = 16, /* 2-D accumulated surface rain [kg m-2] */
+  WFE_F_COUNT = 17
+};
+
+/* GDims (src/core/grid.hpp:22-36). ng must be 3. */
+typedef struct {
+  int nx, ny, nz, ng;
+  double dx, dy, dz;
+} wfe_grid;
+
+/* DynParams (src/dynamics/params.hpp:8-25). Booleans are 0/1. */
+typedef struct {
+  double diff_K;
+  double coriolis_f;
+  int coriolis_use_ub;
+  double rayleigh_zd;
+  double rayleigh_alpha;
+  int acoustic_ns;
+  double acoustic_beta;
+  double acoustic_smdiv;
+  int bc_x_open;
+  int bc_y_open;
+  double cstar;
+  int moisture;
+  int microphysics;
+  int pd_moist;
+  int w_damping;
+  double w_abort;
+} wfe_params;
+
+/* DevProf (src/core/base_state.hpp:16-27): host FP64 arrays. 3-D fields are
+ * padded (NX*NY*NZ); ub and vb are 1-D [NZ]. vb is an additive extension of
+ * the reference (base v wind); NULL means zero. */
+typedef struct {
+  const double *thb, *pib, *rhob, *dthbdz, *thbw, *rhobw, *thvb, *thvbw, *qvb;
+  const double *ub, *vb;
+} wfe_base;
+
+/* DevMetric (src/core/metric.hpp:16-30): host FP64 arrays. */
+typedef struct {
+  const double *zeta_w, *zeta_c, *dzeta_c, *dzeta_w;             /* [NZ]    */
+  const double *h, *hx, *hy, *hx_u, *hy_v, *jac, *fcor, *mapf;   /* [NX*NY] */
+  double zt;
+} wfe_metric;
+
+typedef struct wfe_ctx wfe_ctx;
+
+/* ---- integrator context: Integrator (src/dynamics/integrator.hpp:18-42) ---- */
+
+/* Integrator::init (integrator.cpp:43-69): allocates device state, uploads
+ * base state and metric, precomputes the acoustic coefficients
+ * (kernels.cu:915-927). The state starts at zero. */
+int wfe_create(const wfe_grid* g, const wfe_params* p, const wfe_base* base,
+               const wfe_metric* m, int precision, wfe_ctx** out);
+int wfe_destroy(wfe_ctx* ctx);
+
+/* Integrator::step (integrator.cpp:71-111): one WS2002 RK3 large step with
+ * split-explicit acoustic substeps, moisture stage updates, boundary
+ * relaxation (when set) and Kessler microphysics (when moisture and
+ * microphysics are on). t is the simulation time at the start of the step;
+ * it is not used by the device path (the boundary time fraction is set with
+ * wfe_set_bdy_tfrac). Asynchronous with respect to the host. */
+int wfe_step(wfe_ctx* ctx, double dt, double t);
+
+/* Field3D::upload / download (src/core/field3d.hpp:42-47) of the time-n
+ * state (Integrator::state(), integrator.hpp:24), of the slow tendencies, or
+ * of the 2-D rain (Integrator::rain(), integrator.hpp:25). n is the element
+ * count (npts, or NX*NY for WFE_F_RAIN). Synchronous. */
+int wfe_upload(wfe_ctx* ctx, int field, const double* host, size_t n);
+int wfe_download(wfe_ctx* ctx, int field, double* host, size_t n);
+
+/* field_absmax (kernels.hpp:69-71, kernels.cu:967-979): NaN-safe max|f| over
+ * the interior (w: levels 0..nz). NaN/Inf report +Inf. Synchronous. */
+int wfe_absmax(wfe_ctx* ctx, int field, float* out);
+
+/* cudaDeviceSynchronize as used by the driver (main.cpp:441,468). */
+int wfe_sync(wfe_ctx* ctx);
+
+/* ---- boundary relaxation: BdyManager (src/dynamics/boundary.hpp:16-38) ---- */
+
+/* BdyManager::init weights (boundary.cpp:36-54): 2-D relaxation coefficient
+ * [1/s] on the padded NX*NY grid; activates relaxation in wfe_step. */
+int wfe_set_bdy_weights(wfe_ctx* ctx, const double* wgt, size_t n);
+/* BdyManager::load_into (boundary.cpp:51-81): uploads relaxation target
+ * var (0 u, 1 v, 2 theta', 3 pi', 4 qv) into slot 0 (lo) or 1 (hi). */
+int wfe_set_bdy_target(wfe_ctx* ctx, int slot, int var, const double* host, size_t n);
+/* BdyManager::update's lo <- hi swap (boundary.cpp:83-91). */
+int wfe_swap_bdy_targets(wfe_ctx* ctx);
+/* BdyManager::tfrac (boundary.cpp:93-97) for the next steps. */
+int wfe_set_bdy_tfrac(wfe_ctx* ctx, double tfrac);
+
+/* ---- phase-level entry points: free functions of kernels.hpp:14-71 ---- */
+
+/* apply_bcs (kernels.cu:805-835) on the time-n state. */
+int wfe_apply_bcs(wfe_ctx* ctx);
+/* compute_mass_fluxes + compute_divergence + compute_tendencies
+ * (kernels.cu:837-913) from the time-n state into the tendency fields,
+ * after apply_bcs; dt is the large step (PD limiter, w damping). */
+int wfe_compute_tendencies(wfe_ctx* ctx, double dt);
+/* nsub successive acoustic_substep calls (kernels.cu:929-965) started from
+ * the time-n state with the current tendencies, exactly as one RK3 stage
+ * runs them (fast system restarted from time n, pi'prev = pi'(n));
+ * the result replaces the time-n state. */
+int wfe_acoustic_substeps(wfe_ctx* ctx, double dtau, int nsub);
+/* kessler_step (src/physics/kessler.hpp:17-18) on the time-n state. */
+int wfe_kessler(wfe_ctx* ctx, double dt);
+
+/* ---- measurement (the reference's WFE_PROF section timer,
+ *      src/dynamics/integrator.cpp:18-41, at kernel granularity) ---- */
+
+/* mode 1: record a HIP event pair on the context's stream around every
+ * kernel launch; 0: stop; 2: clear the totals and start. */
+int wfe_kernel_timing(wfe_ctx* ctx, int mode);
+/* Per-kernel totals as text lines "name launches total_ms\n" (synchronous). */
+int wfe_kernel_timing_report(wfe_ctx* ctx, char* buf, size_t n);
+
+/* ---- introspection ---- */
+const char* wfe_last_error(void);
+int wfe_abi_version(void);
+/* Device name of the context's GPU into buf (for run_info.txt, main.cpp:404). */
+int wfe_device_name(char* buf, size_t n);
+
+/* ---- host case builder: Config + Metric + BaseState + initial conditions
+ *      exactly as src/main.cpp:248-383 assembles them ---- */
+
+typedef struct wfe_case wfe_case;
+
+/* Loads an .ini case (Config::load, config.cpp:17-31) with "key=value"
+ * overrides (main.cpp:254-259), builds the grid, DynParams, Metric,
+ * BaseState, file-mode input and the initial state. */
+int wfe_case_load(const char* ini_path, int n_over, const char* const* over, wfe_case** out);
+int wfe_case_destroy(wfe_case* c);
+int wfe_case_grid(const wfe_case* c, wfe_grid* g);
+int wfe_case_params(const wfe_case* c, wfe_params* p);
+/* Run keys: dt, t_end, out_interval, diag_interval, zt. */
+int wfe_case_scalar(const wfe_case* c, const char* name, double* out);
+/* Element count of a named host array, or 0 if unknown. Names: base fields
+ * (thb pib rhob dthbdz thbw rhobw thvb thvbw qvb ub vb), metric fields
+ * (zeta_w zeta_c dzeta_c dzeta_w h hx hy hx_u hy_v jac fcor mapf), initial
+ * state (u v w thp pip qv qc qr), boundary (bdy_wgt, bdy_lo_<v>, bdy_hi_<v>
+ * for v in u v thp pip qv). */
+size_t wfe_case_array_size(const wfe_case* c, const char* name);
+int wfe_case_array(const wfe_case* c, const char* name, double* out, size_t n);
+/* Integrator::init + IC upload (+ BdyManager::init in file mode). */
+int wfe_case_create_ctx(const wfe_case* c, int precision, wfe_ctx** out);
+/* BdyManager::update(t) then tfrac(t) for the context (boundary.cpp:83-97),
+ * reading further boundary files when t passes the next boundary time. */
+int wfe_case_update_bdy(wfe_case* c, wfe_ctx* ctx, double t);
+/* Unused-key list (Config::unused, config.cpp:51-60), newline-separated. */
+int wfe_case_unused_keys(const wfe_case* c, char* buf, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WFE_HIP_H */

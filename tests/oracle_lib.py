@@ -1,0 +1,161 @@
+"""ctypes view of the CPU FP64 oracle (oracle/liboracle.so) for the tests.
+
+Test infrastructure only: it checks the HIP path; it is never the product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+ORACLE_PATH = Path(__file__).resolve().parent.parent / "oracle" / "liboracle.so"
+_DP = C.POINTER(C.c_double)
+_o = None
+
+STATE = ("u", "v", "w", "thp", "pip", "qv", "qc", "qr")
+BASE = ("thb", "pib", "rhob", "dthbdz", "thbw", "rhobw", "thvb", "thvbw", "qvb", "ub", "vb")
+METRIC = ("zeta_w", "zeta_c", "dzeta_c", "dzeta_w", "h", "hx", "hy", "hx_u", "hy_v", "jac",
+          "fcor", "mapf")
+PARAMS = ("diff_K", "coriolis_use_ub", "rayleigh_zd", "rayleigh_alpha", "acoustic_ns",
+          "acoustic_beta", "acoustic_smdiv", "bc_x_open", "bc_y_open", "cstar", "moisture",
+          "microphysics", "pd_moist", "w_damping")
+
+
+def olib() -> C.CDLL:
+    global _o
+    if _o is None:
+        if not ORACLE_PATH.exists():
+            raise ImportError(f"{ORACLE_PATH} missing: run `python -m wfe_amd.build_ext`")
+        L = C.CDLL(str(ORACLE_PATH))
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.c_int] * 4 + [C.c_double] * 4
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_set_param.argtypes = [C.c_void_p, C.c_char_p, C.c_double]
+        L.orc_set_array.argtypes = [C.c_void_p, C.c_char_p, _DP, C.c_long]
+        L.orc_get_array.argtypes = [C.c_void_p, C.c_char_p, _DP, C.c_long]
+        for n in ("orc_init", "orc_apply_bcs"):
+            getattr(L, n).argtypes = [C.c_void_p]
+        L.orc_set_bdy.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.orc_step.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.orc_tendencies.argtypes = [C.c_void_p, C.c_double]
+        L.orc_acoustic.argtypes = [C.c_void_p, C.c_double, C.c_int]
+        L.orc_kessler.argtypes = [C.c_void_p, C.c_double]
+        L.orc_absmax.restype = C.c_double
+        L.orc_absmax.argtypes = [C.c_void_p, C.c_char_p]
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_iface5.restype = C.c_double
+        L.orc_iface5.argtypes = [C.c_double] * 7
+        L.orc_thomas.argtypes = [C.c_int, _DP, _DP, _DP, _DP, _DP]
+        L.orc_qsat_w.restype = C.c_double
+        L.orc_qsat_w.argtypes = [C.c_double, C.c_double]
+        L.orc_qsat_i.restype = C.c_double
+        L.orc_qsat_i.argtypes = [C.c_double, C.c_double]
+        _o = L
+    return _o
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(_DP)
+
+
+class OracleModel:
+    """Oracle instance fed the same FP64 arrays as the GPU context."""
+
+    def __init__(self, grid, zt: float, params: dict, arrays: dict, state: dict | None = None):
+        L = olib()
+        self.grid = grid
+        self._h = C.c_void_p(L.orc_create(grid.nx, grid.ny, grid.nz, grid.ng, grid.dx, grid.dy,
+                                          grid.dz, zt))
+        if not self._h.value:
+            raise ValueError("orc_create rejected the grid")
+        for k in PARAMS:
+            if k in params:
+                assert L.orc_set_param(self._h, k.encode(), float(params[k])) == 0, k
+        for k in BASE + METRIC:
+            if k in arrays:
+                self.set(k, arrays[k])
+        if state:
+            for k, v in state.items():
+                self.set(k, v)
+        L.orc_init(self._h)
+
+    @classmethod
+    def from_case(cls, case) -> "OracleModel":
+        arrays = {k: case.array(k) for k in BASE + METRIC}
+        state = {k: case.array(k) for k in STATE}
+        om = cls(case.grid, case.scalar("zt"), case.param_dict(), arrays, state)
+        try:
+            wgt = case.array("bdy_wgt")
+        except KeyError:
+            wgt = None
+        if wgt is not None and wgt.size:
+            om.set("bdy_wgt", wgt)
+            for v, n in enumerate(("u", "v", "thp", "pip", "qv")):
+                om.set(f"bdy_lo_{v}", case.array(f"bdy_lo_{n}"))
+                om.set(f"bdy_hi_{v}", case.array(f"bdy_hi_{n}"))
+            om.set_bdy(True, 0.0)
+        return om
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value:
+            olib().orc_destroy(h)
+            self._h = None
+
+    def _size(self, name: str) -> int:
+        g = self.grid
+        if name in ("ub", "vb", "zeta_w", "zeta_c", "dzeta_c", "dzeta_w"):
+            return g.NZ
+        if name in ("h", "hx", "hy", "hx_u", "hy_v", "jac", "fcor", "mapf", "rain", "bdy_wgt"):
+            return g.NX * g.NY
+        return g.npts
+
+    def set(self, name: str, a) -> None:
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert olib().orc_set_array(self._h, name.encode(), _p(a), a.size) == 0, name
+
+    def get(self, name: str) -> np.ndarray:
+        a = np.empty(self._size(name), dtype=np.float64)
+        assert olib().orc_get_array(self._h, name.encode(), _p(a), a.size) == 0, name
+        return a
+
+    def set_bdy(self, active: bool, tfrac: float) -> None:
+        olib().orc_set_bdy(self._h, int(active), float(tfrac))
+
+    def step(self, dt: float, t: float = 0.0) -> None:
+        olib().orc_step(self._h, dt, t)
+
+    def run(self, dt: float, n: int) -> None:
+        for s in range(n):
+            self.step(dt, s * dt)
+
+    def apply_bcs(self):
+        olib().orc_apply_bcs(self._h)
+
+    def tendencies(self, dt: float):
+        olib().orc_tendencies(self._h, dt)
+
+    def acoustic(self, dtau: float, nsub: int):
+        olib().orc_acoustic(self._h, dtau, nsub)
+
+    def kessler(self, dt: float):
+        olib().orc_kessler(self._h, dt)
+
+    def absmax(self, name: str) -> float:
+        return olib().orc_absmax(self._h, name.encode())
+
+
+def set_threads(n: int) -> None:
+    olib().orc_set_threads(int(n))
+
+
+def iface5(*a) -> float:
+    return olib().orc_iface5(*[float(x) for x in a])
+
+
+def thomas(a, b, c, d) -> np.ndarray:
+    arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (a, b, c, d)]
+    x = np.empty_like(arrs[3])
+    olib().orc_thomas(len(x), *[_p(t) for t in arrs], _p(x))
+    return x

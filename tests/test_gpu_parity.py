@@ -1,0 +1,162 @@
+"""Parity of the gfx950 HIP path with the CPU FP64 oracle, called through
+the C ABI on the same seeded inputs.
+
+Tolerances (relative L2 over the interior, ||gpu - oracle|| / ||oracle||):
+  FP64 plain, one phase           <= 1e-12   (FMA contraction / libm ulps)
+  FP64 plain, a few large steps   <= 1e-10
+  mixed (FP32 bulk, FP64 pi' and column solve), a few steps <= 2e-5 on
+  u, v, w, theta'; pi' and moisture <= 1e-4.
+"""
+import numpy as np
+import pytest
+
+from helpers import case, interiors, kitchen_sink, oracle_and_gpu, perturbed_state, rel_err
+
+pytestmark = pytest.mark.gpu
+
+STATE = ("u", "v", "w", "thp", "pip", "qv", "qc", "qr")
+
+
+def _cmp(g, om, gi, fields, tol, label):
+    errs = {}
+    for f in fields:
+        nzl = g.nz + 1 if f in ("w",) else g.nz
+        a = g.interior(gi.get(f), nzl)
+        b = g.interior(om.get(f), nzl)
+        errs[f] = rel_err(a, b)
+    print(label, {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    bad = {k: v for k, v in errs.items() if not v <= (tol[k] if isinstance(tol, dict) else tol)}
+    assert not bad, f"{label}: {bad}"
+    return errs
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_tendencies_fp64(seed):
+    c = kitchen_sink()
+    st = perturbed_state(c, seed)
+    om, gi = oracle_and_gpu(c, "fp64", st)
+    om.apply_bcs()
+    om.tendencies(6.0)
+    gi.apply_bcs()
+    gi.compute_tendencies(6.0)
+    _cmp(c.grid, om, gi, ("tu", "tv", "tw", "tthp", "tqv", "tqc", "tqr"), 1e-12,
+         f"tendencies seed={seed}")
+
+
+def test_tendencies_fp64_non_pd_periodic():
+    c = kitchen_sink(pd_moist="off", bc_x="periodic", bc_y="periodic", terrain="none",
+                     diff_K=0)
+    st = perturbed_state(c, 5)
+    om, gi = oracle_and_gpu(c, "fp64", st)
+    om.apply_bcs()
+    om.tendencies(6.0)
+    gi.apply_bcs()
+    gi.compute_tendencies(6.0)
+    _cmp(c.grid, om, gi, ("tu", "tv", "tw", "tthp", "tqv", "tqc", "tqr"), 1e-12,
+         "tendencies non-PD periodic")
+
+
+@pytest.mark.parametrize("nsub,bc", [(1, "open"), (3, "open"), (4, "periodic")])
+def test_acoustic_substeps_fp64(nsub, bc):
+    c = kitchen_sink(bc_x=bc, bc_y=bc)
+    st = perturbed_state(c, 3)
+    om, gi = oracle_and_gpu(c, "fp64", st)
+    om.apply_bcs()
+    om.tendencies(6.0)
+    gi.apply_bcs()
+    gi.compute_tendencies(6.0)
+    om.acoustic(1.0, nsub)
+    gi.acoustic_substeps(1.0, nsub)
+    _cmp(c.grid, om, gi, ("u", "v", "w", "thp", "pip"), 1e-12, f"acoustic n={nsub} {bc}")
+
+
+def test_kessler_fp64():
+    c = kitchen_sink()
+    st = perturbed_state(c, 4, {"u": 0, "v": 0, "w": 0, "thp": 0.5, "pip": 1e-5,
+                                "qv": 3e-3, "qc": 2e-3, "qr": 3e-3})
+    om, gi = oracle_and_gpu(c, "fp64", st)
+    om.kessler(6.0)
+    gi.kessler(6.0)
+    _cmp(c.grid, om, gi, ("thp", "qv", "qc", "qr"), 1e-12, "kessler")
+    g = c.grid
+    r_o = om.get("rain").reshape(g.NY, g.NX)
+    r_g = gi.get("rain").reshape(g.NY, g.NX)
+    assert rel_err(r_g, r_o) < 1e-12
+
+
+FULL_CASES = {
+    "kitchen_sink": lambda: kitchen_sink(),
+    "kitchen_sink_periodic": lambda: kitchen_sink(bc_x="periodic", bc_y="periodic",
+                                                  pd_moist="off"),
+    "cfg1_dry_bubble": lambda: case("cfg1_dry_bubble"),
+    "straka_small": lambda: case("straka", nx=64, ny=4, nz=24, dx=200, dz=200,
+                                 bubble_xc=6400, bubble_zc=2400),
+    "schaer_small": lambda: case("schaer", nx=48, ny=4, nz=24, mtn_xc=12000),
+    "moist_blob_small": lambda: case("moist_blob", nx=48, ny=4, nz=20, qvbub_xc=3000,
+                                     qvbub_zc=2000),
+}
+
+
+@pytest.mark.parametrize("name", sorted(FULL_CASES))
+def test_full_steps_fp64(name):
+    c = FULL_CASES[name]()
+    st = perturbed_state(c, 9)
+    om, gi = oracle_and_gpu(c, "fp64", st)
+    dt = c.scalar("dt")
+    for s in range(4):
+        om.step(dt, s * dt)
+        gi.step(dt, s * dt)
+    fields = STATE if c.params.moisture else STATE[:5]
+    _cmp(c.grid, om, gi, fields, 1e-10, f"{name} fp64 4 steps")
+
+
+@pytest.mark.parametrize("name", sorted(FULL_CASES))
+def test_full_steps_mixed(name):
+    c = FULL_CASES[name]()
+    st = perturbed_state(c, 9)
+    om, gi = oracle_and_gpu(c, "mixed", st)
+    dt = c.scalar("dt")
+    for s in range(4):
+        om.step(dt, s * dt)
+        gi.step(dt, s * dt)
+    tol = {"u": 2e-5, "v": 2e-5, "w": 2e-5, "thp": 2e-5, "pip": 1e-4, "qv": 1e-4,
+           "qc": 1e-4, "qr": 1e-4}
+    fields = STATE if c.params.moisture else STATE[:5]
+    _cmp(c.grid, om, gi, fields, tol, f"{name} mixed 4 steps")
+
+
+def test_boundary_relaxation_fp64():
+    """Davies relaxation toward seeded targets (k_bdy_relax kernels.cu:292-304)."""
+    from wfe_amd import Integrator  # noqa: F401
+    c = kitchen_sink()
+    g = c.grid
+    st = perturbed_state(c, 21)
+    om, gi = oracle_and_gpu(c, "fp64", st)
+    rng = np.random.default_rng(5)
+    wgt = np.zeros(g.NX * g.NY)
+    w2 = wgt.reshape(g.NY, g.NX)
+    w2[g.ng:g.ng + g.ny, g.ng:g.ng + g.nx] = rng.uniform(-1e-3, 2e-3, (g.ny, g.nx))
+    om.set("bdy_wgt", wgt)
+    from wfe_amd._lib import as_dptr, check, lib
+    check(lib().wfe_set_bdy_weights(gi._h, as_dptr(wgt), wgt.size), "weights")
+    for v in range(5):
+        for slot, nm in ((0, "lo"), (1, "hi")):
+            t = np.ascontiguousarray(st[("u", "v", "thp", "pip", "qv")[v]] *
+                                     rng.uniform(0.5, 1.5, g.npts))
+            om.set(f"bdy_{nm}_{v}", t)
+            check(lib().wfe_set_bdy_target(gi._h, slot, v, as_dptr(t), t.size), "target")
+    om.set_bdy(True, 0.3)
+    check(lib().wfe_set_bdy_tfrac(gi._h, 0.3), "tfrac")
+    dt = c.scalar("dt")
+    for s in range(3):
+        om.step(dt)
+        gi.step(dt)
+    _cmp(g, om, gi, STATE, 1e-10, "bdy relax fp64 3 steps")
+
+
+def test_absmax_matches_interior_max():
+    c = case("cfg1_dry_bubble")
+    st = perturbed_state(c, 2)
+    om, gi = oracle_and_gpu(c, "mixed", st)
+    for f in ("w", "thp", "u"):
+        assert gi.absmax(f) == pytest.approx(om.absmax(f), rel=1e-6)

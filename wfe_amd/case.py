@@ -1,0 +1,191 @@
+"""Python mirror of the reference's driver-level interface over the C ABI.
+
+``Case`` is the case file (Config + Metric + BaseState + initial state, built
+natively by ``wfe_case_load``); ``Integrator`` mirrors the reference's
+``Integrator`` (src/dynamics/integrator.hpp:18-42): ``step(dt, t)``,
+``state()`` field access, ``rain()``; every call goes to the gfx950 context.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from ._lib import (FIELDS, STATE_FIELDS, WFE_PREC_FP64, WFE_PREC_MIXED, as_dptr, check, lib,
+                   wfe_base, wfe_grid, wfe_metric, wfe_params, BASE_NAMES, METRIC_NAMES)
+
+PRECISIONS = {"fp64": WFE_PREC_FP64, "mixed": WFE_PREC_MIXED}
+
+
+class Grid:
+    """Padded reference layout (src/core/grid.hpp:22-36)."""
+
+    def __init__(self, nx, ny, nz, ng, dx, dy, dz):
+        self.nx, self.ny, self.nz, self.ng = nx, ny, nz, ng
+        self.dx, self.dy, self.dz = dx, dy, dz
+        self.NX, self.NY, self.NZ = nx + 2 * ng, ny + 2 * ng, nz + 1 + 2 * ng
+
+    @property
+    def npts(self):
+        return self.NX * self.NY * self.NZ
+
+    @property
+    def n_interior(self):
+        return self.nx * self.ny * self.nz
+
+    def as3d(self, a: np.ndarray) -> np.ndarray:
+        """View a padded flat array as [NZ, NY, NX]."""
+        return a.reshape(self.NZ, self.NY, self.NX)
+
+    def interior(self, a: np.ndarray, nzlev: int | None = None) -> np.ndarray:
+        g = self.ng
+        nzl = self.nz if nzlev is None else nzlev
+        return self.as3d(a)[g:g + nzl, g:g + self.ny, g:g + self.nx]
+
+
+class Case:
+    """An .ini case assembled by the native host builder."""
+
+    def __init__(self, ini: str | Path, **overrides):
+        L = lib()
+        self.ini = str(ini)
+        over = [f"{k}={v}".encode() for k, v in overrides.items()]
+        arr = (C.c_char_p * max(1, len(over)))(*over)
+        h = C.c_void_p()
+        check(L.wfe_case_load(self.ini.encode(), len(over), arr, C.byref(h)),
+              f"wfe_case_load({self.ini})")
+        self._h = h
+        gr = wfe_grid()
+        check(L.wfe_case_grid(h, C.byref(gr)), "wfe_case_grid")
+        self.grid = Grid(gr.nx, gr.ny, gr.nz, gr.ng, gr.dx, gr.dy, gr.dz)
+        self._cgrid = gr
+        p = wfe_params()
+        check(L.wfe_case_params(h, C.byref(p)), "wfe_case_params")
+        self.params = p
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value:
+            lib().wfe_case_destroy(h)
+            self._h = None
+
+    def scalar(self, name: str) -> float:
+        out = C.c_double()
+        check(lib().wfe_case_scalar(self._h, name.encode(), C.byref(out)), f"scalar {name}")
+        return out.value
+
+    def array(self, name: str) -> np.ndarray:
+        L = lib()
+        n = L.wfe_case_array_size(self._h, name.encode())
+        if n == 0:
+            raise KeyError(name)
+        a = np.empty(n, dtype=np.float64)
+        check(L.wfe_case_array(self._h, name.encode(), as_dptr(a), n), f"array {name}")
+        return a
+
+    def param_dict(self) -> dict:
+        return {f: getattr(self.params, f) for f, _ in wfe_params._fields_}
+
+    def unused_keys(self) -> list[str]:
+        buf = C.create_string_buffer(8192)
+        check(lib().wfe_case_unused_keys(self._h, buf, len(buf)), "unused keys")
+        return [k for k in buf.value.decode().split("\n") if k]
+
+    def integrator(self, precision: str = "mixed") -> "Integrator":
+        return Integrator.from_case(self, precision)
+
+
+class Integrator:
+    """gfx950 integrator context (src/dynamics/integrator.hpp:18-42)."""
+
+    def __init__(self, handle: C.c_void_p, grid: Grid, case: Case | None, precision: str):
+        self._h = handle
+        self.grid = grid
+        self.case = case
+        self.precision = precision
+
+    @classmethod
+    def from_case(cls, case: Case, precision: str = "mixed") -> "Integrator":
+        h = C.c_void_p()
+        check(lib().wfe_case_create_ctx(case._h, PRECISIONS[precision], C.byref(h)),
+              "wfe_case_create_ctx")
+        return cls(h, case.grid, case, precision)
+
+    @classmethod
+    def from_arrays(cls, grid: Grid, params: wfe_params, arrays: dict, zt: float,
+                    precision: str = "mixed") -> "Integrator":
+        """Integrator::init from explicit FP64 base-state / metric arrays."""
+        keep = {k: np.ascontiguousarray(arrays[k], dtype=np.float64)
+                for k in BASE_NAMES + METRIC_NAMES if k in arrays}
+        b = wfe_base(*[as_dptr(keep[n]) if n in keep else None for n in BASE_NAMES])
+        m = wfe_metric(*[as_dptr(keep[n]) for n in METRIC_NAMES], zt)
+        gr = wfe_grid(grid.nx, grid.ny, grid.nz, grid.ng, grid.dx, grid.dy, grid.dz)
+        h = C.c_void_p()
+        check(lib().wfe_create(C.byref(gr), C.byref(params), C.byref(b), C.byref(m),
+                               PRECISIONS[precision], C.byref(h)), "wfe_create")
+        return cls(h, grid, None, precision)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value:
+            lib().wfe_destroy(h)
+            self._h = None
+
+    # -- stepping
+    def step(self, dt: float, t: float = 0.0) -> None:
+        if self.case is not None:
+            check(lib().wfe_case_update_bdy(self.case._h, self._h, t), "boundary update")
+        check(lib().wfe_step(self._h, dt, t), "wfe_step")
+
+    def run(self, dt: float, nsteps: int, t0: float = 0.0) -> None:
+        for s in range(nsteps):
+            self.step(dt, t0 + s * dt)
+
+    def sync(self) -> None:
+        check(lib().wfe_sync(self._h), "wfe_sync")
+
+    # -- fields
+    def get(self, name: str) -> np.ndarray:
+        n = self.grid.NX * self.grid.NY if name == "rain" else self.grid.npts
+        a = np.empty(n, dtype=np.float64)
+        check(lib().wfe_download(self._h, FIELDS[name], as_dptr(a), n), f"download {name}")
+        return a
+
+    def set(self, name: str, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        check(lib().wfe_upload(self._h, FIELDS[name], as_dptr(a), a.size), f"upload {name}")
+
+    def state(self) -> dict:
+        return {f: self.get(f) for f in STATE_FIELDS}
+
+    def absmax(self, name: str) -> float:
+        out = C.c_float()
+        check(lib().wfe_absmax(self._h, FIELDS[name], C.byref(out)), "wfe_absmax")
+        return out.value
+
+    # -- per-kernel HIP-event timing on the context's stream
+    def kernel_timing(self, mode: int) -> None:
+        check(lib().wfe_kernel_timing(self._h, int(mode)), "wfe_kernel_timing")
+
+    def kernel_report(self) -> dict:
+        buf = C.create_string_buffer(1 << 16)
+        check(lib().wfe_kernel_timing_report(self._h, buf, len(buf)), "timing report")
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, cnt, ms = line.split()
+            out[name] = (int(cnt), float(ms))
+        return out
+
+    # -- phase-level entry points (kernels.hpp free functions)
+    def apply_bcs(self) -> None:
+        check(lib().wfe_apply_bcs(self._h), "wfe_apply_bcs")
+
+    def compute_tendencies(self, dt: float) -> None:
+        check(lib().wfe_compute_tendencies(self._h, dt), "wfe_compute_tendencies")
+
+    def acoustic_substeps(self, dtau: float, nsub: int) -> None:
+        check(lib().wfe_acoustic_substeps(self._h, dtau, nsub), "wfe_acoustic_substeps")
+
+    def kessler(self, dt: float) -> None:
+        check(lib().wfe_kessler(self._h, dt), "wfe_kessler")

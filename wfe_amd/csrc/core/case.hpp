@@ -1,0 +1,77 @@
+#pragma once
+
+#include <array>
+#include <string>
+#include <vector>
+
+#include "core/base_state.hpp"
+#include "core/config.hpp"
+#include "core/grid.hpp"
+#include "core/metric.hpp"
+#include "dynamics/params.hpp"
+#include "io/input.hpp"
+
+namespace wfe {
+
+// Host side of BdyManager (src/dynamics/boundary.hpp:16-38,
+// src/dynamics/boundary.cpp:9-97 of the reference): Davies rim weights
+// (cos^2 over bdy_width cells, 1/bdy_tau at the edge, optional interior
+// nudging 1/nudge_tau) and the two relaxation target levels for u, v,
+// theta', pi', qv, linearly interpolated in time. The device copy of the
+// targets lives in the integrator context; this object reads the files and
+// converts them to padded perturbation fields.
+struct BoundaryHost {
+  bool active = false;
+  std::vector<double> wgt;               // [NX*NY]
+  std::array<std::vector<double>, 5> lo, hi;  // padded 3-D targets
+  double t_lo = 0, t_hi = 0;
+  int idx_hi = 1;
+  bool dirty_hi = false;  // hi reloaded after a lo<-hi swap, not yet uploaded
+  bool swapped = false;
+
+  void init(const GDims& g, const DynParams& dp, const InputData& in,
+            const std::vector<double>& thb3, const std::vector<double>& pib3, int bdy_width,
+            double bdy_tau, double nudge_tau);
+  // Advances the bracketing boundary times past t; returns false on a file
+  // error. Marks swapped/dirty_hi for the device upload.
+  bool update(const GDims& g, const InputData& in, const std::vector<double>& thb3,
+              const std::vector<double>& pib3, double t);
+  double tfrac(double t) const;
+  bool load_into(const GDims& g, const InputData& in, const std::vector<double>& thb3,
+                 const std::vector<double>& pib3, int idx,
+                 std::array<std::vector<double>, 5>& dst);
+};
+
+// A fully assembled case: everything src/main.cpp:248-383 builds before the
+// time loop, kept on the host in FP64.
+struct Case {
+  Config cfg;
+  GDims g;
+  DynParams dp;
+  Metric metric;
+  BaseState base;
+  InputData input;
+  bool file_mode = false;
+  double dt = 0.25, t_end = 1000, out_interval = 100, diag_interval = 10;
+  std::string out_dir;
+  std::string physics;  // "none" | "simple" (surface physics is out of scope)
+  // Initial state, padded 3-D arrays: u v w thp pip qv qc qr.
+  std::array<std::vector<double>, 8> ic;
+  BoundaryHost bdy;
+  std::vector<std::string> warnings;  // e.g. the acoustic CFL advisory
+
+  // Throws std::runtime_error on configuration or input errors.
+  void build();
+};
+
+// Lambert conformal map factor m(lat) for standard parallels lat1, lat2
+// (src/main.cpp:323-340); lat in degrees, interior ny*nx.
+std::vector<double> lambert_map_factor(const std::vector<double>& lat_deg, double lat1_deg,
+                                       double lat2_deg);
+
+// Seeded theta' perturbation exactly as src/main.cpp:105-144 (mt19937 +
+// normal_distribution<float>, six 3-D box-smoothing passes, RMS scaled to
+// amp) added to thp (padded).
+void add_ic_perturbation(const GDims& g, std::vector<double>& thp, double amp, unsigned seed);
+
+}  // namespace wfe

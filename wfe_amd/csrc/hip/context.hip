@@ -1,0 +1,888 @@
+// Integrator context for gfx950: device buffers, the RK3/acoustic schedule
+// and the C ABI of include/wfe_hip.h (dynamics part).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "wfe_hip.h"
+#include "hip/acoustic_kernels.hpp"
+#include "hip/kessler_kernels.hpp"
+#include "hip/slow_kernels.hpp"
+
+namespace wfe {
+
+thread_local std::string g_last_error;
+
+void set_last_error(const std::string& s) { g_last_error = s; }
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct ArgError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+namespace {
+
+#define WFE_HIP(call)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      throw ::wfe::HipError(std::string("HIP error ") + hipGetErrorName(e_) + " at " __FILE__ \
+                     ":" + std::to_string(__LINE__) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+inline void check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    throw HipError(std::string("kernel launch failed in ") + what + ": " + hipGetErrorString(e));
+}
+
+inline unsigned cdiv(int a, int b) { return (unsigned)((a + b - 1) / b); }
+
+// Owns every device allocation of a context (zero-initialised).
+class DevArena {
+ public:
+  ~DevArena() {
+    for (void* p : ptrs_) (void)hipFree(p);
+  }
+  template <class E>
+  E* alloc(size_t n) {
+    void* p = nullptr;
+    WFE_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(E)));
+    ptrs_.push_back(p);
+    WFE_HIP(hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(E)));
+    bytes_ += n * sizeof(E);
+    return static_cast<E*>(p);
+  }
+  size_t bytes() const { return bytes_; }
+
+ private:
+  std::vector<void*> ptrs_;
+  size_t bytes_ = 0;
+};
+
+template <class T>
+struct StateSet {
+  T *u = nullptr, *v = nullptr, *w = nullptr, *thp = nullptr;
+  T *qv = nullptr, *qc = nullptr, *qr = nullptr;
+  double* pip = nullptr;
+};
+
+// Optional per-launch timing with HIP events recorded on the context's
+// stream around each kernel (wfe_kernel_timing). Event pairs are collected
+// lazily; totals are per kernel name.
+class KernelTimer {
+ public:
+  ~KernelTimer() {
+    for (auto& r : pending_) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+  }
+  bool on = false;
+  void begin(const char* name, hipStream_t s) {
+    if (!on) return;
+    cur_ = Rec{name, get(), get()};
+    WFE_HIP(hipEventRecord(cur_.a, s));
+  }
+  void end(hipStream_t s) {
+    if (!on) return;
+    WFE_HIP(hipEventRecord(cur_.b, s));
+    pending_.push_back(cur_);
+    if (pending_.size() > 4096) collect();
+  }
+  void collect() {
+    for (auto& r : pending_) {
+      WFE_HIP(hipEventSynchronize(r.b));
+      float ms = 0;
+      WFE_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+      auto& a = acc_[r.name];
+      a.first += 1;
+      a.second += ms;
+      pool_.push_back(r.a);
+      pool_.push_back(r.b);
+    }
+    pending_.clear();
+  }
+  std::string report() {
+    collect();
+    std::string out;
+    char line[256];
+    for (auto& kv : acc_) {
+      std::snprintf(line, sizeof(line), "%s %ld %.6f\n", kv.first.c_str(), kv.second.first,
+                    kv.second.second);
+      out += line;
+    }
+    return out;
+  }
+  void reset() {
+    collect();
+    acc_.clear();
+  }
+
+ private:
+  struct Rec {
+    const char* name;
+    hipEvent_t a, b;
+  };
+  hipEvent_t get() {
+    if (!pool_.empty()) {
+      hipEvent_t e = pool_.back();
+      pool_.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    WFE_HIP(hipEventCreate(&e));
+    return e;
+  }
+  Rec cur_{};
+  std::vector<Rec> pending_;
+  std::vector<hipEvent_t> pool_;
+  std::map<std::string, std::pair<long, double>> acc_;
+};
+
+#define TIMED(name, ...)               \
+  do {                                 \
+    timer_.begin(name, stream_);       \
+    __VA_ARGS__;                       \
+    timer_.end(stream_);               \
+  } while (0)
+
+}  // namespace
+
+// Type-erased interface used by the C ABI.
+struct CtxBase {
+  virtual ~CtxBase() = default;
+  virtual void step(double dt) = 0;
+  virtual void upload(int field, const double* h, size_t n) = 0;
+  virtual void download(int field, double* h, size_t n) = 0;
+  virtual float absmax(int field) = 0;
+  virtual void sync() = 0;
+  virtual void set_bdy_weights(const double* w, size_t n) = 0;
+  virtual void set_bdy_target(int slot, int var, const double* h, size_t n) = 0;
+  virtual void swap_bdy_targets() = 0;
+  virtual void set_bdy_tfrac(double tf) = 0;
+  virtual void apply_bcs() = 0;
+  virtual void compute_tendencies(double dt) = 0;
+  virtual void acoustic_substeps(double dtau, int nsub) = 0;
+  virtual void kessler(double dt) = 0;
+  virtual size_t device_bytes() const = 0;
+  virtual void kernel_timing(int enable) = 0;
+  virtual std::string kernel_report() = 0;
+};
+
+template <class T>
+class Ctx final : public CtxBase {
+ public:
+  Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& base, const wfe_metric& met);
+  ~Ctx() override {
+    if (stream_) (void)hipStreamDestroy(stream_);
+  }
+  void step(double dt) override;
+  void upload(int field, const double* h, size_t n) override;
+  void download(int field, double* h, size_t n) override;
+  float absmax(int field) override;
+  void sync() override { WFE_HIP(hipStreamSynchronize(stream_)); }
+  void set_bdy_weights(const double* w, size_t n) override;
+  void set_bdy_target(int slot, int var, const double* h, size_t n) override;
+  void swap_bdy_targets() override { std::swap(bdy_lo_, bdy_hi_); }
+  void set_bdy_tfrac(double tf) override { tfrac_ = tf; }
+  void apply_bcs() override { fill_ghosts(S_[cur_]); }
+  void compute_tendencies(double dt) override { slow_tendencies(S_[cur_], dt); }
+  void acoustic_substeps(double dtau, int nsub) override;
+  void kessler(double dt) override { kessler_step(S_[cur_], dt); }
+  size_t device_bytes() const override { return arena_.bytes(); }
+  void kernel_timing(int enable) override {
+    WFE_HIP(hipStreamSynchronize(stream_));
+    if (enable == 2) timer_.reset();
+    timer_.on = enable != 0;
+  }
+  std::string kernel_report() override {
+    WFE_HIP(hipStreamSynchronize(stream_));
+    return timer_.report();
+  }
+
+ private:
+  // host helpers
+  template <class E>
+  E* upload_new(const std::vector<double>& h);
+  template <class E>
+  E* upload_new(const double* h, size_t n);
+  void fill_ghosts(const StateSet<T>& s);
+  void slow_tendencies(const StateSet<T>& est, double dt);
+  void bdy_relax(const StateSet<T>& est);
+  void acoustic_stage(const StateSet<T>& out, double dtau, int nsub);
+  void stage_moisture(const StateSet<T>& out, double dt_stage);
+  void kessler_step(const StateSet<T>& s, double dt);
+  T* field_ptr(int field);
+  dim3 blk3() const { return dim3(64, 4, 1); }
+
+  dev::Grid g_{};
+  dev::Params p_{};
+  wfe_params hp_{};
+  size_t npts_ = 0, n2_ = 0;
+  hipStream_t stream_ = nullptr;
+  DevArena arena_;
+  dev::Base<T> base_{};
+  dev::Metric<T> met_{};
+  dev::Coef<T> coef_{};
+  StateSet<T> S_[2]{};
+  StateSet<T> SE_{};
+  int cur_ = 0;
+  T* wu_[2]{};
+  T* wv_[2]{};
+  T *ww_ = nullptr, *wthp_ = nullptr;
+  double *wpip_ = nullptr, *wpiprev_ = nullptr;
+  T *tu_ = nullptr, *tv_ = nullptr, *tw_ = nullptr, *tth_ = nullptr, *tpi_ = nullptr;
+  T* tq_[3]{};
+  T *mfx_ = nullptr, *mfy_ = nullptr, *mfz_ = nullptr, *div_ = nullptr;
+  T *pfx_[3]{}, *pfy_[3]{}, *pfz_[3]{}, *pratio_[3]{};
+  T* rain_ = nullptr;
+  bool bdy_active_ = false;
+  T *bdy_lo_[5]{}, *bdy_hi_[5]{};
+  T* bdy_wgt_ = nullptr;
+  double tfrac_ = 0;
+  unsigned* d_absmax_ = nullptr;
+  KernelTimer timer_;
+};
+
+template <class T>
+template <class E>
+E* Ctx<T>::upload_new(const double* h, size_t n) {
+  E* d = arena_.alloc<E>(n);
+  if (h) {
+    std::vector<E> tmp(h, h + n);
+    WFE_HIP(hipMemcpy(d, tmp.data(), n * sizeof(E), hipMemcpyHostToDevice));
+  }
+  return d;
+}
+template <class T>
+template <class E>
+E* Ctx<T>::upload_new(const std::vector<double>& h) {
+  return upload_new<E>(h.data(), h.size());
+}
+
+template <class T>
+Ctx<T>::Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& b, const wfe_metric& m) {
+  hp_ = hp;
+  g_.nx = gr.nx;
+  g_.ny = gr.ny;
+  g_.nz = gr.nz;
+  g_.ng = gr.ng;
+  g_.NX = gr.nx + 2 * gr.ng;
+  g_.NY = gr.ny + 2 * gr.ng;
+  g_.NZ = gr.nz + 1 + 2 * gr.ng;
+  g_.plane = g_.NX * g_.NY;
+  g_.dx = gr.dx;
+  g_.dy = gr.dy;
+  npts_ = (size_t)g_.plane * g_.NZ;
+  n2_ = (size_t)g_.plane;
+  p_.diff_K = hp.diff_K;
+  p_.coriolis_use_ub = hp.coriolis_use_ub;
+  p_.rayleigh_zd = hp.rayleigh_zd;
+  p_.rayleigh_alpha = hp.rayleigh_alpha;
+  p_.beta = hp.acoustic_beta;
+  p_.smdiv = hp.acoustic_smdiv;
+  p_.bcx = hp.bc_x_open;
+  p_.bcy = hp.bc_y_open;
+  p_.cstar = hp.cstar;
+  p_.moisture = hp.moisture;
+  p_.pd_moist = hp.moisture && hp.pd_moist;
+  p_.w_damping = hp.w_damping;
+  p_.microphysics = hp.microphysics;
+
+  WFE_HIP(hipStreamCreate(&stream_));
+  const size_t n = npts_;
+  const size_t nzl = (size_t)g_.NZ;
+
+  // Base state: FP64 host arrays -> T (pibar stays FP64).
+  base_.thb = upload_new<T>(b.thb, n);
+  base_.rhob = upload_new<T>(b.rhob, n);
+  base_.dthbdz = upload_new<T>(b.dthbdz, n);
+  base_.thbw = upload_new<T>(b.thbw, n);
+  base_.rhobw = upload_new<T>(b.rhobw, n);
+  base_.qvb = upload_new<T>(b.qvb, n);
+  base_.ub = upload_new<T>(b.ub, nzl);
+  base_.vb = upload_new<T>(b.vb, nzl);  // NULL -> zeros
+  base_.pib = upload_new<double>(b.pib, n);
+
+  met_.zeta_w = upload_new<T>(m.zeta_w, nzl);
+  met_.zeta_c = upload_new<T>(m.zeta_c, nzl);
+  met_.dzeta_c = upload_new<T>(m.dzeta_c, nzl);
+  met_.dzeta_w = upload_new<T>(m.dzeta_w, nzl);
+  met_.hx = upload_new<T>(m.hx, n2_);
+  met_.hy = upload_new<T>(m.hy, n2_);
+  met_.hxu = upload_new<T>(m.hx_u, n2_);
+  met_.hyv = upload_new<T>(m.hy_v, n2_);
+  met_.jac = upload_new<T>(m.jac, n2_);
+  met_.fcor = upload_new<T>(m.fcor, n2_);
+  met_.mapf = upload_new<T>(m.mapf, n2_);
+  met_.zt = (T)m.zt;
+
+  // Acoustic coefficients, formed in FP64 on the host exactly as
+  // k_acou_coef_c / k_acou_coef_f (kernels.cu:452-474) do on the device.
+  {
+    std::vector<double> rtjc(n), rtjx(n), rtjy(n), kt3(n), aw(n);
+    for (int kr = 0; kr < g_.NZ; ++kr)
+      for (int jr = 0; jr < g_.NY; ++jr)
+        for (int ir = 0; ir < g_.NX; ++ir) {
+          const size_t c = ((size_t)kr * g_.NY + jr) * g_.NX + ir;
+          const double J = m.jac[(size_t)jr * g_.NX + ir];
+          rtjc[c] = b.rhob[c] * b.thvb[c] * J;
+          kt3[c] = dev::kRd * b.pib[c] / (dev::kCv * b.rhob[c] * b.thvb[c] * J);
+          aw[c] = b.rhobw[c] * b.thvbw[c];
+        }
+    for (int kr = 0; kr < g_.NZ; ++kr)
+      for (int jr = 0; jr < g_.NY; ++jr)
+        for (int ir = 0; ir < g_.NX; ++ir) {
+          const size_t c = ((size_t)kr * g_.NY + jr) * g_.NX + ir;
+          rtjx[c] = ir > 0 ? 0.5 * (rtjc[c - 1] + rtjc[c]) : rtjc[c];
+          rtjy[c] = jr > 0 ? 0.5 * (rtjc[c - g_.NX] + rtjc[c]) : rtjc[c];
+        }
+    coef_.rtjx = upload_new<T>(rtjx);
+    coef_.rtjy = upload_new<T>(rtjy);
+    coef_.kt3 = upload_new<T>(kt3);
+    coef_.aw = upload_new<T>(aw);
+  }
+
+  auto alloc_set = [&](StateSet<T>& s) {
+    s.u = arena_.alloc<T>(n);
+    s.v = arena_.alloc<T>(n);
+    s.w = arena_.alloc<T>(n);
+    s.thp = arena_.alloc<T>(n);
+    s.qv = arena_.alloc<T>(n);
+    s.qc = arena_.alloc<T>(n);
+    s.qr = arena_.alloc<T>(n);
+    s.pip = arena_.alloc<double>(n);
+  };
+  alloc_set(S_[0]);
+  alloc_set(S_[1]);
+  alloc_set(SE_);
+  for (int q = 0; q < 2; ++q) {
+    wu_[q] = arena_.alloc<T>(n);
+    wv_[q] = arena_.alloc<T>(n);
+  }
+  ww_ = arena_.alloc<T>(n);
+  wthp_ = arena_.alloc<T>(n);
+  wpip_ = arena_.alloc<double>(n);
+  wpiprev_ = arena_.alloc<double>(n);
+  tu_ = arena_.alloc<T>(n);
+  tv_ = arena_.alloc<T>(n);
+  tw_ = arena_.alloc<T>(n);
+  tth_ = arena_.alloc<T>(n);
+  tpi_ = arena_.alloc<T>(n);
+  for (int q = 0; q < 3; ++q) tq_[q] = arena_.alloc<T>(n);
+  mfx_ = arena_.alloc<T>(n);
+  mfy_ = arena_.alloc<T>(n);
+  mfz_ = arena_.alloc<T>(n);
+  div_ = arena_.alloc<T>(n);
+  if (p_.pd_moist)
+    for (int q = 0; q < 3; ++q) {
+      pfx_[q] = arena_.alloc<T>(n);
+      pfy_[q] = arena_.alloc<T>(n);
+      pfz_[q] = arena_.alloc<T>(n);
+      pratio_[q] = arena_.alloc<T>(n);
+    }
+  rain_ = arena_.alloc<T>(n2_);
+  d_absmax_ = arena_.alloc<unsigned>(1);
+  WFE_HIP(hipDeviceSynchronize());
+}
+
+template <class T>
+T* Ctx<T>::field_ptr(int f) {
+  StateSet<T>& s = S_[cur_];
+  switch (f) {
+    case WFE_F_U: return s.u;
+    case WFE_F_V: return s.v;
+    case WFE_F_W: return s.w;
+    case WFE_F_THP: return s.thp;
+    case WFE_F_QV: return s.qv;
+    case WFE_F_QC: return s.qc;
+    case WFE_F_QR: return s.qr;
+    case WFE_F_TU: return tu_;
+    case WFE_F_TV: return tv_;
+    case WFE_F_TW: return tw_;
+    case WFE_F_TTHP: return tth_;
+    case WFE_F_TPIP: return tpi_;
+    case WFE_F_TQV: return tq_[0];
+    case WFE_F_TQC: return tq_[1];
+    case WFE_F_TQR: return tq_[2];
+    case WFE_F_RAIN: return rain_;
+    default: throw ArgError("unknown field selector " + std::to_string(f));
+  }
+}
+
+template <class T>
+void Ctx<T>::upload(int f, const double* h, size_t n) {
+  const size_t want = f == WFE_F_RAIN ? n2_ : npts_;
+  if (n != want) throw ArgError("upload: element count mismatch");
+  WFE_HIP(hipStreamSynchronize(stream_));
+  if (f == WFE_F_PIP) {
+    WFE_HIP(hipMemcpy(S_[cur_].pip, h, n * sizeof(double), hipMemcpyHostToDevice));
+    return;
+  }
+  T* d = field_ptr(f);
+  std::vector<T> tmp(h, h + n);
+  WFE_HIP(hipMemcpy(d, tmp.data(), n * sizeof(T), hipMemcpyHostToDevice));
+}
+
+template <class T>
+void Ctx<T>::download(int f, double* h, size_t n) {
+  const size_t want = f == WFE_F_RAIN ? n2_ : npts_;
+  if (n != want) throw ArgError("download: element count mismatch");
+  WFE_HIP(hipStreamSynchronize(stream_));
+  if (f == WFE_F_PIP) {
+    WFE_HIP(hipMemcpy(h, S_[cur_].pip, n * sizeof(double), hipMemcpyDeviceToHost));
+    return;
+  }
+  const T* d = field_ptr(f);
+  std::vector<T> tmp(n);
+  WFE_HIP(hipMemcpy(tmp.data(), d, n * sizeof(T), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < n; ++i) h[i] = (double)tmp[i];
+}
+
+template <class T>
+float Ctx<T>::absmax(int f) {
+  if (f < 0 || f >= WFE_F_RAIN) throw ArgError("absmax: field must be a 3-D field");
+  WFE_HIP(hipMemsetAsync(d_absmax_, 0, sizeof(unsigned), stream_));
+  const int kmax = f == WFE_F_W ? g_.nz : g_.nz - 1;
+  dim3 grd(cdiv(g_.nx, 64), cdiv(g_.ny, 4), (unsigned)(kmax + 1));
+  if (f == WFE_F_PIP)
+    dev::k_absmax<double><<<grd, dim3(64, 4), 0, stream_>>>(g_, S_[cur_].pip, kmax, d_absmax_);
+  else
+    dev::k_absmax<T><<<grd, dim3(64, 4), 0, stream_>>>(g_, field_ptr(f), kmax, d_absmax_);
+  check_launch("absmax");
+  unsigned bits = 0;
+  WFE_HIP(hipMemcpyAsync(&bits, d_absmax_, sizeof(unsigned), hipMemcpyDeviceToHost, stream_));
+  WFE_HIP(hipStreamSynchronize(stream_));
+  float v;
+  std::memcpy(&v, &bits, sizeof(float));
+  if (std::isnan(v)) v = INFINITY;
+  return v;
+}
+
+template <class T>
+void Ctx<T>::set_bdy_weights(const double* w, size_t n) {
+  if (n != n2_) throw ArgError("bdy weights: element count mismatch");
+  if (!bdy_wgt_) {
+    bdy_wgt_ = arena_.alloc<T>(n2_);
+    for (int v = 0; v < 5; ++v) {
+      bdy_lo_[v] = arena_.alloc<T>(npts_);
+      bdy_hi_[v] = arena_.alloc<T>(npts_);
+    }
+  }
+  std::vector<T> tmp(w, w + n);
+  WFE_HIP(hipMemcpy(bdy_wgt_, tmp.data(), n * sizeof(T), hipMemcpyHostToDevice));
+  bdy_active_ = true;
+}
+
+template <class T>
+void Ctx<T>::set_bdy_target(int slot, int var, const double* h, size_t n) {
+  if (!bdy_wgt_) throw ArgError("bdy target before bdy weights");
+  if (slot < 0 || slot > 1 || var < 0 || var > 4 || n != npts_)
+    throw ArgError("bdy target: bad slot/var/size");
+  WFE_HIP(hipStreamSynchronize(stream_));
+  std::vector<T> tmp(h, h + n);
+  WFE_HIP(hipMemcpy(slot == 0 ? bdy_lo_[var] : bdy_hi_[var], tmp.data(), n * sizeof(T),
+                    hipMemcpyHostToDevice));
+}
+
+template <class T>
+void Ctx<T>::fill_ghosts(const StateSet<T>& s) {
+  dev::FillList<T> L{};
+  dev::FillList<double> Lp{};
+  auto add = [&](T* f, int kind, int xl, int yl) {
+    L.f[L.n] = f;
+    L.kind[L.n] = kind;
+    L.xlast[L.n] = xl;
+    L.ylast[L.n] = yl;
+    ++L.n;
+  };
+  const int nx = g_.nx, ny = g_.ny;
+  add(s.u, dev::kFillScalar, nx, ny - 1);
+  add(s.v, dev::kFillScalar, nx - 1, ny);
+  add(s.w, dev::kFillW, nx - 1, ny - 1);
+  add(s.thp, dev::kFillScalar, nx - 1, ny - 1);
+  if (p_.moisture) {
+    add(s.qv, dev::kFillScalar, nx - 1, ny - 1);
+    add(s.qc, dev::kFillScalar, nx - 1, ny - 1);
+    add(s.qr, dev::kFillScalar, nx - 1, ny - 1);
+  }
+  if constexpr (std::is_same_v<T, double>) {
+    add(s.pip, dev::kFillScalar, nx - 1, ny - 1);
+  } else {
+    Lp.f[0] = s.pip;
+    Lp.kind[0] = dev::kFillScalar;
+    Lp.xlast[0] = nx - 1;
+    Lp.ylast[0] = ny - 1;
+    Lp.n = 1;
+  }
+  const dim3 bz(64, 4), gz(cdiv(g_.NX, 64), cdiv(g_.NY, 4));
+  const dim3 bx(4, 64), gx(cdiv(g_.NY, 64), (unsigned)g_.NZ);
+  const dim3 by(64, 4), gy(cdiv(g_.NX, 64), (unsigned)g_.NZ);
+  TIMED("fill_ghosts", dev::k_fill_z<T><<<gz, bz, 0, stream_>>>(g_, L));
+  if (Lp.n) TIMED("fill_ghosts", dev::k_fill_z<double><<<gz, bz, 0, stream_>>>(g_, Lp));
+  TIMED("fill_ghosts", dev::k_fill_x<T><<<gx, bx, 0, stream_>>>(g_, p_.bcx, L));
+  if (Lp.n) TIMED("fill_ghosts", dev::k_fill_x<double><<<gx, bx, 0, stream_>>>(g_, p_.bcx, Lp));
+  TIMED("fill_ghosts", dev::k_fill_y<T><<<gy, by, 0, stream_>>>(g_, p_.bcy, L));
+  if (Lp.n) TIMED("fill_ghosts", dev::k_fill_y<double><<<gy, by, 0, stream_>>>(g_, p_.bcy, Lp));
+  check_launch("fill_ghosts");
+}
+
+template <class T>
+void Ctx<T>::slow_tendencies(const StateSet<T>& s, double dt) {
+  {
+    dim3 grd(cdiv(g_.nx + 5, 64), cdiv(g_.ny + 5, 4), (unsigned)g_.nz);
+    TIMED("massflux", dev::k_massflux<T><<<grd, blk3(), 0, stream_>>>(g_, base_, met_, s.u, s.v, s.w, mfx_, mfy_,
+                                                    mfz_));
+  }
+  {
+    dim3 grd(cdiv(g_.nx + 2, 64), cdiv(g_.ny + 2, 4), (unsigned)g_.nz);
+    TIMED("divergence", dev::k_divergence<T><<<grd, blk3(), 0, stream_>>>(g_, met_, mfx_, mfy_, mfz_, div_));
+  }
+  const int do_q = p_.moisture && !p_.pd_moist;
+  dev::TendOut o{tu_, tv_, tw_, tth_, {tq_[0], tq_[1], tq_[2]}};
+  {
+    dim3 grd(cdiv(g_.nx, 64), cdiv(g_.ny, 4), (unsigned)g_.nz);
+    TIMED("tendencies", dev::k_tendencies<T><<<grd, blk3(), 0, stream_>>>(g_, p_, base_, met_, (T)dt, s.u, s.v,
+                                                      s.w, s.thp, s.qv, s.qc, s.qr, mfx_,
+                                                      mfy_, mfz_, div_, o, do_q));
+  }
+  if (p_.moisture && p_.pd_moist) {
+    dev::PDSet<T> S{};
+    const T* q[3] = {s.qv, s.qc, s.qr};
+    for (int q3 = 0; q3 < 3; ++q3) {
+      S.q[q3] = q[q3];
+      S.fx[q3] = pfx_[q3];
+      S.fy[q3] = pfy_[q3];
+      S.fz[q3] = pfz_[q3];
+      S.ratio[q3] = pratio_[q3];
+      S.tq[q3] = tq_[q3];
+    }
+    dim3 gf(cdiv(g_.nx + 1, 64), cdiv(g_.ny + 1, 4), (unsigned)(g_.nz + 1));
+    TIMED("pd_flux", dev::k_pd_flux<T><<<gf, blk3(), 0, stream_>>>(g_, mfx_, mfy_, mfz_, S));
+    dim3 gi(cdiv(g_.nx, 64), cdiv(g_.ny, 4), (unsigned)g_.nz);
+    TIMED("pd_ratio", dev::k_pd_ratio<T><<<gi, blk3(), 0, stream_>>>(g_, base_, met_, (T)dt, S));
+    TIMED("pd_apply", dev::k_pd_apply<T><<<gi, blk3(), 0, stream_>>>(g_, p_, base_, met_, div_, S));
+  }
+  check_launch("slow_tendencies");
+}
+
+template <class T>
+void Ctx<T>::bdy_relax(const StateSet<T>& s) {
+  dev::RelaxSet<T> R{};
+  R.f[0] = s.u;
+  R.f[1] = s.v;
+  R.f[2] = s.thp;
+  R.f[3] = s.qv;
+  R.pip = s.pip;
+  for (int v = 0; v < 5; ++v) {
+    R.lo[v] = bdy_lo_[v];
+    R.hi[v] = bdy_hi_[v];
+  }
+  R.t[0] = tu_;
+  R.t[1] = tv_;
+  R.t[2] = tth_;
+  R.t[3] = tpi_;
+  R.t[4] = tq_[0];
+  R.wgt = bdy_wgt_;
+  R.tf = (T)tfrac_;
+  dim3 grd(cdiv(g_.nx, 64), cdiv(g_.ny, 4), (unsigned)g_.nz);
+  TIMED("bdy_relax", dev::k_bdy_relax<T><<<grd, blk3(), 0, stream_>>>(g_, R));
+  check_launch("bdy_relax");
+}
+
+// One RK3 stage of the fast system from the time-n state S_[cur_] into out.
+template <class T>
+void Ctx<T>::acoustic_stage(const StateSet<T>& out, double dtau, int nsub) {
+  const StateSet<T>& sn = S_[cur_];
+  const dim3 guv(cdiv(g_.nx + 1, 64), cdiv(g_.ny + 1, 4), (unsigned)g_.nz);
+  const dim3 gcol(cdiv(g_.nx, 64), (unsigned)g_.ny);
+  for (int s = 0; s < nsub; ++s) {
+    const bool first = s == 0, last = s == nsub - 1;
+    dev::UVArgs<T> a{};
+    a.u_src = first ? sn.u : wu_[(s - 1) & 1];
+    a.v_src = first ? sn.v : wv_[(s - 1) & 1];
+    a.u_dst = last ? out.u : wu_[s & 1];
+    a.v_dst = last ? out.v : wv_[s & 1];
+    a.pcur = first ? sn.pip : wpip_;
+    a.pprev = first ? sn.pip : wpiprev_;
+    a.pn = sn.pip;
+    a.thp_cur = first ? sn.thp : wthp_;
+    a.thp_n = sn.thp;
+    a.qv = sn.qv;
+    a.tu = tu_;
+    a.tv = tv_;
+    a.dtau = (T)dtau;
+    TIMED("acou_uv", dev::k_acou_uv<T><<<guv, blk3(), 0, stream_>>>(g_, p_, base_, met_, a));
+
+    dev::WPArgs<T> w{};
+    w.u = a.u_dst;
+    w.v = a.v_dst;
+    w.w_src = first ? sn.w : ww_;
+    w.pip_src = first ? sn.pip : wpip_;
+    w.thp_src = first ? sn.thp : wthp_;
+    w.w_dst = last ? out.w : ww_;
+    w.pip_dst = last ? out.pip : wpip_;
+    w.piprev_dst = wpiprev_;
+    w.thp_dst = last ? out.thp : wthp_;
+    w.tw = tw_;
+    w.tth = tth_;
+    w.tpi = bdy_active_ ? tpi_ : nullptr;
+    w.qv = sn.qv;
+    w.qc = sn.qc;
+    w.qr = sn.qr;
+    w.dtau = dtau;
+    if (g_.nz + 1 <= 64)
+      TIMED("acou_wpi", dev::k_acou_wpi_col<T, 64><<<gcol, dim3(64, 1), 0, stream_>>>(g_, p_, base_, met_, coef_, w));
+    else
+      TIMED("acou_wpi", dev::k_acou_wpi_col<T, 320>
+          <<<gcol, dim3(64, 1), 0, stream_>>>(g_, p_, base_, met_, coef_, w));
+  }
+  check_launch("acoustic_stage");
+}
+
+template <class T>
+void Ctx<T>::stage_moisture(const StateSet<T>& out, double dt_stage) {
+  const StateSet<T>& sn = S_[cur_];
+  const int n = (int)npts_;
+  TIMED("stage_q", dev::k_stage_q<T><<<cdiv(n, 256), 256, 0, stream_>>>(n, (T)dt_stage, sn.qv, sn.qc, sn.qr,
+                                                       tq_[0], tq_[1], tq_[2], out.qv, out.qc,
+                                                       out.qr));
+  check_launch("stage_moisture");
+}
+
+template <class T>
+void Ctx<T>::kessler_step(const StateSet<T>& s, double dt) {
+  dev::KesArgs<T> a{s.thp, s.pip, s.qv, s.qc, s.qr, rain_, (T)dt};
+  const dim3 gcol(cdiv(g_.nx, 64), (unsigned)g_.ny);
+  if (g_.nz + 1 <= 64)
+    TIMED("kessler", dev::k_kessler_col<T, 64><<<gcol, dim3(64, 1), 0, stream_>>>(g_, base_, met_, a));
+  else
+    TIMED("kessler", dev::k_kessler_col<T, 320><<<gcol, dim3(64, 1), 0, stream_>>>(g_, base_, met_, a));
+  check_launch("kessler");
+}
+
+// Integrator::step (integrator.cpp:71-111) with fixed-role buffers: the
+// time-n set S_[cur_], the stage estimate SE_, the next time level
+// S_[cur_^1], and ping-pong work buffers inside the acoustic loop.
+template <class T>
+void Ctx<T>::step(double dt) {
+  const double stage_dt[3] = {dt / 3, dt / 2, dt};
+  const int ns = std::max(1, hp_.acoustic_ns);
+  const int ns_stage[3] = {1, std::max(1, ns / 2), ns};
+  const StateSet<T>& sn = S_[cur_];
+  const StateSet<T>& snext = S_[cur_ ^ 1];
+  for (int m = 0; m < 3; ++m) {
+    const StateSet<T>& est = m == 0 ? sn : SE_;
+    fill_ghosts(est);
+    slow_tendencies(est, dt);
+    if (bdy_active_) bdy_relax(est);
+    const StateSet<T>& out = m == 2 ? snext : SE_;
+    acoustic_stage(out, stage_dt[m] / ns_stage[m], ns_stage[m]);
+    if (p_.moisture) stage_moisture(out, stage_dt[m]);
+  }
+  cur_ ^= 1;
+  if (p_.moisture && p_.microphysics) kessler_step(S_[cur_], dt);
+}
+
+template <class T>
+void Ctx<T>::acoustic_substeps(double dtau, int nsub) {
+  const StateSet<T>& out = S_[cur_ ^ 1];
+  acoustic_stage(out, dtau, nsub);
+  const StateSet<T>& sn = S_[cur_];
+  WFE_HIP(hipMemcpyAsync(out.qv, sn.qv, npts_ * sizeof(T), hipMemcpyDeviceToDevice, stream_));
+  WFE_HIP(hipMemcpyAsync(out.qc, sn.qc, npts_ * sizeof(T), hipMemcpyDeviceToDevice, stream_));
+  WFE_HIP(hipMemcpyAsync(out.qr, sn.qr, npts_ * sizeof(T), hipMemcpyDeviceToDevice, stream_));
+  cur_ ^= 1;
+}
+
+}  // namespace wfe
+
+// ------------------------------------------------------------------ C ABI
+
+struct wfe_ctx {
+  std::unique_ptr<wfe::CtxBase> impl;
+  int precision = 0;
+};
+
+namespace {
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return WFE_OK;
+  } catch (const wfe::ArgError& e) {
+    wfe::set_last_error(e.what());
+    return WFE_E_ARG;
+  } catch (const wfe::HipError& e) {
+    wfe::set_last_error(e.what());
+    return WFE_E_HIP;
+  } catch (const std::bad_alloc&) {
+    wfe::set_last_error("host allocation failed");
+    return WFE_E_NOMEM;
+  } catch (const std::exception& e) {
+    wfe::set_last_error(e.what());
+    return WFE_E_CONFIG;
+  }
+}
+
+int need_ctx(wfe_ctx* c) {
+  if (!c || !c->impl) {
+    wfe::set_last_error("null context");
+    return WFE_E_ARG;
+  }
+  return WFE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wfe_create(const wfe_grid* g, const wfe_params* p, const wfe_base* b, const wfe_metric* m,
+               int precision, wfe_ctx** out) {
+  return guarded([&] {
+    if (!g || !p || !b || !m || !out) throw wfe::ArgError("null argument");
+    *out = nullptr;
+    if (g->ng != 3) throw wfe::ArgError("ng must be 3");
+    if (g->nx < 4 || g->ny < 3 || g->nz < 4) throw wfe::ArgError("grid too small");
+    if (g->nz + 1 > 320)
+      throw wfe::ArgError("nz too large for the column solvers (nz <= 319)");
+    const double np = (double)(g->nx + 6) * (g->ny + 6) * (g->nz + 7);
+    if (np > 2.0e9) throw wfe::ArgError("padded grid exceeds 2^31 points");
+    const double* req[] = {b->thb, b->pib, b->rhob, b->dthbdz, b->thbw, b->rhobw,
+                           b->thvb, b->thvbw, b->qvb, b->ub, m->zeta_w, m->zeta_c,
+                           m->dzeta_c, m->dzeta_w, m->h, m->hx, m->hy, m->hx_u,
+                           m->hy_v, m->jac, m->fcor, m->mapf};
+    for (const double* r : req)
+      if (!r) throw wfe::ArgError("missing base-state or metric array");
+    if (precision != WFE_PREC_FP64 && precision != WFE_PREC_MIXED)
+      throw wfe::ArgError("unknown precision");
+    auto c = std::make_unique<wfe_ctx>();
+    c->precision = precision;
+    if (precision == WFE_PREC_FP64)
+      c->impl = std::make_unique<wfe::Ctx<double>>(*g, *p, *b, *m);
+    else
+      c->impl = std::make_unique<wfe::Ctx<float>>(*g, *p, *b, *m);
+    *out = c.release();
+  });
+}
+
+int wfe_destroy(wfe_ctx* ctx) {
+  delete ctx;
+  return WFE_OK;
+}
+
+int wfe_step(wfe_ctx* ctx, double dt, double /*t*/) {
+  if (int e = need_ctx(ctx)) return e;
+  return guarded([&] { ctx->impl->step(dt); });
+}
+
+int wfe_upload(wfe_ctx* ctx, int field, const double* host, size_t n) {
+  if (int e = need_ctx(ctx)) return e;
+  if (!host) return WFE_E_ARG;
+  return guarded([&] { ctx->impl->upload(field, host, n); });
+}
+
+int wfe_download(wfe_ctx* ctx, int field, double* host, size_t n) {
+  if (int e = need_ctx(ctx)) return e;
+  if (!host) return WFE_E_ARG;
+  return guarded([&] { ctx->impl->download(field, host, n); });
+}
+
+int wfe_absmax(wfe_ctx* ctx, int field, float* out) {
+  if (int e = need_ctx(ctx)) return e;
+  if (!out) return WFE_E_ARG;
+  return guarded([&] { *out = ctx->impl->absmax(field); });
+}
+
+int wfe_sync(wfe_ctx* ctx) {
+  if (int e = need_ctx(ctx)) return e;
+  return guarded([&] { ctx->impl->sync(); });
+}
+
+int wfe_set_bdy_weights(wfe_ctx* ctx, const double* wgt, size_t n) {
+  if (int e = need_ctx(ctx)) return e;
+  if (!wgt) return WFE_E_ARG;
+  return guarded([&] { ctx->impl->set_bdy_weights(wgt, n); });
+}
+
+int wfe_set_bdy_target(wfe_ctx* ctx, int slot, int var, const double* host, size_t n) {
+  if (int e = need_ctx(ctx)) return e;
+  if (!host) return WFE_E_ARG;
+  return guarded([&] { ctx->impl->set_bdy_target(slot, var, host, n); });
+}
+
+int wfe_swap_bdy_targets(wfe_ctx* ctx) {
+  if (int e = need_ctx(ctx)) return e;
+  return guarded([&] { ctx->impl->swap_bdy_targets(); });
+}
+
+int wfe_set_bdy_tfrac(wfe_ctx* ctx, double tfrac) {
+  if (int e = need_ctx(ctx)) return e;
+  return guarded([&] { ctx->impl->set_bdy_tfrac(tfrac); });
+}
+
+int wfe_apply_bcs(wfe_ctx* ctx) {
+  if (int e = need_ctx(ctx)) return e;
+  return guarded([&] { ctx->impl->apply_bcs(); });
+}
+
+int wfe_compute_tendencies(wfe_ctx* ctx, double dt) {
+  if (int e = need_ctx(ctx)) return e;
+  return guarded([&] { ctx->impl->compute_tendencies(dt); });
+}
+
+int wfe_acoustic_substeps(wfe_ctx* ctx, double dtau, int nsub) {
+  if (int e = need_ctx(ctx)) return e;
+  if (nsub < 1) return WFE_E_ARG;
+  return guarded([&] { ctx->impl->acoustic_substeps(dtau, nsub); });
+}
+
+int wfe_kessler(wfe_ctx* ctx, double dt) {
+  if (int e = need_ctx(ctx)) return e;
+  return guarded([&] { ctx->impl->kessler(dt); });
+}
+
+int wfe_kernel_timing(wfe_ctx* ctx, int mode) {
+  if (int e = need_ctx(ctx)) return e;
+  return guarded([&] { ctx->impl->kernel_timing(mode); });
+}
+
+int wfe_kernel_timing_report(wfe_ctx* ctx, char* buf, size_t n) {
+  if (int e = need_ctx(ctx)) return e;
+  if (!buf || n == 0) return WFE_E_ARG;
+  return guarded([&] {
+    const std::string r = ctx->impl->kernel_report();
+    if (r.size() + 1 > n) throw wfe::ArgError("report buffer too small");
+    std::memcpy(buf, r.c_str(), r.size() + 1);
+  });
+}
+
+const char* wfe_last_error(void) { return wfe::g_last_error.c_str(); }
+
+int wfe_abi_version(void) { return WFE_HIP_ABI_VERSION; }
+
+int wfe_device_name(char* buf, size_t n) {
+  return guarded([&] {
+    if (!buf || n == 0) throw wfe::ArgError("null buffer");
+    int dev = 0;
+    WFE_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop{};
+    WFE_HIP(hipGetDeviceProperties(&prop, dev));
+    std::snprintf(buf, n, "%s", prop.name);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,90 @@
+#pragma once
+// Device-side views and helpers shared by the gfx950 kernels.
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+namespace wfe::dev {
+
+inline constexpr double kGrav = 9.81, kCp = 1004.5, kRd = 287.04, kCv = kCp - kRd;
+inline constexpr double kP00 = 1.0e5, kLv = 2.5e6, kLf = 3.34e5, kLs = kLv + kLf;
+inline constexpr double kRv = 461.5, kEps61 = 0.61, kTfrz = 273.15;
+
+// Padded grid, reference layout (src/core/grid.hpp:22-36). Extents fit int.
+struct Grid {
+  int nx, ny, nz, ng, NX, NY, NZ;
+  int plane;  // NX*NY
+  double dx, dy;
+  __host__ __device__ int id(int i, int j, int k) const {
+    return ((k + ng) * NY + (j + ng)) * NX + (i + ng);
+  }
+  __host__ __device__ int id2(int i, int j) const { return (j + ng) * NX + (i + ng); }
+};
+
+struct Params {
+  double diff_K;
+  int coriolis_use_ub;
+  double rayleigh_zd, rayleigh_alpha;
+  double beta, smdiv;
+  int bcx, bcy;  // 1 = open
+  double cstar;
+  int moisture, pd_moist, w_damping, microphysics;
+};
+
+// Base state (DevProf, src/core/base_state.hpp:16-27). pib stays FP64 so
+// that pi = pibar + pi' is formed in FP64 in every precision mode.
+template <class T>
+struct Base {
+  const T *thb, *rhob, *dthbdz, *thbw, *rhobw, *qvb, *ub, *vb;
+  const double* pib;
+};
+
+// Metric (DevMetric, src/core/metric.hpp:16-30).
+template <class T>
+struct Metric {
+  const T *zeta_w, *zeta_c, *dzeta_c, *dzeta_w;
+  const T *hx, *hy, *hxu, *hyv, *jac, *fcor, *mapf;
+  T zt;
+};
+
+// Constant acoustic coefficients (AcousticCoef, src/dynamics/kernels.hpp:46-50).
+template <class T>
+struct Coef {
+  const T *rtjx, *rtjy, *kt3, *aw;
+};
+
+// 5th-order upwind interface value between q0 and q1 (WS2002;
+// src/dynamics/kernels.cu:51-57): the centred 6-point value minus/plus the
+// 5th undivided difference / 60, upwinded on the sign of vel (>= 0 -> minus).
+template <class T>
+__device__ __forceinline__ T iface5(T qm2, T qm1, T q0, T q1, T q2, T q3, T vel) {
+  const T c = (T)(1.0 / 60.0);
+  const T cen = c * ((T)37 * (q0 + q1) - (T)8 * (qm1 + q2) + (qm2 + q3));
+  const T upw = c * ((q3 - qm2) - (T)5 * (q2 - qm1) + (T)10 * (q1 - q0));
+  return (vel >= (T)0) ? (cen - upw) : (cen + upw);
+}
+
+// sin^2 Rayleigh damping rate above rayleigh_zd (kernels.cu:42-48).
+template <class T>
+__device__ __forceinline__ T ray_alpha(T zeta, T zt, const Params& p) {
+  const T zd = (T)p.rayleigh_zd;
+  if (zd < (T)0) return (T)0;
+  if (zeta <= zd || zt <= zd) return (T)0;
+  const T halfpi = (T)1.5707963267948966;
+  const T s = sin(halfpi * (zeta - zd) / (zt - zd));
+  return (T)p.rayleigh_alpha * s * s;
+}
+
+// Lateral ghost index maps, composed as the reference's fill order leaves
+// them (vertical, then x over all rows, then y over all columns): periodic
+// wrap or zero-gradient clamp to [0, last].
+__device__ __forceinline__ int map_periodic(int i, int n) {
+  i %= n;
+  return i < 0 ? i + n : i;
+}
+__device__ __forceinline__ int map_lat(int i, int n, int last, int open) {
+  if (open) return i < 0 ? 0 : (i > last ? last : i);
+  return map_periodic(i, n);
+}
+
+}  // namespace wfe::dev

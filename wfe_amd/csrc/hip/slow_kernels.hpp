@@ -1,0 +1,479 @@
+#pragma once
+// Stage-level kernels: ghost fills, mass fluxes, slow tendencies,
+// positive-definite moisture transport, boundary relaxation, moisture stage
+// update, |f| max. Templated on the storage/arithmetic type T (double for
+// the plain FP64 mode, float for the mixed mode).
+#include "hip/device_common.hpp"
+
+namespace wfe::dev {
+
+// ------------------------------------------------------------ ghost fills
+// The three fills reproduce apply_bcs (src/dynamics/kernels.cu:805-835):
+// vertical first, then x over every padded row, then y over every padded
+// column, so corners end up as the composed map of the interior.
+
+constexpr int kMaxFill = 9;
+enum FillKind { kFillScalar = 0, kFillW = 1 };
+
+// Up to kMaxFill fields of element type E plus per-field kind / last index.
+template <class E>
+struct FillList {
+  E* f[kMaxFill];
+  int kind[kMaxFill];
+  int xlast[kMaxFill];
+  int ylast[kMaxFill];
+  int n;
+};
+
+// Vertical: zero-gradient (kernels.cu:695-707) or the w rule
+// (kernels.cu:710-723): w(nz) = 0, odd reflection about w(0) below and
+// about w(nz) above.
+template <class E>
+__global__ void k_fill_z(Grid g, FillList<E> L) {
+  const int ir = blockIdx.x * blockDim.x + threadIdx.x;
+  const int jr = blockIdx.y * blockDim.y + threadIdx.y;
+  if (ir >= g.NX || jr >= g.NY) return;
+  const int col = jr * g.NX + ir;
+  const int st = g.plane;
+  for (int q = 0; q < L.n; ++q) {
+    E* f = L.f[q];
+    if (L.kind[q] == kFillW) {
+      f[col + st * (g.nz + g.ng)] = (E)0;
+      const E w0 = f[col + st * g.ng];
+      for (int m = 1; m <= g.ng; ++m) {
+        f[col + st * (g.ng - m)] = (E)2 * w0 - f[col + st * (g.ng + m)];
+        f[col + st * (g.nz + g.ng + m)] = -f[col + st * (g.nz + g.ng - m)];
+      }
+    } else {
+      const E bot = f[col + st * g.ng];
+      const E top = f[col + st * (g.nz - 1 + g.ng)];
+      for (int m = 1; m <= g.ng; ++m) f[col + st * (g.ng - m)] = bot;
+      for (int m = 1; m <= g.ng + 1; ++m) f[col + st * (g.nz - 1 + g.ng + m)] = top;
+    }
+  }
+}
+
+// x ghosts of every padded row: periodic (kernels.cu:725-733) or
+// zero-gradient up to the field's last valid index (kernels.cu:748-758;
+// writes beyond the row are dropped).
+template <class E>
+__global__ void k_fill_x(Grid g, int open, FillList<E> L) {
+  const int m = threadIdx.x;  // 0..ng-1
+  const int jr = blockIdx.x * blockDim.y + threadIdx.y;
+  const int kr = blockIdx.y;
+  if (m >= g.ng || jr >= g.NY || kr >= g.NZ) return;
+  const int row = (kr * g.NY + jr) * g.NX;
+  for (int q = 0; q < L.n; ++q) {
+    E* f = L.f[q];
+    if (open) {
+      const int il = L.xlast[q];
+      const E left = f[row + g.ng];
+      const E right = f[row + g.ng + il];
+      f[row + g.ng - 1 - m] = left;
+      const int ir = g.ng + il + 1 + m;
+      if (ir < g.NX) f[row + ir] = right;
+    } else {
+      f[row + g.ng + g.nx + m] = f[row + g.ng + m];
+      f[row + g.ng - 1 - m] = f[row + g.ng + g.nx - 1 - m];
+    }
+  }
+}
+
+// y ghosts of every padded column (kernels.cu:735-745, 760-770).
+template <class E>
+__global__ void k_fill_y(Grid g, int open, FillList<E> L) {
+  const int ir = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = threadIdx.y;
+  const int kr = blockIdx.y;
+  if (ir >= g.NX || m >= g.ng || kr >= g.NZ) return;
+  const int plane = kr * g.plane;
+  for (int q = 0; q < L.n; ++q) {
+    E* f = L.f[q];
+    if (open) {
+      const int jl = L.ylast[q];
+      const E lo = f[plane + g.ng * g.NX + ir];
+      const E hi = f[plane + (g.ng + jl) * g.NX + ir];
+      f[plane + (g.ng - 1 - m) * g.NX + ir] = lo;
+      const int jr = g.ng + jl + 1 + m;
+      if (jr < g.NY) f[plane + jr * g.NX + ir] = hi;
+    } else {
+      f[plane + (g.ng + g.ny + m) * g.NX + ir] = f[plane + (g.ng + m) * g.NX + ir];
+      f[plane + (g.ng - 1 - m) * g.NX + ir] = f[plane + (g.ng + g.ny - 1 - m) * g.NX + ir];
+    }
+  }
+}
+
+// ------------------------------------------------------------ mass fluxes
+// rho~-weighted contravariant mass fluxes on the ring the advection reads
+// (k_massflux_xy / k_massflux_z, kernels.cu:68-97): mfx, mfy on
+// i,j in [-2, n+2]; mfz = rho_w (w - (u^xz h_x + v^yz h_y)(1 - zeta_w/zt))
+// on i,j in [-2, n+1], k in [1, nz) (k = 0 and nz stay 0: Omega = 0).
+template <class T>
+__global__ void k_massflux(Grid g, Base<T> b, Metric<T> m, const T* __restrict__ u,
+                           const T* __restrict__ v, const T* __restrict__ w, T* mfx, T* mfy,
+                           T* mfz) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) - 2;
+  const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) - 2;
+  const int k = blockIdx.z;
+  if (i > g.nx + 2 || j > g.ny + 2 || k >= g.nz) return;
+  const int c = g.id(i, j, k);
+  const int c2 = g.id2(i, j);
+  const T rj = b.rhob[c] * m.jac[c2];
+  const T rjx = (T)0.5 * (b.rhob[c - 1] * m.jac[c2 - 1] + rj);
+  mfx[c] = rjx * u[c];
+  const T rjy = (T)0.5 * (b.rhob[c - g.NX] * m.jac[c2 - g.NX] + rj);
+  mfy[c] = rjy * v[c];
+  if (k >= 1 && i <= g.nx + 1 && j <= g.ny + 1) {
+    const int kk = k + g.ng;
+    const T fac = (T)1 - m.zeta_w[kk] / m.zt;
+    const int cm = c - g.plane;
+    const T uw = (T)0.25 * (u[cm] + u[cm + 1] + u[c] + u[c + 1]);
+    const T vw = (T)0.25 * (v[cm] + v[cm + g.NX] + v[c] + v[c + g.NX]);
+    const T cross = (uw * m.hx[c2] + vw * m.hy[c2]) * fac;
+    mfz[c] = b.rhobw[c] * (w[c] - cross);
+  }
+}
+
+// Flux divergence for the advective-consistency term (k_divergence
+// kernels.cu:99-110) on i,j in [-1, n].
+template <class T>
+__global__ void k_divergence(Grid g, Metric<T> m, const T* __restrict__ mfx,
+                             const T* __restrict__ mfy, const T* __restrict__ mfz, T* div) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) - 1;
+  const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) - 1;
+  const int k = blockIdx.z;
+  if (i > g.nx || j > g.ny || k >= g.nz) return;
+  const int c = g.id(i, j, k);
+  div[c] = (mfx[c + 1] - mfx[c]) / (T)g.dx + (mfy[c + g.NX] - mfy[c]) / (T)g.dy +
+           (mfz[c + g.plane] - mfz[c]) / m.dzeta_c[k + g.ng];
+}
+
+// Flux through the face whose upper cell is c (stride s along the normal):
+// F = f * iface5(q[c-3s .. c+2s], f).
+template <class T>
+__device__ __forceinline__ T face_flux(const T* __restrict__ q, int c, int s, T f) {
+  return f * iface5(q[c - 3 * s], q[c - 2 * s], q[c - s], q[c], q[c + s], q[c + 2 * s], f);
+}
+
+// Advective-consistent flux-form tendency of a cell-centred scalar at c:
+// -(1/rho~)[div(MF q) - q div(MF)] (k_tend_thp / k_tend_q, kernels.cu:114-179).
+template <class T>
+__device__ __forceinline__ T scalar_adv(const Grid& g, const T* __restrict__ q,
+                                        const T* __restrict__ mfx, const T* __restrict__ mfy,
+                                        const T* __restrict__ mfz, const T* __restrict__ div,
+                                        int c, T dzc, T rt) {
+  const T fxL = mfx[c], fxR = mfx[c + 1];
+  const T fyL = mfy[c], fyR = mfy[c + g.NX];
+  const T fzB = mfz[c], fzT = mfz[c + g.plane];
+  const T FxL = face_flux(q, c, 1, fxL);
+  const T FxR = face_flux(q, c + 1, 1, fxR);
+  const T FyL = face_flux(q, c, g.NX, fyL);
+  const T FyR = face_flux(q, c + g.NX, g.NX, fyR);
+  const T FzB = face_flux(q, c, g.plane, fzB);
+  const T FzT = face_flux(q, c + g.plane, g.plane, fzT);
+  const T fdiv = (FxR - FxL) / (T)g.dx + (FyR - FyL) / (T)g.dy + (FzT - FzB) / dzc;
+  return (-fdiv + q[c] * div[c]) / rt;
+}
+
+// Constant-K Laplacian with the local physical vertical spacing
+// (k_diffuse kernels.cu:430-443).
+template <class T>
+__device__ __forceinline__ T laplacian(const Grid& g, const T* __restrict__ f, int c, T dzl) {
+  const T cc = f[c];
+  return (f[c + 1] - (T)2 * cc + f[c - 1]) / (T)(g.dx * g.dx) +
+         (f[c + g.NX] - (T)2 * cc + f[c - g.NX]) / (T)(g.dy * g.dy) +
+         (f[c + g.plane] - (T)2 * cc + f[c - g.plane]) / (dzl * dzl);
+}
+
+struct TendOut {
+  void *tu, *tv, *tw, *tth, *tq[3];
+};
+
+// All slow tendencies of one interior point (compute_tendencies,
+// kernels.cu:872-913): u, v, w (k >= 1), theta' (+ Rayleigh), the three
+// moisture species when the non-PD scheme is used, and the const-K
+// diffusion of u, v, w, theta' when diff_K > 0.
+template <class T>
+__global__ void k_tendencies(Grid g, Params p, Base<T> b, Metric<T> m, T dt,
+                             const T* __restrict__ u, const T* __restrict__ v,
+                             const T* __restrict__ w, const T* __restrict__ thp,
+                             const T* __restrict__ qv, const T* __restrict__ qc,
+                             const T* __restrict__ qr, const T* __restrict__ mfx,
+                             const T* __restrict__ mfy, const T* __restrict__ mfz,
+                             const T* __restrict__ div, TendOut out, int do_q) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+  const int kk = k + g.ng;
+  const int c = g.id(i, j, k);
+  const int c2 = g.id2(i, j);
+  const int NX = g.NX, PL = g.plane;
+  const T dx = (T)g.dx, dy = (T)g.dy;
+  const T dzc = m.dzeta_c[kk];
+  const T rt = b.rhob[c] * m.jac[c2];
+  const T K = (T)p.diff_K;
+
+  // ---- u (k_tend_u, kernels.cu:308-346)
+  {
+    const T fcL = (T)0.5 * (mfx[c - 1] + mfx[c]);
+    const T fcR = (T)0.5 * (mfx[c] + mfx[c + 1]);
+    const T FxL = face_flux(u, c, 1, fcL);
+    const T FxR = face_flux(u, c + 1, 1, fcR);
+    const T fyB = (T)0.5 * (mfy[c - 1] + mfy[c]);
+    const T fyT = (T)0.5 * (mfy[c - 1 + NX] + mfy[c + NX]);
+    const T FyB = face_flux(u, c, NX, fyB);
+    const T FyT = face_flux(u, c + NX, NX, fyT);
+    const T fzB = (T)0.5 * (mfz[c - 1] + mfz[c]);
+    const T fzT = (T)0.5 * (mfz[c - 1 + PL] + mfz[c + PL]);
+    const T FzB = face_flux(u, c, PL, fzB);
+    const T FzT = face_flux(u, c + PL, PL, fzT);
+    const T fdiv = (FxR - FxL) / dx + (FyT - FyB) / dy + (FzT - FzB) / dzc;
+    const T dv = (T)0.5 * (div[c - 1] + div[c]);
+    const T ru = (T)0.5 * (b.rhob[c - 1] * m.jac[c2 - 1] + rt);
+    const T vavg = (T)0.25 * (v[c - 1] + v[c - 1 + NX] + v[c] + v[c + NX]);
+    const T fc = (T)0.5 * (m.fcor[c2 - 1] + m.fcor[c2]);
+    const T vref = p.coriolis_use_ub ? b.vb[kk] : (T)0;
+    const T relax = -ray_alpha(m.zeta_c[kk], m.zt, p) * (u[c] - b.ub[kk]);
+    T t = (-fdiv + u[c] * dv) / ru + fc * (vavg - vref) + relax;
+    if (K > (T)0) t += K * laplacian(g, u, c, dzc * m.jac[c2]);
+    static_cast<T*>(out.tu)[c] = t;
+  }
+  // ---- v (k_tend_v, kernels.cu:348-385)
+  {
+    const T fcB = (T)0.5 * (mfy[c - NX] + mfy[c]);
+    const T fcT = (T)0.5 * (mfy[c] + mfy[c + NX]);
+    const T FyB = face_flux(v, c, NX, fcB);
+    const T FyT = face_flux(v, c + NX, NX, fcT);
+    const T fxL = (T)0.5 * (mfx[c - NX] + mfx[c]);
+    const T fxR = (T)0.5 * (mfx[c + 1 - NX] + mfx[c + 1]);
+    const T FxL = face_flux(v, c, 1, fxL);
+    const T FxR = face_flux(v, c + 1, 1, fxR);
+    const T fzB = (T)0.5 * (mfz[c - NX] + mfz[c]);
+    const T fzT = (T)0.5 * (mfz[c - NX + PL] + mfz[c + PL]);
+    const T FzB = face_flux(v, c, PL, fzB);
+    const T FzT = face_flux(v, c + PL, PL, fzT);
+    const T fdiv = (FxR - FxL) / dx + (FyT - FyB) / dy + (FzT - FzB) / dzc;
+    const T dv = (T)0.5 * (div[c - NX] + div[c]);
+    const T rv = (T)0.5 * (b.rhob[c - NX] * m.jac[c2 - NX] + rt);
+    const T uavg = (T)0.25 * (u[c - NX] + u[c + 1 - NX] + u[c] + u[c + 1]);
+    const T fc = (T)0.5 * (m.fcor[c2 - NX] + m.fcor[c2]);
+    const T uref = p.coriolis_use_ub ? b.ub[kk] : (T)0;
+    const T relax = -ray_alpha(m.zeta_c[kk], m.zt, p) * (v[c] - b.vb[kk]);
+    T t = (-fdiv + v[c] * dv) / rv - fc * (uavg - uref) + relax;
+    if (K > (T)0) t += K * laplacian(g, v, c, dzc * m.jac[c2]);
+    static_cast<T*>(out.tv)[c] = t;
+  }
+  // ---- w on k >= 1 (k_tend_w, kernels.cu:387-425)
+  if (k >= 1) {
+    const T dzw = m.dzeta_w[kk];
+    const T fxL = (T)0.5 * (mfx[c - PL] + mfx[c]);
+    const T fxR = (T)0.5 * (mfx[c + 1 - PL] + mfx[c + 1]);
+    const T FxL = face_flux(w, c, 1, fxL);
+    const T FxR = face_flux(w, c + 1, 1, fxR);
+    const T fyB = (T)0.5 * (mfy[c - PL] + mfy[c]);
+    const T fyT = (T)0.5 * (mfy[c + NX - PL] + mfy[c + NX]);
+    const T FyB = face_flux(w, c, NX, fyB);
+    const T FyT = face_flux(w, c + NX, NX, fyT);
+    const T fzB = (T)0.5 * (mfz[c - PL] + mfz[c]);
+    const T fzT = (T)0.5 * (mfz[c] + mfz[c + PL]);
+    const T FzB = face_flux(w, c, PL, fzB);
+    const T FzT = face_flux(w, c + PL, PL, fzT);
+    const T fdiv = (FxR - FxL) / dx + (FyT - FyB) / dy + (FzT - FzB) / dzw;
+    const T dv = (T)0.5 * (div[c - PL] + div[c]);
+    const T rw = b.rhobw[c] * m.jac[c2];
+    T relax = -ray_alpha(m.zeta_w[kk], m.zt, p) * w[c];
+    const T wk = w[c];
+    if (p.w_damping) {
+      const T cr = fabs(wk) * dt / (dzw * m.jac[c2]);
+      if (cr > (T)1) relax -= (T)0.3 * (cr - (T)1) * wk / dt;
+    }
+    T t = (-fdiv + wk * dv) / rw + relax;
+    if (K > (T)0) t += K * laplacian(g, w, c, dzw * m.jac[c2]);
+    static_cast<T*>(out.tw)[c] = t;
+  }
+  // ---- theta' (k_tend_thp, kernels.cu:114-146)
+  {
+    T t = scalar_adv(g, thp, mfx, mfy, mfz, div, c, dzc, rt) -
+          ray_alpha(m.zeta_c[kk], m.zt, p) * thp[c];
+    if (K > (T)0) t += K * laplacian(g, thp, c, dzc * m.jac[c2]);
+    static_cast<T*>(out.tth)[c] = t;
+  }
+  // ---- moisture, non-PD scheme (k_tend_q, kernels.cu:149-179)
+  if (do_q) {
+    static_cast<T*>(out.tq[0])[c] = scalar_adv(g, qv, mfx, mfy, mfz, div, c, dzc, rt);
+    static_cast<T*>(out.tq[1])[c] = scalar_adv(g, qc, mfx, mfy, mfz, div, c, dzc, rt);
+    static_cast<T*>(out.tq[2])[c] = scalar_adv(g, qr, mfx, mfy, mfz, div, c, dzc, rt);
+  }
+}
+
+// ------------------------------------------ positive-definite moisture (PD)
+// Skamarock (2006) flux renormalisation (kernels.cu:187-264), the three
+// species in one launch each. F*[c] is the flux through the west / south /
+// bottom face of cell c.
+template <class T>
+struct PDSet {
+  const T* q[3];
+  T* fx[3];
+  T* fy[3];
+  T* fz[3];
+  T* ratio[3];
+  T* tq[3];
+};
+
+template <class T>
+__global__ void k_pd_flux(Grid g, const T* __restrict__ mfx, const T* __restrict__ mfy,
+                          const T* __restrict__ mfz, PDSet<T> S) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i > g.nx || j > g.ny || k > g.nz) return;
+  const int c = g.id(i, j, k);
+  const bool ix = j < g.ny && k < g.nz;              // faces i = 0..nx
+  const bool iy = i < g.nx && k < g.nz;              // faces j = 0..ny
+  const bool iz = i < g.nx && j < g.ny;              // faces k = 0..nz
+  for (int s = 0; s < 3; ++s) {
+    const T* q = S.q[s];
+    if (ix) S.fx[s][c] = face_flux(q, c, 1, mfx[c]);
+    if (iy) S.fy[s][c] = face_flux(q, c, g.NX, mfy[c]);
+    if (iz) S.fz[s][c] = face_flux(q, c, g.plane, mfz[c]);
+  }
+}
+
+template <class T>
+__global__ void k_pd_ratio(Grid g, Base<T> b, Metric<T> m, T dt, PDSet<T> S) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+  const int c = g.id(i, j, k);
+  const T dzc = m.dzeta_c[k + g.ng];
+  const T dx = (T)g.dx, dy = (T)g.dy;
+  const T rt = b.rhob[c] * m.jac[g.id2(i, j)];
+  for (int s = 0; s < 3; ++s) {
+    const T* Fx = S.fx[s];
+    const T* Fy = S.fy[s];
+    const T* Fz = S.fz[s];
+    const T out = fmax((T)0, Fx[c + 1]) / dx + fmax((T)0, -Fx[c]) / dx +
+                  fmax((T)0, Fy[c + g.NX]) / dy + fmax((T)0, -Fy[c]) / dy +
+                  fmax((T)0, Fz[c + g.plane]) / dzc + fmax((T)0, -Fz[c]) / dzc;
+    const T avail = rt * fmax((T)0, S.q[s][c]);
+    const T r = (dt * out > (T)1e-20) ? avail / (dt * out) : (T)1;
+    S.ratio[s][c] = fmin((T)1, r);
+  }
+}
+
+// Renormalised flux divergence + consistency term. Ratio reads outside the
+// interior follow the reference's lateral fill of the ratio field
+// (kernels.cu:865-866); vertically they only multiply the zero fluxes at
+// k = 0 and nz.
+template <class T>
+__global__ void k_pd_apply(Grid g, Params p, Base<T> b, Metric<T> m,
+                           const T* __restrict__ mfdiv, PDSet<T> S) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+  const int c = g.id(i, j, k);
+  const T dzc = m.dzeta_c[k + g.ng];
+  const T rt = b.rhob[c] * m.jac[g.id2(i, j)];
+  auto ridx = [&](int ii, int jj, int kz) {
+    ii = map_lat(ii, g.nx, g.nx - 1, p.bcx);
+    jj = map_lat(jj, g.ny, g.ny - 1, p.bcy);
+    kz = kz < 0 ? 0 : (kz > g.nz - 1 ? g.nz - 1 : kz);
+    return g.id(ii, jj, kz);
+  };
+  for (int s = 0; s < 3; ++s) {
+    const T* Fx = S.fx[s];
+    const T* Fy = S.fy[s];
+    const T* Fz = S.fz[s];
+    const T* R = S.ratio[s];
+    auto sx = [&](int i0) {
+      const T f = Fx[g.id(i0, j, k)];
+      return f * R[ridx(f >= (T)0 ? i0 - 1 : i0, j, k)];
+    };
+    auto sy = [&](int j0) {
+      const T f = Fy[g.id(i, j0, k)];
+      return f * R[ridx(i, f >= (T)0 ? j0 - 1 : j0, k)];
+    };
+    auto sz = [&](int k0) {
+      const T f = Fz[g.id(i, j, k0)];
+      return f * R[ridx(i, j, f >= (T)0 ? k0 - 1 : k0)];
+    };
+    const T fdiv = (sx(i + 1) - sx(i)) / (T)g.dx + (sy(j + 1) - sy(j)) / (T)g.dy +
+                   (sz(k + 1) - sz(k)) / dzc;
+    S.tq[s][c] = (-fdiv + S.q[s][c] * mfdiv[c]) / rt;
+  }
+}
+
+// ------------------------------------------------- boundary relaxation
+// Davies rim relaxation toward the time-interpolated targets
+// (k_bdy_relax kernels.cu:292-304): T += w(i,j) [(1-tf) lo + tf hi - f] for
+// u, v, theta', qv; the pi' tendency is set (it is zero otherwise).
+template <class T>
+struct RelaxSet {
+  const T* f[4];      // u v thp qv (estimate state)
+  const double* pip;  // estimate pi'
+  const T* lo[5];
+  const T* hi[5];
+  T* t[5];  // tu tv tth tpi tqv
+  const T* wgt;
+  T tf;
+};
+
+template <class T>
+__global__ void k_bdy_relax(Grid g, RelaxSet<T> R) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+  const int c = g.id(i, j, k);
+  const T w = R.wgt[g.id2(i, j)];
+  const T tf = R.tf;
+  if (w <= (T)0) {
+    R.t[3][c] = (T)0;
+    return;
+  }
+  const int fi[4] = {0, 1, 2, 4};
+  for (int q = 0; q < 4; ++q) {
+    const int v = fi[q];
+    R.t[v][c] += w * (((T)1 - tf) * R.lo[v][c] + tf * R.hi[v][c] - R.f[q][c]);
+  }
+  R.t[3][c] = w * (((T)1 - tf) * R.lo[3][c] + tf * R.hi[3][c] - (T)R.pip[c]);
+}
+
+// ---------------------------------------------------- moisture stage update
+// out = s_n + dt_stage * T over all padded points (k_stage_q kernels.cu:268-273).
+template <class T>
+__global__ void k_stage_q(int n, T dt, const T* __restrict__ q0, const T* __restrict__ q1,
+                          const T* __restrict__ q2, const T* __restrict__ t0,
+                          const T* __restrict__ t1, const T* __restrict__ t2, T* o0, T* o1,
+                          T* o2) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  o0[idx] = q0[idx] + dt * t0[idx];
+  o1[idx] = q1[idx] + dt * t1[idx];
+  o2[idx] = q2[idx] + dt * t2[idx];
+}
+
+// ------------------------------------------------------------ |f| max
+// NaN-safe max|f| over the interior (k in [0, kmax]) as IEEE bits of a
+// non-negative float: NaN bit patterns order above every finite value
+// (k_absmax kernels.cu:278-289).
+template <class E>
+__global__ void k_absmax(Grid g, const E* __restrict__ f, int kmax, unsigned int* result) {
+  __shared__ unsigned int smax;
+  if (threadIdx.x == 0 && threadIdx.y == 0) smax = 0;
+  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i < g.nx && j < g.ny && k <= kmax) {
+    const float a = fabsf((float)f[g.id(i, j, k)]);
+    atomicMax(&smax, __float_as_uint(a));
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && threadIdx.y == 0) atomicMax(result, smax);
+}
+
+}  // namespace wfe::dev
