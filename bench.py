@@ -38,12 +38,16 @@ ALG_BYTES_STEP = {"mixed": 1520, "fp64": 2712}
 
 # Per-kernel algorithmic bytes per interior point per launch (distinct arrays
 # the kernel must read and write, moist case).
-#  acou_wpi: r u v w thp tw tth thbw dthbdz rtjx rtjy kt3 aw qv qc qr qvb (16)
-#            + pip (8 B); w thp (4 B) + pip pipprev (8 B) written.
-#  acou_uv : r u v thp qv tu tv thb (7) + pip pipprev (8 B); w u v.
+#  acou_wpi      : r u v w thp tw tth thbw dthbdz rtjx rtjy kt3 aw qv qc qr qvb
+#                  (16 x 4 B) + pip (8 B); w thp (4 B) + pip (8 B) written.
+#  acou_uv       : r u v thp qv tu tv thb (7 x 4 B) + pip pipprev (8 B); w u v.
+#  acoustic_fused: one whole substep, the 27-access list above minus the
+#                  pi'prev write (the rotation makes it the previous buffer):
+#                  r 19 x 4 B + pip pipprev 8 B; w u v w thp 4 B + pip 8 B.
 KERNEL_BYTES = {
-    "mixed": {"acou_wpi": 16 * 4 + 8 + 2 * 4 + 2 * 8, "acou_uv": 7 * 4 + 2 * 8 + 2 * 4},
-    "fp64": {"acou_wpi": 21 * 8, "acou_uv": 11 * 8},
+    "mixed": {"acou_wpi": 16 * 4 + 8 + 2 * 4 + 8, "acou_uv": 7 * 4 + 2 * 8 + 2 * 4,
+              "acoustic_fused": 19 * 4 + 2 * 8 + 4 * 4 + 8},
+    "fp64": {"acou_wpi": 20 * 8, "acou_uv": 11 * 8, "acoustic_fused": 26 * 8},
 }
 
 
@@ -78,9 +82,10 @@ def kernel_profile(case, precision: str, warmup: int, steps: int):
     gi.kernel_timing(2)
     for s in range(warmup, warmup + steps):
         gi.step(dt, s * dt)
-    rep = gi.kernel_report()
+    rep = gi.kernel_times()
+    tune = gi.tuning()
     gi.kernel_timing(0)
-    return rep
+    return rep, tune
 
 
 def cpu_baseline(case, budget_s: float):
@@ -141,10 +146,13 @@ def main() -> int:
         errs[f] = float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a))
     del gi_m, gi_d
 
+    from wfe_amd.case import copy_bandwidth
+    copy_bw = copy_bandwidth(1 << 30, 20)
+    log(f"[bench] streaming copy probe: {copy_bw:.0f} GB/s")
     value = N * K / t_m
     value_fp64 = N * K / t_d
     log(f"[bench] kernel profile: {args.profile_steps} steps")
-    rep = kernel_profile(case, "mixed", 5, args.profile_steps)
+    rep, tune = kernel_profile(case, "mixed", 5, args.profile_steps)
     dom = max(rep, key=lambda k: rep[k][1])
     cnt, tot_ms = rep[dom]
     avg_s = tot_ms / cnt / 1e3
@@ -190,6 +198,8 @@ def main() -> int:
         "roofline": roof,
         "kernels_ms": {k: {"launches": v[0], "total_ms": v[1], "avg_us": 1e3 * v[1] / v[0]}
                        for k, v in sorted(rep.items(), key=lambda kv: -kv[1][1])},
+        "acoustic_tuning": tune,
+        "copy_probe_GBps": copy_bw,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

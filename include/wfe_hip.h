@@ -163,6 +163,11 @@ int wfe_kernel_timing(wfe_ctx* ctx, int mode);
 /* Per-kernel totals as text lines "name launches total_ms\n" (synchronous). */
 int wfe_kernel_timing_report(wfe_ctx* ctx, char* buf, size_t n);
 
+/* Device-wide streaming copy of `bytes` (rounded down to 16 B), repeated
+ * reps times on a private stream: achieved read+write bandwidth in GB/s.
+ * No counterpart in the reference; used to state the measured HBM peak. */
+int wfe_copy_probe(size_t bytes, int reps, double* gbps);
+
 /* ---- introspection ---- */
 const char* wfe_last_error(void);
 int wfe_abi_version(void);

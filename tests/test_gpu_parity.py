@@ -160,3 +160,72 @@ def test_absmax_matches_interior_max():
     om, gi = oracle_and_gpu(c, "mixed", st)
     for f in ("w", "thp", "u"):
         assert gi.absmax(f) == pytest.approx(om.absmax(f), rel=1e-6)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 2e-5)])
+def test_column_kernel_variant_matches_oracle(monkeypatch, precision, tol):
+    """The one-thread-per-column w-pi' solve (used when nz > 48) stays in parity."""
+    monkeypatch.setenv("WFE_WPI_KERNEL", "column")
+    c = kitchen_sink()
+    st = perturbed_state(c, 13)
+    om, gi = oracle_and_gpu(c, precision, st)
+    dt = c.scalar("dt")
+    for s in range(3):
+        om.step(dt)
+        gi.step(dt)
+    _cmp(c.grid, om, gi, ("u", "v", "w", "thp"), tol, f"column kernel {precision}")
+
+
+def test_deep_column_uses_general_kernel():
+    """nz = 60 exceeds the tiled kernel's level budget: general kernel path."""
+    c = kitchen_sink(nz=60, dz=300)
+    st = perturbed_state(c, 14)
+    om, gi = oracle_and_gpu(c, "fp64", st)
+    dt = c.scalar("dt")
+    for s in range(2):
+        om.step(dt)
+        gi.step(dt)
+    _cmp(c.grid, om, gi, STATE, 1e-10, "nz=60 fp64")
+
+
+FUSED_SHAPES = ["fused-8x2x16", "fused-16x1x16", "fused-4x4x16", "fused-8x4x8", "fused-32x1x8",
+                "fused-8x2x24"]
+SPLIT_SHAPES = ["split-32x8", "split-32x16", "split-64x8", "split-16x16", "split-col"]
+
+
+@pytest.mark.parametrize("shape", FUSED_SHAPES + SPLIT_SHAPES)
+def test_fused_acoustic_shapes_match_oracle(monkeypatch, shape):
+    """Every tile shape of the fused substep (and the split two-kernel path)
+    is in FP64 parity with the oracle over full steps, open and periodic."""
+    monkeypatch.setenv("WFE_ACOUSTIC_KERNEL", shape)
+    for bc in ("open", "periodic"):
+        c = kitchen_sink(bc_x=bc, bc_y=bc, nx=21, ny=13)  # ragged against every tile
+        st = perturbed_state(c, 31)
+        om, gi = oracle_and_gpu(c, "fp64", st)
+        dt = c.scalar("dt")
+        for s in range(3):
+            om.step(dt)
+            gi.step(dt)
+        _cmp(c.grid, om, gi, STATE, 1e-10, f"{shape} {bc}")
+        assert f"tuned:{shape}" in gi.kernel_report(), gi.kernel_report()
+
+
+def test_fused_shapes_agree_to_rounding(monkeypatch):
+    """Every substep method runs the same arithmetic per point; only the
+    compiler's FMA contraction may differ between instantiations."""
+    c = kitchen_sink(nx=21, ny=13)
+    st = perturbed_state(c, 32)
+    outs = {}
+    for shape in FUSED_SHAPES + SPLIT_SHAPES:
+        monkeypatch.setenv("WFE_ACOUSTIC_KERNEL", shape)
+        from wfe_amd import Integrator
+        gi = Integrator.from_case(c, "mixed")
+        for f, a in st.items():
+            gi.set(f, a)
+        for s in range(2):
+            gi.step(c.scalar("dt"))
+        outs[shape] = {f: gi.get(f) for f in ("u", "v", "w", "thp", "pip")}
+    ref = outs[FUSED_SHAPES[0]]
+    for shape, o in outs.items():
+        for f in ref:
+            assert rel_err(o[f], ref[f]) < 1e-6, (shape, f, rel_err(o[f], ref[f]))

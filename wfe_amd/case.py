@@ -96,6 +96,13 @@ class Case:
         return Integrator.from_case(self, precision)
 
 
+def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 20) -> float:
+    """Achieved HBM read+write bandwidth of a streaming copy [GB/s]."""
+    out = C.c_double()
+    check(lib().wfe_copy_probe(nbytes, reps, C.byref(out)), "wfe_copy_probe")
+    return out.value
+
+
 class Integrator:
     """gfx950 integrator context (src/dynamics/integrator.hpp:18-42)."""
 
@@ -176,6 +183,19 @@ class Integrator:
             name, cnt, ms = line.split()
             out[name] = (int(cnt), float(ms))
         return out
+
+    def kernel_times(self) -> dict:
+        """Launch count and total ms per kernel (tuning entries removed)."""
+        return {k: v for k, v in self.kernel_report().items()
+                if v[0] > 0 and not k.startswith(("tuned:", "cand:"))}
+
+    def tuning(self) -> dict:
+        """The acoustic-substep shape the context chose and every candidate's
+        average time [ms] from its tuning pass."""
+        rep = self.kernel_report()
+        chosen = next((k[6:] for k in rep if k.startswith("tuned:")), None)
+        return {"chosen": chosen,
+                "candidates_ms": {k[5:]: v[1] for k, v in rep.items() if k.startswith("cand:")}}
 
     # -- phase-level entry points (kernels.hpp free functions)
     def apply_bcs(self) -> None:

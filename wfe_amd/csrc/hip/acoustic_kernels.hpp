@@ -309,11 +309,472 @@ __global__ void __launch_bounds__(64) k_acou_wpi_col(Grid g, Params p, Base<T> b
     const int c = c0 + k * PL;
     const double oldpi = a.pip_src[c];
     a.pip_dst[c] = P[k] - E[k] * (Av_at(k + 1) * ds[k + 1] - Av_at(k) * ds[k]);
-    a.piprev_dst[c] = oldpi;
+    if (a.piprev_dst) a.piprev_dst[c] = oldpi;
     const double wc = 0.5 * (ds[k] + ds[k + 1]);
     a.thp_dst[c] = (T)((double)a.thp_src[c] +
                        dtau * ((double)a.tth[c] - wc * (double)b.dthbdz[c]));
     a.w_dst[c] = (T)ds[k];
+  }
+}
+
+// Tiled, k-parallel form of the same column solve. A workgroup owns TX
+// consecutive columns of one row j and NW thread rows; thread row r handles
+// the levels k = r, r + NW, ... . Three phases, separated by barriers:
+//   1. every thread assembles, for its levels, P_k, E_k, Av_k and the
+//      tridiagonal pieces G_k, RHS_k into LDS (FP64) -- all loads of the
+//      column are issued here, in parallel over k;
+//   2. thread row 0 (one lane per column) runs the Thomas elimination and
+//      the back substitution in the reference's order from LDS;
+//   3. every thread writes w, pi', pi'prev and theta' of its levels.
+// LDS: 5 (nz + 1) TX doubles. Levels per thread are bounded by KPT.
+template <int TX, int NW, int KPT>
+struct WpiTile {
+  static constexpr int kThreads = TX * NW;
+  static size_t lds_bytes(int nz) { return (size_t)5 * (nz + 1) * TX * sizeof(double); }
+};
+
+template <class T, int TX, int NW, int KPT>
+__global__ void __launch_bounds__(TX * NW) k_acou_wpi_tile(Grid g, Params p, Base<T> b,
+                                                           Metric<T> m, Coef<T> cf,
+                                                           WPArgs<T> a) {
+  extern __shared__ double lds[];
+  const int L = g.nz + 1;
+  double* sP = lds;          // P_k            k in [0, nz)
+  double* sE = sP + L * TX;  // E_k            k in [0, nz)
+  double* sG = sE + L * TX;  // G_k -> cs_k    k in [1, nz)
+  double* sR = sG + L * TX;  // RHS_k -> ds_k -> W_k, k in [0, nz]
+  double* sA = sR + L * TX;  // Av_k           k in [0, nz]
+
+  const int lx = threadIdx.x, ry = threadIdx.y;
+  const int i = blockIdx.x * TX + lx;
+  const int j = blockIdx.y;
+  const bool col = i < g.nx;
+  const double c1 = (1.0 + p.beta) * 0.5;
+  const double c2 = (1.0 - p.beta) * 0.5;
+  const double dtau = a.dtau;
+  const int NX = g.NX, PL = g.plane;
+  const int c2d = g.id2(col ? i : 0, j);
+  const double J = (double)m.jac[c2d];
+  const double hxc = (double)m.hx[c2d];
+  const double hyc = (double)m.hy[c2d];
+  const double mc = (double)m.mapf[c2d];
+  const double zt = (double)m.zt;
+  const int c0 = g.id(col ? i : 0, j, 0);
+
+  // Values kept in registers from phase 1 to phase 3.
+  double pold[KPT];
+  double thold[KPT], tthk[KPT], dthk[KPT];
+  double w0 = 0.0;
+
+  auto sv = [&](int k) -> double {  // cross-metric vertical flux at w level k
+    if (k <= 0 || k >= g.nz) return 0.0;
+    const int c = c0 + k * PL;
+    const double fac = 1.0 - (double)m.zeta_w[k + g.ng] / zt;
+    const double uw = 0.25 * ((double)a.u[c - PL] + (double)a.u[c - PL + 1] + (double)a.u[c] +
+                              (double)a.u[c + 1]);
+    const double vw = 0.25 * ((double)a.v[c - PL] + (double)a.v[c - PL + NX] +
+                              (double)a.v[c] + (double)a.v[c + NX]);
+    return (double)cf.aw[c] * (uw * hxc + vw * hyc) * fac;
+  };
+  auto av = [&](int k) -> double {
+    return (k <= 0 || k >= g.nz) ? 0.0 : (double)cf.aw[c0 + k * PL];
+  };
+
+  // ---- phase 1: assembly, parallel over k
+#pragma unroll
+  for (int t = 0; t < KPT; ++t) {
+    const int k = ry + NW * t;
+    if (col && k < g.nz) {
+      const int c = c0 + k * PL;
+      const int kk = k + g.ng;
+      const double Kt = (double)cf.kt3[c];
+      const double Dh = mc * (((double)cf.rtjx[c + 1] * (double)a.u[c + 1] -
+                               (double)cf.rtjx[c] * (double)a.u[c]) /
+                                  g.dx +
+                              ((double)cf.rtjy[c + NX] * (double)a.v[c + NX] -
+                               (double)cf.rtjy[c] * (double)a.v[c]) /
+                                  g.dy);
+      const double dzc = (double)m.dzeta_c[kk];
+      const double avk = av(k), avk1 = av(k + 1);
+      const double wk = (double)a.w_src[c];
+      const double wkp = (double)a.w_src[c + PL];
+      const double tpi = a.tpi ? (double)a.tpi[c] : 0.0;
+      const double pk = a.pip_src[c];
+      pold[t] = pk;
+      sP[k * TX + lx] = pk + dtau * tpi -
+                        dtau * Kt *
+                            (Dh - (sv(k + 1) - sv(k)) / dzc +
+                             c2 * (avk1 * wkp - avk * wk) / dzc);
+      sE[k * TX + lx] = dtau * Kt * c1 / dzc;
+      sA[k * TX + lx] = avk;
+      const double thk = (double)a.thp_src[c];
+      thold[t] = thk;
+      tthk[t] = (double)a.tth[c];
+      dthk[t] = (double)b.dthbdz[c];
+      if (k >= 1) {
+        const int cm = c - PL;
+        const double thm = (double)a.thp_src[cm];
+        double thvw = (double)b.thbw[c] + 0.5 * (thm + thk);
+        if (p.moisture) thvw *= 1.0 + kEps61 * 0.5 * ((double)a.qv[cm] + (double)a.qv[c]);
+        const double dzw = (double)m.dzeta_w[kk];
+        sG[k * TX + lx] = dtau * kCp * thvw * c1 / (J * dzw);
+        double buoy = kGrav * 0.5 * (thm + thk) / (double)b.thbw[c];
+        if (p.moisture) {
+          const double dqv = 0.5 * ((double)a.qv[cm] + (double)a.qv[c]) -
+                             0.5 * ((double)b.qvb[cm] + (double)b.qvb[c]);
+          const double qcr = 0.5 * ((double)a.qc[cm] + (double)a.qc[c]) +
+                             0.5 * ((double)a.qr[cm] + (double)a.qr[c]);
+          buoy += kGrav * (kEps61 * dqv - qcr);
+        }
+        sR[k * TX + lx] = wk + dtau * (buoy + (double)a.tw[c]) -
+                          dtau * kCp * thvw * c2 * (pk - a.pip_src[cm]) / (J * dzw);
+      } else {
+        const double us = 0.5 * ((double)a.u[c] + (double)a.u[c + 1]);
+        const double vs = 0.5 * ((double)a.v[c] + (double)a.v[c + NX]);
+        w0 = us * hxc + vs * hyc;
+      }
+    }
+  }
+  if (ry == 0) sA[g.nz * TX + lx] = 0.0;
+  __syncthreads();
+
+  // ---- phase 2: Thomas elimination (kernels.cu:589-628 order), one lane per column
+  if (ry == 0 && col) {
+    double e_prev = sE[lx], av_prev = 0.0, av_cur = sA[TX + lx], p_prev = sP[lx];
+    double cs_prev = 0.0, ds_prev = 0.0;
+    for (int k = 1; k < g.nz; ++k) {
+      const double ek = sE[k * TX + lx];
+      const double G = sG[k * TX + lx];
+      const double R = sR[k * TX + lx];
+      const double pk = sP[k * TX + lx];
+      const double av_next = sA[(k + 1) * TX + lx];
+      const double A = -G * e_prev * av_prev;
+      const double B = 1.0 + G * av_cur * (ek + e_prev);
+      const double C = -G * ek * av_next;
+      const double D = R - G * (pk - p_prev);
+      double cs, ds;
+      if (k == 1) {
+        cs = C / B;
+        ds = D / B;
+      } else {
+        const double mm = B - A * cs_prev;
+        cs = C / mm;
+        ds = (D - A * ds_prev) / mm;
+      }
+      sG[k * TX + lx] = cs;
+      sR[k * TX + lx] = ds;
+      e_prev = ek;
+      av_prev = av_cur;
+      av_cur = av_next;
+      p_prev = pk;
+      cs_prev = cs;
+      ds_prev = ds;
+    }
+    double wabove = 0.0;
+    for (int k = g.nz - 1; k >= 1; --k) {
+      const double wk = sR[k * TX + lx] - sG[k * TX + lx] * wabove;
+      sR[k * TX + lx] = wk;
+      wabove = wk;
+    }
+    sR[lx] = w0;
+    sR[g.nz * TX + lx] = 0.0;
+  }
+  __syncthreads();
+
+  // ---- phase 3: pi', pi'prev, theta', w
+#pragma unroll
+  for (int t = 0; t < KPT; ++t) {
+    const int k = ry + NW * t;
+    if (col && k < g.nz) {
+      const int c = c0 + k * PL;
+      const double Wk = sR[k * TX + lx], Wk1 = sR[(k + 1) * TX + lx];
+      const double Ak = sA[k * TX + lx], Ak1 = sA[(k + 1) * TX + lx];
+      a.pip_dst[c] = sP[k * TX + lx] - sE[k * TX + lx] * (Ak1 * Wk1 - Ak * Wk);
+      if (a.piprev_dst) a.piprev_dst[c] = pold[t];
+      const double wc = 0.5 * (Wk + Wk1);
+      a.thp_dst[c] = (T)(thold[t] + dtau * (tthk[t] - wc * dthk[t]));
+      a.w_dst[c] = (T)Wk;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused acoustic substep: the horizontal u, v step and the w-pi' column solve
+// of one substep in a single launch. A workgroup owns a TX x TY block of
+// columns and NW thread rows (levels k = r, r + NW, ...):
+//   0. u, v of the block plus the halo faces the column solve needs (u at
+//      the block's east edge, v at its north edge) into LDS, and the block's
+//      own faces to memory (radiation edges and the periodic wrap included);
+//   1. assembly of P, E, G, RHS, Av per level from LDS, in FP64;
+//   2. Thomas elimination in the reference's order, one lane per column;
+//   3. w, pi', theta' written.
+// Neighbour reads of pi' and theta' require that their outputs do not alias
+// their inputs: pi' rotates over three buffers (the previous pi' is the
+// input of the substep before, so pi'prev is never written) and theta'
+// ping-pongs.
+template <class T>
+struct FusedArgs {
+  UVArgs<T> uv;  // u/v sources and destinations, pi' cur/prev/n, theta' cur/n, qv, tu, tv
+  const T* w_src;
+  T* w_dst;
+  double* pip_dst;
+  T* thp_dst;
+  const T* tw;
+  const T* tth;
+  const T* tpi;
+  const T* qc;
+  const T* qr;
+};
+
+template <int TX, int TY, int NW, int KPT, class T>
+struct FusedShape {
+  static constexpr int NC = TX * TY;
+  static size_t uv_bytes(int nz) {
+    return (size_t)nz * ((TX + 1) * TY + TX * (TY + 1)) * sizeof(T);
+  }
+  static size_t lds_bytes(int nz) {
+    return (size_t)5 * (nz + 1) * NC * sizeof(double) + uv_bytes(nz);
+  }
+  static bool fits_levels(int nz) { return nz <= NW * KPT; }
+};
+
+template <class T, int TX, int TY, int NW, int KPT>
+__global__ void __launch_bounds__(TX * TY * NW) k_acoustic_fused(Grid g, Params p, Base<T> b,
+                                                                  Metric<T> m, Coef<T> cf,
+                                                                  FusedArgs<T> a) {
+  constexpr int NC = TX * TY;
+  constexpr int UROW = TX + 1;  // sU row length (faces i0 .. i0+TX)
+  extern __shared__ double lds[];
+  const int L = g.nz + 1;
+  double* sP = lds;
+  double* sE = sP + L * NC;
+  double* sG = sE + L * NC;
+  double* sR = sG + L * NC;
+  double* sA = sR + L * NC;
+  T* sU = reinterpret_cast<T*>(sA + L * NC);  // [nz][TY][TX+1]
+  T* sV = sU + g.nz * UROW * TY;              // [nz][TY+1][TX]
+
+  const int lc = threadIdx.x, ry = threadIdx.y;
+  const int ci = lc % TX, cj = lc / TX;
+  const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+  const int i = i0 + ci, j = j0 + cj;
+  const bool own = i < g.nx && j < g.ny;
+  const int ih = min(i0 + TX, g.nx), jh = min(j0 + TY, g.ny);
+  const bool do_uh = own && i == ih - 1;
+  const bool do_vh = own && j == jh - 1;
+  const UVDev<T> d{g, p, b, m, a.uv};
+  const T dtau_t = a.uv.dtau;
+  const double dtau = (double)a.uv.dtau;
+  const int NX = g.NX, PL = g.plane;
+
+  // ---- phase 0: horizontal step of u, v into LDS (and memory)
+#pragma unroll
+  for (int t = 0; t < KPT; ++t) {
+    const int k = ry + NW * t;
+    if (!own || k >= g.nz) continue;
+    const int kk = k + g.ng;
+    const int c = g.id(i, j, k);
+    T un, vn;
+    if (p.bcx && i == 0) {  // west radiation edge (k_bc_rad_u_x :649-660)
+      const T u0 = a.uv.u_src[c];
+      if (u0 >= (T)0) {
+        un = b.ub[kk];
+      } else {
+        const T cb = u0 - (T)p.cstar;
+        un = u0 - dtau_t * cb * (d.u_step(1, j, k) - u0) / (T)g.dx;
+      }
+    } else {
+      un = d.u_step(i, j, k);
+    }
+    if (p.bcy && j == 0) {  // south radiation edge (k_bc_rad_v_y :670-680)
+      const T v0 = a.uv.v_src[c];
+      if (v0 >= (T)0) {
+        vn = b.vb[kk];
+      } else {
+        const T cb = v0 - (T)p.cstar;
+        vn = v0 - dtau_t * cb * (d.v_step(i, 1, k) - v0) / (T)g.dy;
+      }
+    } else {
+      vn = d.v_step(i, j, k);
+    }
+    sU[(k * TY + cj) * UROW + ci] = un;
+    sV[(k * (TY + 1) + cj) * TX + ci] = vn;
+    a.uv.u_dst[c] = un;
+    a.uv.v_dst[c] = vn;
+    if (do_uh) {  // east face of the block
+      T uh;
+      const int ch = c + 1;
+      if (ih < g.nx) {
+        uh = d.u_step(ih, j, k);
+      } else if (!p.bcx) {
+        uh = d.u_step(0, j, k);  // periodic ghost u(nx) = u(0)
+      } else {                   // east radiation edge (k_bc_rad_u_x :661-667)
+        const T u1 = a.uv.u_src[ch];
+        if (u1 <= (T)0) {
+          uh = b.ub[kk];
+        } else {
+          const T cb = u1 + (T)p.cstar;
+          uh = u1 - dtau_t * cb * (u1 - un) / (T)g.dx;
+        }
+      }
+      sU[(k * TY + cj) * UROW + (ih - i0)] = uh;
+      if (ih == g.nx) a.uv.u_dst[ch] = uh;
+    }
+    if (do_vh) {  // north face of the block
+      T vh;
+      const int ch = c + NX;
+      if (jh < g.ny) {
+        vh = d.v_step(i, jh, k);
+      } else if (!p.bcy) {
+        vh = d.v_step(i, 0, k);
+      } else {
+        const T v1 = a.uv.v_src[ch];
+        if (v1 <= (T)0) {
+          vh = b.vb[kk];
+        } else {
+          const T cb = v1 + (T)p.cstar;
+          vh = v1 - dtau_t * cb * (v1 - vn) / (T)g.dy;
+        }
+      }
+      sV[(k * (TY + 1) + (jh - j0)) * TX + ci] = vh;
+      if (jh == g.ny) a.uv.v_dst[ch] = vh;
+    }
+  }
+  __syncthreads();
+
+  // ---- phase 1: column assembly (k_acou_wpi :552-620), FP64
+  const double c1 = (1.0 + p.beta) * 0.5;
+  const double c2 = (1.0 - p.beta) * 0.5;
+  const int c2d = g.id2(own ? i : 0, j < g.ny ? j : 0);
+  const double J = (double)m.jac[c2d];
+  const double hxc = (double)m.hx[c2d];
+  const double hyc = (double)m.hy[c2d];
+  const double mc = (double)m.mapf[c2d];
+  const double zt = (double)m.zt;
+  const int c0 = g.id(own ? i : 0, j < g.ny ? j : 0, 0);
+  auto U = [&](int k, int di) -> double { return (double)sU[(k * TY + cj) * UROW + ci + di]; };
+  auto V = [&](int k, int dj) -> double {
+    return (double)sV[(k * (TY + 1) + cj + dj) * TX + ci];
+  };
+  auto sv = [&](int k) -> double {
+    if (k <= 0 || k >= g.nz) return 0.0;
+    const double fac = 1.0 - (double)m.zeta_w[k + g.ng] / zt;
+    const double uw = 0.25 * (U(k - 1, 0) + U(k - 1, 1) + U(k, 0) + U(k, 1));
+    const double vw = 0.25 * (V(k - 1, 0) + V(k - 1, 1) + V(k, 0) + V(k, 1));
+    return (double)cf.aw[c0 + k * PL] * (uw * hxc + vw * hyc) * fac;
+  };
+  auto av = [&](int k) -> double {
+    return (k <= 0 || k >= g.nz) ? 0.0 : (double)cf.aw[c0 + k * PL];
+  };
+  double pold[KPT], thold[KPT], tthk[KPT], dthk[KPT];
+  double w0 = 0.0;
+#pragma unroll
+  for (int t = 0; t < KPT; ++t) {
+    const int k = ry + NW * t;
+    if (!own || k >= g.nz) continue;
+    const int c = c0 + k * PL;
+    const int kk = k + g.ng;
+    const double Kt = (double)cf.kt3[c];
+    const double Dh =
+        mc * (((double)cf.rtjx[c + 1] * U(k, 1) - (double)cf.rtjx[c] * U(k, 0)) / g.dx +
+              ((double)cf.rtjy[c + NX] * V(k, 1) - (double)cf.rtjy[c] * V(k, 0)) / g.dy);
+    const double dzc = (double)m.dzeta_c[kk];
+    const double avk = av(k), avk1 = av(k + 1);
+    const double wk = (double)a.w_src[c];
+    const double wkp = (double)a.w_src[c + PL];
+    const double tpi = a.tpi ? (double)a.tpi[c] : 0.0;
+    const double pk = a.uv.pcur[c];
+    pold[t] = pk;
+    sP[k * NC + lc] = pk + dtau * tpi -
+                      dtau * Kt *
+                          (Dh - (sv(k + 1) - sv(k)) / dzc + c2 * (avk1 * wkp - avk * wk) / dzc);
+    sE[k * NC + lc] = dtau * Kt * c1 / dzc;
+    sA[k * NC + lc] = avk;
+    const double thk = (double)a.uv.thp_cur[c];
+    thold[t] = thk;
+    tthk[t] = (double)a.tth[c];
+    dthk[t] = (double)b.dthbdz[c];
+    if (k >= 1) {
+      const int cm = c - PL;
+      const double thm = (double)a.uv.thp_cur[cm];
+      double thvw = (double)b.thbw[c] + 0.5 * (thm + thk);
+      if (p.moisture) thvw *= 1.0 + kEps61 * 0.5 * ((double)a.uv.qv[cm] + (double)a.uv.qv[c]);
+      const double dzw = (double)m.dzeta_w[kk];
+      sG[k * NC + lc] = dtau * kCp * thvw * c1 / (J * dzw);
+      double buoy = kGrav * 0.5 * (thm + thk) / (double)b.thbw[c];
+      if (p.moisture) {
+        const double dqv = 0.5 * ((double)a.uv.qv[cm] + (double)a.uv.qv[c]) -
+                           0.5 * ((double)b.qvb[cm] + (double)b.qvb[c]);
+        const double qcr = 0.5 * ((double)a.qc[cm] + (double)a.qc[c]) +
+                           0.5 * ((double)a.qr[cm] + (double)a.qr[c]);
+        buoy += kGrav * (kEps61 * dqv - qcr);
+      }
+      sR[k * NC + lc] = wk + dtau * (buoy + (double)a.tw[c]) -
+                        dtau * kCp * thvw * c2 * (pk - a.uv.pcur[cm]) / (J * dzw);
+    } else {
+      const double us = 0.5 * (U(0, 0) + U(0, 1));
+      const double vs = 0.5 * (V(0, 0) + V(0, 1));
+      w0 = us * hxc + vs * hyc;
+    }
+  }
+  if (ry == 0) sA[g.nz * NC + lc] = 0.0;
+  __syncthreads();
+
+  // ---- phase 2: Thomas elimination (kernels.cu:589-628 order)
+  if (ry == 0 && own) {
+    double e_prev = sE[lc], av_prev = 0.0, av_cur = sA[NC + lc], p_prev = sP[lc];
+    double cs_prev = 0.0, ds_prev = 0.0;
+    for (int k = 1; k < g.nz; ++k) {
+      const double ek = sE[k * NC + lc];
+      const double G = sG[k * NC + lc];
+      const double R = sR[k * NC + lc];
+      const double pk = sP[k * NC + lc];
+      const double av_next = sA[(k + 1) * NC + lc];
+      const double A = -G * e_prev * av_prev;
+      const double B = 1.0 + G * av_cur * (ek + e_prev);
+      const double C = -G * ek * av_next;
+      const double D = R - G * (pk - p_prev);
+      double cs, ds;
+      if (k == 1) {
+        cs = C / B;
+        ds = D / B;
+      } else {
+        const double mm = B - A * cs_prev;
+        cs = C / mm;
+        ds = (D - A * ds_prev) / mm;
+      }
+      sG[k * NC + lc] = cs;
+      sR[k * NC + lc] = ds;
+      e_prev = ek;
+      av_prev = av_cur;
+      av_cur = av_next;
+      p_prev = pk;
+      cs_prev = cs;
+      ds_prev = ds;
+    }
+    double wabove = 0.0;
+    for (int k = g.nz - 1; k >= 1; --k) {
+      const double wk = sR[k * NC + lc] - sG[k * NC + lc] * wabove;
+      sR[k * NC + lc] = wk;
+      wabove = wk;
+    }
+    sR[lc] = w0;
+    sR[g.nz * NC + lc] = 0.0;
+  }
+  __syncthreads();
+
+  // ---- phase 3: w, pi', theta'
+#pragma unroll
+  for (int t = 0; t < KPT; ++t) {
+    const int k = ry + NW * t;
+    if (!own || k >= g.nz) continue;
+    const int c = c0 + k * PL;
+    const double Wk = sR[k * NC + lc], Wk1 = sR[(k + 1) * NC + lc];
+    const double Ak = sA[k * NC + lc], Ak1 = sA[(k + 1) * NC + lc];
+    a.pip_dst[c] = sP[k * NC + lc] - sE[k * NC + lc] * (Ak1 * Wk1 - Ak * Wk);
+    const double wc = 0.5 * (Wk + Wk1);
+    a.thp_dst[c] = (T)(thold[t] + dtau * (tthk[t] - wc * dthk[t]));
+    a.w_dst[c] = (T)Wk;
   }
 }
 

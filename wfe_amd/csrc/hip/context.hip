@@ -4,7 +4,9 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -160,6 +162,50 @@ class KernelTimer {
 
 }  // namespace
 
+// Tile shapes of the fused acoustic substep (TX x TY columns, NW thread
+// rows, KPT levels per thread). All shapes give bitwise-identical results;
+// the context times the ones that fit and keeps the fastest.
+template <class T>
+struct FusedVariant {
+  const char* name;
+  int tx, ty, nw, kpt;
+  size_t (*lds)(int nz);
+  const void* fn;
+  void (*launch)(dim3 grid, size_t lds, hipStream_t s, const dev::Grid& g, const dev::Params& p,
+                 const dev::Base<T>& b, const dev::Metric<T>& m, const dev::Coef<T>& cf,
+                 const dev::FusedArgs<T>& a);
+};
+
+template <class T, int TX, int TY, int NW, int KPT>
+FusedVariant<T> make_variant(const char* name) {
+  using S = dev::FusedShape<TX, TY, NW, KPT, T>;
+  FusedVariant<T> v;
+  v.name = name;
+  v.tx = TX;
+  v.ty = TY;
+  v.nw = NW;
+  v.kpt = KPT;
+  v.lds = [](int nz) { return S::lds_bytes(nz); };
+  v.fn = reinterpret_cast<const void*>(&dev::k_acoustic_fused<T, TX, TY, NW, KPT>);
+  v.launch = [](dim3 grid, size_t lds, hipStream_t s, const dev::Grid& g, const dev::Params& p,
+                const dev::Base<T>& b, const dev::Metric<T>& m, const dev::Coef<T>& cf,
+                const dev::FusedArgs<T>& a) {
+    dev::k_acoustic_fused<T, TX, TY, NW, KPT><<<grid, dim3(TX * TY, NW), lds, s>>>(g, p, b, m, cf, a);
+  };
+  return v;
+}
+
+template <class T>
+std::vector<FusedVariant<T>> fused_variants() {
+  return {
+      make_variant<T, 8, 2, 16, 3>("8x2x16"),   make_variant<T, 16, 1, 16, 3>("16x1x16"),
+      make_variant<T, 4, 4, 16, 3>("4x4x16"),   make_variant<T, 8, 4, 8, 6>("8x4x8"),
+      make_variant<T, 16, 2, 8, 6>("16x2x8"),   make_variant<T, 32, 1, 8, 6>("32x1x8"),
+      make_variant<T, 16, 2, 16, 3>("16x2x16"), make_variant<T, 8, 2, 24, 2>("8x2x24"),
+      make_variant<T, 8, 4, 16, 3>("8x4x16"),   make_variant<T, 32, 2, 8, 6>("32x2x8"),
+  };
+}
+
 // Type-erased interface used by the C ABI.
 struct CtxBase {
   virtual ~CtxBase() = default;
@@ -209,7 +255,10 @@ class Ctx final : public CtxBase {
   }
   std::string kernel_report() override {
     WFE_HIP(hipStreamSynchronize(stream_));
-    return timer_.report();
+    std::string r = timer_.report();
+    if (tuned_) r += "tuned:" + methods_[method_].name + " 0 0\n";
+    r += tune_report_;
+    return r;
   }
 
  private:
@@ -241,8 +290,7 @@ class Ctx final : public CtxBase {
   int cur_ = 0;
   T* wu_[2]{};
   T* wv_[2]{};
-  T *ww_ = nullptr, *wthp_ = nullptr;
-  double *wpip_ = nullptr, *wpiprev_ = nullptr;
+  T* ww_ = nullptr;
   T *tu_ = nullptr, *tv_ = nullptr, *tw_ = nullptr, *tth_ = nullptr, *tpi_ = nullptr;
   T* tq_[3]{};
   T *mfx_ = nullptr, *mfy_ = nullptr, *mfz_ = nullptr, *div_ = nullptr;
@@ -253,6 +301,23 @@ class Ctx final : public CtxBase {
   T* bdy_wgt_ = nullptr;
   double tfrac_ = 0;
   unsigned* d_absmax_ = nullptr;
+  // Acoustic substep methods (split u/v + column-solve pairs and fused
+  // shapes); all run under the same buffer rotation, the fastest is chosen
+  // by a timing pass on the first stage.
+  struct SubstepMethod {
+    std::string name;
+    std::function<void(const dev::FusedArgs<T>&)> run;
+  };
+  std::vector<SubstepMethod> methods_;
+  int method_ = -1;
+  bool tuned_ = false;
+  std::string tune_report_;
+  double* xpip_[3]{};  // rotating pi' work buffers
+  T* xthp_[2]{};       // ping-pong theta' work buffers
+  void build_methods(const hipDeviceProp_t& prop);
+  void tune_acoustic(double dtau);
+  template <int TX, int NW, int KPT>
+  void add_split_tile(const char* name, size_t cap);
   KernelTimer timer_;
 };
 
@@ -371,11 +436,10 @@ Ctx<T>::Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& b, const w
   for (int q = 0; q < 2; ++q) {
     wu_[q] = arena_.alloc<T>(n);
     wv_[q] = arena_.alloc<T>(n);
+    xthp_[q] = arena_.alloc<T>(n);
   }
   ww_ = arena_.alloc<T>(n);
-  wthp_ = arena_.alloc<T>(n);
-  wpip_ = arena_.alloc<double>(n);
-  wpiprev_ = arena_.alloc<double>(n);
+  for (int q = 0; q < 3; ++q) xpip_[q] = arena_.alloc<double>(n);
   tu_ = arena_.alloc<T>(n);
   tv_ = arena_.alloc<T>(n);
   tw_ = arena_.alloc<T>(n);
@@ -395,6 +459,13 @@ Ctx<T>::Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& b, const w
     }
   rain_ = arena_.alloc<T>(n2_);
   d_absmax_ = arena_.alloc<unsigned>(1);
+  {
+    int dev = 0;
+    WFE_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop{};
+    WFE_HIP(hipGetDeviceProperties(&prop, dev));
+    build_methods(prop);
+  }
   WFE_HIP(hipDeviceSynchronize());
 }
 
@@ -603,51 +674,210 @@ void Ctx<T>::bdy_relax(const StateSet<T>& s) {
 }
 
 // One RK3 stage of the fast system from the time-n state S_[cur_] into out.
+// Column-solve arguments of a split substep from the common substep args.
+template <class T>
+dev::WPArgs<T> split_wp_impl(const dev::FusedArgs<T>& a) {
+  dev::WPArgs<T> w{};
+  w.u = a.uv.u_dst;
+  w.v = a.uv.v_dst;
+  w.w_src = a.w_src;
+  w.pip_src = a.uv.pcur;
+  w.thp_src = a.uv.thp_cur;
+  w.w_dst = a.w_dst;
+  w.pip_dst = a.pip_dst;
+  w.piprev_dst = nullptr;  // pi'prev is the previous buffer of the rotation
+  w.thp_dst = a.thp_dst;
+  w.tw = a.tw;
+  w.tth = a.tth;
+  w.tpi = a.tpi;
+  w.qv = a.uv.qv;
+  w.qc = a.qc;
+  w.qr = a.qr;
+  w.dtau = (double)a.uv.dtau;
+  return w;
+}
+
+template <class T>
+template <int TX, int NW, int KPT>
+void Ctx<T>::add_split_tile(const char* name, size_t cap) {
+  using Tile = dev::WpiTile<TX, NW, KPT>;
+  if (g_.nz > NW * KPT) return;
+  const size_t lb = Tile::lds_bytes(g_.nz);
+  if (lb > 65536) {
+    if (lb > cap || hipFuncSetAttribute(
+                        reinterpret_cast<const void*>(&dev::k_acou_wpi_tile<T, TX, NW, KPT>),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb) != hipSuccess) {
+      (void)hipGetLastError();
+      return;
+    }
+  }
+  methods_.push_back({name, [this, lb](const dev::FusedArgs<T>& a) {
+                        const dim3 guv(cdiv(g_.nx + 1, 64), cdiv(g_.ny + 1, 4), (unsigned)g_.nz);
+                        TIMED("acou_uv", dev::k_acou_uv<T><<<guv, blk3(), 0, stream_>>>(
+                                             g_, p_, base_, met_, a.uv));
+                        const dev::WPArgs<T> w = split_wp_impl<T>(a);
+                        const dim3 gt(cdiv(g_.nx, TX), (unsigned)g_.ny);
+                        TIMED("acou_wpi", dev::k_acou_wpi_tile<T, TX, NW, KPT>
+                                          <<<gt, dim3(TX, NW), lb, stream_>>>(g_, p_, base_, met_,
+                                                                              coef_, w));
+                      }});
+}
+
+template <class T>
+void Ctx<T>::build_methods(const hipDeviceProp_t& prop) {
+  const size_t cap = (size_t)prop.sharedMemPerBlock;
+  // General split pair: one thread per column (any nz <= 319).
+  methods_.push_back({"split-col", [this](const dev::FusedArgs<T>& a) {
+                        const dim3 guv(cdiv(g_.nx + 1, 64), cdiv(g_.ny + 1, 4), (unsigned)g_.nz);
+                        TIMED("acou_uv", dev::k_acou_uv<T><<<guv, blk3(), 0, stream_>>>(
+                                             g_, p_, base_, met_, a.uv));
+                        const dev::WPArgs<T> w = split_wp_impl<T>(a);
+                        const dim3 gcol(cdiv(g_.nx, 64), (unsigned)g_.ny);
+                        if (g_.nz + 1 <= 64)
+                          TIMED("acou_wpi", dev::k_acou_wpi_col<T, 64>
+                                            <<<gcol, dim3(64, 1), 0, stream_>>>(g_, p_, base_,
+                                                                                met_, coef_, w));
+                        else
+                          TIMED("acou_wpi", dev::k_acou_wpi_col<T, 320>
+                                            <<<gcol, dim3(64, 1), 0, stream_>>>(g_, p_, base_,
+                                                                                met_, coef_, w));
+                      }});
+  add_split_tile<32, 8, 6>("split-32x8", cap);
+  add_split_tile<32, 16, 3>("split-32x16", cap);
+  add_split_tile<32, 24, 2>("split-32x24", cap);
+  add_split_tile<64, 8, 6>("split-64x8", cap);
+  add_split_tile<64, 16, 3>("split-64x16", cap);
+  add_split_tile<16, 16, 3>("split-16x16", cap);
+  for (const auto& v : fused_variants<T>()) {
+    if (g_.nz > v.nw * v.kpt) continue;
+    const size_t lb = v.lds(g_.nz);
+    if (lb > 65536 && (lb > cap || hipFuncSetAttribute(v.fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                       (int)lb) != hipSuccess)) {
+      (void)hipGetLastError();
+      continue;
+    }
+    const FusedVariant<T> fv = v;
+    methods_.push_back({std::string("fused-") + v.name, [this, fv, lb](const dev::FusedArgs<T>& a) {
+                          const dim3 grid(cdiv(g_.nx, fv.tx), cdiv(g_.ny, fv.ty));
+                          TIMED("acoustic_fused",
+                                fv.launch(grid, lb, stream_, g_, p_, base_, met_, coef_, a));
+                        }});
+  }
+  // Selection: WFE_ACOUSTIC_KERNEL = exact method name, or a family prefix
+  // ("split", "fused"); unset or "auto" = all. WFE_WPI_KERNEL=column keeps
+  // only the one-thread-per-column pair.
+  std::string sel;
+  if (const char* e = std::getenv("WFE_ACOUSTIC_KERNEL")) sel = e;
+  if (const char* e = std::getenv("WFE_WPI_KERNEL"))
+    if (std::string(e) == "column") sel = "split-col";
+  if (!sel.empty() && sel != "auto") {
+    std::vector<SubstepMethod> keep;
+    for (auto& mth : methods_)
+      if (mth.name == sel || mth.name.rfind(sel + "-", 0) == 0) keep.push_back(mth);
+    if (keep.empty()) throw ArgError("WFE_ACOUSTIC_KERNEL=" + sel + " matches no method for this grid");
+    methods_.swap(keep);
+  }
+  if (methods_.size() == 1) {
+    method_ = 0;
+    tuned_ = true;
+  }
+}
+
+// Times every candidate substep method on the current state (writing only
+// work buffers) and keeps the fastest; reported as "tuned:<name>" and
+// "cand:<name> 1 <ms>" lines by wfe_kernel_timing_report.
+template <class T>
+void Ctx<T>::tune_acoustic(double dtau) {
+  const StateSet<T>& sn = S_[cur_];
+  dev::FusedArgs<T> a{};
+  a.uv.u_src = sn.u;
+  a.uv.v_src = sn.v;
+  a.uv.u_dst = wu_[0];
+  a.uv.v_dst = wv_[0];
+  a.uv.pcur = sn.pip;
+  a.uv.pprev = sn.pip;
+  a.uv.pn = sn.pip;
+  a.uv.thp_cur = sn.thp;
+  a.uv.thp_n = sn.thp;
+  a.uv.qv = sn.qv;
+  a.uv.tu = tu_;
+  a.uv.tv = tv_;
+  a.uv.dtau = (T)dtau;
+  a.w_src = sn.w;
+  a.w_dst = ww_;
+  a.pip_dst = xpip_[0];
+  a.thp_dst = xthp_[0];
+  a.tw = tw_;
+  a.tth = tth_;
+  a.tpi = nullptr;
+  a.qc = sn.qc;
+  a.qr = sn.qr;
+  hipEvent_t e0, e1;
+  WFE_HIP(hipEventCreate(&e0));
+  WFE_HIP(hipEventCreate(&e1));
+  const bool was_on = timer_.on;
+  timer_.on = false;
+  float best = 1e30f;
+  int best_m = 0;
+  tune_report_.clear();
+  for (int q = 0; q < (int)methods_.size(); ++q) {
+    for (int r = 0; r < 2; ++r) methods_[q].run(a);
+    WFE_HIP(hipEventRecord(e0, stream_));
+    for (int r = 0; r < 5; ++r) methods_[q].run(a);
+    WFE_HIP(hipEventRecord(e1, stream_));
+    WFE_HIP(hipEventSynchronize(e1));
+    check_launch("tune_acoustic");
+    float ms = 0;
+    WFE_HIP(hipEventElapsedTime(&ms, e0, e1));
+    char buf[128];
+    std::snprintf(buf, sizeof(buf), "cand:%s 1 %.6f\n", methods_[q].name.c_str(), ms / 5);
+    tune_report_ += buf;
+    if (ms < best) {
+      best = ms;
+      best_m = q;
+    }
+  }
+  timer_.on = was_on;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  method_ = best_m;
+  tuned_ = true;
+}
+
+// One RK3 stage of the fast system from the time-n state S_[cur_] into out.
+// pi' rotates over three work buffers: the input of substep s is the output
+// of s-1 and pi'prev is the input of s-1 (time n for s <= 1); theta' and
+// u, v ping-pong; w is updated in place (own column only).
 template <class T>
 void Ctx<T>::acoustic_stage(const StateSet<T>& out, double dtau, int nsub) {
   const StateSet<T>& sn = S_[cur_];
-  const dim3 guv(cdiv(g_.nx + 1, 64), cdiv(g_.ny + 1, 4), (unsigned)g_.nz);
-  const dim3 gcol(cdiv(g_.nx, 64), (unsigned)g_.ny);
+  if (!tuned_) tune_acoustic(dtau);
   for (int s = 0; s < nsub; ++s) {
     const bool first = s == 0, last = s == nsub - 1;
-    dev::UVArgs<T> a{};
-    a.u_src = first ? sn.u : wu_[(s - 1) & 1];
-    a.v_src = first ? sn.v : wv_[(s - 1) & 1];
-    a.u_dst = last ? out.u : wu_[s & 1];
-    a.v_dst = last ? out.v : wv_[s & 1];
-    a.pcur = first ? sn.pip : wpip_;
-    a.pprev = first ? sn.pip : wpiprev_;
-    a.pn = sn.pip;
-    a.thp_cur = first ? sn.thp : wthp_;
-    a.thp_n = sn.thp;
-    a.qv = sn.qv;
-    a.tu = tu_;
-    a.tv = tv_;
-    a.dtau = (T)dtau;
-    TIMED("acou_uv", dev::k_acou_uv<T><<<guv, blk3(), 0, stream_>>>(g_, p_, base_, met_, a));
-
-    dev::WPArgs<T> w{};
-    w.u = a.u_dst;
-    w.v = a.v_dst;
-    w.w_src = first ? sn.w : ww_;
-    w.pip_src = first ? sn.pip : wpip_;
-    w.thp_src = first ? sn.thp : wthp_;
-    w.w_dst = last ? out.w : ww_;
-    w.pip_dst = last ? out.pip : wpip_;
-    w.piprev_dst = wpiprev_;
-    w.thp_dst = last ? out.thp : wthp_;
-    w.tw = tw_;
-    w.tth = tth_;
-    w.tpi = bdy_active_ ? tpi_ : nullptr;
-    w.qv = sn.qv;
-    w.qc = sn.qc;
-    w.qr = sn.qr;
-    w.dtau = dtau;
-    if (g_.nz + 1 <= 64)
-      TIMED("acou_wpi", dev::k_acou_wpi_col<T, 64><<<gcol, dim3(64, 1), 0, stream_>>>(g_, p_, base_, met_, coef_, w));
-    else
-      TIMED("acou_wpi", dev::k_acou_wpi_col<T, 320>
-          <<<gcol, dim3(64, 1), 0, stream_>>>(g_, p_, base_, met_, coef_, w));
+    dev::FusedArgs<T> a{};
+    a.uv.u_src = first ? sn.u : wu_[(s - 1) & 1];
+    a.uv.v_src = first ? sn.v : wv_[(s - 1) & 1];
+    a.uv.u_dst = last ? out.u : wu_[s & 1];
+    a.uv.v_dst = last ? out.v : wv_[s & 1];
+    a.uv.pcur = first ? sn.pip : xpip_[(s - 1) % 3];
+    a.uv.pprev = s <= 1 ? sn.pip : xpip_[(s - 2) % 3];
+    a.uv.pn = sn.pip;
+    a.uv.thp_cur = first ? sn.thp : xthp_[(s - 1) & 1];
+    a.uv.thp_n = sn.thp;
+    a.uv.qv = sn.qv;
+    a.uv.tu = tu_;
+    a.uv.tv = tv_;
+    a.uv.dtau = (T)dtau;
+    a.w_src = first ? sn.w : ww_;
+    a.w_dst = last ? out.w : ww_;
+    a.pip_dst = last ? out.pip : xpip_[s % 3];
+    a.thp_dst = last ? out.thp : xthp_[s & 1];
+    a.tw = tw_;
+    a.tth = tth_;
+    a.tpi = bdy_active_ ? tpi_ : nullptr;
+    a.qc = sn.qc;
+    a.qr = sn.qr;
+    methods_[method_].run(a);
   }
   check_launch("acoustic_stage");
 }
@@ -867,6 +1097,45 @@ int wfe_kernel_timing_report(wfe_ctx* ctx, char* buf, size_t n) {
     const std::string r = ctx->impl->kernel_report();
     if (r.size() + 1 > n) throw wfe::ArgError("report buffer too small");
     std::memcpy(buf, r.c_str(), r.size() + 1);
+  });
+}
+
+int wfe_copy_probe(size_t bytes, int reps, double* gbps) {
+  return guarded([&] {
+    if (!gbps || reps < 1 || bytes < 4096) throw wfe::ArgError("copy probe: bad arguments");
+    const size_t n4 = bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    WFE_HIP(hipMalloc(&a, n4 * 16));
+    WFE_HIP(hipMalloc(&b, n4 * 16));
+    struct Free {
+      void *a, *b;
+      ~Free() {
+        (void)hipFree(a);
+        (void)hipFree(b);
+      }
+    } fr{a, b};
+    WFE_HIP(hipMemset(a, 0, n4 * 16));
+    hipStream_t st;
+    WFE_HIP(hipStreamCreate(&st));
+    hipEvent_t e0, e1;
+    WFE_HIP(hipEventCreate(&e0));
+    WFE_HIP(hipEventCreate(&e1));
+    const unsigned blocks = 256 * 8;
+    wfe::dev::k_copy_probe<<<blocks, 256, 0, st>>>(static_cast<const float4*>(a),
+                                                    static_cast<float4*>(b), n4);
+    WFE_HIP(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; ++r)
+      wfe::dev::k_copy_probe<<<blocks, 256, 0, st>>>(static_cast<const float4*>(a),
+                                                      static_cast<float4*>(b), n4);
+    WFE_HIP(hipEventRecord(e1, st));
+    WFE_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    WFE_HIP(hipEventElapsedTime(&ms, e0, e1));
+    wfe::check_launch("copy probe");
+    *gbps = 2.0 * (double)(n4 * 16) * reps / (ms * 1e-3) / 1e9;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
   });
 }
 

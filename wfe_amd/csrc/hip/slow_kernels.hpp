@@ -476,4 +476,11 @@ __global__ void k_absmax(Grid g, const E* __restrict__ f, int kmax, unsigned int
   if (threadIdx.x == 0 && threadIdx.y == 0) atomicMax(result, smax);
 }
 
+// Streaming copy used to measure the achievable HBM bandwidth and to
+// calibrate the FETCH_SIZE / WRITE_SIZE counters against a known byte count.
+__global__ void k_copy_probe(const float4* __restrict__ a, float4* __restrict__ b, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) b[i] = a[i];
+}
+
 }  // namespace wfe::dev
