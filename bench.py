@@ -116,6 +116,8 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=30)
+    ap.add_argument("--mixed-only", action="store_true",
+                    help="skip the FP64 run and the error comparison (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -132,25 +134,27 @@ def main() -> int:
     log(f"[bench] mixed: {W} warmup + {K} timed steps on {g.nx}x{g.ny}x{g.nz}")
     gi_m, t_m = timed_run(case, "mixed", W, K)
     log(f"[bench] mixed {t_m:.3f} s -> {1e3 * t_m / K:.3f} ms/step")
-    log(f"[bench] fp64: {W} warmup + {K} timed steps")
-    gi_d, t_d = timed_run(case, "fp64", W, K)
-    log(f"[bench] fp64 {t_d:.3f} s -> {1e3 * t_d / K:.3f} ms/step")
-
-    # Error of the mixed path against the plain FP64 path after W + K steps.
-    errs = {}
-    for f in ("u", "v", "w", "thp", "pip", "qv", "qc", "qr"):
-        nzl = g.nz + 1 if f == "w" else g.nz
-        a = g.interior(gi_m.get(f), nzl)
-        b = g.interior(gi_d.get(f), nzl)
-        nb = float(np.linalg.norm(b))
-        errs[f] = float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a))
-    del gi_m, gi_d
+    errs, t_d = None, None
+    if not args.mixed_only:
+        log(f"[bench] fp64: {W} warmup + {K} timed steps")
+        gi_d, t_d = timed_run(case, "fp64", W, K)
+        log(f"[bench] fp64 {t_d:.3f} s -> {1e3 * t_d / K:.3f} ms/step")
+        # Error of the mixed path against the plain FP64 path after W + K steps.
+        errs = {}
+        for f in ("u", "v", "w", "thp", "pip", "qv", "qc", "qr"):
+            nzl = g.nz + 1 if f == "w" else g.nz
+            a = g.interior(gi_m.get(f), nzl)
+            b = g.interior(gi_d.get(f), nzl)
+            nb = float(np.linalg.norm(b))
+            errs[f] = float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a))
+        del gi_d
+    del gi_m
 
     from wfe_amd.case import copy_bandwidth
     copy_bw = copy_bandwidth(1 << 30, 20)
     log(f"[bench] streaming copy probe: {copy_bw:.0f} GB/s")
     value = N * K / t_m
-    value_fp64 = N * K / t_d
+    value_fp64 = N * K / t_d if t_d else None
     log(f"[bench] kernel profile: {args.profile_steps} steps")
     rep, tune = kernel_profile(case, "mixed", 5, args.profile_steps)
     dom = max(rep, key=lambda k: rep[k][1])
@@ -191,7 +195,7 @@ def main() -> int:
                                "stretched to 20 km, dt=6 s, ns=6",
                    "grid": [g.nx, g.ny, g.nz], "precision": "mixed"},
         "fp64_plain": {"value": value_fp64, "ms_per_step": 1e3 * t_d / K,
-                       "speedup_mixed_vs_fp64": value / value_fp64},
+                       "speedup_mixed_vs_fp64": value / value_fp64} if t_d else None,
         "rel_err_vs_fp64": errs,
         "step_roofline": {"alg_bytes_per_pt_step": ALG_BYTES_STEP["mixed"],
                           "achieved_GBps": step_bw / 1e9, "frac": step_bw / HBM_PEAK},

@@ -173,6 +173,108 @@ __global__ void k_acou_uv(Grid g, Params p, Base<T> b, Metric<T> m, UVArgs<T> a)
   }
 }
 
+// LDS-tiled form of k_acou_uv: a 64 x 4 block of faces at one level stages
+// pi* and theta_v once per cell (block + one-cell west/south halo), instead
+// of recomputing them for every face that touches the cell. Faces that need
+// more than that -- radiation edges, the periodic wrap u(nx) = u(0), and the
+// terrain cross term (pi* at k +- 1) -- go through the direct UVDev path,
+// which evaluates the same expressions.
+template <class T>
+__global__ void __launch_bounds__(256) k_acou_uv_tile(Grid g, Params p, Base<T> b, Metric<T> m,
+                                                      UVArgs<T> a) {
+  constexpr int BX = 64, BY = 4;
+  __shared__ double sP[BY + 1][BX + 1];
+  __shared__ T sT[BY + 1][BX + 1];
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const BlockId bid = xcd_block();
+  const int i0 = bid.x * BX, j0 = bid.y * BY;
+  const int k = bid.z;
+  const UVDev<T> d{g, p, b, m, a};
+  // Stage cells i in [i0-1, i0+BX-1], j in [j0-1, j0+BY-1] that exist
+  // (cells up to nx-1 / ny-1 plus the halo, which may be a ghost).
+  for (int e = ty * BX + tx; e < (BX + 1) * (BY + 1); e += BX * BY) {
+    const int r = e / (BX + 1), q = e % (BX + 1);
+    const int ii = i0 - 1 + q, jj = j0 - 1 + r;
+    if (ii <= g.nx - 1 && jj <= g.ny - 1) {
+      sP[r][q] = d.pstar(ii, jj, k);
+      sT[r][q] = d.thv(ii, jj, k);
+    }
+  }
+  __syncthreads();
+  const int i = i0 + tx, j = j0 + ty;
+  if (i > g.nx || j > g.ny || k >= g.nz) return;
+  const int kk = k + g.ng;
+  const int c = g.id(i, j, k);
+  const int c2 = g.id2(i, j);
+  const T dtau = a.dtau;
+  if (j < g.ny) {
+    T un;
+    if (i < g.nx && !(p.bcx && i == 0)) {
+      const T zxu = m.hxu[c2] * ((T)1 - m.zeta_c[kk] / m.zt);
+      if (zxu != (T)0) {
+        un = d.u_step(i, j, k);
+      } else {
+        const T thbu = (T)0.5 * (sT[ty + 1][tx] + sT[ty + 1][tx + 1]);
+        const T mu = (T)0.5 * (m.mapf[c2 - 1] + m.mapf[c2]);
+        const double grad = (sP[ty + 1][tx + 1] - sP[ty + 1][tx]) / g.dx;
+        un = a.u_src[c] + dtau * (-(T)kCp * thbu * mu * (T)grad + a.tu[c]);
+      }
+    } else if (!p.bcx) {
+      un = d.u_step(0, j, k);
+    } else if (i == 0) {
+      const T u0 = a.u_src[c];
+      if (u0 >= (T)0) {
+        un = b.ub[kk];
+      } else {
+        const T cb = u0 - (T)p.cstar;
+        un = u0 - dtau * cb * (d.u_step(1, j, k) - u0) / (T)g.dx;
+      }
+    } else {
+      const T u1 = a.u_src[c];
+      if (u1 <= (T)0) {
+        un = b.ub[kk];
+      } else {
+        const T cb = u1 + (T)p.cstar;
+        un = u1 - dtau * cb * (u1 - d.u_step(g.nx - 1, j, k)) / (T)g.dx;
+      }
+    }
+    a.u_dst[c] = un;
+  }
+  if (i < g.nx) {
+    T vn;
+    if (j < g.ny && !(p.bcy && j == 0)) {
+      const T zyv = m.hyv[c2] * ((T)1 - m.zeta_c[kk] / m.zt);
+      if (zyv != (T)0) {
+        vn = d.v_step(i, j, k);
+      } else {
+        const T thbv = (T)0.5 * (sT[ty][tx + 1] + sT[ty + 1][tx + 1]);
+        const T mv = (T)0.5 * (m.mapf[c2 - g.NX] + m.mapf[c2]);
+        const double grad = (sP[ty + 1][tx + 1] - sP[ty][tx + 1]) / g.dy;
+        vn = a.v_src[c] + dtau * (-(T)kCp * thbv * mv * (T)grad + a.tv[c]);
+      }
+    } else if (!p.bcy) {
+      vn = d.v_step(i, 0, k);
+    } else if (j == 0) {
+      const T v0 = a.v_src[c];
+      if (v0 >= (T)0) {
+        vn = b.vb[kk];
+      } else {
+        const T cb = v0 - (T)p.cstar;
+        vn = v0 - dtau * cb * (d.v_step(i, 1, k) - v0) / (T)g.dy;
+      }
+    } else {
+      const T v1 = a.v_src[c];
+      if (v1 <= (T)0) {
+        vn = b.vb[kk];
+      } else {
+        const T cb = v1 + (T)p.cstar;
+        vn = v1 - dtau * cb * (v1 - d.v_step(i, g.ny - 1, k)) / (T)g.dy;
+      }
+    }
+    a.v_dst[c] = vn;
+  }
+}
+
 template <class T>
 struct WPArgs {
   const T* u;  // after this substep's horizontal step (faces 0..nx)
@@ -346,8 +448,9 @@ __global__ void __launch_bounds__(TX * NW) k_acou_wpi_tile(Grid g, Params p, Bas
   double* sA = sR + L * TX;  // Av_k           k in [0, nz]
 
   const int lx = threadIdx.x, ry = threadIdx.y;
-  const int i = blockIdx.x * TX + lx;
-  const int j = blockIdx.y;
+  const BlockId bid = xcd_block();
+  const int i = bid.x * TX + lx;
+  const int j = bid.y;
   const bool col = i < g.nx;
   const double c1 = (1.0 + p.beta) * 0.5;
   const double c2 = (1.0 - p.beta) * 0.5;
@@ -556,7 +659,8 @@ __global__ void __launch_bounds__(TX * TY * NW) k_acoustic_fused(Grid g, Params 
 
   const int lc = threadIdx.x, ry = threadIdx.y;
   const int ci = lc % TX, cj = lc / TX;
-  const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+  const BlockId bid = xcd_block();
+  const int i0 = bid.x * TX, j0 = bid.y * TY;
   const int i = i0 + ci, j = j0 + cj;
   const bool own = i < g.nx && j < g.ny;
   const int ih = min(i0 + TX, g.nx), jh = min(j0 + TY, g.ny);

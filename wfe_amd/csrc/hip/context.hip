@@ -263,10 +263,9 @@ class Ctx final : public CtxBase {
 
  private:
   // host helpers
+  enum class Shape { k3D, k2D, k1D };  // padded 3-D, padded plane, NZ column
   template <class E>
-  E* upload_new(const std::vector<double>& h);
-  template <class E>
-  E* upload_new(const double* h, size_t n);
+  E* upload_new(const double* h, Shape shape);
   void fill_ghosts(const StateSet<T>& s);
   void slow_tendencies(const StateSet<T>& est, double dt);
   void bdy_relax(const StateSet<T>& est);
@@ -279,7 +278,13 @@ class Ctx final : public CtxBase {
   dev::Grid g_{};
   dev::Params p_{};
   wfe_params hp_{};
-  size_t npts_ = 0, n2_ = 0;
+  size_t npts_ = 0, n2_ = 0;     // device-layout element counts (3-D, 2-D)
+  int hNX_ = 0;                   // reference-layout row length nx + 2 ng
+  size_t hn3_ = 0, hn2_ = 0;      // reference-layout element counts
+  template <class E>
+  std::vector<E> to_dev(const double* h, bool three_d) const;
+  template <class E>
+  void to_host(const std::vector<E>& d, double* h, bool three_d) const;
   hipStream_t stream_ = nullptr;
   DevArena arena_;
   dev::Base<T> base_{};
@@ -321,20 +326,47 @@ class Ctx final : public CtxBase {
   KernelTimer timer_;
 };
 
+// Reference layout (row length nx + 2ng) <-> device layout (pitch NX, i = 0
+// at column xo). Row padding stays zero on the device.
 template <class T>
 template <class E>
-E* Ctx<T>::upload_new(const double* h, size_t n) {
-  E* d = arena_.alloc<E>(n);
-  if (h) {
-    std::vector<E> tmp(h, h + n);
-    WFE_HIP(hipMemcpy(d, tmp.data(), n * sizeof(E), hipMemcpyHostToDevice));
-  }
+std::vector<E> Ctx<T>::to_dev(const double* h, bool three_d) const {
+  const int nk = three_d ? g_.NZ : 1;
+  std::vector<E> d(three_d ? npts_ : n2_, (E)0);
+  for (int kr = 0; kr < nk; ++kr)
+    for (int jr = 0; jr < g_.NY; ++jr) {
+      const double* src = h + ((size_t)kr * g_.NY + jr) * hNX_;
+      E* dst = d.data() + ((size_t)kr * g_.NY + jr) * g_.NX + (g_.xo - g_.ng);
+      for (int ir = 0; ir < hNX_; ++ir) dst[ir] = (E)src[ir];
+    }
   return d;
 }
+
 template <class T>
 template <class E>
-E* Ctx<T>::upload_new(const std::vector<double>& h) {
-  return upload_new<E>(h.data(), h.size());
+void Ctx<T>::to_host(const std::vector<E>& d, double* h, bool three_d) const {
+  const int nk = three_d ? g_.NZ : 1;
+  for (int kr = 0; kr < nk; ++kr)
+    for (int jr = 0; jr < g_.NY; ++jr) {
+      const E* src = d.data() + ((size_t)kr * g_.NY + jr) * g_.NX + (g_.xo - g_.ng);
+      double* dst = h + ((size_t)kr * g_.NY + jr) * hNX_;
+      for (int ir = 0; ir < hNX_; ++ir) dst[ir] = (double)src[ir];
+    }
+}
+
+// Allocates a device array of the given shape and, when h is non-null,
+// uploads h (reference layout, FP64) converted to E; otherwise zeros.
+template <class T>
+template <class E>
+E* Ctx<T>::upload_new(const double* h, Shape shape) {
+  const size_t nd = shape == Shape::k3D ? npts_ : (shape == Shape::k2D ? n2_ : (size_t)g_.NZ);
+  E* d = arena_.alloc<E>(nd);
+  if (h) {
+    const std::vector<E> tmp = shape == Shape::k1D ? std::vector<E>(h, h + nd)
+                                                   : to_dev<E>(h, shape == Shape::k3D);
+    WFE_HIP(hipMemcpy(d, tmp.data(), nd * sizeof(E), hipMemcpyHostToDevice));
+  }
+  return d;
 }
 
 template <class T>
@@ -344,10 +376,14 @@ Ctx<T>::Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& b, const w
   g_.ny = gr.ny;
   g_.nz = gr.nz;
   g_.ng = gr.ng;
-  g_.NX = gr.nx + 2 * gr.ng;
+  g_.xo = dev::kRowAlign;
+  g_.NX = (g_.xo + gr.nx + gr.ng + dev::kRowAlign - 1) / dev::kRowAlign * dev::kRowAlign;
   g_.NY = gr.ny + 2 * gr.ng;
   g_.NZ = gr.nz + 1 + 2 * gr.ng;
   g_.plane = g_.NX * g_.NY;
+  hNX_ = gr.nx + 2 * gr.ng;
+  hn2_ = (size_t)hNX_ * g_.NY;
+  hn3_ = hn2_ * g_.NZ;
   g_.dx = gr.dx;
   g_.dy = gr.dy;
   npts_ = (size_t)g_.plane * g_.NZ;
@@ -368,56 +404,57 @@ Ctx<T>::Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& b, const w
 
   WFE_HIP(hipStreamCreate(&stream_));
   const size_t n = npts_;
-  const size_t nzl = (size_t)g_.NZ;
+  const size_t hn = hn3_;
+  const Shape s3 = Shape::k3D, s2 = Shape::k2D, s1 = Shape::k1D;
 
   // Base state: FP64 host arrays -> T (pibar stays FP64).
-  base_.thb = upload_new<T>(b.thb, n);
-  base_.rhob = upload_new<T>(b.rhob, n);
-  base_.dthbdz = upload_new<T>(b.dthbdz, n);
-  base_.thbw = upload_new<T>(b.thbw, n);
-  base_.rhobw = upload_new<T>(b.rhobw, n);
-  base_.qvb = upload_new<T>(b.qvb, n);
-  base_.ub = upload_new<T>(b.ub, nzl);
-  base_.vb = upload_new<T>(b.vb, nzl);  // NULL -> zeros
-  base_.pib = upload_new<double>(b.pib, n);
+  base_.thb = upload_new<T>(b.thb, s3);
+  base_.rhob = upload_new<T>(b.rhob, s3);
+  base_.dthbdz = upload_new<T>(b.dthbdz, s3);
+  base_.thbw = upload_new<T>(b.thbw, s3);
+  base_.rhobw = upload_new<T>(b.rhobw, s3);
+  base_.qvb = upload_new<T>(b.qvb, s3);
+  base_.ub = upload_new<T>(b.ub, s1);
+  base_.vb = upload_new<T>(b.vb, s1);  // NULL -> zeros
+  base_.pib = upload_new<double>(b.pib, s3);
 
-  met_.zeta_w = upload_new<T>(m.zeta_w, nzl);
-  met_.zeta_c = upload_new<T>(m.zeta_c, nzl);
-  met_.dzeta_c = upload_new<T>(m.dzeta_c, nzl);
-  met_.dzeta_w = upload_new<T>(m.dzeta_w, nzl);
-  met_.hx = upload_new<T>(m.hx, n2_);
-  met_.hy = upload_new<T>(m.hy, n2_);
-  met_.hxu = upload_new<T>(m.hx_u, n2_);
-  met_.hyv = upload_new<T>(m.hy_v, n2_);
-  met_.jac = upload_new<T>(m.jac, n2_);
-  met_.fcor = upload_new<T>(m.fcor, n2_);
-  met_.mapf = upload_new<T>(m.mapf, n2_);
+  met_.zeta_w = upload_new<T>(m.zeta_w, s1);
+  met_.zeta_c = upload_new<T>(m.zeta_c, s1);
+  met_.dzeta_c = upload_new<T>(m.dzeta_c, s1);
+  met_.dzeta_w = upload_new<T>(m.dzeta_w, s1);
+  met_.hx = upload_new<T>(m.hx, s2);
+  met_.hy = upload_new<T>(m.hy, s2);
+  met_.hxu = upload_new<T>(m.hx_u, s2);
+  met_.hyv = upload_new<T>(m.hy_v, s2);
+  met_.jac = upload_new<T>(m.jac, s2);
+  met_.fcor = upload_new<T>(m.fcor, s2);
+  met_.mapf = upload_new<T>(m.mapf, s2);
   met_.zt = (T)m.zt;
 
   // Acoustic coefficients, formed in FP64 on the host exactly as
   // k_acou_coef_c / k_acou_coef_f (kernels.cu:452-474) do on the device.
   {
-    std::vector<double> rtjc(n), rtjx(n), rtjy(n), kt3(n), aw(n);
+    std::vector<double> rtjc(hn), rtjx(hn), rtjy(hn), kt3(hn), aw(hn);
     for (int kr = 0; kr < g_.NZ; ++kr)
       for (int jr = 0; jr < g_.NY; ++jr)
-        for (int ir = 0; ir < g_.NX; ++ir) {
-          const size_t c = ((size_t)kr * g_.NY + jr) * g_.NX + ir;
-          const double J = m.jac[(size_t)jr * g_.NX + ir];
+        for (int ir = 0; ir < hNX_; ++ir) {
+          const size_t c = ((size_t)kr * g_.NY + jr) * hNX_ + ir;
+          const double J = m.jac[(size_t)jr * hNX_ + ir];
           rtjc[c] = b.rhob[c] * b.thvb[c] * J;
           kt3[c] = dev::kRd * b.pib[c] / (dev::kCv * b.rhob[c] * b.thvb[c] * J);
           aw[c] = b.rhobw[c] * b.thvbw[c];
         }
     for (int kr = 0; kr < g_.NZ; ++kr)
       for (int jr = 0; jr < g_.NY; ++jr)
-        for (int ir = 0; ir < g_.NX; ++ir) {
-          const size_t c = ((size_t)kr * g_.NY + jr) * g_.NX + ir;
+        for (int ir = 0; ir < hNX_; ++ir) {
+          const size_t c = ((size_t)kr * g_.NY + jr) * hNX_ + ir;
           rtjx[c] = ir > 0 ? 0.5 * (rtjc[c - 1] + rtjc[c]) : rtjc[c];
-          rtjy[c] = jr > 0 ? 0.5 * (rtjc[c - g_.NX] + rtjc[c]) : rtjc[c];
+          rtjy[c] = jr > 0 ? 0.5 * (rtjc[c - hNX_] + rtjc[c]) : rtjc[c];
         }
-    coef_.rtjx = upload_new<T>(rtjx);
-    coef_.rtjy = upload_new<T>(rtjy);
-    coef_.kt3 = upload_new<T>(kt3);
-    coef_.aw = upload_new<T>(aw);
+    coef_.rtjx = upload_new<T>(rtjx.data(), s3);
+    coef_.rtjy = upload_new<T>(rtjy.data(), s3);
+    coef_.kt3 = upload_new<T>(kt3.data(), s3);
+    coef_.aw = upload_new<T>(aw.data(), s3);
   }
 
   auto alloc_set = [&](StateSet<T>& s) {
@@ -495,31 +532,34 @@ T* Ctx<T>::field_ptr(int f) {
 
 template <class T>
 void Ctx<T>::upload(int f, const double* h, size_t n) {
-  const size_t want = f == WFE_F_RAIN ? n2_ : npts_;
-  if (n != want) throw ArgError("upload: element count mismatch");
+  const bool two_d = f == WFE_F_RAIN;
+  if (n != (two_d ? hn2_ : hn3_)) throw ArgError("upload: element count mismatch");
   WFE_HIP(hipStreamSynchronize(stream_));
   if (f == WFE_F_PIP) {
-    WFE_HIP(hipMemcpy(S_[cur_].pip, h, n * sizeof(double), hipMemcpyHostToDevice));
+    const std::vector<double> tmp = to_dev<double>(h, true);
+    WFE_HIP(hipMemcpy(S_[cur_].pip, tmp.data(), npts_ * sizeof(double), hipMemcpyHostToDevice));
     return;
   }
   T* d = field_ptr(f);
-  std::vector<T> tmp(h, h + n);
-  WFE_HIP(hipMemcpy(d, tmp.data(), n * sizeof(T), hipMemcpyHostToDevice));
+  const std::vector<T> tmp = to_dev<T>(h, !two_d);
+  WFE_HIP(hipMemcpy(d, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice));
 }
 
 template <class T>
 void Ctx<T>::download(int f, double* h, size_t n) {
-  const size_t want = f == WFE_F_RAIN ? n2_ : npts_;
-  if (n != want) throw ArgError("download: element count mismatch");
+  const bool two_d = f == WFE_F_RAIN;
+  if (n != (two_d ? hn2_ : hn3_)) throw ArgError("download: element count mismatch");
   WFE_HIP(hipStreamSynchronize(stream_));
   if (f == WFE_F_PIP) {
-    WFE_HIP(hipMemcpy(h, S_[cur_].pip, n * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<double> tmp(npts_);
+    WFE_HIP(hipMemcpy(tmp.data(), S_[cur_].pip, npts_ * sizeof(double), hipMemcpyDeviceToHost));
+    to_host(tmp, h, true);
     return;
   }
   const T* d = field_ptr(f);
-  std::vector<T> tmp(n);
-  WFE_HIP(hipMemcpy(tmp.data(), d, n * sizeof(T), hipMemcpyDeviceToHost));
-  for (size_t i = 0; i < n; ++i) h[i] = (double)tmp[i];
+  std::vector<T> tmp(two_d ? n2_ : npts_);
+  WFE_HIP(hipMemcpy(tmp.data(), d, tmp.size() * sizeof(T), hipMemcpyDeviceToHost));
+  to_host(tmp, h, !two_d);
 }
 
 template <class T>
@@ -544,7 +584,7 @@ float Ctx<T>::absmax(int f) {
 
 template <class T>
 void Ctx<T>::set_bdy_weights(const double* w, size_t n) {
-  if (n != n2_) throw ArgError("bdy weights: element count mismatch");
+  if (n != hn2_) throw ArgError("bdy weights: element count mismatch");
   if (!bdy_wgt_) {
     bdy_wgt_ = arena_.alloc<T>(n2_);
     for (int v = 0; v < 5; ++v) {
@@ -552,19 +592,19 @@ void Ctx<T>::set_bdy_weights(const double* w, size_t n) {
       bdy_hi_[v] = arena_.alloc<T>(npts_);
     }
   }
-  std::vector<T> tmp(w, w + n);
-  WFE_HIP(hipMemcpy(bdy_wgt_, tmp.data(), n * sizeof(T), hipMemcpyHostToDevice));
+  const std::vector<T> tmp = to_dev<T>(w, false);
+  WFE_HIP(hipMemcpy(bdy_wgt_, tmp.data(), n2_ * sizeof(T), hipMemcpyHostToDevice));
   bdy_active_ = true;
 }
 
 template <class T>
 void Ctx<T>::set_bdy_target(int slot, int var, const double* h, size_t n) {
   if (!bdy_wgt_) throw ArgError("bdy target before bdy weights");
-  if (slot < 0 || slot > 1 || var < 0 || var > 4 || n != npts_)
+  if (slot < 0 || slot > 1 || var < 0 || var > 4 || n != hn3_)
     throw ArgError("bdy target: bad slot/var/size");
   WFE_HIP(hipStreamSynchronize(stream_));
-  std::vector<T> tmp(h, h + n);
-  WFE_HIP(hipMemcpy(slot == 0 ? bdy_lo_[var] : bdy_hi_[var], tmp.data(), n * sizeof(T),
+  const std::vector<T> tmp = to_dev<T>(h, true);
+  WFE_HIP(hipMemcpy(slot == 0 ? bdy_lo_[var] : bdy_hi_[var], tmp.data(), npts_ * sizeof(T),
                     hipMemcpyHostToDevice));
 }
 
@@ -598,15 +638,8 @@ void Ctx<T>::fill_ghosts(const StateSet<T>& s) {
     Lp.ylast[0] = ny - 1;
     Lp.n = 1;
   }
-  const dim3 bz(64, 4), gz(cdiv(g_.NX, 64), cdiv(g_.NY, 4));
-  const dim3 bx(4, 64), gx(cdiv(g_.NY, 64), (unsigned)g_.NZ);
-  const dim3 by(64, 4), gy(cdiv(g_.NX, 64), (unsigned)g_.NZ);
-  TIMED("fill_ghosts", dev::k_fill_z<T><<<gz, bz, 0, stream_>>>(g_, L));
-  if (Lp.n) TIMED("fill_ghosts", dev::k_fill_z<double><<<gz, bz, 0, stream_>>>(g_, Lp));
-  TIMED("fill_ghosts", dev::k_fill_x<T><<<gx, bx, 0, stream_>>>(g_, p_.bcx, L));
-  if (Lp.n) TIMED("fill_ghosts", dev::k_fill_x<double><<<gx, bx, 0, stream_>>>(g_, p_.bcx, Lp));
-  TIMED("fill_ghosts", dev::k_fill_y<T><<<gy, by, 0, stream_>>>(g_, p_.bcy, L));
-  if (Lp.n) TIMED("fill_ghosts", dev::k_fill_y<double><<<gy, by, 0, stream_>>>(g_, p_.bcy, Lp));
+  const dim3 grid(cdiv(hNX_, 64), cdiv(g_.NY, 4), (unsigned)g_.NZ);
+  TIMED("fill_ghosts", dev::k_fill_all<T><<<grid, dim3(64, 4), 0, stream_>>>(g_, p_.bcx, p_.bcy, L, Lp));
   check_launch("fill_ghosts");
 }
 
@@ -713,7 +746,7 @@ void Ctx<T>::add_split_tile(const char* name, size_t cap) {
   }
   methods_.push_back({name, [this, lb](const dev::FusedArgs<T>& a) {
                         const dim3 guv(cdiv(g_.nx + 1, 64), cdiv(g_.ny + 1, 4), (unsigned)g_.nz);
-                        TIMED("acou_uv", dev::k_acou_uv<T><<<guv, blk3(), 0, stream_>>>(
+                        TIMED("acou_uv", dev::k_acou_uv_tile<T><<<guv, blk3(), 0, stream_>>>(
                                              g_, p_, base_, met_, a.uv));
                         const dev::WPArgs<T> w = split_wp_impl<T>(a);
                         const dim3 gt(cdiv(g_.nx, TX), (unsigned)g_.ny);
@@ -729,7 +762,7 @@ void Ctx<T>::build_methods(const hipDeviceProp_t& prop) {
   // General split pair: one thread per column (any nz <= 319).
   methods_.push_back({"split-col", [this](const dev::FusedArgs<T>& a) {
                         const dim3 guv(cdiv(g_.nx + 1, 64), cdiv(g_.ny + 1, 4), (unsigned)g_.nz);
-                        TIMED("acou_uv", dev::k_acou_uv<T><<<guv, blk3(), 0, stream_>>>(
+                        TIMED("acou_uv", dev::k_acou_uv_tile<T><<<guv, blk3(), 0, stream_>>>(
                                              g_, p_, base_, met_, a.uv));
                         const dev::WPArgs<T> w = split_wp_impl<T>(a);
                         const dim3 gcol(cdiv(g_.nx, 64), (unsigned)g_.ny);
@@ -896,7 +929,10 @@ template <class T>
 void Ctx<T>::kessler_step(const StateSet<T>& s, double dt) {
   dev::KesArgs<T> a{s.thp, s.pip, s.qv, s.qc, s.qr, rain_, (T)dt};
   const dim3 gcol(cdiv(g_.nx, 64), (unsigned)g_.ny);
-  if (g_.nz + 1 <= 64)
+  if (g_.nz <= 8 * 6) {
+    const dim3 gt(cdiv(g_.nx, 64), (unsigned)g_.ny);
+    TIMED("kessler", dev::k_kessler_tile<T, 64, 8, 6><<<gt, dim3(64, 8), dev::kessler_tile_lds<64, 8, T>(g_.nz), stream_>>>(g_, base_, met_, a));
+  } else if (g_.nz + 1 <= 64)
     TIMED("kessler", dev::k_kessler_col<T, 64><<<gcol, dim3(64, 1), 0, stream_>>>(g_, base_, met_, a));
   else
     TIMED("kessler", dev::k_kessler_col<T, 320><<<gcol, dim3(64, 1), 0, stream_>>>(g_, base_, met_, a));

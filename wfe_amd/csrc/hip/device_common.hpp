@@ -10,16 +10,48 @@ inline constexpr double kGrav = 9.81, kCp = 1004.5, kRd = 287.04, kCv = kCp - kR
 inline constexpr double kP00 = 1.0e5, kLv = 2.5e6, kLf = 3.34e5, kLs = kLv + kLf;
 inline constexpr double kRv = 461.5, kEps61 = 0.61, kTfrz = 273.15;
 
-// Padded grid, reference layout (src/core/grid.hpp:22-36). Extents fit int.
+// Device layout of the padded grid. The reference's ordering (i fastest, ng
+// ghosts on each side, one extra w level; src/core/grid.hpp:22-36) is kept,
+// but each row is padded to a pitch NX that is a multiple of 32 elements and
+// the interior i = 0 sits at column xo = 32, so every row's interior starts
+// on a 128-byte line (256 bytes in FP64) and tiles of 32/64 columns map onto
+// whole cache lines. Ghost columns i = -ng..-1 sit at xo - ng..xo - 1.
+// Extents fit int.
 struct Grid {
-  int nx, ny, nz, ng, NX, NY, NZ;
+  int nx, ny, nz, ng;
+  int NX;  // row pitch (elements), multiple of 32
+  int NY, NZ;
   int plane;  // NX*NY
+  int xo;     // column of i = 0 within a row
   double dx, dy;
   __host__ __device__ int id(int i, int j, int k) const {
-    return ((k + ng) * NY + (j + ng)) * NX + (i + ng);
+    return ((k + ng) * NY + (j + ng)) * NX + (i + xo);
   }
-  __host__ __device__ int id2(int i, int j) const { return (j + ng) * NX + (i + ng); }
+  __host__ __device__ int id2(int i, int j) const { return (j + ng) * NX + (i + xo); }
 };
+
+inline constexpr int kRowAlign = 32;
+
+// Workgroups are dispatched to the 8 XCDs round-robin by linear id; this
+// bijection gives each XCD a contiguous range of logical tiles, so tiles
+// that share cache lines (neighbours in x and y) share one L2.
+__device__ __forceinline__ int xcd_remap(int id, int n) {
+  const int x = id & 7, slot = id >> 3;
+  const int q = n >> 3, r = n & 7;
+  return x * q + (x < r ? x : r) + slot;
+}
+
+// This workgroup's logical (x, y, z) block after the XCD remap of its linear
+// dispatch id (x fastest, as the dispatcher orders them).
+struct BlockId {
+  int x, y, z;
+};
+__device__ __forceinline__ BlockId xcd_block() {
+  const int gx = gridDim.x, gy = gridDim.y;
+  const int id = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const int l = xcd_remap(id, gx * gy * gridDim.z);
+  return {l % gx, (l / gx) % gy, l / (gx * gy)};
+}
 
 struct Params {
   double diff_K;

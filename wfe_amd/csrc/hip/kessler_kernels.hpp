@@ -3,8 +3,7 @@
 // after the large step by operator splitting (k_kessler,
 // src/physics/kessler.cu:22-139): CFL-subcycled upwind rain sedimentation
 // with surface accumulation, autoconversion, accretion, rain evaporation,
-// one-Newton-step saturation adjustment. One thread per column; the
-// sedimentation is serial in k with a data-dependent subcycle count.
+// one-Newton-step saturation adjustment.
 #include "hip/device_common.hpp"
 
 namespace wfe::dev {
@@ -37,6 +36,7 @@ struct KesArgs {
   T dt;
 };
 
+// One thread per column (any nz): the sedimentation runs serially in k.
 template <class T, int MAXL>
 __global__ void __launch_bounds__(64) k_kessler_col(Grid g, Base<T> b, Metric<T> m,
                                                     KesArgs<T> a) {
@@ -148,6 +148,173 @@ __global__ void __launch_bounds__(64) k_kessler_col(Grid g, Base<T> b, Metric<T>
     a.qc[c] = qck > (T)0 ? qck : (T)0;
     a.qr[c] = qrk > (T)0 ? qrk : (T)0;
   }
+}
+
+// k-parallel form of the same column physics. A workgroup owns TX columns of
+// one row and NW thread rows (levels k = r, r + NW, ...). The reference's
+// sedimentation update reads the level above before that level is updated
+// (kessler.cu:79-86), so every subcycle is a parallel sweep over k: fluxes
+// F_k = rho_k q_k V_k to LDS, barrier, q_k += dts (F_k+1 - F_k)/(rho_k dz_k).
+// The subcycle count is per column; the workgroup loops to its maximum.
+template <class T, int TX, int NW, int KPT>
+__global__ void __launch_bounds__(TX * NW) k_kessler_tile(Grid g, Base<T> b, Metric<T> m,
+                                                          KesArgs<T> a) {
+  extern __shared__ double kes_lds[];
+  T* sF = reinterpret_cast<T*>(kes_lds);  // [nz + 1][TX]
+  T* sR1 = sF + (g.nz + 1) * TX;          // [NW][TX] reduction scratch (max V)
+  T* sR2 = sR1 + NW * TX;                 // [NW][TX] reduction scratch (min dz)
+  __shared__ int s_nsub[TX];
+  __shared__ int s_nmax;
+  const int lx = threadIdx.x, ry = threadIdx.y;
+  const BlockId bid = xcd_block();
+  const int i = bid.x * TX + lx;
+  const int j = bid.y;
+  const bool col = i < g.nx;
+  const int PL = g.plane;
+  const int c0 = g.id(col ? i : 0, j, 0);
+  const T J = m.jac[g.id2(col ? i : 0, j)];
+  const T dt = a.dt;
+  const T rho_sfc = b.rhob[c0];
+
+  T qrl[KPT], vr[KPT], rho[KPT], dzl[KPT], ff[KPT];
+  T vmax = 0, dzmin = (T)1e30;
+#pragma unroll
+  for (int t = 0; t < KPT; ++t) {
+    const int k = ry + NW * t;
+    qrl[t] = 0;
+    vr[t] = 0;
+    rho[t] = 1;
+    dzl[t] = 1;
+    ff[t] = 1;
+    if (!col || k >= g.nz) continue;
+    const int c = c0 + k * PL;
+    const T q = a.qr[c];
+    qrl[t] = q > (T)0 ? q : (T)0;
+    rho[t] = b.rhob[c];
+    dzl[t] = m.dzeta_c[k + g.ng] * J;
+    const T temp = (T)(((double)b.thb[c] + (double)a.thp[c]) * (b.pib[c] + a.pip[c]));
+    ff[t] = temp < (T)kTfrz ? (T)0.35 : (T)1;
+    const T rq = rho[t] * qrl[t];
+    vr[t] = rq > (T)1e-9 ? (T)36.34 * pow(rq, (T)0.1364) * sqrt(rho_sfc / rho[t]) * ff[t] : (T)0;
+    if (vr[t] > vmax) vmax = vr[t];
+    if (dzl[t] < dzmin) dzmin = dzl[t];
+  }
+  sR1[ry * TX + lx] = vmax;
+  sR2[ry * TX + lx] = dzmin;
+  __syncthreads();
+  if (ry == 0) {
+    for (int r = 1; r < NW; ++r) {
+      const T v = sR1[r * TX + lx], d = sR2[r * TX + lx];
+      if (v > vmax) vmax = v;
+      if (d < dzmin) dzmin = d;
+    }
+    s_nsub[lx] = col ? 1 + (int)(vmax * dt / ((T)0.8 * dzmin)) : 0;
+  }
+  __syncthreads();
+  if (ry == 0 && lx == 0) {
+    int mx = 0;
+    for (int q = 0; q < TX; ++q) mx = s_nsub[q] > mx ? s_nsub[q] : mx;
+    s_nmax = mx;
+  }
+  __syncthreads();
+  const int nsub = s_nsub[lx];
+  const int nmax = s_nmax;
+  const T dts = col ? dt / (T)nsub : (T)0;
+  T acc = 0;
+  for (int sc = 0; sc < nmax; ++sc) {
+    const bool act = sc < nsub;
+#pragma unroll
+    for (int t = 0; t < KPT; ++t) {
+      const int k = ry + NW * t;
+      if (act && k < g.nz) sF[k * TX + lx] = rho[t] * qrl[t] * vr[t];
+    }
+    if (ry == 0) sF[g.nz * TX + lx] = (T)0;
+    __syncthreads();
+    if (act) {
+      if (ry == 0) acc += dts * sF[lx];
+#pragma unroll
+      for (int t = 0; t < KPT; ++t) {
+        const int k = ry + NW * t;
+        if (k >= g.nz) continue;
+        const T ftop = sF[(k + 1) * TX + lx];
+        const T fbot = sF[k * TX + lx];
+        qrl[t] += dts * (ftop - fbot) / (rho[t] * dzl[t]);
+        if (sc + 1 < nsub) {
+          const T rq = rho[t] * (qrl[t] > (T)0 ? qrl[t] : (T)0);
+          vr[t] = rq > (T)1e-9
+                      ? (T)36.34 * pow(rq, (T)0.1364) * sqrt(rho_sfc / rho[t]) * ff[t]
+                      : (T)0;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (col && ry == 0) a.rain[g.id2(i, j)] += acc;
+
+  // conversions + saturation adjustment, level by level (kessler.cu:99-158)
+  const T k1 = (T)0.001, qc0 = (T)0.001;
+#pragma unroll
+  for (int t = 0; t < KPT; ++t) {
+    const int k = ry + NW * t;
+    if (!col || k >= g.nz) continue;
+    const int c = c0 + k * PL;
+    const T rhol = rho[t];
+    T qvk = a.qv[c] > (T)0 ? a.qv[c] : (T)0;
+    T qck = a.qc[c] > (T)0 ? a.qc[c] : (T)0;
+    T qrk = qrl[t] > (T)0 ? qrl[t] : (T)0;
+    const double pid = b.pib[c] + a.pip[c];
+    const T pi = (T)pid;
+    T th = b.thb[c] + a.thp[c];
+    T temp = th * pi;
+    const T pres = (T)(kP00 * pow(pid, kCp / kRd));
+    T fl = (temp - (T)233.15) / (T)40;
+    fl = fmax((T)0, fmin((T)1, fl));
+    const T qsw = qsat_w(pres, temp);
+    T qs = fl * qsw + ((T)1 - fl) * qsat_i(pres, temp);
+    const T Leff = fl * (T)kLv + ((T)1 - fl) * (T)kLs;
+    const T dqauto = k1 * (qck > qc0 ? qck - qc0 : (T)0);
+    const T dqacc = (T)2.2 * qck * pow(qrk > (T)1e-9 ? qrk : (T)1e-9, (T)0.875);
+    T dqr = dt * (dqauto + dqacc);
+    if (dqr > qck) dqr = qck;
+    qck -= dqr;
+    qrk += dqr;
+    T thpk = a.thp[c];
+    if (qvk < qs && qrk > (T)0) {
+      const T rq = rhol * qrk;
+      const T vent = (T)1.6 + (T)124.9 * pow(rq, (T)0.2046);
+      const T ern = vent * pow(rq, (T)0.525) / ((T)2.55e8 / (pres * qs) + (T)5.4e5) *
+                    (qs - qvk) / (rhol * qs);
+      T dqe = dt * ern;
+      if (dqe > qrk) dqe = qrk;
+      if (dqe > qs - qvk) dqe = qs - qvk;
+      qrk -= dqe;
+      qvk += dqe;
+      thpk -= Leff * dqe / ((T)kCp * pi);
+      th = b.thb[c] + thpk;
+      temp = th * pi;
+      const T fl2 = fmax((T)0, fmin((T)1, (temp - (T)233.15) / (T)40));
+      qs = fl2 * qsat_w(pres, temp) + ((T)1 - fl2) * qsat_i(pres, temp);
+    }
+    const T gam = Leff * Leff * qs / ((T)kCp * (T)kRv * temp * temp);
+    T dq = (qvk - qs) / ((T)1 + gam);
+    if (dq < (T)0) {
+      T evap = -dq;
+      if (evap > qck) evap = qck;
+      dq = -evap;
+    }
+    qvk -= dq;
+    qck += dq;
+    thpk += Leff * dq / ((T)kCp * pi);
+    a.thp[c] = thpk;
+    a.qv[c] = qvk > (T)0 ? qvk : (T)0;
+    a.qc[c] = qck > (T)0 ? qck : (T)0;
+    a.qr[c] = qrk > (T)0 ? qrk : (T)0;
+  }
+}
+
+template <int TX, int NW, class T>
+inline size_t kessler_tile_lds(int nz) {
+  return (size_t)((nz + 1) * TX + 2 * NW * TX) * sizeof(T);
 }
 
 }  // namespace wfe::dev

@@ -8,9 +8,10 @@
 namespace wfe::dev {
 
 // ------------------------------------------------------------ ghost fills
-// The three fills reproduce apply_bcs (src/dynamics/kernels.cu:805-835):
-// vertical first, then x over every padded row, then y over every padded
-// column, so corners end up as the composed map of the interior.
+// apply_bcs (src/dynamics/kernels.cu:805-835): vertical zero-gradient (w:
+// w(nz) = 0 and odd reflections, :695-723), then x (periodic or
+// zero-gradient up to the field's last valid index, :725-758), then y
+// (:735-770).
 
 constexpr int kMaxFill = 9;
 enum FillKind { kFillScalar = 0, kFillW = 1 };
@@ -25,82 +26,46 @@ struct FillList {
   int n;
 };
 
-// Vertical: zero-gradient (kernels.cu:695-707) or the w rule
-// (kernels.cu:710-723): w(nz) = 0, odd reflection about w(0) below and
-// about w(nz) above.
+// All ghost cells of a state set in one pass. The reference's sequence
+// (vertical fill of every padded column, then x over every padded row, then
+// y over every padded column) leaves each ghost cell equal to the vertical
+// rule applied to the laterally mapped interior column, so every ghost value
+// is computed directly from interior cells, which this kernel never writes
+// (w(nz) = 0 aside, which no other ghost reads).
 template <class E>
-__global__ void k_fill_z(Grid g, FillList<E> L) {
-  const int ir = blockIdx.x * blockDim.x + threadIdx.x;
-  const int jr = blockIdx.y * blockDim.y + threadIdx.y;
-  if (ir >= g.NX || jr >= g.NY) return;
-  const int col = jr * g.NX + ir;
-  const int st = g.plane;
-  for (int q = 0; q < L.n; ++q) {
-    E* f = L.f[q];
-    if (L.kind[q] == kFillW) {
-      f[col + st * (g.nz + g.ng)] = (E)0;
-      const E w0 = f[col + st * g.ng];
-      for (int m = 1; m <= g.ng; ++m) {
-        f[col + st * (g.ng - m)] = (E)2 * w0 - f[col + st * (g.ng + m)];
-        f[col + st * (g.nz + g.ng + m)] = -f[col + st * (g.nz + g.ng - m)];
-      }
-    } else {
-      const E bot = f[col + st * g.ng];
-      const E top = f[col + st * (g.nz - 1 + g.ng)];
-      for (int m = 1; m <= g.ng; ++m) f[col + st * (g.ng - m)] = bot;
-      for (int m = 1; m <= g.ng + 1; ++m) f[col + st * (g.nz - 1 + g.ng + m)] = top;
+__device__ __forceinline__ void fill_one(const Grid& g, E* f, int kind, int xlast, int ylast,
+                                         int bcx, int bcy, int ii, int jj, int kk) {
+  const int im = map_lat(ii, g.nx, xlast, bcx);
+  const int jm = map_lat(jj, g.ny, ylast, bcy);
+  if (kind == kFillW) {
+    if (kk == g.nz) {
+      f[g.id(ii, jj, kk)] = (E)0;
+    } else if (kk < 0) {
+      f[g.id(ii, jj, kk)] = (E)2 * f[g.id(im, jm, 0)] - f[g.id(im, jm, -kk)];
+    } else if (kk > g.nz) {
+      f[g.id(ii, jj, kk)] = -f[g.id(im, jm, 2 * g.nz - kk)];
+    } else if (im != ii || jm != jj) {
+      f[g.id(ii, jj, kk)] = f[g.id(im, jm, kk)];
     }
+  } else {
+    const int km = kk < 0 ? 0 : (kk > g.nz - 1 ? g.nz - 1 : kk);
+    if (im != ii || jm != jj || km != kk) f[g.id(ii, jj, kk)] = f[g.id(im, jm, km)];
   }
 }
 
-// x ghosts of every padded row: periodic (kernels.cu:725-733) or
-// zero-gradient up to the field's last valid index (kernels.cu:748-758;
-// writes beyond the row are dropped).
-template <class E>
-__global__ void k_fill_x(Grid g, int open, FillList<E> L) {
-  const int m = threadIdx.x;  // 0..ng-1
-  const int jr = blockIdx.x * blockDim.y + threadIdx.y;
-  const int kr = blockIdx.y;
-  if (m >= g.ng || jr >= g.NY || kr >= g.NZ) return;
-  const int row = (kr * g.NY + jr) * g.NX;
-  for (int q = 0; q < L.n; ++q) {
-    E* f = L.f[q];
-    if (open) {
-      const int il = L.xlast[q];
-      const E left = f[row + g.ng];
-      const E right = f[row + g.ng + il];
-      f[row + g.ng - 1 - m] = left;
-      const int ir = g.ng + il + 1 + m;
-      if (ir < g.NX) f[row + ir] = right;
-    } else {
-      f[row + g.ng + g.nx + m] = f[row + g.ng + m];
-      f[row + g.ng - 1 - m] = f[row + g.ng + g.nx - 1 - m];
-    }
-  }
-}
-
-// y ghosts of every padded column (kernels.cu:735-745, 760-770).
-template <class E>
-__global__ void k_fill_y(Grid g, int open, FillList<E> L) {
-  const int ir = blockIdx.x * blockDim.x + threadIdx.x;
-  const int m = threadIdx.y;
-  const int kr = blockIdx.y;
-  if (ir >= g.NX || m >= g.ng || kr >= g.NZ) return;
-  const int plane = kr * g.plane;
-  for (int q = 0; q < L.n; ++q) {
-    E* f = L.f[q];
-    if (open) {
-      const int jl = L.ylast[q];
-      const E lo = f[plane + g.ng * g.NX + ir];
-      const E hi = f[plane + (g.ng + jl) * g.NX + ir];
-      f[plane + (g.ng - 1 - m) * g.NX + ir] = lo;
-      const int jr = g.ng + jl + 1 + m;
-      if (jr < g.NY) f[plane + jr * g.NX + ir] = hi;
-    } else {
-      f[plane + (g.ng + g.ny + m) * g.NX + ir] = f[plane + (g.ng + m) * g.NX + ir];
-      f[plane + (g.ng - 1 - m) * g.NX + ir] = f[plane + (g.ng + g.ny - 1 - m) * g.NX + ir];
-    }
-  }
+template <class T>
+__global__ void k_fill_all(Grid g, int bcx, int bcy, FillList<T> L, FillList<double> Lp) {
+  const int ii = (int)(blockIdx.x * blockDim.x + threadIdx.x) - g.ng;
+  const int jj = (int)(blockIdx.y * blockDim.y + threadIdx.y) - g.ng;
+  const int kk = (int)blockIdx.z - g.ng;
+  if (ii >= g.nx + g.ng || jj >= g.ny + g.ng) return;
+  const bool lat = ii < 0 || ii >= g.nx || jj < 0 || jj >= g.ny;
+  const bool vert = kk < 0 || kk >= g.nz;
+  if (!lat && !vert) return;  // interior: identity map for every field
+  for (int q = 0; q < L.n; ++q)
+    fill_one(g, L.f[q], L.kind[q], L.xlast[q], L.ylast[q], bcx, bcy, ii, jj, kk);
+  for (int q = 0; q < Lp.n; ++q)
+    fill_one(g, Lp.f[q], Lp.kind[q], Lp.xlast[q], Lp.ylast[q], bcx, bcy, ii, jj, kk);
 }
 
 // ------------------------------------------------------------ mass fluxes
@@ -112,9 +77,10 @@ template <class T>
 __global__ void k_massflux(Grid g, Base<T> b, Metric<T> m, const T* __restrict__ u,
                            const T* __restrict__ v, const T* __restrict__ w, T* mfx, T* mfy,
                            T* mfz) {
-  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) - 2;
-  const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) - 2;
-  const int k = blockIdx.z;
+  const BlockId bid = xcd_block();
+  const int i = (int)(bid.x * blockDim.x + threadIdx.x) - 2;
+  const int j = (int)(bid.y * blockDim.y + threadIdx.y) - 2;
+  const int k = bid.z;
   if (i > g.nx + 2 || j > g.ny + 2 || k >= g.nz) return;
   const int c = g.id(i, j, k);
   const int c2 = g.id2(i, j);
@@ -139,9 +105,10 @@ __global__ void k_massflux(Grid g, Base<T> b, Metric<T> m, const T* __restrict__
 template <class T>
 __global__ void k_divergence(Grid g, Metric<T> m, const T* __restrict__ mfx,
                              const T* __restrict__ mfy, const T* __restrict__ mfz, T* div) {
-  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) - 1;
-  const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) - 1;
-  const int k = blockIdx.z;
+  const BlockId bid = xcd_block();
+  const int i = (int)(bid.x * blockDim.x + threadIdx.x) - 1;
+  const int j = (int)(bid.y * blockDim.y + threadIdx.y) - 1;
+  const int k = bid.z;
   if (i > g.nx || j > g.ny || k >= g.nz) return;
   const int c = g.id(i, j, k);
   div[c] = (mfx[c + 1] - mfx[c]) / (T)g.dx + (mfy[c + g.NX] - mfy[c]) / (T)g.dy +
@@ -201,9 +168,10 @@ __global__ void k_tendencies(Grid g, Params p, Base<T> b, Metric<T> m, T dt,
                              const T* __restrict__ qr, const T* __restrict__ mfx,
                              const T* __restrict__ mfy, const T* __restrict__ mfz,
                              const T* __restrict__ div, TendOut out, int do_q) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = blockIdx.z;
+  const BlockId bid = xcd_block();
+  const int i = bid.x * blockDim.x + threadIdx.x;
+  const int j = bid.y * blockDim.y + threadIdx.y;
+  const int k = bid.z;
   if (i >= g.nx || j >= g.ny || k >= g.nz) return;
   const int kk = k + g.ng;
   const int c = g.id(i, j, k);
