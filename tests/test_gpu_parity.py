@@ -176,8 +176,11 @@ def test_column_kernel_variant_matches_oracle(monkeypatch, precision, tol):
     _cmp(c.grid, om, gi, ("u", "v", "w", "thp"), tol, f"column kernel {precision}")
 
 
-def test_deep_column_uses_general_kernel():
-    """nz = 60 exceeds the tiled kernel's level budget: general kernel path."""
+@pytest.mark.parametrize("method", ["split-col", "split-32x16x4"])
+def test_deep_column(monkeypatch, method):
+    """nz = 60 exceeds the 48-level tiles: the 64-level tile and the
+    one-thread-per-column solve."""
+    monkeypatch.setenv("WFE_ACOUSTIC_KERNEL", method)
     c = kitchen_sink(nz=60, dz=300)
     st = perturbed_state(c, 14)
     om, gi = oracle_and_gpu(c, "fp64", st)
@@ -185,12 +188,14 @@ def test_deep_column_uses_general_kernel():
     for s in range(2):
         om.step(dt)
         gi.step(dt)
-    _cmp(c.grid, om, gi, STATE, 1e-10, "nz=60 fp64")
+    _cmp(c.grid, om, gi, STATE, 1e-10, f"nz=60 fp64 {method}")
+    assert f"tuned:{method}" in gi.kernel_report()
 
 
 FUSED_SHAPES = ["fused-8x2x16", "fused-16x1x16", "fused-4x4x16", "fused-8x4x8", "fused-32x1x8",
                 "fused-8x2x24"]
-SPLIT_SHAPES = ["split-32x8", "split-32x16", "split-64x8", "split-16x16", "split-col"]
+SPLIT_SHAPES = ["split-32x8", "split-32x16", "split-64x8", "split-64x16", "split-16x16",
+                "split-32x16x4", "split-col"]
 
 
 @pytest.mark.parametrize("shape", FUSED_SHAPES + SPLIT_SHAPES)

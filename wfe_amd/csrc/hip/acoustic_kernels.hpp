@@ -58,9 +58,9 @@ struct UVDev {
   __device__ __forceinline__ T thv(int ii, int jj, int k) const {
     const int c = g.id(ii, jj, k);
     const bool in = (unsigned)ii < (unsigned)g.nx && (unsigned)jj < (unsigned)g.ny;
-    T t = b.thb[c] + (in ? a.thp_cur[c] : a.thp_n[c]);
-    if (p.moisture) t *= (T)1 + (T)kEps61 * a.qv[c];
-    return t;
+    const T t = b.thb[c] + (in ? a.thp_cur[c] : a.thp_n[c]);
+    const T q = a.qv[c];  // allocated in every mode; zero weight when dry
+    return t * ((T)1 + (T)kEps61 * (p.moisture ? q : (T)0));
   }
   // u after the horizontal PGF step at face (i, j, k) (k_acou_uv :496-512).
   __device__ T u_step(int i, int j, int k) const {
@@ -70,7 +70,7 @@ struct UVDev {
     const double ps0 = pstar(i, j, k);
     const T thbu = (T)0.5 * (thv(i - 1, j, k) + thv(i, j, k));
     const T mu = (T)0.5 * (m.mapf[c2 - 1] + m.mapf[c2]);
-    double grad = (ps0 - pstar(i - 1, j, k)) / g.dx;
+    double grad = (ps0 - pstar(i - 1, j, k)) * g.rdx;
     const T zxu = m.hxu[c2] * ((T)1 - m.zeta_c[kk] / m.zt);
     if (zxu != (T)0) {
       const T dzc2 = m.zeta_c[kk + 1] - m.zeta_c[kk - 1];
@@ -90,7 +90,7 @@ struct UVDev {
     const double ps0 = pstar(i, j, k);
     const T thbv = (T)0.5 * (thv(i, j - 1, k) + thv(i, j, k));
     const T mv = (T)0.5 * (m.mapf[c2 - g.NX] + m.mapf[c2]);
-    double grad = (ps0 - pstar(i, j - 1, k)) / g.dy;
+    double grad = (ps0 - pstar(i, j - 1, k)) * g.rdy;
     const T zyv = m.hyv[c2] * ((T)1 - m.zeta_c[kk] / m.zt);
     if (zyv != (T)0) {
       const T dzc2 = m.zeta_c[kk + 1] - m.zeta_c[kk - 1];
@@ -190,6 +190,25 @@ __global__ void __launch_bounds__(256) k_acou_uv_tile(Grid g, Params p, Base<T> 
   const int i0 = bid.x * BX, j0 = bid.y * BY;
   const int k = bid.z;
   const UVDev<T> d{g, p, b, m, a};
+  // This thread's faces: u at (i, j) for j < ny, v at (i, j) for i < nx.
+  // Their own operands are loaded before the staging barrier so that both
+  // sets of loads are in flight together.
+  const int i = i0 + tx, j = j0 + ty;
+  const bool live = i <= g.nx && j <= g.ny;
+  const int c = g.id(live ? i : 0, live ? j : 0, k);
+  const int c2 = g.id2(live ? i : 0, live ? j : 0);
+  T usrc = 0, tus = 0, vsrc = 0, tvs = 0, mapc = 0, mapw = 0, maps = 0, hxu = 0, hyv = 0;
+  if (live) {
+    usrc = a.u_src[c];
+    tus = a.tu[c];
+    vsrc = a.v_src[c];
+    tvs = a.tv[c];
+    mapc = m.mapf[c2];
+    mapw = m.mapf[c2 - 1];
+    maps = m.mapf[c2 - g.NX];
+    hxu = m.hxu[c2];
+    hyv = m.hyv[c2];
+  }
   // Stage cells i in [i0-1, i0+BX-1], j in [j0-1, j0+BY-1] that exist
   // (cells up to nx-1 / ny-1 plus the halo, which may be a ghost).
   for (int e = ty * BX + tx; e < (BX + 1) * (BY + 1); e += BX * BY) {
@@ -201,28 +220,25 @@ __global__ void __launch_bounds__(256) k_acou_uv_tile(Grid g, Params p, Base<T> 
     }
   }
   __syncthreads();
-  const int i = i0 + tx, j = j0 + ty;
-  if (i > g.nx || j > g.ny || k >= g.nz) return;
+  if (!live) return;
   const int kk = k + g.ng;
-  const int c = g.id(i, j, k);
-  const int c2 = g.id2(i, j);
   const T dtau = a.dtau;
   if (j < g.ny) {
     T un;
     if (i < g.nx && !(p.bcx && i == 0)) {
-      const T zxu = m.hxu[c2] * ((T)1 - m.zeta_c[kk] / m.zt);
-      if (zxu != (T)0) {
+      // terrain cross term only where the slope is non-zero (zeta_c < zt)
+      if (hxu != (T)0) {
         un = d.u_step(i, j, k);
       } else {
         const T thbu = (T)0.5 * (sT[ty + 1][tx] + sT[ty + 1][tx + 1]);
-        const T mu = (T)0.5 * (m.mapf[c2 - 1] + m.mapf[c2]);
-        const double grad = (sP[ty + 1][tx + 1] - sP[ty + 1][tx]) / g.dx;
-        un = a.u_src[c] + dtau * (-(T)kCp * thbu * mu * (T)grad + a.tu[c]);
+        const T mu = (T)0.5 * (mapw + mapc);
+        const double grad = (sP[ty + 1][tx + 1] - sP[ty + 1][tx]) * g.rdx;
+        un = usrc + dtau * (-(T)kCp * thbu * mu * (T)grad + tus);
       }
     } else if (!p.bcx) {
       un = d.u_step(0, j, k);
     } else if (i == 0) {
-      const T u0 = a.u_src[c];
+      const T u0 = usrc;
       if (u0 >= (T)0) {
         un = b.ub[kk];
       } else {
@@ -230,7 +246,7 @@ __global__ void __launch_bounds__(256) k_acou_uv_tile(Grid g, Params p, Base<T> 
         un = u0 - dtau * cb * (d.u_step(1, j, k) - u0) / (T)g.dx;
       }
     } else {
-      const T u1 = a.u_src[c];
+      const T u1 = usrc;
       if (u1 <= (T)0) {
         un = b.ub[kk];
       } else {
@@ -243,19 +259,18 @@ __global__ void __launch_bounds__(256) k_acou_uv_tile(Grid g, Params p, Base<T> 
   if (i < g.nx) {
     T vn;
     if (j < g.ny && !(p.bcy && j == 0)) {
-      const T zyv = m.hyv[c2] * ((T)1 - m.zeta_c[kk] / m.zt);
-      if (zyv != (T)0) {
+      if (hyv != (T)0) {
         vn = d.v_step(i, j, k);
       } else {
         const T thbv = (T)0.5 * (sT[ty][tx + 1] + sT[ty + 1][tx + 1]);
-        const T mv = (T)0.5 * (m.mapf[c2 - g.NX] + m.mapf[c2]);
-        const double grad = (sP[ty + 1][tx + 1] - sP[ty][tx + 1]) / g.dy;
-        vn = a.v_src[c] + dtau * (-(T)kCp * thbv * mv * (T)grad + a.tv[c]);
+        const T mv = (T)0.5 * (maps + mapc);
+        const double grad = (sP[ty + 1][tx + 1] - sP[ty][tx + 1]) * g.rdy;
+        vn = vsrc + dtau * (-(T)kCp * thbv * mv * (T)grad + tvs);
       }
     } else if (!p.bcy) {
       vn = d.v_step(i, 0, k);
     } else if (j == 0) {
-      const T v0 = a.v_src[c];
+      const T v0 = vsrc;
       if (v0 >= (T)0) {
         vn = b.vb[kk];
       } else {
@@ -263,7 +278,7 @@ __global__ void __launch_bounds__(256) k_acou_uv_tile(Grid g, Params p, Base<T> 
         vn = v0 - dtau * cb * (d.v_step(i, 1, k) - v0) / (T)g.dy;
       }
     } else {
-      const T v1 = a.v_src[c];
+      const T v1 = vsrc;
       if (v1 <= (T)0) {
         vn = b.vb[kk];
       } else {
@@ -421,31 +436,45 @@ __global__ void __launch_bounds__(64) k_acou_wpi_col(Grid g, Params p, Base<T> b
 
 // Tiled, k-parallel form of the same column solve. A workgroup owns TX
 // consecutive columns of one row j and NW thread rows; thread row r handles
-// the levels k = r, r + NW, ... . Three phases, separated by barriers:
-//   1. every thread assembles, for its levels, P_k, E_k, Av_k and the
-//      tridiagonal pieces G_k, RHS_k into LDS (FP64) -- all loads of the
-//      column are issued here, in parallel over k;
-//   2. thread row 0 (one lane per column) runs the Thomas elimination and
-//      the back substitution in the reference's order from LDS;
-//   3. every thread writes w, pi', pi'prev and theta' of its levels.
-// LDS: 5 (nz + 1) TX doubles. Levels per thread are bounded by KPT.
+// the levels k = r, r + NW, ... . Phases, separated by barriers:
+//   1. every thread assembles, for its levels, P_k, E_k, Av_k, G_k, RHS_k
+//      in registers (all loads of the column are issued here, in parallel
+//      over k) and publishes P_k;
+//   2. every thread forms the tridiagonal row k (A_k, B_k, C_k, D_k of
+//      kernels.cu:589-628) from its registers and P_{k-1}, E_{k-1},
+//      Av_{k-1}, Av_{k+1}, so the serial part carries only the recurrence;
+//   3. thread row 0 (one lane per column) runs the elimination and the back
+//      substitution in the reference's order, with one reciprocal of the
+//      pivot per level (rcp_nr);
+//   4. every thread writes w, pi', pi'prev and theta' of its levels.
+// LDS: 4 (nz + 1) TX doubles. Levels per thread are bounded by KPT.
 template <int TX, int NW, int KPT>
 struct WpiTile {
   static constexpr int kThreads = TX * NW;
-  static size_t lds_bytes(int nz) { return (size_t)5 * (nz + 1) * TX * sizeof(double); }
+  static size_t lds_bytes(int nz) { return (size_t)4 * (nz + 1) * TX * sizeof(double); }
 };
 
-template <class T, int TX, int NW, int KPT>
+// 1/x for well-scaled x (pivots of a diagonally dominant system, base-state
+// theta, the Jacobian): v_rcp_f64 followed by two Newton-Raphson steps
+// r <- r + r (1 - x r), within an ulp or so of the quotient.
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+template <class T, int TX, int NW, int KPT, bool MOIST, bool TPI>
 __global__ void __launch_bounds__(TX * NW) k_acou_wpi_tile(Grid g, Params p, Base<T> b,
                                                            Metric<T> m, Coef<T> cf,
                                                            WPArgs<T> a) {
   extern __shared__ double lds[];
   const int L = g.nz + 1;
-  double* sP = lds;          // P_k            k in [0, nz)
-  double* sE = sP + L * TX;  // E_k            k in [0, nz)
-  double* sG = sE + L * TX;  // G_k -> cs_k    k in [1, nz)
-  double* sR = sG + L * TX;  // RHS_k -> ds_k -> W_k, k in [0, nz]
-  double* sA = sR + L * TX;  // Av_k           k in [0, nz]
+  double* sD = lds;          // P_k, then D_k -> ds_k -> W_k (k in [0, nz])
+  double* sA = sD + L * TX;  // A_k            k in [1, nz)
+  double* sB = sA + L * TX;  // B_k            k in [1, nz)
+  double* sC = sB + L * TX;  // C_k -> cs_k    k in [1, nz)
 
   const int lx = threadIdx.x, ry = threadIdx.y;
   const BlockId bid = xcd_block();
@@ -461,18 +490,21 @@ __global__ void __launch_bounds__(TX * NW) k_acou_wpi_tile(Grid g, Params p, Bas
   const double hxc = (double)m.hx[c2d];
   const double hyc = (double)m.hy[c2d];
   const double mc = (double)m.mapf[c2d];
-  const double zt = (double)m.zt;
+  const double rJ = rcp_nr(J);
+  // Over flat ground the cross-metric flux S_v is exactly zero.
+  const bool flat = hxc == 0.0 && hyc == 0.0;
   const int c0 = g.id(col ? i : 0, j, 0);
 
-  // Values kept in registers from phase 1 to phase 3.
-  double pold[KPT];
-  double thold[KPT], tthk[KPT], dthk[KPT];
+  // Values kept in registers across the phases (storage-type values stay in
+  // T; their FP64 conversion is exact).
+  double Pk[KPT], Ek[KPT], Gk[KPT], Rk[KPT], pold[KPT];
+  T Avk[KPT], Avk1[KPT], thold[KPT], tthk[KPT], dthk[KPT];
   double w0 = 0.0;
 
   auto sv = [&](int k) -> double {  // cross-metric vertical flux at w level k
-    if (k <= 0 || k >= g.nz) return 0.0;
+    if (k <= 0 || k >= g.nz || flat) return 0.0;
     const int c = c0 + k * PL;
-    const double fac = 1.0 - (double)m.zeta_w[k + g.ng] / zt;
+    const double fac = 1.0 - (double)m.zeta_w[k + g.ng] * m.rzt;
     const double uw = 0.25 * ((double)a.u[c - PL] + (double)a.u[c - PL + 1] + (double)a.u[c] +
                               (double)a.u[c + 1]);
     const double vw = 0.25 * ((double)a.v[c - PL] + (double)a.v[c - PL + NX] +
@@ -482,120 +514,194 @@ __global__ void __launch_bounds__(TX * NW) k_acou_wpi_tile(Grid g, Params p, Bas
   auto av = [&](int k) -> double {
     return (k <= 0 || k >= g.nz) ? 0.0 : (double)cf.aw[c0 + k * PL];
   };
+  auto ecoef = [&](int k) -> double {  // E_k = dtau K_t c1 / dzeta_c
+    return dtau * (double)cf.kt3[c0 + k * PL] * c1 * m.rdzc[k + g.ng];
+  };
 
-  // ---- phase 1: assembly, parallel over k
+  // ---- phase 1: assembly, parallel over k. Every load of a level is
+  // issued unconditionally (levels clamped into [0, nz), the padded layout
+  // holds k - 1 and k + 1), so the loads of all KPT levels are in flight
+  // together; the level guards only select results and stores.
 #pragma unroll
   for (int t = 0; t < KPT; ++t) {
     const int k = ry + NW * t;
-    if (col && k < g.nz) {
-      const int c = c0 + k * PL;
-      const int kk = k + g.ng;
-      const double Kt = (double)cf.kt3[c];
-      const double Dh = mc * (((double)cf.rtjx[c + 1] * (double)a.u[c + 1] -
-                               (double)cf.rtjx[c] * (double)a.u[c]) /
-                                  g.dx +
-                              ((double)cf.rtjy[c + NX] * (double)a.v[c + NX] -
-                               (double)cf.rtjy[c] * (double)a.v[c]) /
-                                  g.dy);
-      const double dzc = (double)m.dzeta_c[kk];
-      const double avk = av(k), avk1 = av(k + 1);
-      const double wk = (double)a.w_src[c];
-      const double wkp = (double)a.w_src[c + PL];
-      const double tpi = a.tpi ? (double)a.tpi[c] : 0.0;
-      const double pk = a.pip_src[c];
-      pold[t] = pk;
-      sP[k * TX + lx] = pk + dtau * tpi -
-                        dtau * Kt *
-                            (Dh - (sv(k + 1) - sv(k)) / dzc +
-                             c2 * (avk1 * wkp - avk * wk) / dzc);
-      sE[k * TX + lx] = dtau * Kt * c1 / dzc;
-      sA[k * TX + lx] = avk;
-      const double thk = (double)a.thp_src[c];
-      thold[t] = thk;
-      tthk[t] = (double)a.tth[c];
-      dthk[t] = (double)b.dthbdz[c];
-      if (k >= 1) {
-        const int cm = c - PL;
-        const double thm = (double)a.thp_src[cm];
-        double thvw = (double)b.thbw[c] + 0.5 * (thm + thk);
-        if (p.moisture) thvw *= 1.0 + kEps61 * 0.5 * ((double)a.qv[cm] + (double)a.qv[c]);
-        const double dzw = (double)m.dzeta_w[kk];
-        sG[k * TX + lx] = dtau * kCp * thvw * c1 / (J * dzw);
-        double buoy = kGrav * 0.5 * (thm + thk) / (double)b.thbw[c];
-        if (p.moisture) {
-          const double dqv = 0.5 * ((double)a.qv[cm] + (double)a.qv[c]) -
-                             0.5 * ((double)b.qvb[cm] + (double)b.qvb[c]);
-          const double qcr = 0.5 * ((double)a.qc[cm] + (double)a.qc[c]) +
-                             0.5 * ((double)a.qr[cm] + (double)a.qr[c]);
-          buoy += kGrav * (kEps61 * dqv - qcr);
-        }
-        sR[k * TX + lx] = wk + dtau * (buoy + (double)a.tw[c]) -
-                          dtau * kCp * thvw * c2 * (pk - a.pip_src[cm]) / (J * dzw);
-      } else {
-        const double us = 0.5 * ((double)a.u[c] + (double)a.u[c + 1]);
-        const double vs = 0.5 * ((double)a.v[c] + (double)a.v[c + NX]);
-        w0 = us * hxc + vs * hyc;
-      }
+    const bool act = col && k < g.nz;
+    const int kc = k < g.nz ? k : g.nz - 1;
+    const int c = c0 + kc * PL;
+    const int cm = c - PL;
+    const int kk = kc + g.ng;
+    const double Kt = (double)cf.kt3[c];
+    const double Dh = mc * (((double)cf.rtjx[c + 1] * (double)a.u[c + 1] -
+                             (double)cf.rtjx[c] * (double)a.u[c]) *
+                                g.rdx +
+                            ((double)cf.rtjy[c + NX] * (double)a.v[c + NX] -
+                             (double)cf.rtjy[c] * (double)a.v[c]) *
+                                g.rdy);
+    const double rdzc = m.rdzc[kk];
+    const T awk = kc >= 1 ? cf.aw[c] : (T)0;
+    const T awk1 = kc + 1 < g.nz ? cf.aw[c + PL] : (T)0;
+    const double avk = (double)awk, avk1 = (double)awk1;
+    const double wk = (double)a.w_src[c];
+    const double wkp = (double)a.w_src[c + PL];
+    const double tpi = TPI ? (double)a.tpi[c] : 0.0;
+    const double pk = a.pip_src[c];
+    const double pkm = a.pip_src[cm];
+    const T thkT = a.thp_src[c];
+    const double thk = (double)thkT;
+    const double thm = (double)a.thp_src[cm];
+    const double thbw = (double)b.thbw[c];
+    const double twk = (double)a.tw[c];
+    double mfac = 1.0, mbuoy = 0.0;
+    if constexpr (MOIST) {
+      const double qvw = 0.5 * ((double)a.qv[cm] + (double)a.qv[c]);
+      const double dqv = qvw - 0.5 * ((double)b.qvb[cm] + (double)b.qvb[c]);
+      const double qcr = 0.5 * ((double)a.qc[cm] + (double)a.qc[c]) +
+                         0.5 * ((double)a.qr[cm] + (double)a.qr[c]);
+      mfac = 1.0 + kEps61 * qvw;
+      mbuoy = kGrav * (kEps61 * dqv - qcr);
     }
+    pold[t] = pk;
+    Pk[t] = pk + dtau * tpi -
+            dtau * Kt *
+                (Dh - (sv(kc + 1) - sv(kc)) * rdzc + c2 * (avk1 * wkp - avk * wk) * rdzc);
+    Ek[t] = dtau * Kt * c1 * rdzc;
+    Avk[t] = awk;
+    Avk1[t] = awk1;
+    thold[t] = thkT;
+    tthk[t] = a.tth[c];
+    dthk[t] = b.dthbdz[c];
+    // w level k >= 1: G_k and the right-hand side (kernels.cu:560-585)
+    const double thvw = (thbw + 0.5 * (thm + thk)) * mfac;
+    const double rJdzw = rJ * m.rdzw[kk];
+    const double buoy = kGrav * 0.5 * (thm + thk) * rcp_nr(thbw) + mbuoy;
+    Gk[t] = kc >= 1 ? dtau * kCp * thvw * c1 * rJdzw : 0.0;
+    Rk[t] = kc >= 1 ? wk + dtau * (buoy + twk) - dtau * kCp * thvw * c2 * (pk - pkm) * rJdzw
+                    : 0.0;
+    if (k == 0) {  // diagnostic w(0) = u h_x + v h_y at the surface
+      const double us = 0.5 * ((double)a.u[c] + (double)a.u[c + 1]);
+      const double vs = 0.5 * ((double)a.v[c] + (double)a.v[c + NX]);
+      w0 = us * hxc + vs * hyc;
+    }
+    if (act) sD[k * TX + lx] = Pk[t];
   }
-  if (ry == 0) sA[g.nz * TX + lx] = 0.0;
   __syncthreads();
 
-  // ---- phase 2: Thomas elimination (kernels.cu:589-628 order), one lane per column
+  // ---- phase 2: tridiagonal rows (kernels.cu:596-603), parallel over k
+  double Dk[KPT];
+#pragma unroll
+  for (int t = 0; t < KPT; ++t) {
+    const int k = ry + NW * t;
+    Dk[t] = 0.0;
+    if (col && k >= 1 && k < g.nz) {
+      const double e_prev = ecoef(k - 1);
+      const double av_prev = av(k - 1);
+      const double G = Gk[t];
+      sA[k * TX + lx] = -G * e_prev * av_prev;
+      sB[k * TX + lx] = 1.0 + G * (double)Avk[t] * (Ek[t] + e_prev);
+      sC[k * TX + lx] = -G * Ek[t] * (double)Avk1[t];
+      Dk[t] = Rk[t] - G * (Pk[t] - sD[(k - 1) * TX + lx]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < KPT; ++t) {
+    const int k = ry + NW * t;
+    if (col && k >= 1 && k < g.nz) sD[k * TX + lx] = Dk[t];
+  }
+  __syncthreads();
+
+  // ---- phase 3: elimination (kernels.cu:589-628 order), one lane per column.
+  // Rows are read from LDS in chunks of four, one chunk ahead of the
+  // recurrence (row indices clamped into range; a clamped read is never
+  // used), so only the FP64 chain is serial.
   if (ry == 0 && col) {
-    double e_prev = sE[lx], av_prev = 0.0, av_cur = sA[TX + lx], p_prev = sP[lx];
-    double cs_prev = 0.0, ds_prev = 0.0;
-    for (int k = 1; k < g.nz; ++k) {
-      const double ek = sE[k * TX + lx];
-      const double G = sG[k * TX + lx];
-      const double R = sR[k * TX + lx];
-      const double pk = sP[k * TX + lx];
-      const double av_next = sA[(k + 1) * TX + lx];
-      const double A = -G * e_prev * av_prev;
-      const double B = 1.0 + G * av_cur * (ek + e_prev);
-      const double C = -G * ek * av_next;
-      const double D = R - G * (pk - p_prev);
-      double cs, ds;
-      if (k == 1) {
-        cs = C / B;
-        ds = D / B;
-      } else {
-        const double mm = B - A * cs_prev;
-        cs = C / mm;
-        ds = (D - A * ds_prev) / mm;
+    constexpr int U = 4;
+    const int n1 = g.nz - 1;  // rows k = 1 .. nz-1
+    auto at = [&](int k) { return k * TX + lx; };
+    double A[U], B[U], C[U], D[U], nA[U], nB[U], nC[U], nD[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int r = min(1 + u, n1);
+      A[u] = sA[at(r)];
+      B[u] = sB[at(r)];
+      C[u] = sC[at(r)];
+      D[u] = sD[at(r)];
+    }
+    double cs = 0.0, ds = 0.0;
+    for (int k0 = 1; k0 <= n1; k0 += U) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int r = min(k0 + U + u, n1);
+        nA[u] = sA[at(r)];
+        nB[u] = sB[at(r)];
+        nC[u] = sC[at(r)];
+        nD[u] = sD[at(r)];
       }
-      sG[k * TX + lx] = cs;
-      sR[k * TX + lx] = ds;
-      e_prev = ek;
-      av_prev = av_cur;
-      av_cur = av_next;
-      p_prev = pk;
-      cs_prev = cs;
-      ds_prev = ds;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = k0 + u;
+        if (k <= n1) {
+          const double r = rcp_nr(B[u] - A[u] * cs);
+          ds = (D[u] - A[u] * ds) * r;
+          cs = C[u] * r;
+          sC[at(k)] = cs;
+          sD[at(k)] = ds;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        A[u] = nA[u];
+        B[u] = nB[u];
+        C[u] = nC[u];
+        D[u] = nD[u];
+      }
     }
-    double wabove = 0.0;
-    for (int k = g.nz - 1; k >= 1; --k) {
-      const double wk = sR[k * TX + lx] - sG[k * TX + lx] * wabove;
-      sR[k * TX + lx] = wk;
-      wabove = wk;
+    // W_k = ds_k - cs_k W_{k+1}, k = nz-1 .. 1, with W_nz = 0
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int r = max(n1 - u, 1);
+      C[u] = sC[at(r)];
+      D[u] = sD[at(r)];
     }
-    sR[lx] = w0;
-    sR[g.nz * TX + lx] = 0.0;
+    double wab = 0.0;
+    for (int k0 = n1; k0 >= 1; k0 -= U) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int r = max(k0 - U - u, 1);
+        nC[u] = sC[at(r)];
+        nD[u] = sD[at(r)];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = k0 - u;
+        if (k >= 1) {
+          wab = D[u] - C[u] * wab;
+          sD[at(k)] = wab;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        C[u] = nC[u];
+        D[u] = nD[u];
+      }
+    }
+    sD[lx] = w0;
+    sD[g.nz * TX + lx] = 0.0;
   }
   __syncthreads();
 
-  // ---- phase 3: pi', pi'prev, theta', w
+  // ---- phase 4: pi', pi'prev, theta', w
 #pragma unroll
   for (int t = 0; t < KPT; ++t) {
     const int k = ry + NW * t;
     if (col && k < g.nz) {
       const int c = c0 + k * PL;
-      const double Wk = sR[k * TX + lx], Wk1 = sR[(k + 1) * TX + lx];
-      const double Ak = sA[k * TX + lx], Ak1 = sA[(k + 1) * TX + lx];
-      a.pip_dst[c] = sP[k * TX + lx] - sE[k * TX + lx] * (Ak1 * Wk1 - Ak * Wk);
+      const double Wk = sD[k * TX + lx], Wk1 = sD[(k + 1) * TX + lx];
+      a.pip_dst[c] = Pk[t] - Ek[t] * ((double)Avk1[t] * Wk1 - (double)Avk[t] * Wk);
       if (a.piprev_dst) a.piprev_dst[c] = pold[t];
       const double wc = 0.5 * (Wk + Wk1);
-      a.thp_dst[c] = (T)(thold[t] + dtau * (tthk[t] - wc * dthk[t]));
+      a.thp_dst[c] =
+          (T)((double)thold[t] + dtau * ((double)tthk[t] - wc * (double)dthk[t]));
       a.w_dst[c] = (T)Wk;
     }
   }

@@ -386,6 +386,8 @@ Ctx<T>::Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& b, const w
   hn3_ = hn2_ * g_.NZ;
   g_.dx = gr.dx;
   g_.dy = gr.dy;
+  g_.rdx = 1.0 / gr.dx;
+  g_.rdy = 1.0 / gr.dy;
   npts_ = (size_t)g_.plane * g_.NZ;
   n2_ = (size_t)g_.plane;
   p_.diff_K = hp.diff_K;
@@ -430,6 +432,16 @@ Ctx<T>::Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& b, const w
   met_.fcor = upload_new<T>(m.fcor, s2);
   met_.mapf = upload_new<T>(m.mapf, s2);
   met_.zt = (T)m.zt;
+  met_.rzt = 1.0 / m.zt;
+  {
+    std::vector<double> rc(g_.NZ), rw(g_.NZ);
+    for (int k = 0; k < g_.NZ; ++k) {
+      rc[k] = m.dzeta_c[k] != 0 ? 1.0 / m.dzeta_c[k] : 0.0;
+      rw[k] = m.dzeta_w[k] != 0 ? 1.0 / m.dzeta_w[k] : 0.0;
+    }
+    met_.rdzc = upload_new<double>(rc.data(), s1);
+    met_.rdzw = upload_new<double>(rw.data(), s1);
+  }
 
   // Acoustic coefficients, formed in FP64 on the host exactly as
   // k_acou_coef_c / k_acou_coef_f (kernels.cu:452-474) do on the device.
@@ -638,8 +650,8 @@ void Ctx<T>::fill_ghosts(const StateSet<T>& s) {
     Lp.ylast[0] = ny - 1;
     Lp.n = 1;
   }
-  const dim3 grid(cdiv(hNX_, 64), cdiv(g_.NY, 4), (unsigned)g_.NZ);
-  TIMED("fill_ghosts", dev::k_fill_all<T><<<grid, dim3(64, 4), 0, stream_>>>(g_, p_.bcx, p_.bcy, L, Lp));
+  const int ncell = dev::ghost_count(g_);
+  TIMED("fill_ghosts", dev::k_fill_all<T><<<cdiv(ncell, 256), 256, 0, stream_>>>(g_, p_.bcx, p_.bcy, L, Lp));
   check_launch("fill_ghosts");
 }
 
@@ -648,11 +660,7 @@ void Ctx<T>::slow_tendencies(const StateSet<T>& s, double dt) {
   {
     dim3 grd(cdiv(g_.nx + 5, 64), cdiv(g_.ny + 5, 4), (unsigned)g_.nz);
     TIMED("massflux", dev::k_massflux<T><<<grd, blk3(), 0, stream_>>>(g_, base_, met_, s.u, s.v, s.w, mfx_, mfy_,
-                                                    mfz_));
-  }
-  {
-    dim3 grd(cdiv(g_.nx + 2, 64), cdiv(g_.ny + 2, 4), (unsigned)g_.nz);
-    TIMED("divergence", dev::k_divergence<T><<<grd, blk3(), 0, stream_>>>(g_, met_, mfx_, mfy_, mfz_, div_));
+                                                    mfz_, div_));
   }
   const int do_q = p_.moisture && !p_.pd_moist;
   dev::TendOut o{tu_, tv_, tw_, tth_, {tq_[0], tq_[1], tq_[2]}};
@@ -736,13 +744,18 @@ void Ctx<T>::add_split_tile(const char* name, size_t cap) {
   using Tile = dev::WpiTile<TX, NW, KPT>;
   if (g_.nz > NW * KPT) return;
   const size_t lb = Tile::lds_bytes(g_.nz);
+  // Instantiations by (moisture, pi' tendency present).
+  const void* fns[4] = {reinterpret_cast<const void*>(&dev::k_acou_wpi_tile<T, TX, NW, KPT, false, false>),
+                        reinterpret_cast<const void*>(&dev::k_acou_wpi_tile<T, TX, NW, KPT, false, true>),
+                        reinterpret_cast<const void*>(&dev::k_acou_wpi_tile<T, TX, NW, KPT, true, false>),
+                        reinterpret_cast<const void*>(&dev::k_acou_wpi_tile<T, TX, NW, KPT, true, true>)};
   if (lb > 65536) {
-    if (lb > cap || hipFuncSetAttribute(
-                        reinterpret_cast<const void*>(&dev::k_acou_wpi_tile<T, TX, NW, KPT>),
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb) != hipSuccess) {
-      (void)hipGetLastError();
-      return;
-    }
+    if (lb > cap) return;
+    for (const void* f : fns)
+      if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb) != hipSuccess) {
+        (void)hipGetLastError();
+        return;
+      }
   }
   methods_.push_back({name, [this, lb](const dev::FusedArgs<T>& a) {
                         const dim3 guv(cdiv(g_.nx + 1, 64), cdiv(g_.ny + 1, 4), (unsigned)g_.nz);
@@ -750,9 +763,20 @@ void Ctx<T>::add_split_tile(const char* name, size_t cap) {
                                              g_, p_, base_, met_, a.uv));
                         const dev::WPArgs<T> w = split_wp_impl<T>(a);
                         const dim3 gt(cdiv(g_.nx, TX), (unsigned)g_.ny);
-                        TIMED("acou_wpi", dev::k_acou_wpi_tile<T, TX, NW, KPT>
-                                          <<<gt, dim3(TX, NW), lb, stream_>>>(g_, p_, base_, met_,
-                                                                              coef_, w));
+                        const dim3 bt(TX, NW);
+                        const bool moist = p_.moisture, tpi = w.tpi != nullptr;
+                        if (moist && tpi)
+                          TIMED("acou_wpi", (dev::k_acou_wpi_tile<T, TX, NW, KPT, true, true>
+                                             <<<gt, bt, lb, stream_>>>(g_, p_, base_, met_, coef_, w)));
+                        else if (moist)
+                          TIMED("acou_wpi", (dev::k_acou_wpi_tile<T, TX, NW, KPT, true, false>
+                                             <<<gt, bt, lb, stream_>>>(g_, p_, base_, met_, coef_, w)));
+                        else if (tpi)
+                          TIMED("acou_wpi", (dev::k_acou_wpi_tile<T, TX, NW, KPT, false, true>
+                                             <<<gt, bt, lb, stream_>>>(g_, p_, base_, met_, coef_, w)));
+                        else
+                          TIMED("acou_wpi", (dev::k_acou_wpi_tile<T, TX, NW, KPT, false, false>
+                                             <<<gt, bt, lb, stream_>>>(g_, p_, base_, met_, coef_, w)));
                       }});
 }
 
@@ -777,10 +801,10 @@ void Ctx<T>::build_methods(const hipDeviceProp_t& prop) {
                       }});
   add_split_tile<32, 8, 6>("split-32x8", cap);
   add_split_tile<32, 16, 3>("split-32x16", cap);
-  add_split_tile<32, 24, 2>("split-32x24", cap);
   add_split_tile<64, 8, 6>("split-64x8", cap);
   add_split_tile<64, 16, 3>("split-64x16", cap);
   add_split_tile<16, 16, 3>("split-16x16", cap);
+  add_split_tile<32, 16, 4>("split-32x16x4", cap);  // columns up to 64 levels
   for (const auto& v : fused_variants<T>()) {
     if (g_.nz > v.nw * v.kpt) continue;
     const size_t lb = v.lds(g_.nz);
@@ -918,8 +942,8 @@ void Ctx<T>::acoustic_stage(const StateSet<T>& out, double dtau, int nsub) {
 template <class T>
 void Ctx<T>::stage_moisture(const StateSet<T>& out, double dt_stage) {
   const StateSet<T>& sn = S_[cur_];
-  const int n = (int)npts_;
-  TIMED("stage_q", dev::k_stage_q<T><<<cdiv(n, 256), 256, 0, stream_>>>(n, (T)dt_stage, sn.qv, sn.qc, sn.qr,
+  const dim3 grd(cdiv(g_.nx, 64), cdiv(g_.ny, 4), (unsigned)g_.nz);
+  TIMED("stage_q", dev::k_stage_q<T><<<grd, blk3(), 0, stream_>>>(g_, (T)dt_stage, sn.qv, sn.qc, sn.qr,
                                                        tq_[0], tq_[1], tq_[2], out.qv, out.qc,
                                                        out.qr));
   check_launch("stage_moisture");

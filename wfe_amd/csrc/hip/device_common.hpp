@@ -24,6 +24,7 @@ struct Grid {
   int plane;  // NX*NY
   int xo;     // column of i = 0 within a row
   double dx, dy;
+  double rdx, rdy;  // 1/dx, 1/dy
   __host__ __device__ int id(int i, int j, int k) const {
     return ((k + ng) * NY + (j + ng)) * NX + (i + xo);
   }
@@ -42,15 +43,18 @@ __device__ __forceinline__ int xcd_remap(int id, int n) {
 }
 
 // This workgroup's logical (x, y, z) block after the XCD remap of its linear
-// dispatch id (x fastest, as the dispatcher orders them).
+// dispatch id (x fastest, as the dispatcher orders them). Logical ids run x
+// fastest, then z, then y, so one XCD holds whole columns of a band of rows
+// and the vertical stencil neighbours of a block share its L2.
 struct BlockId {
   int x, y, z;
 };
 __device__ __forceinline__ BlockId xcd_block() {
-  const int gx = gridDim.x, gy = gridDim.y;
+  const int gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
   const int id = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
-  const int l = xcd_remap(id, gx * gy * gridDim.z);
-  return {l % gx, (l / gx) % gy, l / (gx * gy)};
+  const int l = xcd_remap(id, gx * gy * gz);
+  const int t = l / gx;
+  return {l - t * gx, t / gz, t % gz};
 }
 
 struct Params {
@@ -77,6 +81,10 @@ struct Metric {
   const T *zeta_w, *zeta_c, *dzeta_c, *dzeta_w;
   const T *hx, *hy, *hxu, *hyv, *jac, *fcor, *mapf;
   T zt;
+  // FP64 reciprocals for the column solve: 1/dzeta_c, 1/dzeta_w per level
+  // (index k + ng) and 1/zt.
+  const double *rdzc, *rdzw;
+  double rzt;
 };
 
 // Constant acoustic coefficients (AcousticCoef, src/dynamics/kernels.hpp:46-50).

@@ -31,41 +31,91 @@ struct FillList {
 // y over every padded column) leaves each ghost cell equal to the vertical
 // rule applied to the laterally mapped interior column, so every ghost value
 // is computed directly from interior cells, which this kernel never writes
-// (w(nz) = 0 aside, which no other ghost reads).
+// (w(nz) = 0 aside, which no other ghost reads). One thread per ghost cell:
+// the 2 ng + 1 full slabs k in [-ng, 0) and [nz, nz + ng], then the lateral
+// ring of every interior level. All loads of a cell are issued before any
+// store.
+struct GhostCell {
+  int i, j, k;
+};
+
+__device__ __forceinline__ GhostCell ghost_cell(const Grid& g, int idx) {
+  const int hx = g.nx + 2 * g.ng, hy = g.ny + 2 * g.ng;
+  const int plane = hx * hy;
+  const int nslab = 2 * g.ng + 1;
+  GhostCell q;
+  if (idx < nslab * plane) {
+    const int lev = idx / plane, r = idx - lev * plane;
+    q.k = lev < g.ng ? lev - g.ng : g.nz + (lev - g.ng);
+    q.j = r / hx - g.ng;
+    q.i = r - (q.j + g.ng) * hx - g.ng;
+    return q;
+  }
+  const int ring = plane - g.nx * g.ny;
+  int r = idx - nslab * plane;
+  q.k = r / ring;
+  r -= q.k * ring;
+  const int band = g.ng * hx;  // ng full rows below j = 0, and above j = ny - 1
+  if (r < 2 * band) {
+    const int top = r >= band;
+    r -= top * band;
+    const int jr = r / hx;
+    q.j = top ? g.ny + jr : jr - g.ng;
+    q.i = r - jr * hx - g.ng;
+  } else {
+    r -= 2 * band;  // rows j in [0, ny): ng columns west and east
+    q.j = r / (2 * g.ng);
+    const int c = r - q.j * 2 * g.ng;
+    q.i = c < g.ng ? c - g.ng : g.nx + (c - g.ng);
+  }
+  return q;
+}
+
+__host__ __device__ inline int ghost_count(const Grid& g) {
+  const int hx = g.nx + 2 * g.ng, hy = g.ny + 2 * g.ng;
+  return (2 * g.ng + 1) * hx * hy + g.nz * (hx * hy - g.nx * g.ny);
+}
+
+// Value the fill leaves in ghost cell (ii, jj, kk) of f.
 template <class E>
-__device__ __forceinline__ void fill_one(const Grid& g, E* f, int kind, int xlast, int ylast,
-                                         int bcx, int bcy, int ii, int jj, int kk) {
+__device__ __forceinline__ E ghost_value(const Grid& g, const E* f, int kind, int xlast,
+                                         int ylast, int bcx, int bcy, int ii, int jj, int kk) {
   const int im = map_lat(ii, g.nx, xlast, bcx);
   const int jm = map_lat(jj, g.ny, ylast, bcy);
   if (kind == kFillW) {
-    if (kk == g.nz) {
-      f[g.id(ii, jj, kk)] = (E)0;
-    } else if (kk < 0) {
-      f[g.id(ii, jj, kk)] = (E)2 * f[g.id(im, jm, 0)] - f[g.id(im, jm, -kk)];
-    } else if (kk > g.nz) {
-      f[g.id(ii, jj, kk)] = -f[g.id(im, jm, 2 * g.nz - kk)];
-    } else if (im != ii || jm != jj) {
-      f[g.id(ii, jj, kk)] = f[g.id(im, jm, kk)];
-    }
-  } else {
-    const int km = kk < 0 ? 0 : (kk > g.nz - 1 ? g.nz - 1 : kk);
-    if (im != ii || jm != jj || km != kk) f[g.id(ii, jj, kk)] = f[g.id(im, jm, km)];
+    if (kk == g.nz) return (E)0;
+    if (kk < 0) return (E)2 * f[g.id(im, jm, 0)] - f[g.id(im, jm, -kk)];
+    if (kk > g.nz) return -f[g.id(im, jm, 2 * g.nz - kk)];
+    return f[g.id(im, jm, kk)];
   }
+  const int km = kk < 0 ? 0 : (kk > g.nz - 1 ? g.nz - 1 : kk);
+  return f[g.id(im, jm, km)];
 }
 
 template <class T>
-__global__ void k_fill_all(Grid g, int bcx, int bcy, FillList<T> L, FillList<double> Lp) {
-  const int ii = (int)(blockIdx.x * blockDim.x + threadIdx.x) - g.ng;
-  const int jj = (int)(blockIdx.y * blockDim.y + threadIdx.y) - g.ng;
-  const int kk = (int)blockIdx.z - g.ng;
-  if (ii >= g.nx + g.ng || jj >= g.ny + g.ng) return;
-  const bool lat = ii < 0 || ii >= g.nx || jj < 0 || jj >= g.ny;
-  const bool vert = kk < 0 || kk >= g.nz;
-  if (!lat && !vert) return;  // interior: identity map for every field
-  for (int q = 0; q < L.n; ++q)
-    fill_one(g, L.f[q], L.kind[q], L.xlast[q], L.ylast[q], bcx, bcy, ii, jj, kk);
-  for (int q = 0; q < Lp.n; ++q)
-    fill_one(g, Lp.f[q], Lp.kind[q], Lp.xlast[q], Lp.ylast[q], bcx, bcy, ii, jj, kk);
+__global__ void __launch_bounds__(256) k_fill_all(Grid g, int bcx, int bcy, FillList<T> L,
+                                                  FillList<double> Lp) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ghost_count(g)) return;
+  const GhostCell q = ghost_cell(g, idx);
+  T v[kMaxFill];
+  double vp[2];
+#pragma unroll
+  for (int f = 0; f < kMaxFill; ++f)
+    if (f < L.n)
+      v[f] = ghost_value(g, L.f[f], L.kind[f], L.xlast[f], L.ylast[f], bcx, bcy, q.i, q.j, q.k);
+#pragma unroll
+  for (int f = 0; f < 2; ++f)
+    if (f < Lp.n)
+      vp[f] = ghost_value(g, Lp.f[f], Lp.kind[f], Lp.xlast[f], Lp.ylast[f], bcx, bcy, q.i, q.j,
+                          q.k);
+  const int c = g.id(q.i, q.j, q.k);
+#pragma unroll
+  for (int f = 0; f < kMaxFill; ++f)
+    if (f < L.n) L.f[f][c] = v[f];
+#pragma unroll
+  for (int f = 0; f < 2; ++f)
+    if (f < Lp.n) Lp.f[f][c] = vp[f];
 }
 
 // ------------------------------------------------------------ mass fluxes
@@ -73,46 +123,60 @@ __global__ void k_fill_all(Grid g, int bcx, int bcy, FillList<T> L, FillList<dou
 // (k_massflux_xy / k_massflux_z, kernels.cu:68-97): mfx, mfy on
 // i,j in [-2, n+2]; mfz = rho_w (w - (u^xz h_x + v^yz h_y)(1 - zeta_w/zt))
 // on i,j in [-2, n+1], k in [1, nz) (k = 0 and nz stay 0: Omega = 0).
+// The same launch forms the flux divergence for the advective-consistency
+// term (k_divergence kernels.cu:99-110) on i,j in [-1, n], recomputing the
+// three upper-face fluxes of the cell instead of reading them back.
+template <class T>
+struct MassFlux {
+  const Grid& g;
+  const Base<T>& b;
+  const Metric<T>& m;
+  const T* __restrict__ u;
+  const T* __restrict__ v;
+  const T* __restrict__ w;
+  __device__ __forceinline__ T fx(int c, int c2) const {
+    const T rj = b.rhob[c] * m.jac[c2];
+    return (T)0.5 * (b.rhob[c - 1] * m.jac[c2 - 1] + rj) * u[c];
+  }
+  __device__ __forceinline__ T fy(int c, int c2) const {
+    const T rj = b.rhob[c] * m.jac[c2];
+    return (T)0.5 * (b.rhob[c - g.NX] * m.jac[c2 - g.NX] + rj) * v[c];
+  }
+  // k in [1, nz); 0 elsewhere
+  __device__ __forceinline__ T fz(int c, int c2, int k) const {
+    if (k < 1 || k >= g.nz) return (T)0;
+    const T fac = (T)1 - m.zeta_w[k + g.ng] / m.zt;
+    const int cm = c - g.plane;
+    const T uw = (T)0.25 * (u[cm] + u[cm + 1] + u[c] + u[c + 1]);
+    const T vw = (T)0.25 * (v[cm] + v[cm + g.NX] + v[c] + v[c + g.NX]);
+    const T cross = (uw * m.hx[c2] + vw * m.hy[c2]) * fac;
+    return b.rhobw[c] * (w[c] - cross);
+  }
+};
+
 template <class T>
 __global__ void k_massflux(Grid g, Base<T> b, Metric<T> m, const T* __restrict__ u,
                            const T* __restrict__ v, const T* __restrict__ w, T* mfx, T* mfy,
-                           T* mfz) {
+                           T* mfz, T* div) {
   const BlockId bid = xcd_block();
   const int i = (int)(bid.x * blockDim.x + threadIdx.x) - 2;
   const int j = (int)(bid.y * blockDim.y + threadIdx.y) - 2;
   const int k = bid.z;
   if (i > g.nx + 2 || j > g.ny + 2 || k >= g.nz) return;
+  const MassFlux<T> F{g, b, m, u, v, w};
   const int c = g.id(i, j, k);
   const int c2 = g.id2(i, j);
-  const T rj = b.rhob[c] * m.jac[c2];
-  const T rjx = (T)0.5 * (b.rhob[c - 1] * m.jac[c2 - 1] + rj);
-  mfx[c] = rjx * u[c];
-  const T rjy = (T)0.5 * (b.rhob[c - g.NX] * m.jac[c2 - g.NX] + rj);
-  mfy[c] = rjy * v[c];
-  if (k >= 1 && i <= g.nx + 1 && j <= g.ny + 1) {
-    const int kk = k + g.ng;
-    const T fac = (T)1 - m.zeta_w[kk] / m.zt;
-    const int cm = c - g.plane;
-    const T uw = (T)0.25 * (u[cm] + u[cm + 1] + u[c] + u[c + 1]);
-    const T vw = (T)0.25 * (v[cm] + v[cm + g.NX] + v[c] + v[c + g.NX]);
-    const T cross = (uw * m.hx[c2] + vw * m.hy[c2]) * fac;
-    mfz[c] = b.rhobw[c] * (w[c] - cross);
+  const T fxc = F.fx(c, c2), fyc = F.fy(c, c2);
+  mfx[c] = fxc;
+  mfy[c] = fyc;
+  const T fzc = F.fz(c, c2, k);
+  if (k >= 1 && i <= g.nx + 1 && j <= g.ny + 1) mfz[c] = fzc;
+  if (i >= -1 && i <= g.nx && j >= -1 && j <= g.ny) {
+    const T fxr = F.fx(c + 1, c2 + 1);
+    const T fyr = F.fy(c + g.NX, c2 + g.NX);
+    const T fzt = F.fz(c + g.plane, c2, k + 1);
+    div[c] = (fxr - fxc) / (T)g.dx + (fyr - fyc) / (T)g.dy + (fzt - fzc) / m.dzeta_c[k + g.ng];
   }
-}
-
-// Flux divergence for the advective-consistency term (k_divergence
-// kernels.cu:99-110) on i,j in [-1, n].
-template <class T>
-__global__ void k_divergence(Grid g, Metric<T> m, const T* __restrict__ mfx,
-                             const T* __restrict__ mfy, const T* __restrict__ mfz, T* div) {
-  const BlockId bid = xcd_block();
-  const int i = (int)(bid.x * blockDim.x + threadIdx.x) - 1;
-  const int j = (int)(bid.y * blockDim.y + threadIdx.y) - 1;
-  const int k = bid.z;
-  if (i > g.nx || j > g.ny || k >= g.nz) return;
-  const int c = g.id(i, j, k);
-  div[c] = (mfx[c + 1] - mfx[c]) / (T)g.dx + (mfy[c + g.NX] - mfy[c]) / (T)g.dy +
-           (mfz[c + g.plane] - mfz[c]) / m.dzeta_c[k + g.ng];
 }
 
 // Flux through the face whose upper cell is c (stride s along the normal):
@@ -411,17 +475,24 @@ __global__ void k_bdy_relax(Grid g, RelaxSet<T> R) {
 }
 
 // ---------------------------------------------------- moisture stage update
-// out = s_n + dt_stage * T over all padded points (k_stage_q kernels.cu:268-273).
+// out = s_n + dt_stage * T (k_stage_q kernels.cu:268-273). The reference
+// sweeps every padded point; its ghost results (T = 0 there) are replaced
+// by the ghost fill at the start of the next stage before anything reads
+// them, so only the interior is updated here.
 template <class T>
-__global__ void k_stage_q(int n, T dt, const T* __restrict__ q0, const T* __restrict__ q1,
+__global__ void k_stage_q(Grid g, T dt, const T* __restrict__ q0, const T* __restrict__ q1,
                           const T* __restrict__ q2, const T* __restrict__ t0,
                           const T* __restrict__ t1, const T* __restrict__ t2, T* o0, T* o1,
                           T* o2) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n) return;
-  o0[idx] = q0[idx] + dt * t0[idx];
-  o1[idx] = q1[idx] + dt * t1[idx];
-  o2[idx] = q2[idx] + dt * t2[idx];
+  const BlockId bid = xcd_block();
+  const int i = bid.x * blockDim.x + threadIdx.x;
+  const int j = bid.y * blockDim.y + threadIdx.y;
+  const int k = bid.z;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+  const int c = g.id(i, j, k);
+  o0[c] = q0[c] + dt * t0[c];
+  o1[c] = q1[c] + dt * t1[c];
+  o2[c] = q2[c] + dt * t2[c];
 }
 
 // ------------------------------------------------------------ |f| max
