@@ -37,18 +37,35 @@ CASE = ROOT / "cases" / "cfg2_supercell.ini"
 ALG_BYTES_STEP = {"mixed": 1520, "fp64": 2712}
 
 # Per-kernel algorithmic bytes per interior point per launch (distinct arrays
-# the kernel must read and write, moist case).
+# the kernel must read and write, moist case, storage widths of the mode).
 #  acou_wpi      : r u v w thp tw tth thbw dthbdz rtjx rtjy kt3 aw qv qc qr qvb
 #                  (16 x 4 B) + pip (8 B); w thp (4 B) + pip (8 B) written.
 #  acou_uv       : r u v thp qv tu tv thb (7 x 4 B) + pip pipprev (8 B); w u v.
 #  acoustic_fused: one whole substep, the 27-access list above minus the
 #                  pi'prev write (the rotation makes it the previous buffer):
 #                  r 19 x 4 B + pip pipprev 8 B; w u v w thp 4 B + pip 8 B.
+#  tendencies    : r u v w thp qv qc qr mfx mfy mfz div rho rho_w; w Tu Tv Tw
+#                  Tthp Tqv Tqc Tqr.
+#  massflux      : r u v w rho rho_w; w mfx mfy mfz div.
+#  stage_q       : r q_n x 3, Tq x 3; w q x 3.
+#  kessler       : r thp qv qc qr thb rho (4 B) pi' pib (8 B); w thp qv qc qr.
 KERNEL_BYTES = {
     "mixed": {"acou_wpi": 16 * 4 + 8 + 2 * 4 + 8, "acou_uv": 7 * 4 + 2 * 8 + 2 * 4,
-              "acoustic_fused": 19 * 4 + 2 * 8 + 4 * 4 + 8},
-    "fp64": {"acou_wpi": 20 * 8, "acou_uv": 11 * 8, "acoustic_fused": 26 * 8},
+              "acoustic_fused": 19 * 4 + 2 * 8 + 4 * 4 + 8, "tendencies": 13 * 4 + 7 * 4,
+              "massflux": 5 * 4 + 4 * 4, "stage_q": 9 * 4, "kessler": 6 * 4 + 2 * 8 + 4 * 4},
+    "fp64": {"acou_wpi": 20 * 8, "acou_uv": 11 * 8, "acoustic_fused": 26 * 8,
+             "tendencies": 20 * 8, "massflux": 9 * 8, "stage_q": 9 * 8, "kessler": 12 * 8},
 }
+# Kernel symbol of each timed launch name (for the rocprofv3 PMC summary).
+KERNEL_SYMBOL = {"acou_wpi": "k_acou_wpi_tile", "acou_uv": "k_acou_uv_tile",
+                 "acoustic_fused": "k_acoustic_fused", "tendencies": "k_tendencies",
+                 "massflux": "k_massflux", "fill_ghosts": "k_fill_all", "stage_q": "k_stage_q",
+                 "kessler": "k_kessler"}
+# HBM traffic per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+# of this benchmark (written by python -m wfe_amd.prof_summary --json).
+PMC_SUMMARY = ROOT / "profiles" / "pmc_supercell_120x120x40_mixed.json"
+SOLO_KERNELS = ("acou_wpi", "acou_uv", "acoustic_fused", "tendencies", "massflux",
+                "fill_ghosts", "stage_q")
 
 
 def log(*a):
@@ -71,8 +88,10 @@ def timed_run(case, precision: str, warmup: int, steps: int):
     return gi, t1 - t0
 
 
-def kernel_profile(case, precision: str, warmup: int, steps: int):
-    """HIP-event timing of every launch on the context's stream."""
+def kernel_profile(case, precision: str, warmup: int, steps: int, solo_reps: int):
+    """Per-launch HIP-event timing of every kernel of the step on the
+    context's stream, then the heaviest kernels launched back to back
+    between two events (no per-launch event overhead)."""
     from wfe_amd import Integrator
 
     gi = Integrator.from_case(case, precision)
@@ -85,7 +104,65 @@ def kernel_profile(case, precision: str, warmup: int, steps: int):
     rep = gi.kernel_times()
     tune = gi.tuning()
     gi.kernel_timing(0)
-    return rep, tune
+    solo = {}
+    for k in sorted(rep, key=lambda k: -rep[k][1]):
+        if k in SOLO_KERNELS and len(solo) < 4:
+            solo[k] = gi.kernel_bench(k, solo_reps)
+    return rep, tune, solo
+
+
+def pmc_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` (mixed build) from the committed
+    rocprofv3 PMC summary, or None."""
+    if not PMC_SUMMARY.exists():
+        return None
+    d = json.loads(PMC_SUMMARY.read_text())
+    sym = KERNEL_SYMBOL.get(kernel)
+    cands = [(v.get("calls", 0), v) for n, v in d.get("kernels", {}).items()
+             if sym and n.startswith(sym + "<float")]
+    if not cands:
+        return None
+    v = max(cands, key=lambda cv: cv[0])[1]
+    if v.get("fetch_bytes") is None or v.get("write_bytes") is None:
+        return None
+    return v["fetch_bytes"] + v["write_bytes"]
+
+
+def error_growth(case, checkpoints):
+    """Relative L2 error over the interior of u, v, w, theta' against the
+    plain FP64 GPU run, at the given step counts, for (a) the mixed path and
+    (b) the FP64 path started from the initial state rounded to FP32 (the
+    round-off the mixed path starts from), which measures how fast the flow
+    itself amplifies a perturbation of that size."""
+    from wfe_amd import Integrator
+
+    g = case.grid
+    dt = case.scalar("dt")
+    ref = Integrator.from_case(case, "fp64")
+    mix = Integrator.from_case(case, "mixed")
+    per = Integrator.from_case(case, "fp64")
+    for f in ("u", "v", "w", "thp", "pip", "qv", "qc", "qr"):
+        per.set(f, ref.get(f).astype(np.float32).astype(np.float64))
+    out = {"steps": [], "mixed_vs_fp64": {}, "fp64_from_fp32_ic_vs_fp64": {}}
+    fields = ("u", "v", "w", "thp")
+    for f in fields:
+        out["mixed_vs_fp64"][f] = []
+        out["fp64_from_fp32_ic_vs_fp64"][f] = []
+    n = 0
+    for target in checkpoints:
+        while n < target:
+            for gi in (ref, mix, per):
+                gi.step(dt, n * dt)
+            n += 1
+        out["steps"].append(n)
+        for f in fields:
+            nzl = g.nz + 1 if f == "w" else g.nz
+            b = g.interior(ref.get(f), nzl)
+            nb = float(np.linalg.norm(b))
+            for key, gi in (("mixed_vs_fp64", mix), ("fp64_from_fp32_ic_vs_fp64", per)):
+                a = g.interior(gi.get(f), nzl)
+                out[key][f].append(float(np.linalg.norm(a - b) / nb) if nb > 0 else None)
+    return out
 
 
 def cpu_baseline(case, budget_s: float):
@@ -116,6 +193,9 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=30)
+    ap.add_argument("--solo-reps", type=int, default=200)
+    ap.add_argument("--error-steps", type=int, default=600,
+                    help="last checkpoint of the error-growth study (600 steps = 1 h)")
     ap.add_argument("--mixed-only", action="store_true",
                     help="skip the FP64 run and the error comparison (profiling runs)")
     args = ap.parse_args()
@@ -156,19 +236,28 @@ def main() -> int:
     value = N * K / t_m
     value_fp64 = N * K / t_d if t_d else None
     log(f"[bench] kernel profile: {args.profile_steps} steps")
-    rep, tune = kernel_profile(case, "mixed", 5, args.profile_steps)
+    rep, tune, solo = kernel_profile(case, "mixed", 5, args.profile_steps, args.solo_reps)
     dom = max(rep, key=lambda k: rep[k][1])
     cnt, tot_ms = rep[dom]
-    avg_s = tot_ms / cnt / 1e3
     kb = KERNEL_BYTES["mixed"].get(dom)
     roof = None
-    if kb is not None:
+    if kb is not None and dom in solo:
+        avg_s = solo[dom] * 1e-6
         ach = kb * N / avg_s
+        traffic = pmc_traffic(dom)
         roof = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": ach / HBM_PEAK, "traffic": None, "kernel": dom,
-                "avg_launch_us": avg_s * 1e6, "launches": cnt,
-                "alg_bytes_per_launch": kb * N}
+                "frac": ach / HBM_PEAK, "traffic": traffic, "kernel": dom,
+                "avg_launch_us": solo[dom], "avg_launch_us_in_step": 1e3 * tot_ms / cnt,
+                "launches_per_step": cnt / args.profile_steps,
+                "alg_bytes_per_launch": kb * N,
+                "traffic_source": str(PMC_SUMMARY.relative_to(ROOT)) if traffic else None}
     step_bw = ALG_BYTES_STEP["mixed"] * N * K / t_m
+
+    growth = None
+    if not args.mixed_only:
+        cps = [c for c in (1, 10, 50, 100, 300, 600, 1200) if c <= max(args.error_steps, 1)]
+        log(f"[bench] error growth against FP64 at steps {cps}")
+        growth = error_growth(case, cps)
 
     cpu = None
     if not args.no_cpu:
@@ -197,11 +286,13 @@ def main() -> int:
         "fp64_plain": {"value": value_fp64, "ms_per_step": 1e3 * t_d / K,
                        "speedup_mixed_vs_fp64": value / value_fp64} if t_d else None,
         "rel_err_vs_fp64": errs,
+        "error_growth": growth,
         "step_roofline": {"alg_bytes_per_pt_step": ALG_BYTES_STEP["mixed"],
                           "achieved_GBps": step_bw / 1e9, "frac": step_bw / HBM_PEAK},
         "roofline": roof,
         "kernels_ms": {k: {"launches": v[0], "total_ms": v[1], "avg_us": 1e3 * v[1] / v[0]}
                        for k, v in sorted(rep.items(), key=lambda kv: -kv[1][1])},
+        "kernel_solo_us": solo,
         "acoustic_tuning": tune,
         "copy_probe_GBps": copy_bw,
         "cpu_baseline": cpu,

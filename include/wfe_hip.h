@@ -162,6 +162,14 @@ int wfe_kessler(wfe_ctx* ctx, double dt);
 int wfe_kernel_timing(wfe_ctx* ctx, int mode);
 /* Per-kernel totals as text lines "name launches total_ms\n" (synchronous). */
 int wfe_kernel_timing_report(wfe_ctx* ctx, char* buf, size_t n);
+/* Average launch time [us] of one kernel of the step ("acou_wpi", "acou_uv",
+ * "acoustic_fused", "tendencies", "massflux", "fill_ghosts", "stage_q"),
+ * launched reps times back to back on the context's stream between two HIP
+ * events, with the arguments it has in the first substep / stage of a step.
+ * Only work buffers, tendencies, fluxes and ghost cells are written, all of
+ * which the next step recomputes. A measurement helper beside the profile
+ * above; the reference has no counterpart. */
+int wfe_kernel_bench(wfe_ctx* ctx, const char* kernel, int reps, double* avg_us);
 
 /* Device-wide streaming copy of `bytes` (rounded down to 16 B), repeated
  * reps times on a private stream: achieved read+write bandwidth in GB/s.

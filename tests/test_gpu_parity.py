@@ -234,3 +234,24 @@ def test_fused_shapes_agree_to_rounding(monkeypatch):
     for shape, o in outs.items():
         for f in ref:
             assert rel_err(o[f], ref[f]) < 1e-6, (shape, f, rel_err(o[f], ref[f]))
+
+
+def test_kernel_bench_leaves_the_trajectory_unchanged():
+    """wfe_kernel_bench writes only what the next step recomputes."""
+    c = kitchen_sink()
+    st = perturbed_state(c, 41)
+    om, gi = oracle_and_gpu(c, "fp64", st)
+    dt = c.scalar("dt")
+    om.step(dt)
+    gi.step(dt)
+    chosen = gi.tuning()["chosen"]
+    acoustic = ("acoustic_fused",) if chosen.startswith("fused") else ("acou_wpi", "acou_uv")
+    for k in acoustic + ("tendencies", "massflux", "fill_ghosts", "stage_q"):
+        us = gi.kernel_bench(k, 5)
+        assert us > 0, k
+    with pytest.raises(Exception):
+        gi.kernel_bench("no_such_kernel", 5)
+    for s in range(2):
+        om.step(dt)
+        gi.step(dt)
+    _cmp(c.grid, om, gi, STATE, 1e-10, "after kernel_bench")

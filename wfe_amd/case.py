@@ -189,6 +189,14 @@ class Integrator:
         return {k: v for k, v in self.kernel_report().items()
                 if v[0] > 0 and not k.startswith(("tuned:", "cand:"))}
 
+    def kernel_bench(self, kernel: str, reps: int = 200) -> float:
+        """Average launch time [us] of one kernel of the step, launched reps
+        times back to back between two HIP events on the context's stream."""
+        out = C.c_double()
+        check(lib().wfe_kernel_bench(self._h, kernel.encode(), int(reps), C.byref(out)),
+              "wfe_kernel_bench")
+        return out.value
+
     def tuning(self) -> dict:
         """The acoustic-substep shape the context chose and every candidate's
         average time [ms] from its tuning pass."""

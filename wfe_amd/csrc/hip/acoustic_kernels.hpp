@@ -299,7 +299,6 @@ struct WPArgs {
   const T* thp_src;
   T* w_dst;
   double* pip_dst;
-  double* piprev_dst;
   T* thp_dst;
   const T* tw;
   const T* tth;
@@ -424,9 +423,7 @@ __global__ void __launch_bounds__(64) k_acou_wpi_col(Grid g, Params p, Base<T> b
   ds[g.nz] = 0.0;
   for (int k = 0; k < g.nz; ++k) {
     const int c = c0 + k * PL;
-    const double oldpi = a.pip_src[c];
     a.pip_dst[c] = P[k] - E[k] * (Av_at(k + 1) * ds[k + 1] - Av_at(k) * ds[k]);
-    if (a.piprev_dst) a.piprev_dst[c] = oldpi;
     const double wc = 0.5 * (ds[k] + ds[k + 1]);
     a.thp_dst[c] = (T)((double)a.thp_src[c] +
                        dtau * ((double)a.tth[c] - wc * (double)b.dthbdz[c]));
@@ -497,7 +494,7 @@ __global__ void __launch_bounds__(TX * NW) k_acou_wpi_tile(Grid g, Params p, Bas
 
   // Values kept in registers across the phases (storage-type values stay in
   // T; their FP64 conversion is exact).
-  double Pk[KPT], Ek[KPT], Gk[KPT], Rk[KPT], pold[KPT];
+  double Pk[KPT], Ek[KPT], Gk[KPT], Rk[KPT];
   T Avk[KPT], Avk1[KPT], thold[KPT], tthk[KPT], dthk[KPT];
   double w0 = 0.0;
 
@@ -560,7 +557,6 @@ __global__ void __launch_bounds__(TX * NW) k_acou_wpi_tile(Grid g, Params p, Bas
       mfac = 1.0 + kEps61 * qvw;
       mbuoy = kGrav * (kEps61 * dqv - qcr);
     }
-    pold[t] = pk;
     Pk[t] = pk + dtau * tpi -
             dtau * Kt *
                 (Dh - (sv(kc + 1) - sv(kc)) * rdzc + c2 * (avk1 * wkp - avk * wk) * rdzc);
@@ -698,7 +694,6 @@ __global__ void __launch_bounds__(TX * NW) k_acou_wpi_tile(Grid g, Params p, Bas
       const int c = c0 + k * PL;
       const double Wk = sD[k * TX + lx], Wk1 = sD[(k + 1) * TX + lx];
       a.pip_dst[c] = Pk[t] - Ek[t] * ((double)Avk1[t] * Wk1 - (double)Avk[t] * Wk);
-      if (a.piprev_dst) a.piprev_dst[c] = pold[t];
       const double wc = 0.5 * (Wk + Wk1);
       a.thp_dst[c] =
           (T)((double)thold[t] + dtau * ((double)tthk[t] - wc * (double)dthk[t]));
@@ -876,7 +871,7 @@ __global__ void __launch_bounds__(TX * TY * NW) k_acoustic_fused(Grid g, Params 
   auto av = [&](int k) -> double {
     return (k <= 0 || k >= g.nz) ? 0.0 : (double)cf.aw[c0 + k * PL];
   };
-  double pold[KPT], thold[KPT], tthk[KPT], dthk[KPT];
+  double thold[KPT], tthk[KPT], dthk[KPT];
   double w0 = 0.0;
 #pragma unroll
   for (int t = 0; t < KPT; ++t) {
@@ -894,7 +889,6 @@ __global__ void __launch_bounds__(TX * TY * NW) k_acoustic_fused(Grid g, Params 
     const double wkp = (double)a.w_src[c + PL];
     const double tpi = a.tpi ? (double)a.tpi[c] : 0.0;
     const double pk = a.uv.pcur[c];
-    pold[t] = pk;
     sP[k * NC + lc] = pk + dtau * tpi -
                       dtau * Kt *
                           (Dh - (sv(k + 1) - sv(k)) / dzc + c2 * (avk1 * wkp - avk * wk) / dzc);

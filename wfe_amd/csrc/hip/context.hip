@@ -153,12 +153,42 @@ class KernelTimer {
   std::map<std::string, std::pair<long, double>> acc_;
 };
 
-#define TIMED(name, ...)               \
-  do {                                 \
-    timer_.begin(name, stream_);       \
-    __VA_ARGS__;                       \
-    timer_.end(stream_);               \
+// Launch through the per-kernel timer; when a solo measurement of `name`
+// is armed (kernel_bench), the launch is repeated back to back between two
+// events instead.
+#define TIMED(kname_, ...)                                      \
+  do {                                                          \
+    if (solo_.name && std::strcmp(solo_.name, kname_) == 0) {   \
+      solo_.run(stream_, [&] { __VA_ARGS__; });                 \
+    } else {                                                    \
+      timer_.begin(kname_, stream_);                            \
+      __VA_ARGS__;                                              \
+      timer_.end(stream_);                                      \
+    }                                                           \
   } while (0)
+
+// Back-to-back timing of one kernel with its arguments of the moment.
+struct SoloBench {
+  const char* name = nullptr;
+  int reps = 0;
+  double avg_us = -1;
+  template <class F>
+  void run(hipStream_t s, F&& launch) {
+    hipEvent_t e0, e1;
+    WFE_HIP(hipEventCreate(&e0));
+    WFE_HIP(hipEventCreate(&e1));
+    for (int r = 0; r < 3; ++r) launch();
+    WFE_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) launch();
+    WFE_HIP(hipEventRecord(e1, s));
+    WFE_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    WFE_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    avg_us = 1e3 * ms / reps;
+  }
+};
 
 }  // namespace
 
@@ -225,6 +255,7 @@ struct CtxBase {
   virtual size_t device_bytes() const = 0;
   virtual void kernel_timing(int enable) = 0;
   virtual std::string kernel_report() = 0;
+  virtual double kernel_bench(const std::string& name, int reps) = 0;
 };
 
 template <class T>
@@ -253,6 +284,7 @@ class Ctx final : public CtxBase {
     if (enable == 2) timer_.reset();
     timer_.on = enable != 0;
   }
+  double kernel_bench(const std::string& name, int reps) override;
   std::string kernel_report() override {
     WFE_HIP(hipStreamSynchronize(stream_));
     std::string r = timer_.report();
@@ -324,6 +356,7 @@ class Ctx final : public CtxBase {
   template <int TX, int NW, int KPT>
   void add_split_tile(const char* name, size_t cap);
   KernelTimer timer_;
+  SoloBench solo_;
 };
 
 // Reference layout (row length nx + 2ng) <-> device layout (pitch NX, i = 0
@@ -726,7 +759,6 @@ dev::WPArgs<T> split_wp_impl(const dev::FusedArgs<T>& a) {
   w.thp_src = a.uv.thp_cur;
   w.w_dst = a.w_dst;
   w.pip_dst = a.pip_dst;
-  w.piprev_dst = nullptr;  // pi'prev is the previous buffer of the rotation
   w.thp_dst = a.thp_dst;
   w.tw = a.tw;
   w.tth = a.tth;
@@ -963,6 +995,61 @@ void Ctx<T>::kessler_step(const StateSet<T>& s, double dt) {
   check_launch("kessler");
 }
 
+// Average launch time of one kernel of the step, launched back to back
+// with the arguments it has in the first acoustic substep / the first stage
+// of a step. Every kernel it can time writes only work buffers, tendencies,
+// fluxes or ghost cells that the next step recomputes before reading them.
+template <class T>
+double Ctx<T>::kernel_bench(const std::string& name, int reps) {
+  if (reps < 1) throw ArgError("kernel_bench: reps must be positive");
+  WFE_HIP(hipStreamSynchronize(stream_));
+  const StateSet<T>& sn = S_[cur_];
+  solo_ = SoloBench{};
+  solo_.name = name.c_str();
+  solo_.reps = reps;
+  const bool timer_was_on = timer_.on;
+  timer_.on = false;
+  if (name == "acou_uv" || name == "acou_wpi" || name == "acoustic_fused") {
+    if (!tuned_) tune_acoustic(1.0);
+    dev::FusedArgs<T> a{};
+    a.uv.u_src = sn.u;
+    a.uv.v_src = sn.v;
+    a.uv.u_dst = wu_[0];
+    a.uv.v_dst = wv_[0];
+    a.uv.pcur = sn.pip;
+    a.uv.pprev = sn.pip;
+    a.uv.pn = sn.pip;
+    a.uv.thp_cur = sn.thp;
+    a.uv.thp_n = sn.thp;
+    a.uv.qv = sn.qv;
+    a.uv.tu = tu_;
+    a.uv.tv = tv_;
+    a.uv.dtau = (T)1.0;
+    a.w_src = sn.w;
+    a.w_dst = ww_;
+    a.pip_dst = xpip_[0];
+    a.thp_dst = xthp_[0];
+    a.tw = tw_;
+    a.tth = tth_;
+    a.tpi = bdy_active_ ? tpi_ : nullptr;
+    a.qc = sn.qc;
+    a.qr = sn.qr;
+    methods_[method_].run(a);
+  } else if (name == "fill_ghosts") {
+    fill_ghosts(sn);
+  } else if (name == "massflux" || name == "tendencies") {
+    slow_tendencies(sn, 1.0);
+  } else if (name == "stage_q") {
+    stage_moisture(SE_, 1.0);
+  }
+  WFE_HIP(hipStreamSynchronize(stream_));
+  timer_.on = timer_was_on;
+  const double us = solo_.avg_us;
+  solo_ = SoloBench{};
+  if (us < 0) throw ArgError("kernel_bench: '" + name + "' is not launched by this context");
+  return us;
+}
+
 // Integrator::step (integrator.cpp:71-111) with fixed-role buffers: the
 // time-n set S_[cur_], the stage estimate SE_, the next time level
 // S_[cur_^1], and ping-pong work buffers inside the acoustic loop.
@@ -1148,6 +1235,15 @@ int wfe_kessler(wfe_ctx* ctx, double dt) {
 int wfe_kernel_timing(wfe_ctx* ctx, int mode) {
   if (int e = need_ctx(ctx)) return e;
   return guarded([&] { ctx->impl->kernel_timing(mode); });
+}
+
+int wfe_kernel_bench(wfe_ctx* ctx, const char* kernel, int reps, double* avg_us) {
+  if (int e = need_ctx(ctx)) return e;
+  if (!kernel || !avg_us) {
+    wfe::set_last_error("wfe_kernel_bench: null argument");
+    return WFE_E_ARG;
+  }
+  return guarded([&] { *avg_us = ctx->impl->kernel_bench(kernel, reps); });
 }
 
 int wfe_kernel_timing_report(wfe_ctx* ctx, char* buf, size_t n) {
