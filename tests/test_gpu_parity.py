@@ -255,3 +255,65 @@ def test_kernel_bench_leaves_the_trajectory_unchanged():
         om.step(dt)
         gi.step(dt)
     _cmp(c.grid, om, gi, STATE, 1e-10, "after kernel_bench")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_step_graph_replays_match_eager_steps(monkeypatch, precision):
+    """Steps after the first replay a captured hipGraph (one per parity of
+    the state buffers); every replay, with downloads between them, is
+    bit-identical to the eagerly launched step."""
+    c = kitchen_sink()
+    st = perturbed_state(c, 43)
+    dt = c.scalar("dt")
+    runs = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("WFE_GRAPH", mode)
+        monkeypatch.setenv("WFE_ACOUSTIC_KERNEL", "split-32x16")
+        from wfe_amd import Integrator
+        gi = Integrator.from_case(c, precision)
+        for f, a in st.items():
+            gi.set(f, a)
+        snaps = []
+        for s in range(5):
+            gi.step(dt, s * dt)
+            snaps.append({f: gi.get(f) for f in STATE})
+        runs[mode] = snaps
+    for s in range(5):
+        for f in STATE:
+            np.testing.assert_array_equal(runs["1"][s][f], runs["0"][s][f], err_msg=f"{s} {f}")
+
+
+def test_boundary_time_interpolation_and_swap_under_graphs():
+    """The relaxation time fraction changes every step and the target pair
+    swaps mid-run (BdyManager::update, boundary.cpp:83-97): the graph-replayed
+    steps follow both."""
+    c = kitchen_sink()
+    g = c.grid
+    st = perturbed_state(c, 22)
+    om, gi = oracle_and_gpu(c, "fp64", st)
+    rng = np.random.default_rng(6)
+    wgt = np.zeros(g.NX * g.NY)
+    wgt.reshape(g.NY, g.NX)[g.ng:g.ng + g.ny, g.ng:g.ng + g.nx] = rng.uniform(0, 2e-3, (g.ny, g.nx))
+    om.set("bdy_wgt", wgt)
+    from wfe_amd._lib import as_dptr, check, lib
+    check(lib().wfe_set_bdy_weights(gi._h, as_dptr(wgt), wgt.size), "weights")
+    tg = {}
+    for v in range(5):
+        for slot, nm in ((0, "lo"), (1, "hi")):
+            t = np.ascontiguousarray(st[("u", "v", "thp", "pip", "qv")[v]] *
+                                     rng.uniform(0.5, 1.5, g.npts))
+            tg[(nm, v)] = t
+            om.set(f"bdy_{nm}_{v}", t)
+            check(lib().wfe_set_bdy_target(gi._h, slot, v, as_dptr(t), t.size), "target")
+    dt = c.scalar("dt")
+    for s, tf in enumerate((0.1, 0.4, 0.7, 0.2, 0.5)):
+        if s == 3:
+            check(lib().wfe_swap_bdy_targets(gi._h), "swap")
+            for v in range(5):
+                om.set(f"bdy_lo_{v}", tg[("hi", v)])
+                om.set(f"bdy_hi_{v}", tg[("lo", v)])
+        om.set_bdy(True, tf)
+        check(lib().wfe_set_bdy_tfrac(gi._h, tf), "tfrac")
+        om.step(dt)
+        gi.step(dt)
+    _cmp(g, om, gi, STATE, 1e-10, "bdy tfrac + swap, 5 steps")

@@ -263,6 +263,8 @@ class Ctx final : public CtxBase {
  public:
   Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& base, const wfe_metric& met);
   ~Ctx() override {
+    for (hipGraphExec_t& e : gexec_)
+      if (e) (void)hipGraphExecDestroy(e);
     if (stream_) (void)hipStreamDestroy(stream_);
   }
   void step(double dt) override;
@@ -272,8 +274,15 @@ class Ctx final : public CtxBase {
   void sync() override { WFE_HIP(hipStreamSynchronize(stream_)); }
   void set_bdy_weights(const double* w, size_t n) override;
   void set_bdy_target(int slot, int var, const double* h, size_t n) override;
-  void swap_bdy_targets() override { std::swap(bdy_lo_, bdy_hi_); }
-  void set_bdy_tfrac(double tf) override { tfrac_ = tf; }
+  void swap_bdy_targets() override {
+    std::swap(bdy_lo_, bdy_hi_);
+    ++gen_;
+  }
+  void set_bdy_tfrac(double tf) override {
+    tfrac_ = tf;
+    dev::k_set_scalar<T><<<1, 64, 0, stream_>>>(tf_dev_, (T)tf);
+    check_launch("set_bdy_tfrac");
+  }
   void apply_bcs() override { fill_ghosts(S_[cur_]); }
   void compute_tendencies(double dt) override { slow_tendencies(S_[cur_], dt); }
   void acoustic_substeps(double dtau, int nsub) override;
@@ -348,6 +357,16 @@ class Ctx final : public CtxBase {
   std::vector<SubstepMethod> methods_;
   int method_ = -1;
   bool tuned_ = false;
+  // Step graphs, one per parity of cur_ (the two state sets swap roles),
+  // re-captured when dt or anything baked into kernel arguments changes
+  // (generation gen_: boundary buffers, substep method).
+  void step_launches(double dt);
+  bool use_graph_ = true;
+  hipGraphExec_t gexec_[2] = {nullptr, nullptr};
+  double gdt_[2] = {0, 0};
+  unsigned ggen_[2] = {0, 0};
+  unsigned gen_ = 1;
+  T* tf_dev_ = nullptr;
   std::string tune_report_;
   double* xpip_[3]{};  // rotating pi' work buffers
   T* xthp_[2]{};       // ping-pong theta' work buffers
@@ -438,6 +457,7 @@ Ctx<T>::Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& b, const w
   p_.microphysics = hp.microphysics;
 
   WFE_HIP(hipStreamCreate(&stream_));
+  if (const char* e = std::getenv("WFE_GRAPH")) use_graph_ = std::string(e) != "0";
   const size_t n = npts_;
   const size_t hn = hn3_;
   const Shape s3 = Shape::k3D, s2 = Shape::k2D, s1 = Shape::k1D;
@@ -502,6 +522,7 @@ Ctx<T>::Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& b, const w
     coef_.aw = upload_new<T>(aw.data(), s3);
   }
 
+  tf_dev_ = arena_.alloc<T>(1);
   auto alloc_set = [&](StateSet<T>& s) {
     s.u = arena_.alloc<T>(n);
     s.v = arena_.alloc<T>(n);
@@ -640,6 +661,7 @@ void Ctx<T>::set_bdy_weights(const double* w, size_t n) {
   const std::vector<T> tmp = to_dev<T>(w, false);
   WFE_HIP(hipMemcpy(bdy_wgt_, tmp.data(), n2_ * sizeof(T), hipMemcpyHostToDevice));
   bdy_active_ = true;
+  ++gen_;
 }
 
 template <class T>
@@ -741,7 +763,7 @@ void Ctx<T>::bdy_relax(const StateSet<T>& s) {
   R.t[3] = tpi_;
   R.t[4] = tq_[0];
   R.wgt = bdy_wgt_;
-  R.tf = (T)tfrac_;
+  R.tf = tf_dev_;
   dim3 grd(cdiv(g_.nx, 64), cdiv(g_.ny, 4), (unsigned)g_.nz);
   TIMED("bdy_relax", dev::k_bdy_relax<T><<<grd, blk3(), 0, stream_>>>(g_, R));
   check_launch("bdy_relax");
@@ -869,6 +891,7 @@ void Ctx<T>::build_methods(const hipDeviceProp_t& prop) {
   if (methods_.size() == 1) {
     method_ = 0;
     tuned_ = true;
+    ++gen_;
   }
 }
 
@@ -931,6 +954,7 @@ void Ctx<T>::tune_acoustic(double dtau) {
   (void)hipEventDestroy(e1);
   method_ = best_m;
   tuned_ = true;
+  ++gen_;
 }
 
 // One RK3 stage of the fast system from the time-n state S_[cur_] into out.
@@ -1055,6 +1079,45 @@ double Ctx<T>::kernel_bench(const std::string& name, int reps) {
 // S_[cur_^1], and ping-pong work buffers inside the acoustic loop.
 template <class T>
 void Ctx<T>::step(double dt) {
+  // The first step tunes the substep method (synchronous timing), so it and
+  // any step under per-kernel timing run eagerly.
+  if (!use_graph_ || !tuned_ || timer_.on) {
+    step_launches(dt);
+    return;
+  }
+  const int par = cur_;
+  if (!gexec_[par] || gdt_[par] != dt || ggen_[par] != gen_) {
+    if (gexec_[par]) {
+      WFE_HIP(hipGraphExecDestroy(gexec_[par]));
+      gexec_[par] = nullptr;
+    }
+    hipGraph_t gr = nullptr;
+    WFE_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    try {
+      step_launches(dt);
+    } catch (...) {
+      (void)hipStreamEndCapture(stream_, &gr);
+      if (gr) (void)hipGraphDestroy(gr);
+      cur_ = par;
+      throw;
+    }
+    cur_ = par;
+    WFE_HIP(hipStreamEndCapture(stream_, &gr));
+    const hipError_t e = hipGraphInstantiate(&gexec_[par], gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (e != hipSuccess) {
+      gexec_[par] = nullptr;
+      throw HipError(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    gdt_[par] = dt;
+    ggen_[par] = gen_;
+  }
+  WFE_HIP(hipGraphLaunch(gexec_[par], stream_));
+  cur_ ^= 1;
+}
+
+template <class T>
+void Ctx<T>::step_launches(double dt) {
   const double stage_dt[3] = {dt / 3, dt / 2, dt};
   const int ns = std::max(1, hp_.acoustic_ns);
   const int ns_stage[3] = {1, std::max(1, ns / 2), ns};

@@ -450,7 +450,7 @@ struct RelaxSet {
   const T* hi[5];
   T* t[5];  // tu tv tth tpi tqv
   const T* wgt;
-  T tf;
+  const T* tf;  // time fraction between the target pair (device scalar)
 };
 
 template <class T>
@@ -461,7 +461,7 @@ __global__ void k_bdy_relax(Grid g, RelaxSet<T> R) {
   if (i >= g.nx || j >= g.ny || k >= g.nz) return;
   const int c = g.id(i, j, k);
   const T w = R.wgt[g.id2(i, j)];
-  const T tf = R.tf;
+  const T tf = *R.tf;
   if (w <= (T)0) {
     R.t[3][c] = (T)0;
     return;
@@ -472,6 +472,13 @@ __global__ void k_bdy_relax(Grid g, RelaxSet<T> R) {
     R.t[v][c] += w * (((T)1 - tf) * R.lo[v][c] + tf * R.hi[v][c] - R.f[q][c]);
   }
   R.t[3][c] = w * (((T)1 - tf) * R.lo[3][c] + tf * R.hi[3][c] - (T)R.pip[c]);
+}
+
+// One device scalar set in stream order (kept out of kernel arguments so a
+// captured step graph stays valid when the value changes).
+template <class T>
+__global__ void k_set_scalar(T* dst, T v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *dst = v;
 }
 
 // ---------------------------------------------------- moisture stage update
