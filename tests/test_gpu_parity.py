@@ -317,3 +317,21 @@ def test_boundary_time_interpolation_and_swap_under_graphs():
         om.step(dt)
         gi.step(dt)
     _cmp(g, om, gi, STATE, 1e-10, "bdy tfrac + swap, 5 steps")
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", None)])
+def test_regional_file_mode_case(tmp_path, precision, tol):
+    """Synthetic regional case in the reference's file formats: terrain,
+    Lambert map factors, PD moisture, open boundaries with Davies relaxation
+    toward the boundary files."""
+    from wfe_amd import Case
+    from wfe_amd.lam_case import write_lam_case
+    c = Case(write_lam_case(tmp_path, nx=40, ny=30, nz=20, dx=12000.0, top=15000.0, seed=3))
+    om, gi = oracle_and_gpu(c, precision)
+    dt = c.scalar("dt")
+    for s in range(3):
+        om.step(dt, s * dt)
+        gi.step(dt, s * dt)
+    t = tol if tol is not None else {"u": 2e-5, "v": 2e-5, "w": 2e-4, "thp": 2e-5, "pip": 1e-4,
+                                     "qv": 1e-4, "qc": 1e-4, "qr": 1e-4}
+    _cmp(c.grid, om, gi, STATE, t, f"regional case {precision}")
