@@ -174,16 +174,15 @@ __global__ void k_acou_uv(Grid g, Params p, Base<T> b, Metric<T> m, UVArgs<T> a)
 }
 
 // LDS-tiled form of k_acou_uv: a 64 x 4 block of faces at one level stages
-// pi* and theta_v once per cell (block + one-cell west/south halo), instead
-// of recomputing them for every face that touches the cell. Faces that need
-// more than that -- radiation edges, the periodic wrap u(nx) = u(0), and the
-// terrain cross term (pi* at k +- 1) -- go through the direct UVDev path,
-// which evaluates the same expressions.
+// pi* (at k and, over terrain, k +- 1) and theta_v once per cell (block +
+// one-cell west/south halo), instead of recomputing them for every face that
+// touches the cell. The radiation edges and the periodic wrap u(nx) = u(0)
+// go through the direct UVDev path, which evaluates the same expressions.
 template <class T>
 __global__ void __launch_bounds__(256) k_acou_uv_tile(Grid g, Params p, Base<T> b, Metric<T> m,
                                                       UVArgs<T> a) {
   constexpr int BX = 64, BY = 4;
-  __shared__ double sP[BY + 1][BX + 1];
+  __shared__ double sP[3][BY + 1][BX + 1];  // pi* at k-1, k, k+1
   __shared__ T sT[BY + 1][BX + 1];
   const int tx = threadIdx.x, ty = threadIdx.y;
   const BlockId bid = xcd_block();
@@ -215,7 +214,11 @@ __global__ void __launch_bounds__(256) k_acou_uv_tile(Grid g, Params p, Base<T> 
     const int r = e / (BX + 1), q = e % (BX + 1);
     const int ii = i0 - 1 + q, jj = j0 - 1 + r;
     if (ii <= g.nx - 1 && jj <= g.ny - 1) {
-      sP[r][q] = d.pstar(ii, jj, k);
+      sP[1][r][q] = d.pstar(ii, jj, k);
+      if (p.terrain) {
+        sP[0][r][q] = d.pstar(ii, jj, k - 1);
+        sP[2][r][q] = d.pstar(ii, jj, k + 1);
+      }
       sT[r][q] = d.thv(ii, jj, k);
     }
   }
@@ -226,15 +229,21 @@ __global__ void __launch_bounds__(256) k_acou_uv_tile(Grid g, Params p, Base<T> 
   if (j < g.ny) {
     T un;
     if (i < g.nx && !(p.bcx && i == 0)) {
-      // terrain cross term only where the slope is non-zero (zeta_c < zt)
+      const T thbu = (T)0.5 * (sT[ty + 1][tx] + sT[ty + 1][tx + 1]);
+      const T mu = (T)0.5 * (mapw + mapc);
+      double grad = (sP[1][ty + 1][tx + 1] - sP[1][ty + 1][tx]) * g.rdx;
+      // terrain cross term -(z_x / J) dpi*/dzeta where the slope is non-zero
       if (hxu != (T)0) {
-        un = d.u_step(i, j, k);
-      } else {
-        const T thbu = (T)0.5 * (sT[ty + 1][tx] + sT[ty + 1][tx + 1]);
-        const T mu = (T)0.5 * (mapw + mapc);
-        const double grad = (sP[ty + 1][tx + 1] - sP[ty + 1][tx]) * g.rdx;
-        un = usrc + dtau * (-(T)kCp * thbu * mu * (T)grad + tus);
+        const T zxu = hxu * ((T)1 - m.zeta_c[kk] / m.zt);
+        const T dzc2 = m.zeta_c[kk + 1] - m.zeta_c[kk - 1];
+        const T ju = (T)0.5 * (m.jac[c2 - 1] + m.jac[c2]);
+        const double dpdz = 0.5 *
+                            ((sP[2][ty + 1][tx] - sP[0][ty + 1][tx]) +
+                             (sP[2][ty + 1][tx + 1] - sP[0][ty + 1][tx + 1])) /
+                            (double)dzc2;
+        grad -= (double)(zxu / ju) * dpdz;
       }
+      un = usrc + dtau * (-(T)kCp * thbu * mu * (T)grad + tus);
     } else if (!p.bcx) {
       un = d.u_step(0, j, k);
     } else if (i == 0) {
@@ -259,14 +268,20 @@ __global__ void __launch_bounds__(256) k_acou_uv_tile(Grid g, Params p, Base<T> 
   if (i < g.nx) {
     T vn;
     if (j < g.ny && !(p.bcy && j == 0)) {
+      const T thbv = (T)0.5 * (sT[ty][tx + 1] + sT[ty + 1][tx + 1]);
+      const T mv = (T)0.5 * (maps + mapc);
+      double grad = (sP[1][ty + 1][tx + 1] - sP[1][ty][tx + 1]) * g.rdy;
       if (hyv != (T)0) {
-        vn = d.v_step(i, j, k);
-      } else {
-        const T thbv = (T)0.5 * (sT[ty][tx + 1] + sT[ty + 1][tx + 1]);
-        const T mv = (T)0.5 * (maps + mapc);
-        const double grad = (sP[ty + 1][tx + 1] - sP[ty][tx + 1]) * g.rdy;
-        vn = vsrc + dtau * (-(T)kCp * thbv * mv * (T)grad + tvs);
+        const T zyv = hyv * ((T)1 - m.zeta_c[kk] / m.zt);
+        const T dzc2 = m.zeta_c[kk + 1] - m.zeta_c[kk - 1];
+        const T jv = (T)0.5 * (m.jac[c2 - g.NX] + m.jac[c2]);
+        const double dpdz = 0.5 *
+                            ((sP[2][ty][tx + 1] - sP[0][ty][tx + 1]) +
+                             (sP[2][ty + 1][tx + 1] - sP[0][ty + 1][tx + 1])) /
+                            (double)dzc2;
+        grad -= (double)(zyv / jv) * dpdz;
       }
+      vn = vsrc + dtau * (-(T)kCp * thbv * mv * (T)grad + tvs);
     } else if (!p.bcy) {
       vn = d.v_step(i, 0, k);
     } else if (j == 0) {

@@ -485,6 +485,9 @@ Ctx<T>::Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& b, const w
   met_.fcor = upload_new<T>(m.fcor, s2);
   met_.mapf = upload_new<T>(m.mapf, s2);
   met_.zt = (T)m.zt;
+  p_.terrain = 0;
+  for (size_t q = 0; q < hn2_; ++q)
+    if (m.hx[q] != 0 || m.hy[q] != 0 || m.hx_u[q] != 0 || m.hy_v[q] != 0) p_.terrain = 1;
   met_.rzt = 1.0 / m.zt;
   {
     std::vector<double> rc(g_.NZ), rw(g_.NZ);

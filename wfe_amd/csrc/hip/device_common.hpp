@@ -65,6 +65,7 @@ struct Params {
   int bcx, bcy;  // 1 = open
   double cstar;
   int moisture, pd_moist, w_damping, microphysics;
+  int terrain;  // 1 when any terrain slope h_x, h_y is non-zero
 };
 
 // Base state (DevProf, src/core/base_state.hpp:16-27). pib stays FP64 so
