@@ -195,7 +195,7 @@ def test_deep_column(monkeypatch, method):
 FUSED_SHAPES = ["fused-8x2x16", "fused-16x1x16", "fused-4x4x16", "fused-8x4x8", "fused-32x1x8",
                 "fused-8x2x24"]
 SPLIT_SHAPES = ["split-32x8", "split-32x16", "split-64x8", "split-64x16", "split-16x16",
-                "split-32x16x4", "split-col"]
+                "split-32x16x4", "split-32x16w4", "split-16x16w4", "split-col"]
 
 
 @pytest.mark.parametrize("shape", FUSED_SHAPES + SPLIT_SHAPES)

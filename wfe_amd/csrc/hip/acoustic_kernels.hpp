@@ -477,10 +477,13 @@ __device__ __forceinline__ double rcp_nr(double x) {
   return fma(r, e, r);
 }
 
-template <class T, int TX, int NW, int KPT, bool MOIST, bool TPI>
-__global__ void __launch_bounds__(TX * NW) k_acou_wpi_tile(Grid g, Params p, Base<T> b,
-                                                           Metric<T> m, Coef<T> cf,
-                                                           WPArgs<T> a) {
+// WPE: minimum waves per SIMD the register allocation must allow (1 = no
+// constraint); WPE = 4 caps VGPRs at 128 so two 512-thread blocks share a CU
+// and one block's serial elimination overlaps the other's loads.
+template <class T, int TX, int NW, int KPT, bool MOIST, bool TPI, int WPE = 1>
+__global__ void __launch_bounds__(TX * NW, WPE) k_acou_wpi_tile(Grid g, Params p, Base<T> b,
+                                                                Metric<T> m, Coef<T> cf,
+                                                                WPArgs<T> a) {
   extern __shared__ double lds[];
   const int L = g.nz + 1;
   double* sD = lds;          // P_k, then D_k -> ds_k -> W_k (k in [0, nz])

@@ -340,7 +340,7 @@ class Ctx final : public CtxBase {
   T *tu_ = nullptr, *tv_ = nullptr, *tw_ = nullptr, *tth_ = nullptr, *tpi_ = nullptr;
   T* tq_[3]{};
   T *mfx_ = nullptr, *mfy_ = nullptr, *mfz_ = nullptr, *div_ = nullptr;
-  T *pfx_[3]{}, *pfy_[3]{}, *pfz_[3]{}, *pratio_[3]{};
+  T* pratio_[3]{};
   T* rain_ = nullptr;
   bool bdy_active_ = false;
   T *bdy_lo_[5]{}, *bdy_hi_[5]{};
@@ -372,7 +372,7 @@ class Ctx final : public CtxBase {
   T* xthp_[2]{};       // ping-pong theta' work buffers
   void build_methods(const hipDeviceProp_t& prop);
   void tune_acoustic(double dtau);
-  template <int TX, int NW, int KPT>
+  template <int TX, int NW, int KPT, int WPE = 1>
   void add_split_tile(const char* name, size_t cap);
   KernelTimer timer_;
   SoloBench solo_;
@@ -497,6 +497,20 @@ Ctx<T>::Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& b, const w
     }
     met_.rdzc = upload_new<double>(rc.data(), s1);
     met_.rdzw = upload_new<double>(rw.data(), s1);
+    // sin^2 Rayleigh layer above rayleigh_zd (ray_alpha, kernels.cu:42-48)
+    std::vector<double> ac(g_.NZ), aw(g_.NZ);
+    const double zd = hp.rayleigh_zd, zt = m.zt;
+    auto ray = [&](double z) {
+      if (zd < 0 || z <= zd || zt <= zd) return 0.0;
+      const double sn = std::sin(1.5707963267948966 * (z - zd) / (zt - zd));
+      return hp.rayleigh_alpha * sn * sn;
+    };
+    for (int k = 0; k < g_.NZ; ++k) {
+      ac[k] = ray(m.zeta_c[k]);
+      aw[k] = ray(m.zeta_w[k]);
+    }
+    met_.ray_c = upload_new<T>(ac.data(), s1);
+    met_.ray_w = upload_new<T>(aw.data(), s1);
   }
 
   // Acoustic coefficients, formed in FP64 on the host exactly as
@@ -558,9 +572,6 @@ Ctx<T>::Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& b, const w
   div_ = arena_.alloc<T>(n);
   if (p_.pd_moist)
     for (int q = 0; q < 3; ++q) {
-      pfx_[q] = arena_.alloc<T>(n);
-      pfy_[q] = arena_.alloc<T>(n);
-      pfz_[q] = arena_.alloc<T>(n);
       pratio_[q] = arena_.alloc<T>(n);
     }
   rain_ = arena_.alloc<T>(n2_);
@@ -733,17 +744,14 @@ void Ctx<T>::slow_tendencies(const StateSet<T>& s, double dt) {
     const T* q[3] = {s.qv, s.qc, s.qr};
     for (int q3 = 0; q3 < 3; ++q3) {
       S.q[q3] = q[q3];
-      S.fx[q3] = pfx_[q3];
-      S.fy[q3] = pfy_[q3];
-      S.fz[q3] = pfz_[q3];
       S.ratio[q3] = pratio_[q3];
       S.tq[q3] = tq_[q3];
     }
-    dim3 gf(cdiv(g_.nx + 1, 64), cdiv(g_.ny + 1, 4), (unsigned)(g_.nz + 1));
-    TIMED("pd_flux", dev::k_pd_flux<T><<<gf, blk3(), 0, stream_>>>(g_, mfx_, mfy_, mfz_, S));
     dim3 gi(cdiv(g_.nx, 64), cdiv(g_.ny, 4), (unsigned)g_.nz);
-    TIMED("pd_ratio", dev::k_pd_ratio<T><<<gi, blk3(), 0, stream_>>>(g_, base_, met_, (T)dt, S));
-    TIMED("pd_apply", dev::k_pd_apply<T><<<gi, blk3(), 0, stream_>>>(g_, p_, base_, met_, div_, S));
+    TIMED("pd_ratio", dev::k_pd_ratio<T><<<gi, blk3(), 0, stream_>>>(g_, base_, met_, (T)dt, mfx_,
+                                                                    mfy_, mfz_, S));
+    TIMED("pd_apply", dev::k_pd_apply<T><<<gi, blk3(), 0, stream_>>>(g_, p_, base_, met_, mfx_,
+                                                                    mfy_, mfz_, div_, S));
   }
   check_launch("slow_tendencies");
 }
@@ -796,16 +804,16 @@ dev::WPArgs<T> split_wp_impl(const dev::FusedArgs<T>& a) {
 }
 
 template <class T>
-template <int TX, int NW, int KPT>
+template <int TX, int NW, int KPT, int WPE>
 void Ctx<T>::add_split_tile(const char* name, size_t cap) {
   using Tile = dev::WpiTile<TX, NW, KPT>;
   if (g_.nz > NW * KPT) return;
   const size_t lb = Tile::lds_bytes(g_.nz);
   // Instantiations by (moisture, pi' tendency present).
-  const void* fns[4] = {reinterpret_cast<const void*>(&dev::k_acou_wpi_tile<T, TX, NW, KPT, false, false>),
-                        reinterpret_cast<const void*>(&dev::k_acou_wpi_tile<T, TX, NW, KPT, false, true>),
-                        reinterpret_cast<const void*>(&dev::k_acou_wpi_tile<T, TX, NW, KPT, true, false>),
-                        reinterpret_cast<const void*>(&dev::k_acou_wpi_tile<T, TX, NW, KPT, true, true>)};
+  const void* fns[4] = {reinterpret_cast<const void*>(&dev::k_acou_wpi_tile<T, TX, NW, KPT, false, false, WPE>),
+                        reinterpret_cast<const void*>(&dev::k_acou_wpi_tile<T, TX, NW, KPT, false, true, WPE>),
+                        reinterpret_cast<const void*>(&dev::k_acou_wpi_tile<T, TX, NW, KPT, true, false, WPE>),
+                        reinterpret_cast<const void*>(&dev::k_acou_wpi_tile<T, TX, NW, KPT, true, true, WPE>)};
   if (lb > 65536) {
     if (lb > cap) return;
     for (const void* f : fns)
@@ -823,16 +831,16 @@ void Ctx<T>::add_split_tile(const char* name, size_t cap) {
                         const dim3 bt(TX, NW);
                         const bool moist = p_.moisture, tpi = w.tpi != nullptr;
                         if (moist && tpi)
-                          TIMED("acou_wpi", (dev::k_acou_wpi_tile<T, TX, NW, KPT, true, true>
+                          TIMED("acou_wpi", (dev::k_acou_wpi_tile<T, TX, NW, KPT, true, true, WPE>
                                              <<<gt, bt, lb, stream_>>>(g_, p_, base_, met_, coef_, w)));
                         else if (moist)
-                          TIMED("acou_wpi", (dev::k_acou_wpi_tile<T, TX, NW, KPT, true, false>
+                          TIMED("acou_wpi", (dev::k_acou_wpi_tile<T, TX, NW, KPT, true, false, WPE>
                                              <<<gt, bt, lb, stream_>>>(g_, p_, base_, met_, coef_, w)));
                         else if (tpi)
-                          TIMED("acou_wpi", (dev::k_acou_wpi_tile<T, TX, NW, KPT, false, true>
+                          TIMED("acou_wpi", (dev::k_acou_wpi_tile<T, TX, NW, KPT, false, true, WPE>
                                              <<<gt, bt, lb, stream_>>>(g_, p_, base_, met_, coef_, w)));
                         else
-                          TIMED("acou_wpi", (dev::k_acou_wpi_tile<T, TX, NW, KPT, false, false>
+                          TIMED("acou_wpi", (dev::k_acou_wpi_tile<T, TX, NW, KPT, false, false, WPE>
                                              <<<gt, bt, lb, stream_>>>(g_, p_, base_, met_, coef_, w)));
                       }});
 }
@@ -862,6 +870,8 @@ void Ctx<T>::build_methods(const hipDeviceProp_t& prop) {
   add_split_tile<64, 16, 3>("split-64x16", cap);
   add_split_tile<16, 16, 3>("split-16x16", cap);
   add_split_tile<32, 16, 4>("split-32x16x4", cap);  // columns up to 64 levels
+  add_split_tile<32, 16, 3, 4>("split-32x16w4", cap);   // two blocks per CU
+  add_split_tile<16, 16, 3, 4>("split-16x16w4", cap);
   for (const auto& v : fused_variants<T>()) {
     if (g_.nz > v.nw * v.kpt) continue;
     const size_t lb = v.lds(g_.nz);

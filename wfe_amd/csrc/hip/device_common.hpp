@@ -86,6 +86,9 @@ struct Metric {
   // (index k + ng) and 1/zt.
   const double *rdzc, *rdzw;
   double rzt;
+  // sin^2 Rayleigh damping rate per level at cell centres and w levels
+  // (ray_alpha, kernels.cu:42-48), formed on the host in FP64.
+  const T *ray_c, *ray_w;
 };
 
 // Constant acoustic coefficients (AcousticCoef, src/dynamics/kernels.hpp:46-50).
@@ -103,17 +106,6 @@ __device__ __forceinline__ T iface5(T qm2, T qm1, T q0, T q1, T q2, T q3, T vel)
   const T cen = c * ((T)37 * (q0 + q1) - (T)8 * (qm1 + q2) + (qm2 + q3));
   const T upw = c * ((q3 - qm2) - (T)5 * (q2 - qm1) + (T)10 * (q1 - q0));
   return (vel >= (T)0) ? (cen - upw) : (cen + upw);
-}
-
-// sin^2 Rayleigh damping rate above rayleigh_zd (kernels.cu:42-48).
-template <class T>
-__device__ __forceinline__ T ray_alpha(T zeta, T zt, const Params& p) {
-  const T zd = (T)p.rayleigh_zd;
-  if (zd < (T)0) return (T)0;
-  if (zeta <= zd || zt <= zd) return (T)0;
-  const T halfpi = (T)1.5707963267948966;
-  const T s = sin(halfpi * (zeta - zd) / (zt - zd));
-  return (T)p.rayleigh_alpha * s * s;
 }
 
 // Lateral ghost index maps, composed as the reference's fill order leaves

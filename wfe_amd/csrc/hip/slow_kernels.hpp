@@ -266,7 +266,7 @@ __global__ void k_tendencies(Grid g, Params p, Base<T> b, Metric<T> m, T dt,
     const T vavg = (T)0.25 * (v[c - 1] + v[c - 1 + NX] + v[c] + v[c + NX]);
     const T fc = (T)0.5 * (m.fcor[c2 - 1] + m.fcor[c2]);
     const T vref = p.coriolis_use_ub ? b.vb[kk] : (T)0;
-    const T relax = -ray_alpha(m.zeta_c[kk], m.zt, p) * (u[c] - b.ub[kk]);
+    const T relax = -m.ray_c[kk] * (u[c] - b.ub[kk]);
     T t = (-fdiv + u[c] * dv) / ru + fc * (vavg - vref) + relax;
     if (K > (T)0) t += K * laplacian(g, u, c, dzc * m.jac[c2]);
     static_cast<T*>(out.tu)[c] = t;
@@ -291,7 +291,7 @@ __global__ void k_tendencies(Grid g, Params p, Base<T> b, Metric<T> m, T dt,
     const T uavg = (T)0.25 * (u[c - NX] + u[c + 1 - NX] + u[c] + u[c + 1]);
     const T fc = (T)0.5 * (m.fcor[c2 - NX] + m.fcor[c2]);
     const T uref = p.coriolis_use_ub ? b.ub[kk] : (T)0;
-    const T relax = -ray_alpha(m.zeta_c[kk], m.zt, p) * (v[c] - b.vb[kk]);
+    const T relax = -m.ray_c[kk] * (v[c] - b.vb[kk]);
     T t = (-fdiv + v[c] * dv) / rv - fc * (uavg - uref) + relax;
     if (K > (T)0) t += K * laplacian(g, v, c, dzc * m.jac[c2]);
     static_cast<T*>(out.tv)[c] = t;
@@ -314,7 +314,7 @@ __global__ void k_tendencies(Grid g, Params p, Base<T> b, Metric<T> m, T dt,
     const T fdiv = (FxR - FxL) / dx + (FyT - FyB) / dy + (FzT - FzB) / dzw;
     const T dv = (T)0.5 * (div[c - PL] + div[c]);
     const T rw = b.rhobw[c] * m.jac[c2];
-    T relax = -ray_alpha(m.zeta_w[kk], m.zt, p) * w[c];
+    T relax = -m.ray_w[kk] * w[c];
     const T wk = w[c];
     if (p.w_damping) {
       const T cr = fabs(wk) * dt / (dzw * m.jac[c2]);
@@ -327,7 +327,7 @@ __global__ void k_tendencies(Grid g, Params p, Base<T> b, Metric<T> m, T dt,
   // ---- theta' (k_tend_thp, kernels.cu:114-146)
   {
     T t = scalar_adv(g, thp, mfx, mfy, mfz, div, c, dzc, rt) -
-          ray_alpha(m.zeta_c[kk], m.zt, p) * thp[c];
+          m.ray_c[kk] * thp[c];
     if (K > (T)0) t += K * laplacian(g, thp, c, dzc * m.jac[c2]);
     static_cast<T*>(out.tth)[c] = t;
   }
@@ -340,75 +340,77 @@ __global__ void k_tendencies(Grid g, Params p, Base<T> b, Metric<T> m, T dt,
 }
 
 // ------------------------------------------ positive-definite moisture (PD)
-// Skamarock (2006) flux renormalisation (kernels.cu:187-264), the three
-// species in one launch each. F*[c] is the flux through the west / south /
-// bottom face of cell c.
+// Skamarock (2006) flux renormalisation (kernels.cu:187-264): two launches
+// for the three species (limiter ratio, then the renormalised divergence).
 template <class T>
 struct PDSet {
   const T* q[3];
-  T* fx[3];
-  T* fy[3];
-  T* fz[3];
   T* ratio[3];
   T* tq[3];
 };
 
+// Flux of species q through the west / south / bottom face of cell c.
 template <class T>
-__global__ void k_pd_flux(Grid g, const T* __restrict__ mfx, const T* __restrict__ mfy,
-                          const T* __restrict__ mfz, PDSet<T> S) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = blockIdx.z;
-  if (i > g.nx || j > g.ny || k > g.nz) return;
-  const int c = g.id(i, j, k);
-  const bool ix = j < g.ny && k < g.nz;              // faces i = 0..nx
-  const bool iy = i < g.nx && k < g.nz;              // faces j = 0..ny
-  const bool iz = i < g.nx && j < g.ny;              // faces k = 0..nz
-  for (int s = 0; s < 3; ++s) {
-    const T* q = S.q[s];
-    if (ix) S.fx[s][c] = face_flux(q, c, 1, mfx[c]);
-    if (iy) S.fy[s][c] = face_flux(q, c, g.NX, mfy[c]);
-    if (iz) S.fz[s][c] = face_flux(q, c, g.plane, mfz[c]);
+struct PDFlux {
+  const Grid& g;
+  const T* __restrict__ mfx;
+  const T* __restrict__ mfy;
+  const T* __restrict__ mfz;
+  __device__ __forceinline__ T fx(const T* __restrict__ q, int c) const {
+    return face_flux(q, c, 1, mfx[c]);
   }
-}
+  __device__ __forceinline__ T fy(const T* __restrict__ q, int c) const {
+    return face_flux(q, c, g.NX, mfy[c]);
+  }
+  __device__ __forceinline__ T fz(const T* __restrict__ q, int c) const {
+    return face_flux(q, c, g.plane, mfz[c]);
+  }
+};
 
+// Limiter ratio of each species at interior cells (k_pd_ratio
+// kernels.cu:221-238); the six face fluxes (k_qflux_x/y/z :187-218) are
+// formed here and again in k_pd_apply instead of being stored.
 template <class T>
-__global__ void k_pd_ratio(Grid g, Base<T> b, Metric<T> m, T dt, PDSet<T> S) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = blockIdx.z;
+__global__ void k_pd_ratio(Grid g, Base<T> b, Metric<T> m, T dt, const T* __restrict__ mfx,
+                           const T* __restrict__ mfy, const T* __restrict__ mfz, PDSet<T> S) {
+  const BlockId bid = xcd_block();
+  const int i = bid.x * blockDim.x + threadIdx.x;
+  const int j = bid.y * blockDim.y + threadIdx.y;
+  const int k = bid.z;
   if (i >= g.nx || j >= g.ny || k >= g.nz) return;
   const int c = g.id(i, j, k);
   const T dzc = m.dzeta_c[k + g.ng];
   const T dx = (T)g.dx, dy = (T)g.dy;
   const T rt = b.rhob[c] * m.jac[g.id2(i, j)];
+  const PDFlux<T> F{g, mfx, mfy, mfz};
   for (int s = 0; s < 3; ++s) {
-    const T* Fx = S.fx[s];
-    const T* Fy = S.fy[s];
-    const T* Fz = S.fz[s];
-    const T out = fmax((T)0, Fx[c + 1]) / dx + fmax((T)0, -Fx[c]) / dx +
-                  fmax((T)0, Fy[c + g.NX]) / dy + fmax((T)0, -Fy[c]) / dy +
-                  fmax((T)0, Fz[c + g.plane]) / dzc + fmax((T)0, -Fz[c]) / dzc;
-    const T avail = rt * fmax((T)0, S.q[s][c]);
+    const T* q = S.q[s];
+    const T out = fmax((T)0, F.fx(q, c + 1)) / dx + fmax((T)0, -F.fx(q, c)) / dx +
+                  fmax((T)0, F.fy(q, c + g.NX)) / dy + fmax((T)0, -F.fy(q, c)) / dy +
+                  fmax((T)0, F.fz(q, c + g.plane)) / dzc + fmax((T)0, -F.fz(q, c)) / dzc;
+    const T avail = rt * fmax((T)0, q[c]);
     const T r = (dt * out > (T)1e-20) ? avail / (dt * out) : (T)1;
     S.ratio[s][c] = fmin((T)1, r);
   }
 }
 
-// Renormalised flux divergence + consistency term. Ratio reads outside the
-// interior follow the reference's lateral fill of the ratio field
-// (kernels.cu:865-866); vertically they only multiply the zero fluxes at
-// k = 0 and nz.
+// Renormalised flux divergence + consistency term (k_pd_apply
+// kernels.cu:241-264). Ratio reads outside the interior follow the
+// reference's lateral fill of the ratio field (kernels.cu:865-866);
+// vertically they only multiply the zero fluxes at k = 0 and nz.
 template <class T>
-__global__ void k_pd_apply(Grid g, Params p, Base<T> b, Metric<T> m,
+__global__ void k_pd_apply(Grid g, Params p, Base<T> b, Metric<T> m, const T* __restrict__ mfx,
+                           const T* __restrict__ mfy, const T* __restrict__ mfz,
                            const T* __restrict__ mfdiv, PDSet<T> S) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = blockIdx.z;
+  const BlockId bid = xcd_block();
+  const int i = bid.x * blockDim.x + threadIdx.x;
+  const int j = bid.y * blockDim.y + threadIdx.y;
+  const int k = bid.z;
   if (i >= g.nx || j >= g.ny || k >= g.nz) return;
   const int c = g.id(i, j, k);
   const T dzc = m.dzeta_c[k + g.ng];
   const T rt = b.rhob[c] * m.jac[g.id2(i, j)];
+  const PDFlux<T> F{g, mfx, mfy, mfz};
   auto ridx = [&](int ii, int jj, int kz) {
     ii = map_lat(ii, g.nx, g.nx - 1, p.bcx);
     jj = map_lat(jj, g.ny, g.ny - 1, p.bcy);
@@ -416,25 +418,23 @@ __global__ void k_pd_apply(Grid g, Params p, Base<T> b, Metric<T> m,
     return g.id(ii, jj, kz);
   };
   for (int s = 0; s < 3; ++s) {
-    const T* Fx = S.fx[s];
-    const T* Fy = S.fy[s];
-    const T* Fz = S.fz[s];
+    const T* q = S.q[s];
     const T* R = S.ratio[s];
     auto sx = [&](int i0) {
-      const T f = Fx[g.id(i0, j, k)];
+      const T f = F.fx(q, g.id(i0, j, k));
       return f * R[ridx(f >= (T)0 ? i0 - 1 : i0, j, k)];
     };
     auto sy = [&](int j0) {
-      const T f = Fy[g.id(i, j0, k)];
+      const T f = F.fy(q, g.id(i, j0, k));
       return f * R[ridx(i, f >= (T)0 ? j0 - 1 : j0, k)];
     };
     auto sz = [&](int k0) {
-      const T f = Fz[g.id(i, j, k0)];
+      const T f = F.fz(q, g.id(i, j, k0));
       return f * R[ridx(i, j, f >= (T)0 ? k0 - 1 : k0)];
     };
     const T fdiv = (sx(i + 1) - sx(i)) / (T)g.dx + (sy(j + 1) - sy(j)) / (T)g.dy +
                    (sz(k + 1) - sz(k)) / dzc;
-    S.tq[s][c] = (-fdiv + S.q[s][c] * mfdiv[c]) / rt;
+    S.tq[s][c] = (-fdiv + q[c] * mfdiv[c]) / rt;
   }
 }
 
