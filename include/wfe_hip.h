@@ -111,6 +111,22 @@ int wfe_destroy(wfe_ctx* ctx);
  * wfe_set_bdy_tfrac). Asynchronous with respect to the host. */
 int wfe_step(wfe_ctx* ctx, double dt, double t);
 
+/* ---- ensembles: members of any grid size advancing together ----
+ * No counterpart in the reference (which runs one case per process). An
+ * ensemble borrows n existing contexts (each from wfe_create or
+ * wfe_case_create_ctx; all must be at the same step parity, i.e. created
+ * together or stepped the same number of times) and advances all of them by
+ * one large step per wfe_ensemble_step, member q with dt[q]: one hipGraph
+ * holds every member's step as an independent branch, one launch per step.
+ * The first call runs each member's first (tuning) step eagerly. Work issued
+ * afterwards on a member context (download, absmax, its own wfe_step) is
+ * ordered after the ensemble step. Destroy the ensemble before its members. */
+typedef struct wfe_ensemble wfe_ensemble;
+int wfe_ensemble_create(wfe_ctx* const* members, int n, wfe_ensemble** out);
+int wfe_ensemble_destroy(wfe_ensemble* e);
+int wfe_ensemble_step(wfe_ensemble* e, const double* dt);
+int wfe_ensemble_sync(wfe_ensemble* e);
+
 /* Field3D::upload / download (src/core/field3d.hpp:42-47) of the time-n
  * state (Integrator::state(), integrator.hpp:24), of the slow tendencies, or
  * of the 2-D rain (Integrator::rain(), integrator.hpp:25). n is the element

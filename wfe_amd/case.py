@@ -217,3 +217,27 @@ class Integrator:
 
     def kessler(self, dt: float) -> None:
         check(lib().wfe_kessler(self._h, dt), "wfe_kessler")
+
+
+class Ensemble:
+    """Members of any grid size advancing together, one hipGraph launch per
+    step (wfe_ensemble_*). Keeps references to its members."""
+
+    def __init__(self, members: list[Integrator]):
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m._h.value for m in self.members])
+        h = C.c_void_p()
+        check(lib().wfe_ensemble_create(arr, len(self.members), C.byref(h)), "wfe_ensemble_create")
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().wfe_ensemble_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def step(self, dts) -> None:
+        arr = (C.c_double * len(self.members))(*[float(d) for d in dts])
+        check(lib().wfe_ensemble_step(self._h, arr), "wfe_ensemble_step")
+
+    def sync(self) -> None:
+        check(lib().wfe_ensemble_sync(self._h), "wfe_ensemble_sync")

@@ -256,6 +256,16 @@ struct CtxBase {
   virtual void kernel_timing(int enable) = 0;
   virtual std::string kernel_report() = 0;
   virtual double kernel_bench(const std::string& name, int reps) = 0;
+  // Ensemble support: the context's stream, whether its first (tuning) step
+  // has run, the parity of its state buffers, the generation of everything
+  // baked into its kernel arguments, and the step's launches enqueued on
+  // its stream without changing the parity (for graph capture).
+  virtual hipStream_t stream() const = 0;
+  virtual bool ready_for_capture() const = 0;
+  virtual int parity() const = 0;
+  virtual void flip_parity() = 0;
+  virtual unsigned generation() const = 0;
+  virtual void enqueue_step(double dt) = 0;
 };
 
 template <class T>
@@ -294,6 +304,16 @@ class Ctx final : public CtxBase {
     timer_.on = enable != 0;
   }
   double kernel_bench(const std::string& name, int reps) override;
+  hipStream_t stream() const override { return stream_; }
+  bool ready_for_capture() const override { return tuned_ && !timer_.on; }
+  int parity() const override { return cur_; }
+  void flip_parity() override { cur_ ^= 1; }
+  unsigned generation() const override { return gen_; }
+  void enqueue_step(double dt) override {
+    const int par = cur_;
+    step_launches(dt);
+    cur_ = par;
+  }
   std::string kernel_report() override {
     WFE_HIP(hipStreamSynchronize(stream_));
     std::string r = timer_.report();
@@ -1240,6 +1260,146 @@ int wfe_destroy(wfe_ctx* ctx) {
 int wfe_step(wfe_ctx* ctx, double dt, double /*t*/) {
   if (int e = need_ctx(ctx)) return e;
   return guarded([&] { ctx->impl->step(dt); });
+}
+
+// ------------------------------------------------------------ ensembles
+// Members advance together: one hipGraph per step holds every member's step
+// as an independent branch (captured fork/join over the members' streams),
+// so the whole ensemble is one launch per step and members' kernels overlap
+// on the device. One graph per parity of the members' state buffers,
+// re-captured when a time step or a member's generation changes.
+struct wfe_ensemble {
+  std::vector<wfe_ctx*> m;
+  hipStream_t s = nullptr;
+  hipEvent_t fork = nullptr, done = nullptr;
+  std::vector<hipEvent_t> join;
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  std::vector<double> gdt[2];
+  std::vector<unsigned> ggen[2];
+  ~wfe_ensemble() {
+    for (hipGraphExec_t& e : exec)
+      if (e) (void)hipGraphExecDestroy(e);
+    for (hipEvent_t e : join) (void)hipEventDestroy(e);
+    if (fork) (void)hipEventDestroy(fork);
+    if (done) (void)hipEventDestroy(done);
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
+
+namespace {
+
+void ensemble_step(wfe_ensemble& E, const double* dt) {
+  const size_t n = E.m.size();
+  // Order the ensemble stream after each member's own pending work.
+  for (size_t q = 0; q < n; ++q) {
+    WFE_HIP(hipEventRecord(E.join[q], E.m[q]->impl->stream()));
+    WFE_HIP(hipStreamWaitEvent(E.s, E.join[q], 0));
+  }
+  bool ready = true;
+  for (wfe_ctx* c : E.m) ready = ready && c->impl->ready_for_capture();
+  const int par = E.m[0]->impl->parity();
+  for (wfe_ctx* c : E.m)
+    if (c->impl->parity() != par) throw wfe::ArgError("ensemble members out of step");
+  if (!ready) {
+    // First step of a member tunes it (synchronous timing): run eagerly.
+    for (size_t q = 0; q < n; ++q) E.m[q]->impl->step(dt[q]);
+    return;
+  }
+  std::vector<unsigned> gen(n);
+  for (size_t q = 0; q < n; ++q) gen[q] = E.m[q]->impl->generation();
+  const std::vector<double> dts(dt, dt + n);
+  if (!E.exec[par] || E.gdt[par] != dts || E.ggen[par] != gen) {
+    if (E.exec[par]) {
+      WFE_HIP(hipGraphExecDestroy(E.exec[par]));
+      E.exec[par] = nullptr;
+    }
+    hipGraph_t gr = nullptr;
+    WFE_HIP(hipStreamBeginCapture(E.s, hipStreamCaptureModeThreadLocal));
+    try {
+      WFE_HIP(hipEventRecord(E.fork, E.s));
+      for (size_t q = 0; q < n; ++q) {
+        hipStream_t ms = E.m[q]->impl->stream();
+        WFE_HIP(hipStreamWaitEvent(ms, E.fork, 0));
+        E.m[q]->impl->enqueue_step(dt[q]);
+        WFE_HIP(hipEventRecord(E.join[q], ms));
+        WFE_HIP(hipStreamWaitEvent(E.s, E.join[q], 0));
+      }
+    } catch (...) {
+      (void)hipStreamEndCapture(E.s, &gr);
+      if (gr) (void)hipGraphDestroy(gr);
+      throw;
+    }
+    WFE_HIP(hipStreamEndCapture(E.s, &gr));
+    const hipError_t e = hipGraphInstantiate(&E.exec[par], gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (e != hipSuccess) {
+      E.exec[par] = nullptr;
+      throw wfe::HipError(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    E.gdt[par] = dts;
+    E.ggen[par] = gen;
+  }
+  WFE_HIP(hipGraphLaunch(E.exec[par], E.s));
+  // Later work on a member's own stream follows the ensemble step.
+  WFE_HIP(hipEventRecord(E.done, E.s));
+  for (wfe_ctx* c : E.m) {
+    WFE_HIP(hipStreamWaitEvent(c->impl->stream(), E.done, 0));
+    c->impl->flip_parity();
+  }
+}
+
+}  // namespace
+
+int wfe_ensemble_create(wfe_ctx* const* members, int n, wfe_ensemble** out) {
+  if (!members || n < 1 || !out) {
+    wfe::set_last_error("wfe_ensemble_create: need members, n >= 1 and out");
+    return WFE_E_ARG;
+  }
+  for (int q = 0; q < n; ++q) {
+    if (int e = need_ctx(members[q])) return e;
+    for (int r = 0; r < q; ++r)
+      if (members[r] == members[q]) {
+        wfe::set_last_error("wfe_ensemble_create: a context appears twice");
+        return WFE_E_ARG;
+      }
+  }
+  return guarded([&] {
+    auto E = std::make_unique<wfe_ensemble>();
+    E->m.assign(members, members + n);
+    WFE_HIP(hipStreamCreate(&E->s));
+    WFE_HIP(hipEventCreateWithFlags(&E->fork, hipEventDisableTiming));
+    WFE_HIP(hipEventCreateWithFlags(&E->done, hipEventDisableTiming));
+    E->join.assign(n, nullptr);
+    for (int q = 0; q < n; ++q) WFE_HIP(hipEventCreateWithFlags(&E->join[q], hipEventDisableTiming));
+    *out = E.release();
+  });
+}
+
+int wfe_ensemble_destroy(wfe_ensemble* e) {
+  if (!e) return WFE_OK;
+  return guarded([&] {
+    (void)hipStreamSynchronize(e->s);
+    delete e;
+  });
+}
+
+int wfe_ensemble_step(wfe_ensemble* e, const double* dt) {
+  if (!e || !dt) {
+    wfe::set_last_error("wfe_ensemble_step: null argument");
+    return WFE_E_ARG;
+  }
+  return guarded([&] { ensemble_step(*e, dt); });
+}
+
+int wfe_ensemble_sync(wfe_ensemble* e) {
+  if (!e) {
+    wfe::set_last_error("wfe_ensemble_sync: null ensemble");
+    return WFE_E_ARG;
+  }
+  return guarded([&] {
+    WFE_HIP(hipStreamSynchronize(e->s));
+    for (wfe_ctx* c : e->m) WFE_HIP(hipStreamSynchronize(c->impl->stream()));
+  });
 }
 
 int wfe_upload(wfe_ctx* ctx, int field, const double* host, size_t n) {
