@@ -165,6 +165,98 @@ def error_growth(case, checkpoints):
     return out
 
 
+def lam_benchmark(steps: int, warmup: int):
+    """Seeded synthetic regional case, 330 x 220 x 45 at 12 km (terrain,
+    Lambert map factors, PD moisture, Kessler, Davies boundary relaxation
+    toward the boundary files; dt = 60 s, ns = 6), mixed and FP64."""
+    import shutil
+    import tempfile
+
+    from wfe_amd import Case, Integrator
+    from wfe_amd.lam_case import write_lam_case
+
+    d = tempfile.mkdtemp(prefix="wfe_lam_")
+    try:
+        case = Case(write_lam_case(d, hours=0))
+        g = case.grid
+        N = g.n_interior
+        dt = case.scalar("dt")
+        out = {"workload": "regional LAM 330x220x45, dx=12 km, terrain 0-3 km, PD moisture, "
+                           "Kessler, 5-point Davies relaxation, dt=60 s, ns=6",
+               "grid": [g.nx, g.ny, g.nz], "steps": steps}
+        for prec, alg in (("mixed", 1664), ("fp64", 3000)):
+            gi = Integrator.from_case(case, prec)
+            for s_ in range(warmup):
+                gi.step(dt, s_ * dt)
+            gi.sync()
+            t0 = time.perf_counter()
+            for s_ in range(warmup, warmup + steps):
+                gi.step(dt, s_ * dt)
+            gi.sync()
+            el = time.perf_counter() - t0
+            out[prec] = {"value": N * steps / el, "ms_per_step": 1e3 * el / steps,
+                         "step_roofline_frac": alg * N * steps / el / HBM_PEAK,
+                         "alg_bytes_per_pt_step": alg,
+                         "acoustic_method": gi.tuning()["chosen"]}
+            del gi
+        out["speedup_mixed_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+        return out
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+def ensemble_benchmark(steps: int):
+    """Uneven ensemble: 8 members of 200 x 150 x 45 at 12 km (dt = 60 s) and 8
+    of 120 x 120 x 40 at 3 km (dt = 15 s), each a seeded regional case with
+    its own seed plus N(0, 0.5 K) theta noise in the lowest 2 km; mixed
+    precision. Aggregate grid-pt.steps/s of one graph launch per ensemble
+    step against the same 16 contexts stepped one member after another."""
+    import shutil
+    import tempfile
+
+    from wfe_amd import Case, Ensemble, Integrator
+    from wfe_amd.lam_case import write_lam_case
+
+    d = tempfile.mkdtemp(prefix="wfe_ens_")
+    try:
+        members, dts, npts = [], [], 0
+        specs = [dict(nx=200, ny=150, nz=45, dx=12000.0, dt=60.0)] * 8 + \
+                [dict(nx=120, ny=120, nz=40, dx=3000.0, dt=15.0, top=20000.0)] * 8
+        for q, sp in enumerate(specs):
+            case = Case(write_lam_case(f"{d}/m{q:02d}", seed=20261202 + q, hours=0,
+                                       low_noise_K=0.5, **sp))
+            gi = Integrator.from_case(case, "mixed")
+            members.append(gi)
+            dts.append(case.scalar("dt"))
+            npts += case.grid.n_interior
+            shutil.rmtree(f"{d}/m{q:02d}", ignore_errors=True)
+        ens = Ensemble(members)
+        for _ in range(3):
+            ens.step(dts)
+        ens.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ens.step(dts)
+        ens.sync()
+        t_ens = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for gi, dt in zip(members, dts):
+            for _ in range(steps):
+                gi.step(dt)
+            gi.sync()
+        t_seq = time.perf_counter() - t0
+        del ens
+        return {"workload": "8 x 200x150x45 @ 12 km (dt 60 s) + 8 x 120x120x40 @ 3 km "
+                            "(dt 15 s), mixed, one hipGraph launch per ensemble step",
+                "members": len(members), "points_per_round": npts, "steps": steps,
+                "value": npts * steps / t_ens, "ms_per_round": 1e3 * t_ens / steps,
+                "sequential_value": npts * steps / t_seq,
+                "sequential_ms_per_round": 1e3 * t_seq / steps,
+                "speedup_vs_sequential": t_seq / t_ens}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def cpu_baseline(case, budget_s: float):
     """The FP64 oracle on the host cores, on a bounded sample of steps."""
     sys.path.insert(0, str(ROOT / "tests"))
@@ -194,6 +286,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=30)
     ap.add_argument("--solo-reps", type=int, default=200)
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the regional LAM and ensemble measurements")
+    ap.add_argument("--extra-steps", type=int, default=30)
     ap.add_argument("--error-steps", type=int, default=600,
                     help="last checkpoint of the error-growth study (600 steps = 1 h)")
     ap.add_argument("--mixed-only", action="store_true",
@@ -259,6 +354,17 @@ def main() -> int:
         log(f"[bench] error growth against FP64 at steps {cps}")
         growth = error_growth(case, cps)
 
+    lam = ens = None
+    if not args.no_extra:
+        log("[bench] regional LAM 330x220x45")
+        lam = lam_benchmark(args.extra_steps, 3)
+        log(f"[bench] LAM mixed {lam['mixed']['ms_per_step']:.3f} ms/step, "
+            f"fp64 {lam['fp64']['ms_per_step']:.3f} ms/step")
+        log("[bench] uneven ensemble of 16 members")
+        ens = ensemble_benchmark(args.extra_steps)
+        log(f"[bench] ensemble {ens['ms_per_round']:.2f} ms/round vs sequential "
+            f"{ens['sequential_ms_per_round']:.2f} ms/round")
+
     cpu = None
     if not args.no_cpu:
         log(f"[bench] cpu baseline: oracle for ~{args.cpu_budget:.0f} s")
@@ -293,6 +399,8 @@ def main() -> int:
         "kernels_ms": {k: {"launches": v[0], "total_ms": v[1], "avg_us": 1e3 * v[1] / v[0]}
                        for k, v in sorted(rep.items(), key=lambda kv: -kv[1][1])},
         "kernel_solo_us": solo,
+        "regional_lam": lam,
+        "ensemble": ens,
         "acoustic_tuning": tune,
         "copy_probe_GBps": copy_bw,
         "cpu_baseline": cpu,

@@ -24,6 +24,8 @@ The state is synthetic and seeded:
             smooth field, Gaussian-filtered theta perturbation (sigma_theta)
             and u, v perturbations (sigma_u) on a westerly jet profile;
   pi        hydrostatic per column from the perturbed virtual temperature;
+  members   optional white theta noise N(0, low_noise_K) below low_noise_top
+            above ground (ensemble members);
   boundary  the same seeded state at every boundary time (relaxation toward
             a steady target).
 """
@@ -115,7 +117,8 @@ def write_lam_case(out_dir, *, nx=330, ny=220, nz=45, dx=12000.0, seed=20261202,
                    top=25000.0, dz0=250.0, dz_max=1000.0, h_max=3000.0, max_slope=0.3,
                    sigma_theta=1.0, sigma_u=3.0, lat0=45.0, proj_lat1=35.0, proj_lat2=55.0,
                    dt=60.0, acoustic_ns=6, hours=24, bdy_interval_h=24, bdy_width=5,
-                   bdy_tau=600.0, extra: dict | None = None) -> Path:
+                   bdy_tau=600.0, low_noise_K=0.0, low_noise_top=2000.0,
+                   extra: dict | None = None) -> Path:
     """Write the input files and a case .ini into out_dir; returns the .ini."""
     out = Path(out_dir)
     out.mkdir(parents=True, exist_ok=True)
@@ -168,6 +171,12 @@ def write_lam_case(out_dir, *, nx=330, ny=220, nz=45, dx=12000.0, seed=20261202,
     qv3 = np.where(z3 > 15000, 0.0, np.minimum(rh * qsat(p3, T3), 0.018))
     u3 = np.interp(z3, ztab, u_tab) + sigma_u * _smooth_field(rng, z3.shape, None, sig)
     v3 = sigma_u * _smooth_field(rng, z3.shape, None, sig)
+
+    # ensemble member perturbation: white theta noise N(0, low_noise_K) in
+    # the lowest low_noise_top metres
+    if low_noise_K > 0:
+        th3 = th3 + np.where(z3 - h[None] < low_noise_top,
+                             rng.normal(0.0, low_noise_K, z3.shape), 0.0)
 
     # pi per column: hydrostatic with the column's own theta_v, from the mean
     # state's value at the terrain height
