@@ -130,10 +130,11 @@ def pmc_traffic(kernel: str):
 
 def error_growth(case, checkpoints):
     """Relative L2 error over the interior of u, v, w, theta' against the
-    plain FP64 GPU run, at the given step counts, for (a) the mixed path and
+    plain FP64 GPU run, at the given step counts, for (a) the mixed path,
     (b) the FP64 path started from the initial state rounded to FP32 (the
-    round-off the mixed path starts from), which measures how fast the flow
-    itself amplifies a perturbation of that size."""
+    round-off the mixed path starts from) and (c) the FP64 path started from
+    the initial state times (1 + 1e-6 N(0,1)); (b) and (c) measure how fast
+    the flow itself amplifies perturbations of those sizes."""
     from wfe_amd import Integrator
 
     g = case.grid
@@ -141,17 +142,22 @@ def error_growth(case, checkpoints):
     ref = Integrator.from_case(case, "fp64")
     mix = Integrator.from_case(case, "mixed")
     per = Integrator.from_case(case, "fp64")
+    per6 = Integrator.from_case(case, "fp64")
+    rng = np.random.default_rng(20261202)
     for f in ("u", "v", "w", "thp", "pip", "qv", "qc", "qr"):
-        per.set(f, ref.get(f).astype(np.float32).astype(np.float64))
-    out = {"steps": [], "mixed_vs_fp64": {}, "fp64_from_fp32_ic_vs_fp64": {}}
+        a = ref.get(f)
+        per.set(f, a.astype(np.float32).astype(np.float64))
+        per6.set(f, a * (1.0 + 1e-6 * rng.standard_normal(a.shape)))
+    runs = (("mixed_vs_fp64", mix), ("fp64_from_fp32_ic_vs_fp64", per),
+            ("fp64_ic_perturbed_1e-6_vs_fp64", per6))
+    out = {"steps": []}
     fields = ("u", "v", "w", "thp")
-    for f in fields:
-        out["mixed_vs_fp64"][f] = []
-        out["fp64_from_fp32_ic_vs_fp64"][f] = []
+    for key, _ in runs:
+        out[key] = {f: [] for f in fields}
     n = 0
     for target in checkpoints:
         while n < target:
-            for gi in (ref, mix, per):
+            for gi in (ref, mix, per, per6):
                 gi.step(dt, n * dt)
             n += 1
         out["steps"].append(n)
@@ -159,7 +165,7 @@ def error_growth(case, checkpoints):
             nzl = g.nz + 1 if f == "w" else g.nz
             b = g.interior(ref.get(f), nzl)
             nb = float(np.linalg.norm(b))
-            for key, gi in (("mixed_vs_fp64", mix), ("fp64_from_fp32_ic_vs_fp64", per)):
+            for key, gi in runs:
                 a = g.interior(gi.get(f), nzl)
                 out[key][f].append(float(np.linalg.norm(a - b) / nb) if nb > 0 else None)
     return out

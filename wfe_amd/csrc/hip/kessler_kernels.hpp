@@ -25,6 +25,67 @@ __device__ __forceinline__ T qsat_i(T p, T temp) {
   return (T)0.622 * es / den;
 }
 
+// Autoconversion, accretion, rain evaporation and the one-Newton-step
+// saturation adjustment of one point (kessler.cu:99-158), in FP64 whatever
+// the storage type: theta = thetabar + theta' and the saturation deficit
+// q_v - q_s are small differences of large numbers.
+template <class T>
+__device__ __forceinline__ void kessler_convert(T& thp_io, T& qv_io, T& qc_io, T& qr_io,
+                                                double pid, T thb_t, T rho_t, T dt_t) {
+  using W = double;
+  const W k1 = 0.001, qc0 = 0.001;
+  const W dt = (W)dt_t, rhol = (W)rho_t, thb = (W)thb_t;
+  W qvk = qv_io > (T)0 ? (W)qv_io : 0.0;
+  W qck = qc_io > (T)0 ? (W)qc_io : 0.0;
+  W qrk = qr_io > (T)0 ? (W)qr_io : 0.0;
+  const W pi = pid;
+  W th = thb + (W)thp_io;
+  W temp = th * pi;
+  const W pres = kP00 * pow(pid, kCp / kRd);
+  W fl = (temp - 233.15) / 40.0;
+  fl = fmax(0.0, fmin(1.0, fl));
+  const W qsw = qsat_w(pres, temp);
+  W qs = fl * qsw + (1.0 - fl) * qsat_i(pres, temp);
+  const W Leff = fl * kLv + (1.0 - fl) * kLs;
+  const W dqauto = k1 * (qck > qc0 ? qck - qc0 : 0.0);
+  const W dqacc = 2.2 * qck * pow(qrk > 1e-9 ? qrk : 1e-9, 0.875);
+  W dqr = dt * (dqauto + dqacc);
+  if (dqr > qck) dqr = qck;
+  qck -= dqr;
+  qrk += dqr;
+  W thpk = (W)thp_io;
+  if (qvk < qs && qrk > 0.0) {
+    const W rq = rhol * qrk;
+    const W vent = 1.6 + 124.9 * pow(rq, 0.2046);
+    const W ern =
+        vent * pow(rq, 0.525) / (2.55e8 / (pres * qs) + 5.4e5) * (qs - qvk) / (rhol * qs);
+    W dqe = dt * ern;
+    if (dqe > qrk) dqe = qrk;
+    if (dqe > qs - qvk) dqe = qs - qvk;
+    qrk -= dqe;
+    qvk += dqe;
+    thpk -= Leff * dqe / (kCp * pi);
+    th = thb + thpk;
+    temp = th * pi;
+    const W fl2 = fmax(0.0, fmin(1.0, (temp - 233.15) / 40.0));
+    qs = fl2 * qsat_w(pres, temp) + (1.0 - fl2) * qsat_i(pres, temp);
+  }
+  const W gam = Leff * Leff * qs / (kCp * kRv * temp * temp);
+  W dq = (qvk - qs) / (1.0 + gam);
+  if (dq < 0.0) {
+    W evap = -dq;
+    if (evap > qck) evap = qck;
+    dq = -evap;
+  }
+  qvk -= dq;
+  qck += dq;
+  thpk += Leff * dq / (kCp * pi);
+  thp_io = (T)thpk;
+  qv_io = (T)(qvk > 0.0 ? qvk : 0.0);
+  qc_io = (T)(qck > 0.0 ? qck : 0.0);
+  qr_io = (T)(qrk > 0.0 ? qrk : 0.0);
+}
+
 template <class T>
 struct KesArgs {
   T* thp;
@@ -44,7 +105,6 @@ __global__ void __launch_bounds__(64) k_kessler_col(Grid g, Base<T> b, Metric<T>
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= g.nx || j >= g.ny) return;
   const T J = m.jac[g.id2(i, j)];
-  const T k1 = (T)0.001, qc0 = (T)0.001;
   const T dt = a.dt;
   const int PL = g.plane;
   const int c0 = g.id(i, j, 0);
@@ -94,59 +154,12 @@ __global__ void __launch_bounds__(64) k_kessler_col(Grid g, Base<T> b, Metric<T>
   // ---- 2+3) conversions + saturation adjustment, level by level
   for (int k = 0; k < g.nz; ++k) {
     const int c = c0 + k * PL;
-    const T rhol = b.rhob[c];
-    T qvk = a.qv[c] > (T)0 ? a.qv[c] : (T)0;
-    T qck = a.qc[c] > (T)0 ? a.qc[c] : (T)0;
-    T qrk = qrl[k] > (T)0 ? qrl[k] : (T)0;
-    const double pid = b.pib[c] + a.pip[c];  // pibar + pi' in FP64
-    const T pi = (T)pid;
-    T th = b.thb[c] + a.thp[c];
-    T temp = th * pi;
-    const T pres = (T)(kP00 * pow(pid, kCp / kRd));
-    T fl = (temp - (T)233.15) / (T)40;
-    fl = fmax((T)0, fmin((T)1, fl));
-    const T qsw = qsat_w(pres, temp);
-    T qs = fl * qsw + ((T)1 - fl) * qsat_i(pres, temp);
-    const T Leff = fl * (T)kLv + ((T)1 - fl) * (T)kLs;
-
-    const T dqauto = k1 * (qck > qc0 ? qck - qc0 : (T)0);
-    const T dqacc = (T)2.2 * qck * pow(qrk > (T)1e-9 ? qrk : (T)1e-9, (T)0.875);
-    T dqr = dt * (dqauto + dqacc);
-    if (dqr > qck) dqr = qck;
-    qck -= dqr;
-    qrk += dqr;
-
-    T thpk = a.thp[c];
-    if (qvk < qs && qrk > (T)0) {
-      const T rq = rhol * qrk;
-      const T vent = (T)1.6 + (T)124.9 * pow(rq, (T)0.2046);
-      const T ern = vent * pow(rq, (T)0.525) / ((T)2.55e8 / (pres * qs) + (T)5.4e5) *
-                    (qs - qvk) / (rhol * qs);
-      T dqe = dt * ern;
-      if (dqe > qrk) dqe = qrk;
-      if (dqe > qs - qvk) dqe = qs - qvk;
-      qrk -= dqe;
-      qvk += dqe;
-      thpk -= Leff * dqe / ((T)kCp * pi);
-      th = b.thb[c] + thpk;
-      temp = th * pi;
-      const T fl2 = fmax((T)0, fmin((T)1, (temp - (T)233.15) / (T)40));
-      qs = fl2 * qsat_w(pres, temp) + ((T)1 - fl2) * qsat_i(pres, temp);
-    }
-    const T gam = Leff * Leff * qs / ((T)kCp * (T)kRv * temp * temp);
-    T dq = (qvk - qs) / ((T)1 + gam);
-    if (dq < (T)0) {
-      T evap = -dq;
-      if (evap > qck) evap = qck;
-      dq = -evap;
-    }
-    qvk -= dq;
-    qck += dq;
-    thpk += Leff * dq / ((T)kCp * pi);
-    a.thp[c] = thpk;
-    a.qv[c] = qvk > (T)0 ? qvk : (T)0;
-    a.qc[c] = qck > (T)0 ? qck : (T)0;
-    a.qr[c] = qrk > (T)0 ? qrk : (T)0;
+    T thp = a.thp[c], qv = a.qv[c], qc = a.qc[c], qr = qrl[k];
+    kessler_convert(thp, qv, qc, qr, b.pib[c] + a.pip[c], b.thb[c], b.rhob[c], dt);
+    a.thp[c] = thp;
+    a.qv[c] = qv;
+    a.qc[c] = qc;
+    a.qr[c] = qr;
   }
 }
 
@@ -252,63 +265,17 @@ __global__ void __launch_bounds__(TX * NW) k_kessler_tile(Grid g, Base<T> b, Met
   if (col && ry == 0) a.rain[g.id2(i, j)] += acc;
 
   // conversions + saturation adjustment, level by level (kessler.cu:99-158)
-  const T k1 = (T)0.001, qc0 = (T)0.001;
 #pragma unroll
   for (int t = 0; t < KPT; ++t) {
     const int k = ry + NW * t;
     if (!col || k >= g.nz) continue;
     const int c = c0 + k * PL;
-    const T rhol = rho[t];
-    T qvk = a.qv[c] > (T)0 ? a.qv[c] : (T)0;
-    T qck = a.qc[c] > (T)0 ? a.qc[c] : (T)0;
-    T qrk = qrl[t] > (T)0 ? qrl[t] : (T)0;
-    const double pid = b.pib[c] + a.pip[c];
-    const T pi = (T)pid;
-    T th = b.thb[c] + a.thp[c];
-    T temp = th * pi;
-    const T pres = (T)(kP00 * pow(pid, kCp / kRd));
-    T fl = (temp - (T)233.15) / (T)40;
-    fl = fmax((T)0, fmin((T)1, fl));
-    const T qsw = qsat_w(pres, temp);
-    T qs = fl * qsw + ((T)1 - fl) * qsat_i(pres, temp);
-    const T Leff = fl * (T)kLv + ((T)1 - fl) * (T)kLs;
-    const T dqauto = k1 * (qck > qc0 ? qck - qc0 : (T)0);
-    const T dqacc = (T)2.2 * qck * pow(qrk > (T)1e-9 ? qrk : (T)1e-9, (T)0.875);
-    T dqr = dt * (dqauto + dqacc);
-    if (dqr > qck) dqr = qck;
-    qck -= dqr;
-    qrk += dqr;
-    T thpk = a.thp[c];
-    if (qvk < qs && qrk > (T)0) {
-      const T rq = rhol * qrk;
-      const T vent = (T)1.6 + (T)124.9 * pow(rq, (T)0.2046);
-      const T ern = vent * pow(rq, (T)0.525) / ((T)2.55e8 / (pres * qs) + (T)5.4e5) *
-                    (qs - qvk) / (rhol * qs);
-      T dqe = dt * ern;
-      if (dqe > qrk) dqe = qrk;
-      if (dqe > qs - qvk) dqe = qs - qvk;
-      qrk -= dqe;
-      qvk += dqe;
-      thpk -= Leff * dqe / ((T)kCp * pi);
-      th = b.thb[c] + thpk;
-      temp = th * pi;
-      const T fl2 = fmax((T)0, fmin((T)1, (temp - (T)233.15) / (T)40));
-      qs = fl2 * qsat_w(pres, temp) + ((T)1 - fl2) * qsat_i(pres, temp);
-    }
-    const T gam = Leff * Leff * qs / ((T)kCp * (T)kRv * temp * temp);
-    T dq = (qvk - qs) / ((T)1 + gam);
-    if (dq < (T)0) {
-      T evap = -dq;
-      if (evap > qck) evap = qck;
-      dq = -evap;
-    }
-    qvk -= dq;
-    qck += dq;
-    thpk += Leff * dq / ((T)kCp * pi);
-    a.thp[c] = thpk;
-    a.qv[c] = qvk > (T)0 ? qvk : (T)0;
-    a.qc[c] = qck > (T)0 ? qck : (T)0;
-    a.qr[c] = qrk > (T)0 ? qrk : (T)0;
+    T thp = a.thp[c], qv = a.qv[c], qc = a.qc[c], qr = qrl[t];
+    kessler_convert(thp, qv, qc, qr, b.pib[c] + a.pip[c], b.thb[c], rho[t], dt);
+    a.thp[c] = thp;
+    a.qv[c] = qv;
+    a.qc[c] = qc;
+    a.qr[c] = qr;
   }
 }
 
