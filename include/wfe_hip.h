@@ -111,6 +111,28 @@ int wfe_destroy(wfe_ctx* ctx);
  * wfe_set_bdy_tfrac). Asynchronous with respect to the host. */
 int wfe_step(wfe_ctx* ctx, double dt, double t);
 
+/* ---- tangent-linear and adjoint of the dry large step (FP64) ----
+ * No counterpart in the reference. Scope: plain FP64 contexts of dry cases
+ * with periodic lateral boundaries, flat terrain, no Coriolis, no diffusion,
+ * no w damping and no boundary relaxation (WFE_E_CONFIG otherwise); nz <= 63,
+ * acoustic_ns <= 16. Fields are (u, v, w, theta', pi') as five FP64 host
+ * arrays in the padded reference layout (n = padded 3-D element count);
+ * ghost values of inputs are ignored (they are filled from the interior).
+ * The upwind branch of every interface value is taken from the trajectory.
+ *
+ * wfe_tl_step: dy = M'(x) dx for one large step M around the context's
+ *   current state x, which is left unchanged.
+ * wfe_ad_step: xb = M'(x)^T yb, same conventions.
+ * wfe_adj_gradient: J = (1/N) sum_{n=1..N} sum_{i,j} wgt(i,j) (u^2 + v^2)/2
+ *   at the lowest level over N steps from the current state (wgt: padded
+ *   2-D array, n2 elements), and dJ/d(initial u, v, w, theta', pi') in grad;
+ *   the state is kept every ckpt_every steps and each segment recomputed.
+ *   The context's state is left unchanged. */
+int wfe_tl_step(wfe_ctx* ctx, double dt, const double* const* dx, double* const* dy, size_t n);
+int wfe_ad_step(wfe_ctx* ctx, double dt, const double* const* yb, double* const* xb, size_t n);
+int wfe_adj_gradient(wfe_ctx* ctx, double dt, int nsteps, int ckpt_every, const double* wgt2d,
+                     size_t n2, double* J, double* const* grad, size_t n);
+
 /* ---- ensembles: members of any grid size advancing together ----
  * No counterpart in the reference (which runs one case per process). An
  * ensemble borrows n existing contexts (each from wfe_create or

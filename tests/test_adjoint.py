@@ -1,0 +1,168 @@
+"""Tangent-linear and adjoint of the dry large step (FP64): dot-product
+tests, tangent-linear against finite differences of the forward step, and
+the kinetic-energy gradient against finite differences of the objective."""
+import numpy as np
+import pytest
+
+from helpers import case, perturbed_state
+
+pytestmark = pytest.mark.gpu
+
+FIELDS5 = ("u", "v", "w", "thp", "pip")
+
+
+def dry_case(**over):
+    kw = dict(nx=24, ny=20, nz=16, dx=400, dy=400, dz=300, bubble_xc=4800, bubble_yc=4000,
+              bubble_zc=1500, bubble_xr=1500, bubble_yr=1500, bubble_zr=1200, wind_u=6,
+              rayleigh_zd=3000, rayleigh_alpha=0.02, dt=2.0, acoustic_ns=4)
+    kw.update(over)
+    return case("warm_bubble", **kw)
+
+
+def interior_mask(g):
+    m = np.zeros((g.NZ, g.NY, g.NX), dtype=bool)
+    m[g.ng:g.ng + g.nz, g.ng:g.ng + g.ny, g.ng:g.ng + g.nx] = True
+    return m.ravel()
+
+
+def random_state(g, rng, scale):
+    mask = interior_mask(g)
+    return {f: np.where(mask, rng.standard_normal(g.npts) * scale[f], 0.0) for f in FIELDS5}
+
+
+SCALE = {"u": 1.0, "v": 1.0, "w": 0.3, "thp": 0.5, "pip": 1e-4}
+
+
+def make(c, seed):
+    from wfe_amd import Integrator
+    gi = Integrator.from_case(c, "fp64")
+    for f, a in perturbed_state(c, seed).items():
+        gi.set(f, a)
+    return gi
+
+
+def dot(a, b, mask):
+    return sum(float(np.dot(a[f][mask], b[f][mask])) for f in FIELDS5)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_dot_product_one_step(seed):
+    from wfe_amd.adjoint import ad_step, tl_step
+    c = dry_case()
+    g = c.grid
+    gi = make(c, seed)
+    rng = np.random.default_rng(100 + seed)
+    mask = interior_mask(g)
+    dx = random_state(g, rng, SCALE)
+    yb = random_state(g, rng, SCALE)
+    dt = c.scalar("dt")
+    dy = tl_step(gi, dt, dx)
+    xb = ad_step(gi, dt, yb)
+    lhs, rhs = dot(dy, yb, mask), dot(dx, xb, mask)
+    print("dot product", lhs, rhs, abs(lhs - rhs) / abs(lhs))
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+
+
+def test_tangent_linear_matches_finite_differences():
+    from wfe_amd import Integrator
+    from wfe_amd.adjoint import tl_step
+    c = dry_case()
+    g = c.grid
+    gi = make(c, 3)
+    x = {f: gi.get(f) for f in FIELDS5}
+    rng = np.random.default_rng(7)
+    dx = random_state(g, rng, SCALE)
+    dt = c.scalar("dt")
+    dy = tl_step(gi, dt, dx)
+    mask = interior_mask(g)
+    eps = 1e-6
+    outs = []
+    for sgn in (1, -1):
+        gp = Integrator.from_case(c, "fp64")
+        for f in FIELDS5:
+            gp.set(f, x[f] + sgn * eps * dx[f])
+        for f in ("qv", "qc", "qr"):
+            gp.set(f, gi.get(f))
+        gp.step(dt)
+        outs.append({f: gp.get(f) for f in FIELDS5})
+    for f in FIELDS5:
+        fd = (outs[0][f] - outs[1][f])[mask] / (2 * eps)
+        tl = dy[f][mask]
+        err = np.linalg.norm(fd - tl) / np.linalg.norm(tl)
+        print(f, err)
+        assert err < 1e-5, (f, err)
+
+
+def test_kinetic_energy_gradient_matches_finite_differences():
+    from wfe_amd import Integrator
+    from wfe_amd.adjoint import box_weights, ke_gradient
+    c = dry_case()
+    g = c.grid
+    gi = make(c, 4)
+    x = {f: gi.get(f) for f in FIELDS5}
+    dt = c.scalar("dt")
+    w = box_weights(g, 6, 14, 5, 13)
+    nsteps, ckpt = 6, 4
+    J, grad = ke_gradient(gi, dt, nsteps, ckpt, w)
+    # the context is left at the initial state
+    for f in FIELDS5:
+        np.testing.assert_array_equal(gi.get(f), x[f])
+    rng = np.random.default_rng(9)
+    d = random_state(g, rng, SCALE)
+    mask = interior_mask(g)
+    gd = sum(float(np.dot(grad[f][mask], d[f][mask])) for f in FIELDS5)
+
+    def J_of(xs):
+        gp = Integrator.from_case(c, "fp64")
+        for f in FIELDS5:
+            gp.set(f, xs[f])
+        tot = 0.0
+        for s in range(nsteps):
+            gp.step(dt)
+            u = gp.get("u").reshape(g.NZ, g.NY, g.NX)[g.ng]
+            v = gp.get("v").reshape(g.NZ, g.NY, g.NX)[g.ng]
+            ww = w.reshape(g.NY, g.NX)
+            tot += float((ww * 0.5 * (u * u + v * v)).sum()) / nsteps
+        return tot
+
+    assert J == pytest.approx(J_of(x), rel=1e-12)
+    errs = []
+    for eps in (1e-3, 1e-4, 1e-5, 1e-6, 1e-7):
+        jp = J_of({f: x[f] + eps * d[f] for f in FIELDS5})
+        jm = J_of({f: x[f] - eps * d[f] for f in FIELDS5})
+        fd = (jp - jm) / (2 * eps)
+        errs.append(abs(fd - gd) / abs(gd))
+    print("gradient vs FD", gd, errs)
+    assert min(errs) < 1e-6
+
+
+def test_torch_autograd_function():
+    import torch
+    from wfe_amd.adjoint import box_weights, kinetic_energy_function
+    c = dry_case()
+    g = c.grid
+    gi = make(c, 5)
+    KE = kinetic_energy_function()
+    ts = [torch.tensor(gi.get(f), dtype=torch.float64, requires_grad=True) for f in FIELDS5]
+    J = KE.apply(gi, c.scalar("dt"), 3, 2, box_weights(g, 4, 20, 4, 16), *ts)
+    (2.0 * J).backward()
+    from wfe_amd.adjoint import ke_gradient
+    gi2 = make(c, 5)
+    J2, grad = ke_gradient(gi2, c.scalar("dt"), 3, 2, box_weights(g, 4, 20, 4, 16))
+    assert float(J) == pytest.approx(J2, rel=1e-13)
+    # two adjoint runs agree to the summation order of their FP64 atomics
+    for t, f in zip(ts, FIELDS5):
+        scale = np.abs(grad[f]).max() + 1e-300
+        np.testing.assert_allclose(t.grad.numpy(), 2.0 * grad[f], rtol=1e-9, atol=1e-12 * scale)
+
+
+def test_scope_is_enforced():
+    from wfe_amd import Integrator, WfeError
+    from wfe_amd.adjoint import tl_step
+    c = dry_case(bc_x="open")
+    gi = Integrator.from_case(c, "fp64")
+    with pytest.raises(WfeError):
+        tl_step(gi, c.scalar("dt"), {})
+    gm = Integrator.from_case(dry_case(), "mixed")
+    with pytest.raises(WfeError):
+        tl_step(gm, 2.0, {})

@@ -1,0 +1,726 @@
+#pragma once
+// Tangent-linear (TL) and adjoint (AD) kernels of the dry large step in FP64:
+// ghost fill, mass fluxes and divergence, flux-form advection tendencies with
+// the Rayleigh layer, the horizontal acoustic step and the vertically
+// implicit w-pi' column solve. They linearise the forward kernels around a
+// stored trajectory, with the upwind branch of every interface value frozen
+// at the trajectory's mass flux. Scope: periodic lateral boundaries, flat
+// terrain (J = 1, no cross-metric terms), map factor 1, no Coriolis, no
+// diffusion, no vertical-Courant damping, no moisture.
+//
+// Every AD kernel is the statement-by-statement transpose of its TL kernel;
+// contributions to shared inputs are scattered with FP64 atomics.
+#include "hip/device_common.hpp"
+#include "hip/slow_kernels.hpp"
+
+namespace wfe::dev {
+
+// 5th-order upwind interface weights on q[-2..3] for the branch of vel
+// (iface5: centred minus / plus the upwind term).
+__device__ __forceinline__ void iface5_weights(double vel, double w[6]) {
+  const double s = 1.0 / 60.0;
+  if (vel >= 0.0) {
+    w[0] = 2 * s;
+    w[1] = -13 * s;
+    w[2] = 47 * s;
+    w[3] = 27 * s;
+    w[4] = -3 * s;
+    w[5] = 0.0;
+  } else {
+    w[0] = 0.0;
+    w[1] = -3 * s;
+    w[2] = 27 * s;
+    w[3] = 47 * s;
+    w[4] = -13 * s;
+    w[5] = 2 * s;
+  }
+}
+
+__device__ __forceinline__ double iface_at(const double* q, int c, int s, double vel) {
+  return iface5(q[c - 3 * s], q[c - 2 * s], q[c - s], q[c], q[c + s], q[c + 2 * s], vel);
+}
+
+// TL of F = f iface5(q, f) through the face whose upper cell is c.
+__device__ __forceinline__ double tl_face(const double* q, const double* dq, int c, int s,
+                                          double f, double df) {
+  return df * iface_at(q, c, s, f) + f * iface_at(dq, c, s, f);
+}
+
+// AD of the same: scatters Fb f w_n into qb, returns the adjoint of f.
+__device__ __forceinline__ double ad_face(const double* q, double* qb, int c, int s, double f,
+                                          double Fb) {
+  double w[6];
+  iface5_weights(f, w);
+  const double a = Fb * f;
+#pragma unroll
+  for (int n = 0; n < 6; ++n)
+    if (w[n] != 0.0) atomicAdd(&qb[c + (n - 3) * s], a * w[n]);
+  return Fb * iface_at(q, c, s, f);
+}
+
+// ------------------------------------------------------------ ghost fill
+// Five fields (u v w theta' pi'); the fill is linear, so the TL applies the
+// forward fill to the perturbation (k_fill_all) and the AD moves each ghost
+// cell's adjoint onto the interior cells it was copied from.
+struct AdFill {
+  double* f[5];
+  int kind[5];
+  int xlast[5], ylast[5];
+};
+
+__global__ void __launch_bounds__(256) k_ad_fill(Grid g, int bcx, int bcy, AdFill A) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ghost_count(g)) return;
+  const GhostCell q = ghost_cell(g, idx);
+  const int c = g.id(q.i, q.j, q.k);
+#pragma unroll
+  for (int fi = 0; fi < 5; ++fi) {
+    double* f = A.f[fi];
+    const double gb = f[c];
+    f[c] = 0.0;
+    if (gb == 0.0) continue;
+    const int im = map_lat(q.i, g.nx, A.xlast[fi], bcx);
+    const int jm = map_lat(q.j, g.ny, A.ylast[fi], bcy);
+    if (A.kind[fi] == kFillW) {
+      if (q.k == g.nz) continue;
+      if (q.k < 0) {
+        atomicAdd(&f[g.id(im, jm, 0)], 2.0 * gb);
+        atomicAdd(&f[g.id(im, jm, -q.k)], -gb);
+      } else if (q.k > g.nz) {
+        atomicAdd(&f[g.id(im, jm, 2 * g.nz - q.k)], -gb);
+      } else {
+        atomicAdd(&f[g.id(im, jm, q.k)], gb);
+      }
+    } else {
+      const int km = q.k < 0 ? 0 : (q.k > g.nz - 1 ? g.nz - 1 : q.k);
+      atomicAdd(&f[g.id(im, jm, km)], gb);
+    }
+  }
+}
+
+// ------------------------------------------------------------ mass fluxes
+// Linear in (u, v, w): the TL is k_massflux on the perturbation. AD over the
+// same ranges (mfx, mfy on i,j in [-2, n+2]; mfz on [-2, n+1], k in
+// [1, nz); div on [-1, n]).
+__global__ void k_ad_massflux(Grid g, Metric<double> m, double* mfxb, double* mfyb, double* mfzb,
+                              const double* divb) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) - 2;
+  const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) - 2;
+  const int k = blockIdx.z;
+  if (i > g.nx + 2 || j > g.ny + 2 || k >= g.nz) return;
+  const int c = g.id(i, j, k);
+  // divergence -> fluxes (div = (fx(i+1)-fx(i))/dx + (fy(j+1)-fy(j))/dy + (fz(k+1)-fz(k))/dzc)
+  if (i >= -1 && i <= g.nx && j >= -1 && j <= g.ny) {
+    const double d = divb[c];
+    if (d != 0.0) {
+      atomicAdd(&mfxb[c + 1], d / g.dx);
+      atomicAdd(&mfxb[c], -d / g.dx);
+      atomicAdd(&mfyb[c + g.NX], d / g.dy);
+      atomicAdd(&mfyb[c], -d / g.dy);
+      const double dzc = m.dzeta_c[k + g.ng];
+      if (k + 1 < g.nz) atomicAdd(&mfzb[c + g.plane], d / dzc);
+      if (k >= 1) atomicAdd(&mfzb[c], -d / dzc);
+    }
+  }
+}
+
+// Fluxes -> velocities, after k_ad_massflux has gathered every divergence
+// contribution (one thread per flux point; flat terrain, J = 1).
+__global__ void k_ad_massflux_uvw(Grid g, Base<double> b, Metric<double> m, double* ub,
+                                  double* vb, double* wb, const double* mfxb,
+                                  const double* mfyb, const double* mfzb) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) - 2;
+  const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) - 2;
+  const int k = blockIdx.z;
+  if (i > g.nx + 2 || j > g.ny + 2 || k >= g.nz) return;
+  const int c = g.id(i, j, k);
+  const int c2 = g.id2(i, j);
+  const double rj = b.rhob[c] * m.jac[c2];
+  const double rjx = 0.5 * (b.rhob[c - 1] * m.jac[c2 - 1] + rj);
+  const double rjy = 0.5 * (b.rhob[c - g.NX] * m.jac[c2 - g.NX] + rj);
+  ub[c] += rjx * mfxb[c];
+  vb[c] += rjy * mfyb[c];
+  if (k >= 1 && i <= g.nx + 1 && j <= g.ny + 1) wb[c] += b.rhobw[c] * mfzb[c];
+}
+
+// ------------------------------------------------------------ tendencies
+struct TanTraj {  // trajectory of one stage
+  const double *u, *v, *w, *th, *mfx, *mfy, *mfz, *div;
+};
+struct TanPert {  // perturbation (TL) or adjoint (AD) arrays of one stage
+  double *u, *v, *w, *th, *mfx, *mfy, *mfz, *div;
+};
+struct TanTend {  // tendency perturbations (TL out) or adjoints (AD in)
+  double *tu, *tv, *tw, *tth;
+};
+
+__global__ void k_tl_tend(Grid g, Base<double> b, Metric<double> m, TanTraj X, TanPert D,
+                          TanTend out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+  const int kk = k + g.ng;
+  const int c = g.id(i, j, k);
+  const int c2 = g.id2(i, j);
+  const int NX = g.NX, PL = g.plane;
+  const double dx = g.dx, dy = g.dy;
+  const double dzc = m.dzeta_c[kk];
+  const double rt = b.rhob[c] * m.jac[c2];
+  {  // u
+    const double fcL = 0.5 * (X.mfx[c - 1] + X.mfx[c]), dfcL = 0.5 * (D.mfx[c - 1] + D.mfx[c]);
+    const double fcR = 0.5 * (X.mfx[c] + X.mfx[c + 1]), dfcR = 0.5 * (D.mfx[c] + D.mfx[c + 1]);
+    const double fyB = 0.5 * (X.mfy[c - 1] + X.mfy[c]), dfyB = 0.5 * (D.mfy[c - 1] + D.mfy[c]);
+    const double fyT = 0.5 * (X.mfy[c - 1 + NX] + X.mfy[c + NX]),
+                 dfyT = 0.5 * (D.mfy[c - 1 + NX] + D.mfy[c + NX]);
+    const double fzB = 0.5 * (X.mfz[c - 1] + X.mfz[c]), dfzB = 0.5 * (D.mfz[c - 1] + D.mfz[c]);
+    const double fzT = 0.5 * (X.mfz[c - 1 + PL] + X.mfz[c + PL]),
+                 dfzT = 0.5 * (D.mfz[c - 1 + PL] + D.mfz[c + PL]);
+    const double dfd = (tl_face(X.u, D.u, c + 1, 1, fcR, dfcR) - tl_face(X.u, D.u, c, 1, fcL, dfcL)) / dx +
+                       (tl_face(X.u, D.u, c + NX, NX, fyT, dfyT) - tl_face(X.u, D.u, c, NX, fyB, dfyB)) / dy +
+                       (tl_face(X.u, D.u, c + PL, PL, fzT, dfzT) - tl_face(X.u, D.u, c, PL, fzB, dfzB)) / dzc;
+    const double dv = 0.5 * (X.div[c - 1] + X.div[c]), ddv = 0.5 * (D.div[c - 1] + D.div[c]);
+    const double ru = 0.5 * (b.rhob[c - 1] * m.jac[c2 - 1] + rt);
+    out.tu[c] = (-dfd + D.u[c] * dv + X.u[c] * ddv) / ru - m.ray_c[kk] * D.u[c];
+  }
+  {  // v
+    const double fcB = 0.5 * (X.mfy[c - NX] + X.mfy[c]), dfcB = 0.5 * (D.mfy[c - NX] + D.mfy[c]);
+    const double fcT = 0.5 * (X.mfy[c] + X.mfy[c + NX]), dfcT = 0.5 * (D.mfy[c] + D.mfy[c + NX]);
+    const double fxL = 0.5 * (X.mfx[c - NX] + X.mfx[c]), dfxL = 0.5 * (D.mfx[c - NX] + D.mfx[c]);
+    const double fxR = 0.5 * (X.mfx[c + 1 - NX] + X.mfx[c + 1]),
+                 dfxR = 0.5 * (D.mfx[c + 1 - NX] + D.mfx[c + 1]);
+    const double fzB = 0.5 * (X.mfz[c - NX] + X.mfz[c]), dfzB = 0.5 * (D.mfz[c - NX] + D.mfz[c]);
+    const double fzT = 0.5 * (X.mfz[c - NX + PL] + X.mfz[c + PL]),
+                 dfzT = 0.5 * (D.mfz[c - NX + PL] + D.mfz[c + PL]);
+    const double dfd = (tl_face(X.v, D.v, c + 1, 1, fxR, dfxR) - tl_face(X.v, D.v, c, 1, fxL, dfxL)) / dx +
+                       (tl_face(X.v, D.v, c + NX, NX, fcT, dfcT) - tl_face(X.v, D.v, c, NX, fcB, dfcB)) / dy +
+                       (tl_face(X.v, D.v, c + PL, PL, fzT, dfzT) - tl_face(X.v, D.v, c, PL, fzB, dfzB)) / dzc;
+    const double dv = 0.5 * (X.div[c - NX] + X.div[c]), ddv = 0.5 * (D.div[c - NX] + D.div[c]);
+    const double rv = 0.5 * (b.rhob[c - NX] * m.jac[c2 - NX] + rt);
+    out.tv[c] = (-dfd + D.v[c] * dv + X.v[c] * ddv) / rv - m.ray_c[kk] * D.v[c];
+  }
+  if (k >= 1) {  // w
+    const double dzw = m.dzeta_w[kk];
+    const double fxL = 0.5 * (X.mfx[c - PL] + X.mfx[c]), dfxL = 0.5 * (D.mfx[c - PL] + D.mfx[c]);
+    const double fxR = 0.5 * (X.mfx[c + 1 - PL] + X.mfx[c + 1]),
+                 dfxR = 0.5 * (D.mfx[c + 1 - PL] + D.mfx[c + 1]);
+    const double fyB = 0.5 * (X.mfy[c - PL] + X.mfy[c]), dfyB = 0.5 * (D.mfy[c - PL] + D.mfy[c]);
+    const double fyT = 0.5 * (X.mfy[c + NX - PL] + X.mfy[c + NX]),
+                 dfyT = 0.5 * (D.mfy[c + NX - PL] + D.mfy[c + NX]);
+    const double fzB = 0.5 * (X.mfz[c - PL] + X.mfz[c]), dfzB = 0.5 * (D.mfz[c - PL] + D.mfz[c]);
+    const double fzT = 0.5 * (X.mfz[c] + X.mfz[c + PL]), dfzT = 0.5 * (D.mfz[c] + D.mfz[c + PL]);
+    const double dfd = (tl_face(X.w, D.w, c + 1, 1, fxR, dfxR) - tl_face(X.w, D.w, c, 1, fxL, dfxL)) / dx +
+                       (tl_face(X.w, D.w, c + NX, NX, fyT, dfyT) - tl_face(X.w, D.w, c, NX, fyB, dfyB)) / dy +
+                       (tl_face(X.w, D.w, c + PL, PL, fzT, dfzT) - tl_face(X.w, D.w, c, PL, fzB, dfzB)) / dzw;
+    const double dv = 0.5 * (X.div[c - PL] + X.div[c]), ddv = 0.5 * (D.div[c - PL] + D.div[c]);
+    const double rw = b.rhobw[c] * m.jac[c2];
+    out.tw[c] = (-dfd + D.w[c] * dv + X.w[c] * ddv) / rw - m.ray_w[kk] * D.w[c];
+  }
+  {  // theta'
+    const double dfd =
+        (tl_face(X.th, D.th, c + 1, 1, X.mfx[c + 1], D.mfx[c + 1]) -
+         tl_face(X.th, D.th, c, 1, X.mfx[c], D.mfx[c])) / dx +
+        (tl_face(X.th, D.th, c + NX, NX, X.mfy[c + NX], D.mfy[c + NX]) -
+         tl_face(X.th, D.th, c, NX, X.mfy[c], D.mfy[c])) / dy +
+        (tl_face(X.th, D.th, c + PL, PL, X.mfz[c + PL], D.mfz[c + PL]) -
+         tl_face(X.th, D.th, c, PL, X.mfz[c], D.mfz[c])) / dzc;
+    out.tth[c] = (-dfd + D.th[c] * X.div[c] + X.th[c] * D.div[c]) / rt - m.ray_c[kk] * D.th[c];
+  }
+}
+
+__global__ void k_ad_tend(Grid g, Base<double> b, Metric<double> m, TanTraj X, TanPert A,
+                          TanTend in) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+  const int kk = k + g.ng;
+  const int c = g.id(i, j, k);
+  const int c2 = g.id2(i, j);
+  const int NX = g.NX, PL = g.plane;
+  const double dx = g.dx, dy = g.dy;
+  const double dzc = m.dzeta_c[kk];
+  const double rt = b.rhob[c] * m.jac[c2];
+  // Adjoint of a face-averaged flux f = 0.5 (mf[a] + mf[b]).
+  auto avg_b = [](double* mfb, int a, int bb, double fb) {
+    atomicAdd(&mfb[a], 0.5 * fb);
+    atomicAdd(&mfb[bb], 0.5 * fb);
+  };
+  {  // u
+    const double T = in.tu[c];
+    if (T != 0.0) {
+      const double ru = 0.5 * (b.rhob[c - 1] * m.jac[c2 - 1] + rt);
+      const double a = T / ru;
+      const double fdb = -a;
+      const double dv = 0.5 * (X.div[c - 1] + X.div[c]);
+      atomicAdd(&A.u[c], a * dv - m.ray_c[kk] * T);
+      const double dvb = a * X.u[c];
+      atomicAdd(&A.div[c - 1], 0.5 * dvb);
+      atomicAdd(&A.div[c], 0.5 * dvb);
+      const double fcL = 0.5 * (X.mfx[c - 1] + X.mfx[c]);
+      const double fcR = 0.5 * (X.mfx[c] + X.mfx[c + 1]);
+      const double fyB = 0.5 * (X.mfy[c - 1] + X.mfy[c]);
+      const double fyT = 0.5 * (X.mfy[c - 1 + NX] + X.mfy[c + NX]);
+      const double fzB = 0.5 * (X.mfz[c - 1] + X.mfz[c]);
+      const double fzT = 0.5 * (X.mfz[c - 1 + PL] + X.mfz[c + PL]);
+      avg_b(A.mfx, c, c + 1, ad_face(X.u, A.u, c + 1, 1, fcR, fdb / dx));
+      avg_b(A.mfx, c - 1, c, ad_face(X.u, A.u, c, 1, fcL, -fdb / dx));
+      avg_b(A.mfy, c - 1 + NX, c + NX, ad_face(X.u, A.u, c + NX, NX, fyT, fdb / dy));
+      avg_b(A.mfy, c - 1, c, ad_face(X.u, A.u, c, NX, fyB, -fdb / dy));
+      avg_b(A.mfz, c - 1 + PL, c + PL, ad_face(X.u, A.u, c + PL, PL, fzT, fdb / dzc));
+      avg_b(A.mfz, c - 1, c, ad_face(X.u, A.u, c, PL, fzB, -fdb / dzc));
+    }
+  }
+  {  // v
+    const double T = in.tv[c];
+    if (T != 0.0) {
+      const double rv = 0.5 * (b.rhob[c - NX] * m.jac[c2 - NX] + rt);
+      const double a = T / rv;
+      const double fdb = -a;
+      const double dv = 0.5 * (X.div[c - NX] + X.div[c]);
+      atomicAdd(&A.v[c], a * dv - m.ray_c[kk] * T);
+      const double dvb = a * X.v[c];
+      atomicAdd(&A.div[c - NX], 0.5 * dvb);
+      atomicAdd(&A.div[c], 0.5 * dvb);
+      const double fcB = 0.5 * (X.mfy[c - NX] + X.mfy[c]);
+      const double fcT = 0.5 * (X.mfy[c] + X.mfy[c + NX]);
+      const double fxL = 0.5 * (X.mfx[c - NX] + X.mfx[c]);
+      const double fxR = 0.5 * (X.mfx[c + 1 - NX] + X.mfx[c + 1]);
+      const double fzB = 0.5 * (X.mfz[c - NX] + X.mfz[c]);
+      const double fzT = 0.5 * (X.mfz[c - NX + PL] + X.mfz[c + PL]);
+      avg_b(A.mfx, c + 1 - NX, c + 1, ad_face(X.v, A.v, c + 1, 1, fxR, fdb / dx));
+      avg_b(A.mfx, c - NX, c, ad_face(X.v, A.v, c, 1, fxL, -fdb / dx));
+      avg_b(A.mfy, c, c + NX, ad_face(X.v, A.v, c + NX, NX, fcT, fdb / dy));
+      avg_b(A.mfy, c - NX, c, ad_face(X.v, A.v, c, NX, fcB, -fdb / dy));
+      avg_b(A.mfz, c - NX + PL, c + PL, ad_face(X.v, A.v, c + PL, PL, fzT, fdb / dzc));
+      avg_b(A.mfz, c - NX, c, ad_face(X.v, A.v, c, PL, fzB, -fdb / dzc));
+    }
+  }
+  if (k >= 1) {  // w
+    const double T = in.tw[c];
+    if (T != 0.0) {
+      const double dzw = m.dzeta_w[kk];
+      const double rw = b.rhobw[c] * m.jac[c2];
+      const double a = T / rw;
+      const double fdb = -a;
+      const double dv = 0.5 * (X.div[c - PL] + X.div[c]);
+      atomicAdd(&A.w[c], a * dv - m.ray_w[kk] * T);
+      const double dvb = a * X.w[c];
+      atomicAdd(&A.div[c - PL], 0.5 * dvb);
+      atomicAdd(&A.div[c], 0.5 * dvb);
+      const double fxL = 0.5 * (X.mfx[c - PL] + X.mfx[c]);
+      const double fxR = 0.5 * (X.mfx[c + 1 - PL] + X.mfx[c + 1]);
+      const double fyB = 0.5 * (X.mfy[c - PL] + X.mfy[c]);
+      const double fyT = 0.5 * (X.mfy[c + NX - PL] + X.mfy[c + NX]);
+      const double fzB = 0.5 * (X.mfz[c - PL] + X.mfz[c]);
+      const double fzT = 0.5 * (X.mfz[c] + X.mfz[c + PL]);
+      avg_b(A.mfx, c + 1 - PL, c + 1, ad_face(X.w, A.w, c + 1, 1, fxR, fdb / dx));
+      avg_b(A.mfx, c - PL, c, ad_face(X.w, A.w, c, 1, fxL, -fdb / dx));
+      avg_b(A.mfy, c + NX - PL, c + NX, ad_face(X.w, A.w, c + NX, NX, fyT, fdb / dy));
+      avg_b(A.mfy, c - PL, c, ad_face(X.w, A.w, c, NX, fyB, -fdb / dy));
+      avg_b(A.mfz, c, c + PL, ad_face(X.w, A.w, c + PL, PL, fzT, fdb / dzw));
+      avg_b(A.mfz, c - PL, c, ad_face(X.w, A.w, c, PL, fzB, -fdb / dzw));
+    }
+  }
+  {  // theta'
+    const double T = in.tth[c];
+    if (T != 0.0) {
+      const double a = T / rt;
+      const double fdb = -a;
+      atomicAdd(&A.th[c], a * X.div[c] - m.ray_c[kk] * T);
+      atomicAdd(&A.div[c], a * X.th[c]);
+      atomicAdd(&A.mfx[c + 1], ad_face(X.th, A.th, c + 1, 1, X.mfx[c + 1], fdb / dx));
+      atomicAdd(&A.mfx[c], ad_face(X.th, A.th, c, 1, X.mfx[c], -fdb / dx));
+      atomicAdd(&A.mfy[c + NX], ad_face(X.th, A.th, c + NX, NX, X.mfy[c + NX], fdb / dy));
+      atomicAdd(&A.mfy[c], ad_face(X.th, A.th, c, NX, X.mfy[c], -fdb / dy));
+      atomicAdd(&A.mfz[c + PL], ad_face(X.th, A.th, c + PL, PL, X.mfz[c + PL], fdb / dzc));
+      atomicAdd(&A.mfz[c], ad_face(X.th, A.th, c, PL, X.mfz[c], -fdb / dzc));
+    }
+  }
+}
+
+// ------------------------------------------------- horizontal acoustic step
+// u_new = u + dtau (-cp thv_u grad pi* + Tu) on faces i = 0..nx-1 (u(nx) is
+// the periodic image of u(0)), same for v. pi* = pi' + smdiv (pi' - pi'prev)
+// with the substep ghost rule: outside the interior, pi' is read from the
+// mapped interior of the current substep and pi'prev from the time-n ghost
+// cell; theta' outside the interior from the time-n ghost cell.
+struct UVTraj {
+  const double *pcur, *pprev, *pn, *thc, *thn;
+};
+struct UVTan {  // TL: perturbations in, du/dv out; AD: adjoints
+  double *u_src, *v_src, *u_dst, *v_dst, *pcur, *pprev, *pn, *thc, *thn, *tu, *tv;
+};
+
+struct UVLin {
+  const Grid& g;
+  const Params& p;
+  const Base<double>& b;
+  const UVTraj& X;
+  __device__ bool inside(int ii, int jj) const {
+    return (unsigned)ii < (unsigned)g.nx && (unsigned)jj < (unsigned)g.ny;
+  }
+  __device__ int mapped(int ii, int jj, int k) const {
+    return g.id(map_lat(ii, g.nx, g.nx - 1, p.bcx), map_lat(jj, g.ny, g.ny - 1, p.bcy), k);
+  }
+  __device__ double pstar(int ii, int jj, int k) const {
+    const int c = g.id(ii, jj, k);
+    if (inside(ii, jj)) return X.pcur[c] + p.smdiv * (X.pcur[c] - X.pprev[c]);
+    const double pc = X.pcur[mapped(ii, jj, k)];
+    return pc + p.smdiv * (pc - X.pn[c]);
+  }
+  __device__ double thv(int ii, int jj, int k) const {
+    const int c = g.id(ii, jj, k);
+    return b.thb[c] + (inside(ii, jj) ? X.thc[c] : X.thn[c]);
+  }
+  __device__ double dpstar(const UVTan& D, int ii, int jj, int k) const {
+    const int c = g.id(ii, jj, k);
+    if (inside(ii, jj)) return D.pcur[c] + p.smdiv * (D.pcur[c] - D.pprev[c]);
+    const double pc = D.pcur[mapped(ii, jj, k)];
+    return pc + p.smdiv * (pc - D.pn[c]);
+  }
+  __device__ double dthv(const UVTan& D, int ii, int jj, int k) const {
+    const int c = g.id(ii, jj, k);
+    return inside(ii, jj) ? D.thc[c] : D.thn[c];
+  }
+  __device__ void ad_pstar(const UVTan& A, int ii, int jj, int k, double pb) const {
+    const int c = g.id(ii, jj, k);
+    if (inside(ii, jj)) {
+      atomicAdd(&A.pcur[c], (1.0 + p.smdiv) * pb);
+      atomicAdd(&A.pprev[c], -p.smdiv * pb);
+    } else {
+      atomicAdd(&A.pcur[mapped(ii, jj, k)], (1.0 + p.smdiv) * pb);
+      atomicAdd(&A.pn[c], -p.smdiv * pb);
+    }
+  }
+  __device__ void ad_thv(const UVTan& A, int ii, int jj, int k, double tb) const {
+    const int c = g.id(ii, jj, k);
+    atomicAdd(inside(ii, jj) ? &A.thc[c] : &A.thn[c], tb);
+  }
+};
+
+__global__ void k_tl_uv(Grid g, Params p, Base<double> b, Metric<double> m, UVTraj X, UVTan D,
+                        double dtau) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i > g.nx || j > g.ny || k >= g.nz) return;
+  const UVLin L{g, p, b, X};
+  const int c = g.id(i, j, k);
+  if (j < g.ny) {  // u face i (i = nx is the image of face 0)
+    const int iu = i < g.nx ? i : 0;
+    const int cu = g.id(iu, j, k), c2 = g.id2(iu, j);
+    const double mu = 0.5 * (m.mapf[c2 - 1] + m.mapf[c2]);
+    const double thu = 0.5 * (L.thv(iu - 1, j, k) + L.thv(iu, j, k));
+    const double dthu = 0.5 * (L.dthv(D, iu - 1, j, k) + L.dthv(D, iu, j, k));
+    const double grad = (L.pstar(iu, j, k) - L.pstar(iu - 1, j, k)) * g.rdx;
+    const double dgrad = (L.dpstar(D, iu, j, k) - L.dpstar(D, iu - 1, j, k)) * g.rdx;
+    D.u_dst[c] = D.u_src[cu] + dtau * (-kCp * mu * (dthu * grad + thu * dgrad) + D.tu[cu]);
+  }
+  if (i < g.nx) {  // v face j (j = ny is the image of face 0)
+    const int jv = j < g.ny ? j : 0;
+    const int cv = g.id(i, jv, k), c2 = g.id2(i, jv);
+    const double mv = 0.5 * (m.mapf[c2 - g.NX] + m.mapf[c2]);
+    const double thv_ = 0.5 * (L.thv(i, jv - 1, k) + L.thv(i, jv, k));
+    const double dthv_ = 0.5 * (L.dthv(D, i, jv - 1, k) + L.dthv(D, i, jv, k));
+    const double grad = (L.pstar(i, jv, k) - L.pstar(i, jv - 1, k)) * g.rdy;
+    const double dgrad = (L.dpstar(D, i, jv, k) - L.dpstar(D, i, jv - 1, k)) * g.rdy;
+    D.v_dst[c] = D.v_src[cv] + dtau * (-kCp * mv * (dthv_ * grad + thv_ * dgrad) + D.tv[cv]);
+  }
+}
+
+__global__ void k_ad_uv(Grid g, Params p, Base<double> b, Metric<double> m, UVTraj X, UVTan A,
+                        double dtau) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i > g.nx || j > g.ny || k >= g.nz) return;
+  const UVLin L{g, p, b, X};
+  const int c = g.id(i, j, k);
+  if (j < g.ny) {
+    const double ob = A.u_dst[c];
+    if (ob != 0.0) {
+      const int iu = i < g.nx ? i : 0;
+      const int cu = g.id(iu, j, k), c2 = g.id2(iu, j);
+      const double mu = 0.5 * (m.mapf[c2 - 1] + m.mapf[c2]);
+      const double thu = 0.5 * (L.thv(iu - 1, j, k) + L.thv(iu, j, k));
+      const double grad = (L.pstar(iu, j, k) - L.pstar(iu - 1, j, k)) * g.rdx;
+      atomicAdd(&A.u_src[cu], ob);
+      atomicAdd(&A.tu[cu], dtau * ob);
+      const double x = -dtau * kCp * mu * ob;
+      const double thb_ = 0.5 * x * grad;
+      L.ad_thv(A, iu - 1, j, k, thb_);
+      L.ad_thv(A, iu, j, k, thb_);
+      const double gb = x * thu * g.rdx;
+      L.ad_pstar(A, iu, j, k, gb);
+      L.ad_pstar(A, iu - 1, j, k, -gb);
+    }
+  }
+  if (i < g.nx) {
+    const double ob = A.v_dst[c];
+    if (ob != 0.0) {
+      const int jv = j < g.ny ? j : 0;
+      const int cv = g.id(i, jv, k), c2 = g.id2(i, jv);
+      const double mv = 0.5 * (m.mapf[c2 - g.NX] + m.mapf[c2]);
+      const double thv_ = 0.5 * (L.thv(i, jv - 1, k) + L.thv(i, jv, k));
+      const double grad = (L.pstar(i, jv, k) - L.pstar(i, jv - 1, k)) * g.rdy;
+      atomicAdd(&A.v_src[cv], ob);
+      atomicAdd(&A.tv[cv], dtau * ob);
+      const double x = -dtau * kCp * mv * ob;
+      const double thb_ = 0.5 * x * grad;
+      L.ad_thv(A, i, jv - 1, k, thb_);
+      L.ad_thv(A, i, jv, k, thb_);
+      const double gb = x * thv_ * g.rdy;
+      L.ad_pstar(A, i, jv, k, gb);
+      L.ad_pstar(A, i, jv - 1, k, -gb);
+    }
+  }
+}
+
+// ------------------------------------------------ vertically implicit solve
+// One thread per column, levels in local arrays (nz < MAXL). The forward
+// column (kernels.cu:531-645 with flat terrain and no moisture) is
+// recomputed from the trajectory, then linearised.
+struct WPTraj {
+  const double *u, *v;  // after this substep's horizontal step
+  const double *w, *pip, *th;
+};
+struct WPTan {  // TL: perturbations; AD: adjoints
+  double *u, *v, *w, *pip, *th, *tw, *tth;  // inputs
+  double *w_out, *pip_out, *th_out;         // outputs
+};
+
+template <int MAXL>
+struct WPCol {
+  double P[MAXL], E[MAXL], G[MAXL], R[MAXL], Av[MAXL + 1], W[MAXL + 1];
+  double A[MAXL], B[MAXL], C[MAXL], D[MAXL], cs[MAXL], ds[MAXL];
+};
+
+// Forward column: assembly, rows and the Thomas solve.
+template <int MAXL>
+__device__ void wp_forward(const Grid& g, const Params& p, const Base<double>& b,
+                           const Metric<double>& m, const Coef<double>& cf, const WPTraj& X,
+                           int c0, double mc, double dtau, WPCol<MAXL>& s) {
+  const int NX = g.NX, PL = g.plane, nz = g.nz;
+  const double c1 = (1.0 + p.beta) * 0.5, c2 = (1.0 - p.beta) * 0.5;
+  for (int k = 0; k <= nz; ++k) s.Av[k] = (k <= 0 || k >= nz) ? 0.0 : cf.aw[c0 + k * PL];
+  for (int k = 0; k < nz; ++k) {
+    const int c = c0 + k * PL, kk = k + g.ng;
+    const double Kt = cf.kt3[c];
+    const double Dh = mc *
+                      ((cf.rtjx[c + 1] * X.u[c + 1] - cf.rtjx[c] * X.u[c]) * g.rdx +
+                       (cf.rtjy[c + NX] * X.v[c + NX] - cf.rtjy[c] * X.v[c]) * g.rdy);
+    const double rdzc = m.rdzc[kk];
+    s.P[k] = X.pip[c] - dtau * Kt * (Dh + c2 * (s.Av[k + 1] * X.w[c + PL] - s.Av[k] * X.w[c]) * rdzc);
+    s.E[k] = dtau * Kt * c1 * rdzc;
+    if (k >= 1) {
+      const double rJdzw = m.rdzw[kk];  // J = 1
+      const double thvw = b.thbw[c] + 0.5 * (X.th[c - PL] + X.th[c]);
+      s.G[k] = dtau * kCp * thvw * c1 * rJdzw;
+      const double buoy = kGrav * 0.5 * (X.th[c - PL] + X.th[c]) / b.thbw[c];
+      s.R[k] = X.w[c] + dtau * buoy - dtau * kCp * thvw * c2 * (X.pip[c] - X.pip[c - PL]) * rJdzw;
+    }
+  }
+  for (int k = 1; k < nz; ++k) {
+    s.A[k] = -s.G[k] * s.E[k - 1] * s.Av[k - 1];
+    s.B[k] = 1.0 + s.G[k] * s.Av[k] * (s.E[k] + s.E[k - 1]);
+    s.C[k] = -s.G[k] * s.E[k] * s.Av[k + 1];
+    s.D[k] = s.R[k] - s.G[k] * (s.P[k] - s.P[k - 1]);
+  }
+  // the tendency terms enter R (tw) and nothing else of the matrix: the
+  // caller adds dtau tw to R before the solve through D (see below)
+}
+
+template <int MAXL>
+__device__ void thomas(int nz, const double* A, const double* B, const double* C, const double* D,
+                       double* cs, double* ds, double* W) {
+  for (int k = 1; k < nz; ++k) {
+    const double mm = k == 1 ? B[k] : B[k] - A[k] * cs[k - 1];
+    cs[k] = C[k] / mm;
+    ds[k] = (k == 1 ? D[k] : D[k] - A[k] * ds[k - 1]) / mm;
+  }
+  W[nz] = 0.0;
+  for (int k = nz - 1; k >= 1; --k) W[k] = ds[k] - cs[k] * W[k + 1];
+  W[0] = 0.0;
+}
+
+template <int MAXL>
+__global__ void __launch_bounds__(64) k_tl_wpi(Grid g, Params p, Base<double> b,
+                                               Metric<double> m, Coef<double> cf, WPTraj X,
+                                               const double* tw, WPTan D, double dtau) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= g.nx || j >= g.ny) return;
+  const int NX = g.NX, PL = g.plane, nz = g.nz;
+  const int c0 = g.id(i, j, 0);
+  const double c1 = (1.0 + p.beta) * 0.5, c2 = (1.0 - p.beta) * 0.5;
+  WPCol<MAXL> s;
+  const double mc = m.mapf[g.id2(i, j)];
+  wp_forward<MAXL>(g, p, b, m, cf, X, c0, mc, dtau, s);
+  for (int k = 1; k < nz; ++k) s.D[k] += dtau * tw[c0 + k * PL];
+  thomas<MAXL>(nz, s.A, s.B, s.C, s.D, s.cs, s.ds, s.W);
+  // perturbations of P, G, R and the rows
+  double dP[MAXL], dG[MAXL], dR[MAXL], dD[MAXL], dW[MAXL + 1];
+  for (int k = 0; k < nz; ++k) {
+    const int c = c0 + k * PL, kk = k + g.ng;
+    const double Kt = cf.kt3[c];
+    const double dDh = mc *
+                       ((cf.rtjx[c + 1] * D.u[c + 1] - cf.rtjx[c] * D.u[c]) * g.rdx +
+                        (cf.rtjy[c + NX] * D.v[c + NX] - cf.rtjy[c] * D.v[c]) * g.rdy);
+    const double rdzc = m.rdzc[kk];
+    dP[k] = D.pip[c] - dtau * Kt * (dDh + c2 * (s.Av[k + 1] * D.w[c + PL] - s.Av[k] * D.w[c]) * rdzc);
+    dG[k] = 0.0;
+    dR[k] = 0.0;
+    if (k >= 1) {
+      const double rJdzw = m.rdzw[kk];
+      const double thvw = b.thbw[c] + 0.5 * (X.th[c - PL] + X.th[c]);
+      const double dthvw = 0.5 * (D.th[c - PL] + D.th[c]);
+      dG[k] = dtau * kCp * dthvw * c1 * rJdzw;
+      const double dbuoy = kGrav * 0.5 * (D.th[c - PL] + D.th[c]) / b.thbw[c];
+      const double dp = X.pip[c] - X.pip[c - PL], ddp = D.pip[c] - D.pip[c - PL];
+      dR[k] = D.w[c] + dtau * (dbuoy + D.tw[c]) -
+              dtau * kCp * c2 * rJdzw * (dthvw * dp + thvw * ddp);
+    }
+  }
+  // A dW = dD - dA W_{k-1} - dB W_k - dC W_{k+1}
+  for (int k = 1; k < nz; ++k) {
+    const double dA = -dG[k] * s.E[k - 1] * s.Av[k - 1];
+    const double dB = dG[k] * s.Av[k] * (s.E[k] + s.E[k - 1]);
+    const double dC = -dG[k] * s.E[k] * s.Av[k + 1];
+    const double dDk = dR[k] - dG[k] * (s.P[k] - s.P[k - 1]) - s.G[k] * (dP[k] - dP[k - 1]);
+    dD[k] = dDk - dA * s.W[k - 1] - dB * s.W[k] - dC * s.W[k + 1];
+  }
+  double cs2[MAXL], ds2[MAXL];
+  thomas<MAXL>(nz, s.A, s.B, s.C, dD, cs2, ds2, dW);
+  for (int k = 0; k < nz; ++k) {
+    const int c = c0 + k * PL;
+    D.pip_out[c] = dP[k] - s.E[k] * (s.Av[k + 1] * dW[k + 1] - s.Av[k] * dW[k]);
+    D.th_out[c] = D.th[c] + dtau * (D.tth[c] - 0.5 * (dW[k] + dW[k + 1]) * b.dthbdz[c]);
+    D.w_out[c] = dW[k];
+  }
+}
+
+template <int MAXL>
+__global__ void __launch_bounds__(64) k_ad_wpi(Grid g, Params p, Base<double> b,
+                                               Metric<double> m, Coef<double> cf, WPTraj X,
+                                               const double* tw, WPTan A, double dtau) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= g.nx || j >= g.ny) return;
+  const int NX = g.NX, PL = g.plane, nz = g.nz;
+  const int c0 = g.id(i, j, 0);
+  const double c1 = (1.0 + p.beta) * 0.5, c2 = (1.0 - p.beta) * 0.5;
+  WPCol<MAXL> s;
+  const double mc = m.mapf[g.id2(i, j)];
+  wp_forward<MAXL>(g, p, b, m, cf, X, c0, mc, dtau, s);
+  for (int k = 1; k < nz; ++k) s.D[k] += dtau * tw[c0 + k * PL];
+  thomas<MAXL>(nz, s.A, s.B, s.C, s.D, s.cs, s.ds, s.W);
+  // outputs -> W, P
+  double Wb[MAXL + 1], Pb[MAXL], Gb[MAXL], Rb[MAXL];
+  for (int k = 0; k <= nz; ++k) Wb[k] = 0.0;
+  for (int k = 0; k < nz; ++k) {
+    const int c = c0 + k * PL;
+    const double pb = A.pip_out[c], tb = A.th_out[c], wb = A.w_out[c];
+    A.pip_out[c] = 0.0;
+    A.th_out[c] = 0.0;
+    A.w_out[c] = 0.0;
+    Pb[k] = pb;
+    Wb[k + 1] += -pb * s.E[k] * s.Av[k + 1];
+    Wb[k] += pb * s.E[k] * s.Av[k];
+    A.th[c] += tb;
+    A.tth[c] += dtau * tb;
+    const double x = -0.5 * dtau * b.dthbdz[c] * tb;
+    Wb[k] += x;
+    Wb[k + 1] += x;
+    Wb[k] += wb;
+    Gb[k] = 0.0;
+    Rb[k] = 0.0;
+  }
+  // W_k (k = 1..nz-1) solves the rows; W_0 and W_nz are fixed.
+  // lambda = A^{-T} Wb over rows 1..nz-1: transposed system has sub-diagonal
+  // C_{k-1}, diagonal B_k, super-diagonal A_{k+1}.
+  double lam[MAXL], cs2[MAXL], ds2[MAXL];
+  for (int k = 1; k < nz; ++k) {
+    const double sub = k == 1 ? 0.0 : s.C[k - 1];
+    const double mm = k == 1 ? s.B[k] : s.B[k] - sub * cs2[k - 1];
+    const double sup = k + 1 < nz ? s.A[k + 1] : 0.0;
+    cs2[k] = sup / mm;
+    ds2[k] = (k == 1 ? Wb[k] : Wb[k] - sub * ds2[k - 1]) / mm;
+  }
+  for (int k = nz - 1; k >= 1; --k) lam[k] = ds2[k] - (k + 1 < nz ? cs2[k] * lam[k + 1] : 0.0);
+  // rows -> G, R, P (Db = lam; Ab = -lam W_{k-1}, Bb = -lam W_k, Cb = -lam W_{k+1})
+  for (int k = 1; k < nz; ++k) {
+    const double Db = lam[k];
+    const double Ab = -lam[k] * s.W[k - 1], Bb = -lam[k] * s.W[k], Cb = -lam[k] * s.W[k + 1];
+    Gb[k] += Ab * (-s.E[k - 1] * s.Av[k - 1]) + Bb * s.Av[k] * (s.E[k] + s.E[k - 1]) +
+             Cb * (-s.E[k] * s.Av[k + 1]) - Db * (s.P[k] - s.P[k - 1]);
+    Rb[k] += Db;
+    A.tw[c0 + k * PL] += dtau * Db;
+    Pb[k] += -Db * s.G[k];
+    Pb[k - 1] += Db * s.G[k];
+  }
+  // G, R, P -> inputs
+  for (int k = 0; k < nz; ++k) {
+    const int c = c0 + k * PL, kk = k + g.ng;
+    const double Kt = cf.kt3[c];
+    const double rdzc = m.rdzc[kk];
+    // P_k = pip - dtau Kt (Dh + c2 (Av_{k+1} w_{k+1} - Av_k w_k) rdzc)
+    A.pip[c] += Pb[k];
+    const double Dhb = -dtau * Kt * Pb[k] * mc;
+    A.w[c + PL] += Dhb * c2 * s.Av[k + 1] * rdzc;
+    A.w[c] += -Dhb * c2 * s.Av[k] * rdzc;
+    atomicAdd(&A.u[c + 1], Dhb * cf.rtjx[c + 1] * g.rdx);
+    atomicAdd(&A.u[c], -Dhb * cf.rtjx[c] * g.rdx);
+    atomicAdd(&A.v[c + NX], Dhb * cf.rtjy[c + NX] * g.rdy);
+    atomicAdd(&A.v[c], -Dhb * cf.rtjy[c] * g.rdy);
+    if (k >= 1) {
+      const double rJdzw = m.rdzw[kk];
+      const double thvw = b.thbw[c] + 0.5 * (X.th[c - PL] + X.th[c]);
+      const double dp = X.pip[c] - X.pip[c - PL];
+      // G = dtau cp thvw c1 rJdzw; R = w + dtau g 0.5 (thm + thk)/thbw - dtau cp thvw c2 dp rJdzw
+      const double thvwb = Gb[k] * dtau * kCp * c1 * rJdzw - Rb[k] * dtau * kCp * c2 * dp * rJdzw;
+      const double thsum_b = 0.5 * thvwb + Rb[k] * dtau * kGrav * 0.5 / b.thbw[c];
+      A.th[c - PL] += thsum_b;
+      A.th[c] += thsum_b;
+      A.w[c] += Rb[k];
+      const double dpb = -Rb[k] * dtau * kCp * thvw * c2 * rJdzw;
+      A.pip[c] += dpb;
+      A.pip[c - PL] -= dpb;
+    }
+  }
+}
+
+// ------------------------------------------------------------ objective
+// y += a x over n points.
+__global__ void k_axpy(int n, double a, const double* __restrict__ x, double* y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += a * x[i];
+}
+
+// J += scale sum_{i,j} wgt(i,j) 0.5 (u^2 + v^2) at the lowest level.
+__global__ void k_ke_objective(Grid g, const double* __restrict__ u, const double* __restrict__ v,
+                               const double* __restrict__ wgt, double scale, double* J) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= g.nx || j >= g.ny) return;
+  const double w = wgt[g.id2(i, j)];
+  if (w == 0.0) return;
+  const int c = g.id(i, j, 0);
+  atomicAdd(J, scale * w * 0.5 * (u[c] * u[c] + v[c] * v[c]));
+}
+
+// Its gradient: ub += scale wgt u, vb += scale wgt v at the lowest level.
+__global__ void k_ke_gradient(Grid g, const double* __restrict__ u, const double* __restrict__ v,
+                              const double* __restrict__ wgt, double scale, double* ub,
+                              double* vb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= g.nx || j >= g.ny) return;
+  const double w = wgt[g.id2(i, j)];
+  if (w == 0.0) return;
+  const int c = g.id(i, j, 0);
+  ub[c] += scale * w * u[c];
+  vb[c] += scale * w * v[c];
+}
+
+}  // namespace wfe::dev

@@ -18,6 +18,7 @@
 #include "hip/acoustic_kernels.hpp"
 #include "hip/kessler_kernels.hpp"
 #include "hip/slow_kernels.hpp"
+#include "hip/adjoint_kernels.hpp"
 
 namespace wfe {
 
@@ -253,6 +254,8 @@ struct CtxBase {
   virtual void acoustic_substeps(double dtau, int nsub) = 0;
   virtual void kessler(double dt) = 0;
   virtual size_t device_bytes() const = 0;
+  virtual size_t host_n3() const = 0;  // reference-layout element counts
+  virtual size_t host_n2() const = 0;
   virtual void kernel_timing(int enable) = 0;
   virtual std::string kernel_report() = 0;
   virtual double kernel_bench(const std::string& name, int reps) = 0;
@@ -268,8 +271,11 @@ struct CtxBase {
   virtual void enqueue_step(double dt) = 0;
 };
 
+class Adjoint;
+
 template <class T>
 class Ctx final : public CtxBase {
+  friend class Adjoint;
  public:
   Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& base, const wfe_metric& met);
   ~Ctx() override {
@@ -298,6 +304,8 @@ class Ctx final : public CtxBase {
   void acoustic_substeps(double dtau, int nsub) override;
   void kessler(double dt) override { kessler_step(S_[cur_], dt); }
   size_t device_bytes() const override { return arena_.bytes(); }
+  size_t host_n3() const override { return hn3_; }
+  size_t host_n2() const override { return hn2_; }
   void kernel_timing(int enable) override {
     WFE_HIP(hipStreamSynchronize(stream_));
     if (enable == 2) timer_.reset();
@@ -1180,6 +1188,442 @@ void Ctx<T>::acoustic_substeps(double dtau, int nsub) {
   cur_ ^= 1;
 }
 
+// ------------------------------------------------------------------ adjoint
+// Tangent-linear and adjoint of the dry large step (FP64 contexts): the
+// forward step is re-run with every stage estimate, tendency, mass flux and
+// substep output kept (the trajectory), then the TL kernels run forward or
+// the AD kernels backward through the same sequence. Gradients over many
+// steps keep the state every `ckpt` steps and recompute each segment.
+class Adjoint {
+ public:
+  struct Set5 {
+    double *u = nullptr, *v = nullptr, *w = nullptr, *th = nullptr, *pip = nullptr;
+  };
+  struct Set4 {
+    double *a = nullptr, *b = nullptr, *c = nullptr, *d = nullptr;
+  };
+  static constexpr int kMaxSub = 16;
+
+  explicit Adjoint(Ctx<double>& f) : F(f) {
+    const auto& hp = F.hp_;
+    std::string why;
+    if (hp.moisture) why = "moisture is on";
+    else if (hp.bc_x_open || hp.bc_y_open) why = "lateral boundaries are not periodic";
+    else if (F.p_.terrain) why = "terrain slopes are non-zero";
+    else if (hp.diff_K != 0) why = "diffusion is on";
+    else if (hp.w_damping) why = "w damping is on";
+    else if (F.bdy_active_) why = "boundary relaxation is on";
+    else if (F.g_.nz + 1 > 64) why = "more than 63 levels";
+    else if (std::max(1, hp.acoustic_ns) > kMaxSub) why = "more than 16 acoustic substeps";
+    if (why.empty()) {
+      std::vector<double> fc(F.n2_);
+      WFE_HIP(hipMemcpy(fc.data(), F.met_.fcor, F.n2_ * sizeof(double), hipMemcpyDeviceToHost));
+      for (double x : fc)
+        if (x != 0) why = "Coriolis is on";
+    }
+    if (!why.empty()) throw std::runtime_error("adjoint: scope is the dry periodic flat step (" + why + ")");
+    ns_ = std::max(1, hp.acoustic_ns);
+    nsub_[0] = 1;
+    nsub_[1] = std::max(1, ns_ / 2);
+    nsub_[2] = ns_;
+    const size_t n = F.npts_;
+    auto set5 = [&](Set5& s) {
+      s.u = A.alloc<double>(n);
+      s.v = A.alloc<double>(n);
+      s.w = A.alloc<double>(n);
+      s.th = A.alloc<double>(n);
+      s.pip = A.alloc<double>(n);
+    };
+    auto set4 = [&](Set4& s) {
+      s.a = A.alloc<double>(n);
+      s.b = A.alloc<double>(n);
+      s.c = A.alloc<double>(n);
+      s.d = A.alloc<double>(n);
+    };
+    for (int m = 0; m < 3; ++m) {
+      set5(est_[m]);
+      set4(mf_[m]);
+      set4(T_[m]);
+      set5(dest_[m]);
+      set4(dmf_[m]);
+      set4(dT_[m]);
+      for (int s = 0; s < nsub_[m]; ++s) {
+        set5(sub_[m][s]);
+        set5(dsub_[m][s]);
+      }
+    }
+    set5(x0_);
+    set5(xb_);
+    set5(tmp_);
+    zero_q_ = A.alloc<double>(n);
+    jsum_ = A.alloc<double>(1);
+    wgt_ = A.alloc<double>(F.n2_);
+  }
+
+  // Forward step of F from its current state, keeping the trajectory.
+  void record_step(double dt) {
+    Ctx<double>& f = F;
+    hipStream_t st = f.stream_;
+    if (!f.tuned_) f.tune_acoustic(dt / ns_);
+    const StateSet<double> sn = f.S_[f.cur_];
+    f.fill_ghosts(sn);
+    copy5(est_[0], as5(sn), st);
+    const double sdt[3] = {dt / 3, dt / 2, dt};
+    for (int m = 0; m < 3; ++m) {
+      if (m > 0) {
+        copy5(est_[m], sub_[m - 1][nsub_[m - 1] - 1], st);
+        f.fill_ghosts(est_set(m, sn));
+      }
+      f.slow_tendencies(est_set(m, sn), dt);
+      copy(mf_[m].a, f.mfx_, st);
+      copy(mf_[m].b, f.mfy_, st);
+      copy(mf_[m].c, f.mfz_, st);
+      copy(mf_[m].d, f.div_, st);
+      copy(T_[m].a, f.tu_, st);
+      copy(T_[m].b, f.tv_, st);
+      copy(T_[m].c, f.tw_, st);
+      copy(T_[m].d, f.tth_, st);
+      const double dtau = sdt[m] / nsub_[m];
+      for (int s = 0; s < nsub_[m]; ++s) {
+        const Set5 src = s == 0 ? as5(sn) : sub_[m][s - 1];
+        const Set5& dst = sub_[m][s];
+        dev::FusedArgs<double> a{};
+        a.uv.u_src = src.u;
+        a.uv.v_src = src.v;
+        a.uv.u_dst = dst.u;
+        a.uv.v_dst = dst.v;
+        a.uv.pcur = src.pip;
+        a.uv.pprev = s <= 1 ? sn.pip : sub_[m][s - 2].pip;
+        a.uv.pn = sn.pip;
+        a.uv.thp_cur = src.th;
+        a.uv.thp_n = sn.thp;
+        a.uv.qv = zero_q_;
+        a.uv.tu = T_[m].a;
+        a.uv.tv = T_[m].b;
+        a.uv.dtau = dtau;
+        a.w_src = src.w;
+        a.w_dst = dst.w;
+        a.pip_dst = dst.pip;
+        a.thp_dst = dst.th;
+        a.tw = T_[m].c;
+        a.tth = T_[m].d;
+        a.tpi = nullptr;
+        a.qc = zero_q_;
+        a.qr = zero_q_;
+        f.methods_[f.method_].run(a);
+      }
+    }
+    copy5(as5(f.S_[f.cur_ ^ 1]), sub_[2][nsub_[2] - 1], st);
+    f.cur_ ^= 1;
+    check_launch("adjoint record");
+  }
+
+  // dy = TL(dx) around the recorded step (device buffers).
+  void tl(double dt, const Set5& dx, const Set5& dy) {
+    Ctx<double>& f = F;
+    hipStream_t st = f.stream_;
+    const double sdt[3] = {dt / 3, dt / 2, dt};
+    copy5(dest_[0], dx, st);
+    fill5(dest_[0]);
+    for (int m = 0; m < 3; ++m) {
+      if (m > 0) {
+        copy5(dest_[m], dsub_[m - 1][nsub_[m - 1] - 1], st);
+        fill5(dest_[m]);
+      }
+      {
+        dim3 grd(cdiv(f.g_.nx + 5, 64), cdiv(f.g_.ny + 5, 4), (unsigned)f.g_.nz);
+        dev::k_massflux<double><<<grd, dim3(64, 4), 0, st>>>(f.g_, f.base_, f.met_, dest_[m].u,
+                                                             dest_[m].v, dest_[m].w, dmf_[m].a,
+                                                             dmf_[m].b, dmf_[m].c, dmf_[m].d);
+      }
+      dev::k_tl_tend<<<grid3(), dim3(64, 4), 0, st>>>(f.g_, f.base_, f.met_, traj(m), pert(m),
+                                                       tend(dT_[m]));
+      const double dtau = sdt[m] / nsub_[m];
+      for (int s = 0; s < nsub_[m]; ++s) {
+        const Set5 xs = s == 0 ? est_[0] : sub_[m][s - 1];
+        const Set5 ds = s == 0 ? dest_[0] : dsub_[m][s - 1];
+        dev::UVTraj X{xs.pip, s <= 1 ? est_[0].pip : sub_[m][s - 2].pip, est_[0].pip, xs.th,
+                      est_[0].th};
+        dev::UVTan D{ds.u, ds.v, dsub_[m][s].u, dsub_[m][s].v, ds.pip,
+                     s <= 1 ? dest_[0].pip : dsub_[m][s - 2].pip, dest_[0].pip, ds.th,
+                     dest_[0].th, dT_[m].a, dT_[m].b};
+        dev::k_tl_uv<<<grid_uv(), dim3(64, 4), 0, st>>>(f.g_, f.p_, f.base_, f.met_, X, D, dtau);
+        dev::WPTraj W{sub_[m][s].u, sub_[m][s].v, xs.w, xs.pip, xs.th};
+        dev::WPTan DW{dsub_[m][s].u, dsub_[m][s].v, ds.w, ds.pip, ds.th, dT_[m].c, dT_[m].d,
+                      dsub_[m][s].w, dsub_[m][s].pip, dsub_[m][s].th};
+        dev::k_tl_wpi<64><<<grid_col(), 64, 0, st>>>(f.g_, f.p_, f.base_, f.met_, f.coef_, W,
+                                                       T_[m].c, DW, dtau);
+      }
+    }
+    copy5(dy, dsub_[2][nsub_[2] - 1], st);
+    check_launch("adjoint tl");
+  }
+
+  // xb = AD(yb) around the recorded step (device buffers; yb is read only).
+  void ad(double dt, const Set5& yb, const Set5& xb) {
+    Ctx<double>& f = F;
+    hipStream_t st = f.stream_;
+    const double sdt[3] = {dt / 3, dt / 2, dt};
+    for (int m = 0; m < 3; ++m) {
+      zero5(dest_[m]);
+      zero4(dmf_[m]);
+      zero4(dT_[m]);
+      for (int s = 0; s < nsub_[m]; ++s) zero5(dsub_[m][s]);
+    }
+    copy5(dsub_[2][nsub_[2] - 1], yb, st);
+    for (int m = 2; m >= 0; --m) {
+      const double dtau = sdt[m] / nsub_[m];
+      for (int s = nsub_[m] - 1; s >= 0; --s) {
+        const Set5 xs = s == 0 ? est_[0] : sub_[m][s - 1];
+        const Set5 ds = s == 0 ? dest_[0] : dsub_[m][s - 1];
+        dev::WPTraj W{sub_[m][s].u, sub_[m][s].v, xs.w, xs.pip, xs.th};
+        dev::WPTan AW{dsub_[m][s].u, dsub_[m][s].v, ds.w, ds.pip, ds.th, dT_[m].c, dT_[m].d,
+                      dsub_[m][s].w, dsub_[m][s].pip, dsub_[m][s].th};
+        dev::k_ad_wpi<64><<<grid_col(), 64, 0, st>>>(f.g_, f.p_, f.base_, f.met_, f.coef_, W,
+                                                       T_[m].c, AW, dtau);
+        dev::UVTraj X{xs.pip, s <= 1 ? est_[0].pip : sub_[m][s - 2].pip, est_[0].pip, xs.th,
+                      est_[0].th};
+        dev::UVTan AU{ds.u, ds.v, dsub_[m][s].u, dsub_[m][s].v, ds.pip,
+                      s <= 1 ? dest_[0].pip : dsub_[m][s - 2].pip, dest_[0].pip, ds.th,
+                      dest_[0].th, dT_[m].a, dT_[m].b};
+        dev::k_ad_uv<<<grid_uv(), dim3(64, 4), 0, st>>>(f.g_, f.p_, f.base_, f.met_, X, AU, dtau);
+      }
+      dev::k_ad_tend<<<grid3(), dim3(64, 4), 0, st>>>(f.g_, f.base_, f.met_, traj(m), pert(m),
+                                                       tend(dT_[m]));
+      {
+        dim3 grd(cdiv(f.g_.nx + 5, 64), cdiv(f.g_.ny + 5, 4), (unsigned)f.g_.nz);
+        dev::k_ad_massflux<<<grd, dim3(64, 4), 0, st>>>(f.g_, f.met_, dmf_[m].a, dmf_[m].b,
+                                                        dmf_[m].c, dmf_[m].d);
+        dev::k_ad_massflux_uvw<<<grd, dim3(64, 4), 0, st>>>(f.g_, f.base_, f.met_, dest_[m].u,
+                                                            dest_[m].v, dest_[m].w, dmf_[m].a,
+                                                            dmf_[m].b, dmf_[m].c);
+      }
+      if (m > 0) {
+        ad_fill5(dest_[m]);
+        add5(dsub_[m - 1][nsub_[m - 1] - 1], dest_[m]);
+      }
+    }
+    ad_fill5(dest_[0]);
+    copy5(xb, dest_[0], st);
+    check_launch("adjoint ad");
+  }
+
+  // Single-step TL / AD through host arrays (reference layout), around the
+  // context's current state, which is left unchanged.
+  void tl_step_host(double dt, const double* const* dx, double* const* dy) {
+    save_state(x0_);
+    record_step(dt);
+    upload5(tmp_, dx);
+    tl(dt, tmp_, xb_);
+    download5(xb_, dy);
+    restore_state(x0_);
+  }
+  void ad_step_host(double dt, const double* const* yb, double* const* xb) {
+    save_state(x0_);
+    record_step(dt);
+    upload5(tmp_, yb);
+    ad(dt, tmp_, xb_);
+    download5(xb_, xb);
+    restore_state(x0_);
+  }
+
+  // J = (1/N) sum_{n=1..N} sum_{i,j} wgt(i,j) 0.5 (u^2 + v^2) at the lowest
+  // level, and its gradient with respect to the initial state.
+  double gradient(double dt, int nsteps, int ckpt, const double* wgt2d, double* const* grad) {
+    if (nsteps < 1 || ckpt < 1) throw ArgError("adjoint gradient: nsteps, ckpt must be >= 1");
+    Ctx<double>& f = F;
+    hipStream_t st = f.stream_;
+    {
+      const std::vector<double> w = f.to_dev<double>(wgt2d, false);
+      WFE_HIP(hipMemcpy(wgt_, w.data(), f.n2_ * sizeof(double), hipMemcpyHostToDevice));
+    }
+    const double scale = 1.0 / nsteps;
+    DevArena B;  // checkpoints and segment states, freed on return
+    save_state(x0_);
+    const int nck = (nsteps + ckpt - 1) / ckpt;
+    std::vector<Set5> cks(nck);
+    for (auto& c : cks) {
+      c.u = B.alloc<double>(f.npts_);
+      c.v = B.alloc<double>(f.npts_);
+      c.w = B.alloc<double>(f.npts_);
+      c.th = B.alloc<double>(f.npts_);
+      c.pip = B.alloc<double>(f.npts_);
+    }
+    std::vector<Set5> seg(ckpt);
+    for (auto& c : seg) {
+      c.u = B.alloc<double>(f.npts_);
+      c.v = B.alloc<double>(f.npts_);
+      c.w = B.alloc<double>(f.npts_);
+      c.th = B.alloc<double>(f.npts_);
+      c.pip = B.alloc<double>(f.npts_);
+    }
+    // forward sweep: checkpoints and J
+    WFE_HIP(hipMemsetAsync(jsum_, 0, sizeof(double), st));
+    for (int n = 0; n < nsteps; ++n) {
+      if (n % ckpt == 0) save_state(cks[n / ckpt]);
+      record_step(dt);
+      objective(as5(f.S_[f.cur_]), scale);
+    }
+    double J = 0;
+    WFE_HIP(hipMemcpyAsync(&J, jsum_, sizeof(double), hipMemcpyDeviceToHost, st));
+    // backward sweep
+    zero5(xb_);
+    for (int c = nck - 1; c >= 0; --c) {
+      const int n0 = c * ckpt, n1 = std::min(nsteps, n0 + ckpt);
+      restore_state(cks[c]);
+      for (int n = n0; n < n1; ++n) {
+        save_state(seg[n - n0]);
+        record_step(dt);
+      }
+      for (int n = n1 - 1; n >= n0; --n) {
+        // dJ/dx_{n+1} from the state after step n
+        restore_state(seg[n - n0]);
+        record_step(dt);
+        objective_grad(as5(f.S_[f.cur_]), xb_, scale);
+        copy5(tmp_, xb_, st);
+        ad(dt, tmp_, xb_);
+      }
+    }
+    download5(xb_, grad);
+    restore_state(x0_);
+    WFE_HIP(hipStreamSynchronize(st));
+    return J;
+  }
+
+ private:
+  static Set5 as5(const StateSet<double>& s) { return Set5{s.u, s.v, s.w, s.thp, s.pip}; }
+  StateSet<double> est_set(int m, const StateSet<double>& sn) const {
+    StateSet<double> e = sn;
+    e.u = est_[m].u;
+    e.v = est_[m].v;
+    e.w = est_[m].w;
+    e.thp = est_[m].th;
+    e.pip = est_[m].pip;
+    return e;
+  }
+  dev::TanTraj traj(int m) const {
+    return dev::TanTraj{est_[m].u, est_[m].v, est_[m].w, est_[m].th,
+                        mf_[m].a, mf_[m].b, mf_[m].c, mf_[m].d};
+  }
+  dev::TanPert pert(int m) const {
+    return dev::TanPert{dest_[m].u, dest_[m].v, dest_[m].w, dest_[m].th,
+                        dmf_[m].a, dmf_[m].b, dmf_[m].c, dmf_[m].d};
+  }
+  static dev::TanTend tend(const Set4& t) { return dev::TanTend{t.a, t.b, t.c, t.d}; }
+  dim3 grid3() const { return dim3(cdiv(F.g_.nx, 64), cdiv(F.g_.ny, 4), (unsigned)F.g_.nz); }
+  dim3 grid_uv() const {
+    return dim3(cdiv(F.g_.nx + 1, 64), cdiv(F.g_.ny + 1, 4), (unsigned)F.g_.nz);
+  }
+  dim3 grid_col() const { return dim3(cdiv(F.g_.nx, 64), (unsigned)F.g_.ny); }
+
+  void copy(double* d, const double* s, hipStream_t st) {
+    WFE_HIP(hipMemcpyAsync(d, s, F.npts_ * sizeof(double), hipMemcpyDeviceToDevice, st));
+  }
+  void copy5(const Set5& d, const Set5& s, hipStream_t st) {
+    copy(d.u, s.u, st);
+    copy(d.v, s.v, st);
+    copy(d.w, s.w, st);
+    copy(d.th, s.th, st);
+    copy(d.pip, s.pip, st);
+  }
+  void zero(double* d) {
+    WFE_HIP(hipMemsetAsync(d, 0, F.npts_ * sizeof(double), F.stream_));
+  }
+  void zero5(const Set5& s) {
+    zero(s.u);
+    zero(s.v);
+    zero(s.w);
+    zero(s.th);
+    zero(s.pip);
+  }
+  void zero4(const Set4& s) {
+    zero(s.a);
+    zero(s.b);
+    zero(s.c);
+    zero(s.d);
+  }
+  dev::FillList<double> fill_list(const Set5& s) const {
+    dev::FillList<double> L{};
+    const int nx = F.g_.nx, ny = F.g_.ny;
+    double* f[5] = {s.u, s.v, s.w, s.th, s.pip};
+    const int kind[5] = {dev::kFillScalar, dev::kFillScalar, dev::kFillW, dev::kFillScalar,
+                         dev::kFillScalar};
+    const int xl[5] = {nx, nx - 1, nx - 1, nx - 1, nx - 1};
+    const int yl[5] = {ny - 1, ny, ny - 1, ny - 1, ny - 1};
+    for (int q = 0; q < 5; ++q) {
+      L.f[q] = f[q];
+      L.kind[q] = kind[q];
+      L.xlast[q] = xl[q];
+      L.ylast[q] = yl[q];
+    }
+    L.n = 5;
+    return L;
+  }
+  void fill5(const Set5& s) {
+    const dev::FillList<double> L = fill_list(s);
+    const dev::FillList<double> none{};
+    const int ncell = dev::ghost_count(F.g_);
+    dev::k_fill_all<double><<<cdiv(ncell, 256), 256, 0, F.stream_>>>(F.g_, F.p_.bcx, F.p_.bcy, L,
+                                                                     none);
+  }
+  void ad_fill5(const Set5& s) {
+    const dev::FillList<double> L = fill_list(s);
+    dev::AdFill A{};
+    for (int q = 0; q < 5; ++q) {
+      A.f[q] = L.f[q];
+      A.kind[q] = L.kind[q];
+      A.xlast[q] = L.xlast[q];
+      A.ylast[q] = L.ylast[q];
+    }
+    const int ncell = dev::ghost_count(F.g_);
+    dev::k_ad_fill<<<cdiv(ncell, 256), 256, 0, F.stream_>>>(F.g_, F.p_.bcx, F.p_.bcy, A);
+  }
+  void add5(const Set5& d, const Set5& s) {
+    const int n = (int)F.npts_;
+    double* dd[5] = {d.u, d.v, d.w, d.th, d.pip};
+    const double* ss[5] = {s.u, s.v, s.w, s.th, s.pip};
+    for (int q = 0; q < 5; ++q)
+      dev::k_axpy<<<cdiv(n, 256), 256, 0, F.stream_>>>(n, 1.0, ss[q], dd[q]);
+  }
+  void objective(const Set5& x, double scale) {
+    dev::k_ke_objective<<<dim3(cdiv(F.g_.nx, 64), cdiv(F.g_.ny, 4)), dim3(64, 4), 0,
+                          F.stream_>>>(F.g_, x.u, x.v, wgt_, scale, jsum_);
+  }
+  void objective_grad(const Set5& x, const Set5& xb, double scale) {
+    dev::k_ke_gradient<<<dim3(cdiv(F.g_.nx, 64), cdiv(F.g_.ny, 4)), dim3(64, 4), 0,
+                         F.stream_>>>(F.g_, x.u, x.v, wgt_, scale, xb.u, xb.v);
+  }
+  void save_state(const Set5& d) { copy5(d, as5(F.S_[F.cur_]), F.stream_); }
+  void restore_state(const Set5& s) { copy5(as5(F.S_[F.cur_]), s, F.stream_); }
+  void upload5(const Set5& d, const double* const* h) {
+    double* dd[5] = {d.u, d.v, d.w, d.th, d.pip};
+    WFE_HIP(hipStreamSynchronize(F.stream_));
+    for (int q = 0; q < 5; ++q) {
+      const std::vector<double> t = F.to_dev<double>(h[q], true);
+      WFE_HIP(hipMemcpy(dd[q], t.data(), F.npts_ * sizeof(double), hipMemcpyHostToDevice));
+    }
+  }
+  void download5(const Set5& s, double* const* h) {
+    const double* ss[5] = {s.u, s.v, s.w, s.th, s.pip};
+    WFE_HIP(hipStreamSynchronize(F.stream_));
+    std::vector<double> t(F.npts_);
+    for (int q = 0; q < 5; ++q) {
+      WFE_HIP(hipMemcpy(t.data(), ss[q], F.npts_ * sizeof(double), hipMemcpyDeviceToHost));
+      F.to_host(t, h[q], true);
+    }
+  }
+
+  Ctx<double>& F;
+  DevArena A;
+  int ns_ = 1, nsub_[3] = {1, 1, 1};
+  Set5 est_[3], dest_[3], sub_[3][kMaxSub], dsub_[3][kMaxSub];
+  Set4 mf_[3], dmf_[3], T_[3], dT_[3];
+  Set5 x0_, xb_, tmp_;
+  double* zero_q_ = nullptr;
+  double* jsum_ = nullptr;
+  double* wgt_ = nullptr;
+};
+
 }  // namespace wfe
 
 // ------------------------------------------------------------------ C ABI
@@ -1187,6 +1631,7 @@ void Ctx<T>::acoustic_substeps(double dtau, int nsub) {
 struct wfe_ctx {
   std::unique_ptr<wfe::CtxBase> impl;
   int precision = 0;
+  std::unique_ptr<wfe::Adjoint> adj;  // created on first use (FP64 contexts)
 };
 
 namespace {
@@ -1217,6 +1662,21 @@ int need_ctx(wfe_ctx* c) {
     return WFE_E_ARG;
   }
   return WFE_OK;
+}
+
+void check_n3(wfe_ctx* c, size_t n) {
+  if (n != c->impl->host_n3()) throw wfe::ArgError("3-D array element count mismatch");
+}
+void check_n2(wfe_ctx* c, size_t n) {
+  if (n != c->impl->host_n2()) throw wfe::ArgError("2-D array element count mismatch");
+}
+
+wfe::Adjoint& adjoint_of(wfe_ctx* ctx) {
+  if (ctx->precision != WFE_PREC_FP64)
+    throw wfe::ArgError("adjoint: needs a plain FP64 context (WFE_PREC_FP64)");
+  if (!ctx->adj)
+    ctx->adj = std::make_unique<wfe::Adjoint>(*static_cast<wfe::Ctx<double>*>(ctx->impl.get()));
+  return *ctx->adj;
 }
 
 }  // namespace
@@ -1260,6 +1720,45 @@ int wfe_destroy(wfe_ctx* ctx) {
 int wfe_step(wfe_ctx* ctx, double dt, double /*t*/) {
   if (int e = need_ctx(ctx)) return e;
   return guarded([&] { ctx->impl->step(dt); });
+}
+
+// ------------------------------------------------------------ adjoint
+int wfe_tl_step(wfe_ctx* ctx, double dt, const double* const* dx, double* const* dy, size_t n) {
+  if (int e = need_ctx(ctx)) return e;
+  return guarded([&] {
+    if (!dx || !dy) throw wfe::ArgError("wfe_tl_step: null field arrays");
+    for (int q = 0; q < 5; ++q)
+      if (!dx[q] || !dy[q]) throw wfe::ArgError("wfe_tl_step: null field array");
+    wfe::Adjoint& a = adjoint_of(ctx);
+    check_n3(ctx, n);
+    a.tl_step_host(dt, dx, dy);
+  });
+}
+
+int wfe_ad_step(wfe_ctx* ctx, double dt, const double* const* yb, double* const* xb, size_t n) {
+  if (int e = need_ctx(ctx)) return e;
+  return guarded([&] {
+    if (!yb || !xb) throw wfe::ArgError("wfe_ad_step: null field arrays");
+    for (int q = 0; q < 5; ++q)
+      if (!yb[q] || !xb[q]) throw wfe::ArgError("wfe_ad_step: null field array");
+    wfe::Adjoint& a = adjoint_of(ctx);
+    check_n3(ctx, n);
+    a.ad_step_host(dt, yb, xb);
+  });
+}
+
+int wfe_adj_gradient(wfe_ctx* ctx, double dt, int nsteps, int ckpt_every, const double* wgt2d,
+                     size_t n2, double* J, double* const* grad, size_t n) {
+  if (int e = need_ctx(ctx)) return e;
+  return guarded([&] {
+    if (!wgt2d || !J || !grad) throw wfe::ArgError("wfe_adj_gradient: null argument");
+    for (int q = 0; q < 5; ++q)
+      if (!grad[q]) throw wfe::ArgError("wfe_adj_gradient: null gradient array");
+    wfe::Adjoint& a = adjoint_of(ctx);
+    check_n3(ctx, n);
+    check_n2(ctx, n2);
+    *J = a.gradient(dt, nsteps, ckpt_every, wgt2d, grad);
+  });
 }
 
 // ------------------------------------------------------------ ensembles
