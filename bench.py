@@ -263,6 +263,75 @@ def ensemble_benchmark(steps: int):
         shutil.rmtree(d, ignore_errors=True)
 
 
+def adjoint_benchmark(nsteps: int, ckpt: int):
+    """Dry supercell-shape case (cases/cfg5_adjoint_dry.ini, FP64): gradient of
+    the mean lowest-level kinetic energy in a seeded 20 x 20 km box with
+    respect to the initial (u, v, w, theta', pi') through nsteps steps with a
+    checkpoint every ckpt steps; one-step dot-product test; finite-difference
+    check of the gradient along a seeded direction for eps = 1e-3 .. 1e-7."""
+    from wfe_amd import Case, Integrator
+    from wfe_amd.adjoint import FIELDS5, ad_step, box_weights, ke_gradient, tl_step
+
+    case = Case(ROOT / "cases" / "cfg5_adjoint_dry.ini")
+    g = case.grid
+    dt = case.scalar("dt")
+    rng = np.random.default_rng(20261202)
+    i0, j0 = (int(x) for x in rng.integers(10, g.nx - 30, size=2))
+    w = box_weights(g, i0, i0 + 20, j0, j0 + 20)
+    gi = Integrator.from_case(case, "fp64")
+    mask = np.zeros((g.NZ, g.NY, g.NX), dtype=bool)
+    mask[g.ng:g.ng + g.nz, g.ng:g.ng + g.ny, g.ng:g.ng + g.nx] = True
+    mask = mask.ravel()
+    scale = {"u": 1.0, "v": 1.0, "w": 0.3, "thp": 0.5, "pip": 1e-4}
+
+    def rand():
+        return {f: np.where(mask, rng.standard_normal(g.npts) * scale[f], 0.0) for f in FIELDS5}
+
+    dx, yb = rand(), rand()
+    dy = tl_step(gi, dt, dx)
+    xb = ad_step(gi, dt, yb)
+    lhs = sum(float(np.dot(dy[f][mask], yb[f][mask])) for f in FIELDS5)
+    rhs = sum(float(np.dot(dx[f][mask], xb[f][mask])) for f in FIELDS5)
+
+    gi.sync()
+    t0 = time.perf_counter()
+    J, grad = ke_gradient(gi, dt, nsteps, ckpt, w)
+    t_adj = time.perf_counter() - t0
+    x0 = {f: gi.get(f) for f in FIELDS5}
+    d = rand()
+    gd = sum(float(np.dot(grad[f][mask], d[f][mask])) for f in FIELDS5)
+
+    def J_of(xs):
+        gp = Integrator.from_case(case, "fp64")
+        for f in FIELDS5:
+            gp.set(f, xs[f])
+        tot = 0.0
+        ww = w.reshape(g.NY, g.NX)
+        for _ in range(nsteps):
+            gp.step(dt)
+            u = gp.get("u").reshape(g.NZ, g.NY, g.NX)[g.ng]
+            v = gp.get("v").reshape(g.NZ, g.NY, g.NX)[g.ng]
+            tot += float((ww * 0.5 * (u * u + v * v)).sum()) / nsteps
+        return tot
+
+    t0 = time.perf_counter()
+    J_fwd = J_of(x0)
+    t_fwd = time.perf_counter() - t0
+    fd = {}
+    for eps in (1e-3, 1e-4, 1e-5, 1e-6, 1e-7):
+        jp = J_of({f: x0[f] + eps * d[f] for f in FIELDS5})
+        jm = J_of({f: x0[f] - eps * d[f] for f in FIELDS5})
+        fd[f"{eps:.0e}"] = abs((jp - jm) / (2 * eps) - gd) / abs(gd)
+    return {"workload": "dry 120x120x40 (supercell levels), isentropic, periodic, FP64; "
+                        f"J = mean lowest-level KE in box i {i0}..{i0 + 20}, j {j0}..{j0 + 20}",
+            "steps": nsteps, "checkpoint_every": ckpt,
+            "J": J, "J_forward_check_rel": abs(J - J_fwd) / abs(J_fwd),
+            "dot_product_rel": abs(lhs - rhs) / abs(lhs),
+            "gradient_fd_rel_err": fd, "gradient_dot_direction": gd,
+            "adjoint_s": t_adj, "forward_s": t_fwd,
+            "adjoint_over_forward": t_adj / t_fwd}
+
+
 def cpu_baseline(case, budget_s: float):
     """The FP64 oracle on the host cores, on a bounded sample of steps."""
     sys.path.insert(0, str(ROOT / "tests"))
@@ -295,6 +364,7 @@ def main() -> int:
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the regional LAM and ensemble measurements")
     ap.add_argument("--extra-steps", type=int, default=30)
+    ap.add_argument("--adjoint-steps", type=int, default=600)
     ap.add_argument("--error-steps", type=int, default=600,
                     help="last checkpoint of the error-growth study (600 steps = 1 h)")
     ap.add_argument("--mixed-only", action="store_true",
@@ -371,6 +441,13 @@ def main() -> int:
         log(f"[bench] ensemble {ens['ms_per_round']:.2f} ms/round vs sequential "
             f"{ens['sequential_ms_per_round']:.2f} ms/round")
 
+    adj = None
+    if not args.no_extra:
+        log(f"[bench] adjoint: dry 120x120x40, {args.adjoint_steps} steps")
+        adj = adjoint_benchmark(args.adjoint_steps, 50)
+        log(f"[bench] adjoint {adj['adjoint_s']:.1f} s, dot product {adj['dot_product_rel']:.1e}, "
+            f"FD {adj['gradient_fd_rel_err']}")
+
     cpu = None
     if not args.no_cpu:
         log(f"[bench] cpu baseline: oracle for ~{args.cpu_budget:.0f} s")
@@ -407,6 +484,7 @@ def main() -> int:
         "kernel_solo_us": solo,
         "regional_lam": lam,
         "ensemble": ens,
+        "adjoint": adj,
         "acoustic_tuning": tune,
         "copy_probe_GBps": copy_bw,
         "cpu_baseline": cpu,
