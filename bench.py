@@ -58,7 +58,7 @@ KERNEL_BYTES = {
 }
 # Kernel symbol of each timed launch name (for the rocprofv3 PMC summary).
 KERNEL_SYMBOL = {"acou_wpi": "k_acou_wpi_tile", "acou_uv": "k_acou_uv_tile",
-                 "acoustic_fused": "k_acoustic_fused", "tendencies": "k_tendencies",
+                 "acoustic_fused": "k_acou_row", "tendencies": "k_tendencies",
                  "massflux": "k_massflux", "fill_ghosts": "k_fill_all", "stage_q": "k_stage_q",
                  "kessler": "k_kessler"}
 # HBM traffic per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE passes

@@ -176,7 +176,7 @@ def test_column_kernel_variant_matches_oracle(monkeypatch, precision, tol):
     _cmp(c.grid, om, gi, ("u", "v", "w", "thp"), tol, f"column kernel {precision}")
 
 
-@pytest.mark.parametrize("method", ["split-col", "split-32x16x4"])
+@pytest.mark.parametrize("method", ["split-col", "split-32x16x4", "fused-32x16x4"])
 def test_deep_column(monkeypatch, method):
     """nz = 60 exceeds the 48-level tiles: the 64-level tile and the
     one-thread-per-column solve."""
@@ -192,8 +192,7 @@ def test_deep_column(monkeypatch, method):
     assert f"tuned:{method}" in gi.kernel_report()
 
 
-FUSED_SHAPES = ["fused-8x2x16", "fused-16x1x16", "fused-4x4x16", "fused-8x4x8", "fused-32x1x8",
-                "fused-8x2x24"]
+FUSED_SHAPES = ["fused-64x16", "fused-64x8", "fused-32x16", "fused-32x8", "fused-32x16x4"]
 SPLIT_SHAPES = ["split-32x8", "split-32x16", "split-64x8", "split-64x16", "split-16x16",
                 "split-32x16x4", "split-32x16w4", "split-16x16w4", "split-col"]
 
@@ -203,15 +202,17 @@ def test_fused_acoustic_shapes_match_oracle(monkeypatch, shape):
     """Every tile shape of the fused substep (and the split two-kernel path)
     is in FP64 parity with the oracle over full steps, open and periodic."""
     monkeypatch.setenv("WFE_ACOUSTIC_KERNEL", shape)
-    for bc in ("open", "periodic"):
-        c = kitchen_sink(bc_x=bc, bc_y=bc, nx=21, ny=13)  # ragged against every tile
+    # ragged against every tile; the second shape spans several row blocks in x
+    dims = [(21, 13), (70, 5)] if shape.startswith("fused") else [(21, 13)]
+    for bc, (nx, ny) in [(b, d) for d in dims for b in ("open", "periodic")]:
+        c = kitchen_sink(bc_x=bc, bc_y=bc, nx=nx, ny=ny)
         st = perturbed_state(c, 31)
         om, gi = oracle_and_gpu(c, "fp64", st)
         dt = c.scalar("dt")
         for s in range(3):
             om.step(dt)
             gi.step(dt)
-        _cmp(c.grid, om, gi, STATE, 1e-10, f"{shape} {bc}")
+        _cmp(c.grid, om, gi, STATE, 1e-10, f"{shape} {bc} {nx}x{ny}")
         assert f"tuned:{shape}" in gi.kernel_report(), gi.kernel_report()
 
 
@@ -371,3 +372,31 @@ def test_ensemble_graph_matches_members_stepped_alone(monkeypatch):
         for f in STATE:
             np.testing.assert_array_equal(b.get(f), a.get(f), err_msg=f"member {q} {f}")
     del ens
+
+
+@pytest.mark.parametrize("method", ["split-64x16", "fused-64x8", "fused-32x16"])
+def test_uniform_columns_match_per_column_reads(monkeypatch, method):
+    """Over flat ground with a horizontally uniform base state the acoustic
+    kernels read base-state and coefficient profiles from one column; the
+    results are bitwise those of reading every column. Terrain turns it off."""
+    monkeypatch.setenv("WFE_ACOUSTIC_KERNEL", method)
+    from wfe_amd import Integrator
+
+    flat = kitchen_sink(terrain="none")
+    st = perturbed_state(flat, 41)
+    outs = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("WFE_UNIFORM_COLUMNS", mode)
+        gi = Integrator.from_case(flat, "mixed")
+        for f, a in st.items():
+            gi.set(f, a)
+        for s in range(3):
+            gi.step(flat.scalar("dt"))
+        assert gi.tuning()["uniform_columns"] == (mode == "1")
+        outs[mode] = {f: gi.get(f) for f in STATE}
+    for f in STATE:
+        assert np.array_equal(outs["1"][f], outs["0"][f]), f
+    monkeypatch.delenv("WFE_UNIFORM_COLUMNS")
+    gi = Integrator.from_case(kitchen_sink(), "mixed")
+    gi.step(flat.scalar("dt"))
+    assert not gi.tuning()["uniform_columns"]

@@ -187,7 +187,7 @@ class Integrator:
     def kernel_times(self) -> dict:
         """Launch count and total ms per kernel (tuning entries removed)."""
         return {k: v for k, v in self.kernel_report().items()
-                if v[0] > 0 and not k.startswith(("tuned:", "cand:"))}
+                if v[0] > 0 and not k.startswith(("tuned:", "cand:", "uniform_columns:"))}
 
     def kernel_bench(self, kernel: str, reps: int = 200) -> float:
         """Average launch time [us] of one kernel of the step, launched reps
@@ -203,7 +203,8 @@ class Integrator:
         rep = self.kernel_report()
         chosen = next((k[6:] for k in rep if k.startswith("tuned:")), None)
         return {"chosen": chosen,
-                "candidates_ms": {k[5:]: v[1] for k, v in rep.items() if k.startswith("cand:")}}
+                "candidates_ms": {k[5:]: v[1] for k, v in rep.items() if k.startswith("cand:")},
+                "uniform_columns": any(k == "uniform_columns:1" for k in rep)}
 
     # -- phase-level entry points (kernels.hpp free functions)
     def apply_bcs(self) -> None:

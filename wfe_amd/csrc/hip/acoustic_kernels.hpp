@@ -14,18 +14,18 @@ namespace wfe::dev {
 
 template <class T>
 struct UVArgs {
-  const T* u_src;
-  const T* v_src;
-  T* u_dst;
-  T* v_dst;
-  const double* pcur;   // pi' entering the substep
-  const double* pprev;  // pi' of the previous substep (time n at substep 0)
-  const double* pn;     // time-n pi' (ghost cells filled)
-  const T* thp_cur;
-  const T* thp_n;  // time-n theta' (ghost cells filled)
-  const T* qv;     // stage-constant qv (time n, ghosts filled)
-  const T* tu;
-  const T* tv;
+  CArr<T> u_src;
+  CArr<T> v_src;
+  Arr<T> u_dst;
+  Arr<T> v_dst;
+  CArr<double> pcur;   // pi' entering the substep
+  CArr<double> pprev;  // pi' of the previous substep (time n at substep 0)
+  CArr<double> pn;     // time-n pi' (ghost cells filled)
+  CArr<T> thp_cur;
+  CArr<T> thp_n;  // time-n theta' (ghost cells filled)
+  CArr<T> qv;     // stage-constant qv (time n, ghosts filled)
+  CArr<T> tu;
+  CArr<T> tv;
   T dtau;
 };
 
@@ -58,7 +58,7 @@ struct UVDev {
   __device__ __forceinline__ T thv(int ii, int jj, int k) const {
     const int c = g.id(ii, jj, k);
     const bool in = (unsigned)ii < (unsigned)g.nx && (unsigned)jj < (unsigned)g.ny;
-    const T t = b.thb[c] + (in ? a.thp_cur[c] : a.thp_n[c]);
+    const T t = b.thb[p.hom ? g.id(0, 0, k) : c] + (in ? a.thp_cur[c] : a.thp_n[c]);
     const T q = a.qv[c];  // allocated in every mode; zero weight when dry
     return t * ((T)1 + (T)kEps61 * (p.moisture ? q : (T)0));
   }
@@ -307,20 +307,20 @@ __global__ void __launch_bounds__(256) k_acou_uv_tile(Grid g, Params p, Base<T> 
 
 template <class T>
 struct WPArgs {
-  const T* u;  // after this substep's horizontal step (faces 0..nx)
-  const T* v;
-  const T* w_src;
-  const double* pip_src;
-  const T* thp_src;
-  T* w_dst;
-  double* pip_dst;
-  T* thp_dst;
-  const T* tw;
-  const T* tth;
-  const T* tpi;  // nullptr when the pi' slow tendency is zero
-  const T* qv;
-  const T* qc;
-  const T* qr;
+  CArr<T> u;  // after this substep's horizontal step (faces 0..nx)
+  CArr<T> v;
+  CArr<T> w_src;
+  CArr<double> pip_src;
+  CArr<T> thp_src;
+  Arr<T> w_dst;
+  Arr<double> pip_dst;
+  Arr<T> thp_dst;
+  CArr<T> tw;
+  CArr<T> tth;
+  CArr<T> tpi;  // nullptr when the pi' slow tendency is zero
+  CArr<T> qv;
+  CArr<T> qc;
+  CArr<T> qr;
   double dtau;
 };
 
@@ -466,20 +466,93 @@ struct WpiTile {
   static size_t lds_bytes(int nz) { return (size_t)4 * (nz + 1) * TX * sizeof(double); }
 };
 
-// 1/x for well-scaled x (pivots of a diagonally dominant system, base-state
-// theta, the Jacobian): v_rcp_f64 followed by two Newton-Raphson steps
-// r <- r + r (1 - x r), within an ulp or so of the quotient.
-__device__ __forceinline__ double rcp_nr(double x) {
-  double r = __builtin_amdgcn_rcp(x);
-  double e = fma(-x, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-x, r, 1.0);
-  return fma(r, e, r);
-}
 
 // WPE: minimum waves per SIMD the register allocation must allow (1 = no
 // constraint); WPE = 4 caps VGPRs at 128 so two 512-thread blocks share a CU
 // and one block's serial elimination overlaps the other's loads.
+// Thomas elimination of the rows k = 1 .. nz-1 of one column held in LDS
+// (row k of lane lx at k * TX + lx; C and D are overwritten by the
+// eliminated c_k, d_k), in the reference's order (kernels.cu:589-628). Rows
+// are read in chunks of four, one chunk ahead of the recurrence (indices
+// clamped into range; a clamped read is never used), so only the FP64 chain
+// is serial. On return sD holds W_k for k in [0, nz], W_0 = w0, W_nz = 0.
+template <int TX>
+__device__ __forceinline__ void thomas_lds(double* sA, double* sB, double* sC, double* sD,
+                                           int lx, int nz, double w0) {
+  constexpr int U = 4;
+  const int n1 = nz - 1;  // rows k = 1 .. nz-1
+  auto at = [&](int k) { return k * TX + lx; };
+  double A[U], B[U], C[U], D[U], nA[U], nB[U], nC[U], nD[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int r = min(1 + u, n1);
+    A[u] = sA[at(r)];
+    B[u] = sB[at(r)];
+    C[u] = sC[at(r)];
+    D[u] = sD[at(r)];
+  }
+  double cs = 0.0, ds = 0.0;
+  for (int k0 = 1; k0 <= n1; k0 += U) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int r = min(k0 + U + u, n1);
+      nA[u] = sA[at(r)];
+      nB[u] = sB[at(r)];
+      nC[u] = sC[at(r)];
+      nD[u] = sD[at(r)];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + u;
+      if (k <= n1) {
+        const double r = rcp_nr(B[u] - A[u] * cs);
+        ds = (D[u] - A[u] * ds) * r;
+        cs = C[u] * r;
+        sC[at(k)] = cs;
+        sD[at(k)] = ds;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      A[u] = nA[u];
+      B[u] = nB[u];
+      C[u] = nC[u];
+      D[u] = nD[u];
+    }
+  }
+  // W_k = ds_k - cs_k W_{k+1}, k = nz-1 .. 1, with W_nz = 0
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int r = max(n1 - u, 1);
+    C[u] = sC[at(r)];
+    D[u] = sD[at(r)];
+  }
+  double wab = 0.0;
+  for (int k0 = n1; k0 >= 1; k0 -= U) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int r = max(k0 - U - u, 1);
+      nC[u] = sC[at(r)];
+      nD[u] = sD[at(r)];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 - u;
+      if (k >= 1) {
+        wab = D[u] - C[u] * wab;
+        sD[at(k)] = wab;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      C[u] = nC[u];
+      D[u] = nD[u];
+    }
+  }
+  sD[lx] = w0;
+  sD[nz * TX + lx] = 0.0;
+}
+
 template <class T, int TX, int NW, int KPT, bool MOIST, bool TPI, int WPE = 1>
 __global__ void __launch_bounds__(TX * NW, WPE) k_acou_wpi_tile(Grid g, Params p, Base<T> b,
                                                                 Metric<T> m, Coef<T> cf,
@@ -509,6 +582,10 @@ __global__ void __launch_bounds__(TX * NW, WPE) k_acou_wpi_tile(Grid g, Params p
   // Over flat ground the cross-metric flux S_v is exactly zero.
   const bool flat = hxc == 0.0 && hyc == 0.0;
   const int c0 = g.id(col ? i : 0, j, 0);
+  // Base-state and coefficient column: column (0, 0) when every column holds
+  // the same profile (one cache line per level for the whole grid).
+  const int cb = p.hom ? g.id(0, 0, 0) : c0;
+  const int ox = p.hom ? 0 : 1, oy = p.hom ? 0 : NX;
 
   // Values kept in registers across the phases (storage-type values stay in
   // T; their FP64 conversion is exact).
@@ -524,13 +601,13 @@ __global__ void __launch_bounds__(TX * NW, WPE) k_acou_wpi_tile(Grid g, Params p
                               (double)a.u[c + 1]);
     const double vw = 0.25 * ((double)a.v[c - PL] + (double)a.v[c - PL + NX] +
                               (double)a.v[c] + (double)a.v[c + NX]);
-    return (double)cf.aw[c] * (uw * hxc + vw * hyc) * fac;
+    return (double)cf.aw[cb + k * PL] * (uw * hxc + vw * hyc) * fac;
   };
   auto av = [&](int k) -> double {
-    return (k <= 0 || k >= g.nz) ? 0.0 : (double)cf.aw[c0 + k * PL];
+    return (k <= 0 || k >= g.nz) ? 0.0 : (double)cf.aw[cb + k * PL];
   };
   auto ecoef = [&](int k) -> double {  // E_k = dtau K_t c1 / dzeta_c
-    return dtau * (double)cf.kt3[c0 + k * PL] * c1 * m.rdzc[k + g.ng];
+    return dtau * (double)cf.kt3[cb + k * PL] * c1 * m.rdzc[k + g.ng];
   };
 
   // ---- phase 1: assembly, parallel over k. Every load of a level is
@@ -545,16 +622,17 @@ __global__ void __launch_bounds__(TX * NW, WPE) k_acou_wpi_tile(Grid g, Params p
     const int c = c0 + kc * PL;
     const int cm = c - PL;
     const int kk = kc + g.ng;
-    const double Kt = (double)cf.kt3[c];
-    const double Dh = mc * (((double)cf.rtjx[c + 1] * (double)a.u[c + 1] -
-                             (double)cf.rtjx[c] * (double)a.u[c]) *
+    const int cc = cb + kc * PL;  // base-state / coefficient index
+    const double Kt = (double)cf.kt3[cc];
+    const double Dh = mc * (((double)cf.rtjx[cc + ox] * (double)a.u[c + 1] -
+                             (double)cf.rtjx[cc] * (double)a.u[c]) *
                                 g.rdx +
-                            ((double)cf.rtjy[c + NX] * (double)a.v[c + NX] -
-                             (double)cf.rtjy[c] * (double)a.v[c]) *
+                            ((double)cf.rtjy[cc + oy] * (double)a.v[c + NX] -
+                             (double)cf.rtjy[cc] * (double)a.v[c]) *
                                 g.rdy);
     const double rdzc = m.rdzc[kk];
-    const T awk = kc >= 1 ? cf.aw[c] : (T)0;
-    const T awk1 = kc + 1 < g.nz ? cf.aw[c + PL] : (T)0;
+    const T awk = kc >= 1 ? cf.aw[cc] : (T)0;
+    const T awk1 = kc + 1 < g.nz ? cf.aw[cc + PL] : (T)0;
     const double avk = (double)awk, avk1 = (double)awk1;
     const double wk = (double)a.w_src[c];
     const double wkp = (double)a.w_src[c + PL];
@@ -564,12 +642,12 @@ __global__ void __launch_bounds__(TX * NW, WPE) k_acou_wpi_tile(Grid g, Params p
     const T thkT = a.thp_src[c];
     const double thk = (double)thkT;
     const double thm = (double)a.thp_src[cm];
-    const double thbw = (double)b.thbw[c];
+    const double thbw = (double)b.thbw[cc];
     const double twk = (double)a.tw[c];
     double mfac = 1.0, mbuoy = 0.0;
     if constexpr (MOIST) {
       const double qvw = 0.5 * ((double)a.qv[cm] + (double)a.qv[c]);
-      const double dqv = qvw - 0.5 * ((double)b.qvb[cm] + (double)b.qvb[c]);
+      const double dqv = qvw - 0.5 * ((double)b.qvb[cc - PL] + (double)b.qvb[cc]);
       const double qcr = 0.5 * ((double)a.qc[cm] + (double)a.qc[c]) +
                          0.5 * ((double)a.qr[cm] + (double)a.qr[c]);
       mfac = 1.0 + kEps61 * qvw;
@@ -583,7 +661,7 @@ __global__ void __launch_bounds__(TX * NW, WPE) k_acou_wpi_tile(Grid g, Params p
     Avk1[t] = awk1;
     thold[t] = thkT;
     tthk[t] = a.tth[c];
-    dthk[t] = b.dthbdz[c];
+    dthk[t] = b.dthbdz[cc];
     // w level k >= 1: G_k and the right-hand side (kernels.cu:560-585)
     const double thvw = (thbw + 0.5 * (thm + thk)) * mfac;
     const double rJdzw = rJ * m.rdzw[kk];
@@ -624,84 +702,8 @@ __global__ void __launch_bounds__(TX * NW, WPE) k_acou_wpi_tile(Grid g, Params p
   }
   __syncthreads();
 
-  // ---- phase 3: elimination (kernels.cu:589-628 order), one lane per column.
-  // Rows are read from LDS in chunks of four, one chunk ahead of the
-  // recurrence (row indices clamped into range; a clamped read is never
-  // used), so only the FP64 chain is serial.
-  if (ry == 0 && col) {
-    constexpr int U = 4;
-    const int n1 = g.nz - 1;  // rows k = 1 .. nz-1
-    auto at = [&](int k) { return k * TX + lx; };
-    double A[U], B[U], C[U], D[U], nA[U], nB[U], nC[U], nD[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int r = min(1 + u, n1);
-      A[u] = sA[at(r)];
-      B[u] = sB[at(r)];
-      C[u] = sC[at(r)];
-      D[u] = sD[at(r)];
-    }
-    double cs = 0.0, ds = 0.0;
-    for (int k0 = 1; k0 <= n1; k0 += U) {
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int r = min(k0 + U + u, n1);
-        nA[u] = sA[at(r)];
-        nB[u] = sB[at(r)];
-        nC[u] = sC[at(r)];
-        nD[u] = sD[at(r)];
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int k = k0 + u;
-        if (k <= n1) {
-          const double r = rcp_nr(B[u] - A[u] * cs);
-          ds = (D[u] - A[u] * ds) * r;
-          cs = C[u] * r;
-          sC[at(k)] = cs;
-          sD[at(k)] = ds;
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        A[u] = nA[u];
-        B[u] = nB[u];
-        C[u] = nC[u];
-        D[u] = nD[u];
-      }
-    }
-    // W_k = ds_k - cs_k W_{k+1}, k = nz-1 .. 1, with W_nz = 0
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int r = max(n1 - u, 1);
-      C[u] = sC[at(r)];
-      D[u] = sD[at(r)];
-    }
-    double wab = 0.0;
-    for (int k0 = n1; k0 >= 1; k0 -= U) {
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int r = max(k0 - U - u, 1);
-        nC[u] = sC[at(r)];
-        nD[u] = sD[at(r)];
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int k = k0 - u;
-        if (k >= 1) {
-          wab = D[u] - C[u] * wab;
-          sD[at(k)] = wab;
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        C[u] = nC[u];
-        D[u] = nD[u];
-      }
-    }
-    sD[lx] = w0;
-    sD[g.nz * TX + lx] = 0.0;
-  }
+  // ---- phase 3: elimination (kernels.cu:589-628 order), one lane per column
+  if (ry == 0 && col) thomas_lds<TX>(sA, sB, sC, sD, lx, g.nz, w0);
   __syncthreads();
 
   // ---- phase 4: pi', pi'prev, theta', w
@@ -722,281 +724,362 @@ __global__ void __launch_bounds__(TX * NW, WPE) k_acou_wpi_tile(Grid g, Params p
 
 // ---------------------------------------------------------------------------
 // Fused acoustic substep: the horizontal u, v step and the w-pi' column solve
-// of one substep in a single launch. A workgroup owns a TX x TY block of
-// columns and NW thread rows (levels k = r, r + NW, ...):
-//   0. u, v of the block plus the halo faces the column solve needs (u at
-//      the block's east edge, v at its north edge) into LDS, and the block's
-//      own faces to memory (radiation edges and the periodic wrap included);
-//   1. assembly of P, E, G, RHS, Av per level from LDS, in FP64;
-//   2. Thomas elimination in the reference's order, one lane per column;
-//   3. w, pi', theta' written.
+// of one substep in a single launch, on rows of columns. A workgroup is TX
+// lanes along x by NW thread rows (levels k = r, r + NW, ...) at one j. Lane
+// l holds cell i = i0 - 1 + l: lanes 1 .. TX-2 own the columns of the block
+// (TX - 2 of them), lane 0 is the west halo cell and lane TX-1 the cell whose
+// west face is the east face of the last column.
+//   0. every lane forms u at its west face and v at the faces j and j+1 of
+//      its cell (radiation edges and the periodic wrap as k_acou_uv_tile);
+//      u at the east face is the next lane's value (a lane shuffle), v(j+1)
+//      is recomputed here rather than read from the block at j+1. The owned
+//      faces are written out;
+//   1. assembly of P, E, G and the right-hand side per level in FP64, as
+//      k_acou_wpi_tile (u, v from registers; over terrain the vertical
+//      cross-metric flux takes u, v at k-1 from LDS);
+//   2. rows A..D of the tridiagonal system in LDS;
+//   3. Thomas elimination, one lane per column, FP64;
+//   4. w, pi', theta' written.
 // Neighbour reads of pi' and theta' require that their outputs do not alias
 // their inputs: pi' rotates over three buffers (the previous pi' is the
 // input of the substep before, so pi'prev is never written) and theta'
-// ping-pongs.
+// ping-pongs; u, v ping-pong; w is only read and written in its own column.
 template <class T>
 struct FusedArgs {
   UVArgs<T> uv;  // u/v sources and destinations, pi' cur/prev/n, theta' cur/n, qv, tu, tv
-  const T* w_src;
-  T* w_dst;
-  double* pip_dst;
-  T* thp_dst;
-  const T* tw;
-  const T* tth;
-  const T* tpi;
-  const T* qc;
-  const T* qr;
+  CArr<T> w_src;
+  Arr<T> w_dst;
+  Arr<double> pip_dst;
+  Arr<T> thp_dst;
+  CArr<T> tw;
+  CArr<T> tth;
+  CArr<T> tpi;
+  CArr<T> qc;
+  CArr<T> qr;
 };
 
-template <int TX, int TY, int NW, int KPT, class T>
-struct FusedShape {
-  static constexpr int NC = TX * TY;
-  static size_t uv_bytes(int nz) {
-    return (size_t)nz * ((TX + 1) * TY + TX * (TY + 1)) * sizeof(T);
+template <int TX, int NW, int KPT>
+struct RowShape {
+  static constexpr int NC = TX - 2;  // columns per workgroup
+  // Four FP64 row arrays of (nz + 1) levels, plus u, v(j), v(j+1) per level
+  // over terrain.
+  static size_t lds_bytes(int nz, bool terrain, size_t tsize) {
+    return (size_t)4 * (nz + 1) * TX * sizeof(double) +
+           (terrain ? (size_t)3 * nz * TX * tsize : 0);
   }
-  static size_t lds_bytes(int nz) {
-    return (size_t)5 * (nz + 1) * NC * sizeof(double) + uv_bytes(nz);
-  }
-  static bool fits_levels(int nz) { return nz <= NW * KPT; }
 };
 
-template <class T, int TX, int TY, int NW, int KPT>
-__global__ void __launch_bounds__(TX * TY * NW) k_acoustic_fused(Grid g, Params p, Base<T> b,
-                                                                  Metric<T> m, Coef<T> cf,
-                                                                  FusedArgs<T> a) {
-  constexpr int NC = TX * TY;
-  constexpr int UROW = TX + 1;  // sU row length (faces i0 .. i0+TX)
+template <class T, int TX, int NW, int KPT, bool MOIST, bool TPI>
+__global__ void __launch_bounds__(TX * NW) k_acou_row(Grid g, Params p, Base<T> b, Metric<T> m,
+                                                     Coef<T> cf, FusedArgs<T> a) {
+  static_assert(TX == 32 || TX == 64, "a row of lanes is half or all of a wavefront");
   extern __shared__ double lds[];
   const int L = g.nz + 1;
-  double* sP = lds;
-  double* sE = sP + L * NC;
-  double* sG = sE + L * NC;
-  double* sR = sG + L * NC;
-  double* sA = sR + L * NC;
-  T* sU = reinterpret_cast<T*>(sA + L * NC);  // [nz][TY][TX+1]
-  T* sV = sU + g.nz * UROW * TY;              // [nz][TY+1][TX]
+  double* sD = lds;          // P_k, then D_k -> ds_k -> W_k (k in [0, nz])
+  double* sA = sD + L * TX;  // A_k            k in [1, nz)
+  double* sB = sA + L * TX;  // B_k            k in [1, nz)
+  double* sC = sB + L * TX;  // C_k -> cs_k    k in [1, nz)
+  T* sU = reinterpret_cast<T*>(sC + L * TX);  // terrain: u, v(j), v(j+1) per level
+  T* sV0 = sU + g.nz * TX;
+  T* sV1 = sV0 + g.nz * TX;
 
-  const int lc = threadIdx.x, ry = threadIdx.y;
-  const int ci = lc % TX, cj = lc / TX;
+  const int lx = threadIdx.x, ry = threadIdx.y;
   const BlockId bid = xcd_block();
-  const int i0 = bid.x * TX, j0 = bid.y * TY;
-  const int i = i0 + ci, j = j0 + cj;
-  const bool own = i < g.nx && j < g.ny;
-  const int ih = min(i0 + TX, g.nx), jh = min(j0 + TY, g.ny);
-  const bool do_uh = own && i == ih - 1;
-  const bool do_vh = own && j == jh - 1;
-  const UVDev<T> d{g, p, b, m, a.uv};
-  const T dtau_t = a.uv.dtau;
-  const double dtau = (double)a.uv.dtau;
+  const int ci = bid.x * (TX - 2) - 1 + lx;  // this lane's cell; its u face is ci
+  const int j = bid.y;
+  const bool col = lx >= 1 && lx <= TX - 2 && ci < g.nx;  // owns column (ci, j)
+  const int cx = ci < g.nx ? ci : g.nx;  // lanes past the first east ghost repeat it
   const int NX = g.NX, PL = g.plane;
+  const UVArgs<T>& q = a.uv;
+  const T dtau_t = q.dtau;
+  const double dtau = (double)q.dtau;
+  const int nxm = g.nx - 1, nym = g.ny - 1;
 
-  // ---- phase 0: horizontal step of u, v into LDS (and memory)
+  // Lateral addressing of the three rows j-1, j, j+1 of this lane's cell:
+  // level-0 index, level-0 index of the mapped interior cell (ghost reads of
+  // the current pi') and whether the cell is interior.
+  int crow[3], cmap[3];
+  bool inrow[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const int jj = j - 1 + r;
+    crow[r] = g.id(cx, jj, 0);
+    inrow[r] = cx >= 0 && cx < g.nx && jj >= 0 && jj < g.ny;
+    cmap[r] = g.id(map_lat(cx, g.nx, nxm, p.bcx), map_lat(jj, g.ny, nym, p.bcy), 0);
+  }
+  // pi* = pi' + smdiv (pi' - pi'prev) of row r at level kz (UVDev::pstar).
+  auto pstar = [&](int r, int kz) -> double {
+    const bool in = inrow[r] && (unsigned)kz < (unsigned)g.nz;
+    const int c = crow[r] + kz * PL;
+    const int km = kz < 0 ? 0 : (kz > g.nz - 1 ? g.nz - 1 : kz);
+    const int cm = in ? c : cmap[r] + km * PL;
+    const double pc = q.pcur[cm];
+    const double pp = (in ? q.pprev : q.pn)[c];
+    return pc + p.smdiv * (pc - pp);
+  };
+  // Base-state and coefficient column: column (0, 0) when every column holds
+  // the same profile.
+  const int chom = g.id(0, 0, 0);
+  // theta_v of row r at level k (UVDev::thv), and theta' itself.
+  auto thv = [&](int r, int k, T& th) -> T {
+    const int c = crow[r] + k * PL;
+    th = (inrow[r] ? q.thp_cur : q.thp_n)[c];
+    const T t = b.thb[p.hom ? chom + k * PL : c] + th;
+    if constexpr (MOIST) return t * ((T)1 + (T)kEps61 * q.qv[c]);
+    return t;
+  };
+
+  const int c2 = g.id2(cx, j);
+  const T mapc = m.mapf[c2], mapw = m.mapf[c2 - 1], maps = m.mapf[c2 - NX],
+          mapn = m.mapf[c2 + NX];
+  const T hxu = m.hxu[c2], hyv0 = m.hyv[c2], hyv1 = m.hyv[c2 + NX];
+  const T mu = (T)0.5 * (mapw + mapc);
+  const T mv0 = (T)0.5 * (maps + mapc), mv1 = (T)0.5 * (mapc + mapn);
+  const T ju = (T)0.5 * (m.jac[c2 - 1] + m.jac[c2]);
+  const T jv0 = (T)0.5 * (m.jac[c2 - NX] + m.jac[c2]);
+  const T jv1 = (T)0.5 * (m.jac[c2] + m.jac[c2 + NX]);
+  // The face-nx lane of a periodic x row, and the faces of an open edge.
+  const bool uwrap = !p.bcx && ci == g.nx;
+  const bool urad_w = p.bcx && ci == 0, urad_e = p.bcx && ci == g.nx;
+  const bool vwrap = !p.bcy && j == nym;  // v(j+1) = v(ny) = v(0)
+  const bool vrad_s = p.bcy && j == 0, vrad_n = p.bcy && j == nym;
+  const UVDev<T> d{g, p, b, m, q};
+
+  // ---- phase 0: horizontal step
+  T Un[KPT], Ue[KPT], V0[KPT], V1[KPT], Th[KPT];
 #pragma unroll
   for (int t = 0; t < KPT; ++t) {
     const int k = ry + NW * t;
-    if (!own || k >= g.nz) continue;
-    const int kk = k + g.ng;
-    const int c = g.id(i, j, k);
-    T un, vn;
-    if (p.bcx && i == 0) {  // west radiation edge (k_bc_rad_u_x :649-660)
-      const T u0 = a.uv.u_src[c];
-      if (u0 >= (T)0) {
+    const int kc = k < g.nz ? k : g.nz - 1;
+    const int kk = kc + g.ng;
+    T thm, th0, thp1;
+    const double ps_m = pstar(0, kc), ps_0 = pstar(1, kc), ps_p = pstar(2, kc);
+    const T tv_m = thv(0, kc, thm), tv_0 = thv(1, kc, th0), tv_p = thv(2, kc, thp1);
+    Th[t] = th0;
+    const int c = crow[1] + kc * PL;
+    const T usrc = q.u_src[c], tus = q.tu[c];
+    const T v0s = q.v_src[c], tv0s = q.tv[c];
+    const T v1s = q.v_src[c + NX], tv1s = q.tv[c + NX];
+    // west neighbour cell of this lane's u face
+    const double ps_w = __shfl_up(ps_0, 1, TX);
+    const T tv_w = __shfl_up(tv_0, 1, TX);
+    double gu = (ps_0 - ps_w) * g.rdx;
+    double gv0 = (ps_0 - ps_m) * g.rdy;
+    double gv1 = (ps_p - ps_0) * g.rdy;
+    if (p.terrain) {  // -(z_x / J) dpi*/dzeta on the faces with a slope
+      const T fac = (T)1 - m.zeta_c[kk] / m.zt;
+      const T dzc2 = m.zeta_c[kk + 1] - m.zeta_c[kk - 1];
+      const double dm = pstar(0, kc + 1) - pstar(0, kc - 1);
+      const double d0 = pstar(1, kc + 1) - pstar(1, kc - 1);
+      const double dp = pstar(2, kc + 1) - pstar(2, kc - 1);
+      const double dw = __shfl_up(d0, 1, TX);
+      if (hxu != (T)0) gu -= (double)(hxu * fac / ju) * (0.5 * (dw + d0) / (double)dzc2);
+      if (hyv0 != (T)0) gv0 -= (double)(hyv0 * fac / jv0) * (0.5 * (dm + d0) / (double)dzc2);
+      if (hyv1 != (T)0) gv1 -= (double)(hyv1 * fac / jv1) * (0.5 * (d0 + dp) / (double)dzc2);
+    }
+    const T thbu = (T)0.5 * (tv_w + tv_0);
+    const T thbv0 = (T)0.5 * (tv_m + tv_0);
+    const T thbv1 = (T)0.5 * (tv_0 + tv_p);
+    T un = usrc + dtau_t * (-(T)kCp * thbu * mu * (T)gu + tus);
+    T vn0 = v0s + dtau_t * (-(T)kCp * thbv0 * mv0 * (T)gv0 + tv0s);
+    T vn1 = v1s + dtau_t * (-(T)kCp * thbv1 * mv1 * (T)gv1 + tv1s);
+    (void)thm;
+    (void)thp1;
+    // open x edges (k_bc_rad_u_x kernels.cu:649-667): u at face 1 / nx-1 is
+    // the neighbour lane's interior value
+    const T u_east = __shfl_down(un, 1, TX);
+    const T u_west = __shfl_up(un, 1, TX);
+    if (uwrap) un = d.u_step(0, j, kc);  // periodic u(nx) = u(0)
+    if (urad_w) {
+      if (usrc >= (T)0) {
         un = b.ub[kk];
       } else {
-        const T cb = u0 - (T)p.cstar;
-        un = u0 - dtau_t * cb * (d.u_step(1, j, k) - u0) / (T)g.dx;
+        const T cb = usrc - (T)p.cstar;
+        un = usrc - dtau_t * cb * (u_east - usrc) / (T)g.dx;
       }
-    } else {
-      un = d.u_step(i, j, k);
     }
-    if (p.bcy && j == 0) {  // south radiation edge (k_bc_rad_v_y :670-680)
-      const T v0 = a.uv.v_src[c];
-      if (v0 >= (T)0) {
-        vn = b.vb[kk];
+    if (urad_e) {
+      if (usrc <= (T)0) {
+        un = b.ub[kk];
       } else {
-        const T cb = v0 - (T)p.cstar;
-        vn = v0 - dtau_t * cb * (d.v_step(i, 1, k) - v0) / (T)g.dy;
+        const T cb = usrc + (T)p.cstar;
+        un = usrc - dtau_t * cb * (usrc - u_west) / (T)g.dx;
       }
-    } else {
-      vn = d.v_step(i, j, k);
     }
-    sU[(k * TY + cj) * UROW + ci] = un;
-    sV[(k * (TY + 1) + cj) * TX + ci] = vn;
-    a.uv.u_dst[c] = un;
-    a.uv.v_dst[c] = vn;
-    if (do_uh) {  // east face of the block
-      T uh;
-      const int ch = c + 1;
-      if (ih < g.nx) {
-        uh = d.u_step(ih, j, k);
-      } else if (!p.bcx) {
-        uh = d.u_step(0, j, k);  // periodic ghost u(nx) = u(0)
-      } else {                   // east radiation edge (k_bc_rad_u_x :661-667)
-        const T u1 = a.uv.u_src[ch];
-        if (u1 <= (T)0) {
-          uh = b.ub[kk];
-        } else {
-          const T cb = u1 + (T)p.cstar;
-          uh = u1 - dtau_t * cb * (u1 - un) / (T)g.dx;
-        }
-      }
-      sU[(k * TY + cj) * UROW + (ih - i0)] = uh;
-      if (ih == g.nx) a.uv.u_dst[ch] = uh;
-    }
-    if (do_vh) {  // north face of the block
-      T vh;
-      const int ch = c + NX;
-      if (jh < g.ny) {
-        vh = d.v_step(i, jh, k);
-      } else if (!p.bcy) {
-        vh = d.v_step(i, 0, k);
+    // open y edges (k_bc_rad_v_y :670-688) and the periodic v(ny) = v(0)
+    const T vg0 = vn0, vg1 = vn1;
+    if (vrad_s) {
+      if (v0s >= (T)0) {
+        vn0 = b.vb[kk];
       } else {
-        const T v1 = a.uv.v_src[ch];
-        if (v1 <= (T)0) {
-          vh = b.vb[kk];
-        } else {
-          const T cb = v1 + (T)p.cstar;
-          vh = v1 - dtau_t * cb * (v1 - vn) / (T)g.dy;
-        }
+        const T cb = v0s - (T)p.cstar;
+        vn0 = v0s - dtau_t * cb * (vg1 - v0s) / (T)g.dy;
       }
-      sV[(k * (TY + 1) + (jh - j0)) * TX + ci] = vh;
-      if (jh == g.ny) a.uv.v_dst[ch] = vh;
+    }
+    if (vwrap) vn1 = d.v_step(cx, 0, kc);
+    if (vrad_n) {
+      if (v1s <= (T)0) {
+        vn1 = b.vb[kk];
+      } else {
+        const T cb = v1s + (T)p.cstar;
+        vn1 = v1s - dtau_t * cb * (v1s - vg0) / (T)g.dy;
+      }
+    }
+    Un[t] = un;
+    Ue[t] = __shfl_down(un, 1, TX);
+    V0[t] = vn0;
+    V1[t] = vn1;
+    if (k < g.nz) {
+      // owned faces: the west face of an owned column, and face nx (only
+      // the last block of the row has a lane at cell nx)
+      if (col || ci == g.nx) q.u_dst[c] = un;
+      if (col) {
+        q.v_dst[c] = vn0;
+        if (j == nym) q.v_dst[c + NX] = vn1;
+      }
+      if (p.terrain) {
+        sU[k * TX + lx] = un;
+        sV0[k * TX + lx] = vn0;
+        sV1[k * TX + lx] = vn1;
+      }
     }
   }
-  __syncthreads();
+  if (p.terrain) __syncthreads();
 
-  // ---- phase 1: column assembly (k_acou_wpi :552-620), FP64
+  // ---- phase 1: assembly (k_acou_wpi kernels.cu:552-585), FP64
   const double c1 = (1.0 + p.beta) * 0.5;
-  const double c2 = (1.0 - p.beta) * 0.5;
-  const int c2d = g.id2(own ? i : 0, j < g.ny ? j : 0);
+  const double c2w = (1.0 - p.beta) * 0.5;
+  const int c2d = g.id2(col ? ci : 0, j);
   const double J = (double)m.jac[c2d];
   const double hxc = (double)m.hx[c2d];
   const double hyc = (double)m.hy[c2d];
   const double mc = (double)m.mapf[c2d];
-  const double zt = (double)m.zt;
-  const int c0 = g.id(own ? i : 0, j < g.ny ? j : 0, 0);
-  auto U = [&](int k, int di) -> double { return (double)sU[(k * TY + cj) * UROW + ci + di]; };
-  auto V = [&](int k, int dj) -> double {
-    return (double)sV[(k * (TY + 1) + cj + dj) * TX + ci];
-  };
-  auto sv = [&](int k) -> double {
-    if (k <= 0 || k >= g.nz) return 0.0;
-    const double fac = 1.0 - (double)m.zeta_w[k + g.ng] / zt;
-    const double uw = 0.25 * (U(k - 1, 0) + U(k - 1, 1) + U(k, 0) + U(k, 1));
-    const double vw = 0.25 * (V(k - 1, 0) + V(k - 1, 1) + V(k, 0) + V(k, 1));
-    return (double)cf.aw[c0 + k * PL] * (uw * hxc + vw * hyc) * fac;
+  const double rJ = rcp_nr(J);
+  const bool flat = hxc == 0.0 && hyc == 0.0;
+  const int c0 = g.id(col ? ci : 0, j, 0);
+  const int cb = p.hom ? chom : c0;
+  const int ox = p.hom ? 0 : 1, oy = p.hom ? 0 : NX;
+  const int lxe = lx < TX - 1 ? lx + 1 : lx;
+  auto sv = [&](int k) -> double {  // cross-metric vertical flux at w level k
+    if (k <= 0 || k >= g.nz || flat) return 0.0;
+    const double fac = 1.0 - (double)m.zeta_w[k + g.ng] * m.rzt;
+    const double uw = 0.25 * ((double)sU[(k - 1) * TX + lx] + (double)sU[(k - 1) * TX + lxe] +
+                              (double)sU[k * TX + lx] + (double)sU[k * TX + lxe]);
+    const double vw = 0.25 * ((double)sV0[(k - 1) * TX + lx] + (double)sV1[(k - 1) * TX + lx] +
+                              (double)sV0[k * TX + lx] + (double)sV1[k * TX + lx]);
+    return (double)cf.aw[cb + k * PL] * (uw * hxc + vw * hyc) * fac;
   };
   auto av = [&](int k) -> double {
-    return (k <= 0 || k >= g.nz) ? 0.0 : (double)cf.aw[c0 + k * PL];
+    return (k <= 0 || k >= g.nz) ? 0.0 : (double)cf.aw[cb + k * PL];
   };
-  double thold[KPT], tthk[KPT], dthk[KPT];
+  auto ecoef = [&](int k) -> double {  // E_k = dtau K_t c1 / dzeta_c
+    return dtau * (double)cf.kt3[cb + k * PL] * c1 * m.rdzc[k + g.ng];
+  };
+
+  double Pk[KPT], Ek[KPT], Gk[KPT], Rk[KPT];
+  T Avk[KPT], Avk1[KPT], tthk[KPT], dthk[KPT];
   double w0 = 0.0;
 #pragma unroll
   for (int t = 0; t < KPT; ++t) {
     const int k = ry + NW * t;
-    if (!own || k >= g.nz) continue;
-    const int c = c0 + k * PL;
-    const int kk = k + g.ng;
-    const double Kt = (double)cf.kt3[c];
-    const double Dh =
-        mc * (((double)cf.rtjx[c + 1] * U(k, 1) - (double)cf.rtjx[c] * U(k, 0)) / g.dx +
-              ((double)cf.rtjy[c + NX] * V(k, 1) - (double)cf.rtjy[c] * V(k, 0)) / g.dy);
-    const double dzc = (double)m.dzeta_c[kk];
-    const double avk = av(k), avk1 = av(k + 1);
+    const bool act = col && k < g.nz;
+    const int kc = k < g.nz ? k : g.nz - 1;
+    const int c = c0 + kc * PL;
+    const int cm = c - PL;
+    const int kk = kc + g.ng;
+    const int cc = cb + kc * PL;  // base-state / coefficient index
+    const double Kt = (double)cf.kt3[cc];
+    const double Dh = mc * (((double)cf.rtjx[cc + ox] * (double)Ue[t] -
+                             (double)cf.rtjx[cc] * (double)Un[t]) *
+                                g.rdx +
+                            ((double)cf.rtjy[cc + oy] * (double)V1[t] -
+                             (double)cf.rtjy[cc] * (double)V0[t]) *
+                                g.rdy);
+    const double rdzc = m.rdzc[kk];
+    const T awk = kc >= 1 ? cf.aw[cc] : (T)0;
+    const T awk1 = kc + 1 < g.nz ? cf.aw[cc + PL] : (T)0;
+    const double avk = (double)awk, avk1 = (double)awk1;
     const double wk = (double)a.w_src[c];
     const double wkp = (double)a.w_src[c + PL];
-    const double tpi = a.tpi ? (double)a.tpi[c] : 0.0;
-    const double pk = a.uv.pcur[c];
-    sP[k * NC + lc] = pk + dtau * tpi -
-                      dtau * Kt *
-                          (Dh - (sv(k + 1) - sv(k)) / dzc + c2 * (avk1 * wkp - avk * wk) / dzc);
-    sE[k * NC + lc] = dtau * Kt * c1 / dzc;
-    sA[k * NC + lc] = avk;
-    const double thk = (double)a.uv.thp_cur[c];
-    thold[t] = thk;
-    tthk[t] = (double)a.tth[c];
-    dthk[t] = (double)b.dthbdz[c];
-    if (k >= 1) {
-      const int cm = c - PL;
-      const double thm = (double)a.uv.thp_cur[cm];
-      double thvw = (double)b.thbw[c] + 0.5 * (thm + thk);
-      if (p.moisture) thvw *= 1.0 + kEps61 * 0.5 * ((double)a.uv.qv[cm] + (double)a.uv.qv[c]);
-      const double dzw = (double)m.dzeta_w[kk];
-      sG[k * NC + lc] = dtau * kCp * thvw * c1 / (J * dzw);
-      double buoy = kGrav * 0.5 * (thm + thk) / (double)b.thbw[c];
-      if (p.moisture) {
-        const double dqv = 0.5 * ((double)a.uv.qv[cm] + (double)a.uv.qv[c]) -
-                           0.5 * ((double)b.qvb[cm] + (double)b.qvb[c]);
-        const double qcr = 0.5 * ((double)a.qc[cm] + (double)a.qc[c]) +
-                           0.5 * ((double)a.qr[cm] + (double)a.qr[c]);
-        buoy += kGrav * (kEps61 * dqv - qcr);
-      }
-      sR[k * NC + lc] = wk + dtau * (buoy + (double)a.tw[c]) -
-                        dtau * kCp * thvw * c2 * (pk - a.uv.pcur[cm]) / (J * dzw);
-    } else {
-      const double us = 0.5 * (U(0, 0) + U(0, 1));
-      const double vs = 0.5 * (V(0, 0) + V(0, 1));
+    const double tpi = TPI ? (double)a.tpi[c] : 0.0;
+    const double pk = q.pcur[c];
+    const double pkm = q.pcur[cm];
+    const double thk = (double)Th[t];
+    const double thm = (double)q.thp_cur[cm];
+    const double thbw = (double)b.thbw[cc];
+    const double twk = (double)a.tw[c];
+    double mfac = 1.0, mbuoy = 0.0;
+    if constexpr (MOIST) {
+      const double qvw = 0.5 * ((double)q.qv[cm] + (double)q.qv[c]);
+      const double dqv = qvw - 0.5 * ((double)b.qvb[cc - PL] + (double)b.qvb[cc]);
+      const double qcr = 0.5 * ((double)a.qc[cm] + (double)a.qc[c]) +
+                         0.5 * ((double)a.qr[cm] + (double)a.qr[c]);
+      mfac = 1.0 + kEps61 * qvw;
+      mbuoy = kGrav * (kEps61 * dqv - qcr);
+    }
+    Pk[t] = pk + dtau * tpi -
+            dtau * Kt *
+                (Dh - (sv(kc + 1) - sv(kc)) * rdzc + c2w * (avk1 * wkp - avk * wk) * rdzc);
+    Ek[t] = dtau * Kt * c1 * rdzc;
+    Avk[t] = awk;
+    Avk1[t] = awk1;
+    tthk[t] = a.tth[c];
+    dthk[t] = b.dthbdz[cc];
+    const double thvw = (thbw + 0.5 * (thm + thk)) * mfac;
+    const double rJdzw = rJ * m.rdzw[kk];
+    const double buoy = kGrav * 0.5 * (thm + thk) * rcp_nr(thbw) + mbuoy;
+    Gk[t] = kc >= 1 ? dtau * kCp * thvw * c1 * rJdzw : 0.0;
+    Rk[t] = kc >= 1 ? wk + dtau * (buoy + twk) - dtau * kCp * thvw * c2w * (pk - pkm) * rJdzw
+                    : 0.0;
+    if (k == 0) {  // diagnostic w(0) = u h_x + v h_y at the surface
+      const double us = 0.5 * ((double)Un[t] + (double)Ue[t]);
+      const double vs = 0.5 * ((double)V0[t] + (double)V1[t]);
       w0 = us * hxc + vs * hyc;
     }
-  }
-  if (ry == 0) sA[g.nz * NC + lc] = 0.0;
-  __syncthreads();
-
-  // ---- phase 2: Thomas elimination (kernels.cu:589-628 order)
-  if (ry == 0 && own) {
-    double e_prev = sE[lc], av_prev = 0.0, av_cur = sA[NC + lc], p_prev = sP[lc];
-    double cs_prev = 0.0, ds_prev = 0.0;
-    for (int k = 1; k < g.nz; ++k) {
-      const double ek = sE[k * NC + lc];
-      const double G = sG[k * NC + lc];
-      const double R = sR[k * NC + lc];
-      const double pk = sP[k * NC + lc];
-      const double av_next = sA[(k + 1) * NC + lc];
-      const double A = -G * e_prev * av_prev;
-      const double B = 1.0 + G * av_cur * (ek + e_prev);
-      const double C = -G * ek * av_next;
-      const double D = R - G * (pk - p_prev);
-      double cs, ds;
-      if (k == 1) {
-        cs = C / B;
-        ds = D / B;
-      } else {
-        const double mm = B - A * cs_prev;
-        cs = C / mm;
-        ds = (D - A * ds_prev) / mm;
-      }
-      sG[k * NC + lc] = cs;
-      sR[k * NC + lc] = ds;
-      e_prev = ek;
-      av_prev = av_cur;
-      av_cur = av_next;
-      p_prev = pk;
-      cs_prev = cs;
-      ds_prev = ds;
-    }
-    double wabove = 0.0;
-    for (int k = g.nz - 1; k >= 1; --k) {
-      const double wk = sR[k * NC + lc] - sG[k * NC + lc] * wabove;
-      sR[k * NC + lc] = wk;
-      wabove = wk;
-    }
-    sR[lc] = w0;
-    sR[g.nz * NC + lc] = 0.0;
+    if (act) sD[k * TX + lx] = Pk[t];
   }
   __syncthreads();
 
-  // ---- phase 3: w, pi', theta'
+  // ---- phase 2: tridiagonal rows (kernels.cu:596-603)
+  double Dk[KPT];
 #pragma unroll
   for (int t = 0; t < KPT; ++t) {
     const int k = ry + NW * t;
-    if (!own || k >= g.nz) continue;
-    const int c = c0 + k * PL;
-    const double Wk = sR[k * NC + lc], Wk1 = sR[(k + 1) * NC + lc];
-    const double Ak = sA[k * NC + lc], Ak1 = sA[(k + 1) * NC + lc];
-    a.pip_dst[c] = sP[k * NC + lc] - sE[k * NC + lc] * (Ak1 * Wk1 - Ak * Wk);
-    const double wc = 0.5 * (Wk + Wk1);
-    a.thp_dst[c] = (T)(thold[t] + dtau * (tthk[t] - wc * dthk[t]));
-    a.w_dst[c] = (T)Wk;
+    Dk[t] = 0.0;
+    if (col && k >= 1 && k < g.nz) {
+      const double e_prev = ecoef(k - 1);
+      const double av_prev = av(k - 1);
+      const double G = Gk[t];
+      sA[k * TX + lx] = -G * e_prev * av_prev;
+      sB[k * TX + lx] = 1.0 + G * (double)Avk[t] * (Ek[t] + e_prev);
+      sC[k * TX + lx] = -G * Ek[t] * (double)Avk1[t];
+      Dk[t] = Rk[t] - G * (Pk[t] - sD[(k - 1) * TX + lx]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < KPT; ++t) {
+    const int k = ry + NW * t;
+    if (col && k >= 1 && k < g.nz) sD[k * TX + lx] = Dk[t];
+  }
+  __syncthreads();
+
+  // ---- phase 3: elimination (kernels.cu:589-628 order), one lane per column
+  if (ry == 0 && col) thomas_lds<TX>(sA, sB, sC, sD, lx, g.nz, w0);
+  __syncthreads();
+
+  // ---- phase 4: pi', theta', w
+#pragma unroll
+  for (int t = 0; t < KPT; ++t) {
+    const int k = ry + NW * t;
+    if (col && k < g.nz) {
+      const int c = c0 + k * PL;
+      const double Wk = sD[k * TX + lx], Wk1 = sD[(k + 1) * TX + lx];
+      a.pip_dst[c] = Pk[t] - Ek[t] * ((double)Avk1[t] * Wk1 - (double)Avk[t] * Wk);
+      const double wc = 0.5 * (Wk + Wk1);
+      a.thp_dst[c] = (T)((double)Th[t] + dtau * ((double)tthk[t] - wc * (double)dthk[t]));
+      a.w_dst[c] = (T)Wk;
+    }
   }
 }
 

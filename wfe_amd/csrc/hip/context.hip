@@ -193,50 +193,6 @@ struct SoloBench {
 
 }  // namespace
 
-// Tile shapes of the fused acoustic substep (TX x TY columns, NW thread
-// rows, KPT levels per thread). All shapes give bitwise-identical results;
-// the context times the ones that fit and keeps the fastest.
-template <class T>
-struct FusedVariant {
-  const char* name;
-  int tx, ty, nw, kpt;
-  size_t (*lds)(int nz);
-  const void* fn;
-  void (*launch)(dim3 grid, size_t lds, hipStream_t s, const dev::Grid& g, const dev::Params& p,
-                 const dev::Base<T>& b, const dev::Metric<T>& m, const dev::Coef<T>& cf,
-                 const dev::FusedArgs<T>& a);
-};
-
-template <class T, int TX, int TY, int NW, int KPT>
-FusedVariant<T> make_variant(const char* name) {
-  using S = dev::FusedShape<TX, TY, NW, KPT, T>;
-  FusedVariant<T> v;
-  v.name = name;
-  v.tx = TX;
-  v.ty = TY;
-  v.nw = NW;
-  v.kpt = KPT;
-  v.lds = [](int nz) { return S::lds_bytes(nz); };
-  v.fn = reinterpret_cast<const void*>(&dev::k_acoustic_fused<T, TX, TY, NW, KPT>);
-  v.launch = [](dim3 grid, size_t lds, hipStream_t s, const dev::Grid& g, const dev::Params& p,
-                const dev::Base<T>& b, const dev::Metric<T>& m, const dev::Coef<T>& cf,
-                const dev::FusedArgs<T>& a) {
-    dev::k_acoustic_fused<T, TX, TY, NW, KPT><<<grid, dim3(TX * TY, NW), lds, s>>>(g, p, b, m, cf, a);
-  };
-  return v;
-}
-
-template <class T>
-std::vector<FusedVariant<T>> fused_variants() {
-  return {
-      make_variant<T, 8, 2, 16, 3>("8x2x16"),   make_variant<T, 16, 1, 16, 3>("16x1x16"),
-      make_variant<T, 4, 4, 16, 3>("4x4x16"),   make_variant<T, 8, 4, 8, 6>("8x4x8"),
-      make_variant<T, 16, 2, 8, 6>("16x2x8"),   make_variant<T, 32, 1, 8, 6>("32x1x8"),
-      make_variant<T, 16, 2, 16, 3>("16x2x16"), make_variant<T, 8, 2, 24, 2>("8x2x24"),
-      make_variant<T, 8, 4, 16, 3>("8x4x16"),   make_variant<T, 32, 2, 8, 6>("32x2x8"),
-  };
-}
-
 // Type-erased interface used by the C ABI.
 struct CtxBase {
   virtual ~CtxBase() = default;
@@ -326,6 +282,7 @@ class Ctx final : public CtxBase {
     WFE_HIP(hipStreamSynchronize(stream_));
     std::string r = timer_.report();
     if (tuned_) r += "tuned:" + methods_[method_].name + " 0 0\n";
+    r += std::string("uniform_columns:") + (p_.hom ? "1" : "0") + " 0 0\n";
     r += tune_report_;
     return r;
   }
@@ -402,6 +359,8 @@ class Ctx final : public CtxBase {
   void tune_acoustic(double dtau);
   template <int TX, int NW, int KPT, int WPE = 1>
   void add_split_tile(const char* name, size_t cap);
+  template <int TX, int NW, int KPT>
+  void add_row(const char* name, size_t cap);
   KernelTimer timer_;
   SoloBench solo_;
 };
@@ -561,6 +520,26 @@ Ctx<T>::Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& b, const w
           rtjx[c] = ir > 0 ? 0.5 * (rtjc[c - 1] + rtjc[c]) : rtjc[c];
           rtjy[c] = jr > 0 ? 0.5 * (rtjc[c - hNX_] + rtjc[c]) : rtjc[c];
         }
+    // Column-uniform profiles over the interior and one ghost ring at the
+    // levels 0 .. nz-1 whose values the acoustic kernels use (equal bits,
+    // NaN included).
+    auto uniform = [&](const double* a) {
+      if (!a) return true;
+      for (int kr = gr.ng; kr < gr.ng + gr.nz; ++kr) {
+        const double ref = a[((size_t)kr * g_.NY + gr.ng) * hNX_ + gr.ng];
+        for (int jr = gr.ng - 1; jr <= gr.ng + gr.ny; ++jr)
+          for (int ir = gr.ng - 1; ir <= gr.ng + gr.nx; ++ir) {
+            const double v = a[((size_t)kr * g_.NY + jr) * hNX_ + ir];
+            if (std::memcmp(&v, &ref, sizeof v) != 0) return false;
+          }
+      }
+      return true;
+    };
+    p_.hom = !p_.terrain && uniform(b.thb) && uniform(b.thbw) && uniform(b.dthbdz) &&
+             uniform(b.qvb) && uniform(rtjx.data()) && uniform(rtjy.data()) &&
+             uniform(kt3.data()) && uniform(aw.data());
+    if (const char* e = std::getenv("WFE_UNIFORM_COLUMNS"))
+      if (std::string(e) == "0") p_.hom = 0;
     coef_.rtjx = upload_new<T>(rtjx.data(), s3);
     coef_.rtjy = upload_new<T>(rtjy.data(), s3);
     coef_.kt3 = upload_new<T>(kt3.data(), s3);
@@ -874,6 +853,44 @@ void Ctx<T>::add_split_tile(const char* name, size_t cap) {
 }
 
 template <class T>
+template <int TX, int NW, int KPT>
+void Ctx<T>::add_row(const char* name, size_t cap) {
+  using Shape = dev::RowShape<TX, NW, KPT>;
+  if (g_.nz > NW * KPT) return;
+  const size_t lb = Shape::lds_bytes(g_.nz, p_.terrain != 0, sizeof(T));
+  // Instantiations by (moisture, pi' tendency present).
+  const void* fns[4] = {reinterpret_cast<const void*>(&dev::k_acou_row<T, TX, NW, KPT, false, false>),
+                        reinterpret_cast<const void*>(&dev::k_acou_row<T, TX, NW, KPT, false, true>),
+                        reinterpret_cast<const void*>(&dev::k_acou_row<T, TX, NW, KPT, true, false>),
+                        reinterpret_cast<const void*>(&dev::k_acou_row<T, TX, NW, KPT, true, true>)};
+  if (lb > 65536) {
+    if (lb > cap) return;
+    for (const void* f : fns)
+      if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb) != hipSuccess) {
+        (void)hipGetLastError();
+        return;
+      }
+  }
+  methods_.push_back({name, [this, lb](const dev::FusedArgs<T>& a) {
+                        const dim3 gt(cdiv(g_.nx, Shape::NC), (unsigned)g_.ny);
+                        const dim3 bt(TX, NW);
+                        const bool moist = p_.moisture, tpi = a.tpi != nullptr;
+                        if (moist && tpi)
+                          TIMED("acoustic_fused", (dev::k_acou_row<T, TX, NW, KPT, true, true>
+                                                   <<<gt, bt, lb, stream_>>>(g_, p_, base_, met_, coef_, a)));
+                        else if (moist)
+                          TIMED("acoustic_fused", (dev::k_acou_row<T, TX, NW, KPT, true, false>
+                                                   <<<gt, bt, lb, stream_>>>(g_, p_, base_, met_, coef_, a)));
+                        else if (tpi)
+                          TIMED("acoustic_fused", (dev::k_acou_row<T, TX, NW, KPT, false, true>
+                                                   <<<gt, bt, lb, stream_>>>(g_, p_, base_, met_, coef_, a)));
+                        else
+                          TIMED("acoustic_fused", (dev::k_acou_row<T, TX, NW, KPT, false, false>
+                                                   <<<gt, bt, lb, stream_>>>(g_, p_, base_, met_, coef_, a)));
+                      }});
+}
+
+template <class T>
 void Ctx<T>::build_methods(const hipDeviceProp_t& prop) {
   const size_t cap = (size_t)prop.sharedMemPerBlock;
   // General split pair: one thread per column (any nz <= 319).
@@ -900,21 +917,11 @@ void Ctx<T>::build_methods(const hipDeviceProp_t& prop) {
   add_split_tile<32, 16, 4>("split-32x16x4", cap);  // columns up to 64 levels
   add_split_tile<32, 16, 3, 4>("split-32x16w4", cap);   // two blocks per CU
   add_split_tile<16, 16, 3, 4>("split-16x16w4", cap);
-  for (const auto& v : fused_variants<T>()) {
-    if (g_.nz > v.nw * v.kpt) continue;
-    const size_t lb = v.lds(g_.nz);
-    if (lb > 65536 && (lb > cap || hipFuncSetAttribute(v.fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                       (int)lb) != hipSuccess)) {
-      (void)hipGetLastError();
-      continue;
-    }
-    const FusedVariant<T> fv = v;
-    methods_.push_back({std::string("fused-") + v.name, [this, fv, lb](const dev::FusedArgs<T>& a) {
-                          const dim3 grid(cdiv(g_.nx, fv.tx), cdiv(g_.ny, fv.ty));
-                          TIMED("acoustic_fused",
-                                fv.launch(grid, lb, stream_, g_, p_, base_, met_, coef_, a));
-                        }});
-  }
+  add_row<64, 16, 3>("fused-64x16", cap);
+  add_row<64, 8, 6>("fused-64x8", cap);
+  add_row<32, 16, 3>("fused-32x16", cap);
+  add_row<32, 8, 6>("fused-32x8", cap);
+  add_row<32, 16, 4>("fused-32x16x4", cap);  // columns up to 64 levels
   // Selection: WFE_ACOUSTIC_KERNEL = exact method name, or a family prefix
   // ("split", "fused"); unset or "auto" = all. WFE_WPI_KERNEL=column keeps
   // only the one-thread-per-column pair.

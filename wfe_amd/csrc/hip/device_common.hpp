@@ -3,8 +3,31 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <type_traits>
 
 namespace wfe::dev {
+
+// Device array view. Element i is addressed through a 32-bit unsigned byte
+// offset from the base, so loads and stores take the scalar-base plus
+// vector-offset form and need no 64-bit address arithmetic per access.
+// Indices into the padded layout are never negative (ghost layers are part
+// of it) and every array is below 4 GiB. Converts to and from T*.
+template <class T>
+struct Arr {
+  T* p;
+  Arr() = default;
+  __host__ __device__ constexpr Arr(T* q) : p(q) {}
+  __host__ __device__ constexpr Arr(std::nullptr_t) : p(nullptr) {}
+  template <class U, class = std::enable_if_t<std::is_convertible_v<U*, T*>>>
+  __host__ __device__ constexpr Arr(Arr<U> o) : p(o.p) {}
+  __host__ __device__ constexpr operator T*() const { return p; }
+  __device__ __forceinline__ T& operator[](int i) const {
+    using B = std::conditional_t<std::is_const_v<T>, const char, char>;
+    return *reinterpret_cast<T*>(reinterpret_cast<B*>(p) + (unsigned)i * (unsigned)sizeof(T));
+  }
+};
+template <class T>
+using CArr = Arr<const T>;
 
 inline constexpr double kGrav = 9.81, kCp = 1004.5, kRd = 287.04, kCv = kCp - kRd;
 inline constexpr double kP00 = 1.0e5, kLv = 2.5e6, kLf = 3.34e5, kLs = kLv + kLf;
@@ -66,36 +89,57 @@ struct Params {
   double cstar;
   int moisture, pd_moist, w_damping, microphysics;
   int terrain;  // 1 when any terrain slope h_x, h_y is non-zero
+  // 1 when the base state and the acoustic coefficients hold the same
+  // profile in every column (flat terrain, horizontally uniform base state):
+  // kernels then read those profiles from column (0, 0).
+  int hom;
 };
 
 // Base state (DevProf, src/core/base_state.hpp:16-27). pib stays FP64 so
 // that pi = pibar + pi' is formed in FP64 in every precision mode.
 template <class T>
 struct Base {
-  const T *thb, *rhob, *dthbdz, *thbw, *rhobw, *qvb, *ub, *vb;
-  const double* pib;
+  CArr<T> thb, rhob, dthbdz, thbw, rhobw, qvb, ub, vb;
+  CArr<double> pib;
 };
 
 // Metric (DevMetric, src/core/metric.hpp:16-30).
 template <class T>
 struct Metric {
-  const T *zeta_w, *zeta_c, *dzeta_c, *dzeta_w;
-  const T *hx, *hy, *hxu, *hyv, *jac, *fcor, *mapf;
+  CArr<T> zeta_w, zeta_c, dzeta_c, dzeta_w;
+  CArr<T> hx, hy, hxu, hyv, jac, fcor, mapf;
   T zt;
   // FP64 reciprocals for the column solve: 1/dzeta_c, 1/dzeta_w per level
   // (index k + ng) and 1/zt.
-  const double *rdzc, *rdzw;
+  CArr<double> rdzc, rdzw;
   double rzt;
   // sin^2 Rayleigh damping rate per level at cell centres and w levels
   // (ray_alpha, kernels.cu:42-48), formed on the host in FP64.
-  const T *ray_c, *ray_w;
+  CArr<T> ray_c, ray_w;
 };
 
 // Constant acoustic coefficients (AcousticCoef, src/dynamics/kernels.hpp:46-50).
 template <class T>
 struct Coef {
-  const T *rtjx, *rtjy, *kt3, *aw;
+  CArr<T> rtjx, rtjy, kt3, aw;
 };
+
+// 1/x for well-scaled x (pivots of a diagonally dominant system, base-state
+// theta, the Jacobian): v_rcp_f64 followed by two Newton-Raphson steps
+// r <- r + r (1 - x r), within an ulp or so of the quotient.
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// FP32 1/x for well-scaled x: v_rcp_f32 then one Newton-Raphson step.
+__device__ __forceinline__ float rcp_nr(float x) {
+  const float r = __builtin_amdgcn_rcpf(x);
+  return fmaf(r, fmaf(-x, r, 1.0f), r);
+}
 
 // 5th-order upwind interface value between q0 and q1 (WS2002;
 // src/dynamics/kernels.cu:51-57): the centred 6-point value minus/plus the

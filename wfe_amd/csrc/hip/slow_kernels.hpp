@@ -131,9 +131,9 @@ struct MassFlux {
   const Grid& g;
   const Base<T>& b;
   const Metric<T>& m;
-  const T* __restrict__ u;
-  const T* __restrict__ v;
-  const T* __restrict__ w;
+  CArr<T> u;
+  CArr<T> v;
+  CArr<T> w;
   __device__ __forceinline__ T fx(int c, int c2) const {
     const T rj = b.rhob[c] * m.jac[c2];
     return (T)0.5 * (b.rhob[c - 1] * m.jac[c2 - 1] + rj) * u[c];
@@ -145,7 +145,7 @@ struct MassFlux {
   // k in [1, nz); 0 elsewhere
   __device__ __forceinline__ T fz(int c, int c2, int k) const {
     if (k < 1 || k >= g.nz) return (T)0;
-    const T fac = (T)1 - m.zeta_w[k + g.ng] / m.zt;
+    const T fac = (T)1 - m.zeta_w[k + g.ng] * (T)m.rzt;
     const int cm = c - g.plane;
     const T uw = (T)0.25 * (u[cm] + u[cm + 1] + u[c] + u[c + 1]);
     const T vw = (T)0.25 * (v[cm] + v[cm + g.NX] + v[c] + v[c + g.NX]);
@@ -155,9 +155,11 @@ struct MassFlux {
 };
 
 template <class T>
-__global__ void k_massflux(Grid g, Base<T> b, Metric<T> m, const T* __restrict__ u,
-                           const T* __restrict__ v, const T* __restrict__ w, T* mfx, T* mfy,
-                           T* mfz, T* div) {
+__global__ void k_massflux(Grid g, Base<T> b, Metric<T> m, const T* __restrict__ u_,
+                           const T* __restrict__ v_, const T* __restrict__ w_, T* mfx_,
+                           T* mfy_, T* mfz_, T* div_) {
+  const CArr<T> u{u_}, v{v_}, w{w_};
+  const Arr<T> mfx{mfx_}, mfy{mfy_}, mfz{mfz_}, div{div_};
   const BlockId bid = xcd_block();
   const int i = (int)(bid.x * blockDim.x + threadIdx.x) - 2;
   const int j = (int)(bid.y * blockDim.y + threadIdx.y) - 2;
@@ -175,24 +177,30 @@ __global__ void k_massflux(Grid g, Base<T> b, Metric<T> m, const T* __restrict__
     const T fxr = F.fx(c + 1, c2 + 1);
     const T fyr = F.fy(c + g.NX, c2 + g.NX);
     const T fzt = F.fz(c + g.plane, c2, k + 1);
-    div[c] = (fxr - fxc) / (T)g.dx + (fyr - fyc) / (T)g.dy + (fzt - fzc) / m.dzeta_c[k + g.ng];
+    div[c] = (fxr - fxc) * (T)g.rdx + (fyr - fyc) * (T)g.rdy + (fzt - fzc) * (T)m.rdzc[k + g.ng];
   }
 }
 
 // Flux through the face whose upper cell is c (stride s along the normal):
 // F = f * iface5(q[c-3s .. c+2s], f).
-template <class T>
-__device__ __forceinline__ T face_flux(const T* __restrict__ q, int c, int s, T f) {
+template <class A, class T>
+__device__ __forceinline__ T face_flux(const A& q, int c, int s, T f) {
   return f * iface5(q[c - 3 * s], q[c - 2 * s], q[c - s], q[c], q[c + s], q[c + 2 * s], f);
 }
 
-// Advective-consistent flux-form tendency of a cell-centred scalar at c:
-// -(1/rho~)[div(MF q) - q div(MF)] (k_tend_thp / k_tend_q, kernels.cu:114-179).
+// Reciprocal grid spacings of one point (1/dx, 1/dy, 1/dzeta).
 template <class T>
-__device__ __forceinline__ T scalar_adv(const Grid& g, const T* __restrict__ q,
-                                        const T* __restrict__ mfx, const T* __restrict__ mfy,
-                                        const T* __restrict__ mfz, const T* __restrict__ div,
-                                        int c, T dzc, T rt) {
+struct Rsp {
+  T x, y, z;
+};
+
+// Advective-consistent flux-form tendency of a cell-centred scalar at c:
+// -(1/rho~)[div(MF q) - q div(MF)] (k_tend_thp / k_tend_q, kernels.cu:114-179),
+// with the divisions by dx, dy, dzeta and rho~ taken as products with
+// reciprocals (rrt = 1/rho~).
+template <class A, class T>
+__device__ __forceinline__ T scalar_adv(const Grid& g, const A& q, const A& mfx, const A& mfy,
+                                        const A& mfz, const A& div, int c, Rsp<T> r, T rrt) {
   const T fxL = mfx[c], fxR = mfx[c + 1];
   const T fyL = mfy[c], fyR = mfy[c + g.NX];
   const T fzB = mfz[c], fzT = mfz[c + g.plane];
@@ -202,18 +210,18 @@ __device__ __forceinline__ T scalar_adv(const Grid& g, const T* __restrict__ q,
   const T FyR = face_flux(q, c + g.NX, g.NX, fyR);
   const T FzB = face_flux(q, c, g.plane, fzB);
   const T FzT = face_flux(q, c + g.plane, g.plane, fzT);
-  const T fdiv = (FxR - FxL) / (T)g.dx + (FyR - FyL) / (T)g.dy + (FzT - FzB) / dzc;
-  return (-fdiv + q[c] * div[c]) / rt;
+  const T fdiv = (FxR - FxL) * r.x + (FyR - FyL) * r.y + (FzT - FzB) * r.z;
+  return (-fdiv + q[c] * div[c]) * rrt;
 }
 
 // Constant-K Laplacian with the local physical vertical spacing
-// (k_diffuse kernels.cu:430-443).
-template <class T>
-__device__ __forceinline__ T laplacian(const Grid& g, const T* __restrict__ f, int c, T dzl) {
+// (k_diffuse kernels.cu:430-443); rdz2 = 1 / (dzeta J)^2.
+template <class A, class T>
+__device__ __forceinline__ T laplacian(const Grid& g, const A& f, int c, T rdz2) {
   const T cc = f[c];
-  return (f[c + 1] - (T)2 * cc + f[c - 1]) / (T)(g.dx * g.dx) +
-         (f[c + g.NX] - (T)2 * cc + f[c - g.NX]) / (T)(g.dy * g.dy) +
-         (f[c + g.plane] - (T)2 * cc + f[c - g.plane]) / (dzl * dzl);
+  return (f[c + 1] - (T)2 * cc + f[c - 1]) * (T)(g.rdx * g.rdx) +
+         (f[c + g.NX] - (T)2 * cc + f[c - g.NX]) * (T)(g.rdy * g.rdy) +
+         (f[c + g.plane] - (T)2 * cc + f[c - g.plane]) * rdz2;
 }
 
 struct TendOut {
@@ -223,15 +231,18 @@ struct TendOut {
 // All slow tendencies of one interior point (compute_tendencies,
 // kernels.cu:872-913): u, v, w (k >= 1), theta' (+ Rayleigh), the three
 // moisture species when the non-PD scheme is used, and the const-K
-// diffusion of u, v, w, theta' when diff_K > 0.
+// diffusion of u, v, w, theta' when diff_K > 0. Divisions by grid spacings
+// and densities are products with reciprocals (rcp_nr for the densities).
 template <class T>
-__global__ void k_tendencies(Grid g, Params p, Base<T> b, Metric<T> m, T dt,
-                             const T* __restrict__ u, const T* __restrict__ v,
-                             const T* __restrict__ w, const T* __restrict__ thp,
-                             const T* __restrict__ qv, const T* __restrict__ qc,
-                             const T* __restrict__ qr, const T* __restrict__ mfx,
-                             const T* __restrict__ mfy, const T* __restrict__ mfz,
-                             const T* __restrict__ div, TendOut out, int do_q) {
+__global__ void __launch_bounds__(256) k_tendencies(Grid g, Params p, Base<T> b, Metric<T> m, T dt,
+                             const T* __restrict__ u_, const T* __restrict__ v_,
+                             const T* __restrict__ w_, const T* __restrict__ thp_,
+                             const T* __restrict__ qv_, const T* __restrict__ qc_,
+                             const T* __restrict__ qr_, const T* __restrict__ mfx_,
+                             const T* __restrict__ mfy_, const T* __restrict__ mfz_,
+                             const T* __restrict__ div_, TendOut out, int do_q) {
+  const CArr<T> u{u_}, v{v_}, w{w_}, thp{thp_}, qv{qv_}, qc{qc_}, qr{qr_};
+  const CArr<T> mfx{mfx_}, mfy{mfy_}, mfz{mfz_}, div{div_};
   const BlockId bid = xcd_block();
   const int i = bid.x * blockDim.x + threadIdx.x;
   const int j = bid.y * blockDim.y + threadIdx.y;
@@ -241,10 +252,13 @@ __global__ void k_tendencies(Grid g, Params p, Base<T> b, Metric<T> m, T dt,
   const int c = g.id(i, j, k);
   const int c2 = g.id2(i, j);
   const int NX = g.NX, PL = g.plane;
-  const T dx = (T)g.dx, dy = (T)g.dy;
-  const T dzc = m.dzeta_c[kk];
-  const T rt = b.rhob[c] * m.jac[c2];
+  const T rdzc = (T)m.rdzc[kk];
+  const Rsp<T> rc{(T)g.rdx, (T)g.rdy, rdzc};
+  const T jc = m.jac[c2];
+  const T rt = b.rhob[c] * jc;
   const T K = (T)p.diff_K;
+  const T rjc = rcp_nr(jc);
+  auto rdz2 = [&](T rdz) { return rdz * rdz * rjc * rjc; };  // 1 / (dzeta J)^2
 
   // ---- u (k_tend_u, kernels.cu:308-346)
   {
@@ -260,16 +274,16 @@ __global__ void k_tendencies(Grid g, Params p, Base<T> b, Metric<T> m, T dt,
     const T fzT = (T)0.5 * (mfz[c - 1 + PL] + mfz[c + PL]);
     const T FzB = face_flux(u, c, PL, fzB);
     const T FzT = face_flux(u, c + PL, PL, fzT);
-    const T fdiv = (FxR - FxL) / dx + (FyT - FyB) / dy + (FzT - FzB) / dzc;
+    const T fdiv = (FxR - FxL) * rc.x + (FyT - FyB) * rc.y + (FzT - FzB) * rc.z;
     const T dv = (T)0.5 * (div[c - 1] + div[c]);
     const T ru = (T)0.5 * (b.rhob[c - 1] * m.jac[c2 - 1] + rt);
     const T vavg = (T)0.25 * (v[c - 1] + v[c - 1 + NX] + v[c] + v[c + NX]);
     const T fc = (T)0.5 * (m.fcor[c2 - 1] + m.fcor[c2]);
     const T vref = p.coriolis_use_ub ? b.vb[kk] : (T)0;
     const T relax = -m.ray_c[kk] * (u[c] - b.ub[kk]);
-    T t = (-fdiv + u[c] * dv) / ru + fc * (vavg - vref) + relax;
-    if (K > (T)0) t += K * laplacian(g, u, c, dzc * m.jac[c2]);
-    static_cast<T*>(out.tu)[c] = t;
+    T t = (-fdiv + u[c] * dv) * rcp_nr(ru) + fc * (vavg - vref) + relax;
+    if (K > (T)0) t += K * laplacian(g, u, c, rdz2(rdzc));
+    Arr<T>(static_cast<T*>(out.tu))[c] = t;
   }
   // ---- v (k_tend_v, kernels.cu:348-385)
   {
@@ -285,20 +299,20 @@ __global__ void k_tendencies(Grid g, Params p, Base<T> b, Metric<T> m, T dt,
     const T fzT = (T)0.5 * (mfz[c - NX + PL] + mfz[c + PL]);
     const T FzB = face_flux(v, c, PL, fzB);
     const T FzT = face_flux(v, c + PL, PL, fzT);
-    const T fdiv = (FxR - FxL) / dx + (FyT - FyB) / dy + (FzT - FzB) / dzc;
+    const T fdiv = (FxR - FxL) * rc.x + (FyT - FyB) * rc.y + (FzT - FzB) * rc.z;
     const T dv = (T)0.5 * (div[c - NX] + div[c]);
     const T rv = (T)0.5 * (b.rhob[c - NX] * m.jac[c2 - NX] + rt);
     const T uavg = (T)0.25 * (u[c - NX] + u[c + 1 - NX] + u[c] + u[c + 1]);
     const T fc = (T)0.5 * (m.fcor[c2 - NX] + m.fcor[c2]);
     const T uref = p.coriolis_use_ub ? b.ub[kk] : (T)0;
     const T relax = -m.ray_c[kk] * (v[c] - b.vb[kk]);
-    T t = (-fdiv + v[c] * dv) / rv - fc * (uavg - uref) + relax;
-    if (K > (T)0) t += K * laplacian(g, v, c, dzc * m.jac[c2]);
-    static_cast<T*>(out.tv)[c] = t;
+    T t = (-fdiv + v[c] * dv) * rcp_nr(rv) - fc * (uavg - uref) + relax;
+    if (K > (T)0) t += K * laplacian(g, v, c, rdz2(rdzc));
+    Arr<T>(static_cast<T*>(out.tv))[c] = t;
   }
   // ---- w on k >= 1 (k_tend_w, kernels.cu:387-425)
   if (k >= 1) {
-    const T dzw = m.dzeta_w[kk];
+    const T rdzw = (T)m.rdzw[kk];
     const T fxL = (T)0.5 * (mfx[c - PL] + mfx[c]);
     const T fxR = (T)0.5 * (mfx[c + 1 - PL] + mfx[c + 1]);
     const T FxL = face_flux(w, c, 1, fxL);
@@ -311,31 +325,31 @@ __global__ void k_tendencies(Grid g, Params p, Base<T> b, Metric<T> m, T dt,
     const T fzT = (T)0.5 * (mfz[c] + mfz[c + PL]);
     const T FzB = face_flux(w, c, PL, fzB);
     const T FzT = face_flux(w, c + PL, PL, fzT);
-    const T fdiv = (FxR - FxL) / dx + (FyT - FyB) / dy + (FzT - FzB) / dzw;
+    const T fdiv = (FxR - FxL) * rc.x + (FyT - FyB) * rc.y + (FzT - FzB) * rdzw;
     const T dv = (T)0.5 * (div[c - PL] + div[c]);
-    const T rw = b.rhobw[c] * m.jac[c2];
+    const T rw = b.rhobw[c] * jc;
     T relax = -m.ray_w[kk] * w[c];
     const T wk = w[c];
     if (p.w_damping) {
-      const T cr = fabs(wk) * dt / (dzw * m.jac[c2]);
-      if (cr > (T)1) relax -= (T)0.3 * (cr - (T)1) * wk / dt;
+      const T cr = fabs(wk) * dt * rdzw * rjc;
+      if (cr > (T)1) relax -= (T)0.3 * (cr - (T)1) * wk * rcp_nr(dt);
     }
-    T t = (-fdiv + wk * dv) / rw + relax;
-    if (K > (T)0) t += K * laplacian(g, w, c, dzw * m.jac[c2]);
-    static_cast<T*>(out.tw)[c] = t;
+    T t = (-fdiv + wk * dv) * rcp_nr(rw) + relax;
+    if (K > (T)0) t += K * laplacian(g, w, c, rdz2(rdzw));
+    Arr<T>(static_cast<T*>(out.tw))[c] = t;
   }
+  const T rrt = rcp_nr(rt);
   // ---- theta' (k_tend_thp, kernels.cu:114-146)
   {
-    T t = scalar_adv(g, thp, mfx, mfy, mfz, div, c, dzc, rt) -
-          m.ray_c[kk] * thp[c];
-    if (K > (T)0) t += K * laplacian(g, thp, c, dzc * m.jac[c2]);
-    static_cast<T*>(out.tth)[c] = t;
+    T t = scalar_adv(g, thp, mfx, mfy, mfz, div, c, rc, rrt) - m.ray_c[kk] * thp[c];
+    if (K > (T)0) t += K * laplacian(g, thp, c, rdz2(rdzc));
+    Arr<T>(static_cast<T*>(out.tth))[c] = t;
   }
   // ---- moisture, non-PD scheme (k_tend_q, kernels.cu:149-179)
   if (do_q) {
-    static_cast<T*>(out.tq[0])[c] = scalar_adv(g, qv, mfx, mfy, mfz, div, c, dzc, rt);
-    static_cast<T*>(out.tq[1])[c] = scalar_adv(g, qc, mfx, mfy, mfz, div, c, dzc, rt);
-    static_cast<T*>(out.tq[2])[c] = scalar_adv(g, qr, mfx, mfy, mfz, div, c, dzc, rt);
+    Arr<T>(static_cast<T*>(out.tq[0]))[c] = scalar_adv(g, qv, mfx, mfy, mfz, div, c, rc, rrt);
+    Arr<T>(static_cast<T*>(out.tq[1]))[c] = scalar_adv(g, qc, mfx, mfy, mfz, div, c, rc, rrt);
+    Arr<T>(static_cast<T*>(out.tq[2]))[c] = scalar_adv(g, qr, mfx, mfy, mfz, div, c, rc, rrt);
   }
 }
 
