@@ -56,6 +56,22 @@ KERNEL_BYTES = {
     "fp64": {"acou_wpi": 20 * 8, "acou_uv": 11 * 8, "acoustic_fused": 26 * 8,
              "tendencies": 20 * 8, "massflux": 9 * 8, "stage_q": 9 * 8, "kessler": 12 * 8},
 }
+# Over flat ground with a horizontally uniform base state the acoustic kernels
+# read their base-state and coefficient profiles from one column: these
+# per-column arrays then drop out of the kernel's algorithmic bytes.
+#  acou_wpi      : thbw dthbdz rtjx rtjy kt3 aw qvb
+#  acou_uv       : thb
+#  acoustic_fused: the eight above
+UNIFORM_COLUMN_ARRAYS = {"acou_wpi": 7, "acou_uv": 1, "acoustic_fused": 8}
+
+
+def kernel_bytes(kernel: str, precision: str, uniform: bool):
+    kb = KERNEL_BYTES[precision].get(kernel)
+    if kb is None or not uniform:
+        return kb
+    return kb - UNIFORM_COLUMN_ARRAYS.get(kernel, 0) * (4 if precision == "mixed" else 8)
+
+
 # Kernel symbol of each timed launch name (for the rocprofv3 PMC summary).
 KERNEL_SYMBOL = {"acou_wpi": "k_acou_wpi_tile", "acou_uv": "k_acou_uv_tile",
                  "acoustic_fused": "k_acou_row", "tendencies": "k_tendencies",
@@ -410,7 +426,7 @@ def main() -> int:
     rep, tune, solo = kernel_profile(case, "mixed", 5, args.profile_steps, args.solo_reps)
     dom = max(rep, key=lambda k: rep[k][1])
     cnt, tot_ms = rep[dom]
-    kb = KERNEL_BYTES["mixed"].get(dom)
+    kb = kernel_bytes(dom, "mixed", bool(tune.get("uniform_columns")))
     roof = None
     if kb is not None and dom in solo:
         avg_s = solo[dom] * 1e-6
@@ -421,6 +437,7 @@ def main() -> int:
                 "avg_launch_us": solo[dom], "avg_launch_us_in_step": 1e3 * tot_ms / cnt,
                 "launches_per_step": cnt / args.profile_steps,
                 "alg_bytes_per_launch": kb * N,
+                "uniform_columns": bool(tune.get("uniform_columns")),
                 "traffic_source": str(PMC_SUMMARY.relative_to(ROOT)) if traffic else None}
     step_bw = ALG_BYTES_STEP["mixed"] * N * K / t_m
 

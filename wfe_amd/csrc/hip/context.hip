@@ -293,7 +293,8 @@ class Ctx final : public CtxBase {
   template <class E>
   E* upload_new(const double* h, Shape shape);
   void fill_ghosts(const StateSet<T>& s);
-  void slow_tendencies(const StateSet<T>& est, double dt);
+  void slow_tendencies(const StateSet<T>& est, double dt, const StateSet<T>* qout = nullptr,
+                       double dt_stage = 0);
   void bdy_relax(const StateSet<T>& est);
   void acoustic_stage(const StateSet<T>& out, double dtau, int nsub);
   void stage_moisture(const StateSet<T>& out, double dt_stage);
@@ -317,7 +318,7 @@ class Ctx final : public CtxBase {
   dev::Metric<T> met_{};
   dev::Coef<T> coef_{};
   StateSet<T> S_[2]{};
-  StateSet<T> SE_{};
+  StateSet<T> SE_[2]{};  // stage estimates (ping-pong: a stage never writes what it reads)
   int cur_ = 0;
   T* wu_[2]{};
   T* wv_[2]{};
@@ -559,7 +560,8 @@ Ctx<T>::Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& b, const w
   };
   alloc_set(S_[0]);
   alloc_set(S_[1]);
-  alloc_set(SE_);
+  alloc_set(SE_[0]);
+  alloc_set(SE_[1]);
   for (int q = 0; q < 2; ++q) {
     wu_[q] = arena_.alloc<T>(n);
     wv_[q] = arena_.alloc<T>(n);
@@ -732,7 +734,8 @@ void Ctx<T>::fill_ghosts(const StateSet<T>& s) {
 }
 
 template <class T>
-void Ctx<T>::slow_tendencies(const StateSet<T>& s, double dt) {
+void Ctx<T>::slow_tendencies(const StateSet<T>& s, double dt, const StateSet<T>* qout,
+                             double dt_stage) {
   {
     dim3 grd(cdiv(g_.nx + 5, 64), cdiv(g_.ny + 5, 4), (unsigned)g_.nz);
     TIMED("massflux", dev::k_massflux<T><<<grd, blk3(), 0, stream_>>>(g_, base_, met_, s.u, s.v, s.w, mfx_, mfy_,
@@ -740,11 +743,23 @@ void Ctx<T>::slow_tendencies(const StateSet<T>& s, double dt) {
   }
   const int do_q = p_.moisture && !p_.pd_moist;
   dev::TendOut o{tu_, tv_, tw_, tth_, {tq_[0], tq_[1], tq_[2]}};
+  dev::QStage<T> qs{};
+  if (qout) {
+    const StateSet<T>& sn = S_[cur_];
+    qs.qn[0] = sn.qv;
+    qs.qn[1] = sn.qc;
+    qs.qn[2] = sn.qr;
+    qs.qo[0] = qout->qv;
+    qs.qo[1] = qout->qc;
+    qs.qo[2] = qout->qr;
+    qs.dts = (T)dt_stage;
+    qs.on = 1;
+  }
   {
     dim3 grd(cdiv(g_.nx, 64), cdiv(g_.ny, 4), (unsigned)g_.nz);
     TIMED("tendencies", dev::k_tendencies<T><<<grd, blk3(), 0, stream_>>>(g_, p_, base_, met_, (T)dt, s.u, s.v,
                                                       s.w, s.thp, s.qv, s.qc, s.qr, mfx_,
-                                                      mfy_, mfz_, div_, o, do_q));
+                                                      mfy_, mfz_, div_, o, do_q, qs));
   }
   if (p_.moisture && p_.pd_moist) {
     dev::PDSet<T> S{};
@@ -758,7 +773,7 @@ void Ctx<T>::slow_tendencies(const StateSet<T>& s, double dt) {
     TIMED("pd_ratio", dev::k_pd_ratio<T><<<gi, blk3(), 0, stream_>>>(g_, base_, met_, (T)dt, mfx_,
                                                                     mfy_, mfz_, S));
     TIMED("pd_apply", dev::k_pd_apply<T><<<gi, blk3(), 0, stream_>>>(g_, p_, base_, met_, mfx_,
-                                                                    mfy_, mfz_, div_, S));
+                                                                    mfy_, mfz_, div_, S, qs));
   }
   check_launch("slow_tendencies");
 }
@@ -1112,7 +1127,7 @@ double Ctx<T>::kernel_bench(const std::string& name, int reps) {
   } else if (name == "massflux" || name == "tendencies") {
     slow_tendencies(sn, 1.0);
   } else if (name == "stage_q") {
-    stage_moisture(SE_, 1.0);
+    stage_moisture(SE_[0], 1.0);
   }
   WFE_HIP(hipStreamSynchronize(stream_));
   timer_.on = timer_was_on;
@@ -1171,14 +1186,17 @@ void Ctx<T>::step_launches(double dt) {
   const int ns_stage[3] = {1, std::max(1, ns / 2), ns};
   const StateSet<T>& sn = S_[cur_];
   const StateSet<T>& snext = S_[cur_ ^ 1];
+  // Without boundary relaxation (which adds to T_q after the tendencies)
+  // the moisture stage update is applied where T_q is formed.
+  const bool fuse_q = p_.moisture && !bdy_active_;
   for (int m = 0; m < 3; ++m) {
-    const StateSet<T>& est = m == 0 ? sn : SE_;
+    const StateSet<T>& est = m == 0 ? sn : SE_[m - 1];
+    const StateSet<T>& out = m == 2 ? snext : SE_[m];
     fill_ghosts(est);
-    slow_tendencies(est, dt);
+    slow_tendencies(est, dt, fuse_q ? &out : nullptr, stage_dt[m]);
     if (bdy_active_) bdy_relax(est);
-    const StateSet<T>& out = m == 2 ? snext : SE_;
     acoustic_stage(out, stage_dt[m] / ns_stage[m], ns_stage[m]);
-    if (p_.moisture) stage_moisture(out, stage_dt[m]);
+    if (p_.moisture && !fuse_q) stage_moisture(out, stage_dt[m]);
   }
   cur_ ^= 1;
   if (p_.moisture && p_.microphysics) kessler_step(S_[cur_], dt);

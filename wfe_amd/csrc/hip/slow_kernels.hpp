@@ -228,6 +228,17 @@ struct TendOut {
   void *tu, *tv, *tw, *tth, *tq[3];
 };
 
+// Moisture stage update q_out = q_n + dt_stage T_q (k_stage_q,
+// kernels.cu:268-273) applied where T_q is formed, so T_q is never stored;
+// on = 0 stores T_q instead.
+template <class T>
+struct QStage {
+  CArr<T> qn[3];
+  Arr<T> qo[3];
+  T dts;
+  int on;
+};
+
 // All slow tendencies of one interior point (compute_tendencies,
 // kernels.cu:872-913): u, v, w (k >= 1), theta' (+ Rayleigh), the three
 // moisture species when the non-PD scheme is used, and the const-K
@@ -240,7 +251,8 @@ __global__ void __launch_bounds__(256) k_tendencies(Grid g, Params p, Base<T> b,
                              const T* __restrict__ qv_, const T* __restrict__ qc_,
                              const T* __restrict__ qr_, const T* __restrict__ mfx_,
                              const T* __restrict__ mfy_, const T* __restrict__ mfz_,
-                             const T* __restrict__ div_, TendOut out, int do_q) {
+                             const T* __restrict__ div_, TendOut out, int do_q,
+                             QStage<T> qs) {
   const CArr<T> u{u_}, v{v_}, w{w_}, thp{thp_}, qv{qv_}, qc{qc_}, qr{qr_};
   const CArr<T> mfx{mfx_}, mfy{mfy_}, mfz{mfz_}, div{div_};
   const BlockId bid = xcd_block();
@@ -347,9 +359,18 @@ __global__ void __launch_bounds__(256) k_tendencies(Grid g, Params p, Base<T> b,
   }
   // ---- moisture, non-PD scheme (k_tend_q, kernels.cu:149-179)
   if (do_q) {
-    Arr<T>(static_cast<T*>(out.tq[0]))[c] = scalar_adv(g, qv, mfx, mfy, mfz, div, c, rc, rrt);
-    Arr<T>(static_cast<T*>(out.tq[1]))[c] = scalar_adv(g, qc, mfx, mfy, mfz, div, c, rc, rrt);
-    Arr<T>(static_cast<T*>(out.tq[2]))[c] = scalar_adv(g, qr, mfx, mfy, mfz, div, c, rc, rrt);
+    const T t0 = scalar_adv(g, qv, mfx, mfy, mfz, div, c, rc, rrt);
+    const T t1 = scalar_adv(g, qc, mfx, mfy, mfz, div, c, rc, rrt);
+    const T t2 = scalar_adv(g, qr, mfx, mfy, mfz, div, c, rc, rrt);
+    if (qs.on) {
+      qs.qo[0][c] = qs.qn[0][c] + qs.dts * t0;
+      qs.qo[1][c] = qs.qn[1][c] + qs.dts * t1;
+      qs.qo[2][c] = qs.qn[2][c] + qs.dts * t2;
+    } else {
+      Arr<T>(static_cast<T*>(out.tq[0]))[c] = t0;
+      Arr<T>(static_cast<T*>(out.tq[1]))[c] = t1;
+      Arr<T>(static_cast<T*>(out.tq[2]))[c] = t2;
+    }
   }
 }
 
@@ -415,15 +436,15 @@ __global__ void k_pd_ratio(Grid g, Base<T> b, Metric<T> m, T dt, const T* __rest
 template <class T>
 __global__ void k_pd_apply(Grid g, Params p, Base<T> b, Metric<T> m, const T* __restrict__ mfx,
                            const T* __restrict__ mfy, const T* __restrict__ mfz,
-                           const T* __restrict__ mfdiv, PDSet<T> S) {
+                           const T* __restrict__ mfdiv, PDSet<T> S, QStage<T> qs) {
   const BlockId bid = xcd_block();
   const int i = bid.x * blockDim.x + threadIdx.x;
   const int j = bid.y * blockDim.y + threadIdx.y;
   const int k = bid.z;
   if (i >= g.nx || j >= g.ny || k >= g.nz) return;
   const int c = g.id(i, j, k);
-  const T dzc = m.dzeta_c[k + g.ng];
-  const T rt = b.rhob[c] * m.jac[g.id2(i, j)];
+  const T rdzc = (T)m.rdzc[k + g.ng];
+  const T rrt = rcp_nr(b.rhob[c] * m.jac[g.id2(i, j)]);
   const PDFlux<T> F{g, mfx, mfy, mfz};
   auto ridx = [&](int ii, int jj, int kz) {
     ii = map_lat(ii, g.nx, g.nx - 1, p.bcx);
@@ -446,9 +467,13 @@ __global__ void k_pd_apply(Grid g, Params p, Base<T> b, Metric<T> m, const T* __
       const T f = F.fz(q, g.id(i, j, k0));
       return f * R[ridx(i, j, f >= (T)0 ? k0 - 1 : k0)];
     };
-    const T fdiv = (sx(i + 1) - sx(i)) / (T)g.dx + (sy(j + 1) - sy(j)) / (T)g.dy +
-                   (sz(k + 1) - sz(k)) / dzc;
-    S.tq[s][c] = (-fdiv + q[c] * mfdiv[c]) / rt;
+    const T fdiv = (sx(i + 1) - sx(i)) * (T)g.rdx + (sy(j + 1) - sy(j)) * (T)g.rdy +
+                   (sz(k + 1) - sz(k)) * rdzc;
+    const T tq = (-fdiv + q[c] * mfdiv[c]) * rrt;
+    if (qs.on)
+      qs.qo[s][c] = qs.qn[s][c] + qs.dts * tq;
+    else
+      S.tq[s][c] = tq;
   }
 }
 
