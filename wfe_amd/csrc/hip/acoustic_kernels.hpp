@@ -451,14 +451,15 @@ __global__ void __launch_bounds__(64) k_acou_wpi_col(Grid g, Params p, Base<T> b
 // the levels k = r, r + NW, ... . Phases, separated by barriers:
 //   1. every thread assembles, for its levels, P_k, E_k, Av_k, G_k, RHS_k
 //      in registers (all loads of the column are issued here, in parallel
-//      over k) and publishes P_k;
+//      over k) and publishes P_k, E_k, Av_k in the D, C, B rows of LDS;
 //   2. every thread forms the tridiagonal row k (A_k, B_k, C_k, D_k of
 //      kernels.cu:589-628) from its registers and P_{k-1}, E_{k-1},
-//      Av_{k-1}, Av_{k+1}, so the serial part carries only the recurrence;
-//   3. thread row 0 (one lane per column) runs the elimination and the back
-//      substitution in the reference's order, with one reciprocal of the
-//      pivot per level (rcp_nr);
-//   4. every thread writes w, pi', pi'prev and theta' of its levels.
+//      Av_{k-1} read from LDS, and writes it after a barrier, so the serial
+//      part carries only the recurrence;
+//   3. the twisted tridiagonal solve (twist_sweep_top / _bottom,
+//      twist_row_coefs, twist_meet, twist_back): thread rows 0 and 1 (one
+//      lane per column each) eliminate from the top and the bottom;
+//   4. every thread writes w, pi' and theta' of its levels.
 // LDS: 4 (nz + 1) TX doubles. Levels per thread are bounded by KPT.
 template <int TX, int NW, int KPT>
 struct WpiTile {
@@ -470,87 +471,250 @@ struct WpiTile {
 // WPE: minimum waves per SIMD the register allocation must allow (1 = no
 // constraint); WPE = 4 caps VGPRs at 128 so two 512-thread blocks share a CU
 // and one block's serial elimination overlaps the other's loads.
-// Thomas elimination of the rows k = 1 .. nz-1 of one column held in LDS
-// (row k of lane lx at k * TX + lx; C and D are overwritten by the
-// eliminated c_k, d_k), in the reference's order (kernels.cu:589-628). Rows
-// are read in chunks of four, one chunk ahead of the recurrence (indices
-// clamped into range; a clamped read is never used), so only the FP64 chain
-// is serial. On return sD holds W_k for k in [0, nz], W_0 = w0, W_nz = 0.
+// Tridiagonal solve of the rows k = 1 .. nz-1 of TX columns held in LDS
+// (row k of lane lx at k * TX + lx; A_k x_{k-1} + B_k x_k + C_k x_{k+1} = D_k)
+// by a twisted factorization: two lanes per column, in two wavefronts,
+// eliminate at the same time from the top (rows 1 .. h-1) and from the
+// bottom (rows n1 .. h+1, n1 = nz-1), meet at row h, and substitute back
+// outwards, so each serial chain is half the column. No division sits on a
+// serial chain:
+//   1. sweeps over principal minors. Top: p_k = B_k p_{k-1} - A_k C_{k-1}
+//      p_{k-2} (p_0 = 1, p_{-1} = 0) and e_k = D_k p_{k-1} - A_k e_{k-1}
+//      (e_0 = 0), stored as p_{k-1}, p_k, e_k over A_k, B_k, D_k. Bottom:
+//      q_k = B_k q_{k+1} - C_k A_{k+1} q_{k+2} (q_{n1+1} = 1, q_{n1+2} = 0)
+//      and f_k = D_k q_{k+1} - C_k f_{k+1} (f_{n1+1} = 0), stored as
+//      q_{k+1}, q_k, f_k over C_k, B_k, D_k. One dependent FMA per row and
+//      chain; after every chunk of rows the running minors are rescaled by
+//      the same power of two (exact), each row keeping its own scale;
+//   2. (all threads, parallel over k) the eliminated rows. Top: x_k = d_k -
+//      c_k x_{k+1} with c_k = C_k p_{k-1} / p_k, d_k = e_k / p_k (the
+//      reference's Thomas coefficients, kernels.cu:589-628). Bottom: x_k =
+//      g_k - a_k x_{k-1} with a_k = A_k q_{k+1} / q_k, g_k = f_k / q_k;
+//   3. x_h = (D_h - A_h d_{h-1} - C_h g_{h+1}) / (B_h - A_h c_{h-1} - C_h
+//      a_{h+1}) in both lanes, then (after a barrier) back substitution
+//      upwards (top lane) and downwards (bottom lane). sD ends as W_k,
+//      W_0 = w0, W_nz = 0.
+// The phases are separated by workgroup barriers in the caller. Rows are
+// read in chunks of kTwistChunk, one chunk ahead of the recurrence (row
+// indices clamped into the padded range; a clamped read is never used), and
+// the running minors are rescaled after every chunk.
+__host__ __device__ constexpr int twist_row(int nz) { return nz / 2; }  // h, 1 <= h <= n1
+// Rows read ahead of each serial chain (two chunk buffers of this many rows
+// stay in registers).
+inline constexpr int kTwistChunk = 4;
+
 template <int TX>
-__device__ __forceinline__ void thomas_lds(double* sA, double* sB, double* sC, double* sD,
-                                           int lx, int nz, double w0) {
-  constexpr int U = 4;
-  const int n1 = nz - 1;  // rows k = 1 .. nz-1
+__device__ __forceinline__ void twist_sweep_top(double* sA, double* sB, const double* sC,
+                                                double* sD, int lx, int nz) {
+  constexpr int U = kTwistChunk;
+  const int last = twist_row(nz) - 1;  // rows 1 .. h-1
   auto at = [&](int k) { return k * TX + lx; };
-  double A[U], B[U], C[U], D[U], nA[U], nB[U], nC[U], nD[U];
+  double A[U], B[U], Cm[U], D[U], nA[U], nB[U], nCm[U], nD[U];  // Cm = C_{k-1}
 #pragma unroll
   for (int u = 0; u < U; ++u) {
-    const int r = min(1 + u, n1);
+    const int r = max(min(1 + u, last), 0);
     A[u] = sA[at(r)];
+    B[u] = sB[at(r)];
+    Cm[u] = r >= 2 ? sC[at(r - 1)] : 0.0;
+    D[u] = sD[at(r)];
+  }
+  double p1 = 1.0, p2 = 0.0, e1 = 0.0;  // p_{k-1}, p_{k-2}, e_{k-1}
+  for (int k0 = 1; k0 <= last; k0 += U) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int r = min(k0 + U + u, last);
+      nA[u] = sA[at(r)];
+      nB[u] = sB[at(r)];
+      nCm[u] = sC[at(max(r - 1, 1))];
+      nD[u] = sD[at(r)];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + u;
+      if (k <= last) {
+        const double pk = fma(B[u], p1, -(A[u] * Cm[u]) * p2);
+        const double ek = fma(-A[u], e1, D[u] * p1);
+        sA[at(k)] = p1;
+        sB[at(k)] = pk;
+        sD[at(k)] = ek;
+        p2 = p1;
+        p1 = pk;
+        e1 = ek;
+      }
+    }
+    const int ex = __builtin_amdgcn_frexp_exp(p1);
+    p1 = ldexp(p1, -ex);
+    p2 = ldexp(p2, -ex);
+    e1 = ldexp(e1, -ex);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      A[u] = nA[u];
+      B[u] = nB[u];
+      Cm[u] = nCm[u];
+      D[u] = nD[u];
+    }
+  }
+}
+
+template <int TX>
+__device__ __forceinline__ void twist_sweep_bottom(const double* sA, double* sB, double* sC,
+                                                   double* sD, int lx, int nz) {
+  constexpr int U = kTwistChunk;
+  const int n1 = nz - 1, first = twist_row(nz) + 1;  // rows n1 .. h+1
+  auto at = [&](int k) { return k * TX + lx; };
+  double Ap[U], B[U], C[U], D[U], nAp[U], nB[U], nC[U], nD[U];  // Ap = A_{k+1}
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int r = min(max(n1 - u, first), n1);
+    Ap[u] = r <= n1 - 1 ? sA[at(r + 1)] : 0.0;
     B[u] = sB[at(r)];
     C[u] = sC[at(r)];
     D[u] = sD[at(r)];
   }
-  double cs = 0.0, ds = 0.0;
-  for (int k0 = 1; k0 <= n1; k0 += U) {
+  double q1 = 1.0, q2 = 0.0, f1 = 0.0;  // q_{k+1}, q_{k+2}, f_{k+1}
+  for (int k0 = n1; k0 >= first; k0 -= U) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int r = min(k0 + U + u, n1);
-      nA[u] = sA[at(r)];
+      const int r = max(k0 - U - u, first);
+      nAp[u] = sA[at(min(r + 1, n1))];
       nB[u] = sB[at(r)];
       nC[u] = sC[at(r)];
       nD[u] = sD[at(r)];
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int k = k0 + u;
-      if (k <= n1) {
-        const double r = rcp_nr(B[u] - A[u] * cs);
-        ds = (D[u] - A[u] * ds) * r;
-        cs = C[u] * r;
-        sC[at(k)] = cs;
-        sD[at(k)] = ds;
+      const int k = k0 - u;
+      if (k >= first) {
+        const double qk = fma(B[u], q1, -(C[u] * Ap[u]) * q2);
+        const double fk = fma(-C[u], f1, D[u] * q1);
+        sC[at(k)] = q1;
+        sB[at(k)] = qk;
+        sD[at(k)] = fk;
+        q2 = q1;
+        q1 = qk;
+        f1 = fk;
       }
     }
+    const int ex = __builtin_amdgcn_frexp_exp(q1);
+    q1 = ldexp(q1, -ex);
+    q2 = ldexp(q2, -ex);
+    f1 = ldexp(f1, -ex);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      A[u] = nA[u];
+      Ap[u] = nAp[u];
       B[u] = nB[u];
       C[u] = nC[u];
       D[u] = nD[u];
     }
   }
-  // W_k = ds_k - cs_k W_{k+1}, k = nz-1 .. 1, with W_nz = 0
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    const int r = max(n1 - u, 1);
-    C[u] = sC[at(r)];
-    D[u] = sD[at(r)];
+}
+
+// Phase 2 for row k (any thread): the eliminated row from its minors.
+__device__ __forceinline__ void twist_row_coefs(double* sA, const double* sB, double* sC,
+                                                double* sD, int at, int k, int nz) {
+  const int h = twist_row(nz);
+  if (k < h) {  // c_k over C_k, d_k over D_k
+    const double r = rcp_nr(sB[at]);
+    sC[at] = sC[at] * sA[at] * r;
+    sD[at] = sD[at] * r;
+  } else if (k > h) {  // a_k over A_k, g_k over D_k
+    const double r = rcp_nr(sB[at]);
+    sA[at] = sA[at] * sC[at] * r;
+    sD[at] = sD[at] * r;
   }
-  double wab = 0.0;
-  for (int k0 = n1; k0 >= 1; k0 -= U) {
+}
+
+// Phase 3a: the meeting row's unknown x_h (read-only; the caller holds a
+// barrier between this and the writes of phase 3b).
+template <int TX>
+__device__ __forceinline__ double twist_meet(const double* sA, const double* sB,
+                                             const double* sC, const double* sD, int lx,
+                                             int nz) {
+  const int n1 = nz - 1, h = twist_row(nz);
+  auto at = [&](int k) { return k * TX + lx; };
+  double den = sB[at(h)], num = sD[at(h)];
+  if (h - 1 >= 1) {
+    den -= sA[at(h)] * sC[at(h - 1)];
+    num -= sA[at(h)] * sD[at(h - 1)];
+  }
+  if (h + 1 <= n1) {
+    den -= sC[at(h)] * sA[at(h + 1)];
+    num -= sC[at(h)] * sD[at(h + 1)];
+  }
+  return num / den;
+}
+
+// Phase 3b: back substitution from x_h upwards (top) or downwards (bottom).
+template <int TX>
+__device__ __forceinline__ void twist_back(const double* sA, const double* sC, double* sD,
+                                           int lx, int nz, bool top, double xh, double w0) {
+  constexpr int U = kTwistChunk;
+  const int n1 = nz - 1, h = twist_row(nz);
+  auto at = [&](int k) { return k * TX + lx; };
+  if (top) {  // x_k = d_k - c_k x_{k+1}, k = h-1 .. 1
+    double C[U], D[U], nC[U], nD[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int r = max(k0 - U - u, 1);
-      nC[u] = sC[at(r)];
-      nD[u] = sD[at(r)];
+      const int r = max(h - 1 - u, 0);
+      C[u] = sC[at(r)];
+      D[u] = sD[at(r)];
     }
+    double x = xh;
+    for (int k0 = h - 1; k0 >= 1; k0 -= U) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int k = k0 - u;
-      if (k >= 1) {
-        wab = D[u] - C[u] * wab;
-        sD[at(k)] = wab;
+      for (int u = 0; u < U; ++u) {
+        const int r = max(k0 - U - u, 0);
+        nC[u] = sC[at(r)];
+        nD[u] = sD[at(r)];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = k0 - u;
+        if (k >= 1) {
+          x = D[u] - C[u] * x;
+          sD[at(k)] = x;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        C[u] = nC[u];
+        D[u] = nD[u];
       }
     }
+    sD[at(h)] = xh;
+    sD[lx] = w0;
+  } else {  // x_k = g_k - a_k x_{k-1}, k = h+1 .. n1
+    double Av[U], D[U], nAv[U], nD[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      C[u] = nC[u];
-      D[u] = nD[u];
+      const int r = min(h + 1 + u, nz);
+      Av[u] = sA[at(r)];
+      D[u] = sD[at(r)];
     }
+    double x = xh;
+    for (int k0 = h + 1; k0 <= n1; k0 += U) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int r = min(k0 + U + u, nz);
+        nAv[u] = sA[at(r)];
+        nD[u] = sD[at(r)];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = k0 + u;
+        if (k <= n1) {
+          x = D[u] - Av[u] * x;
+          sD[at(k)] = x;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        Av[u] = nAv[u];
+        D[u] = nD[u];
+      }
+    }
+    sD[at(nz)] = 0.0;
   }
-  sD[lx] = w0;
-  sD[nz * TX + lx] = 0.0;
 }
 
 template <class T, int TX, int NW, int KPT, bool MOIST, bool TPI, int WPE = 1>
@@ -602,12 +766,6 @@ __global__ void __launch_bounds__(TX * NW, WPE) k_acou_wpi_tile(Grid g, Params p
     const double vw = 0.25 * ((double)a.v[c - PL] + (double)a.v[c - PL + NX] +
                               (double)a.v[c] + (double)a.v[c + NX]);
     return (double)cf.aw[cb + k * PL] * (uw * hxc + vw * hyc) * fac;
-  };
-  auto av = [&](int k) -> double {
-    return (k <= 0 || k >= g.nz) ? 0.0 : (double)cf.aw[cb + k * PL];
-  };
-  auto ecoef = [&](int k) -> double {  // E_k = dtau K_t c1 / dzeta_c
-    return dtau * (double)cf.kt3[cb + k * PL] * c1 * m.rdzc[k + g.ng];
   };
 
   // ---- phase 1: assembly, parallel over k. Every load of a level is
@@ -674,36 +832,60 @@ __global__ void __launch_bounds__(TX * NW, WPE) k_acou_wpi_tile(Grid g, Params p
       const double vs = 0.5 * ((double)a.v[c] + (double)a.v[c + NX]);
       w0 = us * hxc + vs * hyc;
     }
-    if (act) sD[k * TX + lx] = Pk[t];
+    if (act) {
+      sD[k * TX + lx] = Pk[t];
+      sC[k * TX + lx] = Ek[t];
+      sB[k * TX + lx] = (double)Avk[t];
+    }
   }
   __syncthreads();
 
-  // ---- phase 2: tridiagonal rows (kernels.cu:596-603), parallel over k
-  double Dk[KPT];
+  // ---- phase 2: tridiagonal rows (kernels.cu:596-603), parallel over k,
+  // with E, A_v and P of the level below read from LDS
+  double Ra[KPT], Rb[KPT], Rc[KPT], Dk[KPT];
 #pragma unroll
   for (int t = 0; t < KPT; ++t) {
     const int k = ry + NW * t;
-    Dk[t] = 0.0;
+    Ra[t] = Rb[t] = Rc[t] = Dk[t] = 0.0;
     if (col && k >= 1 && k < g.nz) {
-      const double e_prev = ecoef(k - 1);
-      const double av_prev = av(k - 1);
+      const int bl = (k - 1) * TX + lx;
+      const double e_prev = sC[bl];
+      const double av_prev = sB[bl];
       const double G = Gk[t];
-      sA[k * TX + lx] = -G * e_prev * av_prev;
-      sB[k * TX + lx] = 1.0 + G * (double)Avk[t] * (Ek[t] + e_prev);
-      sC[k * TX + lx] = -G * Ek[t] * (double)Avk1[t];
-      Dk[t] = Rk[t] - G * (Pk[t] - sD[(k - 1) * TX + lx]);
+      Ra[t] = -G * e_prev * av_prev;
+      Rb[t] = 1.0 + G * (double)Avk[t] * (Ek[t] + e_prev);
+      Rc[t] = -G * Ek[t] * (double)Avk1[t];
+      Dk[t] = Rk[t] - G * (Pk[t] - sD[bl]);
     }
   }
   __syncthreads();
 #pragma unroll
   for (int t = 0; t < KPT; ++t) {
     const int k = ry + NW * t;
-    if (col && k >= 1 && k < g.nz) sD[k * TX + lx] = Dk[t];
+    if (col && k >= 1 && k < g.nz) {
+      sA[k * TX + lx] = Ra[t];
+      sB[k * TX + lx] = Rb[t];
+      sC[k * TX + lx] = Rc[t];
+      sD[k * TX + lx] = Dk[t];
+    }
   }
   __syncthreads();
 
-  // ---- phase 3: elimination (kernels.cu:589-628 order), one lane per column
-  if (ry == 0 && col) thomas_lds<TX>(sA, sB, sC, sD, lx, g.nz, w0);
+  // ---- phase 3: tridiagonal solve, twisted at row nz/2 (thread rows 0 and
+  // 1 take the top and the bottom of each column)
+  if (ry == 0 && col) twist_sweep_top<TX>(sA, sB, sC, sD, lx, g.nz);
+  if (ry == 1 && col) twist_sweep_bottom<TX>(sA, sB, sC, sD, lx, g.nz);
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < KPT; ++t) {
+    const int k = ry + NW * t;
+    if (col && k >= 1 && k < g.nz) twist_row_coefs(sA, sB, sC, sD, k * TX + lx, k, g.nz);
+  }
+  __syncthreads();
+  double xh = 0.0;
+  if (ry <= 1 && col) xh = twist_meet<TX>(sA, sB, sC, sD, lx, g.nz);
+  __syncthreads();
+  if (ry <= 1 && col) twist_back<TX>(sA, sC, sD, lx, g.nz, ry == 0, xh, w0);
   __syncthreads();
 
   // ---- phase 4: pi', pi'prev, theta', w
@@ -969,12 +1151,6 @@ __global__ void __launch_bounds__(TX * NW) k_acou_row(Grid g, Params p, Base<T> 
                               (double)sV0[k * TX + lx] + (double)sV1[k * TX + lx]);
     return (double)cf.aw[cb + k * PL] * (uw * hxc + vw * hyc) * fac;
   };
-  auto av = [&](int k) -> double {
-    return (k <= 0 || k >= g.nz) ? 0.0 : (double)cf.aw[cb + k * PL];
-  };
-  auto ecoef = [&](int k) -> double {  // E_k = dtau K_t c1 / dzeta_c
-    return dtau * (double)cf.kt3[cb + k * PL] * c1 * m.rdzc[k + g.ng];
-  };
 
   double Pk[KPT], Ek[KPT], Gk[KPT], Rk[KPT];
   T Avk[KPT], Avk1[KPT], tthk[KPT], dthk[KPT];
@@ -1040,20 +1216,23 @@ __global__ void __launch_bounds__(TX * NW) k_acou_row(Grid g, Params p, Base<T> 
   }
   __syncthreads();
 
-  // ---- phase 2: tridiagonal rows (kernels.cu:596-603)
+  // ---- phase 2: tridiagonal rows (kernels.cu:596-603); E and A_v of the
+  // level below are re-read (cached lines), P of the level below from LDS
   double Dk[KPT];
 #pragma unroll
   for (int t = 0; t < KPT; ++t) {
     const int k = ry + NW * t;
     Dk[t] = 0.0;
     if (col && k >= 1 && k < g.nz) {
-      const double e_prev = ecoef(k - 1);
-      const double av_prev = av(k - 1);
+      const int bl = (k - 1) * TX + lx;
+      const double e_prev =
+          dtau * (double)cf.kt3[cb + (k - 1) * PL] * c1 * m.rdzc[k - 1 + g.ng];
+      const double av_prev = k - 1 >= 1 ? (double)cf.aw[cb + (k - 1) * PL] : 0.0;
       const double G = Gk[t];
       sA[k * TX + lx] = -G * e_prev * av_prev;
       sB[k * TX + lx] = 1.0 + G * (double)Avk[t] * (Ek[t] + e_prev);
       sC[k * TX + lx] = -G * Ek[t] * (double)Avk1[t];
-      Dk[t] = Rk[t] - G * (Pk[t] - sD[(k - 1) * TX + lx]);
+      Dk[t] = Rk[t] - G * (Pk[t] - sD[bl]);
     }
   }
   __syncthreads();
@@ -1064,8 +1243,21 @@ __global__ void __launch_bounds__(TX * NW) k_acou_row(Grid g, Params p, Base<T> 
   }
   __syncthreads();
 
-  // ---- phase 3: elimination (kernels.cu:589-628 order), one lane per column
-  if (ry == 0 && col) thomas_lds<TX>(sA, sB, sC, sD, lx, g.nz, w0);
+  // ---- phase 3: tridiagonal solve, twisted at row nz/2 (thread rows 0 and
+  // 1 take the top and the bottom of each column)
+  if (ry == 0 && col) twist_sweep_top<TX>(sA, sB, sC, sD, lx, g.nz);
+  if (ry == 1 && col) twist_sweep_bottom<TX>(sA, sB, sC, sD, lx, g.nz);
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < KPT; ++t) {
+    const int k = ry + NW * t;
+    if (col && k >= 1 && k < g.nz) twist_row_coefs(sA, sB, sC, sD, k * TX + lx, k, g.nz);
+  }
+  __syncthreads();
+  double xh = 0.0;
+  if (ry <= 1 && col) xh = twist_meet<TX>(sA, sB, sC, sD, lx, g.nz);
+  __syncthreads();
+  if (ry <= 1 && col) twist_back<TX>(sA, sC, sD, lx, g.nz, ry == 0, xh, w0);
   __syncthreads();
 
   // ---- phase 4: pi', theta', w
