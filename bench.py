@@ -38,31 +38,34 @@ ALG_BYTES_STEP = {"mixed": 1520, "fp64": 2712}
 
 # Per-kernel algorithmic bytes per interior point per launch (distinct arrays
 # the kernel must read and write, moist case, storage widths of the mode).
-#  acou_wpi      : r u v w thp tw tth thbw dthbdz rtjx rtjy kt3 aw qv qc qr qvb
-#                  (16 x 4 B) + pip (8 B); w thp (4 B) + pip (8 B) written.
+# The moist part of the substep buoyancy arrives inside Tw, so the substep
+# kernels read qv only (for theta_v).
+#  acou_wpi      : r u v w thp tw tth thbw dthbdz rtjx rtjy kt3 aw qv
+#                  (13 x 4 B) + pip (8 B); w thp (4 B) + pip (8 B) written.
 #  acou_uv       : r u v thp qv tu tv thb (7 x 4 B) + pip pipprev (8 B); w u v.
-#  acoustic_fused: one whole substep, the 27-access list above minus the
-#                  pi'prev write (the rotation makes it the previous buffer):
-#                  r 19 x 4 B + pip pipprev 8 B; w u v w thp 4 B + pip 8 B.
-#  tendencies    : r u v w thp qv qc qr mfx mfy mfz div rho rho_w; w Tu Tv Tw
-#                  Tthp Tqv Tqc Tqr.
+#  acoustic_fused: one whole substep: r u v thp qv tu tv thb w tw tth thbw
+#                  dthbdz rtjx rtjy kt3 aw (16 x 4 B) + pip pipprev (8 B);
+#                  w u v w thp (4 B) + pip (8 B).
+#  tendencies    : r u v w thp qv qc qr mfx mfy mfz div rho rho_w, and in
+#                  stages 2-3 the time-n qv qc qr plus qbar_v for the stage
+#                  update and the moist buoyancy; w Tu Tv Tw Tthp q x 3.
 #  massflux      : r u v w rho rho_w; w mfx mfy mfz div.
-#  stage_q       : r q_n x 3, Tq x 3; w q x 3.
+#  stage_q       : r q_n x 3, Tq x 3; w q x 3 (boundary-relaxation runs only).
 #  kessler       : r thp qv qc qr thb rho (4 B) pi' pib (8 B); w thp qv qc qr.
 KERNEL_BYTES = {
-    "mixed": {"acou_wpi": 16 * 4 + 8 + 2 * 4 + 8, "acou_uv": 7 * 4 + 2 * 8 + 2 * 4,
-              "acoustic_fused": 19 * 4 + 2 * 8 + 4 * 4 + 8, "tendencies": 13 * 4 + 7 * 4,
+    "mixed": {"acou_wpi": 13 * 4 + 8 + 2 * 4 + 8, "acou_uv": 7 * 4 + 2 * 8 + 2 * 4,
+              "acoustic_fused": 16 * 4 + 2 * 8 + 4 * 4 + 8, "tendencies": 17 * 4 + 7 * 4,
               "massflux": 5 * 4 + 4 * 4, "stage_q": 9 * 4, "kessler": 6 * 4 + 2 * 8 + 4 * 4},
-    "fp64": {"acou_wpi": 20 * 8, "acou_uv": 11 * 8, "acoustic_fused": 26 * 8,
-             "tendencies": 20 * 8, "massflux": 9 * 8, "stage_q": 9 * 8, "kessler": 12 * 8},
+    "fp64": {"acou_wpi": 17 * 8, "acou_uv": 11 * 8, "acoustic_fused": 23 * 8,
+             "tendencies": 24 * 8, "massflux": 9 * 8, "stage_q": 9 * 8, "kessler": 12 * 8},
 }
 # Over flat ground with a horizontally uniform base state the acoustic kernels
 # read their base-state and coefficient profiles from one column: these
 # per-column arrays then drop out of the kernel's algorithmic bytes.
-#  acou_wpi      : thbw dthbdz rtjx rtjy kt3 aw qvb
+#  acou_wpi      : thbw dthbdz rtjx rtjy kt3 aw
 #  acou_uv       : thb
-#  acoustic_fused: the eight above
-UNIFORM_COLUMN_ARRAYS = {"acou_wpi": 7, "acou_uv": 1, "acoustic_fused": 8}
+#  acoustic_fused: the seven above
+UNIFORM_COLUMN_ARRAYS = {"acou_wpi": 6, "acou_uv": 1, "acoustic_fused": 7}
 
 
 def kernel_bytes(kernel: str, precision: str, uniform: bool):

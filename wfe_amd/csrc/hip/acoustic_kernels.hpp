@@ -315,12 +315,10 @@ struct WPArgs {
   Arr<T> w_dst;
   Arr<double> pip_dst;
   Arr<T> thp_dst;
-  CArr<T> tw;
+  CArr<T> tw;   // w tendency including the moist buoyancy (see k_tendencies)
   CArr<T> tth;
   CArr<T> tpi;  // nullptr when the pi' slow tendency is zero
   CArr<T> qv;
-  CArr<T> qc;
-  CArr<T> qr;
   double dtau;
 };
 
@@ -330,6 +328,12 @@ struct WPArgs {
 //   E_k = dtau K_t c1 / dzeta
 //   -G E_{k-1} A_{k-1} w_{k-1} + (1 + G A_k (E_k + E_{k-1})) w_k
 //       - G E_k A_{k+1} w_{k+1} = RHS_k - G (P_k - P_{k-1})
+// with RHS_k = w + dtau (g theta'_w / thbar_w + T_w) - dtau c_p theta_v,w c2
+// d(pi')/dzeta / J. The moist part of the reference's buoyancy,
+// g (0.61 (q_v - qbar_v) - q_c - q_r) at the w level from the time-n moisture
+// (kernels.cu:560-585), is constant over the substeps of a step and arrives
+// already added to T_w (k_tendencies, or k_fold_moist_buoyancy on the
+// phase-level path), so the substeps read only q_v (for theta_v).
 // Thomas elimination bottom-up, back substitution top-down, diagnostic
 // surface w(0) = u h_x + v h_y, then pi', pi'prev and theta' updates.
 template <class T, int MAXL>
@@ -403,14 +407,7 @@ __global__ void __launch_bounds__(64) k_acou_wpi_col(Grid g, Params p, Base<T> b
     const double A = -G * E[k - 1] * Av_at(k - 1);
     const double B = 1.0 + G * Av_at(k) * (E[k] + E[k - 1]);
     const double C = -G * E[k] * Av_at(k + 1);
-    double buoy = kGrav * 0.5 * (thp_m + thp_c) / (double)b.thbw[c];
-    if (p.moisture) {
-      const double dqv = 0.5 * ((double)a.qv[cm] + (double)a.qv[c]) -
-                         0.5 * ((double)b.qvb[cm] + (double)b.qvb[c]);
-      const double qcr = 0.5 * ((double)a.qc[cm] + (double)a.qc[c]) +
-                         0.5 * ((double)a.qr[cm] + (double)a.qr[c]);
-      buoy += kGrav * (kEps61 * dqv - qcr);
-    }
+    const double buoy = kGrav * 0.5 * (thp_m + thp_c) / (double)b.thbw[c];
     const double RHS = (double)a.w_src[c] + dtau * (buoy + (double)a.tw[c]) -
                        dtau * kCp * thvw * c2 * (a.pip_src[c] - a.pip_src[cm]) / (J * dzw);
     const double D = RHS - G * (P[k] - P[k - 1]);
@@ -754,7 +751,7 @@ __global__ void __launch_bounds__(TX * NW, WPE) k_acou_wpi_tile(Grid g, Params p
   // Values kept in registers across the phases (storage-type values stay in
   // T; their FP64 conversion is exact).
   double Pk[KPT], Ek[KPT], Gk[KPT], Rk[KPT];
-  T Avk[KPT], Avk1[KPT], thold[KPT], tthk[KPT], dthk[KPT];
+  T Avk[KPT], Avk1[KPT];
   double w0 = 0.0;
 
   auto sv = [&](int k) -> double {  // cross-metric vertical flux at w level k
@@ -802,28 +799,18 @@ __global__ void __launch_bounds__(TX * NW, WPE) k_acou_wpi_tile(Grid g, Params p
     const double thm = (double)a.thp_src[cm];
     const double thbw = (double)b.thbw[cc];
     const double twk = (double)a.tw[c];
-    double mfac = 1.0, mbuoy = 0.0;
-    if constexpr (MOIST) {
-      const double qvw = 0.5 * ((double)a.qv[cm] + (double)a.qv[c]);
-      const double dqv = qvw - 0.5 * ((double)b.qvb[cc - PL] + (double)b.qvb[cc]);
-      const double qcr = 0.5 * ((double)a.qc[cm] + (double)a.qc[c]) +
-                         0.5 * ((double)a.qr[cm] + (double)a.qr[c]);
-      mfac = 1.0 + kEps61 * qvw;
-      mbuoy = kGrav * (kEps61 * dqv - qcr);
-    }
+    double mfac = 1.0;
+    if constexpr (MOIST) mfac = 1.0 + kEps61 * 0.5 * ((double)a.qv[cm] + (double)a.qv[c]);
     Pk[t] = pk + dtau * tpi -
             dtau * Kt *
                 (Dh - (sv(kc + 1) - sv(kc)) * rdzc + c2 * (avk1 * wkp - avk * wk) * rdzc);
     Ek[t] = dtau * Kt * c1 * rdzc;
     Avk[t] = awk;
     Avk1[t] = awk1;
-    thold[t] = thkT;
-    tthk[t] = a.tth[c];
-    dthk[t] = b.dthbdz[cc];
     // w level k >= 1: G_k and the right-hand side (kernels.cu:560-585)
     const double thvw = (thbw + 0.5 * (thm + thk)) * mfac;
     const double rJdzw = rJ * m.rdzw[kk];
-    const double buoy = kGrav * 0.5 * (thm + thk) * rcp_nr(thbw) + mbuoy;
+    const double buoy = kGrav * 0.5 * (thm + thk) * rcp_nr(thbw);
     Gk[t] = kc >= 1 ? dtau * kCp * thvw * c1 * rJdzw : 0.0;
     Rk[t] = kc >= 1 ? wk + dtau * (buoy + twk) - dtau * kCp * thvw * c2 * (pk - pkm) * rJdzw
                     : 0.0;
@@ -888,7 +875,8 @@ __global__ void __launch_bounds__(TX * NW, WPE) k_acou_wpi_tile(Grid g, Params p
   if (ry <= 1 && col) twist_back<TX>(sA, sC, sD, lx, g.nz, ry == 0, xh, w0);
   __syncthreads();
 
-  // ---- phase 4: pi', pi'prev, theta', w
+  // ---- phase 4: pi', theta', w (theta', T_theta and dtheta/dz re-read here
+  // rather than held in registers across the solve)
 #pragma unroll
   for (int t = 0; t < KPT; ++t) {
     const int k = ry + NW * t;
@@ -897,8 +885,8 @@ __global__ void __launch_bounds__(TX * NW, WPE) k_acou_wpi_tile(Grid g, Params p
       const double Wk = sD[k * TX + lx], Wk1 = sD[(k + 1) * TX + lx];
       a.pip_dst[c] = Pk[t] - Ek[t] * ((double)Avk1[t] * Wk1 - (double)Avk[t] * Wk);
       const double wc = 0.5 * (Wk + Wk1);
-      a.thp_dst[c] =
-          (T)((double)thold[t] + dtau * ((double)tthk[t] - wc * (double)dthk[t]));
+      a.thp_dst[c] = (T)((double)a.thp_src[c] +
+                         dtau * ((double)a.tth[c] - wc * (double)b.dthbdz[cb + k * PL]));
       a.w_dst[c] = (T)Wk;
     }
   }
@@ -936,8 +924,6 @@ struct FusedArgs {
   CArr<T> tw;
   CArr<T> tth;
   CArr<T> tpi;
-  CArr<T> qc;
-  CArr<T> qr;
 };
 
 template <int TX, int NW, int KPT>
@@ -1153,7 +1139,7 @@ __global__ void __launch_bounds__(TX * NW) k_acou_row(Grid g, Params p, Base<T> 
   };
 
   double Pk[KPT], Ek[KPT], Gk[KPT], Rk[KPT];
-  T Avk[KPT], Avk1[KPT], tthk[KPT], dthk[KPT];
+  T Avk[KPT], Avk1[KPT];
   double w0 = 0.0;
 #pragma unroll
   for (int t = 0; t < KPT; ++t) {
@@ -1184,26 +1170,17 @@ __global__ void __launch_bounds__(TX * NW) k_acou_row(Grid g, Params p, Base<T> 
     const double thm = (double)q.thp_cur[cm];
     const double thbw = (double)b.thbw[cc];
     const double twk = (double)a.tw[c];
-    double mfac = 1.0, mbuoy = 0.0;
-    if constexpr (MOIST) {
-      const double qvw = 0.5 * ((double)q.qv[cm] + (double)q.qv[c]);
-      const double dqv = qvw - 0.5 * ((double)b.qvb[cc - PL] + (double)b.qvb[cc]);
-      const double qcr = 0.5 * ((double)a.qc[cm] + (double)a.qc[c]) +
-                         0.5 * ((double)a.qr[cm] + (double)a.qr[c]);
-      mfac = 1.0 + kEps61 * qvw;
-      mbuoy = kGrav * (kEps61 * dqv - qcr);
-    }
+    double mfac = 1.0;
+    if constexpr (MOIST) mfac = 1.0 + kEps61 * 0.5 * ((double)q.qv[cm] + (double)q.qv[c]);
     Pk[t] = pk + dtau * tpi -
             dtau * Kt *
                 (Dh - (sv(kc + 1) - sv(kc)) * rdzc + c2w * (avk1 * wkp - avk * wk) * rdzc);
     Ek[t] = dtau * Kt * c1 * rdzc;
     Avk[t] = awk;
     Avk1[t] = awk1;
-    tthk[t] = a.tth[c];
-    dthk[t] = b.dthbdz[cc];
     const double thvw = (thbw + 0.5 * (thm + thk)) * mfac;
     const double rJdzw = rJ * m.rdzw[kk];
-    const double buoy = kGrav * 0.5 * (thm + thk) * rcp_nr(thbw) + mbuoy;
+    const double buoy = kGrav * 0.5 * (thm + thk) * rcp_nr(thbw);
     Gk[t] = kc >= 1 ? dtau * kCp * thvw * c1 * rJdzw : 0.0;
     Rk[t] = kc >= 1 ? wk + dtau * (buoy + twk) - dtau * kCp * thvw * c2w * (pk - pkm) * rJdzw
                     : 0.0;
@@ -1260,16 +1237,19 @@ __global__ void __launch_bounds__(TX * NW) k_acou_row(Grid g, Params p, Base<T> 
   if (ry <= 1 && col) twist_back<TX>(sA, sC, sD, lx, g.nz, ry == 0, xh, w0);
   __syncthreads();
 
-  // ---- phase 4: pi', theta', w
+  // ---- phase 4: pi', theta', w (theta', T_theta and dtheta/dz re-read
+  // here rather than held in registers across the solve)
 #pragma unroll
   for (int t = 0; t < KPT; ++t) {
     const int k = ry + NW * t;
     if (col && k < g.nz) {
       const int c = c0 + k * PL;
+      const int cc = cb + k * PL;
       const double Wk = sD[k * TX + lx], Wk1 = sD[(k + 1) * TX + lx];
       a.pip_dst[c] = Pk[t] - Ek[t] * ((double)Avk1[t] * Wk1 - (double)Avk[t] * Wk);
       const double wc = 0.5 * (Wk + Wk1);
-      a.thp_dst[c] = (T)((double)Th[t] + dtau * ((double)tthk[t] - wc * (double)dthk[t]));
+      a.thp_dst[c] = (T)((double)q.thp_cur[c] +
+                         dtau * ((double)a.tth[c] - wc * (double)b.dthbdz[cc]));
       a.w_dst[c] = (T)Wk;
     }
   }

@@ -294,7 +294,7 @@ class Ctx final : public CtxBase {
   E* upload_new(const double* h, Shape shape);
   void fill_ghosts(const StateSet<T>& s);
   void slow_tendencies(const StateSet<T>& est, double dt, const StateSet<T>* qout = nullptr,
-                       double dt_stage = 0);
+                       double dt_stage = 0, bool fold_buoyancy = false);
   void bdy_relax(const StateSet<T>& est);
   void acoustic_stage(const StateSet<T>& out, double dtau, int nsub);
   void stage_moisture(const StateSet<T>& out, double dt_stage);
@@ -324,6 +324,10 @@ class Ctx final : public CtxBase {
   T* wv_[2]{};
   T* ww_ = nullptr;
   T *tu_ = nullptr, *tv_ = nullptr, *tw_ = nullptr, *tth_ = nullptr, *tpi_ = nullptr;
+  // T_w with the moist buoyancy for the phase-level substeps (moist cases);
+  // tw_sub_ points at it while they run, else the substeps read tw_.
+  T* twf_ = nullptr;
+  const T* tw_sub_ = nullptr;
   T* tq_[3]{};
   T *mfx_ = nullptr, *mfy_ = nullptr, *mfz_ = nullptr, *div_ = nullptr;
   T* pratio_[3]{};
@@ -574,6 +578,7 @@ Ctx<T>::Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& b, const w
   tw_ = arena_.alloc<T>(n);
   tth_ = arena_.alloc<T>(n);
   tpi_ = arena_.alloc<T>(n);
+  if (p_.moisture) twf_ = arena_.alloc<T>(n);
   for (int q = 0; q < 3; ++q) tq_[q] = arena_.alloc<T>(n);
   mfx_ = arena_.alloc<T>(n);
   mfy_ = arena_.alloc<T>(n);
@@ -735,7 +740,7 @@ void Ctx<T>::fill_ghosts(const StateSet<T>& s) {
 
 template <class T>
 void Ctx<T>::slow_tendencies(const StateSet<T>& s, double dt, const StateSet<T>* qout,
-                             double dt_stage) {
+                             double dt_stage, bool fold_buoyancy) {
   {
     dim3 grd(cdiv(g_.nx + 5, 64), cdiv(g_.ny + 5, 4), (unsigned)g_.nz);
     TIMED("massflux", dev::k_massflux<T><<<grd, blk3(), 0, stream_>>>(g_, base_, met_, s.u, s.v, s.w, mfx_, mfy_,
@@ -755,11 +760,20 @@ void Ctx<T>::slow_tendencies(const StateSet<T>& s, double dt, const StateSet<T>*
     qs.dts = (T)dt_stage;
     qs.on = 1;
   }
+  // The moist buoyancy of the substeps (from the time-n moisture) joins T_w.
+  dev::MoistBuoy<T> mb{};
+  if (fold_buoyancy && p_.moisture) {
+    const StateSet<T>& sn = S_[cur_];
+    mb.qv = sn.qv;
+    mb.qc = sn.qc;
+    mb.qr = sn.qr;
+    mb.on = 1;
+  }
   {
     dim3 grd(cdiv(g_.nx, 64), cdiv(g_.ny, 4), (unsigned)g_.nz);
     TIMED("tendencies", dev::k_tendencies<T><<<grd, blk3(), 0, stream_>>>(g_, p_, base_, met_, (T)dt, s.u, s.v,
                                                       s.w, s.thp, s.qv, s.qc, s.qr, mfx_,
-                                                      mfy_, mfz_, div_, o, do_q, qs));
+                                                      mfy_, mfz_, div_, o, do_q, qs, mb));
   }
   if (p_.moisture && p_.pd_moist) {
     dev::PDSet<T> S{};
@@ -819,8 +833,6 @@ dev::WPArgs<T> split_wp_impl(const dev::FusedArgs<T>& a) {
   w.tth = a.tth;
   w.tpi = a.tpi;
   w.qv = a.uv.qv;
-  w.qc = a.qc;
-  w.qr = a.qr;
   w.dtau = (double)a.uv.dtau;
   return w;
 }
@@ -985,8 +997,6 @@ void Ctx<T>::tune_acoustic(double dtau) {
   a.tw = tw_;
   a.tth = tth_;
   a.tpi = nullptr;
-  a.qc = sn.qc;
-  a.qr = sn.qr;
   hipEvent_t e0, e1;
   WFE_HIP(hipEventCreate(&e0));
   WFE_HIP(hipEventCreate(&e1));
@@ -1048,11 +1058,9 @@ void Ctx<T>::acoustic_stage(const StateSet<T>& out, double dtau, int nsub) {
     a.w_dst = last ? out.w : ww_;
     a.pip_dst = last ? out.pip : xpip_[s % 3];
     a.thp_dst = last ? out.thp : xthp_[s & 1];
-    a.tw = tw_;
+    a.tw = tw_sub_ ? tw_sub_ : tw_;
     a.tth = tth_;
     a.tpi = bdy_active_ ? tpi_ : nullptr;
-    a.qc = sn.qc;
-    a.qr = sn.qr;
     methods_[method_].run(a);
   }
   check_launch("acoustic_stage");
@@ -1119,8 +1127,6 @@ double Ctx<T>::kernel_bench(const std::string& name, int reps) {
     a.tw = tw_;
     a.tth = tth_;
     a.tpi = bdy_active_ ? tpi_ : nullptr;
-    a.qc = sn.qc;
-    a.qr = sn.qr;
     methods_[method_].run(a);
   } else if (name == "fill_ghosts") {
     fill_ghosts(sn);
@@ -1193,7 +1199,7 @@ void Ctx<T>::step_launches(double dt) {
     const StateSet<T>& est = m == 0 ? sn : SE_[m - 1];
     const StateSet<T>& out = m == 2 ? snext : SE_[m];
     fill_ghosts(est);
-    slow_tendencies(est, dt, fuse_q ? &out : nullptr, stage_dt[m]);
+    slow_tendencies(est, dt, fuse_q ? &out : nullptr, stage_dt[m], true);
     if (bdy_active_) bdy_relax(est);
     acoustic_stage(out, stage_dt[m] / ns_stage[m], ns_stage[m]);
     if (p_.moisture && !fuse_q) stage_moisture(out, stage_dt[m]);
@@ -1205,7 +1211,18 @@ void Ctx<T>::step_launches(double dt) {
 template <class T>
 void Ctx<T>::acoustic_substeps(double dtau, int nsub) {
   const StateSet<T>& out = S_[cur_ ^ 1];
+  if (p_.moisture) {  // the stand-alone T_w lacks the moist buoyancy
+    const StateSet<T>& sn = S_[cur_];
+    dev::MoistBuoy<T> mb{sn.qv, sn.qc, sn.qr, 1};
+    if (g_.nz > 1) {
+      const dim3 grd(cdiv(g_.nx, 64), cdiv(g_.ny, 4), (unsigned)(g_.nz - 1));
+      dev::k_fold_moist_buoyancy<T><<<grd, blk3(), 0, stream_>>>(g_, base_, mb, tw_, twf_);
+      check_launch("fold_moist_buoyancy");
+    }
+    tw_sub_ = twf_;
+  }
   acoustic_stage(out, dtau, nsub);
+  tw_sub_ = nullptr;
   const StateSet<T>& sn = S_[cur_];
   WFE_HIP(hipMemcpyAsync(out.qv, sn.qv, npts_ * sizeof(T), hipMemcpyDeviceToDevice, stream_));
   WFE_HIP(hipMemcpyAsync(out.qc, sn.qc, npts_ * sizeof(T), hipMemcpyDeviceToDevice, stream_));
@@ -1333,8 +1350,6 @@ class Adjoint {
         a.tw = T_[m].c;
         a.tth = T_[m].d;
         a.tpi = nullptr;
-        a.qc = zero_q_;
-        a.qr = zero_q_;
         f.methods_[f.method_].run(a);
       }
     }

@@ -228,6 +228,26 @@ struct TendOut {
   void *tu, *tv, *tw, *tth, *tq[3];
 };
 
+// Moist part of the w-level buoyancy from the time-n moisture,
+// g (0.61 (q_v - qbar_v) - q_c - q_r) averaged to the w level (the moisture
+// terms of k_acou_wpi's right-hand side, kernels.cu:560-585); it is constant
+// over the substeps of a step, so it is added to T_w once per stage.
+template <class T>
+struct MoistBuoy {
+  CArr<T> qv, qc, qr;
+  int on;
+};
+
+template <class T>
+__device__ __forceinline__ T moist_buoyancy_w(const Base<T>& b, const MoistBuoy<T>& mb, int c,
+                                              int PL) {
+  const int cm = c - PL;
+  const T qvw = (T)0.5 * (mb.qv[cm] + mb.qv[c]);
+  const T dqv = qvw - (T)0.5 * (b.qvb[cm] + b.qvb[c]);
+  const T qcr = (T)0.5 * (mb.qc[cm] + mb.qc[c]) + (T)0.5 * (mb.qr[cm] + mb.qr[c]);
+  return (T)kGrav * ((T)kEps61 * dqv - qcr);
+}
+
 // Moisture stage update q_out = q_n + dt_stage T_q (k_stage_q,
 // kernels.cu:268-273) applied where T_q is formed, so T_q is never stored;
 // on = 0 stores T_q instead.
@@ -252,7 +272,7 @@ __global__ void __launch_bounds__(256) k_tendencies(Grid g, Params p, Base<T> b,
                              const T* __restrict__ qr_, const T* __restrict__ mfx_,
                              const T* __restrict__ mfy_, const T* __restrict__ mfz_,
                              const T* __restrict__ div_, TendOut out, int do_q,
-                             QStage<T> qs) {
+                             QStage<T> qs, MoistBuoy<T> mb) {
   const CArr<T> u{u_}, v{v_}, w{w_}, thp{thp_}, qv{qv_}, qc{qc_}, qr{qr_};
   const CArr<T> mfx{mfx_}, mfy{mfy_}, mfz{mfz_}, div{div_};
   const BlockId bid = xcd_block();
@@ -348,6 +368,7 @@ __global__ void __launch_bounds__(256) k_tendencies(Grid g, Params p, Base<T> b,
     }
     T t = (-fdiv + wk * dv) * rcp_nr(rw) + relax;
     if (K > (T)0) t += K * laplacian(g, w, c, rdz2(rdzw));
+    if (mb.on) t += moist_buoyancy_w(b, mb, c, PL);
     Arr<T>(static_cast<T*>(out.tw))[c] = t;
   }
   const T rrt = rcp_nr(rt);
@@ -372,6 +393,20 @@ __global__ void __launch_bounds__(256) k_tendencies(Grid g, Params p, Base<T> b,
       Arr<T>(static_cast<T*>(out.tq[2]))[c] = t2;
     }
   }
+}
+
+// T_w + the moist buoyancy into a separate buffer (phase-level substeps,
+// whose T_w comes from the stand-alone tendency call without it).
+template <class T>
+__global__ void k_fold_moist_buoyancy(Grid g, Base<T> b, MoistBuoy<T> mb, const T* __restrict__ tw,
+                                      T* __restrict__ out) {
+  const BlockId bid = xcd_block();
+  const int i = bid.x * blockDim.x + threadIdx.x;
+  const int j = bid.y * blockDim.y + threadIdx.y;
+  const int k = bid.z + 1;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+  const int c = g.id(i, j, k);
+  out[c] = tw[c] + moist_buoyancy_w(b, mb, c, g.plane);
 }
 
 // ------------------------------------------ positive-definite moisture (PD)
