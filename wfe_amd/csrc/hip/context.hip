@@ -525,15 +525,15 @@ Ctx<T>::Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& b, const w
           rtjx[c] = ir > 0 ? 0.5 * (rtjc[c - 1] + rtjc[c]) : rtjc[c];
           rtjy[c] = jr > 0 ? 0.5 * (rtjc[c - hNX_] + rtjc[c]) : rtjc[c];
         }
-    // Column-uniform profiles over the interior and one ghost ring at the
-    // levels 0 .. nz-1 whose values the acoustic kernels use (equal bits,
-    // NaN included).
+    // Column-uniform profiles over every column of the padded grid at the
+    // levels 0 .. nz-1 whose values the kernels use (equal bits, NaN
+    // included).
     auto uniform = [&](const double* a) {
       if (!a) return true;
       for (int kr = gr.ng; kr < gr.ng + gr.nz; ++kr) {
         const double ref = a[((size_t)kr * g_.NY + gr.ng) * hNX_ + gr.ng];
-        for (int jr = gr.ng - 1; jr <= gr.ng + gr.ny; ++jr)
-          for (int ir = gr.ng - 1; ir <= gr.ng + gr.nx; ++ir) {
+        for (int jr = 0; jr < g_.NY; ++jr)
+          for (int ir = 0; ir < hNX_; ++ir) {
             const double v = a[((size_t)kr * g_.NY + jr) * hNX_ + ir];
             if (std::memcmp(&v, &ref, sizeof v) != 0) return false;
           }
@@ -541,10 +541,12 @@ Ctx<T>::Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& b, const w
       return true;
     };
     p_.hom = !p_.terrain && uniform(b.thb) && uniform(b.thbw) && uniform(b.dthbdz) &&
-             uniform(b.qvb) && uniform(rtjx.data()) && uniform(rtjy.data()) &&
-             uniform(kt3.data()) && uniform(aw.data());
+             uniform(b.qvb) && uniform(b.rhob) && uniform(b.rhobw) && uniform(b.pib) &&
+             uniform(rtjx.data()) && uniform(rtjy.data()) && uniform(kt3.data()) &&
+             uniform(aw.data());
     if (const char* e = std::getenv("WFE_UNIFORM_COLUMNS"))
       if (std::string(e) == "0") p_.hom = 0;
+    base_.hom = p_.hom;
     coef_.rtjx = upload_new<T>(rtjx.data(), s3);
     coef_.rtjy = upload_new<T>(rtjy.data(), s3);
     coef_.kt3 = upload_new<T>(kt3.data(), s3);

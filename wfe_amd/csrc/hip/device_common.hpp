@@ -101,6 +101,12 @@ template <class T>
 struct Base {
   CArr<T> thb, rhob, dthbdz, thbw, rhobw, qvb, ub, vb;
   CArr<double> pib;
+  // 1 when every column holds the same profile (flat ground, horizontally
+  // uniform base state): at(g, c, k) then reads column (0, 0) of level k.
+  int hom;
+  __device__ __forceinline__ int at(const Grid& g, int c, int k) const {
+    return hom ? g.id(0, 0, k) : c;
+  }
 };
 
 // Metric (DevMetric, src/core/metric.hpp:16-30).

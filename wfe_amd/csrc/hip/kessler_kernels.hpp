@@ -187,7 +187,8 @@ __global__ void __launch_bounds__(TX * NW) k_kessler_tile(Grid g, Base<T> b, Met
   const int c0 = g.id(col ? i : 0, j, 0);
   const T J = m.jac[g.id2(col ? i : 0, j)];
   const T dt = a.dt;
-  const T rho_sfc = b.rhob[c0];
+  const int cb0 = b.hom ? g.id(0, 0, 0) : c0;  // base-state column
+  const T rho_sfc = b.rhob[cb0];
 
   T qrl[KPT], vr[KPT], rho[KPT], dzl[KPT], ff[KPT];
   T vmax = 0, dzmin = (T)1e30;
@@ -203,9 +204,10 @@ __global__ void __launch_bounds__(TX * NW) k_kessler_tile(Grid g, Base<T> b, Met
     const int c = c0 + k * PL;
     const T q = a.qr[c];
     qrl[t] = q > (T)0 ? q : (T)0;
-    rho[t] = b.rhob[c];
+    const int cb = cb0 + k * PL;
+    rho[t] = b.rhob[cb];
     dzl[t] = m.dzeta_c[k + g.ng] * J;
-    const T temp = (T)(((double)b.thb[c] + (double)a.thp[c]) * (b.pib[c] + a.pip[c]));
+    const T temp = (T)(((double)b.thb[cb] + (double)a.thp[c]) * (b.pib[cb] + a.pip[c]));
     ff[t] = temp < (T)kTfrz ? (T)0.35 : (T)1;
     const T rq = rho[t] * qrl[t];
     vr[t] = rq > (T)1e-9 ? (T)36.34 * pow(rq, (T)0.1364) * sqrt(rho_sfc / rho[t]) * ff[t] : (T)0;
@@ -271,7 +273,8 @@ __global__ void __launch_bounds__(TX * NW) k_kessler_tile(Grid g, Base<T> b, Met
     if (!col || k >= g.nz) continue;
     const int c = c0 + k * PL;
     T thp = a.thp[c], qv = a.qv[c], qc = a.qc[c], qr = qrl[t];
-    kessler_convert(thp, qv, qc, qr, b.pib[c] + a.pip[c], b.thb[c], rho[t], dt);
+    const int cb = cb0 + k * PL;
+    kessler_convert(thp, qv, qc, qr, b.pib[cb] + a.pip[c], b.thb[cb], rho[t], dt);
     a.thp[c] = thp;
     a.qv[c] = qv;
     a.qc[c] = qc;

@@ -134,13 +134,14 @@ struct MassFlux {
   CArr<T> u;
   CArr<T> v;
   CArr<T> w;
+  int k;  // level of the faces fx, fy
   __device__ __forceinline__ T fx(int c, int c2) const {
-    const T rj = b.rhob[c] * m.jac[c2];
-    return (T)0.5 * (b.rhob[c - 1] * m.jac[c2 - 1] + rj) * u[c];
+    const T rj = b.rhob[b.at(g, c, k)] * m.jac[c2];
+    return (T)0.5 * (b.rhob[b.at(g, c - 1, k)] * m.jac[c2 - 1] + rj) * u[c];
   }
   __device__ __forceinline__ T fy(int c, int c2) const {
-    const T rj = b.rhob[c] * m.jac[c2];
-    return (T)0.5 * (b.rhob[c - g.NX] * m.jac[c2 - g.NX] + rj) * v[c];
+    const T rj = b.rhob[b.at(g, c, k)] * m.jac[c2];
+    return (T)0.5 * (b.rhob[b.at(g, c - g.NX, k)] * m.jac[c2 - g.NX] + rj) * v[c];
   }
   // k in [1, nz); 0 elsewhere
   __device__ __forceinline__ T fz(int c, int c2, int k) const {
@@ -150,7 +151,7 @@ struct MassFlux {
     const T uw = (T)0.25 * (u[cm] + u[cm + 1] + u[c] + u[c + 1]);
     const T vw = (T)0.25 * (v[cm] + v[cm + g.NX] + v[c] + v[c + g.NX]);
     const T cross = (uw * m.hx[c2] + vw * m.hy[c2]) * fac;
-    return b.rhobw[c] * (w[c] - cross);
+    return b.rhobw[b.at(g, c, k)] * (w[c] - cross);
   }
 };
 
@@ -165,7 +166,7 @@ __global__ void k_massflux(Grid g, Base<T> b, Metric<T> m, const T* __restrict__
   const int j = (int)(bid.y * blockDim.y + threadIdx.y) - 2;
   const int k = bid.z;
   if (i > g.nx + 2 || j > g.ny + 2 || k >= g.nz) return;
-  const MassFlux<T> F{g, b, m, u, v, w};
+  const MassFlux<T> F{g, b, m, u, v, w, k};
   const int c = g.id(i, j, k);
   const int c2 = g.id2(i, j);
   const T fxc = F.fx(c, c2), fyc = F.fy(c, c2);
@@ -240,10 +241,10 @@ struct MoistBuoy {
 
 template <class T>
 __device__ __forceinline__ T moist_buoyancy_w(const Base<T>& b, const MoistBuoy<T>& mb, int c,
-                                              int PL) {
+                                              int cb, int PL) {
   const int cm = c - PL;
   const T qvw = (T)0.5 * (mb.qv[cm] + mb.qv[c]);
-  const T dqv = qvw - (T)0.5 * (b.qvb[cm] + b.qvb[c]);
+  const T dqv = qvw - (T)0.5 * (b.qvb[cb - PL] + b.qvb[cb]);
   const T qcr = (T)0.5 * (mb.qc[cm] + mb.qc[c]) + (T)0.5 * (mb.qr[cm] + mb.qr[c]);
   return (T)kGrav * ((T)kEps61 * dqv - qcr);
 }
@@ -287,7 +288,8 @@ __global__ void __launch_bounds__(256) k_tendencies(Grid g, Params p, Base<T> b,
   const T rdzc = (T)m.rdzc[kk];
   const Rsp<T> rc{(T)g.rdx, (T)g.rdy, rdzc};
   const T jc = m.jac[c2];
-  const T rt = b.rhob[c] * jc;
+  const int cb = b.at(g, c, k);  // base-state index of this point
+  const T rt = b.rhob[cb] * jc;
   const T K = (T)p.diff_K;
   const T rjc = rcp_nr(jc);
   auto rdz2 = [&](T rdz) { return rdz * rdz * rjc * rjc; };  // 1 / (dzeta J)^2
@@ -308,7 +310,7 @@ __global__ void __launch_bounds__(256) k_tendencies(Grid g, Params p, Base<T> b,
     const T FzT = face_flux(u, c + PL, PL, fzT);
     const T fdiv = (FxR - FxL) * rc.x + (FyT - FyB) * rc.y + (FzT - FzB) * rc.z;
     const T dv = (T)0.5 * (div[c - 1] + div[c]);
-    const T ru = (T)0.5 * (b.rhob[c - 1] * m.jac[c2 - 1] + rt);
+    const T ru = (T)0.5 * (b.rhob[b.at(g, c - 1, k)] * m.jac[c2 - 1] + rt);
     const T vavg = (T)0.25 * (v[c - 1] + v[c - 1 + NX] + v[c] + v[c + NX]);
     const T fc = (T)0.5 * (m.fcor[c2 - 1] + m.fcor[c2]);
     const T vref = p.coriolis_use_ub ? b.vb[kk] : (T)0;
@@ -333,7 +335,7 @@ __global__ void __launch_bounds__(256) k_tendencies(Grid g, Params p, Base<T> b,
     const T FzT = face_flux(v, c + PL, PL, fzT);
     const T fdiv = (FxR - FxL) * rc.x + (FyT - FyB) * rc.y + (FzT - FzB) * rc.z;
     const T dv = (T)0.5 * (div[c - NX] + div[c]);
-    const T rv = (T)0.5 * (b.rhob[c - NX] * m.jac[c2 - NX] + rt);
+    const T rv = (T)0.5 * (b.rhob[b.at(g, c - NX, k)] * m.jac[c2 - NX] + rt);
     const T uavg = (T)0.25 * (u[c - NX] + u[c + 1 - NX] + u[c] + u[c + 1]);
     const T fc = (T)0.5 * (m.fcor[c2 - NX] + m.fcor[c2]);
     const T uref = p.coriolis_use_ub ? b.ub[kk] : (T)0;
@@ -359,7 +361,7 @@ __global__ void __launch_bounds__(256) k_tendencies(Grid g, Params p, Base<T> b,
     const T FzT = face_flux(w, c + PL, PL, fzT);
     const T fdiv = (FxR - FxL) * rc.x + (FyT - FyB) * rc.y + (FzT - FzB) * rdzw;
     const T dv = (T)0.5 * (div[c - PL] + div[c]);
-    const T rw = b.rhobw[c] * jc;
+    const T rw = b.rhobw[cb] * jc;
     T relax = -m.ray_w[kk] * w[c];
     const T wk = w[c];
     if (p.w_damping) {
@@ -368,7 +370,7 @@ __global__ void __launch_bounds__(256) k_tendencies(Grid g, Params p, Base<T> b,
     }
     T t = (-fdiv + wk * dv) * rcp_nr(rw) + relax;
     if (K > (T)0) t += K * laplacian(g, w, c, rdz2(rdzw));
-    if (mb.on) t += moist_buoyancy_w(b, mb, c, PL);
+    if (mb.on) t += moist_buoyancy_w(b, mb, c, cb, PL);
     Arr<T>(static_cast<T*>(out.tw))[c] = t;
   }
   const T rrt = rcp_nr(rt);
@@ -406,7 +408,7 @@ __global__ void k_fold_moist_buoyancy(Grid g, Base<T> b, MoistBuoy<T> mb, const 
   const int k = bid.z + 1;
   if (i >= g.nx || j >= g.ny || k >= g.nz) return;
   const int c = g.id(i, j, k);
-  out[c] = tw[c] + moist_buoyancy_w(b, mb, c, g.plane);
+  out[c] = tw[c] + moist_buoyancy_w(b, mb, c, b.at(g, c, k), g.plane);
 }
 
 // ------------------------------------------ positive-definite moisture (PD)
@@ -451,7 +453,7 @@ __global__ void k_pd_ratio(Grid g, Base<T> b, Metric<T> m, T dt, const T* __rest
   const int c = g.id(i, j, k);
   const T dzc = m.dzeta_c[k + g.ng];
   const T dx = (T)g.dx, dy = (T)g.dy;
-  const T rt = b.rhob[c] * m.jac[g.id2(i, j)];
+  const T rt = b.rhob[b.at(g, c, k)] * m.jac[g.id2(i, j)];
   const PDFlux<T> F{g, mfx, mfy, mfz};
   for (int s = 0; s < 3; ++s) {
     const T* q = S.q[s];
@@ -479,7 +481,7 @@ __global__ void k_pd_apply(Grid g, Params p, Base<T> b, Metric<T> m, const T* __
   if (i >= g.nx || j >= g.ny || k >= g.nz) return;
   const int c = g.id(i, j, k);
   const T rdzc = (T)m.rdzc[k + g.ng];
-  const T rrt = rcp_nr(b.rhob[c] * m.jac[g.id2(i, j)]);
+  const T rrt = rcp_nr(b.rhob[b.at(g, c, k)] * m.jac[g.id2(i, j)]);
   const PDFlux<T> F{g, mfx, mfy, mfz};
   auto ridx = [&](int ii, int jj, int kz) {
     ii = map_lat(ii, g.nx, g.nx - 1, p.bcx);
