@@ -328,6 +328,9 @@ class Ctx final : public CtxBase {
   // tw_sub_ points at it while they run, else the substeps read tw_.
   T* twf_ = nullptr;
   const T* tw_sub_ = nullptr;
+  // Levels per thread of the tendency kernel (WFE_TENDENCY_LEVELS: 1, 2, 4;
+  // 2 measured fastest on the supercell).
+  int tend_levels_ = 2;
   T* tq_[3]{};
   T *mfx_ = nullptr, *mfy_ = nullptr, *mfz_ = nullptr, *div_ = nullptr;
   T* pratio_[3]{};
@@ -450,6 +453,11 @@ Ctx<T>::Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& b, const w
 
   WFE_HIP(hipStreamCreate(&stream_));
   if (const char* e = std::getenv("WFE_GRAPH")) use_graph_ = std::string(e) != "0";
+  if (const char* e = std::getenv("WFE_TENDENCY_LEVELS")) {
+    const int v = std::atoi(e);
+    if (v != 1 && v != 2 && v != 4) throw ArgError("WFE_TENDENCY_LEVELS must be 1, 2 or 4");
+    tend_levels_ = v;
+  }
   const size_t n = npts_;
   const size_t hn = hn3_;
   const Shape s3 = Shape::k3D, s2 = Shape::k2D, s1 = Shape::k1D;
@@ -749,10 +757,11 @@ void Ctx<T>::slow_tendencies(const StateSet<T>& s, double dt, const StateSet<T>*
                                                     mfz_, div_));
   }
   const int do_q = p_.moisture && !p_.pd_moist;
-  dev::TendOut o{tu_, tv_, tw_, tth_, {tq_[0], tq_[1], tq_[2]}};
+  const StateSet<T>& sn = S_[cur_];
+  // Stage update of q where T_q is formed (qout), and the moist buoyancy of
+  // the substeps (from the time-n moisture) added to T_w (fold_buoyancy).
   dev::QStage<T> qs{};
   if (qout) {
-    const StateSet<T>& sn = S_[cur_];
     qs.qn[0] = sn.qv;
     qs.qn[1] = sn.qc;
     qs.qn[2] = sn.qr;
@@ -762,21 +771,26 @@ void Ctx<T>::slow_tendencies(const StateSet<T>& s, double dt, const StateSet<T>*
     qs.dts = (T)dt_stage;
     qs.on = 1;
   }
-  // The moist buoyancy of the substeps (from the time-n moisture) joins T_w.
-  dev::MoistBuoy<T> mb{};
-  if (fold_buoyancy && p_.moisture) {
-    const StateSet<T>& sn = S_[cur_];
-    mb.qv = sn.qv;
-    mb.qc = sn.qc;
-    mb.qr = sn.qr;
-    mb.on = 1;
-  }
-  {
-    dim3 grd(cdiv(g_.nx, 64), cdiv(g_.ny, 4), (unsigned)g_.nz);
-    TIMED("tendencies", dev::k_tendencies<T><<<grd, blk3(), 0, stream_>>>(g_, p_, base_, met_, (T)dt, s.u, s.v,
-                                                      s.w, s.thp, s.qv, s.qc, s.qr, mfx_,
-                                                      mfy_, mfz_, div_, o, do_q, qs, mb));
-  }
+  const int stage_on = qout && do_q;
+  const int buoy_on = fold_buoyancy && p_.moisture;
+  T* tq0 = stage_on ? qout->qv : tq_[0];
+  T* tq1 = stage_on ? qout->qc : tq_[1];
+  T* tq2 = stage_on ? qout->qr : tq_[2];
+  auto launch = [&](auto kz) {
+    constexpr int KZ = decltype(kz)::value;
+    const dim3 grd(cdiv(g_.nx, 64), cdiv(g_.ny, 4), (unsigned)cdiv(g_.nz, KZ));
+    TIMED("tendencies",
+          (dev::k_tendencies<T, KZ><<<grd, blk3(), 0, stream_>>>(
+              g_, p_, base_, met_, (T)dt, s.u, s.v, s.w, s.thp, s.qv, s.qc, s.qr, mfx_, mfy_, mfz_,
+              div_, tu_, tv_, tw_, tth_, tq0, tq1, tq2, sn.qv, sn.qc, sn.qr, do_q, (T)dt_stage,
+              stage_on, buoy_on)));
+  };
+  if (tend_levels_ == 4)
+    launch(std::integral_constant<int, 4>{});
+  else if (tend_levels_ == 2)
+    launch(std::integral_constant<int, 2>{});
+  else
+    launch(std::integral_constant<int, 1>{});
   if (p_.moisture && p_.pd_moist) {
     dev::PDSet<T> S{};
     const T* q[3] = {s.qv, s.qc, s.qr};

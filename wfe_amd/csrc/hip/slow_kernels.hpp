@@ -225,10 +225,6 @@ __device__ __forceinline__ T laplacian(const Grid& g, const A& f, int c, T rdz2)
          (f[c + g.plane] - (T)2 * cc + f[c - g.plane]) * rdz2;
 }
 
-struct TendOut {
-  void *tu, *tv, *tw, *tth, *tq[3];
-};
-
 // Moist part of the w-level buoyancy from the time-n moisture,
 // g (0.61 (q_v - qbar_v) - q_c - q_r) averaged to the w level (the moisture
 // terms of k_acou_wpi's right-hand side, kernels.cu:560-585); it is constant
@@ -239,13 +235,13 @@ struct MoistBuoy {
   int on;
 };
 
-template <class T>
-__device__ __forceinline__ T moist_buoyancy_w(const Base<T>& b, const MoistBuoy<T>& mb, int c,
-                                              int cb, int PL) {
+template <class T, class A>
+__device__ __forceinline__ T moist_buoyancy_w(const Base<T>& b, const A& qv, const A& qc,
+                                              const A& qr, int c, int cb, int PL) {
   const int cm = c - PL;
-  const T qvw = (T)0.5 * (mb.qv[cm] + mb.qv[c]);
+  const T qvw = (T)0.5 * (qv[cm] + qv[c]);
   const T dqv = qvw - (T)0.5 * (b.qvb[cb - PL] + b.qvb[cb]);
-  const T qcr = (T)0.5 * (mb.qc[cm] + mb.qc[c]) + (T)0.5 * (mb.qr[cm] + mb.qr[c]);
+  const T qcr = (T)0.5 * (qc[cm] + qc[c]) + (T)0.5 * (qr[cm] + qr[c]);
   return (T)kGrav * ((T)kEps61 * dqv - qcr);
 }
 
@@ -260,139 +256,152 @@ struct QStage {
   int on;
 };
 
-// All slow tendencies of one interior point (compute_tendencies,
-// kernels.cu:872-913): u, v, w (k >= 1), theta' (+ Rayleigh), the three
-// moisture species when the non-PD scheme is used, and the const-K
-// diffusion of u, v, w, theta' when diff_K > 0. Divisions by grid spacings
-// and densities are products with reciprocals (rcp_nr for the densities).
-template <class T>
-__global__ void __launch_bounds__(256) k_tendencies(Grid g, Params p, Base<T> b, Metric<T> m, T dt,
-                             const T* __restrict__ u_, const T* __restrict__ v_,
-                             const T* __restrict__ w_, const T* __restrict__ thp_,
-                             const T* __restrict__ qv_, const T* __restrict__ qc_,
-                             const T* __restrict__ qr_, const T* __restrict__ mfx_,
-                             const T* __restrict__ mfy_, const T* __restrict__ mfz_,
-                             const T* __restrict__ div_, TendOut out, int do_q,
-                             QStage<T> qs, MoistBuoy<T> mb) {
+// All slow tendencies (compute_tendencies, kernels.cu:872-913) of KZ
+// consecutive levels of one column per thread: u, v, w (k >= 1), theta'
+// (+ Rayleigh), the three moisture species when the non-PD scheme is used,
+// and the const-K diffusion of u, v, w, theta' when diff_K > 0. Every array
+// pointer is restrict-qualified, so the overlapping vertical stencil reads
+// of consecutive levels can be loaded once. Divisions by grid spacings and
+// densities are products with reciprocals (rcp_nr for the densities).
+//   tq0..2   : T_q, or with stage_on the stage's q = q_n + dts T_q
+//              (k_stage_q, kernels.cu:268-273) so T_q is never stored;
+//   qn0..2   : the time-n moisture (stage update and, with buoy_on, the
+//              moist buoyancy added to T_w).
+template <class T, int KZ>
+__global__ void __launch_bounds__(256) k_tendencies(
+    Grid g, Params p, Base<T> b, Metric<T> m, T dt, const T* __restrict__ u_,
+    const T* __restrict__ v_, const T* __restrict__ w_, const T* __restrict__ thp_,
+    const T* __restrict__ qv_, const T* __restrict__ qc_, const T* __restrict__ qr_,
+    const T* __restrict__ mfx_, const T* __restrict__ mfy_, const T* __restrict__ mfz_,
+    const T* __restrict__ div_, T* __restrict__ tu_, T* __restrict__ tv_, T* __restrict__ tw_,
+    T* __restrict__ tth_, T* __restrict__ tq0_, T* __restrict__ tq1_, T* __restrict__ tq2_,
+    const T* __restrict__ qn0_, const T* __restrict__ qn1_, const T* __restrict__ qn2_, int do_q,
+    T dts, int stage_on, int buoy_on) {
   const CArr<T> u{u_}, v{v_}, w{w_}, thp{thp_}, qv{qv_}, qc{qc_}, qr{qr_};
   const CArr<T> mfx{mfx_}, mfy{mfy_}, mfz{mfz_}, div{div_};
+  const CArr<T> qn0{qn0_}, qn1{qn1_}, qn2{qn2_};
+  const Arr<T> tu{tu_}, tv{tv_}, tw{tw_}, tth{tth_}, tq0{tq0_}, tq1{tq1_}, tq2{tq2_};
   const BlockId bid = xcd_block();
   const int i = bid.x * blockDim.x + threadIdx.x;
   const int j = bid.y * blockDim.y + threadIdx.y;
-  const int k = bid.z;
-  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
-  const int kk = k + g.ng;
-  const int c = g.id(i, j, k);
-  const int c2 = g.id2(i, j);
-  const int NX = g.NX, PL = g.plane;
-  const T rdzc = (T)m.rdzc[kk];
-  const Rsp<T> rc{(T)g.rdx, (T)g.rdy, rdzc};
-  const T jc = m.jac[c2];
-  const int cb = b.at(g, c, k);  // base-state index of this point
-  const T rt = b.rhob[cb] * jc;
-  const T K = (T)p.diff_K;
-  const T rjc = rcp_nr(jc);
-  auto rdz2 = [&](T rdz) { return rdz * rdz * rjc * rjc; };  // 1 / (dzeta J)^2
+  if (i >= g.nx || j >= g.ny) return;
+#pragma unroll
+  for (int kz = 0; kz < KZ; ++kz) {
+    const int k = bid.z * KZ + kz;
+    if (k >= g.nz) break;
+    const int kk = k + g.ng;
+    const int c = g.id(i, j, k);
+    const int c2 = g.id2(i, j);
+    const int NX = g.NX, PL = g.plane;
+    const T rdzc = (T)m.rdzc[kk];
+    const Rsp<T> rc{(T)g.rdx, (T)g.rdy, rdzc};
+    const T jc = m.jac[c2];
+    const int cb = b.at(g, c, k);  // base-state index of this point
+    const T rt = b.rhob[cb] * jc;
+    const T K = (T)p.diff_K;
+    const T rjc = rcp_nr(jc);
+    auto rdz2 = [&](T rdz) { return rdz * rdz * rjc * rjc; };  // 1 / (dzeta J)^2
 
-  // ---- u (k_tend_u, kernels.cu:308-346)
-  {
-    const T fcL = (T)0.5 * (mfx[c - 1] + mfx[c]);
-    const T fcR = (T)0.5 * (mfx[c] + mfx[c + 1]);
-    const T FxL = face_flux(u, c, 1, fcL);
-    const T FxR = face_flux(u, c + 1, 1, fcR);
-    const T fyB = (T)0.5 * (mfy[c - 1] + mfy[c]);
-    const T fyT = (T)0.5 * (mfy[c - 1 + NX] + mfy[c + NX]);
-    const T FyB = face_flux(u, c, NX, fyB);
-    const T FyT = face_flux(u, c + NX, NX, fyT);
-    const T fzB = (T)0.5 * (mfz[c - 1] + mfz[c]);
-    const T fzT = (T)0.5 * (mfz[c - 1 + PL] + mfz[c + PL]);
-    const T FzB = face_flux(u, c, PL, fzB);
-    const T FzT = face_flux(u, c + PL, PL, fzT);
-    const T fdiv = (FxR - FxL) * rc.x + (FyT - FyB) * rc.y + (FzT - FzB) * rc.z;
-    const T dv = (T)0.5 * (div[c - 1] + div[c]);
-    const T ru = (T)0.5 * (b.rhob[b.at(g, c - 1, k)] * m.jac[c2 - 1] + rt);
-    const T vavg = (T)0.25 * (v[c - 1] + v[c - 1 + NX] + v[c] + v[c + NX]);
-    const T fc = (T)0.5 * (m.fcor[c2 - 1] + m.fcor[c2]);
-    const T vref = p.coriolis_use_ub ? b.vb[kk] : (T)0;
-    const T relax = -m.ray_c[kk] * (u[c] - b.ub[kk]);
-    T t = (-fdiv + u[c] * dv) * rcp_nr(ru) + fc * (vavg - vref) + relax;
-    if (K > (T)0) t += K * laplacian(g, u, c, rdz2(rdzc));
-    Arr<T>(static_cast<T*>(out.tu))[c] = t;
-  }
-  // ---- v (k_tend_v, kernels.cu:348-385)
-  {
-    const T fcB = (T)0.5 * (mfy[c - NX] + mfy[c]);
-    const T fcT = (T)0.5 * (mfy[c] + mfy[c + NX]);
-    const T FyB = face_flux(v, c, NX, fcB);
-    const T FyT = face_flux(v, c + NX, NX, fcT);
-    const T fxL = (T)0.5 * (mfx[c - NX] + mfx[c]);
-    const T fxR = (T)0.5 * (mfx[c + 1 - NX] + mfx[c + 1]);
-    const T FxL = face_flux(v, c, 1, fxL);
-    const T FxR = face_flux(v, c + 1, 1, fxR);
-    const T fzB = (T)0.5 * (mfz[c - NX] + mfz[c]);
-    const T fzT = (T)0.5 * (mfz[c - NX + PL] + mfz[c + PL]);
-    const T FzB = face_flux(v, c, PL, fzB);
-    const T FzT = face_flux(v, c + PL, PL, fzT);
-    const T fdiv = (FxR - FxL) * rc.x + (FyT - FyB) * rc.y + (FzT - FzB) * rc.z;
-    const T dv = (T)0.5 * (div[c - NX] + div[c]);
-    const T rv = (T)0.5 * (b.rhob[b.at(g, c - NX, k)] * m.jac[c2 - NX] + rt);
-    const T uavg = (T)0.25 * (u[c - NX] + u[c + 1 - NX] + u[c] + u[c + 1]);
-    const T fc = (T)0.5 * (m.fcor[c2 - NX] + m.fcor[c2]);
-    const T uref = p.coriolis_use_ub ? b.ub[kk] : (T)0;
-    const T relax = -m.ray_c[kk] * (v[c] - b.vb[kk]);
-    T t = (-fdiv + v[c] * dv) * rcp_nr(rv) - fc * (uavg - uref) + relax;
-    if (K > (T)0) t += K * laplacian(g, v, c, rdz2(rdzc));
-    Arr<T>(static_cast<T*>(out.tv))[c] = t;
-  }
-  // ---- w on k >= 1 (k_tend_w, kernels.cu:387-425)
-  if (k >= 1) {
-    const T rdzw = (T)m.rdzw[kk];
-    const T fxL = (T)0.5 * (mfx[c - PL] + mfx[c]);
-    const T fxR = (T)0.5 * (mfx[c + 1 - PL] + mfx[c + 1]);
-    const T FxL = face_flux(w, c, 1, fxL);
-    const T FxR = face_flux(w, c + 1, 1, fxR);
-    const T fyB = (T)0.5 * (mfy[c - PL] + mfy[c]);
-    const T fyT = (T)0.5 * (mfy[c + NX - PL] + mfy[c + NX]);
-    const T FyB = face_flux(w, c, NX, fyB);
-    const T FyT = face_flux(w, c + NX, NX, fyT);
-    const T fzB = (T)0.5 * (mfz[c - PL] + mfz[c]);
-    const T fzT = (T)0.5 * (mfz[c] + mfz[c + PL]);
-    const T FzB = face_flux(w, c, PL, fzB);
-    const T FzT = face_flux(w, c + PL, PL, fzT);
-    const T fdiv = (FxR - FxL) * rc.x + (FyT - FyB) * rc.y + (FzT - FzB) * rdzw;
-    const T dv = (T)0.5 * (div[c - PL] + div[c]);
-    const T rw = b.rhobw[cb] * jc;
-    T relax = -m.ray_w[kk] * w[c];
-    const T wk = w[c];
-    if (p.w_damping) {
-      const T cr = fabs(wk) * dt * rdzw * rjc;
-      if (cr > (T)1) relax -= (T)0.3 * (cr - (T)1) * wk * rcp_nr(dt);
+    // ---- u (k_tend_u, kernels.cu:308-346)
+    {
+      const T fcL = (T)0.5 * (mfx[c - 1] + mfx[c]);
+      const T fcR = (T)0.5 * (mfx[c] + mfx[c + 1]);
+      const T FxL = face_flux(u, c, 1, fcL);
+      const T FxR = face_flux(u, c + 1, 1, fcR);
+      const T fyB = (T)0.5 * (mfy[c - 1] + mfy[c]);
+      const T fyT = (T)0.5 * (mfy[c - 1 + NX] + mfy[c + NX]);
+      const T FyB = face_flux(u, c, NX, fyB);
+      const T FyT = face_flux(u, c + NX, NX, fyT);
+      const T fzB = (T)0.5 * (mfz[c - 1] + mfz[c]);
+      const T fzT = (T)0.5 * (mfz[c - 1 + PL] + mfz[c + PL]);
+      const T FzB = face_flux(u, c, PL, fzB);
+      const T FzT = face_flux(u, c + PL, PL, fzT);
+      const T fdiv = (FxR - FxL) * rc.x + (FyT - FyB) * rc.y + (FzT - FzB) * rc.z;
+      const T dv = (T)0.5 * (div[c - 1] + div[c]);
+      const T ru = (T)0.5 * (b.rhob[b.at(g, c - 1, k)] * m.jac[c2 - 1] + rt);
+      const T vavg = (T)0.25 * (v[c - 1] + v[c - 1 + NX] + v[c] + v[c + NX]);
+      const T fc = (T)0.5 * (m.fcor[c2 - 1] + m.fcor[c2]);
+      const T vref = p.coriolis_use_ub ? b.vb[kk] : (T)0;
+      const T relax = -m.ray_c[kk] * (u[c] - b.ub[kk]);
+      T t = (-fdiv + u[c] * dv) * rcp_nr(ru) + fc * (vavg - vref) + relax;
+      if (K > (T)0) t += K * laplacian(g, u, c, rdz2(rdzc));
+      tu[c] = t;
     }
-    T t = (-fdiv + wk * dv) * rcp_nr(rw) + relax;
-    if (K > (T)0) t += K * laplacian(g, w, c, rdz2(rdzw));
-    if (mb.on) t += moist_buoyancy_w(b, mb, c, cb, PL);
-    Arr<T>(static_cast<T*>(out.tw))[c] = t;
-  }
-  const T rrt = rcp_nr(rt);
-  // ---- theta' (k_tend_thp, kernels.cu:114-146)
-  {
-    T t = scalar_adv(g, thp, mfx, mfy, mfz, div, c, rc, rrt) - m.ray_c[kk] * thp[c];
-    if (K > (T)0) t += K * laplacian(g, thp, c, rdz2(rdzc));
-    Arr<T>(static_cast<T*>(out.tth))[c] = t;
-  }
-  // ---- moisture, non-PD scheme (k_tend_q, kernels.cu:149-179)
-  if (do_q) {
-    const T t0 = scalar_adv(g, qv, mfx, mfy, mfz, div, c, rc, rrt);
-    const T t1 = scalar_adv(g, qc, mfx, mfy, mfz, div, c, rc, rrt);
-    const T t2 = scalar_adv(g, qr, mfx, mfy, mfz, div, c, rc, rrt);
-    if (qs.on) {
-      qs.qo[0][c] = qs.qn[0][c] + qs.dts * t0;
-      qs.qo[1][c] = qs.qn[1][c] + qs.dts * t1;
-      qs.qo[2][c] = qs.qn[2][c] + qs.dts * t2;
-    } else {
-      Arr<T>(static_cast<T*>(out.tq[0]))[c] = t0;
-      Arr<T>(static_cast<T*>(out.tq[1]))[c] = t1;
-      Arr<T>(static_cast<T*>(out.tq[2]))[c] = t2;
+    // ---- v (k_tend_v, kernels.cu:348-385)
+    {
+      const T fcB = (T)0.5 * (mfy[c - NX] + mfy[c]);
+      const T fcT = (T)0.5 * (mfy[c] + mfy[c + NX]);
+      const T FyB = face_flux(v, c, NX, fcB);
+      const T FyT = face_flux(v, c + NX, NX, fcT);
+      const T fxL = (T)0.5 * (mfx[c - NX] + mfx[c]);
+      const T fxR = (T)0.5 * (mfx[c + 1 - NX] + mfx[c + 1]);
+      const T FxL = face_flux(v, c, 1, fxL);
+      const T FxR = face_flux(v, c + 1, 1, fxR);
+      const T fzB = (T)0.5 * (mfz[c - NX] + mfz[c]);
+      const T fzT = (T)0.5 * (mfz[c - NX + PL] + mfz[c + PL]);
+      const T FzB = face_flux(v, c, PL, fzB);
+      const T FzT = face_flux(v, c + PL, PL, fzT);
+      const T fdiv = (FxR - FxL) * rc.x + (FyT - FyB) * rc.y + (FzT - FzB) * rc.z;
+      const T dv = (T)0.5 * (div[c - NX] + div[c]);
+      const T rv = (T)0.5 * (b.rhob[b.at(g, c - NX, k)] * m.jac[c2 - NX] + rt);
+      const T uavg = (T)0.25 * (u[c - NX] + u[c + 1 - NX] + u[c] + u[c + 1]);
+      const T fc = (T)0.5 * (m.fcor[c2 - NX] + m.fcor[c2]);
+      const T uref = p.coriolis_use_ub ? b.ub[kk] : (T)0;
+      const T relax = -m.ray_c[kk] * (v[c] - b.vb[kk]);
+      T t = (-fdiv + v[c] * dv) * rcp_nr(rv) - fc * (uavg - uref) + relax;
+      if (K > (T)0) t += K * laplacian(g, v, c, rdz2(rdzc));
+      tv[c] = t;
+    }
+    // ---- w on k >= 1 (k_tend_w, kernels.cu:387-425)
+    if (k >= 1) {
+      const T rdzw = (T)m.rdzw[kk];
+      const T fxL = (T)0.5 * (mfx[c - PL] + mfx[c]);
+      const T fxR = (T)0.5 * (mfx[c + 1 - PL] + mfx[c + 1]);
+      const T FxL = face_flux(w, c, 1, fxL);
+      const T FxR = face_flux(w, c + 1, 1, fxR);
+      const T fyB = (T)0.5 * (mfy[c - PL] + mfy[c]);
+      const T fyT = (T)0.5 * (mfy[c + NX - PL] + mfy[c + NX]);
+      const T FyB = face_flux(w, c, NX, fyB);
+      const T FyT = face_flux(w, c + NX, NX, fyT);
+      const T fzB = (T)0.5 * (mfz[c - PL] + mfz[c]);
+      const T fzT = (T)0.5 * (mfz[c] + mfz[c + PL]);
+      const T FzB = face_flux(w, c, PL, fzB);
+      const T FzT = face_flux(w, c + PL, PL, fzT);
+      const T fdiv = (FxR - FxL) * rc.x + (FyT - FyB) * rc.y + (FzT - FzB) * rdzw;
+      const T dv = (T)0.5 * (div[c - PL] + div[c]);
+      const T rw = b.rhobw[cb] * jc;
+      T relax = -m.ray_w[kk] * w[c];
+      const T wk = w[c];
+      if (p.w_damping) {
+        const T cr = fabs(wk) * dt * rdzw * rjc;
+        if (cr > (T)1) relax -= (T)0.3 * (cr - (T)1) * wk * rcp_nr(dt);
+      }
+      T t = (-fdiv + wk * dv) * rcp_nr(rw) + relax;
+      if (K > (T)0) t += K * laplacian(g, w, c, rdz2(rdzw));
+      if (buoy_on) t += moist_buoyancy_w(b, qn0, qn1, qn2, c, cb, PL);
+      tw[c] = t;
+    }
+    const T rrt = rcp_nr(rt);
+    // ---- theta' (k_tend_thp, kernels.cu:114-146)
+    {
+      T t = scalar_adv(g, thp, mfx, mfy, mfz, div, c, rc, rrt) - m.ray_c[kk] * thp[c];
+      if (K > (T)0) t += K * laplacian(g, thp, c, rdz2(rdzc));
+      tth[c] = t;
+    }
+    // ---- moisture, non-PD scheme (k_tend_q, kernels.cu:149-179)
+    if (do_q) {
+      const T t0 = scalar_adv(g, qv, mfx, mfy, mfz, div, c, rc, rrt);
+      const T t1 = scalar_adv(g, qc, mfx, mfy, mfz, div, c, rc, rrt);
+      const T t2 = scalar_adv(g, qr, mfx, mfy, mfz, div, c, rc, rrt);
+      if (stage_on) {  // the stage's q = q_n + dt_stage T_q
+        tq0[c] = qn0[c] + dts * t0;
+        tq1[c] = qn1[c] + dts * t1;
+        tq2[c] = qn2[c] + dts * t2;
+      } else {
+        tq0[c] = t0;
+        tq1[c] = t1;
+        tq2[c] = t2;
+      }
     }
   }
 }
@@ -408,7 +417,7 @@ __global__ void k_fold_moist_buoyancy(Grid g, Base<T> b, MoistBuoy<T> mb, const 
   const int k = bid.z + 1;
   if (i >= g.nx || j >= g.ny || k >= g.nz) return;
   const int c = g.id(i, j, k);
-  out[c] = tw[c] + moist_buoyancy_w(b, mb, c, b.at(g, c, k), g.plane);
+  out[c] = tw[c] + moist_buoyancy_w(b, mb.qv, mb.qc, mb.qr, c, b.at(g, c, k), g.plane);
 }
 
 // ------------------------------------------ positive-definite moisture (PD)
