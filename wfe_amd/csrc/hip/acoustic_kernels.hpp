@@ -937,9 +937,12 @@ struct RowShape {
   }
 };
 
-template <class T, int TX, int NW, int KPT, bool MOIST, bool TPI>
-__global__ void __launch_bounds__(TX * NW) k_acou_row(Grid g, Params p, Base<T> b, Metric<T> m,
-                                                     Coef<T> cf, FusedArgs<T> a) {
+// WPE: minimum waves per SIMD the register allocation must allow (1 = no
+// cap; 4 with 512-thread blocks caps VGPRs at 128 so two blocks share a CU).
+template <class T, int TX, int NW, int KPT, bool MOIST, bool TPI, int WPE = 1>
+__global__ void __launch_bounds__(TX * NW, WPE) k_acou_row(Grid g, Params p, Base<T> b,
+                                                          Metric<T> m, Coef<T> cf,
+                                                          FusedArgs<T> a) {
   static_assert(TX == 32 || TX == 64, "a row of lanes is half or all of a wavefront");
   extern __shared__ double lds[];
   const int L = g.nz + 1;

@@ -367,7 +367,7 @@ class Ctx final : public CtxBase {
   void tune_acoustic(double dtau);
   template <int TX, int NW, int KPT, int WPE = 1>
   void add_split_tile(const char* name, size_t cap);
-  template <int TX, int NW, int KPT>
+  template <int TX, int NW, int KPT, int WPE = 1>
   void add_row(const char* name, size_t cap);
   KernelTimer timer_;
   SoloBench solo_;
@@ -896,16 +896,16 @@ void Ctx<T>::add_split_tile(const char* name, size_t cap) {
 }
 
 template <class T>
-template <int TX, int NW, int KPT>
+template <int TX, int NW, int KPT, int WPE>
 void Ctx<T>::add_row(const char* name, size_t cap) {
   using Shape = dev::RowShape<TX, NW, KPT>;
   if (g_.nz > NW * KPT) return;
   const size_t lb = Shape::lds_bytes(g_.nz, p_.terrain != 0, sizeof(T));
   // Instantiations by (moisture, pi' tendency present).
-  const void* fns[4] = {reinterpret_cast<const void*>(&dev::k_acou_row<T, TX, NW, KPT, false, false>),
-                        reinterpret_cast<const void*>(&dev::k_acou_row<T, TX, NW, KPT, false, true>),
-                        reinterpret_cast<const void*>(&dev::k_acou_row<T, TX, NW, KPT, true, false>),
-                        reinterpret_cast<const void*>(&dev::k_acou_row<T, TX, NW, KPT, true, true>)};
+  const void* fns[4] = {reinterpret_cast<const void*>(&dev::k_acou_row<T, TX, NW, KPT, false, false, WPE>),
+                        reinterpret_cast<const void*>(&dev::k_acou_row<T, TX, NW, KPT, false, true, WPE>),
+                        reinterpret_cast<const void*>(&dev::k_acou_row<T, TX, NW, KPT, true, false, WPE>),
+                        reinterpret_cast<const void*>(&dev::k_acou_row<T, TX, NW, KPT, true, true, WPE>)};
   if (lb > 65536) {
     if (lb > cap) return;
     for (const void* f : fns)
@@ -919,16 +919,16 @@ void Ctx<T>::add_row(const char* name, size_t cap) {
                         const dim3 bt(TX, NW);
                         const bool moist = p_.moisture, tpi = a.tpi != nullptr;
                         if (moist && tpi)
-                          TIMED("acoustic_fused", (dev::k_acou_row<T, TX, NW, KPT, true, true>
+                          TIMED("acoustic_fused", (dev::k_acou_row<T, TX, NW, KPT, true, true, WPE>
                                                    <<<gt, bt, lb, stream_>>>(g_, p_, base_, met_, coef_, a)));
                         else if (moist)
-                          TIMED("acoustic_fused", (dev::k_acou_row<T, TX, NW, KPT, true, false>
+                          TIMED("acoustic_fused", (dev::k_acou_row<T, TX, NW, KPT, true, false, WPE>
                                                    <<<gt, bt, lb, stream_>>>(g_, p_, base_, met_, coef_, a)));
                         else if (tpi)
-                          TIMED("acoustic_fused", (dev::k_acou_row<T, TX, NW, KPT, false, true>
+                          TIMED("acoustic_fused", (dev::k_acou_row<T, TX, NW, KPT, false, true, WPE>
                                                    <<<gt, bt, lb, stream_>>>(g_, p_, base_, met_, coef_, a)));
                         else
-                          TIMED("acoustic_fused", (dev::k_acou_row<T, TX, NW, KPT, false, false>
+                          TIMED("acoustic_fused", (dev::k_acou_row<T, TX, NW, KPT, false, false, WPE>
                                                    <<<gt, bt, lb, stream_>>>(g_, p_, base_, met_, coef_, a)));
                       }});
 }
@@ -965,6 +965,7 @@ void Ctx<T>::build_methods(const hipDeviceProp_t& prop) {
   add_row<32, 16, 3>("fused-32x16", cap);
   add_row<32, 8, 6>("fused-32x8", cap);
   add_row<32, 16, 4>("fused-32x16x4", cap);  // columns up to 64 levels
+  add_row<32, 16, 3, 4>("fused-32x16w4", cap);  // two blocks per CU
   // Selection: WFE_ACOUSTIC_KERNEL = exact method name, or a family prefix
   // ("split", "fused"); unset or "auto" = all. WFE_WPI_KERNEL=column keeps
   // only the one-thread-per-column pair.
