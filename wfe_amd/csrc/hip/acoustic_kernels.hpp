@@ -929,10 +929,10 @@ struct FusedArgs {
 template <int TX, int NW, int KPT>
 struct RowShape {
   static constexpr int NC = TX - 2;  // columns per workgroup
-  // Four FP64 row arrays of (nz + 1) levels, plus u, v(j), v(j+1) per level
-  // over terrain.
+  // Six FP64 arrays of (nz + 1) levels (rows A..D, E, A_v), plus u, v(j),
+  // v(j+1) per level over terrain.
   static size_t lds_bytes(int nz, bool terrain, size_t tsize) {
-    return (size_t)4 * (nz + 1) * TX * sizeof(double) +
+    return (size_t)6 * (nz + 1) * TX * sizeof(double) +
            (terrain ? (size_t)3 * nz * TX * tsize : 0);
   }
 };
@@ -950,7 +950,9 @@ __global__ void __launch_bounds__(TX * NW, WPE) k_acou_row(Grid g, Params p, Bas
   double* sA = sD + L * TX;  // A_k            k in [1, nz)
   double* sB = sA + L * TX;  // B_k            k in [1, nz)
   double* sC = sB + L * TX;  // C_k -> cs_k    k in [1, nz)
-  T* sU = reinterpret_cast<T*>(sC + L * TX);  // terrain: u, v(j), v(j+1) per level
+  double* sE = sC + L * TX;  // E_k            k in [0, nz)
+  double* sV = sE + L * TX;  // A_v at level k k in [0, nz)
+  T* sU = reinterpret_cast<T*>(sV + L * TX);  // terrain: u, v(j), v(j+1) per level
   T* sV0 = sU + g.nz * TX;
   T* sV1 = sV0 + g.nz * TX;
 
@@ -1192,12 +1194,16 @@ __global__ void __launch_bounds__(TX * NW, WPE) k_acou_row(Grid g, Params p, Bas
       const double vs = 0.5 * ((double)V0[t] + (double)V1[t]);
       w0 = us * hxc + vs * hyc;
     }
-    if (act) sD[k * TX + lx] = Pk[t];
+    if (act) {  // P, E and A_v of this level for the rows of the level above
+      sD[k * TX + lx] = Pk[t];
+      sE[k * TX + lx] = Ek[t];
+      sV[k * TX + lx] = (double)Avk[t];
+    }
   }
   __syncthreads();
 
-  // ---- phase 2: tridiagonal rows (kernels.cu:596-603); E and A_v of the
-  // level below are re-read (cached lines), P of the level below from LDS
+  // ---- phase 2: tridiagonal rows (kernels.cu:596-603), with P, E and A_v
+  // of the level below from LDS
   double Dk[KPT];
 #pragma unroll
   for (int t = 0; t < KPT; ++t) {
@@ -1205,9 +1211,8 @@ __global__ void __launch_bounds__(TX * NW, WPE) k_acou_row(Grid g, Params p, Bas
     Dk[t] = 0.0;
     if (col && k >= 1 && k < g.nz) {
       const int bl = (k - 1) * TX + lx;
-      const double e_prev =
-          dtau * (double)cf.kt3[cb + (k - 1) * PL] * c1 * m.rdzc[k - 1 + g.ng];
-      const double av_prev = k - 1 >= 1 ? (double)cf.aw[cb + (k - 1) * PL] : 0.0;
+      const double e_prev = sE[bl];
+      const double av_prev = sV[bl];
       const double G = Gk[t];
       sA[k * TX + lx] = -G * e_prev * av_prev;
       sB[k * TX + lx] = 1.0 + G * (double)Avk[t] * (Ek[t] + e_prev);
