@@ -443,6 +443,13 @@ __global__ void __launch_bounds__(64) k_acou_wpi_col(Grid g, Params p, Base<T> b
   }
 }
 
+// Rows read ahead of each serial chain (two chunk buffers of this many rows
+// stay in registers).
+inline constexpr int kTwistChunk = 4;
+// LDS rows kept free before the first and after the last LDS array of the
+// column kernels, for the read-ahead of the solve.
+inline constexpr int kTwistPad = 2 * kTwistChunk;
+
 // Tiled, k-parallel form of the same column solve. A workgroup owns TX
 // consecutive columns of one row j and NW thread rows; thread row r handles
 // the levels k = r, r + NW, ... . Phases, separated by barriers:
@@ -457,11 +464,14 @@ __global__ void __launch_bounds__(64) k_acou_wpi_col(Grid g, Params p, Base<T> b
 //      twist_row_coefs, twist_meet, twist_back): thread rows 0 and 1 (one
 //      lane per column each) eliminate from the top and the bottom;
 //   4. every thread writes w, pi' and theta' of its levels.
-// LDS: 4 (nz + 1) TX doubles. Levels per thread are bounded by KPT.
+// LDS: 4 (nz + 1) + 2 kTwistPad rows of TX doubles. Levels per thread are
+// bounded by KPT.
 template <int TX, int NW, int KPT>
 struct WpiTile {
   static constexpr int kThreads = TX * NW;
-  static size_t lds_bytes(int nz) { return (size_t)4 * (nz + 1) * TX * sizeof(double); }
+  static size_t lds_bytes(int nz) {
+    return ((size_t)4 * (nz + 1) + 2 * kTwistPad) * TX * sizeof(double);
+  }
 };
 
 
@@ -492,13 +502,14 @@ struct WpiTile {
 //      upwards (top lane) and downwards (bottom lane). sD ends as W_k,
 //      W_0 = w0, W_nz = 0.
 // The phases are separated by workgroup barriers in the caller. Rows are
-// read in chunks of kTwistChunk, one chunk ahead of the recurrence (row
-// indices clamped into the padded range; a clamped read is never used), and
-// the running minors are rescaled after every chunk.
+// read in chunks of kTwistChunk, one chunk ahead of the recurrence, at fixed
+// offsets: the reads run up to 2 kTwistChunk - 2 rows past either end of a
+// column, which the caller's LDS layout covers (kTwistPad rows before the
+// first row array and after the last LDS array; a row read past the end of
+// one array lands in the next one), and such values are never used. Whole
+// chunks run without per-row tests; the running minors are rescaled after
+// each of them.
 __host__ __device__ constexpr int twist_row(int nz) { return nz / 2; }  // h, 1 <= h <= n1
-// Rows read ahead of each serial chain (two chunk buffers of this many rows
-// stay in registers).
-inline constexpr int kTwistChunk = 4;
 
 template <int TX>
 __device__ __forceinline__ void twist_sweep_top(double* sA, double* sB, const double* sC,
@@ -509,36 +520,33 @@ __device__ __forceinline__ void twist_sweep_top(double* sA, double* sB, const do
   double A[U], B[U], Cm[U], D[U], nA[U], nB[U], nCm[U], nD[U];  // Cm = C_{k-1}
 #pragma unroll
   for (int u = 0; u < U; ++u) {
-    const int r = max(min(1 + u, last), 0);
-    A[u] = sA[at(r)];
-    B[u] = sB[at(r)];
-    Cm[u] = r >= 2 ? sC[at(r - 1)] : 0.0;
-    D[u] = sD[at(r)];
+    A[u] = sA[at(1 + u)];
+    B[u] = sB[at(1 + u)];
+    Cm[u] = u == 0 ? 0.0 : sC[at(u)];
+    D[u] = sD[at(1 + u)];
   }
   double p1 = 1.0, p2 = 0.0, e1 = 0.0;  // p_{k-1}, p_{k-2}, e_{k-1}
-  for (int k0 = 1; k0 <= last; k0 += U) {
+  auto row = [&](int k, int u) {
+    const double pk = fma(B[u], p1, -(A[u] * Cm[u]) * p2);
+    const double ek = fma(-A[u], e1, D[u] * p1);
+    sA[at(k)] = p1;
+    sB[at(k)] = pk;
+    sD[at(k)] = ek;
+    p2 = p1;
+    p1 = pk;
+    e1 = ek;
+  };
+  int k0 = 1;
+  for (; k0 + U - 1 <= last; k0 += U) {  // whole chunks
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int r = min(k0 + U + u, last);
-      nA[u] = sA[at(r)];
-      nB[u] = sB[at(r)];
-      nCm[u] = sC[at(max(r - 1, 1))];
-      nD[u] = sD[at(r)];
+      nA[u] = sA[at(k0 + U + u)];
+      nB[u] = sB[at(k0 + U + u)];
+      nCm[u] = sC[at(k0 + U + u - 1)];
+      nD[u] = sD[at(k0 + U + u)];
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int k = k0 + u;
-      if (k <= last) {
-        const double pk = fma(B[u], p1, -(A[u] * Cm[u]) * p2);
-        const double ek = fma(-A[u], e1, D[u] * p1);
-        sA[at(k)] = p1;
-        sB[at(k)] = pk;
-        sD[at(k)] = ek;
-        p2 = p1;
-        p1 = pk;
-        e1 = ek;
-      }
-    }
+    for (int u = 0; u < U; ++u) row(k0 + u, u);
     const int ex = __builtin_amdgcn_frexp_exp(p1);
     p1 = ldexp(p1, -ex);
     p2 = ldexp(p2, -ex);
@@ -551,6 +559,9 @@ __device__ __forceinline__ void twist_sweep_top(double* sA, double* sB, const do
       D[u] = nD[u];
     }
   }
+#pragma unroll
+  for (int u = 0; u < U - 1; ++u)  // the remaining rows of a partial chunk
+    if (k0 + u <= last) row(k0 + u, u);
 }
 
 template <int TX>
@@ -562,36 +573,33 @@ __device__ __forceinline__ void twist_sweep_bottom(const double* sA, double* sB,
   double Ap[U], B[U], C[U], D[U], nAp[U], nB[U], nC[U], nD[U];  // Ap = A_{k+1}
 #pragma unroll
   for (int u = 0; u < U; ++u) {
-    const int r = min(max(n1 - u, first), n1);
-    Ap[u] = r <= n1 - 1 ? sA[at(r + 1)] : 0.0;
-    B[u] = sB[at(r)];
-    C[u] = sC[at(r)];
-    D[u] = sD[at(r)];
+    Ap[u] = u == 0 ? 0.0 : sA[at(n1 - u + 1)];
+    B[u] = sB[at(n1 - u)];
+    C[u] = sC[at(n1 - u)];
+    D[u] = sD[at(n1 - u)];
   }
   double q1 = 1.0, q2 = 0.0, f1 = 0.0;  // q_{k+1}, q_{k+2}, f_{k+1}
-  for (int k0 = n1; k0 >= first; k0 -= U) {
+  auto row = [&](int k, int u) {
+    const double qk = fma(B[u], q1, -(C[u] * Ap[u]) * q2);
+    const double fk = fma(-C[u], f1, D[u] * q1);
+    sC[at(k)] = q1;
+    sB[at(k)] = qk;
+    sD[at(k)] = fk;
+    q2 = q1;
+    q1 = qk;
+    f1 = fk;
+  };
+  int k0 = n1;
+  for (; k0 - U + 1 >= first; k0 -= U) {  // whole chunks
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int r = max(k0 - U - u, first);
-      nAp[u] = sA[at(min(r + 1, n1))];
-      nB[u] = sB[at(r)];
-      nC[u] = sC[at(r)];
-      nD[u] = sD[at(r)];
+      nAp[u] = sA[at(k0 - U - u + 1)];
+      nB[u] = sB[at(k0 - U - u)];
+      nC[u] = sC[at(k0 - U - u)];
+      nD[u] = sD[at(k0 - U - u)];
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int k = k0 - u;
-      if (k >= first) {
-        const double qk = fma(B[u], q1, -(C[u] * Ap[u]) * q2);
-        const double fk = fma(-C[u], f1, D[u] * q1);
-        sC[at(k)] = q1;
-        sB[at(k)] = qk;
-        sD[at(k)] = fk;
-        q2 = q1;
-        q1 = qk;
-        f1 = fk;
-      }
-    }
+    for (int u = 0; u < U; ++u) row(k0 - u, u);
     const int ex = __builtin_amdgcn_frexp_exp(q1);
     q1 = ldexp(q1, -ex);
     q2 = ldexp(q2, -ex);
@@ -604,6 +612,9 @@ __device__ __forceinline__ void twist_sweep_bottom(const double* sA, double* sB,
       D[u] = nD[u];
     }
   }
+#pragma unroll
+  for (int u = 0; u < U - 1; ++u)
+    if (k0 - u >= first) row(k0 - u, u);
 }
 
 // Phase 2 for row k (any thread): the eliminated row from its minors.
@@ -648,29 +659,25 @@ __device__ __forceinline__ void twist_back(const double* sA, const double* sC, d
   constexpr int U = kTwistChunk;
   const int n1 = nz - 1, h = twist_row(nz);
   auto at = [&](int k) { return k * TX + lx; };
+  double x = xh;
   if (top) {  // x_k = d_k - c_k x_{k+1}, k = h-1 .. 1
     double C[U], D[U], nC[U], nD[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int r = max(h - 1 - u, 0);
-      C[u] = sC[at(r)];
-      D[u] = sD[at(r)];
+      C[u] = sC[at(h - 1 - u)];
+      D[u] = sD[at(h - 1 - u)];
     }
-    double x = xh;
-    for (int k0 = h - 1; k0 >= 1; k0 -= U) {
+    int k0 = h - 1;
+    for (; k0 - U + 1 >= 1; k0 -= U) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int r = max(k0 - U - u, 0);
-        nC[u] = sC[at(r)];
-        nD[u] = sD[at(r)];
+        nC[u] = sC[at(k0 - U - u)];
+        nD[u] = sD[at(k0 - U - u)];
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int k = k0 - u;
-        if (k >= 1) {
-          x = D[u] - C[u] * x;
-          sD[at(k)] = x;
-        }
+        x = D[u] - C[u] * x;
+        sD[at(k0 - u)] = x;
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -678,31 +685,32 @@ __device__ __forceinline__ void twist_back(const double* sA, const double* sC, d
         D[u] = nD[u];
       }
     }
+#pragma unroll
+    for (int u = 0; u < U - 1; ++u)
+      if (k0 - u >= 1) {
+        x = D[u] - C[u] * x;
+        sD[at(k0 - u)] = x;
+      }
     sD[at(h)] = xh;
     sD[lx] = w0;
   } else {  // x_k = g_k - a_k x_{k-1}, k = h+1 .. n1
     double Av[U], D[U], nAv[U], nD[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int r = min(h + 1 + u, nz);
-      Av[u] = sA[at(r)];
-      D[u] = sD[at(r)];
+      Av[u] = sA[at(h + 1 + u)];
+      D[u] = sD[at(h + 1 + u)];
     }
-    double x = xh;
-    for (int k0 = h + 1; k0 <= n1; k0 += U) {
+    int k0 = h + 1;
+    for (; k0 + U - 1 <= n1; k0 += U) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int r = min(k0 + U + u, nz);
-        nAv[u] = sA[at(r)];
-        nD[u] = sD[at(r)];
+        nAv[u] = sA[at(k0 + U + u)];
+        nD[u] = sD[at(k0 + U + u)];
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int k = k0 + u;
-        if (k <= n1) {
-          x = D[u] - Av[u] * x;
-          sD[at(k)] = x;
-        }
+        x = D[u] - Av[u] * x;
+        sD[at(k0 + u)] = x;
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -710,6 +718,12 @@ __device__ __forceinline__ void twist_back(const double* sA, const double* sC, d
         D[u] = nD[u];
       }
     }
+#pragma unroll
+    for (int u = 0; u < U - 1; ++u)
+      if (k0 + u <= n1) {
+        x = D[u] - Av[u] * x;
+        sD[at(k0 + u)] = x;
+      }
     sD[at(nz)] = 0.0;
   }
 }
@@ -720,7 +734,7 @@ __global__ void __launch_bounds__(TX * NW, WPE) k_acou_wpi_tile(Grid g, Params p
                                                                 WPArgs<T> a) {
   extern __shared__ double lds[];
   const int L = g.nz + 1;
-  double* sD = lds;          // P_k, then D_k -> ds_k -> W_k (k in [0, nz])
+  double* sD = lds + kTwistPad * TX;  // P_k, then D_k -> ds_k -> W_k (k in [0, nz])
   double* sA = sD + L * TX;  // A_k            k in [1, nz)
   double* sB = sA + L * TX;  // B_k            k in [1, nz)
   double* sC = sB + L * TX;  // C_k -> cs_k    k in [1, nz)
@@ -932,7 +946,7 @@ struct RowShape {
   // Six FP64 arrays of (nz + 1) levels (rows A..D, E, A_v), plus u, v(j),
   // v(j+1) per level over terrain.
   static size_t lds_bytes(int nz, bool terrain, size_t tsize) {
-    return (size_t)6 * (nz + 1) * TX * sizeof(double) +
+    return ((size_t)6 * (nz + 1) + 2 * kTwistPad) * TX * sizeof(double) +
            (terrain ? (size_t)3 * nz * TX * tsize : 0);
   }
 };
@@ -946,7 +960,7 @@ __global__ void __launch_bounds__(TX * NW, WPE) k_acou_row(Grid g, Params p, Bas
   static_assert(TX == 32 || TX == 64, "a row of lanes is half or all of a wavefront");
   extern __shared__ double lds[];
   const int L = g.nz + 1;
-  double* sD = lds;          // P_k, then D_k -> ds_k -> W_k (k in [0, nz])
+  double* sD = lds + kTwistPad * TX;  // P_k, then D_k -> ds_k -> W_k (k in [0, nz])
   double* sA = sD + L * TX;  // A_k            k in [1, nz)
   double* sB = sA + L * TX;  // B_k            k in [1, nz)
   double* sC = sB + L * TX;  // C_k -> cs_k    k in [1, nz)
