@@ -25,6 +25,11 @@ __device__ __forceinline__ T qsat_i(T p, T temp) {
   return (T)0.622 * es / den;
 }
 
+// x^y for x > 0 as exp(y ln x) in FP64 (the library pow carries extra
+// precision for all x and y that these rates do not need; the difference is
+// a few ulps).
+__device__ __forceinline__ double pow_pos(double x, double y) { return exp(y * log(x)); }
+
 // Autoconversion, accretion, rain evaporation and the one-Newton-step
 // saturation adjustment of one point (kessler.cu:99-158), in FP64 whatever
 // the storage type: theta = thetabar + theta' and the saturation deficit
@@ -41,14 +46,14 @@ __device__ __forceinline__ void kessler_convert(T& thp_io, T& qv_io, T& qc_io, T
   const W pi = pid;
   W th = thb + (W)thp_io;
   W temp = th * pi;
-  const W pres = kP00 * pow(pid, kCp / kRd);
+  const W pres = kP00 * pow_pos(pid, kCp / kRd);
   W fl = (temp - 233.15) / 40.0;
   fl = fmax(0.0, fmin(1.0, fl));
   const W qsw = qsat_w(pres, temp);
   W qs = fl * qsw + (1.0 - fl) * qsat_i(pres, temp);
   const W Leff = fl * kLv + (1.0 - fl) * kLs;
   const W dqauto = k1 * (qck > qc0 ? qck - qc0 : 0.0);
-  const W dqacc = 2.2 * qck * pow(qrk > 1e-9 ? qrk : 1e-9, 0.875);
+  const W dqacc = 2.2 * qck * pow_pos(qrk > 1e-9 ? qrk : 1e-9, 0.875);
   W dqr = dt * (dqauto + dqacc);
   if (dqr > qck) dqr = qck;
   qck -= dqr;
@@ -56,9 +61,10 @@ __device__ __forceinline__ void kessler_convert(T& thp_io, T& qv_io, T& qc_io, T
   W thpk = (W)thp_io;
   if (qvk < qs && qrk > 0.0) {
     const W rq = rhol * qrk;
-    const W vent = 1.6 + 124.9 * pow(rq, 0.2046);
+    const W lrq = log(rq);
+    const W vent = 1.6 + 124.9 * exp(0.2046 * lrq);
     const W ern =
-        vent * pow(rq, 0.525) / (2.55e8 / (pres * qs) + 5.4e5) * (qs - qvk) / (rhol * qs);
+        vent * exp(0.525 * lrq) / (2.55e8 / (pres * qs) + 5.4e5) * (qs - qvk) / (rhol * qs);
     W dqe = dt * ern;
     if (dqe > qrk) dqe = qrk;
     if (dqe > qs - qvk) dqe = qs - qvk;
