@@ -193,7 +193,7 @@ def test_deep_column(monkeypatch, method):
 
 
 FUSED_SHAPES = ["fused-64x16", "fused-64x8", "fused-32x16", "fused-32x8", "fused-32x16x4",
-                "fused-32x16w4"]
+                "fused-32x16w4", "fused-64x14", "fused-64x10x4"]
 SPLIT_SHAPES = ["split-32x8", "split-32x16", "split-64x8", "split-64x16", "split-16x16",
                 "split-32x16x4", "split-32x16w4", "split-16x16w4", "split-col"]
 

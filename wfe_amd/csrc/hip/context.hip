@@ -961,6 +961,8 @@ void Ctx<T>::build_methods(const hipDeviceProp_t& prop) {
   add_split_tile<32, 16, 3, 4>("split-32x16w4", cap);   // two blocks per CU
   add_split_tile<16, 16, 3, 4>("split-16x16w4", cap);
   add_row<64, 16, 3>("fused-64x16", cap);
+  add_row<64, 14, 3>("fused-64x14", cap);     // 42 level slots (nz <= 42)
+  add_row<64, 10, 4>("fused-64x10x4", cap);   // 40 level slots (nz <= 40)
   add_row<64, 8, 6>("fused-64x8", cap);
   add_row<32, 16, 3>("fused-32x16", cap);
   add_row<32, 8, 6>("fused-32x8", cap);
