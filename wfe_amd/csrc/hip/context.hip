@@ -331,6 +331,14 @@ class Ctx final : public CtxBase {
   // Levels per thread of the tendency kernel (WFE_TENDENCY_LEVELS: 1, 2, 4;
   // 2 measured fastest on the supercell).
   int tend_levels_ = 2;
+  // Columns per workgroup row of the tendency kernel (WFE_TENDENCY_TILE:
+  // 64 -> 64x4 threads, 32 -> 32x8; the two measured alike).
+  int tend_tx_ = 64;
+  // Kessler workgroup shape for columns of at most 48 levels
+  // (WFE_KESSLER_TILE: 64x8, 32x8, 32x16, 16x16, 64x16, 32x24, 16x48 columns
+  // x thread rows). 32x16 (3 levels per thread, 480 workgroups on the
+  // 120x120x40 supercell) measured 18.6 us against 21.8 us for 64x8.
+  std::string kes_tile_ = "32x16";
   T* tq_[3]{};
   T *mfx_ = nullptr, *mfy_ = nullptr, *mfz_ = nullptr, *div_ = nullptr;
   T* pratio_[3]{};
@@ -457,6 +465,18 @@ Ctx<T>::Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& b, const w
     const int v = std::atoi(e);
     if (v != 1 && v != 2 && v != 4) throw ArgError("WFE_TENDENCY_LEVELS must be 1, 2 or 4");
     tend_levels_ = v;
+  }
+  if (const char* e = std::getenv("WFE_TENDENCY_TILE")) {
+    const int v = std::atoi(e);
+    if (v != 64 && v != 32) throw ArgError("WFE_TENDENCY_TILE must be 64 or 32");
+    tend_tx_ = v;
+  }
+  if (const char* e = std::getenv("WFE_KESSLER_TILE")) {
+    const std::string v = e;
+    if (v != "64x8" && v != "32x8" && v != "32x16" && v != "16x16" && v != "64x16" &&
+        v != "32x24" && v != "16x48")
+      throw ArgError("WFE_KESSLER_TILE must be 64x8, 32x8, 32x16, 16x16, 64x16, 32x24 or 16x48");
+    kes_tile_ = v;
   }
   const size_t n = npts_;
   const size_t hn = hn3_;
@@ -778,9 +798,10 @@ void Ctx<T>::slow_tendencies(const StateSet<T>& s, double dt, const StateSet<T>*
   T* tq2 = stage_on ? qout->qr : tq_[2];
   auto launch = [&](auto kz) {
     constexpr int KZ = decltype(kz)::value;
-    const dim3 grd(cdiv(g_.nx, 64), cdiv(g_.ny, 4), (unsigned)cdiv(g_.nz, KZ));
+    const int ty = 256 / tend_tx_;
+    const dim3 grd(cdiv(g_.nx, tend_tx_), cdiv(g_.ny, ty), (unsigned)cdiv(g_.nz, KZ));
     TIMED("tendencies",
-          (dev::k_tendencies<T, KZ><<<grd, blk3(), 0, stream_>>>(
+          (dev::k_tendencies<T, KZ><<<grd, dim3(tend_tx_, ty), 0, stream_>>>(
               g_, p_, base_, met_, (T)dt, s.u, s.v, s.w, s.thp, s.qv, s.qc, s.qr, mfx_, mfy_, mfz_,
               div_, tu_, tv_, tw_, tth_, tq0, tq1, tq2, sn.qv, sn.qc, sn.qr, do_q, (T)dt_stage,
               stage_on, buoy_on)));
@@ -1099,9 +1120,35 @@ template <class T>
 void Ctx<T>::kessler_step(const StateSet<T>& s, double dt) {
   dev::KesArgs<T> a{s.thp, s.pip, s.qv, s.qc, s.qr, rain_, (T)dt};
   const dim3 gcol(cdiv(g_.nx, 64), (unsigned)g_.ny);
-  if (g_.nz <= 8 * 6) {
-    const dim3 gt(cdiv(g_.nx, 64), (unsigned)g_.ny);
-    TIMED("kessler", dev::k_kessler_tile<T, 64, 8, 6><<<gt, dim3(64, 8), dev::kessler_tile_lds<64, 8, T>(g_.nz), stream_>>>(g_, base_, met_, a));
+  // Tile shapes: TX columns, NW thread rows, KPT = 48 / NW levels per thread.
+  auto tile = [&](auto tx, auto nw) {
+    constexpr int TX = decltype(tx)::value, NW = decltype(nw)::value;
+    const dim3 gt(cdiv(g_.nx, TX), (unsigned)g_.ny);
+    TIMED("kessler", (dev::k_kessler_tile<T, TX, NW, 48 / NW><<<gt, dim3(TX, NW),
+                                                                 dev::kessler_tile_lds<TX, NW, T>(g_.nz),
+                                                                 stream_>>>(g_, base_, met_, a)));
+  };
+  using I8 = std::integral_constant<int, 8>;
+  using I16 = std::integral_constant<int, 16>;
+  using I32 = std::integral_constant<int, 32>;
+  using I64 = std::integral_constant<int, 64>;
+  if (g_.nz <= 48) {
+    if (kes_tile_ == "32x8")
+      tile(I32{}, I8{});
+    else if (kes_tile_ == "32x16")
+      tile(I32{}, I16{});
+    else if (kes_tile_ == "16x16")
+      tile(I16{}, I16{});
+    else if (kes_tile_ == "64x16")
+      tile(I64{}, I16{});
+    else if (kes_tile_ == "32x24")
+      tile(I32{}, std::integral_constant<int, 24>{});
+    else if (kes_tile_ == "16x48")
+      tile(I16{}, std::integral_constant<int, 48>{});
+    else if (kes_tile_ == "64x8")
+      tile(I64{}, I8{});
+    else
+      tile(I32{}, I16{});
   } else if (g_.nz + 1 <= 64)
     TIMED("kessler", dev::k_kessler_col<T, 64><<<gcol, dim3(64, 1), 0, stream_>>>(g_, base_, met_, a));
   else
