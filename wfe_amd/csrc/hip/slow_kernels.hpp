@@ -76,12 +76,19 @@ __host__ __device__ inline int ghost_count(const Grid& g) {
   return (2 * g.ng + 1) * hx * hy + g.nz * (hx * hy - g.nx * g.ny);
 }
 
-// Value the fill leaves in ghost cell (ii, jj, kk) of f.
+// Periodic wrap of an index that lies a few periods at most outside [0, n),
+// without an integer division (ghost indices are within ng of the interior).
+__device__ __forceinline__ int wrap_near(int i, int n) {
+  while (i < 0) i += n;
+  while (i >= n) i -= n;
+  return i;
+}
+
+// Value the fill leaves in ghost cell (., ., kk) of f, read from the
+// laterally mapped interior column (im, jm).
 template <class E>
-__device__ __forceinline__ E ghost_value(const Grid& g, const E* f, int kind, int xlast,
-                                         int ylast, int bcx, int bcy, int ii, int jj, int kk) {
-  const int im = map_lat(ii, g.nx, xlast, bcx);
-  const int jm = map_lat(jj, g.ny, ylast, bcy);
+__device__ __forceinline__ E ghost_value(const Grid& g, const E* f, int kind, int im, int jm,
+                                         int kk) {
   if (kind == kFillW) {
     if (kk == g.nz) return (E)0;
     if (kk < 0) return (E)2 * f[g.id(im, jm, 0)] - f[g.id(im, jm, -kk)];
@@ -98,17 +105,20 @@ __global__ void __launch_bounds__(256) k_fill_all(Grid g, int bcx, int bcy, Fill
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= ghost_count(g)) return;
   const GhostCell q = ghost_cell(g, idx);
+  // Lateral maps (map_lat): the periodic wrap is the same for every field,
+  // the open-boundary clamp depends on the field's last index.
+  const int ip = wrap_near(q.i, g.nx), jp = wrap_near(q.j, g.ny);
+  auto mx = [&](int last) { return bcx ? (q.i < 0 ? 0 : (q.i > last ? last : q.i)) : ip; };
+  auto my = [&](int last) { return bcy ? (q.j < 0 ? 0 : (q.j > last ? last : q.j)) : jp; };
   T v[kMaxFill];
   double vp[2];
 #pragma unroll
   for (int f = 0; f < kMaxFill; ++f)
-    if (f < L.n)
-      v[f] = ghost_value(g, L.f[f], L.kind[f], L.xlast[f], L.ylast[f], bcx, bcy, q.i, q.j, q.k);
+    if (f < L.n) v[f] = ghost_value(g, L.f[f], L.kind[f], mx(L.xlast[f]), my(L.ylast[f]), q.k);
 #pragma unroll
   for (int f = 0; f < 2; ++f)
     if (f < Lp.n)
-      vp[f] = ghost_value(g, Lp.f[f], Lp.kind[f], Lp.xlast[f], Lp.ylast[f], bcx, bcy, q.i, q.j,
-                          q.k);
+      vp[f] = ghost_value(g, Lp.f[f], Lp.kind[f], mx(Lp.xlast[f]), my(Lp.ylast[f]), q.k);
   const int c = g.id(q.i, q.j, q.k);
 #pragma unroll
   for (int f = 0; f < kMaxFill; ++f)
