@@ -84,6 +84,45 @@ def test_kessler_fp64():
     assert rel_err(r_g, r_o) < 1e-12
 
 
+@pytest.mark.parametrize("tile", ["64x8", "32x8", "32x16", "16x16", "64x16", "32x24", "16x48"])
+def test_kessler_tile_shapes_match_oracle(monkeypatch, tile):
+    """Every Kessler workgroup shape (WFE_KESSLER_TILE) gives the oracle's
+    result on a grid ragged against every tile width, 45 levels deep."""
+    monkeypatch.setenv("WFE_KESSLER_TILE", tile)
+    c = kitchen_sink(nx=37, ny=7, nz=45)
+    st = perturbed_state(c, 7, {"u": 0, "v": 0, "w": 0, "thp": 0.5, "pip": 1e-5,
+                                "qv": 3e-3, "qc": 2e-3, "qr": 4e-3})
+    om, gi = oracle_and_gpu(c, "fp64", st)
+    om.kessler(6.0)
+    gi.kessler(6.0)
+    _cmp(c.grid, om, gi, ("thp", "qv", "qc", "qr"), 1e-12, f"kessler {tile}")
+    g = c.grid
+    assert rel_err(gi.get("rain").reshape(g.NY, g.NX), om.get("rain").reshape(g.NY, g.NX)) < 1e-12
+
+
+def test_tendency_tile_32_matches_oracle(monkeypatch):
+    """The 32x8 tendency workgroups (WFE_TENDENCY_TILE=32) match the oracle."""
+    monkeypatch.setenv("WFE_TENDENCY_TILE", "32")
+    c = kitchen_sink(nx=37, ny=11)
+    st = perturbed_state(c, 8)
+    om, gi = oracle_and_gpu(c, "fp64", st)
+    om.apply_bcs()
+    om.tendencies(6.0)
+    gi.apply_bcs()
+    gi.compute_tendencies(6.0)
+    _cmp(c.grid, om, gi, ("tu", "tv", "tw", "tthp", "tqv", "tqc", "tqr"), 1e-12,
+         "tendencies 32x8")
+
+
+@pytest.mark.parametrize("var,bad", [("WFE_KESSLER_TILE", "8x8"), ("WFE_TENDENCY_TILE", "48"),
+                                     ("WFE_TENDENCY_LEVELS", "3")])
+def test_tile_settings_reject_bad_values(monkeypatch, var, bad):
+    from wfe_amd import Integrator
+    monkeypatch.setenv(var, bad)
+    with pytest.raises(Exception, match=var):
+        Integrator.from_case(kitchen_sink(), "fp64")
+
+
 FULL_CASES = {
     "kitchen_sink": lambda: kitchen_sink(),
     "kitchen_sink_periodic": lambda: kitchen_sink(bc_x="periodic", bc_y="periodic",

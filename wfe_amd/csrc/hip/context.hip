@@ -459,7 +459,6 @@ Ctx<T>::Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& b, const w
   p_.w_damping = hp.w_damping;
   p_.microphysics = hp.microphysics;
 
-  WFE_HIP(hipStreamCreate(&stream_));
   if (const char* e = std::getenv("WFE_GRAPH")) use_graph_ = std::string(e) != "0";
   if (const char* e = std::getenv("WFE_TENDENCY_LEVELS")) {
     const int v = std::atoi(e);
@@ -478,6 +477,7 @@ Ctx<T>::Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& b, const w
       throw ArgError("WFE_KESSLER_TILE must be 64x8, 32x8, 32x16, 16x16, 64x16, 32x24 or 16x48");
     kes_tile_ = v;
   }
+  WFE_HIP(hipStreamCreate(&stream_));
   const size_t n = npts_;
   const size_t hn = hn3_;
   const Shape s3 = Shape::k3D, s2 = Shape::k2D, s1 = Shape::k1D;
