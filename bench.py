@@ -388,11 +388,20 @@ def main() -> int:
                     help="last checkpoint of the error-growth study (600 steps = 1 h)")
     ap.add_argument("--mixed-only", action="store_true",
                     help="skip the FP64 run and the error comparison (profiling runs)")
+    ap.add_argument("--lam-only", action="store_true",
+                    help="run only the regional LAM measurement and print its JSON (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is described, not measured)"}))
         return 2
+    if args.lam_only:
+        # The copy probe runs first so counter passes have their calibration launch.
+        from wfe_amd.case import copy_bandwidth
+        bw = copy_bandwidth(1 << 30, 20)
+        print(json.dumps({"regional_lam": lam_benchmark(args.extra_steps, 3),
+                          "copy_probe_GBps": bw}), flush=True)
+        return 0
 
     from wfe_amd import Case
 

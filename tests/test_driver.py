@@ -77,3 +77,15 @@ def test_driver_runs_a_case_and_matches_the_library(tmp_path, monkeypatch):
     rain = np.fromfile(tmp_path / "rain_000010.bin", dtype=np.float32).reshape(ny, nx)
     r2 = gi.get("rain").reshape(g.NY, g.NX)[g.ng:g.ng + ny, g.ng:g.ng + nx].astype(np.float32)
     assert rel_err(rain, r2) <= 1e-6
+
+
+@pytest.mark.gpu
+def test_driver_w_guard_exits_3_with_emergency_output(tmp_path):
+    """|w|max above w_abort at a 10-step check stops the run with exit code 3
+    after writing the current state (main.cpp:429-437)."""
+    over = {**SMALL, "out_dir": tmp_path, "out_interval": 60, "diag_interval": 60,
+            "w_abort": 1e-6}
+    r = _run([SUPERCELL, *[f"{k}={v}" for k, v in over.items()]], cwd=tmp_path)
+    assert r.returncode == 3, (r.stdout[-1000:], r.stderr[-1000:])
+    assert "exceeded the limit" in r.stderr
+    assert (tmp_path / "w_000010.bin").stat().st_size == 4 * 24 * 20 * 19
