@@ -136,6 +136,26 @@ FULL_CASES = {
 }
 
 
+def test_narrowest_periodic_strip_fp64():
+    """The narrowest strip a context takes (ny = 3, the ghost width): every
+    ghost row j = -3 .. 5 wraps onto an interior row, on every kernel of a
+    step. A narrower strip is rejected when the context is created."""
+    from wfe_amd import Integrator
+    from wfe_amd._lib import WfeError
+
+    c = kitchen_sink(nx=21, ny=3, bc_x="periodic", bc_y="periodic", pd_moist="off")
+    st = perturbed_state(c, 51)
+    om, gi = oracle_and_gpu(c, "fp64", st)
+    dt = c.scalar("dt")
+    for s in range(3):
+        om.step(dt, s * dt)
+        gi.step(dt, s * dt)
+    _cmp(c.grid, om, gi, STATE, 1e-10, "periodic strip ny=3 fp64 3 steps")
+    narrow = kitchen_sink(nx=21, ny=2, bc_x="periodic", bc_y="periodic", pd_moist="off")
+    with pytest.raises(WfeError, match="grid too small"):
+        Integrator.from_case(narrow, "fp64")
+
+
 @pytest.mark.parametrize("name", sorted(FULL_CASES))
 def test_full_steps_fp64(name):
     c = FULL_CASES[name]()

@@ -1795,7 +1795,8 @@ int wfe_create(const wfe_grid* g, const wfe_params* p, const wfe_base* b, const 
     if (!g || !p || !b || !m || !out) throw wfe::ArgError("null argument");
     *out = nullptr;
     if (g->ng != 3) throw wfe::ArgError("ng must be 3");
-    if (g->nx < 4 || g->ny < 3 || g->nz < 4) throw wfe::ArgError("grid too small");
+    if (g->nx < 4 || g->ny < 3 || g->nz < 4)
+      throw wfe::ArgError("grid too small (nx >= 4, ny >= 3, nz >= 4)");
     if (g->nz + 1 > 320)
       throw wfe::ArgError("nz too large for the column solvers (nz <= 319)");
     const double np = (double)(g->nx + 6) * (g->ny + 6) * (g->nz + 7);
