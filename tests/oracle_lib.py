@@ -5,11 +5,15 @@ Test infrastructure only: it checks the HIP path; it is never the product.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
 
-ORACLE_PATH = Path(__file__).resolve().parent.parent / "oracle" / "liboracle.so"
+# WFE_ORACLE_LIB points the tests at another build of the oracle (the
+# mutation test loads deliberately broken copies through it).
+ORACLE_PATH = Path(os.environ.get("WFE_ORACLE_LIB") or
+                   Path(__file__).resolve().parent.parent / "oracle" / "liboracle.so")
 _DP = C.POINTER(C.c_double)
 _o = None
 
