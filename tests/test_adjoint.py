@@ -166,3 +166,50 @@ def test_scope_is_enforced():
     gm = Integrator.from_case(dry_case(), "mixed")
     with pytest.raises(WfeError):
         tl_step(gm, 2.0, {})
+
+
+def test_scope_is_checked_again_after_relaxation_is_switched_on():
+    """wfe_set_bdy_weights after a first adjoint call changes the model the
+    step advances; the adjoint entries must refuse it rather than keep
+    returning derivatives of the unrelaxed step."""
+    import ctypes as C
+    from wfe_amd import WfeError
+    from wfe_amd._lib import lib
+    from wfe_amd.adjoint import tl_step
+    c = dry_case()
+    g = c.grid
+    gi = make(c, 6)
+    tl_step(gi, c.scalar("dt"), {})  # in scope: sets the adjoint up
+    w = np.full(g.NX * g.NY, 1e-3)
+    assert lib().wfe_set_bdy_weights(gi._h, w.ctypes.data_as(C.POINTER(C.c_double)), w.size) == 0
+    with pytest.raises(WfeError, match=r"\(-3\).*boundary relaxation"):
+        tl_step(gi, c.scalar("dt"), {})
+
+
+def test_adjoint_entries_leave_state_set_and_ensembles_intact():
+    """tl_step / ad_step / the gradient run forward steps internally; the
+    context must come back in the same state set with the same values, so an
+    ensemble built with a context that was never differentiated still steps
+    in lock-step with it, and both match a fresh context stepped alone."""
+    from wfe_amd import Ensemble, Integrator
+    from wfe_amd.adjoint import ad_step, box_weights, ke_gradient, tl_step
+    c = dry_case()
+    g = c.grid
+    dt = c.scalar("dt")
+    ga, gb, ref = make(c, 8), make(c, 8), make(c, 8)
+    x = {f: ga.get(f) for f in FIELDS5}
+    tl_step(ga, dt, {})
+    ad_step(ga, dt, {})
+    ke_gradient(ga, dt, 3, 2, box_weights(g, 4, 20, 4, 16))
+    for f in FIELDS5:
+        np.testing.assert_array_equal(ga.get(f), x[f])
+    ens = Ensemble([ga, gb])  # raises if the members are out of step
+    for _ in range(2):
+        ens.step([dt, dt])
+        ref.step(dt)
+    ens.sync()
+    for f in FIELDS5:
+        np.testing.assert_array_equal(ga.get(f), ref.get(f))
+        np.testing.assert_array_equal(gb.get(f), ref.get(f))
+    del ens
+    assert isinstance(ga, Integrator)

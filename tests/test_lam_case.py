@@ -4,7 +4,8 @@ import numpy as np
 import pytest
 
 from wfe_amd import Case
-from wfe_amd.lam_case import lambert_map_factor, us_standard, write_lam_case
+from wfe_amd.lam_case import (lambert_grid_latlon, lambert_map_factor, us_standard,
+                               write_lam_case)
 
 SMALL = dict(nx=24, ny=18, nz=16, dx=12000.0, top=12000.0, seed=7)
 
@@ -49,16 +50,35 @@ def test_case_loads_with_consistent_base_state(small_case):
     assert c.params.bc_x_open and c.params.pd_moist
 
 
-def test_full_domain_map_factors_and_terrain_slopes():
-    # 330 x 220 at 12 km around 45 N with standard parallels 35 / 55 N
-    ny, dx, R = 220, 12000.0, 6.371e6
-    y = (np.arange(ny) + 0.5 - ny / 2) * dx
-    lat = 45.0 + np.degrees(y / R)
-    m = lambert_map_factor(lat, 35.0, 55.0)
-    assert 0.95 <= m.min() and m.max() <= 1.05
-    # exactly 1 on the standard parallels
-    np.testing.assert_allclose(lambert_map_factor(np.array([35.0, 55.0]), 35.0, 55.0), 1.0,
+def test_full_domain_map_factors_span_the_configured_band():
+    """330 x 220 at 12 km on the generator's default Lambert grid: the map
+    factors the driver forms from the lat field cover 0.95..1.05."""
+    lat, lon = lambert_grid_latlon(330, 220, 12000.0, 37.0, 10.0, 31.0, 67.0)
+    m = lambert_map_factor(lat, 31.0, 67.0)
+    assert 0.950 <= m.min() < 0.951 and 1.049 < m.max() <= 1.0501
+    # exactly 1 on the standard parallels, below 1 between them
+    np.testing.assert_allclose(lambert_map_factor(np.array([31.0, 67.0]), 31.0, 67.0), 1.0,
                                rtol=1e-12)
+    assert lambert_map_factor(np.array([49.0]), 31.0, 67.0)[0] < 1.0
+
+
+def test_lambert_grid_is_conformal_at_the_spacing():
+    """The inverse projection places neighbouring cell centres dx / m apart on
+    the sphere (m: the map factor), which pins the grid against the formula."""
+    R, dx = 6.371e6, 12000.0
+    lat, lon = lambert_grid_latlon(40, 30, dx, 37.0, 10.0, 31.0, 67.0)
+    la, lo = np.radians(lat), np.radians(lon)
+
+    def dist(a1, o1, a2, o2):  # haversine great-circle distance
+        h = np.sin((a2 - a1) / 2) ** 2 + np.cos(a1) * np.cos(a2) * np.sin((o2 - o1) / 2) ** 2
+        return 2 * R * np.arcsin(np.sqrt(h))
+
+    dxs = dist(la[:, :-1], lo[:, :-1], la[:, 1:], lo[:, 1:])
+    dys = dist(la[:-1], lo[:-1], la[1:], lo[1:])
+    mx = lambert_map_factor(np.degrees(0.5 * (la[:, :-1] + la[:, 1:])), 31.0, 67.0)
+    my = lambert_map_factor(np.degrees(0.5 * (la[:-1] + la[1:])), 31.0, 67.0)
+    np.testing.assert_allclose(dxs * mx, dx, rtol=2e-4)
+    np.testing.assert_allclose(dys * my, dx, rtol=2e-4)
 
 
 def test_us_standard_atmosphere_known_values():

@@ -1313,23 +1313,8 @@ class Adjoint {
   static constexpr int kMaxSub = 16;
 
   explicit Adjoint(Ctx<double>& f) : F(f) {
+    check_scope();
     const auto& hp = F.hp_;
-    std::string why;
-    if (hp.moisture) why = "moisture is on";
-    else if (hp.bc_x_open || hp.bc_y_open) why = "lateral boundaries are not periodic";
-    else if (F.p_.terrain) why = "terrain slopes are non-zero";
-    else if (hp.diff_K != 0) why = "diffusion is on";
-    else if (hp.w_damping) why = "w damping is on";
-    else if (F.bdy_active_) why = "boundary relaxation is on";
-    else if (F.g_.nz + 1 > 64) why = "more than 63 levels";
-    else if (std::max(1, hp.acoustic_ns) > kMaxSub) why = "more than 16 acoustic substeps";
-    if (why.empty()) {
-      std::vector<double> fc(F.n2_);
-      WFE_HIP(hipMemcpy(fc.data(), F.met_.fcor, F.n2_ * sizeof(double), hipMemcpyDeviceToHost));
-      for (double x : fc)
-        if (x != 0) why = "Coriolis is on";
-    }
-    if (!why.empty()) throw std::runtime_error("adjoint: scope is the dry periodic flat step (" + why + ")");
     ns_ = std::max(1, hp.acoustic_ns);
     nsub_[0] = 1;
     nsub_[1] = std::max(1, ns_ / 2);
@@ -1366,6 +1351,29 @@ class Adjoint {
     zero_q_ = A.alloc<double>(n);
     jsum_ = A.alloc<double>(1);
     wgt_ = A.alloc<double>(F.n2_);
+  }
+
+  // The linearised model covers the dry periodic flat step only. The
+  // context can change after the adjoint was set up (boundary relaxation is
+  // switched on by wfe_set_bdy_weights), so every entry checks again.
+  void check_scope() const {
+    const auto& hp = F.hp_;
+    std::string why;
+    if (hp.moisture) why = "moisture is on";
+    else if (hp.bc_x_open || hp.bc_y_open) why = "lateral boundaries are not periodic";
+    else if (F.p_.terrain) why = "terrain slopes are non-zero";
+    else if (hp.diff_K != 0) why = "diffusion is on";
+    else if (hp.w_damping) why = "w damping is on";
+    else if (F.bdy_active_) why = "boundary relaxation is on";
+    else if (F.g_.nz + 1 > 64) why = "more than 63 levels";
+    else if (std::max(1, hp.acoustic_ns) > kMaxSub) why = "more than 16 acoustic substeps";
+    if (why.empty()) {
+      std::vector<double> fc(F.n2_);
+      WFE_HIP(hipMemcpy(fc.data(), F.met_.fcor, F.n2_ * sizeof(double), hipMemcpyDeviceToHost));
+      for (double x : fc)
+        if (x != 0) why = "Coriolis is on";
+    }
+    if (!why.empty()) throw std::runtime_error("adjoint: scope is the dry periodic flat step (" + why + ")");
   }
 
   // Forward step of F from its current state, keeping the trajectory.
@@ -1516,20 +1524,29 @@ class Adjoint {
 
   // Single-step TL / AD through host arrays (reference layout), around the
   // context's current state, which is left unchanged.
+  // The host entries leave the context as they found it: the saved state
+  // goes back into the same state set, and the set that is current (which
+  // record_step flips) is restored, so ensembles see the parity unchanged.
   void tl_step_host(double dt, const double* const* dx, double* const* dy) {
+    check_scope();
+    const int par = F.cur_;
     save_state(x0_);
     record_step(dt);
     upload5(tmp_, dx);
     tl(dt, tmp_, xb_);
     download5(xb_, dy);
+    F.cur_ = par;
     restore_state(x0_);
   }
   void ad_step_host(double dt, const double* const* yb, double* const* xb) {
+    check_scope();
+    const int par = F.cur_;
     save_state(x0_);
     record_step(dt);
     upload5(tmp_, yb);
     ad(dt, tmp_, xb_);
     download5(xb_, xb);
+    F.cur_ = par;
     restore_state(x0_);
   }
 
@@ -1537,8 +1554,10 @@ class Adjoint {
   // level, and its gradient with respect to the initial state.
   double gradient(double dt, int nsteps, int ckpt, const double* wgt2d, double* const* grad) {
     if (nsteps < 1 || ckpt < 1) throw ArgError("adjoint gradient: nsteps, ckpt must be >= 1");
+    check_scope();
     Ctx<double>& f = F;
     hipStream_t st = f.stream_;
+    const int par = f.cur_;
     {
       const std::vector<double> w = f.to_dev<double>(wgt2d, false);
       WFE_HIP(hipMemcpy(wgt_, w.data(), f.n2_ * sizeof(double), hipMemcpyHostToDevice));
@@ -1591,6 +1610,7 @@ class Adjoint {
       }
     }
     download5(xb_, grad);
+    f.cur_ = par;
     restore_state(x0_);
     WFE_HIP(hipStreamSynchronize(st));
     return J;
@@ -1799,8 +1819,17 @@ int wfe_create(const wfe_grid* g, const wfe_params* p, const wfe_base* b, const 
       throw wfe::ArgError("grid too small (nx >= 4, ny >= 3, nz >= 4)");
     if (g->nz + 1 > 320)
       throw wfe::ArgError("nz too large for the column solvers (nz <= 319)");
-    const double np = (double)(g->nx + 6) * (g->ny + 6) * (g->nz + 7);
-    if (np > 2.0e9) throw wfe::ArgError("padded grid exceeds 2^31 points");
+    // Kernels address every 3-D array with a 32-bit unsigned byte offset
+    // (dev::Arr), so the largest array -- an FP64 field in the device layout,
+    // rows padded to a pitch of whole 32-element groups after 32 leading
+    // columns -- must stay below 4 GiB: at most 2^29 padded points.
+    constexpr int kA = wfe::dev::kRowAlign;
+    const double pitch = (double)((kA + g->nx + g->ng + kA - 1) / kA * kA);
+    const double np = pitch * (g->ny + 6) * (g->nz + 7);
+    if (np * sizeof(double) >= 4294967296.0)
+      throw wfe::ArgError("grid too large: the padded device grid (row pitch " +
+                          std::to_string((long)pitch) + ") holds " + std::to_string((long)np) +
+                          " points; an FP64 field must stay below 4 GiB (2^29 points)");
     const double* req[] = {b->thb, b->pib, b->rhob, b->dthbdz, b->thbw, b->rhobw,
                            b->thvb, b->thvbw, b->qvb, b->ub, m->zeta_w, m->zeta_c,
                            m->dzeta_c, m->dzeta_w, m->h, m->hx, m->hy, m->hx_u,

@@ -18,8 +18,10 @@ factors from the lat field and proj_lat1/2 as src/main.cpp:323-339 does).
 The state is synthetic and seeded:
   terrain   k^-2 spectral random field scaled to 0..h_max, smoothed until the
             steepest slope is <= max_slope;
-  lat field so that the reference's Lambert formula gives map factors of
-            about 0.95..1.05 over the domain; f = 2 Omega sin(lat);
+  lat field the cell centres of a Lambert conformal grid (centre 37 N, standard
+            parallels 31 N and 67 N by default), on which the reference's
+            Lambert formula gives map factors 0.950..1.050 over the
+            330 x 220 x 12 km domain; f = 2 Omega sin(lat);
   theta, qv US standard atmosphere, relative humidity 40..90 % from a seeded
             smooth field, Gaussian-filtered theta perturbation (sigma_theta)
             and u, v perturbations (sigma_u) on a westerly jet profile;
@@ -115,7 +117,7 @@ def spectral_terrain(rng, nx, ny, dx, h_max, max_slope):
 
 def write_lam_case(out_dir, *, nx=330, ny=220, nz=45, dx=12000.0, seed=20261202,
                    top=25000.0, dz0=250.0, dz_max=1000.0, h_max=3000.0, max_slope=0.3,
-                   sigma_theta=1.0, sigma_u=3.0, lat0=45.0, proj_lat1=35.0, proj_lat2=55.0,
+                   sigma_theta=1.0, sigma_u=3.0, lat0=37.0, proj_lat1=31.0, proj_lat2=67.0,
                    dt=60.0, acoustic_ns=6, hours=24, bdy_interval_h=24, bdy_width=5,
                    bdy_tau=600.0, low_noise_K=0.0, low_noise_top=2000.0,
                    extra: dict | None = None) -> Path:
@@ -128,13 +130,10 @@ def write_lam_case(out_dir, *, nx=330, ny=220, nz=45, dx=12000.0, seed=20261202,
     zt = zw[-1]
     zc = 0.5 * (zw[:-1] + zw[1:])
 
-    # horizontal grid, lat/lon (metres from the centre on a sphere)
-    x = (np.arange(nx) + 0.5 - nx / 2) * dx
-    y = (np.arange(ny) + 0.5 - ny / 2) * dx
-    X, Y = np.meshgrid(x, y)
-    R = 6.371e6
-    lat = lat0 + np.degrees(Y / R)
-    lon = 10.0 + np.degrees(X / (R * np.cos(np.radians(lat))))
+    # horizontal grid: a Lambert conformal grid centred at (lat0, 10E) with
+    # the case's standard parallels, so the map factors the driver forms
+    # from lat (src/main.cpp:323-339) are those of the projection itself
+    lat, lon = lambert_grid_latlon(nx, ny, dx, lat0, 10.0, proj_lat1, proj_lat2)
     fcor = 2 * OMEGA * np.sin(np.radians(lat))
     h = spectral_terrain(rng, nx, ny, dx, h_max, max_slope)
     tsk = 288.15 - 0.0065 * h
@@ -223,15 +222,38 @@ def write_lam_case(out_dir, *, nx=330, ny=220, nz=45, dx=12000.0, seed=20261202,
     return p
 
 
+def _cone_constant(phi1, phi2):
+    q = math.pi / 4
+    if abs(phi1 - phi2) < 1e-6:
+        return math.sin(phi1)
+    return math.log(math.cos(phi1) / math.cos(phi2)) / \
+        math.log(math.tan(q + phi2 / 2) / math.tan(q + phi1 / 2))
+
+
+def lambert_grid_latlon(nx, ny, dx, lat0, lon0, lat1, lat2, radius=6.371e6):
+    """Latitude and longitude [deg] of the cell centres of an nx x ny grid of
+    spacing dx on a spherical Lambert conformal conic projection (standard
+    parallels lat1, lat2) centred at (lat0, lon0): the inverse projection of
+    Snyder (1987, eqs. 14-4, 15-1 to 15-5). Shape [ny, nx]."""
+    q = math.pi / 4
+    phi1, phi2, phi0 = (math.radians(a) for a in (lat1, lat2, lat0))
+    n = _cone_constant(phi1, phi2)
+    F = math.cos(phi1) * math.tan(q + phi1 / 2) ** n / n
+    rho0 = radius * F / math.tan(q + phi0 / 2) ** n
+    x = (np.arange(nx) + 0.5 - nx / 2) * dx
+    y = (np.arange(ny) + 0.5 - ny / 2) * dx
+    X, Y = np.meshgrid(x, y)
+    rho = math.copysign(1.0, n) * np.sqrt(X ** 2 + (rho0 - Y) ** 2)
+    lat = 2 * np.arctan((radius * F / rho) ** (1 / n)) - math.pi / 2
+    lon = math.radians(lon0) + np.arctan2(X, rho0 - Y) / n
+    return np.degrees(lat), np.degrees(lon)
+
+
 def lambert_map_factor(lat_deg, lat1, lat2):
     """Isotropic Lambert conformal map factor (src/main.cpp:326-339)."""
     phi1, phi2 = math.radians(lat1), math.radians(lat2)
     q = math.pi / 4
-    if abs(phi1 - phi2) < 1e-6:
-        n = math.sin(phi1)
-    else:
-        n = math.log(math.cos(phi1) / math.cos(phi2)) / \
-            math.log(math.tan(q + phi2 / 2) / math.tan(q + phi1 / 2))
+    n = _cone_constant(phi1, phi2)
     A = math.cos(phi1) * math.tan(q + phi1 / 2) ** n
     phi = np.radians(lat_deg)
     return A / (np.tan(q + phi / 2) ** n * np.cos(phi))
