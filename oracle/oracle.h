@@ -50,6 +50,25 @@ double orc_absmax(const orc_model* m, const char* name);
 void orc_set_threads(int n);
 int orc_get_threads(void);
 
+/* ---- the oracle's own case assembly (case_builder.cpp): Config, file-mode
+ * input, Metric, BaseState, initial conditions and BdyManager restated
+ * from the reference (src/main.cpp:245-383 and the files it calls). ---- */
+typedef struct orc_case orc_case;
+/* .ini plus "key=value" overrides; NULL on any configuration/input error. */
+orc_case* orc_case_load(const char* ini, int n_over, const char* const* over);
+void orc_case_destroy(orc_case* c);
+/* nx, ny, nz into dims3; dx, dy, dz into d3. */
+int orc_case_dims(const orc_case* c, int* dims3, double* d3);
+/* Run scalars (dt, t_end, zt, file_mode) and DynParams by name. */
+int orc_case_value(const orc_case* c, const char* name, double* out);
+/* Arrays by the product's names (base, metric, initial state, bdy_wgt,
+ * bdy_lo_<v>, bdy_hi_<v> for v in u v thp pip qv). */
+long orc_case_array_size(const orc_case* c, const char* name);
+int orc_case_array(const orc_case* c, const char* name, double* out, long n);
+/* BdyManager::update(t) + tfrac(t) (boundary.cpp:83-97) into a model:
+ * new targets after a boundary-file crossing, then the time fraction. */
+int orc_case_update_bdy(orc_case* c, orc_model* m, double t);
+
 /* Stand-alone pieces for closed-form checks. */
 double orc_iface5(double qm2, double qm1, double q0, double q1, double q2, double q3, double vel);
 /* Thomas solve of a_k x_{k-1} + b_k x_k + c_k x_{k+1} = d_k, k = 0..n-1,

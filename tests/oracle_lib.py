@@ -55,12 +55,79 @@ def olib() -> C.CDLL:
         L.orc_qsat_w.argtypes = [C.c_double, C.c_double]
         L.orc_qsat_i.restype = C.c_double
         L.orc_qsat_i.argtypes = [C.c_double, C.c_double]
+        L.orc_case_load.restype = C.c_void_p
+        L.orc_case_load.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_char_p)]
+        L.orc_case_destroy.argtypes = [C.c_void_p]
+        L.orc_case_dims.argtypes = [C.c_void_p, C.POINTER(C.c_int), _DP]
+        L.orc_case_value.argtypes = [C.c_void_p, C.c_char_p, _DP]
+        L.orc_case_array_size.restype = C.c_long
+        L.orc_case_array_size.argtypes = [C.c_void_p, C.c_char_p]
+        L.orc_case_array.argtypes = [C.c_void_p, C.c_char_p, _DP, C.c_long]
+        L.orc_case_update_bdy.argtypes = [C.c_void_p, C.c_void_p, C.c_double]
         _o = L
     return _o
 
 
 def _p(a: np.ndarray):
     return a.ctypes.data_as(_DP)
+
+
+class OracleCase:
+    """The oracle's own case assembly (oracle/case_builder.cpp) from an .ini
+    file plus key=value overrides: grid, run values, DynParams and every
+    host array, built independently of the product's builders."""
+
+    def __init__(self, ini, **over):
+        L = olib()
+        ov = [f"{k}={v}".encode() for k, v in over.items()]
+        arr = (C.c_char_p * max(1, len(ov)))(*ov)
+        self._h = C.c_void_p(L.orc_case_load(str(ini).encode(), len(ov), arr))
+        if not self._h.value:
+            raise ValueError(f"oracle case builder rejected {ini} {over}")
+        dims = (C.c_int * 3)()
+        d = (C.c_double * 3)()
+        L.orc_case_dims(self._h, dims, d)
+        from wfe_amd import Grid
+        self.grid = Grid(dims[0], dims[1], dims[2], 3, d[0], d[1], d[2])
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value:
+            olib().orc_case_destroy(h)
+            self._h = None
+
+    def value(self, name: str) -> float:
+        out = C.c_double()
+        if olib().orc_case_value(self._h, name.encode(), C.byref(out)) != 0:
+            raise KeyError(name)
+        return out.value
+
+    def array(self, name: str) -> np.ndarray:
+        n = olib().orc_case_array_size(self._h, name.encode())
+        if n <= 0:
+            raise KeyError(name)
+        a = np.empty(n, dtype=np.float64)
+        assert olib().orc_case_array(self._h, name.encode(), _p(a), n) == 0, name
+        return a
+
+    def params(self) -> dict:
+        return {k: self.value(k) for k in PARAMS}
+
+    def model(self) -> "OracleModel":
+        """An oracle model of this case: base state, metric, initial state
+        and (file mode) the boundary weights and first targets, all from the
+        oracle's own builders; step it with step_case()."""
+        arrays = {k: self.array(k) for k in BASE + METRIC}
+        state = {k: self.array(k) for k in STATE}
+        om = OracleModel(self.grid, self.value("zt"), self.params(), arrays, state)
+        if self.value("file_mode"):
+            om.set("bdy_wgt", self.array("bdy_wgt"))
+            for v, n in enumerate(("u", "v", "thp", "pip", "qv")):
+                om.set(f"bdy_lo_{v}", self.array(f"bdy_lo_{n}"))
+                om.set(f"bdy_hi_{v}", self.array(f"bdy_hi_{n}"))
+            om.set_bdy(True, 0.0)
+        om.case = self
+        return om
 
 
 class OracleModel:
@@ -127,7 +194,12 @@ class OracleModel:
     def set_bdy(self, active: bool, tfrac: float) -> None:
         olib().orc_set_bdy(self._h, int(active), float(tfrac))
 
+    case = None
+
     def step(self, dt: float, t: float = 0.0) -> None:
+        if self.case is not None:
+            rc = olib().orc_case_update_bdy(self.case._h, self._h, t)
+            assert rc == 0, f"oracle boundary update failed ({rc}) at t={t}"
         olib().orc_step(self._h, dt, t)
 
     def run(self, dt: float, n: int) -> None:

@@ -55,7 +55,8 @@ def _build_and_pin(name: str, workdir: Path) -> tuple[str, int, str]:
     src.write_text(_mutate(ORACLE_SRC.read_text(), old, new, occ))
     so = workdir / f"lib{name}.so"
     r = subprocess.run(["g++", "-O1", "-std=c++17", "-fPIC", "-fopenmp", "-shared",
-                        f"-I{ROOT / 'oracle'}", str(src), "-o", str(so)],
+                        f"-I{ROOT / 'oracle'}", str(src), str(ROOT / "oracle" / "case_builder.cpp"),
+                        "-o", str(so)],
                        capture_output=True, text=True)
     if r.returncode != 0:
         return name, -100, r.stderr[-2000:]

@@ -64,6 +64,21 @@ struct Case {
   void build();
 };
 
+// Idealised initial state into c.ic (src/main.cpp:28-54, 146-195): the
+// cos^2 theta' bubble, the base wind on its faces, WK82 vapour, the vapour
+// blob.
+void idealised_initial_state(Case& c);
+
+// Full-field input (u, v, theta, pi, qv at the cell centres, nz*ny*nx, i
+// fastest) as padded perturbation fields on the C-grid: u on the west faces
+// 0..nx and v on the south faces 0..ny-1 from the neighbouring centres,
+// theta' = theta - thb, pi' = pi - pib, qv (src/main.cpp:58-100; the same
+// conversion as the boundary targets, src/dynamics/boundary.cpp:51-81).
+std::array<std::vector<double>, 5> perturbation_state(const GDims& g,
+                                                      const std::vector<double>& thb,
+                                                      const std::vector<double>& pib,
+                                                      const std::vector<double>* raw);
+
 // Lambert conformal map factor m(lat) for standard parallels lat1, lat2
 // (src/main.cpp:323-340); lat in degrees, interior ny*nx.
 std::vector<double> lambert_map_factor(const std::vector<double>& lat_deg, double lat1_deg,
