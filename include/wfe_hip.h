@@ -160,6 +160,56 @@ int wfe_download(wfe_ctx* ctx, int field, double* host, size_t n);
  * the interior (w: levels 0..nz). NaN/Inf report +Inf. Synchronous. */
 int wfe_absmax(wfe_ctx* ctx, int field, float* out);
 
+/* ---- output path ---- */
+
+/* The driver's diagnostics record (Diag, src/main.cpp:197-200). */
+typedef struct {
+  double wmax;    /* max |w| over the interior, levels 0..nz */
+  double thmin;   /* min(0, theta'), levels 0..nz-1 */
+  double thmax;   /* max(0, theta') */
+  double pipmax;  /* max |pi'| */
+  double qcmax;   /* max(0, qc); 0 for dry contexts */
+  double qrmax;   /* max(0, qr); 0 for dry contexts */
+  int finite;     /* 0 when any w, theta', pi' or qr value is NaN or Inf */
+} wfe_diag;
+
+/* diagnose (src/main.cpp:202-240) as a device reduction over the interior
+ * of the time-n state: the same extremes, start values and NaN rules as the
+ * host loops (NaN never enters an extreme, it clears `finite`; Inf enters),
+ * with only the record copied back instead of five whole fields.
+ * Synchronous. */
+int wfe_diagnose(wfe_ctx* ctx, wfe_diag* out);
+
+/* diagnose (src/main.cpp:202-240) over caller-owned device fields, as the
+ * reference's diagnose takes any State: w, theta', qc, qr are float (MIXED)
+ * or double (FP64) device arrays and pi' a double device array, all in the
+ * reference's padded layout (idx above: row length nx+2ng, ng = 3); qc and
+ * qr both NULL for a dry state. stream is a hipStream_t (NULL: the null
+ * stream) ordered after the writes of the fields. Allocates its workgroup
+ * partial records on the device for the call. Synchronous. */
+int wfe_diagnose_fields(const wfe_grid* g, int precision, const void* w, const void* thp,
+                        const double* pip, const void* qc, const void* qr, void* stream,
+                        wfe_diag* out);
+
+/* Writer::write_field (src/io/writer.cpp:89-107) without the full download:
+ * packs the interior of a state field (WFE_F_U .. WFE_F_QR) or the rain
+ * (WFE_F_RAIN) as float32 in the output order (i fastest, k slowest; nz
+ * levels, nz+1 for w, 1 for the rain; n must be nx*ny*levels) on the
+ * context's stream, then copies it into dst on a copy stream. Returns at
+ * once; dst is written once the copy has run (see wfe_output_wait). Later
+ * steps do not wait for the copy. dst from wfe_host_alloc (pinned) makes the
+ * copy asynchronous; other host memory works but the copy then blocks. */
+int wfe_pack_field_async(wfe_ctx* ctx, int field, float* dst, size_t n);
+/* Ticket for every copy enqueued so far by wfe_pack_field_async. */
+int wfe_output_mark(wfe_ctx* ctx, long long* ticket);
+/* Blocks until the copies of a ticket are in host memory. Callable from a
+ * thread other than the one stepping the context. The last 64 tickets are
+ * valid. */
+int wfe_output_wait(wfe_ctx* ctx, long long ticket);
+/* Pinned (page-locked) host memory for wfe_pack_field_async destinations. */
+int wfe_host_alloc(size_t bytes, void** out);
+int wfe_host_free(void* p);
+
 /* cudaDeviceSynchronize as used by the driver (main.cpp:441,468). */
 int wfe_sync(wfe_ctx* ctx);
 

@@ -55,6 +55,12 @@ class wfe_metric(C.Structure):
     _fields_ = [(n, _DP) for n in METRIC_NAMES] + [("zt", C.c_double)]
 
 
+class wfe_diag(C.Structure):
+    _fields_ = [("wmax", C.c_double), ("thmin", C.c_double), ("thmax", C.c_double),
+                ("pipmax", C.c_double), ("qcmax", C.c_double), ("qrmax", C.c_double),
+                ("finite", C.c_int)]
+
+
 _lib = None
 
 # (name, restype, argtypes) of every entry point in include/wfe_hip.h.
@@ -67,6 +73,15 @@ SIGNATURES = [
     ("wfe_download", C.c_int, [C.c_void_p, C.c_int, _DP, C.c_size_t]),
     ("wfe_absmax", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float)]),
     ("wfe_sync", C.c_int, [C.c_void_p]),
+    ("wfe_diagnose", C.c_int, [C.c_void_p, C.POINTER(wfe_diag)]),
+    ("wfe_diagnose_fields", C.c_int, [C.POINTER(wfe_grid), C.c_int, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.POINTER(wfe_diag)]),
+    ("wfe_pack_field_async", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_size_t]),
+    ("wfe_output_mark", C.c_int, [C.c_void_p, C.POINTER(C.c_longlong)]),
+    ("wfe_output_wait", C.c_int, [C.c_void_p, C.c_longlong]),
+    ("wfe_host_alloc", C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
+    ("wfe_host_free", C.c_int, [C.c_void_p]),
     ("wfe_set_bdy_weights", C.c_int, [C.c_void_p, _DP, C.c_size_t]),
     ("wfe_set_bdy_target", C.c_int, [C.c_void_p, C.c_int, C.c_int, _DP, C.c_size_t]),
     ("wfe_swap_bdy_targets", C.c_int, [C.c_void_p]),

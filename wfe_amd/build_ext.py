@@ -34,9 +34,10 @@ HIP_FLAGS = [
 ]
 CXX_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-Wall", "-Wno-sign-compare"]
 
-HIP_SOURCES = ["hip/context.hip"]
+HIP_SOURCES = ["hip/context.hip", "hip/diag_api.hip"]
 HOST_SOURCES = [
-    "core/config.cpp", "core/metric.cpp", "core/base_state.cpp", "core/case.cpp",
+    "core/config.cpp", "core/metric.cpp", "core/base_state.cpp", "core/initial_state.cpp",
+    "core/boundary_host.cpp", "core/case.cpp",
     "io/input.cpp", "io/writer.cpp", "capi/case_api.cpp", "dynamics/integrator.cpp",
 ]
 APP_SOURCES = ["apps/wfe_main.cpp"]
@@ -69,7 +70,8 @@ def build(verbose: bool = False) -> dict:
     for s in APP_SOURCES:
         jobs.append([CXX, *CXX_FLAGS, *inc, "-c", str(CSRC / s), "-o", str(_obj(s))])
     jobs.append([CXX, "-O2", "-std=c++17", "-fPIC", "-fopenmp", "-shared", "-Wall",
-                 str(ORACLE / "oracle.cpp"), "-o", str(ORACLE / "liboracle.so.tmp")])
+                 str(ORACLE / "oracle.cpp"), str(ORACLE / "case_builder.cpp"),
+                 "-o", str(ORACLE / "liboracle.so.tmp")])
     with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 4)) as ex:
         list(ex.map(_run, jobs))
     (ORACLE / "liboracle.so.tmp").replace(ORACLE / "liboracle.so")

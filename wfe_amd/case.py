@@ -12,7 +12,7 @@ from pathlib import Path
 
 import numpy as np
 
-from ._lib import (FIELDS, STATE_FIELDS, WFE_PREC_FP64, WFE_PREC_MIXED, as_dptr, check, lib,
+from ._lib import (wfe_diag, FIELDS, STATE_FIELDS, WFE_PREC_FP64, WFE_PREC_MIXED, as_dptr, check, lib,
                    wfe_base, wfe_grid, wfe_metric, wfe_params, BASE_NAMES, METRIC_NAMES)
 
 PRECISIONS = {"fp64": WFE_PREC_FP64, "mixed": WFE_PREC_MIXED}
@@ -170,6 +170,27 @@ class Integrator:
         out = C.c_float()
         check(lib().wfe_absmax(self._h, FIELDS[name], C.byref(out)), "wfe_absmax")
         return out.value
+
+    def diagnose(self) -> dict:
+        """Interior diagnostics of the time-n state as a device reduction
+        (wfe_diagnose): wmax, thmin, thmax, pipmax, qcmax, qrmax, finite."""
+        d = wfe_diag()
+        check(lib().wfe_diagnose(self._h, C.byref(d)), "wfe_diagnose")
+        return {f: getattr(d, f) for f, _ in wfe_diag._fields_}
+
+    def packed(self, name: str) -> np.ndarray:
+        """Interior of a state field (or the rain) as float32 in output order
+        through wfe_pack_field_async + wfe_output_mark/wait."""
+        g = self.grid
+        nlev = g.nz + 1 if name == "w" else (1 if name == "rain" else g.nz)
+        a = np.empty(g.nx * g.ny * nlev, dtype=np.float32)
+        check(lib().wfe_pack_field_async(self._h, FIELDS[name],
+                                         a.ctypes.data_as(C.POINTER(C.c_float)), a.size),
+              f"pack {name}")
+        t = C.c_longlong()
+        check(lib().wfe_output_mark(self._h, C.byref(t)), "wfe_output_mark")
+        check(lib().wfe_output_wait(self._h, t.value), "wfe_output_wait")
+        return a
 
     # -- per-kernel HIP-event timing on the context's stream
     def kernel_timing(self, mode: int) -> None:

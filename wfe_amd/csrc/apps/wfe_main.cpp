@@ -22,8 +22,9 @@ struct Diag {
   bool finite = true;
 };
 
-// Interior diagnostics (src/main.cpp:197-240).
-Diag diagnose(const wfe::GDims& g, wfe::State& s, std::vector<double>& buf, bool moist) {
+// Interior diagnostics (src/main.cpp:197-240) on the host from whole-field
+// downloads: the reference's path, kept for synchronous output runs.
+Diag diagnose_host(const wfe::GDims& g, wfe::State& s, std::vector<double>& buf, bool moist) {
   Diag d;
   s.w.download(buf.data());
   for (int k = 0; k <= g.nz; ++k)
@@ -64,6 +65,21 @@ Diag diagnose(const wfe::GDims& g, wfe::State& s, std::vector<double>& buf, bool
           d.qrmax = std::max(d.qrmax, v);
         }
   }
+  return d;
+}
+
+// The same record as a device reduction: only the record is copied back.
+Diag diagnose_device(wfe_ctx* ctx) {
+  wfe_diag r{};
+  wfe::wfe_check(wfe_diagnose(ctx, &r), "diagnose");
+  Diag d;
+  d.wmax = r.wmax;
+  d.thmin = r.thmin;
+  d.thmax = r.thmax;
+  d.pipmax = r.pipmax;
+  d.qcmax = r.qcmax;
+  d.qrmax = r.qrmax;
+  d.finite = r.finite != 0;
   return d;
 }
 
@@ -156,7 +172,7 @@ int main(int argc, char** argv) {
     if (k != "precision")
       std::fprintf(stderr, "WARNING: config key never read (typo?): %s\n", k.c_str());
 
-  std::vector<double> diag_buf(g.npts());
+  std::vector<double> diag_buf(writer.asynchronous() ? 0 : g.npts());
   const int nsteps = (int)std::ceil(c.t_end / c.dt);
   const int diag_steps = std::max(1, (int)std::round(c.diag_interval / c.dt));
   const int out_steps = std::max(1, (int)std::round(c.out_interval / c.dt));
@@ -173,13 +189,15 @@ int main(int argc, char** argv) {
                                "writing emergency output and stopping\n",
                        wm, step, t);
           writer.write(ctx, step, t);
+          writer.finish();
           wfe_case_destroy(cs);
           return 3;
         }
       }
       if (step % diag_steps == 0 || step == nsteps) {
-        wfe_check(wfe_sync(ctx), "sync");
-        const Diag d = diagnose(g, integ.state(), diag_buf, c.dp.moisture);
+        const Diag d = writer.asynchronous()
+                           ? diagnose_device(ctx)
+                           : diagnose_host(g, integ.state(), diag_buf, c.dp.moisture);
         if (c.dp.moisture)
           std::printf("adim %6d  t=%7.1fs  |w|max=%7.3f  th'=[%+.2f,%+.2f]K  "
                       "qc=%.2fg/kg qr=%.2fg/kg\n",
@@ -190,16 +208,17 @@ int main(int argc, char** argv) {
         std::fflush(stdout);
         if (!d.finite) {
           std::fprintf(stderr, "ERROR: NaN/Inf detected, simulation stopped.\n");
+          writer.finish();
           wfe_case_destroy(cs);
           return 2;
         }
       }
       if (step % out_steps == 0 || step == nsteps) {
-        writer.write(ctx, step, t);
-        if (c.dp.moisture) writer.write_field2d(ctx, WFE_F_RAIN, "rain", step);
+        writer.write(ctx, step, t, c.dp.moisture);
       }
     }
     wfe_check(wfe_sync(ctx), "sync");
+    writer.finish();
     const double wall =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     std::printf("bitti: %d adim, %.1f s duvar zamani, %.1f adim/s, gercek-zaman orani %.0fx\n",

@@ -9,6 +9,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -19,6 +20,7 @@
 #include "hip/kessler_kernels.hpp"
 #include "hip/slow_kernels.hpp"
 #include "hip/adjoint_kernels.hpp"
+#include "hip/output_kernels.hpp"
 
 namespace wfe {
 
@@ -200,6 +202,14 @@ struct CtxBase {
   virtual void upload(int field, const double* h, size_t n) = 0;
   virtual void download(int field, double* h, size_t n) = 0;
   virtual float absmax(int field) = 0;
+  // Interior diagnostics record (dev::kDg* slots) of the time-n state.
+  virtual void diagnose(double* rec) = 0;
+  // Interior of a state field (or the rain) packed as float32 in output
+  // order and copied asynchronously into dst; output_mark returns a ticket
+  // for the copies enqueued so far, output_wait blocks until they are done.
+  virtual void pack_async(int field, float* dst, size_t n) = 0;
+  virtual long long output_mark() = 0;
+  virtual void output_wait(long long ticket) = 0;
   virtual void sync() = 0;
   virtual void set_bdy_weights(const double* w, size_t n) = 0;
   virtual void set_bdy_target(int slot, int var, const double* h, size_t n) = 0;
@@ -237,12 +247,26 @@ class Ctx final : public CtxBase {
   ~Ctx() override {
     for (hipGraphExec_t& e : gexec_)
       if (e) (void)hipGraphExecDestroy(e);
+    if (ostream_) {
+      (void)hipStreamSynchronize(ostream_);
+      (void)hipStreamDestroy(ostream_);
+    }
+    for (hipEvent_t& e : out_packed_)
+      if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t& e : out_copied_)
+      if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t& e : out_ring_)
+      if (e) (void)hipEventDestroy(e);
     if (stream_) (void)hipStreamDestroy(stream_);
   }
   void step(double dt) override;
   void upload(int field, const double* h, size_t n) override;
   void download(int field, double* h, size_t n) override;
   float absmax(int field) override;
+  void diagnose(double* rec) override;
+  void pack_async(int field, float* dst, size_t n) override;
+  long long output_mark() override;
+  void output_wait(long long ticket) override;
   void sync() override { WFE_HIP(hipStreamSynchronize(stream_)); }
   void set_bdy_weights(const double* w, size_t n) override;
   void set_bdy_target(int slot, int var, const double* h, size_t n) override;
@@ -348,6 +372,23 @@ class Ctx final : public CtxBase {
   T* bdy_wgt_ = nullptr;
   double tfrac_ = 0;
   unsigned* d_absmax_ = nullptr;
+  // Diagnostics partial records (one per workgroup of k_diag_partial) and
+  // the final record after them.
+  double* diag_part_ = nullptr;
+  int diag_nb_ = 0;
+  // Asynchronous output: a float32 staging array per packed field, filled
+  // on stream_ and copied to the host on ostream_ (default priority), with
+  // per-field events so a field is not repacked before its last copy ended;
+  // out_ring_ holds the tickets of output_mark.
+  hipStream_t ostream_ = nullptr;
+  float* out_stage_[WFE_F_COUNT]{};
+  hipEvent_t out_packed_[WFE_F_COUNT]{};
+  hipEvent_t out_copied_[WFE_F_COUNT]{};
+  bool out_used_[WFE_F_COUNT]{};
+  static constexpr int kOutRing = 64;
+  hipEvent_t out_ring_[kOutRing]{};
+  long long out_seq_ = 0;
+  std::mutex out_mu_;
   // Acoustic substep methods (split u/v + column-solve pairs and fused
   // shapes); all run under the same buffer rotation, the fastest is chosen
   // by a timing pass on the first stage.
@@ -620,6 +661,8 @@ Ctx<T>::Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& b, const w
     }
   rain_ = arena_.alloc<T>(n2_);
   d_absmax_ = arena_.alloc<unsigned>(1);
+  diag_nb_ = (int)(cdiv(g_.nx, 64) * cdiv(g_.ny, 4) * cdiv(g_.nz + 1, dev::kDiagLevels));
+  diag_part_ = arena_.alloc<double>((size_t)(diag_nb_ + 1) * dev::kDgSlots);
   {
     int dev = 0;
     WFE_HIP(hipGetDevice(&dev));
@@ -704,6 +747,83 @@ float Ctx<T>::absmax(int f) {
   std::memcpy(&v, &bits, sizeof(float));
   if (std::isnan(v)) v = INFINITY;
   return v;
+}
+
+template <class T>
+void Ctx<T>::diagnose(double* rec) {
+  const StateSet<T>& s = S_[cur_];
+  const dim3 grd(cdiv(g_.nx, 64), cdiv(g_.ny, 4), cdiv(g_.nz + 1, dev::kDiagLevels));
+  dev::k_diag_partial<T><<<grd, dim3(64, 4), 0, stream_>>>(
+      g_, s.w, s.thp, s.pip, p_.moisture ? s.qc : nullptr, p_.moisture ? s.qr : nullptr,
+      diag_part_);
+  check_launch("diag_partial");
+  double* fin = diag_part_ + (size_t)diag_nb_ * dev::kDgSlots;
+  dev::k_diag_final<<<1, 256, 0, stream_>>>(diag_part_, diag_nb_, fin);
+  check_launch("diag_final");
+  WFE_HIP(hipMemcpyAsync(rec, fin, dev::kDgSlots * sizeof(double), hipMemcpyDeviceToHost,
+                         stream_));
+  WFE_HIP(hipStreamSynchronize(stream_));
+}
+
+template <class T>
+void Ctx<T>::pack_async(int f, float* dst, size_t n) {
+  const bool is_state = f >= WFE_F_U && f <= WFE_F_QR;
+  if (!is_state && f != WFE_F_RAIN)
+    throw ArgError("pack: field must be a state field or the rain");
+  if (!p_.moisture && (f == WFE_F_QV || f == WFE_F_QC || f == WFE_F_QR || f == WFE_F_RAIN))
+    throw ArgError("pack: moisture fields of a dry context");
+  const int nlev = f == WFE_F_W ? g_.nz + 1 : (f == WFE_F_RAIN ? 1 : g_.nz);
+  if (n != (size_t)g_.nx * g_.ny * nlev) throw ArgError("pack: element count mismatch");
+  if (!ostream_) {
+    WFE_HIP(hipStreamCreate(&ostream_));
+    for (int q = 0; q < WFE_F_COUNT; ++q) {
+      WFE_HIP(hipEventCreateWithFlags(&out_packed_[q], hipEventDisableTiming));
+      WFE_HIP(hipEventCreateWithFlags(&out_copied_[q], hipEventDisableTiming));
+    }
+  }
+  if (!out_stage_[f]) {
+    WFE_HIP(hipStreamSynchronize(stream_));
+    out_stage_[f] = arena_.alloc<float>((size_t)g_.nx * g_.ny * (g_.nz + 1));
+  }
+  // Do not repack over a copy still in flight.
+  if (out_used_[f]) WFE_HIP(hipStreamWaitEvent(stream_, out_copied_[f], 0));
+  out_used_[f] = true;
+  const dim3 grd(cdiv(g_.nx, 64), cdiv(g_.ny, 4), (unsigned)nlev);
+  if (f == WFE_F_PIP)
+    dev::k_pack_interior<double><<<grd, dim3(64, 4), 0, stream_>>>(g_, S_[cur_].pip, 0,
+                                                                   out_stage_[f]);
+  else
+    dev::k_pack_interior<T><<<grd, dim3(64, 4), 0, stream_>>>(g_, field_ptr(f),
+                                                              f == WFE_F_RAIN, out_stage_[f]);
+  check_launch("pack_interior");
+  WFE_HIP(hipEventRecord(out_packed_[f], stream_));
+  WFE_HIP(hipStreamWaitEvent(ostream_, out_packed_[f], 0));
+  WFE_HIP(hipMemcpyAsync(dst, out_stage_[f], n * sizeof(float), hipMemcpyDeviceToHost,
+                         ostream_));
+  WFE_HIP(hipEventRecord(out_copied_[f], ostream_));
+}
+
+template <class T>
+long long Ctx<T>::output_mark() {
+  std::lock_guard<std::mutex> lk(out_mu_);
+  if (!ostream_) throw ArgError("output_mark: nothing has been packed");
+  const long long t = out_seq_++;
+  hipEvent_t& e = out_ring_[t % kOutRing];
+  if (!e) WFE_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  WFE_HIP(hipEventRecord(e, ostream_));
+  return t;
+}
+
+template <class T>
+void Ctx<T>::output_wait(long long t) {
+  hipEvent_t e;
+  {
+    std::lock_guard<std::mutex> lk(out_mu_);
+    if (t < 0 || t >= out_seq_ || t < out_seq_ - kOutRing)
+      throw ArgError("output_wait: unknown or expired ticket");
+    e = out_ring_[t % kOutRing];
+  }
+  WFE_HIP(hipEventSynchronize(e));
 }
 
 template <class T>
@@ -2053,6 +2173,62 @@ int wfe_absmax(wfe_ctx* ctx, int field, float* out) {
   if (int e = need_ctx(ctx)) return e;
   if (!out) return WFE_E_ARG;
   return guarded([&] { *out = ctx->impl->absmax(field); });
+}
+
+int wfe_diagnose(wfe_ctx* ctx, wfe_diag* out) {
+  if (int e = need_ctx(ctx)) return e;
+  if (!out) {
+    wfe::set_last_error("wfe_diagnose: null output");
+    return WFE_E_ARG;
+  }
+  return guarded([&] {
+    double r[wfe::dev::kDgSlots];
+    ctx->impl->diagnose(r);
+    out->wmax = r[wfe::dev::kDgW];
+    out->thmin = r[wfe::dev::kDgThMin];
+    out->thmax = r[wfe::dev::kDgThMax];
+    out->pipmax = r[wfe::dev::kDgPip];
+    out->qcmax = r[wfe::dev::kDgQc];
+    out->qrmax = r[wfe::dev::kDgQr];
+    out->finite = r[wfe::dev::kDgFinite] != 0.0;
+  });
+}
+
+int wfe_pack_field_async(wfe_ctx* ctx, int field, float* dst, size_t n) {
+  if (int e = need_ctx(ctx)) return e;
+  if (!dst) {
+    wfe::set_last_error("wfe_pack_field_async: null destination");
+    return WFE_E_ARG;
+  }
+  return guarded([&] { ctx->impl->pack_async(field, dst, n); });
+}
+
+int wfe_output_mark(wfe_ctx* ctx, long long* ticket) {
+  if (int e = need_ctx(ctx)) return e;
+  if (!ticket) {
+    wfe::set_last_error("wfe_output_mark: null ticket");
+    return WFE_E_ARG;
+  }
+  return guarded([&] { *ticket = ctx->impl->output_mark(); });
+}
+
+int wfe_output_wait(wfe_ctx* ctx, long long ticket) {
+  if (int e = need_ctx(ctx)) return e;
+  return guarded([&] { ctx->impl->output_wait(ticket); });
+}
+
+int wfe_host_alloc(size_t bytes, void** out) {
+  return guarded([&] {
+    if (!out || bytes == 0) throw wfe::ArgError("wfe_host_alloc: bad arguments");
+    *out = nullptr;
+    WFE_HIP(hipHostMalloc(out, bytes, hipHostMallocDefault));
+  });
+}
+
+int wfe_host_free(void* p) {
+  return guarded([&] {
+    if (p) WFE_HIP(hipHostFree(p));
+  });
 }
 
 int wfe_sync(wfe_ctx* ctx) {

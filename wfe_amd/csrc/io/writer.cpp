@@ -1,10 +1,31 @@
 #include "io/writer.hpp"
 
 #include <cstdio>
+#include <cstdlib>
 #include <filesystem>
 #include <stdexcept>
 
 namespace wfe {
+
+Writer::Writer() {
+  if (const char* e = std::getenv("WFE_SYNC_OUTPUT")) async_ = std::string(e) != "1";
+}
+
+Writer::~Writer() {
+  try {
+    finish();
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "%s\n", e.what());
+  }
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    stop_ = true;
+  }
+  cv_.notify_all();
+  if (thread_.joinable()) thread_.join();
+  for (float*& p : slot_mem_)
+    if (p) (void)wfe_host_free(p);
+}
 
 void Writer::init(const GDims& g, const std::string& out_dir, double dt, bool moisture,
                   bool has_tsk) {
@@ -12,8 +33,6 @@ void Writer::init(const GDims& g, const std::string& out_dir, double dt, bool mo
   dir_ = out_dir;
   moist_ = moisture;
   std::filesystem::create_directories(dir_);
-  full_.resize(g.npts());
-  out_.resize((size_t)g.nx * g.ny * (g.nz + 1));
   const std::string meta = dir_ + "/meta.json";
   std::FILE* f = std::fopen(meta.c_str(), "w");
   if (!f) throw std::runtime_error("cannot write " + meta);
@@ -29,6 +48,43 @@ void Writer::init(const GDims& g, const std::string& out_dir, double dt, bool mo
   if (has_tsk) std::fprintf(f, ", \"tsk\": 1, \"pblh\": 1, \"t2m\": 1, \"u10\": 1");
   std::fprintf(f, "}\n}\n");
   std::fclose(f);
+
+  per_field_ = (size_t)g.nx * g.ny * (g.nz + 1);
+  if (async_) {
+    for (int s = 0; s < kSlots; ++s) {
+      void* p = nullptr;
+      if (wfe_host_alloc(per_field_ * kFieldsPerSlot * sizeof(float), &p) != WFE_OK)
+        throw std::runtime_error(std::string("pinned output buffer: ") + wfe_last_error());
+      slot_mem_[s] = static_cast<float*>(p);
+    }
+    thread_ = std::thread([this] { worker(); });
+  } else {
+    full_.resize(g.npts());
+    out_.resize(per_field_);
+  }
+}
+
+std::vector<Writer::Item> Writer::items_for(bool with_rain) const {
+  const size_t n2 = (size_t)g_.nx * g_.ny;
+  const size_t n3 = n2 * g_.nz;
+  std::vector<Item> it = {{"u", WFE_F_U, n3},
+                          {"v", WFE_F_V, n3},
+                          {"w", WFE_F_W, n3 + n2},
+                          {"thp", WFE_F_THP, n3},
+                          {"pip", WFE_F_PIP, n3}};
+  if (moist_) {
+    it.push_back({"qv", WFE_F_QV, n3});
+    it.push_back({"qc", WFE_F_QC, n3});
+    it.push_back({"qr", WFE_F_QR, n3});
+    if (with_rain) it.push_back({"rain", WFE_F_RAIN, n2});
+  }
+  return it;
+}
+
+std::string Writer::path_of(const std::string& name, int step) const {
+  char path[1024];
+  std::snprintf(path, sizeof(path), "%s/%s_%06d.bin", dir_.c_str(), name.c_str(), step);
+  return path;
 }
 
 void Writer::write_raw(const std::string& path, const float* data, size_t n) {
@@ -39,43 +95,108 @@ void Writer::write_raw(const std::string& path, const float* data, size_t n) {
   if (wrote != n) throw std::runtime_error("short write to " + path);
 }
 
-void Writer::write_field(wfe_ctx* ctx, int field, const char* name, int step, int nzlev) {
-  if (wfe_download(ctx, field, full_.data(), full_.size()) != WFE_OK)
-    throw std::runtime_error(std::string("download failed: ") + wfe_last_error());
-  size_t o = 0;
-  for (int k = 0; k < nzlev; ++k)
-    for (int j = 0; j < g_.ny; ++j)
-      for (int i = 0; i < g_.nx; ++i) out_[o++] = (float)full_[g_.idx(i, j, k)];
-  char path[1024];
-  std::snprintf(path, sizeof(path), "%s/%s_%06d.bin", dir_.c_str(), name, step);
-  write_raw(path, out_.data(), o);
+// The reference's path (src/io/writer.cpp:89-107): whole-field download,
+// interior packed on the host.
+void Writer::write_sync(wfe_ctx* ctx, int step, const std::vector<Item>& items) {
+  for (const Item& it : items) {
+    const bool two_d = it.field == WFE_F_RAIN;
+    const size_t nfull = two_d ? g_.n2() : g_.npts();
+    if (wfe_download(ctx, it.field, full_.data(), nfull) != WFE_OK)
+      throw std::runtime_error(std::string("download failed: ") + wfe_last_error());
+    const int nlev = two_d ? 1 : (it.field == WFE_F_W ? g_.nz + 1 : g_.nz);
+    size_t o = 0;
+    for (int k = 0; k < nlev; ++k)
+      for (int j = 0; j < g_.ny; ++j)
+        for (int i = 0; i < g_.nx; ++i)
+          out_[o++] = (float)full_[two_d ? g_.idx2(i, j) : g_.idx(i, j, k)];
+    write_raw(path_of(it.name, step), out_.data(), o);
+  }
 }
 
-void Writer::write(wfe_ctx* ctx, int step, double t) {
-  write_field(ctx, WFE_F_U, "u", step, g_.nz);
-  write_field(ctx, WFE_F_V, "v", step, g_.nz);
-  write_field(ctx, WFE_F_W, "w", step, g_.nz + 1);
-  write_field(ctx, WFE_F_THP, "thp", step, g_.nz);
-  write_field(ctx, WFE_F_PIP, "pip", step, g_.nz);
-  if (moist_) {
-    write_field(ctx, WFE_F_QV, "qv", step, g_.nz);
-    write_field(ctx, WFE_F_QC, "qc", step, g_.nz);
-    write_field(ctx, WFE_F_QR, "qr", step, g_.nz);
+void Writer::rethrow_locked() {
+  if (err_) {
+    std::exception_ptr e = err_;
+    err_ = nullptr;
+    std::rethrow_exception(e);
+  }
+}
+
+void Writer::write(wfe_ctx* ctx, int step, double t, bool with_rain) {
+  const std::vector<Item> items = items_for(with_rain);
+  if (!async_) {
+    write_sync(ctx, step, items);
+  } else {
+    int slot;
+    {
+      std::unique_lock<std::mutex> lk(mu_);
+      rethrow_locked();
+      slot = next_slot_;
+      cv_.wait(lk, [&] { return !slot_busy_[slot] || err_; });
+      rethrow_locked();
+      slot_busy_[slot] = true;
+      next_slot_ = (slot + 1) % kSlots;
+    }
+    Job job{ctx, slot, step, 0, items};
+    try {
+      for (size_t q = 0; q < items.size(); ++q) {
+        float* dst = slot_mem_[slot] + q * per_field_;
+        if (wfe_pack_field_async(ctx, items[q].field, dst, items[q].n) != WFE_OK)
+          throw std::runtime_error(std::string("output pack failed: ") + wfe_last_error());
+      }
+      if (wfe_output_mark(ctx, &job.ticket) != WFE_OK)
+        throw std::runtime_error(std::string("output mark failed: ") + wfe_last_error());
+    } catch (...) {
+      std::lock_guard<std::mutex> lk(mu_);
+      slot_busy_[slot] = false;
+      throw;
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      queue_.push_back(std::move(job));
+    }
+    cv_.notify_all();
   }
   std::printf("  cikti yazildi: step %d (t = %.1f s)\n", step, t);
 }
 
-void Writer::write_field2d(wfe_ctx* ctx, int field, const char* name, int step) {
-  std::vector<double> h2(g_.n2());
-  if (wfe_download(ctx, field, h2.data(), h2.size()) != WFE_OK)
-    throw std::runtime_error(std::string("download failed: ") + wfe_last_error());
-  std::vector<float> o((size_t)g_.nx * g_.ny);
-  size_t idx = 0;
-  for (int j = 0; j < g_.ny; ++j)
-    for (int i = 0; i < g_.nx; ++i) o[idx++] = (float)h2[g_.idx2(i, j)];
-  char path[1024];
-  std::snprintf(path, sizeof(path), "%s/%s_%06d.bin", dir_.c_str(), name, step);
-  write_raw(path, o.data(), o.size());
+void Writer::worker() {
+  for (;;) {
+    Job job;
+    {
+      std::unique_lock<std::mutex> lk(mu_);
+      cv_.wait(lk, [&] { return stop_ || !queue_.empty(); });
+      if (queue_.empty()) return;
+      job = std::move(queue_.front());
+      queue_.pop_front();
+    }
+    try {
+      if (wfe_output_wait(job.ctx, job.ticket) != WFE_OK)
+        throw std::runtime_error(std::string("output copy failed: ") + wfe_last_error());
+      for (size_t q = 0; q < job.items.size(); ++q)
+        write_raw(path_of(job.items[q].name, job.step), slot_mem_[job.slot] + q * per_field_,
+                  job.items[q].n);
+    } catch (...) {
+      std::lock_guard<std::mutex> lk(mu_);
+      if (!err_) err_ = std::current_exception();
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      slot_busy_[job.slot] = false;
+    }
+    cv_.notify_all();
+  }
+}
+
+void Writer::finish() {
+  if (!async_) return;
+  std::unique_lock<std::mutex> lk(mu_);
+  cv_.wait(lk, [&] {
+    if (!queue_.empty()) return false;
+    for (bool b : slot_busy_)
+      if (b) return false;
+    return true;
+  });
+  rethrow_locked();
 }
 
 void Writer::write_static(const char* name, const std::vector<float>& data) {
