@@ -1,21 +1,29 @@
 #!/usr/bin/env python3
 """Benchmark of the large-step dynamics hot path on one MI355X.
 
-Workload: the moist supercell case (cases/cfg2_supercell.ini): 120 x 120 x 40
-grid points, dx = 1 km, geometric levels to 20 km, WK82 sounding, quarter-
-circle hodograph, Kessler microphysics, open boundaries, dt = 6 s, 6 acoustic
-substeps. A "step" is one large RK3 step (3 stages, 1 + 3 + 6 acoustic
-substeps, Kessler). Inputs are resident in HBM before the timed region.
+Headline workload: the real-shape regional case, 330 x 220 x 45 points at
+12 km (3.27 M interior points), seeded spectral terrain 0-3 km, Lambert map
+factors 0.95-1.05, US standard atmosphere with seeded perturbations, PD
+moisture transport, Kessler microphysics, 5-point Davies relaxation toward
+boundary files every 3 h (read as the run crosses each boundary time),
+dt = 60 s, 6 acoustic substeps. The case is written in the reference's
+file formats (wfe_input.ini, wfe_init.bin, wfe_bdy_%03d.bin) and loaded by
+the native case builder. A "step" is one large RK3 step (3 stages, 1 + 3 + 6
+acoustic substeps, Kessler) plus the boundary time update; inputs are
+resident in HBM before the timed region.
 
-Prints one JSON line: grid-point·steps/s of the mixed-precision path (value),
-the plain-FP64 path beside it, the relative error of the mixed path against
-the FP64 path after the same steps, the whole-step and dominant-kernel
-roofline against 8 TB/s HBM3E, and the CPU oracle timed on the host cores.
+Prints one JSON line: grid-point.steps/s of the mixed-precision path
+(value), the plain-FP64 path beside it, the error of the mixed path against
+the FP64 path, the acoustic substep's roofline against 8 TB/s HBM3E, the
+whole step's roofline, the FP64 CPU oracle timed on the host cores, and the
+moist supercell (120 x 120 x 40), the uneven ensemble and the adjoint as
+further keys.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -27,21 +35,31 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
-CASE = ROOT / "cases" / "cfg2_supercell.ini"
+SUPERCELL = ROOT / "cases" / "cfg2_supercell.ini"
+LAM_GRID = dict(nx=330, ny=220, nz=45)
+LAM_BDY_INTERVAL_H = 3
 
 # Algorithmic HBM bytes per interior point and large step: every phase that
 # is separated by a global dependency reads each distinct 3-D input once and
 # writes each output once, at its storage width (mass fluxes/divergence,
-# halos, 1-D/2-D arrays not counted). Moist + Kessler, ns = 6 (10 substeps):
-# 339 accesses; mixed: pi', pi'prev 8 B, the rest 4 B.
-ALG_BYTES_STEP = {"mixed": 1520, "fp64": 2712}
+# halos, 1-D/2-D arrays not counted). ns = 6 (10 substeps). Mixed: pi',
+# pi'prev 8 B, the rest 4 B.
+#  regional (moist + PD + Kessler): 375 accesses
+#  supercell (moist + Kessler): 339 accesses
+ALG_BYTES_STEP = {"regional": {"mixed": 1664, "fp64": 3000},
+                  "supercell": {"mixed": 1520, "fp64": 2712}}
+# One acoustic substep, moist: reads u v w pi' pi'prev theta' Tu Tv Tw
+# Ttheta thb thbw dthb/dz rtjx rtjy kt3 aw qv qc qr qvbar (21), writes u v w
+# pi' pi'prev theta' (6): 4 x 8 B (pi', pi'prev read and written) + 23 x 4 B
+# in the mixed mode, 27 x 8 B in FP64.
+SUBSTEP_BYTES = {"mixed": 124, "fp64": 216}
 
 # Per-kernel algorithmic bytes per interior point per launch (distinct arrays
 # the kernel must read and write, moist case, storage widths of the mode).
 # The moist part of the substep buoyancy arrives inside Tw, so the substep
 # kernels read qv only (for theta_v).
 #  acou_wpi      : r u v w thp tw tth thbw dthbdz rtjx rtjy kt3 aw qv
-#                  (13 x 4 B) + pip (8 B); w thp (4 B) + pip (8 B) written.
+#                  (13 x 4 B) + pip (8 B); w w thp (4 B) + pip (8 B).
 #  acou_uv       : r u v thp qv tu tv thb (7 x 4 B) + pip pipprev (8 B); w u v.
 #  acoustic_fused: one whole substep: r u v thp qv tu tv thb w tw tth thbw
 #                  dthbdz rtjx rtjy kt3 aw (16 x 4 B) + pip pipprev (8 B);
@@ -82,9 +100,11 @@ KERNEL_SYMBOL = {"acou_wpi": "k_acou_wpi_tile", "acou_uv": "k_acou_uv_tile",
                  "kessler": "k_kessler"}
 # HBM traffic per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE passes
 # of this benchmark (written by python -m wfe_amd.prof_summary --json).
-PMC_SUMMARY = ROOT / "profiles" / "pmc_supercell_120x120x40_mixed.json"
+PMC_SUMMARY = {"regional": ROOT / "profiles" / "pmc_regional_lam_330x220x45_mixed.json",
+               "supercell": ROOT / "profiles" / "pmc_supercell_120x120x40_mixed.json"}
 SOLO_KERNELS = ("acou_wpi", "acou_uv", "acoustic_fused", "tendencies", "massflux",
                 "fill_ghosts", "stage_q")
+SUBSTEP_KERNELS = ("acoustic_fused", "acou_wpi", "acou_uv")
 
 
 def log(*a):
@@ -109,8 +129,9 @@ def timed_run(case, precision: str, warmup: int, steps: int):
 
 def kernel_profile(case, precision: str, warmup: int, steps: int, solo_reps: int):
     """Per-launch HIP-event timing of every kernel of the step on the
-    context's stream, then the heaviest kernels launched back to back
-    between two events (no per-launch event overhead)."""
+    context's stream, then the heaviest kernels (and every acoustic substep
+    kernel that ran) launched back to back between two events, with no
+    per-launch event overhead."""
     from wfe_amd import Integrator
 
     gi = Integrator.from_case(case, precision)
@@ -125,17 +146,18 @@ def kernel_profile(case, precision: str, warmup: int, steps: int, solo_reps: int
     gi.kernel_timing(0)
     solo = {}
     for k in sorted(rep, key=lambda k: -rep[k][1]):
-        if k in SOLO_KERNELS and len(solo) < 4:
+        if k in SOLO_KERNELS and (len(solo) < 4 or k in SUBSTEP_KERNELS):
             solo[k] = gi.kernel_bench(k, solo_reps)
     return rep, tune, solo
 
 
-def pmc_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` (mixed build) from the committed
-    rocprofv3 PMC summary, or None."""
-    if not PMC_SUMMARY.exists():
+def pmc_traffic(kernel: str, which: str):
+    """HBM-side bytes per launch of `kernel` (mixed build) from the committed
+    rocprofv3 PMC summary of the workload, or None."""
+    path = PMC_SUMMARY[which]
+    if not path.exists():
         return None
-    d = json.loads(PMC_SUMMARY.read_text())
+    d = json.loads(path.read_text())
     sym = KERNEL_SYMBOL.get(kernel)
     cands = [(v.get("calls", 0), v) for n, v in d.get("kernels", {}).items()
              if sym and n.startswith(sym + "<float")]
@@ -145,6 +167,35 @@ def pmc_traffic(kernel: str):
     if v.get("fetch_bytes") is None or v.get("write_bytes") is None:
         return None
     return v["fetch_bytes"] + v["write_bytes"]
+
+
+def substep_roofline(rep, solo, N: int, steps: int, which: str):
+    """Roofline of the acoustic substep, the step's dominant work: the
+    substep's algorithmic bytes per interior point (moist, mixed: 124 B) x N
+    over the average duration of one substep's launches, measured back to
+    back with HIP events on the context's stream. A substep is one launch of
+    the fused kernel, or one launch each of the u/v and column kernels."""
+    if "acoustic_fused" in rep and "acoustic_fused" in solo:
+        kernels = ("acoustic_fused",)
+    elif "acou_wpi" in solo and "acou_uv" in solo:
+        kernels = ("acou_uv", "acou_wpi")
+    else:
+        return None
+    avg_us = sum(solo[k] for k in kernels)
+    alg = SUBSTEP_BYTES["mixed"] * N
+    ach = alg / (avg_us * 1e-6)
+    traffic = [pmc_traffic(k, which) for k in kernels]
+    traffic = None if any(t is None for t in traffic) else sum(traffic)
+    in_step = sum(1e3 * rep[k][1] / rep[k][0] for k in kernels)
+    return {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": ach / HBM_PEAK, "traffic": traffic,
+            "kernel": " + ".join(KERNEL_SYMBOL[k] for k in kernels),
+            "avg_launch_us": avg_us, "avg_launch_us_in_step": in_step,
+            "launches_per_step": rep[kernels[0]][0] / steps,
+            "alg_bytes_per_pt": SUBSTEP_BYTES["mixed"], "alg_bytes_per_launch": alg,
+            "traffic_label": "beyond-L2 bytes (FETCH_SIZE + WRITE_SIZE, scaled by a copy "
+                             "probe; Infinity Cache hits included)",
+            "traffic_source": str(PMC_SUMMARY[which].relative_to(ROOT)) if traffic else None}
 
 
 def error_growth(case, checkpoints):
@@ -190,44 +241,59 @@ def error_growth(case, checkpoints):
     return out
 
 
-def lam_benchmark(steps: int, warmup: int):
-    """Seeded synthetic regional case, 330 x 220 x 45 at 12 km (terrain,
-    Lambert map factors, PD moisture, Kessler, Davies boundary relaxation
-    toward the boundary files; dt = 60 s, ns = 6), mixed and FP64."""
-    import shutil
-    import tempfile
-
-    from wfe_amd import Case, Integrator
+def write_regional(d: str, total_steps: int):
+    """Seeded regional case, 330 x 220 x 45 at 12 km, with boundary files
+    every 3 h up to the end of the run (at least one interval)."""
     from wfe_amd.lam_case import write_lam_case
 
-    d = tempfile.mkdtemp(prefix="wfe_lam_")
-    try:
-        case = Case(write_lam_case(d, hours=0))
-        g = case.grid
-        N = g.n_interior
-        dt = case.scalar("dt")
-        out = {"workload": "regional LAM 330x220x45, dx=12 km, terrain 0-3 km, PD moisture, "
-                           "Kessler, 5-point Davies relaxation, dt=60 s, ns=6",
-               "grid": [g.nx, g.ny, g.nz], "steps": steps}
-        for prec, alg in (("mixed", 1664), ("fp64", 3000)):
-            gi = Integrator.from_case(case, prec)
-            for s_ in range(warmup):
-                gi.step(dt, s_ * dt)
-            gi.sync()
-            t0 = time.perf_counter()
-            for s_ in range(warmup, warmup + steps):
-                gi.step(dt, s_ * dt)
-            gi.sync()
-            el = time.perf_counter() - t0
-            out[prec] = {"value": N * steps / el, "ms_per_step": 1e3 * el / steps,
-                         "step_roofline_frac": alg * N * steps / el / HBM_PEAK,
-                         "alg_bytes_per_pt_step": alg,
-                         "acoustic_method": gi.tuning()["chosen"]}
-            del gi
-        out["speedup_mixed_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
-        return out
-    finally:
-        shutil.rmtree(d, ignore_errors=True)
+    hours = LAM_BDY_INTERVAL_H * max(1, math.ceil(total_steps * 60.0 / 3600.0 /
+                                                  LAM_BDY_INTERVAL_H))
+    return write_lam_case(d, hours=hours, bdy_interval_h=LAM_BDY_INTERVAL_H, **LAM_GRID)
+
+
+def rel_errors(g, a_int, b_int, fields=("u", "v", "w", "thp", "pip", "qv", "qc", "qr")):
+    errs = {}
+    for f in fields:
+        nzl = g.nz + 1 if f == "w" else g.nz
+        a = g.interior(a_int.get(f), nzl)
+        b = g.interior(b_int.get(f), nzl)
+        nb = float(np.linalg.norm(b))
+        errs[f] = float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a))
+    return errs
+
+
+def supercell_benchmark(steps: int, warmup: int, error_steps: int, profile_steps: int,
+                        solo_reps: int):
+    """Moist supercell (cases/cfg2_supercell.ini): 120 x 120 x 40, dx = 1 km,
+    WK82 sounding, quarter-circle hodograph, Kessler, dt = 6 s; mixed and
+    FP64, the error growth against FP64 up to error_steps, and the substep
+    roofline (the working set fits the 256 MB Infinity Cache, so this
+    figure is not an HBM measurement)."""
+    from wfe_amd import Case
+
+    case = Case(SUPERCELL)
+    g = case.grid
+    N = g.n_interior
+    gi_m, t_m = timed_run(case, "mixed", warmup, steps)
+    gi_d, t_d = timed_run(case, "fp64", warmup, steps)
+    errs = rel_errors(g, gi_m, gi_d)
+    del gi_m, gi_d
+    rep, tune, solo = kernel_profile(case, "mixed", 5, profile_steps, solo_reps)
+    cps = [c for c in (1, 10, 50, 100, 300, 600, 1200) if c <= max(error_steps, 1)]
+    growth = error_growth(case, cps) if error_steps > 0 else None
+    return {"workload": "moist supercell with Kessler, 120x120x40, dx=1 km, stretched to "
+                        "20 km, dt=6 s, ns=6 (Infinity-Cache resident)",
+            "grid": [g.nx, g.ny, g.nz], "steps": steps,
+            "mixed": {"value": N * steps / t_m, "ms_per_step": 1e3 * t_m / steps},
+            "fp64": {"value": N * steps / t_d, "ms_per_step": 1e3 * t_d / steps},
+            "speedup_mixed_vs_fp64": t_d / t_m,
+            "rel_err_vs_fp64": errs, "error_growth": growth,
+            "step_roofline_frac": ALG_BYTES_STEP["supercell"]["mixed"] * N * steps / t_m
+            / HBM_PEAK,
+            "substep_roofline": substep_roofline(rep, solo, N, profile_steps, "supercell"),
+            "acoustic_method": tune.get("chosen"),
+            "kernels_ms": {k: {"launches": v[0], "total_ms": v[1], "avg_us": 1e3 * v[1] / v[0]}
+                           for k, v in sorted(rep.items(), key=lambda kv: -kv[1][1])}}
 
 
 def ensemble_benchmark(steps: int):
@@ -351,140 +417,134 @@ def adjoint_benchmark(nsteps: int, ckpt: int):
             "adjoint_over_forward": t_adj / t_fwd}
 
 
-def cpu_baseline(case, budget_s: float):
-    """The FP64 oracle on the host cores, on a bounded sample of steps."""
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(case, budget_s: float, one_thread_steps: int):
+    """The FP64 oracle on the host cores, from the initial state, on a
+    bounded sample of steps (at most budget_s seconds), with every host
+    thread OpenMP sees; then one_thread_steps steps on one thread."""
     sys.path.insert(0, str(ROOT / "tests"))
     from oracle_lib import OracleModel, olib
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    olib().orc_set_threads(threads)
-    om = OracleModel.from_case(case)
     dt = case.scalar("dt")
-    n = 0
-    t0 = time.perf_counter()
-    while True:
-        om.step(dt, n * dt)
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or n >= 400:
-            break
-    return n, el, threads
+    N = case.grid.n_interior
+
+    def run(threads, max_steps, budget):
+        olib().orc_set_threads(threads)
+        om = OracleModel.from_case(case)
+        n = 0
+        t0 = time.perf_counter()
+        while True:
+            om.step(dt, n * dt)
+            n += 1
+            el = time.perf_counter() - t0
+            if el >= budget or n >= max_steps:
+                return n, el
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    n, el = run(threads, 400, budget_s)
+    out = {"value": N * n / el, "unit": "grid-pt·steps/s", "cores": threads, "kind": "port",
+           "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+           "sample": f"FP64 oracle (the CPU restatement of the reference path), regional "
+                     f"330x220x45 from the initial state: the first {n} large steps in "
+                     f"{el:.1f} s on {threads} OpenMP threads, rate scaled to "
+                     f"grid-pt·steps/s (the reference itself has no CPU path)"}
+    if one_thread_steps > 0:
+        n1, el1 = run(1, one_thread_steps, 1e9)
+        out["one_thread"] = {"value": N * n1 / el1, "steps": n1, "seconds": el1}
+    olib().orc_set_threads(threads)
+    return out
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=1440, help="timed steps (1440 = 24 h)")
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-one-thread-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile-steps", type=int, default=30)
-    ap.add_argument("--solo-reps", type=int, default=200)
+    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--solo-reps", type=int, default=100)
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the regional LAM and ensemble measurements")
+                    help="skip the supercell, ensemble and adjoint measurements")
     ap.add_argument("--extra-steps", type=int, default=30)
+    ap.add_argument("--supercell-steps", type=int, default=200)
     ap.add_argument("--adjoint-steps", type=int, default=600)
     ap.add_argument("--error-steps", type=int, default=600,
-                    help="last checkpoint of the error-growth study (600 steps = 1 h)")
+                    help="last checkpoint of the supercell error-growth study (600 = 1 h)")
     ap.add_argument("--mixed-only", action="store_true",
-                    help="skip the FP64 run and the error comparison (profiling runs)")
-    ap.add_argument("--lam-only", action="store_true",
-                    help="run only the regional LAM measurement and print its JSON (profiling)")
+                    help="only the timed mixed-precision headline steps (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is described, not measured)"}))
         return 2
-    if args.lam_only:
-        # The copy probe runs first so counter passes have their calibration launch.
-        from wfe_amd.case import copy_bandwidth
-        bw = copy_bandwidth(1 << 30, 20)
-        print(json.dumps({"regional_lam": lam_benchmark(args.extra_steps, 3),
-                          "copy_probe_GBps": bw}), flush=True)
-        return 0
+
+    import shutil
+    import tempfile
 
     from wfe_amd import Case
+    from wfe_amd.case import copy_bandwidth
 
-    case = Case(CASE)
-    g = case.grid
-    N = g.n_interior
     K, W = args.steps, args.warmup
-
-    log(f"[bench] mixed: {W} warmup + {K} timed steps on {g.nx}x{g.ny}x{g.nz}")
-    gi_m, t_m = timed_run(case, "mixed", W, K)
-    log(f"[bench] mixed {t_m:.3f} s -> {1e3 * t_m / K:.3f} ms/step")
-    errs, t_d = None, None
-    if not args.mixed_only:
+    # The copy probe runs first so counter passes have their calibration launch.
+    copy_bw = copy_bandwidth(1 << 30, 20)
+    log(f"[bench] streaming copy probe: {copy_bw:.0f} GB/s")
+    tmp = tempfile.mkdtemp(prefix="wfe_lam_")
+    try:
+        log("[bench] writing the regional case 330x220x45 (boundary files every 3 h)")
+        case = Case(write_regional(tmp, W + K))
+        g = case.grid
+        N = g.n_interior
+        log(f"[bench] mixed: {W} warmup + {K} timed steps on {g.nx}x{g.ny}x{g.nz}")
+        gi_m, t_m = timed_run(case, "mixed", W, K)
+        log(f"[bench] mixed {t_m:.3f} s -> {1e3 * t_m / K:.3f} ms/step")
+        value = N * K / t_m
+        if args.mixed_only:
+            print(json.dumps({"metric": "regional mixed (profiling run)", "value": value,
+                              "ms_per_step": 1e3 * t_m / K, "copy_probe_GBps": copy_bw}))
+            return 0
         log(f"[bench] fp64: {W} warmup + {K} timed steps")
         gi_d, t_d = timed_run(case, "fp64", W, K)
         log(f"[bench] fp64 {t_d:.3f} s -> {1e3 * t_d / K:.3f} ms/step")
-        # Error of the mixed path against the plain FP64 path after W + K steps.
-        errs = {}
-        for f in ("u", "v", "w", "thp", "pip", "qv", "qc", "qr"):
-            nzl = g.nz + 1 if f == "w" else g.nz
-            a = g.interior(gi_m.get(f), nzl)
-            b = g.interior(gi_d.get(f), nzl)
-            nb = float(np.linalg.norm(b))
-            errs[f] = float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a))
-        del gi_d
-    del gi_m
+        errs = rel_errors(g, gi_m, gi_d)
+        tune_m = gi_m.tuning()
+        tune_d = gi_d.tuning()
+        del gi_m, gi_d
+        log(f"[bench] kernel profile: {args.profile_steps} steps")
+        rep, tune, solo = kernel_profile(case, "mixed", 5, args.profile_steps, args.solo_reps)
+        roof = substep_roofline(rep, solo, N, args.profile_steps, "regional")
+        cpu = None
+        if not args.no_cpu:
+            log(f"[bench] cpu baseline: oracle for ~{args.cpu_budget:.0f} s, then "
+                f"{args.cpu_one_thread_steps} steps on one thread")
+            cpu = cpu_baseline(case, args.cpu_budget, args.cpu_one_thread_steps)
+        del case
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
 
-    from wfe_amd.case import copy_bandwidth
-    copy_bw = copy_bandwidth(1 << 30, 20)
-    log(f"[bench] streaming copy probe: {copy_bw:.0f} GB/s")
-    value = N * K / t_m
-    value_fp64 = N * K / t_d if t_d else None
-    log(f"[bench] kernel profile: {args.profile_steps} steps")
-    rep, tune, solo = kernel_profile(case, "mixed", 5, args.profile_steps, args.solo_reps)
-    dom = max(rep, key=lambda k: rep[k][1])
-    cnt, tot_ms = rep[dom]
-    kb = kernel_bytes(dom, "mixed", bool(tune.get("uniform_columns")))
-    roof = None
-    if kb is not None and dom in solo:
-        avg_s = solo[dom] * 1e-6
-        ach = kb * N / avg_s
-        traffic = pmc_traffic(dom)
-        roof = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": ach / HBM_PEAK, "traffic": traffic, "kernel": dom,
-                "avg_launch_us": solo[dom], "avg_launch_us_in_step": 1e3 * tot_ms / cnt,
-                "launches_per_step": cnt / args.profile_steps,
-                "alg_bytes_per_launch": kb * N,
-                "uniform_columns": bool(tune.get("uniform_columns")),
-                "traffic_source": str(PMC_SUMMARY.relative_to(ROOT)) if traffic else None}
-    step_bw = ALG_BYTES_STEP["mixed"] * N * K / t_m
-
-    growth = None
-    if not args.mixed_only:
-        cps = [c for c in (1, 10, 50, 100, 300, 600, 1200) if c <= max(args.error_steps, 1)]
-        log(f"[bench] error growth against FP64 at steps {cps}")
-        growth = error_growth(case, cps)
-
-    lam = ens = None
+    sc = ens = adj = None
     if not args.no_extra:
-        log("[bench] regional LAM 330x220x45")
-        lam = lam_benchmark(args.extra_steps, 3)
-        log(f"[bench] LAM mixed {lam['mixed']['ms_per_step']:.3f} ms/step, "
-            f"fp64 {lam['fp64']['ms_per_step']:.3f} ms/step")
+        log("[bench] moist supercell 120x120x40")
+        sc = supercell_benchmark(args.supercell_steps, 10, args.error_steps,
+                                 args.profile_steps, args.solo_reps)
         log("[bench] uneven ensemble of 16 members")
         ens = ensemble_benchmark(args.extra_steps)
         log(f"[bench] ensemble {ens['ms_per_round']:.2f} ms/round vs sequential "
             f"{ens['sequential_ms_per_round']:.2f} ms/round")
-
-    adj = None
-    if not args.no_extra:
         log(f"[bench] adjoint: dry 120x120x40, {args.adjoint_steps} steps")
         adj = adjoint_benchmark(args.adjoint_steps, 50)
-        log(f"[bench] adjoint {adj['adjoint_s']:.1f} s, dot product {adj['dot_product_rel']:.1e}, "
-            f"FD {adj['gradient_fd_rel_err']}")
 
-    cpu = None
-    if not args.no_cpu:
-        log(f"[bench] cpu baseline: oracle for ~{args.cpu_budget:.0f} s")
-        n, el, thr = cpu_baseline(case, args.cpu_budget)
-        cpu = {"value": N * n / el, "unit": "grid-pt·steps/s", "cores": thr, "kind": "port",
-               "sample": f"FP64 oracle, cfg2 supercell 120x120x40 from the initial state, "
-                         f"{n} large steps in {el:.1f} s on {thr} OpenMP threads"}
-
+    alg = ALG_BYTES_STEP["regional"]
     out = {
         "metric": "grid-pt·steps/s FP32-mixed vs FP64 plain at stated error; % HBM peak",
         "value": value,
@@ -497,26 +557,31 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 (pi' and column solve f64)",
-        "data": "synthetic (analytic WK82 sounding, quarter-circle hodograph, +3 K bubble)",
-        "config": {"workload": "moist supercell with Kessler, 120x120x40, dx=1 km, "
-                               "stretched to 20 km, dt=6 s, ns=6",
+        "data": "synthetic (seeded regional case in the reference's file formats: spectral "
+                "terrain, Lambert map factors, US standard atmosphere with seeded "
+                "perturbations, boundary files every 3 h)",
+        "config": {"workload": "real-shape regional LAM 330x220x45 at 12 km, terrain 0-3 km, "
+                               "map factors 0.95-1.05, PD moisture, Kessler, 5-point Davies "
+                               "relaxation, dt=60 s, ns=6",
                    "grid": [g.nx, g.ny, g.nz], "precision": "mixed"},
-        "fp64_plain": {"value": value_fp64, "ms_per_step": 1e3 * t_d / K,
-                       "speedup_mixed_vs_fp64": value / value_fp64} if t_d else None,
-        "rel_err_vs_fp64": errs,
-        "error_growth": growth,
-        "step_roofline": {"alg_bytes_per_pt_step": ALG_BYTES_STEP["mixed"],
-                          "achieved_GBps": step_bw / 1e9, "frac": step_bw / HBM_PEAK},
+        "fp64_plain": {"value": N * K / t_d, "ms_per_step": 1e3 * t_d / K,
+                       "speedup_mixed_vs_fp64": t_d / t_m},
+        "rel_err_vs_fp64": {"steps": W + K, **errs},
+        "step_roofline": {"alg_bytes_per_pt_step": alg["mixed"],
+                          "achieved_GBps": alg["mixed"] * N * K / t_m / 1e9,
+                          "frac": alg["mixed"] * N * K / t_m / HBM_PEAK,
+                          "fp64_frac": alg["fp64"] * N * K / t_d / HBM_PEAK},
         "roofline": roof,
         "kernels_ms": {k: {"launches": v[0], "total_ms": v[1], "avg_us": 1e3 * v[1] / v[0]}
                        for k, v in sorted(rep.items(), key=lambda kv: -kv[1][1])},
         "kernel_solo_us": solo,
-        "regional_lam": lam,
-        "ensemble": ens,
-        "adjoint": adj,
-        "acoustic_tuning": tune,
+        "acoustic_tuning": {"mixed": tune_m.get("chosen"), "fp64": tune_d.get("chosen"),
+                            "candidates_ms": tune.get("candidates_ms")},
         "copy_probe_GBps": copy_bw,
         "cpu_baseline": cpu,
+        "supercell": sc,
+        "ensemble": ens,
+        "adjoint": adj,
     }
     print(json.dumps(out))
     return 0

@@ -1,0 +1,300 @@
+"""Every benchmark configuration under the GPU tests, at its real shape and,
+where the oracle is fast enough, against the oracle built by its own case
+assembly (oracle/case_builder.cpp) -- so the product's host builders, file
+reader and boundary manager are inside the comparison too.
+
+  cfg 1  dry warm bubble 81 x 4 x 41, FP64, all 600 steps vs the oracle;
+  cfg 2  moist supercell 120 x 120 x 40, mixed, all 1200 steps: finite,
+         |w| below the guard, moisture positive; FP64 vs mixed error;
+  cfg 3  regional 330 x 220 x 45, mixed, all 1440 steps (24 h) with boundary
+         files every 3 h: finite, |w| guard, PD positivity; a small regional
+         case crossing two boundary files against the oracle;
+  cfg 4  an uneven ensemble of seeded regional members at two grid sizes,
+         advanced together, each member against its own oracle;
+  cfg 5  adjoint at the 120 x 120 x 40 supercell shape (dot product, finite
+         differences) and a reference gradient from finite differences of
+         the FP64 oracle on a small grid.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from helpers import CASES, kitchen_sink, oracle_and_gpu, perturbed_state, rel_err
+from oracle_lib import OracleCase
+
+pytestmark = pytest.mark.gpu
+
+STATE = ("u", "v", "w", "thp", "pip", "qv", "qc", "qr")
+
+
+def _errs(g, a, b, fields):
+    out = {}
+    for f in fields:
+        nzl = g.nz + 1 if f == "w" else g.nz
+        out[f] = rel_err(g.interior(a.get(f), nzl), g.interior(b.get(f), nzl))
+    return out
+
+
+def _guards(gi, g, w_abort, pd=True):
+    d = gi.diagnose()
+    print("diag", {k: (f"{v:.4g}" if isinstance(v, float) else v) for k, v in d.items()},
+          flush=True)
+    assert d["finite"] == 1
+    assert d["wmax"] < w_abort
+    if pd:
+        qv = g.interior(gi.get("qv"), g.nz)
+        assert qv.min() >= -1e-6, qv.min()
+    return d
+
+
+def test_cfg1_warm_bubble_all_600_steps_match_the_oracle():
+    from wfe_amd import Case, Integrator
+    ini = CASES / "cfg1_dry_bubble.ini"
+    c, oc = Case(ini), OracleCase(ini)
+    om = oc.model()
+    gi = Integrator.from_case(c, "fp64")
+    dt = c.scalar("dt")
+    nsteps = int(round(c.scalar("t_end") / dt))
+    assert nsteps == 600
+    for s in range(nsteps):
+        gi.step(dt, s * dt)
+        om.step(dt, s * dt)
+    e = _errs(c.grid, gi, om, ("u", "v", "w", "thp", "pip"))
+    print("cfg1 600 steps", {k: f"{v:.2e}" for k, v in e.items()}, flush=True)
+    assert all(v <= 1e-10 for v in e.values()), e
+    # the bubble has risen: w_max of the reference's gate range
+    assert 5.0 < gi.diagnose()["wmax"] < 30.0
+
+
+def test_cfg2_supercell_all_1200_steps_mixed_stay_bounded():
+    from wfe_amd import Case, Integrator
+    c = Case(CASES / "cfg2_supercell.ini")
+    g = c.grid
+    dt = c.scalar("dt")
+    mix = Integrator.from_case(c, "mixed")
+    ref = Integrator.from_case(c, "fp64")
+    for s in range(1200):
+        mix.step(dt, s * dt)
+        ref.step(dt, s * dt)
+        if s + 1 in (600, 1200):
+            e = _errs(g, mix, ref, ("u", "v", "w", "thp"))
+            print(f"cfg2 step {s + 1} mixed vs fp64", {k: f"{v:.2e}" for k, v in e.items()},
+                  flush=True)
+            d = _guards(mix, g, c.params.w_abort, pd=False)
+            assert d["qrmax"] > 1e-3  # the storm rains
+    qv = g.interior(mix.get("qv"), g.nz)
+    assert qv.min() > -1e-4  # no PD limiter in this case: only small undershoots
+
+
+def test_cfg3_regional_all_1440_steps_mixed_with_boundary_files(tmp_path):
+    from wfe_amd import Case, Integrator
+    from wfe_amd.lam_case import write_lam_case
+    ini = write_lam_case(tmp_path / "lam", hours=24, bdy_interval_h=3)
+    c = Case(ini)
+    g = c.grid
+    assert (g.nx, g.ny, g.nz) == (330, 220, 45)
+    dt = c.scalar("dt")
+    gi = Integrator.from_case(c, "mixed")
+    lo0 = c.array("bdy_hi_thp").copy()
+    for s in range(1440):
+        gi.step(dt, s * dt)
+        if (s + 1) % 360 == 0:
+            print(f"cfg3 step {s + 1}", flush=True)
+            _guards(gi, g, c.params.w_abort)
+    # the product's boundary manager advanced through the files (the targets
+    # it holds now differ from the ones it started with)
+    assert not np.array_equal(c.array("bdy_hi_thp"), lo0)
+
+
+def test_regional_boundary_files_and_time_interpolation_match_the_oracle(tmp_path):
+    """150 steps of 60 s cross the 1 h and 2 h boundary files: the product's
+    file reader, target conversion, lo/hi swap and time fraction against the
+    oracle's own restatement of BdyManager (boundary.cpp:9-97)."""
+    from wfe_amd import Case, Integrator
+    from wfe_amd.lam_case import write_lam_case
+    ini = write_lam_case(tmp_path / "lam", nx=40, ny=30, nz=20, dx=12000.0, top=15000.0,
+                         seed=5, hours=3, bdy_interval_h=1, bdy_shift_cells_per_h=3.0)
+    c, oc = Case(ini), OracleCase(ini)
+    g = c.grid
+    dt = c.scalar("dt")
+    om = oc.model()
+    gd = Integrator.from_case(c, "fp64")
+    gm = Integrator.from_case(c, "mixed")
+    for s in range(150):
+        for m in (om, gd, gm):
+            m.step(dt, s * dt)
+    e = _errs(g, gd, om, STATE)
+    print("regional fp64 vs oracle", {k: f"{v:.2e}" for k, v in e.items()}, flush=True)
+    assert all(v <= 1e-10 for v in e.values()), e
+    em = _errs(g, gm, om, ("u", "v", "w", "thp", "pip", "qv"))
+    print("regional mixed vs oracle", {k: f"{v:.2e}" for k, v in em.items()}, flush=True)
+    assert all(v <= 1e-3 for v in em.values()), em
+    for f in ("u", "thp"):
+        assert np.array_equal(c.array(f"bdy_hi_{f}"), oc.array(f"bdy_hi_{f}"))
+
+
+def test_cfg4_uneven_ensemble_members_match_their_oracles(tmp_path):
+    """Seeded regional members at two grid sizes and time steps (12 km, 60 s
+    and 3 km, 15 s) with member theta' noise, advanced together by one graph
+    launch per ensemble step; every member against its own oracle."""
+    from wfe_amd import Case, Ensemble, Integrator
+    from wfe_amd.lam_case import write_lam_case
+    specs = [dict(nx=40, ny=30, nz=20, dx=12000.0, top=15000.0, dt=60.0)] * 2 + \
+            [dict(nx=32, ny=32, nz=16, dx=3000.0, top=12000.0, dt=15.0, h_max=800.0)] * 2
+    cases, oracles, members, dts = [], [], [], []
+    for q, sp in enumerate(specs):
+        ini = write_lam_case(tmp_path / f"m{q}", seed=20261202 + q, hours=0, low_noise_K=0.5,
+                             **sp)
+        c = Case(ini)
+        cases.append(c)
+        oracles.append(OracleCase(ini).model())
+        members.append(Integrator.from_case(c, "fp64"))
+        dts.append(c.scalar("dt"))
+    ens = Ensemble(members)
+    nsteps = 8
+    for s in range(nsteps):
+        ens.step(dts)
+        for om, dt in zip(oracles, dts):
+            om.step(dt, s * dt)
+    ens.sync()
+    for q, (c, gi, om) in enumerate(zip(cases, members, oracles)):
+        e = _errs(c.grid, gi, om, STATE)
+        print(f"member {q}", {k: f"{v:.2e}" for k, v in e.items()}, flush=True)
+        assert all(v <= 1e-10 for v in e.values()), (q, e)
+    # members differ from each other (own seeds and noise)
+    a = cases[0].grid.interior(members[0].get("thp"))
+    b = cases[1].grid.interior(members[1].get("thp"))
+    assert rel_err(a, b) > 1e-3
+
+
+def _box(g, rng, size_cells):
+    from wfe_amd.adjoint import box_weights
+    i0, j0 = (int(x) for x in rng.integers(0, min(g.nx, g.ny) - size_cells, size=2))
+    return box_weights(g, i0, i0 + size_cells, j0, j0 + size_cells)
+
+
+def test_cfg5_adjoint_at_the_supercell_shape():
+    """Dot-product test and finite differences of the gradient at 120 x 120 x
+    40 with the supercell's levels: 20 steps with a checkpoint every 5."""
+    from wfe_amd import Case, Integrator
+    from wfe_amd.adjoint import FIELDS5, ad_step, ke_gradient, tl_step
+    c = Case(CASES / "cfg5_adjoint_dry.ini")
+    g = c.grid
+    assert (g.nx, g.ny, g.nz) == (120, 120, 40)
+    dt = c.scalar("dt")
+    gi = Integrator.from_case(c, "fp64")
+    rng = np.random.default_rng(20261202)
+    mask = np.zeros((g.NZ, g.NY, g.NX), dtype=bool)
+    mask[g.ng:g.ng + g.nz, g.ng:g.ng + g.ny, g.ng:g.ng + g.nx] = True
+    mask = mask.ravel()
+    scale = {"u": 1.0, "v": 1.0, "w": 0.3, "thp": 0.5, "pip": 1e-4}
+
+    def rand():
+        return {f: np.where(mask, rng.standard_normal(g.npts) * scale[f], 0.0) for f in FIELDS5}
+
+    dx, yb = rand(), rand()
+    dy = tl_step(gi, dt, dx)
+    xb = ad_step(gi, dt, yb)
+    lhs = sum(float(np.dot(dy[f][mask], yb[f][mask])) for f in FIELDS5)
+    rhs = sum(float(np.dot(dx[f][mask], xb[f][mask])) for f in FIELDS5)
+    print("cfg5 dot product", lhs, rhs, abs(lhs - rhs) / abs(lhs), flush=True)
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+
+    w = _box(g, rng, 20)
+    nsteps = 20
+    J, grad = ke_gradient(gi, dt, nsteps, 5, w)
+    x0 = {f: gi.get(f) for f in FIELDS5}
+    d = rand()
+    gd = sum(float(np.dot(grad[f][mask], d[f][mask])) for f in FIELDS5)
+
+    def J_of(xs):
+        gp = Integrator.from_case(c, "fp64")
+        for f in FIELDS5:
+            gp.set(f, xs[f])
+        tot, ww = 0.0, w.reshape(g.NY, g.NX)
+        for _ in range(nsteps):
+            gp.step(dt)
+            u = gp.get("u").reshape(g.NZ, g.NY, g.NX)[g.ng]
+            v = gp.get("v").reshape(g.NZ, g.NY, g.NX)[g.ng]
+            tot += float((ww * 0.5 * (u * u + v * v)).sum()) / nsteps
+        return tot
+
+    assert J == pytest.approx(J_of(x0), rel=1e-12)
+    eps = 1e-6
+    fd = (J_of({f: x0[f] + eps * d[f] for f in FIELDS5}) -
+          J_of({f: x0[f] - eps * d[f] for f in FIELDS5})) / (2 * eps)
+    print("cfg5 gradient vs FD", gd, fd, abs(fd - gd) / abs(gd), flush=True)
+    assert abs(fd - gd) <= 1e-5 * abs(gd)
+
+
+def test_adjoint_gradient_against_finite_differences_of_the_oracle():
+    """Reference gradient: the objective evaluated by the FP64 oracle (its own
+    case assembly and step) at x0 +- eps d, against the hand-written adjoint's
+    gradient along d, on a small dry periodic grid."""
+    from wfe_amd import Case, Integrator
+    from wfe_amd.adjoint import FIELDS5, box_weights, ke_gradient
+    over = dict(nx=24, ny=20, nz=16, dx=400, dy=400, dz=300, bubble_xc=4800, bubble_yc=4000,
+                bubble_zc=1500, bubble_xr=1500, bubble_yr=1500, bubble_zr=1200, wind_u=6,
+                rayleigh_zd=3000, rayleigh_alpha=0.02, dt=2.0, acoustic_ns=4)
+    ini = CASES / "warm_bubble.ini"
+    c = Case(ini, **over)
+    g = c.grid
+    dt = c.scalar("dt")
+    st = perturbed_state(c, 3)
+    gi = Integrator.from_case(c, "fp64")
+    for f in FIELDS5:
+        gi.set(f, st[f])
+    w = box_weights(g, 6, 14, 5, 13)
+    nsteps = 6
+    J, grad = ke_gradient(gi, dt, nsteps, 2, w)
+    rng = np.random.default_rng(7)
+    mask = np.zeros((g.NZ, g.NY, g.NX), dtype=bool)
+    mask[g.ng:g.ng + g.nz, g.ng:g.ng + g.ny, g.ng:g.ng + g.nx] = True
+    mask = mask.ravel()
+    scale = {"u": 1.0, "v": 1.0, "w": 0.3, "thp": 0.5, "pip": 1e-4}
+    d = {f: np.where(mask, rng.standard_normal(g.npts) * scale[f], 0.0) for f in FIELDS5}
+    gd = sum(float(np.dot(grad[f][mask], d[f][mask])) for f in FIELDS5)
+
+    def J_oracle(eps):
+        om = OracleCase(ini, **over).model()
+        for f in FIELDS5:
+            om.set(f, st[f] + eps * d[f])
+        tot, ww = 0.0, w.reshape(g.NY, g.NX)
+        for s in range(nsteps):
+            om.step(dt, s * dt)
+            u = om.get("u").reshape(g.NZ, g.NY, g.NX)[g.ng]
+            v = om.get("v").reshape(g.NZ, g.NY, g.NX)[g.ng]
+            tot += float((ww * 0.5 * (u * u + v * v)).sum()) / nsteps
+        return tot
+
+    assert J == pytest.approx(J_oracle(0.0), rel=1e-12)
+    for eps, tol in ((1e-4, 1e-4), (1e-6, 1e-6)):
+        fd = (J_oracle(eps) - J_oracle(-eps)) / (2 * eps)
+        print(f"oracle FD eps={eps:g}: {fd!r} adjoint {gd!r} rel {abs(fd - gd) / abs(gd):.2e}",
+              flush=True)
+        assert abs(fd - gd) <= tol * abs(gd)
+
+
+SLOT_LIMITS = [("fused-64x14", 42), ("fused-64x10x4", 40), ("fused-64x16", 48),
+               ("fused-64x8", 48), ("fused-32x16", 48), ("fused-32x8", 48),
+               ("fused-32x16w4", 48), ("fused-32x16x4", 64), ("split-64x16", 48),
+               ("split-32x16x4", 64), ("split-32x16w4", 48)]
+
+
+@pytest.mark.parametrize("shape,nz", SLOT_LIMITS, ids=[f"{s}-nz{n}" for s, n in SLOT_LIMITS])
+def test_acoustic_shapes_at_their_exact_level_limit(monkeypatch, shape, nz):
+    """Each substep tile at the deepest column it takes (every level slot
+    holds a level), pinned through WFE_ACOUSTIC_KERNEL, FP64 vs the oracle."""
+    monkeypatch.setenv("WFE_ACOUSTIC_KERNEL", shape)
+    c = kitchen_sink(nx=21, ny=13, nz=nz, dz=round(10800.0 / nz, 3), terrain="none")
+    st = perturbed_state(c, 41)
+    om, gi = oracle_and_gpu(c, "fp64", st)
+    dt = c.scalar("dt")
+    for s in range(2):
+        om.step(dt)
+        gi.step(dt)
+    e = _errs(c.grid, gi, om, STATE)
+    print(shape, nz, {k: f"{v:.1e}" for k, v in e.items()}, flush=True)
+    assert all(v <= 1e-10 for v in e.values()), e
+    assert f"tuned:{shape}" in gi.kernel_report()

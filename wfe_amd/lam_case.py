@@ -28,8 +28,10 @@ The state is synthetic and seeded:
   pi        hydrostatic per column from the perturbed virtual temperature;
   members   optional white theta noise N(0, low_noise_K) below low_noise_top
             above ground (ensemble members);
-  boundary  the same seeded state at every boundary time (relaxation toward
-            a steady target).
+  boundary  the initial state at hour 0; at later boundary times the same
+            seeded state with its perturbation fields carried east by
+            bdy_shift_cells_per_h grid cells per hour (pi recomputed
+            hydrostatically), so the relaxation target moves in time.
 """
 from __future__ import annotations
 
@@ -120,6 +122,7 @@ def write_lam_case(out_dir, *, nx=330, ny=220, nz=45, dx=12000.0, seed=20261202,
                    sigma_theta=1.0, sigma_u=3.0, lat0=37.0, proj_lat1=31.0, proj_lat2=67.0,
                    dt=60.0, acoustic_ns=6, hours=24, bdy_interval_h=24, bdy_width=5,
                    bdy_tau=600.0, low_noise_K=0.0, low_noise_top=2000.0,
+                   bdy_shift_cells_per_h=2.0,
                    extra: dict | None = None) -> Path:
     """Write the input files and a case .ini into out_dir; returns the .ini."""
     out = Path(out_dir)
@@ -165,26 +168,41 @@ def write_lam_case(out_dir, *, nx=330, ny=220, nz=45, dx=12000.0, seed=20261202,
     # 3-D state: the mean state at the cell heights plus seeded smooth noise
     T3, p3 = us_standard(z3)
     sig = (2.0, 6.0, 6.0)  # grid cells (k, j, i)
-    th3 = np.interp(z3, ztab, th_tab) + sigma_theta * _smooth_field(rng, z3.shape, None, sig)
-    rh = np.clip(rh_mean(z3) + 0.12 * _smooth_field(rng, z3.shape, None, sig), 0.4, 0.9)
-    qv3 = np.where(z3 > 15000, 0.0, np.minimum(rh * qsat(p3, T3), 0.018))
-    u3 = np.interp(z3, ztab, u_tab) + sigma_u * _smooth_field(rng, z3.shape, None, sig)
-    v3 = sigma_u * _smooth_field(rng, z3.shape, None, sig)
+    n_th = _smooth_field(rng, z3.shape, None, sig)
+    n_rh = _smooth_field(rng, z3.shape, None, sig)
+    n_u = _smooth_field(rng, z3.shape, None, sig)
+    n_v = _smooth_field(rng, z3.shape, None, sig)
+    th_mean = np.interp(z3, ztab, th_tab)
+    u_mean = np.interp(z3, ztab, u_tab)
+
+    def state(shift: int, member_noise=None):
+        """(u, v, theta, pi, qv) with the noise fields rolled `shift` cells
+        east (they are periodic, so the roll stays smooth)."""
+        roll = (lambda a: np.roll(a, shift, axis=2)) if shift else (lambda a: a)
+        th3 = th_mean + sigma_theta * roll(n_th)
+        rh = np.clip(rh_mean(z3) + 0.12 * roll(n_rh), 0.4, 0.9)
+        qv3 = np.where(z3 > 15000, 0.0, np.minimum(rh * qsat(p3, T3), 0.018))
+        u3 = u_mean + sigma_u * roll(n_u)
+        v3 = sigma_u * roll(n_v)
+        if member_noise is not None:
+            th3 = th3 + member_noise
+        # pi per column: hydrostatic with the column's own theta_v, from the
+        # mean state's value at the terrain height
+        thv3 = th3 * (1 + 0.61 * qv3)
+        pi3 = np.empty_like(th3)
+        pi3[0] = np.interp(h, zf, pib_f) - GRAV * (z3[0] - h) / (CP * thv3[0])
+        for k in range(1, nz):
+            thm = 0.5 * (thv3[k] + thv3[k - 1])
+            pi3[k] = pi3[k - 1] - GRAV * (z3[k] - z3[k - 1]) / (CP * thm)
+        return u3, v3, th3, pi3, qv3
 
     # ensemble member perturbation: white theta noise N(0, low_noise_K) in
-    # the lowest low_noise_top metres
+    # the lowest low_noise_top metres (initial state only)
+    member = None
     if low_noise_K > 0:
-        th3 = th3 + np.where(z3 - h[None] < low_noise_top,
-                             rng.normal(0.0, low_noise_K, z3.shape), 0.0)
-
-    # pi per column: hydrostatic with the column's own theta_v, from the mean
-    # state's value at the terrain height
-    thv3 = th3 * (1 + 0.61 * qv3)
-    pi3 = np.empty_like(th3)
-    pi3[0] = np.interp(h, zf, pib_f) - GRAV * (z3[0] - h) / (CP * thv3[0])
-    for k in range(1, nz):
-        thm = 0.5 * (thv3[k] + thv3[k - 1])
-        pi3[k] = pi3[k - 1] - GRAV * (z3[k] - z3[k - 1]) / (CP * thm)
+        member = np.where(z3 - h[None] < low_noise_top,
+                          rng.normal(0.0, low_noise_K, z3.shape), 0.0)
+    u3, v3, th3, pi3, qv3 = state(0, member)
 
     f32 = np.float32
     with open(out / "wfe_init.bin", "wb") as f:
@@ -194,10 +212,15 @@ def write_lam_case(out_dir, *, nx=330, ny=220, nz=45, dx=12000.0, seed=20261202,
             a.astype(f32).tofile(f)
         for a in (u3, v3, th3, pi3, qv3):
             a.astype(f32).tofile(f)
-    bdy = np.concatenate([a.ravel() for a in (u3, v3, th3, pi3, qv3)]).astype(f32)
+    # boundary files: the initial state at hour 0, then the same state with
+    # its perturbations carried east by bdy_shift_cells_per_h cells per hour
     n_bdy = hours // bdy_interval_h + 1
     for b in range(n_bdy):
-        bdy.tofile(out / f"wfe_bdy_{b * bdy_interval_h:03d}.bin")
+        hr = b * bdy_interval_h
+        fields = (u3, v3, th3, pi3, qv3) if b == 0 else \
+            state(int(round(hr * bdy_shift_cells_per_h)))
+        np.concatenate([a.ravel() for a in fields]).astype(f32).tofile(
+            out / f"wfe_bdy_{hr:03d}.bin")
     meta = [f"version = 3", f"nx = {nx}", f"ny = {ny}", f"nz = {nz}", f"np_prof = {npf}",
             f"start = 2026120200", f"bdy_interval = {bdy_interval_h * 3600}",
             f"n_bdy = {n_bdy}", f"hours = {hours}"]
