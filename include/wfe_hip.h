@@ -294,7 +294,9 @@ int wfe_case_array(const wfe_case* c, const char* name, double* out, size_t n);
 /* Integrator::init + IC upload (+ BdyManager::init in file mode). */
 int wfe_case_create_ctx(const wfe_case* c, int precision, wfe_ctx** out);
 /* BdyManager::update(t) then tfrac(t) for the context (boundary.cpp:83-97),
- * reading further boundary files when t passes the next boundary time. */
+ * reading further boundary files when t passes the next boundary time. Each
+ * context created from the case keeps its own bracket of boundary times, so
+ * one case can drive several contexts that stand at different times. */
 int wfe_case_update_bdy(wfe_case* c, wfe_ctx* ctx, double t);
 /* Unused-key list (Config::unused, config.cpp:51-60), newline-separated. */
 int wfe_case_unused_keys(const wfe_case* c, char* buf, size_t n);

@@ -89,3 +89,30 @@ def test_driver_w_guard_exits_3_with_emergency_output(tmp_path):
     assert r.returncode == 3, (r.stdout[-1000:], r.stderr[-1000:])
     assert "exceeded the limit" in r.stderr
     assert (tmp_path / "w_000010.bin").stat().st_size == 4 * 24 * 20 * 19
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_integrator_init_path_matches_the_case_path(precision):
+    """wfe::Integrator::init (grid, base-state and metric views, DynParams;
+    the reference's init, integrator.hpp:18-42) with the initial state
+    uploaded through state().<field>.upload steps bit-identically to the
+    context the case builder creates in one call (terrain, open boundaries,
+    moisture, PD, Kessler)."""
+    exe = ROOT / "wfe_amd" / "lib" / "wfe_integrator_check"
+    args = [SUPERCELL, "steps=3", f"precision={precision}", "nx=24", "ny=20", "nz=18",
+            "dx=2000", "dy=2000", "dz=600", "dt=6", "terrain=gauss", "mtn_h0=600",
+            "mtn_a=8000", "mtn_xc=24000", "mtn_yc=20000", "pd_moist=on", "coriolis_f=1e-4",
+            "bubble_xc=24000", "bubble_yc=20000", "bubble_xr=8000", "bubble_yr=8000"]
+    r = subprocess.run([str(exe), *map(str, args)], capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr[-2000:], flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "identical" in r.stdout
+
+
+def test_integrator_check_rejects_file_mode(tmp_path):
+    from wfe_amd.lam_case import write_lam_case
+    ini = write_lam_case(tmp_path / "lam", nx=12, ny=10, nz=8, dx=12000.0, top=5000.0, hours=0)
+    exe = ROOT / "wfe_amd" / "lib" / "wfe_integrator_check"
+    r = subprocess.run([str(exe), str(ini)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and "file-mode" in r.stderr

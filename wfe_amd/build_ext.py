@@ -40,7 +40,7 @@ HOST_SOURCES = [
     "core/boundary_host.cpp", "core/case.cpp",
     "io/input.cpp", "io/writer.cpp", "capi/case_api.cpp", "dynamics/integrator.cpp",
 ]
-APP_SOURCES = ["apps/wfe_main.cpp"]
+APP_SOURCES = ["apps/wfe_main.cpp", "apps/integrator_check.cpp"]
 
 
 def _run(cmd: list[str]) -> None:
@@ -82,9 +82,10 @@ def build(verbose: bool = False) -> dict:
           "-o", str(lib) + ".tmp"])
     Path(str(lib) + ".tmp").replace(lib)
     exe = LIBDIR / "wfe"
-    _run([CXX, str(_obj(APP_SOURCES[0])), "-o", str(exe) + ".tmp", f"-L{LIBDIR}", "-lwfe",
-          "-Wl,-rpath,$ORIGIN", "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"])
-    Path(str(exe) + ".tmp").replace(exe)
+    for src, out in ((APP_SOURCES[0], exe), (APP_SOURCES[1], LIBDIR / "wfe_integrator_check")):
+        _run([CXX, str(_obj(src)), "-o", str(out) + ".tmp", f"-L{LIBDIR}", "-lwfe",
+              "-Wl,-rpath,$ORIGIN", "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"])
+        Path(str(out) + ".tmp").replace(out)
     if verbose:
         print(f"built {lib}, {exe}, {ORACLE / 'liboracle.so'}")
     return {"lib": str(lib), "exe": str(exe), "oracle": str(ORACLE / "liboracle.so")}

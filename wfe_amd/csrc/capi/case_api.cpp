@@ -1,8 +1,11 @@
 // C ABI of the host case builder (include/wfe_hip.h, wfe_case_*): the
 // Config + Metric + BaseState + initial-condition assembly of
 // src/main.cpp:248-383, exposed for the driver, the Python layer and tests.
+#include <algorithm>
+#include <array>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -15,8 +18,18 @@ namespace wfe {
 void set_last_error(const std::string& s);
 }
 
+// A case and, per context it drives, that context's boundary bracket (the
+// BdyManager state of src/dynamics/boundary.cpp:46-48,83-91), so one case
+// can drive several contexts that stand at different times. Converted
+// boundary files are kept for the two most recent file indices.
 struct wfe_case {
   wfe::Case c;
+  struct Bracket {
+    int idx_hi = 1;
+    double t_lo = 0, t_hi = 0;
+  };
+  std::map<const wfe_ctx*, Bracket> brackets;
+  std::map<int, std::array<std::vector<double>, 5>> files;
 };
 
 namespace {
@@ -183,6 +196,12 @@ int wfe_case_create_ctx(const wfe_case* cs, int precision, wfe_ctx** out) {
   if (rc != WFE_OK) {
     wfe_destroy(*out);
     *out = nullptr;
+    return rc;
+  }
+  if (c.bdy.active) {
+    wfe_case::Bracket b;
+    b.t_hi = c.input.bdy_interval;
+    const_cast<wfe_case*>(cs)->brackets[*out] = b;
   }
   return rc;
 }
@@ -191,26 +210,48 @@ int wfe_case_update_bdy(wfe_case* cs, wfe_ctx* ctx, double t) {
   if (!cs || !ctx) return WFE_E_ARG;
   wfe::Case& c = cs->c;
   if (!c.bdy.active) return WFE_OK;
-  bool ok = true;
-  int rc = guarded_case([&] {
-    c.bdy.swapped = false;
-    c.bdy.dirty_hi = false;
-    ok = c.bdy.update(c.g, c.input, c.base.thb, c.base.pib, t);
-  });
-  if (rc != WFE_OK) return rc;
-  if (!ok) {
-    wfe::set_last_error("cannot read the next boundary file");
-    return WFE_E_IO;
-  }
-  if (c.bdy.dirty_hi) {
-    // The host lo/hi now hold the advanced pair; mirror it on the device.
-    for (int v = 0; v < 5 && rc == WFE_OK; ++v) {
-      rc = wfe_set_bdy_target(ctx, 0, v, c.bdy.lo[v].data(), c.bdy.lo[v].size());
-      if (rc == WFE_OK) rc = wfe_set_bdy_target(ctx, 1, v, c.bdy.hi[v].data(), c.bdy.hi[v].size());
+  // this context's bracket; a context first seen here holds the targets of
+  // wfe_case_create_ctx (the first two boundary times)
+  auto ins = cs->brackets.emplace(ctx, wfe_case::Bracket{});
+  wfe_case::Bracket& b = ins.first->second;
+  if (ins.second) b.t_hi = c.input.bdy_interval;
+  int rc = WFE_OK;
+  // BdyManager::update (boundary.cpp:83-91): while t has passed the upper
+  // time and a later file exists, the upper target becomes the lower one
+  // (a buffer swap on the device) and the next file is uploaded as the upper
+  while (t >= b.t_hi && b.idx_hi + 1 < c.input.n_bdy) {
+    const int next = b.idx_hi + 1;
+    auto f = cs->files.find(next);
+    if (f == cs->files.end()) {
+      std::array<std::vector<double>, 5> tg;
+      bool ok = false;
+      rc = guarded_case([&] { ok = c.bdy.load_into(c.g, c.input, c.base.thb, c.base.pib, next, tg); });
+      if (rc != WFE_OK) return rc;
+      if (!ok) {
+        wfe::set_last_error("cannot read boundary file " + std::to_string(next));
+        return WFE_E_IO;
+      }
+      while (cs->files.size() >= 2) cs->files.erase(cs->files.begin());
+      f = cs->files.emplace(next, std::move(tg)).first;
     }
+    rc = wfe_swap_bdy_targets(ctx);
+    for (int v = 0; v < 5 && rc == WFE_OK; ++v)
+      rc = wfe_set_bdy_target(ctx, 1, v, f->second[v].data(), f->second[v].size());
     if (rc != WFE_OK) return rc;
+    // the host copy shows the pair of the most recently advanced context
+    c.bdy.lo.swap(c.bdy.hi);
+    c.bdy.hi = f->second;
+    c.bdy.idx_hi = next;
+    b.idx_hi = next;
+    b.t_lo = b.t_hi;
+    b.t_hi += c.input.bdy_interval;
+    c.bdy.t_lo = b.t_lo;
+    c.bdy.t_hi = b.t_hi;
   }
-  return wfe_set_bdy_tfrac(ctx, c.bdy.tfrac(t));
+  // BdyManager::tfrac (boundary.cpp:93-97)
+  double tf = 0;
+  if (b.t_hi > b.t_lo) tf = std::clamp((t - b.t_lo) / (b.t_hi - b.t_lo), 0.0, 1.0);
+  return wfe_set_bdy_tfrac(ctx, tf);
 }
 
 int wfe_case_unused_keys(const wfe_case* c, char* buf, size_t n) {

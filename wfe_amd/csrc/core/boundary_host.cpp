@@ -1,7 +1,7 @@
 // Host side of the lateral boundary forcing (BdyManager,
-// src/dynamics/boundary.cpp:9-97): Davies rim weights, the two bracketing
-// target states read from the boundary files, and the time fraction between
-// them.
+// src/dynamics/boundary.cpp:9-81): Davies rim weights and the target states
+// read from the boundary files. The per-context bracket and time fraction
+// live with the case handle (wfe_case_update_bdy).
 #include <algorithm>
 #include <cmath>
 #include <stdexcept>
@@ -60,27 +60,6 @@ void BoundaryHost::init(const GDims& g, const DynParams& dp, const InputData& in
   t_lo = 0;
   t_hi = in.bdy_interval;
   idx_hi = 1;
-}
-
-bool BoundaryHost::update(const GDims& g, const InputData& in, const std::vector<double>& thb3,
-                          const std::vector<double>& pib3, double t) {
-  // step the bracket forward past t while later files exist: the upper
-  // target becomes the lower one and the next file is read as the upper
-  while (t >= t_hi && idx_hi + 1 < in.n_bdy) {
-    std::swap(lo, hi);
-    swapped = !swapped;
-    ++idx_hi;
-    if (!load_into(g, in, thb3, pib3, idx_hi, hi)) return false;
-    dirty_hi = true;
-    t_lo = t_hi;
-    t_hi += in.bdy_interval;
-  }
-  return true;
-}
-
-double BoundaryHost::tfrac(double t) const {
-  if (t_hi <= t_lo) return 0;
-  return std::clamp((t - t_lo) / (t_hi - t_lo), 0.0, 1.0);
 }
 
 }  // namespace wfe

@@ -14,29 +14,24 @@
 namespace wfe {
 
 // Host side of BdyManager (src/dynamics/boundary.hpp:16-38,
-// src/dynamics/boundary.cpp:9-97 of the reference): Davies rim weights
+// src/dynamics/boundary.cpp:9-81 of the reference): Davies rim weights
 // (cos^2 over bdy_width cells, 1/bdy_tau at the edge, optional interior
-// nudging 1/nudge_tau) and the two relaxation target levels for u, v,
-// theta', pi', qv, linearly interpolated in time. The device copy of the
-// targets lives in the integrator context; this object reads the files and
-// converts them to padded perturbation fields.
+// nudging 1/nudge_tau) and the relaxation targets for u, v, theta', pi', qv
+// read from the boundary files as padded perturbation fields. The pair
+// bracketing a context's time and the time fraction are advanced per context
+// by wfe_case_update_bdy (boundary.cpp:83-97); lo/hi and t_lo/t_hi here hold
+// the first pair, then the pair of the most recently advanced context.
 struct BoundaryHost {
   bool active = false;
-  std::vector<double> wgt;               // [NX*NY]
+  std::vector<double> wgt;                    // [NX*NY]
   std::array<std::vector<double>, 5> lo, hi;  // padded 3-D targets
   double t_lo = 0, t_hi = 0;
   int idx_hi = 1;
-  bool dirty_hi = false;  // hi reloaded after a lo<-hi swap, not yet uploaded
-  bool swapped = false;
 
   void init(const GDims& g, const DynParams& dp, const InputData& in,
             const std::vector<double>& thb3, const std::vector<double>& pib3, int bdy_width,
             double bdy_tau, double nudge_tau);
-  // Advances the bracketing boundary times past t; returns false on a file
-  // error. Marks swapped/dirty_hi for the device upload.
-  bool update(const GDims& g, const InputData& in, const std::vector<double>& thb3,
-              const std::vector<double>& pib3, double t);
-  double tfrac(double t) const;
+  // Targets of boundary file idx into dst; false on a file error.
   bool load_into(const GDims& g, const InputData& in, const std::vector<double>& thb3,
                  const std::vector<double>& pib3, int idx,
                  std::array<std::vector<double>, 5>& dst);
