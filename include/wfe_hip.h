@@ -206,6 +206,14 @@ int wfe_output_mark(wfe_ctx* ctx, long long* ticket);
  * thread other than the one stepping the context. The last 64 tickets are
  * valid. */
 int wfe_output_wait(wfe_ctx* ctx, long long ticket);
+/* Device buffers for callers without a HIP runtime of their own (the
+ * Python tools load this library's runtime; a second HIP runtime in the
+ * same process cannot share the GPU): allocation, release and synchronous
+ * copies to and from host memory. */
+int wfe_device_alloc(size_t bytes, void** out);
+int wfe_device_free(void* p);
+int wfe_device_upload(void* dev, const void* host, size_t bytes);
+int wfe_device_download(void* host, const void* dev, size_t bytes);
 /* Pinned (page-locked) host memory for wfe_pack_field_async destinations. */
 int wfe_host_alloc(size_t bytes, void** out);
 int wfe_host_free(void* p);

@@ -385,10 +385,15 @@ def test_regional_file_mode_case(tmp_path, precision, tol):
     """Synthetic regional case in the reference's file formats: terrain,
     Lambert map factors, PD moisture, open boundaries with Davies relaxation
     toward the boundary files."""
-    from wfe_amd import Case
+    from oracle_lib import OracleCase
+    from wfe_amd import Case, Integrator
     from wfe_amd.lam_case import write_lam_case
-    c = Case(write_lam_case(tmp_path, nx=40, ny=30, nz=20, dx=12000.0, top=15000.0, seed=3))
-    om, gi = oracle_and_gpu(c, precision)
+    ini = write_lam_case(tmp_path, nx=40, ny=30, nz=20, dx=12000.0, top=15000.0, seed=3)
+    c = Case(ini)
+    # the oracle from its own case assembly, with its own boundary manager
+    # (the relaxation targets are interpolated in time between two files)
+    om = OracleCase(ini).model()
+    gi = Integrator.from_case(c, precision)
     dt = c.scalar("dt")
     for s in range(3):
         om.step(dt, s * dt)

@@ -80,6 +80,10 @@ SIGNATURES = [
     ("wfe_pack_field_async", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_size_t]),
     ("wfe_output_mark", C.c_int, [C.c_void_p, C.POINTER(C.c_longlong)]),
     ("wfe_output_wait", C.c_int, [C.c_void_p, C.c_longlong]),
+    ("wfe_device_alloc", C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
+    ("wfe_device_free", C.c_int, [C.c_void_p]),
+    ("wfe_device_upload", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("wfe_device_download", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     ("wfe_host_alloc", C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
     ("wfe_host_free", C.c_int, [C.c_void_p]),
     ("wfe_set_bdy_weights", C.c_int, [C.c_void_p, _DP, C.c_size_t]),
@@ -148,3 +152,30 @@ def check(rc: int, what: str) -> None:
 def as_dptr(a: np.ndarray):
     assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
     return a.ctypes.data_as(_DP)
+
+
+class DeviceArray:
+    """A device copy of a numpy array in memory this library allocates
+    (wfe_device_alloc), freed with the object."""
+
+    def __init__(self, a: np.ndarray):
+        self.dtype = a.dtype
+        self.shape = a.shape
+        self.nbytes = a.nbytes
+        self.ptr = C.c_void_p()
+        check(lib().wfe_device_alloc(max(1, self.nbytes), C.byref(self.ptr)), "wfe_device_alloc")
+        a = np.ascontiguousarray(a)
+        check(lib().wfe_device_upload(self.ptr, a.ctypes.data_as(C.c_void_p), self.nbytes),
+              "wfe_device_upload")
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        check(lib().wfe_device_download(out.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes),
+              "wfe_device_download")
+        return out
+
+    def __del__(self):
+        p = getattr(self, "ptr", None)
+        if p is not None and p.value and _lib is not None:
+            lib().wfe_device_free(p)
+            self.ptr = None

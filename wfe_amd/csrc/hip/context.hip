@@ -2217,6 +2217,34 @@ int wfe_output_wait(wfe_ctx* ctx, long long ticket) {
   return guarded([&] { ctx->impl->output_wait(ticket); });
 }
 
+int wfe_device_alloc(size_t bytes, void** out) {
+  return guarded([&] {
+    if (!out || bytes == 0) throw wfe::ArgError("wfe_device_alloc: bad arguments");
+    *out = nullptr;
+    WFE_HIP(hipMalloc(out, bytes));
+  });
+}
+
+int wfe_device_free(void* p) {
+  return guarded([&] {
+    if (p) WFE_HIP(hipFree(p));
+  });
+}
+
+int wfe_device_upload(void* dev, const void* host, size_t bytes) {
+  return guarded([&] {
+    if (!dev || !host) throw wfe::ArgError("wfe_device_upload: null pointer");
+    WFE_HIP(hipMemcpy(dev, host, bytes, hipMemcpyHostToDevice));
+  });
+}
+
+int wfe_device_download(void* host, const void* dev, size_t bytes) {
+  return guarded([&] {
+    if (!dev || !host) throw wfe::ArgError("wfe_device_download: null pointer");
+    WFE_HIP(hipMemcpy(host, dev, bytes, hipMemcpyDeviceToHost));
+  });
+}
+
 int wfe_host_alloc(size_t bytes, void** out) {
   return guarded([&] {
     if (!out || bytes == 0) throw wfe::ArgError("wfe_host_alloc: bad arguments");
