@@ -191,6 +191,21 @@ int wfe_diagnose_fields(const wfe_grid* g, int precision, const void* w, const v
                         const double* pip, const void* qc, const void* qr, void* stream,
                         wfe_diag* out);
 
+/* wfe_diagnose_fields without any allocation. No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, hosts that forbid hipMalloc).
+ * The workgroup records go to a fixed 1 MiB device workspace inside the
+ * library, one per GPU, held until the process ends. Limit: at most 16384
+ * workgroups of 64 x 4 columns x 8 levels, i.e. ceil(nx/64) * ceil(ny/4) *
+ * ceil((nz+1)/8) <= 16384; larger grids are rejected with WFE_E_ARG. Calls
+ * on the same GPU share the workspace: calls that overlap in time on one
+ * GPU (several host threads or streams) race and must be serialised by the
+ * caller. Same arguments, result and synchronisation as
+ * wfe_diagnose_fields. */
+int wfe_diagnose_fields_noalloc(const wfe_grid* g, int precision, const void* w,
+                                const void* thp, const double* pip, const void* qc,
+                                const void* qr, void* stream, wfe_diag* out);
+
 /* Writer::write_field (src/io/writer.cpp:89-107) without the full download:
  * packs the interior of a state field (WFE_F_U .. WFE_F_QR) or the rain
  * (WFE_F_RAIN) as float32 in the output order (i fastest, k slowest; nz

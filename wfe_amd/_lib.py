@@ -77,6 +77,9 @@ SIGNATURES = [
     ("wfe_diagnose_fields", C.c_int, [C.POINTER(wfe_grid), C.c_int, C.c_void_p, C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.POINTER(wfe_diag)]),
+    ("wfe_diagnose_fields_noalloc", C.c_int, [C.POINTER(wfe_grid), C.c_int, C.c_void_p,
+                                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.c_void_p, C.POINTER(wfe_diag)]),
     ("wfe_pack_field_async", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_size_t]),
     ("wfe_output_mark", C.c_int, [C.c_void_p, C.POINTER(C.c_longlong)]),
     ("wfe_output_wait", C.c_int, [C.c_void_p, C.c_longlong]),

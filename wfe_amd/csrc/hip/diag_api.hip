@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -13,6 +14,10 @@
 namespace wfe {
 void set_last_error(const std::string& s);
 }
+
+// Workspace of wfe_diagnose_fields_noalloc (see below): 16385 records of 8
+// doubles, about 1 MiB per GPU.
+static __device__ __align__(256) double scratch[(16384 + 1) * 8];
 
 namespace {
 
@@ -117,6 +122,36 @@ void check_fields(int precision, const void* w, const void* thp, const double* p
     throw DiagError(WFE_E_ARG, "qc and qr must both be given or both be null");
 }
 
+// Workspace of the allocation-free entry point: partial records of at most
+// kNoAllocPartials workgroups plus the final record, in one file-scope device
+// array. Every GPU holds its own copy at its own address; host code must use
+// the address hipGetSymbolAddress returns for the current device, never the
+// array's name (a host-side stand-in).
+constexpr int kNoAllocPartials = 16384;
+static_assert(sizeof(scratch) ==
+                  (size_t)(kNoAllocPartials + 1) * wfe::dev::kDgSlots * sizeof(double),
+              "workspace size");
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+double* device_scratch() {
+  int dev = 0;
+  hip_ok(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) throw DiagError(WFE_E_ARG, "device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    hip_ok(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!ptr) throw DiagError(WFE_E_HIP, "hipGetSymbolAddress(scratch) returned null");
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  return g_scratch_ptr[dev];
+}
+
 }  // namespace
 
 extern "C" {
@@ -138,6 +173,28 @@ int wfe_diagnose_fields(const wfe_grid* g, int precision, const void* w, const v
       ~Free() { (void)hipFree(p); }
     } fr{part};
     run_diag(d, precision, w, thp, pip, qc, qr, st, part, nb, out);
+  });
+}
+
+int wfe_diagnose_fields_noalloc(const wfe_grid* g, int precision, const void* w,
+                                const void* thp, const double* pip, const void* qc,
+                                const void* qr, void* stream, wfe_diag* out) {
+  return guarded_diag("wfe_diagnose_fields_noalloc", [&] {
+    const wfe::dev::Grid d = ref_grid(g);
+    check_fields(precision, w, thp, pip, qc, qr, out);
+    const dim3 grd = diag_grid(d);
+    const long long nb = (long long)grd.x * grd.y * grd.z;
+    if (nb > kNoAllocPartials)
+      throw DiagError(WFE_E_ARG,
+                      "grid too large for the fixed workspace: " + std::to_string(nb) +
+                          " workgroups of 64 x 4 columns x " +
+                          std::to_string(wfe::dev::kDiagLevels) + " levels, at most " +
+                          std::to_string(kNoAllocPartials) +
+                          " (use wfe_diagnose_fields for larger grids)");
+    // Every record the kernels read is written earlier in the same call, so
+    // nothing left by a previous call needs clearing.
+    run_diag(d, precision, w, thp, pip, qc, qr, static_cast<hipStream_t>(stream),
+             device_scratch(), (int)nb, out);
   });
 }
 
