@@ -116,3 +116,28 @@ def test_integrator_check_rejects_file_mode(tmp_path):
     exe = ROOT / "wfe_amd" / "lib" / "wfe_integrator_check"
     r = subprocess.run([str(exe), str(ini)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 1 and "file-mode" in r.stderr
+
+
+@pytest.mark.gpu
+def test_driver_profile_table_and_launch_sync(tmp_path):
+    """WFE_PROF=1 prints the reference's section table from per-kernel times
+    (integrator.cpp:113-122); WFE_SYNC=1 waits after every launch (step
+    graphs off) and gives the same diagnostics as the default run."""
+    import os
+    over = {**SMALL, "out_dir": tmp_path, "out_interval": 60, "diag_interval": 30}
+    args = [SUPERCELL, *[f"{k}={v}" for k, v in over.items()]]
+    base = subprocess.run([str(EXE), *map(str, args)], capture_output=True, text=True,
+                          timeout=300, cwd=tmp_path)
+    prof = subprocess.run([str(EXE), *map(str, args)], capture_output=True, text=True,
+                          timeout=300, cwd=tmp_path, env=dict(os.environ, WFE_PROF="1"))
+    sync = subprocess.run([str(EXE), *map(str, args)], capture_output=True, text=True,
+                          timeout=300, cwd=tmp_path, env=dict(os.environ, WFE_SYNC="1"))
+    print(prof.stdout[-1500:], flush=True)
+    for r in (base, prof, sync):
+        assert r.returncode == 0, r.stderr
+    assert "profil (WFE_PROF=1" in prof.stdout
+    for section in ("sinir kosullari", "kutle akisi+div", "yavas tendency", "akustik dongu",
+                    "kessler"):
+        assert section in prof.stdout, section
+    lines = lambda r: [l for l in r.stdout.splitlines() if l.startswith("adim")]  # noqa: E731
+    assert lines(base) == lines(prof) == lines(sync)

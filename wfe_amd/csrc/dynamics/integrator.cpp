@@ -3,6 +3,10 @@
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
+#include <sstream>
+#include <string>
+#include <vector>
 
 namespace wfe {
 
@@ -32,6 +36,9 @@ void Integrator::bind() {
   rain_ = Field3D(ctx_, WFE_F_RAIN, g_.n2());
   const char* e = std::getenv("WFE_PROF");
   prof_on_ = e && e[0] == '1';
+  // per-kernel HIP-event timing on the context's stream (steps then run
+  // eagerly, one event pair around each launch)
+  if (prof_on_) wfe_check(wfe_kernel_timing(ctx_, 2), "kernel timing");
 }
 
 // Integrator::init (src/dynamics/integrator.cpp:43-69).
@@ -61,26 +68,45 @@ void Integrator::adopt(const GDims& g, const DynParams& dp, wfe_ctx* ctx) {
 }
 
 // Integrator::step (src/dynamics/integrator.cpp:71-111): boundary update,
-// then the device step. With WFE_PROF=1 each step is timed synchronously.
+// then the device step.
 void Integrator::step(double dt, double t) {
   if (bdy_ && bdy_->active()) bdy_->update(ctx_, t);
-  std::chrono::steady_clock::time_point t0;
-  if (prof_on_) {
-    wfe_check(wfe_sync(ctx_), "sync");
-    t0 = std::chrono::steady_clock::now();
-  }
   wfe_check(wfe_step(ctx_, dt, t), "Integrator::step");
-  if (prof_on_) {
-    wfe_check(wfe_sync(ctx_), "sync");
-    wall_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    ++steps_;
-  }
+  ++steps_;
 }
 
+// The WFE_PROF=1 section table of src/dynamics/integrator.cpp:17-41,113-122:
+// kernel times summed into the reference's eight sections, in its format.
 void Integrator::print_profile() const {
   if (!prof_on_ || steps_ == 0) return;
-  std::printf("\nprofile (WFE_PROF=1): %ld steps, %.3f s GPU-synchronous, %.3f ms/step\n",
-              steps_, wall_, 1e3 * wall_ / (double)steps_);
+  static const char* kSection[8] = {"sinir kosullari", "kutle akisi+div", "yavas tendency",
+                                    "sinir relaks",    "akustik dongu",   "nem asama",
+                                    "kessler",         "yuzey/PBL"};
+  struct Map {
+    const char* kernel;
+    int section;
+  };
+  static const Map kMap[] = {{"fill_ghosts", 0}, {"massflux", 1},      {"tendencies", 2},
+                             {"pd_ratio", 2},    {"pd_apply", 2},      {"bdy_relax", 3},
+                             {"acou_uv", 4},     {"acou_wpi", 4},      {"acoustic_fused", 4},
+                             {"stage_q", 5},     {"kessler", 6},       {"sfc_scalar", 7},
+                             {"sfc_u", 7},       {"sfc_v", 7},         {"set_bdy_tfrac", 3}};
+  std::vector<char> buf(1 << 16);
+  wfe_check(wfe_kernel_timing_report(ctx_, buf.data(), buf.size()), "kernel timing report");
+  double sec[8] = {};
+  std::istringstream in(buf.data());
+  std::string name;
+  long launches;
+  double ms;
+  while (in >> name >> launches >> ms)
+    for (const Map& m : kMap)
+      if (name == m.kernel) sec[m.section] += ms * 1e-3;
+  double tot = 0;
+  for (double v : sec) tot += v;
+  std::printf("\nprofil (WFE_PROF=1, toplam %.2fs GPU kernel time, %ld adim):\n", tot, steps_);
+  for (int q = 0; q < 8; ++q)
+    if (sec[q] > 0)
+      std::printf("  %-16s %7.2fs  %5.1f%%\n", kSection[q], sec[q], 100.0 * sec[q] / tot);
 }
 
 }  // namespace wfe

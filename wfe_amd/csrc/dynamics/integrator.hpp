@@ -74,7 +74,6 @@ class Integrator {
   Field3D rain_{};
   BoundarySource* bdy_ = nullptr;
   bool prof_on_ = false;
-  double wall_ = 0;
   long steps_ = 0;
 };
 

@@ -45,10 +45,29 @@ namespace {
                      ":" + std::to_string(__LINE__) + ": " + hipGetErrorString(e_));      \
   } while (0)
 
+// WFE_SYNC=1 (the reference's launch-check switch, src/core/cuda_check.hpp:
+// 20-32): every launch check also waits for the device, so an asynchronous
+// fault is reported at the kernel that caused it; step graphs are then off.
+// Launches recorded into a graph are not waited for (capture_depth > 0).
+bool sync_launches() {
+  static const bool on = [] {
+    const char* e = std::getenv("WFE_SYNC");
+    return e && e[0] == '1';
+  }();
+  return on;
+}
+thread_local int capture_depth = 0;
+
+struct CaptureScope {
+  CaptureScope() { ++capture_depth; }
+  ~CaptureScope() { --capture_depth; }
+};
+
 inline void check_launch(const char* what) {
   hipError_t e = hipGetLastError();
+  if (e == hipSuccess && sync_launches() && capture_depth == 0) e = hipDeviceSynchronize();
   if (e != hipSuccess)
-    throw HipError(std::string("kernel launch failed in ") + what + ": " + hipGetErrorString(e));
+    throw HipError(std::string("kernel error in ") + what + ": " + hipGetErrorString(e));
 }
 
 inline unsigned cdiv(int a, int b) { return (unsigned)((a + b - 1) / b); }
@@ -501,6 +520,7 @@ Ctx<T>::Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& b, const w
   p_.microphysics = hp.microphysics;
 
   if (const char* e = std::getenv("WFE_GRAPH")) use_graph_ = std::string(e) != "0";
+  if (sync_launches()) use_graph_ = false;
   if (const char* e = std::getenv("WFE_TENDENCY_LEVELS")) {
     const int v = std::atoi(e);
     if (v != 1 && v != 2 && v != 4) throw ArgError("WFE_TENDENCY_LEVELS must be 1, 2 or 4");
@@ -1349,6 +1369,7 @@ void Ctx<T>::step(double dt) {
     hipGraph_t gr = nullptr;
     WFE_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     try {
+      const CaptureScope cap;
       step_launches(dt);
     } catch (...) {
       (void)hipStreamEndCapture(stream_, &gr);
@@ -2071,6 +2092,7 @@ void ensemble_step(wfe_ensemble& E, const double* dt) {
     hipGraph_t gr = nullptr;
     WFE_HIP(hipStreamBeginCapture(E.s, hipStreamCaptureModeThreadLocal));
     try {
+      const wfe::CaptureScope cap;
       WFE_HIP(hipEventRecord(E.fork, E.s));
       for (size_t q = 0; q < n; ++q) {
         hipStream_t ms = E.m[q]->impl->stream();
