@@ -1,3 +1,5 @@
+// Derived in part from Xassemblianist/WFE, Copyright (c) 2026 Omer Kaan (Xassemblianist),
+// MIT License; the licence text is in THIRD_PARTY_NOTICES at the repository root.
 // case_builder.cpp -- CPU FP64 restatement of the reference's host-side case
 // assembly: Config (src/core/config.cpp:17-60), file-mode input
 // (src/io/input.cpp:19-73), Metric::build (src/core/metric.cpp:12-168),

@@ -1,3 +1,5 @@
+// Derived in part from Xassemblianist/WFE, Copyright (c) 2026 Omer Kaan (Xassemblianist),
+// MIT License; the licence text is in THIRD_PARTY_NOTICES at the repository root.
 // CPU FP64 restatement of the WFE large-step dynamics hot path.
 // TEST INFRASTRUCTURE ONLY (see oracle.h): the checker for the HIP path and
 // the CPU baseline. Every function cites the reference file:line it follows.

@@ -1,3 +1,5 @@
+// Derived in part from Xassemblianist/WFE, Copyright (c) 2026 Omer Kaan (Xassemblianist),
+// MIT License; the licence text is in THIRD_PARTY_NOTICES at the repository root.
 // wfe <case.ini> [key=value ...] -- the model driver of src/main.cpp:245-477
 // over the gfx950 integrator: same case keys, stdout diagnostic lines, output
 // files and exit codes (0 ok, 1 config/setup/runtime error, 2 NaN in the

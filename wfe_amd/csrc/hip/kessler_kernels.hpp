@@ -1,4 +1,6 @@
 #pragma once
+// Derived in part from Xassemblianist/WFE, Copyright (c) 2026 Omer Kaan (Xassemblianist),
+// MIT License; the licence text is in THIRD_PARTY_NOTICES at the repository root.
 // Kessler (1969) warm rain with the reference's mixed-phase blend, applied
 // after the large step by operator splitting (k_kessler,
 // src/physics/kessler.cu:22-139): CFL-subcycled upwind rain sedimentation

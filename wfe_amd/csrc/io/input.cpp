@@ -1,3 +1,5 @@
+// Derived in part from Xassemblianist/WFE, Copyright (c) 2026 Omer Kaan (Xassemblianist),
+// MIT License; the licence text is in THIRD_PARTY_NOTICES at the repository root.
 #include "io/input.hpp"
 
 #include <cstdio>

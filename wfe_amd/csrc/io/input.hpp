@@ -1,4 +1,6 @@
 #pragma once
+// Derived in part from Xassemblianist/WFE, Copyright (c) 2026 Omer Kaan (Xassemblianist),
+// MIT License; the licence text is in THIRD_PARTY_NOTICES at the repository root.
 
 #include <string>
 #include <vector>
