@@ -48,7 +48,10 @@ enum {
   WFE_F_U = 0, WFE_F_V, WFE_F_W, WFE_F_THP, WFE_F_PIP, WFE_F_QV, WFE_F_QC, WFE_F_QR,
   WFE_F_TU = 8, WFE_F_TV, WFE_F_TW, WFE_F_TTHP, WFE_F_TPIP, WFE_F_TQV, WFE_F_TQC, WFE_F_TQR,
   WFE_F_RAIN = 16, /* 2-D accumulated surface rain [kg m-2] */
-  WFE_F_COUNT = 17
+  /* 2-D surface-physics fields (after wfe_set_physics): skin temperature
+   * [K], boundary-layer height [m], 2 m temperature [K], 10 m wind [m/s] */
+  WFE_F_TSK = 17, WFE_F_PBLH, WFE_F_T2M, WFE_F_U10,
+  WFE_F_COUNT = 21
 };
 
 /* GDims (src/core/grid.hpp:22-36). ng must be 3. */
@@ -105,11 +108,32 @@ int wfe_destroy(wfe_ctx* ctx);
 
 /* Integrator::step (integrator.cpp:71-111): one WS2002 RK3 large step with
  * split-explicit acoustic substeps, moisture stage updates, boundary
- * relaxation (when set) and Kessler microphysics (when moisture and
- * microphysics are on). t is the simulation time at the start of the step;
- * it is not used by the device path (the boundary time fraction is set with
- * wfe_set_bdy_tfrac). Asynchronous with respect to the host. */
+ * relaxation (when set), Kessler microphysics (when moisture and
+ * microphysics are on) and the surface physics (when set). t is the
+ * simulation time at the start of the step: the surface physics takes its
+ * hour of day at t + dt; the boundary time fraction is set separately with
+ * wfe_set_bdy_tfrac. Asynchronous with respect to the host. */
 int wfe_step(wfe_ctx* ctx, double dt, double t);
+
+/* SfcPBL::init (src/physics/surface.cu:440-482) + Integrator::set_physics
+ * (src/dynamics/integrator.hpp:27): surface/PBL/soil/radiation column
+ * physics (surface.cu:87-437) applied to the state after every step.
+ * tsk, land, lat, lon, soilw: FP64 host arrays on the padded 2-D plane
+ * (NX*NY, n2 elements; interior used): skin temperature [K], land mask (> 0.5
+ * land), latitude and longitude [deg], soil moisture [m3 m-3]. Deep soil and
+ * the four soil layers start at tsk. start_hour_utc: hour of day of t = 0;
+ * doy: day of year (1..366); nonlocal: 1 for the nonlocal (Troen-Mahrt /
+ * Hong-Pan) boundary layer, 0 for local-Ri diffusivity. The skin
+ * temperature, boundary-layer height, 2 m temperature and 10 m wind are
+ * read back with WFE_F_TSK .. WFE_F_U10. Ensemble steps do not advance the
+ * hour of day of members with surface physics. */
+typedef struct {
+  const double *tsk, *land, *lat, *lon, *soilw;
+  double start_hour_utc;
+  int doy;
+  int nonlocal;
+} wfe_sfc;
+int wfe_set_physics(wfe_ctx* ctx, const wfe_sfc* sfc, size_t n2);
 
 /* ---- tangent-linear and adjoint of the dry large step (FP64) ----
  * No counterpart in the reference. Scope: plain FP64 contexts of dry cases
@@ -150,9 +174,10 @@ int wfe_ensemble_step(wfe_ensemble* e, const double* dt);
 int wfe_ensemble_sync(wfe_ensemble* e);
 
 /* Field3D::upload / download (src/core/field3d.hpp:42-47) of the time-n
- * state (Integrator::state(), integrator.hpp:24), of the slow tendencies, or
- * of the 2-D rain (Integrator::rain(), integrator.hpp:25). n is the element
- * count (npts, or NX*NY for WFE_F_RAIN). Synchronous. */
+ * state (Integrator::state(), integrator.hpp:24), of the slow tendencies, of
+ * the 2-D rain (Integrator::rain(), integrator.hpp:25) or (download only) of
+ * the 2-D surface fields (SfcPBL::tsk/pblh/t2m/u10, surface.hpp:26-29). n is
+ * the element count (npts, or NX*NY for 2-D fields). Synchronous. */
 int wfe_upload(wfe_ctx* ctx, int field, const double* host, size_t n);
 int wfe_download(wfe_ctx* ctx, int field, double* host, size_t n);
 
@@ -305,16 +330,19 @@ int wfe_case_load(const char* ini_path, int n_over, const char* const* over, wfe
 int wfe_case_destroy(wfe_case* c);
 int wfe_case_grid(const wfe_case* c, wfe_grid* g);
 int wfe_case_params(const wfe_case* c, wfe_params* p);
-/* Run keys: dt, t_end, out_interval, diag_interval, zt. */
+/* Run keys: dt, t_end, out_interval, diag_interval, zt; surface physics:
+ * sfc_on, sfc_start_hour, sfc_doy, sfc_nonlocal. */
 int wfe_case_scalar(const wfe_case* c, const char* name, double* out);
 /* Element count of a named host array, or 0 if unknown. Names: base fields
  * (thb pib rhob dthbdz thbw rhobw thvb thvbw qvb ub vb), metric fields
  * (zeta_w zeta_c dzeta_c dzeta_w h hx hy hx_u hy_v jac fcor mapf), initial
  * state (u v w thp pip qv qc qr), boundary (bdy_wgt, bdy_lo_<v>, bdy_hi_<v>
- * for v in u v thp pip qv). */
+ * for v in u v thp pip qv), surface physics (sfc_tsk sfc_land sfc_lat
+ * sfc_lon sfc_soilw). */
 size_t wfe_case_array_size(const wfe_case* c, const char* name);
 int wfe_case_array(const wfe_case* c, const char* name, double* out, size_t n);
-/* Integrator::init + IC upload (+ BdyManager::init in file mode). */
+/* Integrator::init + IC upload (+ BdyManager::init in file mode, + the
+ * surface physics with physics = simple). */
 int wfe_case_create_ctx(const wfe_case* c, int precision, wfe_ctx** out);
 /* BdyManager::update(t) then tfrac(t) for the context (boundary.cpp:83-97),
  * reading further boundary files when t passes the next boundary time. Each

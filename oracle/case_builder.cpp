@@ -93,7 +93,7 @@ struct Dims {
 // ---------------------------------------------------------------- input
 // src/io/input.cpp:10-73
 struct Input {
-  std::string dir;
+  std::string dir, start;
   std::vector<double> prof_z, prof_th, prof_qv, prof_u;
   std::vector<double> h, fcor, tsk, land, lat, lon, soilw;
   std::vector<double> u, v, th, pi, qv;
@@ -118,6 +118,7 @@ struct Input {
     int npf = meta.get_int("np_prof", 0);
     bdy_interval = meta.get_real("bdy_interval", 10800);
     n_bdy = meta.get_int("n_bdy", 0);
+    start = meta.get_str("start", "");
     FILE* f = std::fopen((d + "/wfe_init.bin").c_str(), "rb");
     if (!f) return false;
     size_t n2 = (size_t)g.nx * g.ny, n3 = n2 * g.nz;
@@ -530,6 +531,8 @@ struct Case {
   bool file_mode = false;
   std::map<std::string, double> scalars, params;
   std::array<std::vector<double>, 8> ic;  // u v w thp pip qv qc qr
+  // surface physics initial fields (src/main.cpp:364-377, surface.cu:465-472)
+  std::vector<double> sfc_tsk, sfc_land, sfc_lat, sfc_lon, sfc_soilw;
 };
 
 // src/main.cpp:28-54
@@ -758,7 +761,36 @@ bool build_case(Case& cs) {
                      cfg.get_int("bdy_width", 8), cfg.get_real("bdy_tau", 600),
                      cfg.get_real("nudge_tau", 0)))
       return false;
+    // physics = simple (main.cpp:364-377): day of year and UTC hour of the
+    // input's start time, local or nonlocal boundary layer
+    cs.scalars["sfc_on"] = 0;
+    if (cfg.get_str("physics", "none") == "simple") {
+      const std::string& st = cs.input.start;
+      if (st.size() < 10) return false;
+      int mo = std::atoi(st.substr(4, 2).c_str());
+      int dy = std::atoi(st.substr(6, 2).c_str());
+      int hh = std::atoi(st.substr(8, 2).c_str());
+      static const int cum[12] = {0, 31, 59, 90, 120, 151, 181, 212, 243, 273, 304, 334};
+      if (mo < 1 || mo > 12) return false;
+      cs.scalars["sfc_on"] = 1;
+      cs.scalars["sfc_doy"] = cum[mo - 1] + dy;
+      cs.scalars["sfc_start_hour"] = hh;
+      cs.scalars["sfc_nonlocal"] = cfg.get_str("pbl", "local") == "nonlocal";
+      auto up2 = [&](const std::vector<double>& src) {  // surface.cu:465-471
+        std::vector<double> h(g.n2(), 0.0);
+        for (int j = 0; j < g.ny; ++j)
+          for (int i = 0; i < g.nx; ++i)
+            h[(size_t)(j + g.ng) * g.NX + (i + g.ng)] = src[(size_t)j * g.nx + i];
+        return h;
+      };
+      cs.sfc_tsk = up2(cs.input.tsk);
+      cs.sfc_land = up2(cs.input.land);
+      cs.sfc_lat = up2(cs.input.lat);
+      cs.sfc_lon = up2(cs.input.lon);
+      cs.sfc_soilw = up2(cs.input.soilw);
+    }
   } else {  // :379-382
+    cs.scalars["sfc_on"] = 0;
     init_bubble(cs);
     init_wind(cs);
     init_moisture(cs);
@@ -781,7 +813,9 @@ const std::vector<double>* array_of(const Case& cs, const std::string& n) {
       {"fcor", &m.fcor},     {"mapf", &m.mapf},     {"u", &cs.ic[0]},
       {"v", &cs.ic[1]},      {"w", &cs.ic[2]},      {"thp", &cs.ic[3]},
       {"pip", &cs.ic[4]},    {"qv", &cs.ic[5]},     {"qc", &cs.ic[6]},
-      {"qr", &cs.ic[7]},     {"bdy_wgt", &cs.bdy.wgt}};
+      {"qr", &cs.ic[7]},     {"bdy_wgt", &cs.bdy.wgt}, {"sfc_tsk", &cs.sfc_tsk},
+      {"sfc_land", &cs.sfc_land}, {"sfc_lat", &cs.sfc_lat}, {"sfc_lon", &cs.sfc_lon},
+      {"sfc_soilw", &cs.sfc_soilw}};
   for (const auto& e : t)
     if (n == e.first) return e.second;
   static const char* vn[5] = {"u", "v", "thp", "pip", "qv"};

@@ -95,6 +95,10 @@ struct orc_model {
   int bdy_active = 0;
   double tfrac = 0;
   Vec bdy_lo[5], bdy_hi[5], bdy_wgt;
+  // surface/PBL physics (SfcPBL, src/physics/surface.hpp:19-43)
+  int phys_on = 0, phys_nonlocal = 0, phys_doy = 1;
+  double phys_start_hour = 0;
+  Vec tsk, tdeep, land, lat, lon, soilw, cdv, pblh, t2m, u10, km, soilt;
 
   // ------------------------------------------------------------- helpers
   double ray_alpha(double zeta) const {  // kernels.cu:42-48
@@ -882,8 +886,336 @@ struct orc_model {
     }
   }
 
+
+  // ------------------------------------- surface/PBL physics (src/physics/surface.cu)
+  static constexpr double kKappa = 0.4, kSigma = 5.67e-8, kS0 = 1361;  // surface.cu:24-27
+  static constexpr double kKmMax = 1000, kKmMin = 0.1;
+  static double psi_m(double zeta) {  // surface.cu:31-37
+    if (zeta >= 0) return -5 * std::fmin(zeta, 2.0);
+    const double x = std::pow(1 - 16 * zeta, 0.25);
+    const double halfpi = 1.5707963267948966;
+    return 2 * std::log((1 + x) * 0.5) + std::log((1 + x * x) * 0.5) - 2 * std::atan(x) +
+           halfpi;
+  }
+  static double psi_h(double zeta) {  // surface.cu:38-42
+    if (zeta >= 0) return -5 * std::fmin(zeta, 2.0);
+    const double x = std::pow(1 - 16 * zeta, 0.25);
+    return 2 * std::log((1 + x * x) * 0.5);
+  }
+  static double soil_C(double w) { return 1.2e6 + 3.2e6 * std::fmin(std::fmax(w, 0.0), 0.4); }
+  static double soil_K(double w) { return 0.25 + 3.0 * std::fmin(std::fmax(w, 0.0), 0.4); }
+  // Column-implicit vertical diffusion (surface.cu:61-84).
+  static void diffuse_column(int nz, double dt, const double* rho, const double* rhow,
+                             const double* dzc, const double* dzw, const double* Kw,
+                             double src0, double* phi, const double* fcg) {
+    std::vector<double> A(nz), B(nz), C(nz), D(nz);
+    for (int k = 0; k < nz; ++k) {
+      const double a = (k > 0) ? rhow[k] * Kw[k] / dzw[k] : 0.0;
+      const double c = (k < nz - 1) ? rhow[k + 1] * Kw[k + 1] / dzw[k + 1] : 0.0;
+      const double m = rho[k] * dzc[k] / dt;
+      A[k] = -a;
+      B[k] = m + a + c;
+      C[k] = -c;
+      D[k] = m * phi[k];
+      if (fcg) D[k] += fcg[k] - fcg[k + 1];
+    }
+    D[0] += src0;
+    for (int k = 1; k < nz; ++k) {
+      const double w = A[k] / B[k - 1];
+      B[k] -= w * C[k - 1];
+      D[k] -= w * D[k - 1];
+    }
+    phi[nz - 1] = D[nz - 1] / B[nz - 1];
+    for (int k = nz - 2; k >= 0; --k) phi[k] = (D[k] - C[k] * phi[k + 1]) / B[k];
+  }
+
+  // k_sfc_scalar (surface.cu:87-387) for column (i, j) of state s.
+  void sfc_scalar_column(int i, int j, double dt, double utc, State& s) {
+    const int nz = g.nz;
+    const long c2 = g.id2(i, j);
+    const double J = jac[c2];
+    std::vector<double> rho(nz), rhow(nz), dzc(nz), dzw(nz), uc(nz), vc(nz), thv(nz),
+        Kw(nz + 1), thcol(nz), qvcol(nz);
+    double lwp = 0;
+    for (int k = 0; k < nz; ++k) {  // :101-114
+      const int kk = k + g.ng;
+      const long c = g.id(i, j, k);
+      rho[k] = rhob[c];
+      rhow[k] = rhobw[c];
+      dzc[k] = dzeta_c[kk] * J;
+      dzw[k] = dzeta_w[kk] * J;
+      uc[k] = 0.5 * (s.u[g.id(i, j, k)] + s.u[g.id(i + 1, j, k)]);
+      vc[k] = 0.5 * (s.v[g.id(i, j, k)] + s.v[g.id(i, j + 1, k)]);
+      thcol[k] = thb[c] + s.thp[c];
+      qvcol[k] = s.qv[c];
+      thv[k] = thcol[k] * (1 + kEps61 * qvcol[k]);
+      lwp += rho[k] * (s.qc[c] > 0 ? s.qc[c] : 0.0) * dzc[k] * 1000;
+    }
+    // surface layer (Louis 1979), :117-159
+    const bool onland = land[c2] > 0.5;
+    const double z0 = onland ? 0.1 : 2e-4;
+    const double alb = onland ? 0.2 : 0.08;
+    double beta;
+    if (onland) {
+      double f = (soilw[c2] - 0.08) / (0.32 - 0.08);
+      f = std::fmin(1.0, std::fmax(0.0, f));
+      beta = std::fmax(0.05, f * f);
+    } else {
+      beta = 1;
+    }
+    const double z1 = 0.5 * dzc[0];
+    double V1 = std::sqrt(uc[0] * uc[0] + vc[0] * vc[0]);
+    if (V1 < 0.5) V1 = 0.5;
+    const long c0 = g.id(i, j, 0);
+    const double pi1 = pib[c0] + s.pip[c0];
+    const double p1 = kP00 * std::pow(pi1, kCp / kRd);
+    const double Ts = tsk[c2];
+    const double pis = pi1 + kGrav * z1 / (kCp * thv[0]);
+    const double ths = Ts / pis;
+    const double qs_s = qsat_tetens(p1, Ts);
+    const double thvs = ths * (1 + kEps61 * (beta * qs_s + (1 - beta) * qvcol[0]));
+    const double Rib = kGrav * z1 * (thv[0] - thvs) / (thvs * V1 * V1);
+    double a2 = kKappa / std::log(z1 / z0);
+    a2 *= a2;
+    double Fm, Fh;
+    if (Rib < 0) {
+      const double sq = std::sqrt(-Rib * z1 / z0);
+      Fm = 1 - 9.4 * Rib / (1 + 7.4 * 9.4 * a2 * sq);
+      Fh = 1 - 9.4 * Rib / (1 + 5.3 * 9.4 * a2 * sq);
+    } else {
+      const double d = 1 + 4.7 * Rib;
+      Fm = Fh = 1 / (d * d);
+    }
+    const double Cd = a2 * Fm, Ch = a2 * Fh;
+    cdv[c2] = Cd * V1;
+    // 2 m and 10 m diagnostics (:166-184)
+    const double z0h = z0 * 0.1;
+    const double lnz1z0 = std::log(z1 / z0);
+    double zeta1 = (Rib >= 0) ? Rib * lnz1z0 / (1 - 5 * std::fmin(Rib, 0.19)) : Rib * lnz1z0;
+    zeta1 = std::fmax(-8.0, std::fmin(2.0, zeta1));
+    const double zol = zeta1 / z1;
+    const double den_m = lnz1z0 - psi_m(zeta1) + psi_m(z0 * zol);
+    const double num_m = std::log(10 / z0) - psi_m(10 * zol) + psi_m(z0 * zol);
+    const double r10 = std::fmax(0.0, std::fmin(1.2, num_m / std::fmax(den_m, 0.1)));
+    u10[c2] = V1 * r10;
+    const double den_h = std::log(z1 / z0h) - psi_h(zeta1) + psi_h(z0h * zol);
+    const double num_h = std::log(2 / z0h) - psi_h(2 * zol) + psi_h(z0h * zol);
+    const double r2 = std::fmax(0.0, std::fmin(1.2, num_h / std::fmax(den_h, 0.1)));
+    const double th2 = ths + (thcol[0] - ths) * r2;
+    t2m[c2] = th2 * pi1;
+    double shf = Ch * V1 * (ths - thcol[0]);
+    double qflx = Ch * V1 * beta * (qs_s - qvcol[0]);
+    if (!onland && qflx < 0) qflx = 0;
+    // radiation (:188-244)
+    const double decl = 0.40928 * std::sin(6.2832 * (284 + phys_doy) / 365.0);
+    const double latr = lat[c2] * 0.0174533;
+    const double hsol = utc + lon[c2] / 15;
+    const double H = 0.2618 * (hsol - 12);
+    double cosz = std::sin(latr) * std::sin(decl) + std::cos(latr) * std::cos(decl) * std::cos(H);
+    if (cosz < 0) cosz = 0;
+    std::vector<double> Tlev(nz), epslw(nz), Fup(nz + 1), Fdn(nz + 1), radht(nz), swabs(nz);
+    for (int k = 0; k < nz; ++k) {
+      const long c = g.id(i, j, k);
+      const double pik = pib[c] + s.pip[c];
+      Tlev[k] = thcol[k] * pik;
+      const double du = rho[k] * qvcol[k] * dzc[k];
+      const double dl = rho[k] * (s.qc[c] > 0 ? s.qc[c] : 0.0) * dzc[k] * 1000;
+      epslw[k] = 1 - std::exp(-(0.22 * du + 0.30 * dl));
+    }
+    Fdn[nz] = 0;
+    for (int k = nz - 1; k >= 0; --k)
+      Fdn[k] = Fdn[k + 1] * (1 - epslw[k]) + epslw[k] * kSigma * Tlev[k] * Tlev[k] * Tlev[k] * Tlev[k];
+    const double lwd = Fdn[0];
+    Fup[0] = 0.97 * kSigma * Ts * Ts * Ts * Ts + 0.03 * Fdn[0];
+    for (int k = 0; k < nz; ++k)
+      Fup[k + 1] = Fup[k] * (1 - epslw[k]) + epslw[k] * kSigma * Tlev[k] * Tlev[k] * Tlev[k] * Tlev[k];
+    const double mu = std::fmax(cosz, 0.05);
+    const double cldalb = lwp / (lwp + 60);
+    double Sdn = kS0 * cosz * 0.95 * (1 - cldalb);
+    for (int k = nz - 1; k >= 0; --k) {
+      const long c = g.id(i, j, k);
+      const double du = rho[k] * qvcol[k] * dzc[k];
+      const double dl = rho[k] * (s.qc[c] > 0 ? s.qc[c] : 0.0) * dzc[k] * 1000;
+      const double tau = std::exp(-(0.008 * du + 0.15 * dl) / mu);
+      const double ab = Sdn * (1 - tau);
+      swabs[k] = ab;
+      Sdn *= tau;
+    }
+    const double swn = Sdn * (1 - alb);
+    for (int k = 0; k < nz; ++k) {
+      const long c = g.id(i, j, k);
+      const double pik = pib[c] + s.pip[c];
+      const double lwconv = (Fdn[k + 1] - Fdn[k]) + (Fup[k] - Fup[k + 1]);
+      const double net = lwconv + swabs[k];
+      radht[k] = net / (rho[k] * kCp * dzc[k]) / pik;
+    }
+    // multi-layer soil (:247-289)
+    const double lwu = 0.97 * kSigma * Ts * Ts * Ts * Ts;
+    const double G = swn + 0.97 * lwd - lwu - rho[0] * kCp * shf * pi1 - rho[0] * kLv * qflx;
+    if (onland) {
+      static const double kDz[4] = {0.1, 0.3, 0.6, 1.0};
+      const double Cso = soil_C(soilw[c2]), Kso = soil_K(soilw[c2]);
+      const long n2 = g.n2();
+      double Tso[4], A[4], B[4], C[4], D[4];
+      for (int L = 0; L < 4; ++L) Tso[L] = soilt[L * n2 + c2];
+      for (int L = 0; L < 4; ++L) {
+        const double cap = Cso * kDz[L] / dt;
+        const double ku = (L > 0) ? Kso / (0.5 * (kDz[L - 1] + kDz[L])) : 0.0;
+        const double kl = (L < 3) ? Kso / (0.5 * (kDz[L] + kDz[L + 1])) : Kso / kDz[L];
+        A[L] = -ku;
+        C[L] = (L < 3) ? -kl : 0.0;
+        B[L] = cap + ku + kl;
+        D[L] = cap * Tso[L];
+      }
+      D[0] += G;
+      D[3] += (Kso / kDz[3]) * tdeep[c2];
+      for (int L = 1; L < 4; ++L) {
+        const double w = A[L] / B[L - 1];
+        B[L] -= w * C[L - 1];
+        D[L] -= w * D[L - 1];
+      }
+      Tso[3] = D[3] / B[3];
+      for (int L = 2; L >= 0; --L) Tso[L] = (D[L] - C[L] * Tso[L + 1]) / B[L];
+      for (int L = 0; L < 4; ++L) {
+        if (Tso[L] < 200) Tso[L] = 200;
+        if (Tso[L] > 350) Tso[L] = 350;
+        soilt[L * n2 + c2] = Tso[L];
+      }
+      tsk[c2] = Tso[0];
+    }
+    // diffusivity (:292-305)
+    auto local_K = [&](int k) -> double {
+      const double dz = dzw[k];
+      const double du = (uc[k] - uc[k - 1]) / dz;
+      const double dv = (vc[k] - vc[k - 1]) / dz;
+      const double S2 = du * du + dv * dv + 1e-8;
+      const double thvw = 0.5 * (thv[k - 1] + thv[k]);
+      const double N2 = kGrav / thvw * (thv[k] - thv[k - 1]) / dz;
+      const double Ri = N2 / S2;
+      const double zw = zeta_w[k + g.ng] * J;
+      const double l = kKappa * zw / (1 + kKappa * zw / 150);
+      if (Ri < 0) return l * l * std::sqrt(S2) * std::sqrt(1 - 16 * Ri);
+      const double d = 1 + 5 * Ri;
+      return l * l * std::sqrt(S2) / (d * d);
+    };
+    std::vector<double> fcg_th(nz + 1, 0.0), fcg_qv(nz + 1, 0.0);
+    Kw[0] = 0;
+    if (phys_nonlocal) {  // :311-370
+      const double ust = std::sqrt(Cd) * V1;
+      const double thv1 = thv[0];
+      const double wthv0 = shf * (1 + kEps61 * qvcol[0]) + kEps61 * thcol[0] * qflx;
+      const double Ric = 0.25;
+      double hpbl = zeta_c[(nz - 1) + g.ng] * J;
+      double ws = ust > 0.1 ? ust : 0.1;
+      for (int it = 0; it < 3; ++it) {
+        const double exc = (wthv0 > 0) ? 6.8 * wthv0 / ws : 0.0;
+        const double thv_s = thv1 + exc;
+        double prevRb = 0;
+        hpbl = zeta_c[(nz - 1) + g.ng] * J;
+        for (int k = 1; k < nz; ++k) {
+          const double zc = zeta_c[k + g.ng] * J;
+          const double shear = uc[k] * uc[k] + vc[k] * vc[k] + 0.1;
+          const double Rb = kGrav * (thv[k] - thv_s) * zc / (thv1 * shear);
+          if (Rb >= Ric) {
+            const double zcm = zeta_c[(k - 1) + g.ng] * J;
+            double f = (Ric - prevRb) / (Rb - prevRb + 1e-9);
+            f = std::fmin(1.0, std::fmax(0.0, f));
+            hpbl = zcm + f * (zc - zcm);
+            break;
+          }
+          prevRb = Rb;
+        }
+        const double zc0 = zeta_c[g.ng] * J;
+        if (hpbl < zc0) hpbl = zc0;
+        const double wstar = (wthv0 > 0) ? std::cbrt(kGrav / thv1 * wthv0 * hpbl) : 0.0;
+        ws = std::cbrt(ust * ust * ust + 0.6 * wstar * wstar * wstar);
+        if (ws < 0.1) ws = 0.1;
+      }
+      pblh[c2] = hpbl;
+      const double gth = (wthv0 > 0) ? 6.8 * shf / (ws * hpbl) : 0.0;
+      const double gqv = (wthv0 > 0) ? 6.8 * qflx / (ws * hpbl) : 0.0;
+      for (int k = 1; k < nz; ++k) {
+        const double zw = zeta_w[k + g.ng] * J;
+        double Kv;
+        if (wthv0 > 0 && zw < hpbl) {
+          const double zf = zw / hpbl;
+          Kv = kKappa * ws * zw * (1 - zf) * (1 - zf);
+          fcg_th[k] = rhow[k] * Kv * gth;
+          fcg_qv[k] = rhow[k] * Kv * gqv;
+        } else {
+          Kv = local_K(k);
+          fcg_th[k] = fcg_qv[k] = 0;
+        }
+        if (Kv < kKmMin) Kv = kKmMin;
+        if (Kv > kKmMax) Kv = kKmMax;
+        Kw[k] = Kv;
+        km[g.id(i, j, k)] = Kv;
+      }
+    } else {  // :371-382
+      pblh[c2] = 0;
+      for (int k = 1; k < nz; ++k) {
+        double Kv = local_K(k);
+        if (Kv < kKmMin) Kv = kKmMin;
+        if (Kv > kKmMax) Kv = kKmMax;
+        Kw[k] = Kv;
+        km[g.id(i, j, k)] = Kv;
+      }
+    }
+    // implicit scalar mixing (:385-386) and radiative heating (:389-397)
+    diffuse_column(nz, dt, rho.data(), rhow.data(), dzc.data(), dzw.data(), Kw.data(),
+                   rho[0] * shf, thcol.data(), fcg_th.data());
+    diffuse_column(nz, dt, rho.data(), rhow.data(), dzc.data(), dzw.data(), Kw.data(),
+                   rho[0] * qflx, qvcol.data(), fcg_qv.data());
+    for (int k = 0; k < nz; ++k) {
+      const long c = g.id(i, j, k);
+      double dth = dt * radht[k];
+      if (dth > 0.02) dth = 0.02;
+      if (dth < -0.02) dth = -0.02;
+      s.thp[c] = thcol[k] - thb[c] + dth;
+      const double q = qvcol[k];
+      s.qv[c] = q > 0 ? q : 0.0;
+    }
+  }
+  // k_sfc_u / k_sfc_v (surface.cu:390-437): face column (fi, fj) averaged
+  // from the cells (fi - di, fj - dj) and (fi, fj); surface drag + mixing.
+  void sfc_face_column(Vec& f, int fi, int fj, int di, int dj, double dt) {
+    const int nz = g.nz;
+    const long a2 = g.id2(fi - di, fj - dj), b2 = g.id2(fi, fj);
+    const double J = 0.5 * (jac[a2] + jac[b2]);
+    std::vector<double> rho(nz), rhow(nz), dzc(nz), dzw(nz), Kw(nz), phi(nz);
+    for (int k = 0; k < nz; ++k) {
+      const int kk = k + g.ng;
+      const long a = g.id(fi - di, fj - dj, k), c = g.id(fi, fj, k);
+      rho[k] = 0.5 * (rhob[a] + rhob[c]);
+      rhow[k] = 0.5 * (rhobw[a] + rhobw[c]);
+      dzc[k] = dzeta_c[kk] * J;
+      dzw[k] = dzeta_w[kk] * J;
+      Kw[k] = (k > 0) ? 0.5 * (km[a] + km[c]) : 0.0;
+      phi[k] = f[c];
+    }
+    const double cdva = 0.5 * (cdv[a2] + cdv[b2]);
+    const double src0 = -rho[0] * cdva * phi[0];
+    diffuse_column(nz, dt, rho.data(), rhow.data(), dzc.data(), dzw.data(), Kw.data(), src0,
+                   phi.data(), nullptr);
+    for (int k = 0; k < nz; ++k) f[g.id(fi, fj, k)] = phi[k];
+  }
+  // SfcPBL::step (surface.cu:499-514): t is the time at the end of the step.
+  void surface_step(double dt, double t, State& s) {
+    double utc = phys_start_hour + t / 3600;
+    utc = utc - 24 * std::floor(utc / 24);
+#pragma omp parallel for collapse(2)
+    for (int j = 0; j < g.ny; ++j)
+      for (int i = 0; i < g.nx; ++i) sfc_scalar_column(i, j, dt, utc, s);
+#pragma omp parallel for collapse(2)
+    for (int j = 0; j < g.ny; ++j)
+      for (int i = 1; i < g.nx; ++i) sfc_face_column(s.u, i, j, 1, 0, dt);
+#pragma omp parallel for collapse(2)
+    for (int j = 1; j < g.ny; ++j)
+      for (int i = 0; i < g.nx; ++i) sfc_face_column(s.v, i, j, 0, 1, dt);
+  }
+
   // Integrator::step (integrator.cpp:71-111).
-  void step(double dt) {
+  void step(double dt, double t = 0) {
     const double stage_dt[3] = {dt / 3, dt / 2, dt};
     const int ns = std::max(1, p.acoustic_ns);
     const int ns_stage[3] = {1, std::max(1, ns / 2), ns};
@@ -903,6 +1235,7 @@ struct orc_model {
     }
     sn.swap(sstage);
     if (p.moisture && p.microphysics) kessler(dt, sn);
+    if (phys_on) surface_step(dt, t + dt, sn);  // integrator.cpp:109
   }
 
   Vec* array(const std::string& n) {
@@ -922,7 +1255,10 @@ struct orc_model {
                     {"jac", &jac}, {"fcor", &fcor}, {"mapf", &mapf}, {"rain", &rain},
                     {"bdy_wgt", &bdy_wgt}, {"mfx", &mfx}, {"mfy", &mfy}, {"mfz", &mfz},
                     {"div", &div}, {"rtjx", &rtjx}, {"rtjy", &rtjy}, {"kt3", &kt3}, {"aw", &aw},
-                    {"piprev", &piprev}};
+                    {"piprev", &piprev}, {"tsk", &tsk}, {"land", &land}, {"lat", &lat},
+                    {"lon", &lon}, {"soilw", &soilw}, {"cdv", &cdv}, {"pblh", &pblh},
+                    {"t2m", &t2m}, {"u10", &u10}, {"km", &km}, {"soilt", &soilt},
+                    {"tdeep", &tdeep}};
     for (const NV& e : t)
       if (n == e.n) return e.v;
     for (int v = 0; v < 5; ++v) {
@@ -936,8 +1272,11 @@ struct orc_model {
         n == "dzeta_w")
       return g.NZ;
     if (n == "h" || n == "hx" || n == "hy" || n == "hx_u" || n == "hy_v" || n == "jac" ||
-        n == "fcor" || n == "mapf" || n == "rain" || n == "bdy_wgt")
+        n == "fcor" || n == "mapf" || n == "rain" || n == "bdy_wgt" || n == "tsk" ||
+        n == "land" || n == "lat" || n == "lon" || n == "soilw" || n == "cdv" || n == "pblh" ||
+        n == "t2m" || n == "u10" || n == "tdeep")
       return g.n2();
+    if (n == "soilt") return 4 * g.n2();
     return g.n3();
   }
 };
@@ -958,8 +1297,11 @@ orc_model* orc_create(int nx, int ny, int nz, int ng, double dx, double dy, doub
   for (Vec* v : {&m->ub, &m->vb, &m->zeta_w, &m->zeta_c, &m->dzeta_c, &m->dzeta_w})
     v->assign(n1, 0.0);
   for (Vec* v : {&m->h, &m->hx, &m->hy, &m->hxu, &m->hyv, &m->jac, &m->fcor, &m->mapf,
-                 &m->rain, &m->bdy_wgt})
+                 &m->rain, &m->bdy_wgt, &m->tsk, &m->tdeep, &m->land, &m->lat, &m->lon,
+                 &m->soilw, &m->cdv, &m->pblh, &m->t2m, &m->u10})
     v->assign(n2, 0.0);
+  m->km.assign(n3, 0.0);
+  m->soilt.assign(4 * n2, 0.0);
   m->sn.alloc(n3);
   m->sstage.alloc(n3);
   m->swork.alloc(n3);
@@ -1019,8 +1361,22 @@ int orc_set_bdy(orc_model* m, int active, double tfrac) {
   return 0;
 }
 
-int orc_step(orc_model* m, double dt, double /*t*/) {
-  m->step(dt);
+int orc_set_physics(orc_model* m, int nonlocal, double start_hour_utc, int doy) {
+  // SfcPBL::init (surface.cu:440-482): deep soil and the four soil layers
+  // start at the surface temperature
+  m->phys_on = 1;
+  m->phys_nonlocal = nonlocal;
+  m->phys_start_hour = start_hour_utc;
+  m->phys_doy = doy;
+  m->tdeep = m->tsk;
+  const long n2 = m->g.n2();
+  for (int L = 0; L < 4; ++L)
+    std::copy(m->tsk.begin(), m->tsk.end(), m->soilt.begin() + L * n2);
+  return 0;
+}
+
+int orc_step(orc_model* m, double dt, double t) {
+  m->step(dt, t);
   return 0;
 }
 
@@ -1094,6 +1450,14 @@ void orc_thomas(int n, const double* a, const double* b, const double* c, const 
 }
 
 double orc_qsat_w(double p, double T) { return qsat_tetens(p, T); }
+double orc_sfc_psi(int heat, double zeta) {
+  return heat ? orc_model::psi_h(zeta) : orc_model::psi_m(zeta);
+}
+void orc_sfc_diffuse(int nz, double dt, const double* rho, const double* rhow, const double* dzc,
+                     const double* dzw, const double* Kw, double src0, double* phi,
+                     const double* fcg) {
+  orc_model::diffuse_column(nz, dt, rho, rhow, dzc, dzw, Kw, src0, phi, fcg);
+}
 double orc_qsat_i(double p, double T) { return qsat_ice(p, T); }
 
 }  // extern "C"

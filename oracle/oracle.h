@@ -43,7 +43,13 @@ int orc_get_array(const orc_model* m, const char* name, double* data, long n);
 /* Integrator::init: acoustic coefficients (kernels.cu:915-927). */
 int orc_init(orc_model* m);
 int orc_set_bdy(orc_model* m, int active, double tfrac);
+/* Integrator::step (integrator.cpp:71-111); t is the time at the start of
+ * the step (the surface physics uses t + dt). */
 int orc_step(orc_model* m, double dt, double t);
+/* Integrator::set_physics + SfcPBL::init (surface.cu:440-482): the 2-D
+ * arrays tsk, land, lat, lon, soilw (padded, set with orc_set_array first)
+ * drive the surface/PBL column physics after every step. */
+int orc_set_physics(orc_model* m, int nonlocal, double start_hour_utc, int doy);
 int orc_apply_bcs(orc_model* m);
 int orc_tendencies(orc_model* m, double dt);
 int orc_acoustic(orc_model* m, double dtau, int nsub);
@@ -78,6 +84,14 @@ double orc_iface5(double qm2, double qm1, double q0, double q1, double q2, doubl
 void orc_thomas(int n, const double* a, const double* b, const double* c, const double* d,
                 double* x);
 double orc_qsat_w(double p, double T);
+/* Businger-Dyer stability function of zeta = z/L, momentum (heat = 0) or
+ * heat (heat = 1) (surface.cu:31-42). */
+double orc_sfc_psi(int heat, double zeta);
+/* Column-implicit vertical diffusion (surface.cu:61-84) in place of phi;
+ * fcg (nz + 1 entries) may be NULL. */
+void orc_sfc_diffuse(int nz, double dt, const double* rho, const double* rhow, const double* dzc,
+                     const double* dzw, const double* Kw, double src0, double* phi,
+                     const double* fcg);
 double orc_qsat_i(double p, double T);
 
 #ifdef __cplusplus

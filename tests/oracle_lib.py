@@ -42,6 +42,10 @@ def olib() -> C.CDLL:
             getattr(L, n).argtypes = [C.c_void_p]
         L.orc_set_bdy.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.orc_step.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.orc_set_physics.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int]
+        L.orc_sfc_psi.restype = C.c_double
+        L.orc_sfc_psi.argtypes = [C.c_int, C.c_double]
+        L.orc_sfc_diffuse.argtypes = [C.c_int, C.c_double] + [_DP] * 5 + [C.c_double, _DP, _DP]
         L.orc_tendencies.argtypes = [C.c_void_p, C.c_double]
         L.orc_acoustic.argtypes = [C.c_void_p, C.c_double, C.c_int]
         L.orc_kessler.argtypes = [C.c_void_p, C.c_double]
@@ -126,6 +130,11 @@ class OracleCase:
                 om.set(f"bdy_lo_{v}", self.array(f"bdy_lo_{n}"))
                 om.set(f"bdy_hi_{v}", self.array(f"bdy_hi_{n}"))
             om.set_bdy(True, 0.0)
+        if self.value("sfc_on"):
+            for n in ("tsk", "land", "lat", "lon", "soilw"):
+                om.set(n, self.array(f"sfc_{n}"))
+            olib().orc_set_physics(om._h, int(self.value("sfc_nonlocal")),
+                                   self.value("sfc_start_hour"), int(self.value("sfc_doy")))
         om.case = self
         return om
 
@@ -178,8 +187,11 @@ class OracleModel:
         g = self.grid
         if name in ("ub", "vb", "zeta_w", "zeta_c", "dzeta_c", "dzeta_w"):
             return g.NZ
-        if name in ("h", "hx", "hy", "hx_u", "hy_v", "jac", "fcor", "mapf", "rain", "bdy_wgt"):
+        if name in ("h", "hx", "hy", "hx_u", "hy_v", "jac", "fcor", "mapf", "rain", "bdy_wgt",
+                    "tsk", "land", "lat", "lon", "soilw", "cdv", "pblh", "t2m", "u10", "tdeep"):
             return g.NX * g.NY
+        if name == "soilt":
+            return 4 * g.NX * g.NY
         return g.npts
 
     def set(self, name: str, a) -> None:

@@ -63,6 +63,21 @@ def test_regional_cases(tmp_path, kw):
     _compare(Case(ini), OracleCase(ini), True)
 
 
+@pytest.mark.parametrize("pbl", ["local", "nonlocal"])
+def test_surface_physics_fields(tmp_path, pbl):
+    """physics = simple: the surface fields, the day of year and hour of the
+    input's start time and the boundary-layer scheme (src/main.cpp:364-377)."""
+    ini = write_lam_case(tmp_path / "case", nx=24, ny=18, nz=16, dx=12000.0, top=12000.0,
+                         seed=4, hours=0, extra={"physics": "simple", "pbl": pbl})
+    prod, orc = Case(ini), OracleCase(ini)
+    _compare(prod, orc, True)
+    for k in ("sfc_on", "sfc_doy", "sfc_start_hour", "sfc_nonlocal"):
+        assert prod.scalar(k) == orc.value(k), k
+    assert prod.scalar("sfc_doy") == 336 and prod.scalar("sfc_nonlocal") == (pbl == "nonlocal")
+    for n in ("tsk", "land", "lat", "lon", "soilw"):
+        assert prod.array(f"sfc_{n}").tobytes() == orc.array(f"sfc_{n}").tobytes(), n
+
+
 def test_full_regional_case(tmp_path):
     """The 330 x 220 x 45 regional case at 12 km with map factors 0.95-1.05."""
     ini = write_lam_case(tmp_path / "case", hours=3, bdy_interval_h=3)

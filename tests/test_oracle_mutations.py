@@ -20,7 +20,8 @@ import pytest
 
 ROOT = Path(__file__).resolve().parent.parent
 ORACLE_SRC = ROOT / "oracle" / "oracle.cpp"
-PINS = ["tests/test_oracle_closed_forms.py", "tests/test_oracle_pins.py", "tests/test_oracle.py"]
+PINS = ["tests/test_oracle_closed_forms.py", "tests/test_oracle_pins.py", "tests/test_oracle.py",
+        "tests/test_oracle_surface.py"]
 
 # name -> (exact text in oracle.cpp, replacement, occurrence to replace (0-based))
 MUTATIONS = {
@@ -38,6 +39,12 @@ MUTATIONS = {
     "fall_speed_density_ratio": ("std::sqrt(rho_sfc / rhol[k]) *\n                                    fallfac(c)",
                                  "std::sqrt(rhol[k] / rho_sfc) *\n                                    fallfac(c)", 0),
     "ice_fall_factor": ("T < kTfrz ? 0.35 : 1.0", "T < kTfrz ? 0.5 : 1.0", 0),
+    "psi_m_unstable_power": ("std::pow(1 - 16 * zeta, 0.25);", "std::pow(1 - 16 * zeta, 0.5);", 0),
+    "psi_h_stable_slope": ("if (zeta >= 0) return -5 * std::fmin(zeta, 2.0);\n    const double x = "
+                           "std::pow(1 - 16 * zeta, 0.25);\n    return 2 * std::log((1 + x * x)",
+                           "if (zeta >= 0) return -4 * std::fmin(zeta, 2.0);\n    const double x = "
+                           "std::pow(1 - 16 * zeta, 0.25);\n    return 2 * std::log((1 + x * x)", 0),
+    "diffusion_upper_conductance": ("      B[k] = m + a + c;", "      B[k] = m + a + 0.5 * c;", 0),
 }
 
 

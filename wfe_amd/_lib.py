@@ -21,7 +21,7 @@ WFE_PREC_MIXED = 1
 FIELDS = {
     "u": 0, "v": 1, "w": 2, "thp": 3, "pip": 4, "qv": 5, "qc": 6, "qr": 7,
     "tu": 8, "tv": 9, "tw": 10, "tthp": 11, "tpip": 12, "tqv": 13, "tqc": 14, "tqr": 15,
-    "rain": 16,
+    "rain": 16, "tsk": 17, "pblh": 18, "t2m": 19, "u10": 20,
 }
 STATE_FIELDS = ("u", "v", "w", "thp", "pip", "qv", "qc", "qr")
 
@@ -55,6 +55,11 @@ class wfe_metric(C.Structure):
     _fields_ = [(n, _DP) for n in METRIC_NAMES] + [("zt", C.c_double)]
 
 
+class wfe_sfc(C.Structure):
+    _fields_ = [("tsk", _DP), ("land", _DP), ("lat", _DP), ("lon", _DP), ("soilw", _DP),
+                ("start_hour_utc", C.c_double), ("doy", C.c_int), ("nonlocal", C.c_int)]
+
+
 class wfe_diag(C.Structure):
     _fields_ = [("wmax", C.c_double), ("thmin", C.c_double), ("thmax", C.c_double),
                 ("pipmax", C.c_double), ("qcmax", C.c_double), ("qrmax", C.c_double),
@@ -69,6 +74,7 @@ SIGNATURES = [
                              C.POINTER(wfe_metric), C.c_int, C.POINTER(C.c_void_p)]),
     ("wfe_destroy", C.c_int, [C.c_void_p]),
     ("wfe_step", C.c_int, [C.c_void_p, C.c_double, C.c_double]),
+    ("wfe_set_physics", C.c_int, [C.c_void_p, C.POINTER(wfe_sfc), C.c_size_t]),
     ("wfe_upload", C.c_int, [C.c_void_p, C.c_int, _DP, C.c_size_t]),
     ("wfe_download", C.c_int, [C.c_void_p, C.c_int, _DP, C.c_size_t]),
     ("wfe_absmax", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float)]),

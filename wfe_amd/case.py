@@ -154,7 +154,8 @@ class Integrator:
 
     # -- fields
     def get(self, name: str) -> np.ndarray:
-        n = self.grid.NX * self.grid.NY if name == "rain" else self.grid.npts
+        two_d = name in ("rain", "tsk", "pblh", "t2m", "u10")
+        n = self.grid.NX * self.grid.NY if two_d else self.grid.npts
         a = np.empty(n, dtype=np.float64)
         check(lib().wfe_download(self._h, FIELDS[name], as_dptr(a), n), f"download {name}")
         return a
@@ -182,7 +183,8 @@ class Integrator:
         """Interior of a state field (or the rain) as float32 in output order
         through wfe_pack_field_async + wfe_output_mark/wait."""
         g = self.grid
-        nlev = g.nz + 1 if name == "w" else (1 if name == "rain" else g.nz)
+        two_d = name in ("rain", "tsk", "pblh", "t2m", "u10")
+        nlev = g.nz + 1 if name == "w" else (1 if two_d else g.nz)
         a = np.empty(g.nx * g.ny * nlev, dtype=np.float32)
         check(lib().wfe_pack_field_async(self._h, FIELDS[name],
                                          a.ctypes.data_as(C.POINTER(C.c_float)), a.size),

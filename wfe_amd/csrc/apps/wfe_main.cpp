@@ -134,9 +134,9 @@ int main(int argc, char** argv) {
               g.ny, g.nz, g.dx, c.dt, c.t_end);
   std::printf("precision: %s\n", precision == WFE_PREC_FP64 ? "fp64" : "mixed (fp32 + fp64 pi')");
   for (const std::string& w : c.warnings) std::fprintf(stderr, "%s\n", w.c_str());
-  if (c.physics == "simple")
-    std::fprintf(stderr, "note: physics=simple (surface/PBL) is not part of this build; "
-                         "running dynamics + microphysics only\n");
+  if (c.sfc.on)
+    std::printf("fizik: yuzey + %s PBL + toprak + radyasyon (doy=%d, %02d UTC)\n",
+                c.sfc.nonlocal ? "nonlocal" : "yerel-K", c.sfc.doy, (int)c.sfc.start_hour);
 
   wfe_ctx* ctx = nullptr;
   wfe_check(wfe_case_create_ctx(cs, precision, &ctx), "context");
@@ -147,7 +147,7 @@ int main(int argc, char** argv) {
 
   Writer writer;
   try {
-    writer.init(g, c.out_dir, c.dt, c.dp.moisture, false);
+    writer.init(g, c.out_dir, c.dt, c.dp.moisture, c.sfc.on);
     std::vector<float> zc((size_t)g.nx * g.ny * g.nz);
     size_t o = 0;
     for (int k = 0; k < g.nz; ++k)
@@ -216,7 +216,7 @@ int main(int argc, char** argv) {
         }
       }
       if (step % out_steps == 0 || step == nsteps) {
-        writer.write(ctx, step, t, c.dp.moisture);
+        writer.write(ctx, step, t, true);
       }
     }
     wfe_check(wfe_sync(ctx), "sync");

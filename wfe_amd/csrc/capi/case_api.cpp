@@ -65,7 +65,9 @@ const std::vector<double>* case_array(const wfe::Case& c, const std::string& n) 
                   {"fcor", &m.fcor},     {"mapf", &m.mapf},       {"u", &c.ic[0]},
                   {"v", &c.ic[1]},       {"w", &c.ic[2]},         {"thp", &c.ic[3]},
                   {"pip", &c.ic[4]},     {"qv", &c.ic[5]},        {"qc", &c.ic[6]},
-                  {"qr", &c.ic[7]},      {"bdy_wgt", &c.bdy.wgt}};
+                  {"qr", &c.ic[7]},      {"bdy_wgt", &c.bdy.wgt},
+                  {"sfc_tsk", &c.sfc.tsk}, {"sfc_land", &c.sfc.land}, {"sfc_lat", &c.sfc.lat},
+                  {"sfc_lon", &c.sfc.lon}, {"sfc_soilw", &c.sfc.soilw}};
   for (const NV& e : t)
     if (n == e.n) return e.v;
   static const char* vn[5] = {"u", "v", "thp", "pip", "qv"};
@@ -147,6 +149,10 @@ int wfe_case_scalar(const wfe_case* c, const char* name, double* out) {
   else if (n == "out_interval") *out = c->c.out_interval;
   else if (n == "diag_interval") *out = c->c.diag_interval;
   else if (n == "zt") *out = c->c.metric.zt;
+  else if (n == "sfc_on") *out = c->c.sfc.on;
+  else if (n == "sfc_start_hour") *out = c->c.sfc.start_hour;
+  else if (n == "sfc_doy") *out = c->c.sfc.doy;
+  else if (n == "sfc_nonlocal") *out = c->c.sfc.nonlocal;
   else {
     wfe::set_last_error("unknown scalar: " + n);
     return WFE_E_ARG;
@@ -192,6 +198,11 @@ int wfe_case_create_ctx(const wfe_case* cs, int precision, wfe_ctx** out) {
       if (rc == WFE_OK)
         rc = wfe_set_bdy_target(*out, 1, v, c.bdy.hi[v].data(), c.bdy.hi[v].size());
     }
+  }
+  if (rc == WFE_OK && c.sfc.on) {
+    const wfe_sfc sf{c.sfc.tsk.data(), c.sfc.land.data(), c.sfc.lat.data(), c.sfc.lon.data(),
+                     c.sfc.soilw.data(), c.sfc.start_hour, c.sfc.doy, c.sfc.nonlocal ? 1 : 0};
+    rc = wfe_set_physics(*out, &sf, c.sfc.tsk.size());
   }
   if (rc != WFE_OK) {
     wfe_destroy(*out);

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <stdexcept>
 
 #include "core/case.hpp"
@@ -99,6 +100,29 @@ void Case::build() {
                       (unsigned)cfg.get_int("ens_seed", 0));
   bdy.init(g, dp, input, base.thb, base.pib, cfg.get_int("bdy_width", 8),
            cfg.get_real("bdy_tau", 600), cfg.get_real("nudge_tau", 0));
+  if (physics == "simple") {
+    // start = YYYYMMDDHH: day of year (no leap day) and UTC hour
+    const std::string& st = input.start;
+    if (st.size() < 10) throw std::runtime_error("physics=simple needs the input start time");
+    const int mo = std::atoi(st.substr(4, 2).c_str()), day = std::atoi(st.substr(6, 2).c_str());
+    if (mo < 1 || mo > 12) throw std::runtime_error("input start time has no valid month");
+    static const int kCumDays[12] = {0, 31, 59, 90, 120, 151, 181, 212, 243, 273, 304, 334};
+    sfc.on = true;
+    sfc.doy = kCumDays[mo - 1] + day;
+    sfc.start_hour = std::atoi(st.substr(8, 2).c_str());
+    sfc.nonlocal = cfg.get_str("pbl", "local") == "nonlocal";
+    auto padded = [&](const std::vector<double>& src) {
+      std::vector<double> h(g.n2(), 0.0);
+      for (int j = 0; j < g.ny; ++j)
+        for (int i = 0; i < g.nx; ++i) h[g.idx2(i, j)] = src[(size_t)j * g.nx + i];
+      return h;
+    };
+    sfc.tsk = padded(input.tsk);
+    sfc.land = padded(input.land);
+    sfc.lat = padded(input.lat);
+    sfc.lon = padded(input.lon);
+    sfc.soilw = padded(input.soilw);
+  }
 }
 
 }  // namespace wfe

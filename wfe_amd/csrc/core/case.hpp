@@ -49,7 +49,17 @@ struct Case {
   bool file_mode = false;
   double dt = 0.25, t_end = 1000, out_interval = 100, diag_interval = 10;
   std::string out_dir;
-  std::string physics;  // "none" | "simple" (surface physics is out of scope)
+  std::string physics;  // "none" | "simple" (surface/PBL physics, file mode only)
+  // Surface physics initial fields (SfcPBL::init, surface.cu:440-482):
+  // padded 2-D arrays with the input's interior and zero ghosts; the hour
+  // and day of year of the input's start time (src/main.cpp:364-377).
+  struct Surface {
+    bool on = false;
+    bool nonlocal = false;
+    double start_hour = 0;
+    int doy = 1;
+    std::vector<double> tsk, land, lat, lon, soilw;
+  } sfc;
   // Initial state, padded 3-D arrays: u v w thp pip qv qc qr.
   std::array<std::vector<double>, 8> ic;
   BoundaryHost bdy;

@@ -21,6 +21,7 @@
 #include "hip/slow_kernels.hpp"
 #include "hip/adjoint_kernels.hpp"
 #include "hip/output_kernels.hpp"
+#include "hip/surface_kernels.hpp"
 
 namespace wfe {
 
@@ -217,7 +218,10 @@ struct SoloBench {
 // Type-erased interface used by the C ABI.
 struct CtxBase {
   virtual ~CtxBase() = default;
-  virtual void step(double dt) = 0;
+  // One large step; t is the time at its start (the surface physics uses
+  // the hour of day at t + dt).
+  virtual void step(double dt, double t) = 0;
+  virtual void set_physics(const wfe_sfc& sfc) = 0;
   virtual void upload(int field, const double* h, size_t n) = 0;
   virtual void download(int field, double* h, size_t n) = 0;
   virtual float absmax(int field) = 0;
@@ -278,7 +282,8 @@ class Ctx final : public CtxBase {
       if (e) (void)hipEventDestroy(e);
     if (stream_) (void)hipStreamDestroy(stream_);
   }
-  void step(double dt) override;
+  void step(double dt, double t) override;
+  void set_physics(const wfe_sfc& sfc) override;
   void upload(int field, const double* h, size_t n) override;
   void download(int field, double* h, size_t n) override;
   float absmax(int field) override;
@@ -391,6 +396,15 @@ class Ctx final : public CtxBase {
   T* bdy_wgt_ = nullptr;
   double tfrac_ = 0;
   unsigned* d_absmax_ = nullptr;
+  // Surface/PBL column physics after every step (set_physics): FP64 fields
+  // and the hour of day at the end of the step as a device scalar, so a
+  // captured step graph stays valid.
+  bool phys_on_ = false;
+  double phys_start_hour_ = 0;
+  dev::SfcArgs sfc_{};
+  double* utc_dev_ = nullptr;
+  void surface_step(const StateSet<T>& s, double dt);
+  double* phys_field(int f) const;
   // Diagnostics partial records (one per workgroup of k_diag_partial) and
   // the final record after them.
   double* diag_part_ = nullptr;
@@ -734,6 +748,15 @@ void Ctx<T>::upload(int f, const double* h, size_t n) {
 
 template <class T>
 void Ctx<T>::download(int f, double* h, size_t n) {
+  if (f >= WFE_F_TSK && f <= WFE_F_U10) {
+    if (!phys_on_) throw ArgError("download: surface fields need the surface physics");
+    if (n != hn2_) throw ArgError("download: element count mismatch");
+    WFE_HIP(hipStreamSynchronize(stream_));
+    std::vector<double> tmp(n2_);
+    WFE_HIP(hipMemcpy(tmp.data(), phys_field(f), n2_ * sizeof(double), hipMemcpyDeviceToHost));
+    to_host(tmp, h, false);
+    return;
+  }
   const bool two_d = f == WFE_F_RAIN;
   if (n != (two_d ? hn2_ : hn3_)) throw ArgError("download: element count mismatch");
   WFE_HIP(hipStreamSynchronize(stream_));
@@ -788,11 +811,13 @@ void Ctx<T>::diagnose(double* rec) {
 template <class T>
 void Ctx<T>::pack_async(int f, float* dst, size_t n) {
   const bool is_state = f >= WFE_F_U && f <= WFE_F_QR;
-  if (!is_state && f != WFE_F_RAIN)
-    throw ArgError("pack: field must be a state field or the rain");
+  const bool is_sfc = f >= WFE_F_TSK && f <= WFE_F_U10;
+  if (!is_state && f != WFE_F_RAIN && !is_sfc)
+    throw ArgError("pack: field must be a state field, the rain or a surface field");
+  if (is_sfc && !phys_on_) throw ArgError("pack: surface fields need the surface physics");
   if (!p_.moisture && (f == WFE_F_QV || f == WFE_F_QC || f == WFE_F_QR || f == WFE_F_RAIN))
     throw ArgError("pack: moisture fields of a dry context");
-  const int nlev = f == WFE_F_W ? g_.nz + 1 : (f == WFE_F_RAIN ? 1 : g_.nz);
+  const int nlev = f == WFE_F_W ? g_.nz + 1 : (f == WFE_F_RAIN || is_sfc ? 1 : g_.nz);
   if (n != (size_t)g_.nx * g_.ny * nlev) throw ArgError("pack: element count mismatch");
   if (!ostream_) {
     WFE_HIP(hipStreamCreate(&ostream_));
@@ -811,6 +836,9 @@ void Ctx<T>::pack_async(int f, float* dst, size_t n) {
   const dim3 grd(cdiv(g_.nx, 64), cdiv(g_.ny, 4), (unsigned)nlev);
   if (f == WFE_F_PIP)
     dev::k_pack_interior<double><<<grd, dim3(64, 4), 0, stream_>>>(g_, S_[cur_].pip, 0,
+                                                                   out_stage_[f]);
+  else if (is_sfc)
+    dev::k_pack_interior<double><<<grd, dim3(64, 4), 0, stream_>>>(g_, phys_field(f), 1,
                                                                    out_stage_[f]);
   else
     dev::k_pack_interior<T><<<grd, dim3(64, 4), 0, stream_>>>(g_, field_ptr(f),
@@ -1353,7 +1381,13 @@ double Ctx<T>::kernel_bench(const std::string& name, int reps) {
 // time-n set S_[cur_], the stage estimate SE_, the next time level
 // S_[cur_^1], and ping-pong work buffers inside the acoustic loop.
 template <class T>
-void Ctx<T>::step(double dt) {
+void Ctx<T>::step(double dt, double t) {
+  if (phys_on_) {  // hour of day at the end of the step (surface.cu:501-502)
+    double utc = phys_start_hour_ + (t + dt) / 3600;
+    utc = utc - 24 * std::floor(utc / 24);
+    dev::k_set_scalar<double><<<1, 64, 0, stream_>>>(utc_dev_, utc);
+    check_launch("set_utc");
+  }
   // The first step tunes the substep method (synchronous timing), so it and
   // any step under per-kernel timing run eagerly.
   if (!use_graph_ || !tuned_ || timer_.on) {
@@ -1413,6 +1447,78 @@ void Ctx<T>::step_launches(double dt) {
   }
   cur_ ^= 1;
   if (p_.moisture && p_.microphysics) kessler_step(S_[cur_], dt);
+  if (phys_on_) surface_step(S_[cur_], dt);  // integrator.cpp:109
+}
+
+template <class T>
+void Ctx<T>::surface_step(const StateSet<T>& s, double dt) {
+  dev::SfcArgs a = sfc_;
+  a.dt = dt;
+  const dev::SfcState<T> st{s.thp, s.qv, s.u, s.v, s.qc, s.pip};
+  auto run = [&](auto kmax) {
+    constexpr int KMAX = decltype(kmax)::value;
+    TIMED("sfc_scalar", (dev::k_sfc_scalar<T, KMAX><<<dim3(cdiv(g_.nx, 64), (unsigned)g_.ny), 64,
+                                                       0, stream_>>>(g_, base_, met_, st, a)));
+    if (g_.nx > 1)
+      TIMED("sfc_u", (dev::k_sfc_face<T, KMAX, 0><<<dim3(cdiv(g_.nx - 1, 64), (unsigned)g_.ny), 64,
+                                                    0, stream_>>>(g_, base_, met_, s.u, a)));
+    if (g_.ny > 1)
+      TIMED("sfc_v", (dev::k_sfc_face<T, KMAX, 1><<<dim3(cdiv(g_.nx, 64), (unsigned)(g_.ny - 1)),
+                                                    64, 0, stream_>>>(g_, base_, met_, s.v, a)));
+  };
+  if (g_.nz <= 64)
+    run(std::integral_constant<int, 64>{});
+  else
+    run(std::integral_constant<int, 320>{});
+  check_launch("surface_step");
+}
+
+template <class T>
+double* Ctx<T>::phys_field(int f) const {
+  switch (f) {
+    case WFE_F_TSK: return sfc_.tsk;
+    case WFE_F_PBLH: return sfc_.pblh;
+    case WFE_F_T2M: return sfc_.t2m;
+    case WFE_F_U10: return sfc_.u10;
+    default: return nullptr;
+  }
+}
+
+// Integrator::set_physics + SfcPBL::init (surface.cu:440-482): the surface
+// fields of the input, deep soil and all four soil layers at the initial
+// surface temperature.
+template <class T>
+void Ctx<T>::set_physics(const wfe_sfc& sf) {
+  const double* need[] = {sf.tsk, sf.land, sf.lat, sf.lon, sf.soilw};
+  for (const double* q : need)
+    if (!q) throw ArgError("set_physics: missing surface field");
+  if (sf.doy < 1 || sf.doy > 366) throw ArgError("set_physics: day of year out of range");
+  WFE_HIP(hipStreamSynchronize(stream_));
+  if (!phys_on_) {
+    for (double** f : {&sfc_.tsk, &sfc_.tdeep, &sfc_.land, &sfc_.lat, &sfc_.lon, &sfc_.soilw,
+                       &sfc_.cdv, &sfc_.pblh, &sfc_.t2m, &sfc_.u10})
+      *f = arena_.alloc<double>(n2_);
+    sfc_.km = arena_.alloc<double>(npts_);
+    sfc_.soilt = arena_.alloc<double>(4 * n2_);
+    utc_dev_ = arena_.alloc<double>(1);
+    sfc_.utc = utc_dev_;
+  }
+  auto up = [&](double* d, const double* h) {
+    const std::vector<double> tmp = to_dev<double>(h, false);
+    WFE_HIP(hipMemcpy(d, tmp.data(), n2_ * sizeof(double), hipMemcpyHostToDevice));
+  };
+  up(sfc_.tsk, sf.tsk);
+  up(sfc_.tdeep, sf.tsk);
+  up(sfc_.land, sf.land);
+  up(sfc_.lat, sf.lat);
+  up(sfc_.lon, sf.lon);
+  up(sfc_.soilw, sf.soilw);
+  for (int L = 0; L < 4; ++L) up(sfc_.soilt + (size_t)L * n2_, sf.tsk);
+  sfc_.doy = sf.doy;
+  sfc_.nonlocal = sf.nonlocal ? 1 : 0;
+  phys_start_hour_ = sf.start_hour_utc;
+  phys_on_ = true;
+  ++gen_;
 }
 
 template <class T>
@@ -1506,6 +1612,7 @@ class Adjoint {
     else if (hp.diff_K != 0) why = "diffusion is on";
     else if (hp.w_damping) why = "w damping is on";
     else if (F.bdy_active_) why = "boundary relaxation is on";
+    else if (F.phys_on_) why = "the surface physics is on";
     else if (F.g_.nz + 1 > 64) why = "more than 63 levels";
     else if (std::max(1, hp.acoustic_ns) > kMaxSub) why = "more than 16 acoustic substeps";
     if (why.empty()) {
@@ -1994,9 +2101,21 @@ int wfe_destroy(wfe_ctx* ctx) {
   return WFE_OK;
 }
 
-int wfe_step(wfe_ctx* ctx, double dt, double /*t*/) {
+int wfe_set_physics(wfe_ctx* ctx, const wfe_sfc* sfc, size_t n2) {
   if (int e = need_ctx(ctx)) return e;
-  return guarded([&] { ctx->impl->step(dt); });
+  if (!sfc) {
+    wfe::set_last_error("wfe_set_physics: null argument");
+    return WFE_E_ARG;
+  }
+  return guarded([&] {
+    check_n2(ctx, n2);
+    ctx->impl->set_physics(*sfc);
+  });
+}
+
+int wfe_step(wfe_ctx* ctx, double dt, double t) {
+  if (int e = need_ctx(ctx)) return e;
+  return guarded([&] { ctx->impl->step(dt, t); });
 }
 
 // ------------------------------------------------------------ adjoint
@@ -2078,7 +2197,7 @@ void ensemble_step(wfe_ensemble& E, const double* dt) {
     if (c->impl->parity() != par) throw wfe::ArgError("ensemble members out of step");
   if (!ready) {
     // First step of a member tunes it (synchronous timing): run eagerly.
-    for (size_t q = 0; q < n; ++q) E.m[q]->impl->step(dt[q]);
+    for (size_t q = 0; q < n; ++q) E.m[q]->impl->step(dt[q], 0.0);
     return;
   }
   std::vector<unsigned> gen(n);

@@ -34,6 +34,7 @@ void Writer::init(const GDims& g, const std::string& out_dir, double dt, bool mo
   g_ = g;
   dir_ = out_dir;
   moist_ = moisture;
+  sfc_ = has_tsk;
   std::filesystem::create_directories(dir_);
   const std::string meta = dir_ + "/meta.json";
   std::FILE* f = std::fopen(meta.c_str(), "w");
@@ -66,7 +67,7 @@ void Writer::init(const GDims& g, const std::string& out_dir, double dt, bool mo
   }
 }
 
-std::vector<Writer::Item> Writer::items_for(bool with_rain) const {
+std::vector<Writer::Item> Writer::items_for(bool with_2d) const {
   const size_t n2 = (size_t)g_.nx * g_.ny;
   const size_t n3 = n2 * g_.nz;
   std::vector<Item> it = {{"u", WFE_F_U, n3},
@@ -78,7 +79,13 @@ std::vector<Writer::Item> Writer::items_for(bool with_rain) const {
     it.push_back({"qv", WFE_F_QV, n3});
     it.push_back({"qc", WFE_F_QC, n3});
     it.push_back({"qr", WFE_F_QR, n3});
-    if (with_rain) it.push_back({"rain", WFE_F_RAIN, n2});
+    if (with_2d) it.push_back({"rain", WFE_F_RAIN, n2});
+  }
+  if (with_2d && sfc_) {  // src/main.cpp:460-465
+    it.push_back({"tsk", WFE_F_TSK, n2});
+    it.push_back({"pblh", WFE_F_PBLH, n2});
+    it.push_back({"t2m", WFE_F_T2M, n2});
+    it.push_back({"u10", WFE_F_U10, n2});
   }
   return it;
 }
@@ -101,7 +108,7 @@ void Writer::write_raw(const std::string& path, const float* data, size_t n) {
 // interior packed on the host.
 void Writer::write_sync(wfe_ctx* ctx, int step, const std::vector<Item>& items) {
   for (const Item& it : items) {
-    const bool two_d = it.field == WFE_F_RAIN;
+    const bool two_d = it.field >= WFE_F_RAIN;
     const size_t nfull = two_d ? g_.n2() : g_.npts();
     if (wfe_download(ctx, it.field, full_.data(), nfull) != WFE_OK)
       throw std::runtime_error(std::string("download failed: ") + wfe_last_error());
@@ -123,8 +130,8 @@ void Writer::rethrow_locked() {
   }
 }
 
-void Writer::write(wfe_ctx* ctx, int step, double t, bool with_rain) {
-  const std::vector<Item> items = items_for(with_rain);
+void Writer::write(wfe_ctx* ctx, int step, double t, bool with_2d) {
+  const std::vector<Item> items = items_for(with_2d);
   if (!async_) {
     write_sync(ctx, step, items);
   } else {

@@ -39,8 +39,9 @@ class Writer {
 
   // Throws std::runtime_error when meta.json cannot be written.
   void init(const GDims& g, const std::string& out_dir, double dt, bool moisture, bool has_tsk);
-  // The state snapshot of a step; with_rain adds rain_<step>.bin (moist runs).
-  void write(wfe_ctx* ctx, int step, double t, bool with_rain = false);
+  // The state snapshot of a step; with_2d adds the 2-D fields of an output
+  // step: rain (moist runs) and tsk, pblh, t2m, u10 (surface physics).
+  void write(wfe_ctx* ctx, int step, double t, bool with_2d = false);
   void write_static(const char* name, const std::vector<float>& data);
   // Waits until every queued snapshot is on disk; rethrows a writer error.
   void finish();
@@ -60,7 +61,7 @@ class Writer {
     long long ticket;
     std::vector<Item> items;
   };
-  std::vector<Item> items_for(bool with_rain) const;
+  std::vector<Item> items_for(bool with_2d) const;
   void write_sync(wfe_ctx* ctx, int step, const std::vector<Item>& items);
   void write_raw(const std::string& path, const float* data, size_t n);
   std::string path_of(const std::string& name, int step) const;
@@ -70,6 +71,7 @@ class Writer {
   GDims g_{};
   std::string dir_;
   bool moist_ = false;
+  bool sfc_ = false;
   bool async_ = true;
   std::vector<double> full_;
   std::vector<float> out_;

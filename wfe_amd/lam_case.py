@@ -28,6 +28,9 @@ The state is synthetic and seeded:
   pi        hydrostatic per column from the perturbed virtual temperature;
   members   optional white theta noise N(0, low_noise_K) below low_noise_top
             above ground (ensemble members);
+  surface   land where the terrain exceeds 2 % of h_max (sea at 290 K
+            elsewhere), skin temperature from a 6.5 K/km lapse, soil moisture
+            0.12..0.35 west to east (used with physics = simple);
   boundary  the initial state at hour 0; at later boundary times the same
             seeded state with its perturbation fields carried east by
             bdy_shift_cells_per_h grid cells per hour (pi recomputed
@@ -139,9 +142,11 @@ def write_lam_case(out_dir, *, nx=330, ny=220, nz=45, dx=12000.0, seed=20261202,
     lat, lon = lambert_grid_latlon(nx, ny, dx, lat0, 10.0, proj_lat1, proj_lat2)
     fcor = 2 * OMEGA * np.sin(np.radians(lat))
     h = spectral_terrain(rng, nx, ny, dx, h_max, max_slope)
-    tsk = 288.15 - 0.0065 * h
-    land = np.ones((ny, nx))
-    soilw = np.full((ny, nx), 0.25)
+    # land where the terrain rises above 2 % of its maximum, sea elsewhere
+    # (a sea surface at 290 K); a soil-moisture gradient from west to east
+    land = (h > 0.02 * h_max).astype(np.float64)
+    tsk = np.where(land > 0.5, 288.15 - 0.0065 * h, 290.0)
+    soilw = np.broadcast_to(np.linspace(0.12, 0.35, nx)[None, :], (ny, nx)).copy()
 
     # physical heights of the cell centres (Gal-Chen)
     z3 = h[None] + zc[:, None, None] * (zt - h[None]) / zt
