@@ -204,7 +204,10 @@ def error_growth(case, checkpoints):
     (b) the FP64 path started from the initial state rounded to FP32 (the
     round-off the mixed path starts from) and (c) the FP64 path started from
     the initial state times (1 + 1e-6 N(0,1)); (b) and (c) measure how fast
-    the flow itself amplifies perturbations of those sizes."""
+    the flow itself amplifies perturbations of those sizes; (d) the FP64
+    path with its prognostic state rounded to FP32 after every step: FP64
+    arithmetic with FP32 storage, the error floor that storing the state in
+    FP32 sets whatever the arithmetic precision."""
     from wfe_amd import Integrator
 
     g = case.grid
@@ -213,13 +216,16 @@ def error_growth(case, checkpoints):
     mix = Integrator.from_case(case, "mixed")
     per = Integrator.from_case(case, "fp64")
     per6 = Integrator.from_case(case, "fp64")
+    sto = Integrator.from_case(case, "fp64")
+    # pi' keeps FP64 storage in the mixed path as well
+    stored32 = ("u", "v", "w", "thp", "qv", "qc", "qr")
     rng = np.random.default_rng(20261202)
     for f in ("u", "v", "w", "thp", "pip", "qv", "qc", "qr"):
         a = ref.get(f)
         per.set(f, a.astype(np.float32).astype(np.float64))
         per6.set(f, a * (1.0 + 1e-6 * rng.standard_normal(a.shape)))
     runs = (("mixed_vs_fp64", mix), ("fp64_from_fp32_ic_vs_fp64", per),
-            ("fp64_ic_perturbed_1e-6_vs_fp64", per6))
+            ("fp64_ic_perturbed_1e-6_vs_fp64", per6), ("fp64_fp32_storage_vs_fp64", sto))
     out = {"steps": []}
     fields = ("u", "v", "w", "thp")
     for key, _ in runs:
@@ -227,8 +233,10 @@ def error_growth(case, checkpoints):
     n = 0
     for target in checkpoints:
         while n < target:
-            for gi in (ref, mix, per, per6):
+            for gi in (ref, mix, per, per6, sto):
                 gi.step(dt, n * dt)
+            for f in stored32:
+                sto.set(f, sto.get(f).astype(np.float32).astype(np.float64))
             n += 1
         out["steps"].append(n)
         for f in fields:
