@@ -46,6 +46,9 @@ enum {
 /* Field selectors for upload/download/absmax. */
 enum {
   WFE_F_U = 0, WFE_F_V, WFE_F_W, WFE_F_THP, WFE_F_PIP, WFE_F_QV, WFE_F_QC, WFE_F_QR,
+  /* Slow tendencies. WFE_F_TPIP is the Davies rim relaxation term of pi'
+   * (zero elsewhere, kernels.cu:901,981-995): the substeps read it only in
+   * columns with a positive rim weight. */
   WFE_F_TU = 8, WFE_F_TV, WFE_F_TW, WFE_F_TTHP, WFE_F_TPIP, WFE_F_TQV, WFE_F_TQC, WFE_F_TQR,
   WFE_F_RAIN = 16, /* 2-D accumulated surface rain [kg m-2] */
   /* 2-D surface-physics fields (after wfe_set_physics): skin temperature

@@ -318,6 +318,10 @@ struct WPArgs {
   CArr<T> tw;   // w tendency including the moist buoyancy (see k_tendencies)
   CArr<T> tth;
   CArr<T> tpi;  // nullptr when the pi' slow tendency is zero
+  // Davies rim weight (2-D) when tpi is set: the pi' tendency is the rim
+  // relaxation term alone, zero where the weight is zero, so it is read only
+  // in columns with a positive weight.
+  CArr<T> tpw;
   CArr<T> qv;
   double dtau;
 };
@@ -387,7 +391,7 @@ __global__ void __launch_bounds__(64) k_acou_wpi_col(Grid g, Params p, Base<T> b
     const double dzc = (double)m.dzeta_c[k + g.ng];
     const double wk = (double)a.w_src[c];
     const double wkp = (double)a.w_src[c + PL];
-    const double tpi = a.tpi ? (double)a.tpi[c] : 0.0;
+    const double tpi = a.tpi && a.tpw[c2d] > (T)0 ? (double)a.tpi[c] : 0.0;
     P[k] = a.pip_src[c] + dtau * tpi -
            dtau * Kt * (Dh - (sv_hi - sv_lo) / dzc + c2 * (av_hi * wkp - av_lo * wk) / dzc);
     E[k] = dtau * Kt * c1 / dzc;
@@ -756,6 +760,7 @@ __global__ void __launch_bounds__(TX * NW, WPE) k_acou_wpi_tile(Grid g, Params p
   const double rJ = rcp_nr(J);
   // Over flat ground the cross-metric flux S_v is exactly zero.
   const bool flat = hxc == 0.0 && hyc == 0.0;
+  const bool rim = TPI && a.tpw[c2d] > (T)0;  // columns with a pi' tendency
   const int c0 = g.id(col ? i : 0, j, 0);
   // Base-state and coefficient column: column (0, 0) when every column holds
   // the same profile (one cache line per level for the whole grid).
@@ -805,7 +810,7 @@ __global__ void __launch_bounds__(TX * NW, WPE) k_acou_wpi_tile(Grid g, Params p
     const double avk = (double)awk, avk1 = (double)awk1;
     const double wk = (double)a.w_src[c];
     const double wkp = (double)a.w_src[c + PL];
-    const double tpi = TPI ? (double)a.tpi[c] : 0.0;
+    const double tpi = rim ? (double)a.tpi[c] : 0.0;
     const double pk = a.pip_src[c];
     const double pkm = a.pip_src[cm];
     const T thkT = a.thp_src[c];
@@ -938,6 +943,7 @@ struct FusedArgs {
   CArr<T> tw;
   CArr<T> tth;
   CArr<T> tpi;
+  CArr<T> tpw;  // rim weight gating the tpi reads (as WPArgs::tpw)
 };
 
 template <int TX, int NW, int KPT>
@@ -1143,6 +1149,7 @@ __global__ void __launch_bounds__(TX * NW, WPE) k_acou_row(Grid g, Params p, Bas
   const double mc = (double)m.mapf[c2d];
   const double rJ = rcp_nr(J);
   const bool flat = hxc == 0.0 && hyc == 0.0;
+  const bool rim = TPI && a.tpw[c2d] > (T)0;  // columns with a pi' tendency
   const int c0 = g.id(col ? ci : 0, j, 0);
   const int cb = p.hom ? chom : c0;
   const int ox = p.hom ? 0 : 1, oy = p.hom ? 0 : NX;
@@ -1182,7 +1189,7 @@ __global__ void __launch_bounds__(TX * NW, WPE) k_acou_row(Grid g, Params p, Bas
     const double avk = (double)awk, avk1 = (double)awk1;
     const double wk = (double)a.w_src[c];
     const double wkp = (double)a.w_src[c + PL];
-    const double tpi = TPI ? (double)a.tpi[c] : 0.0;
+    const double tpi = rim ? (double)a.tpi[c] : 0.0;
     const double pk = q.pcur[c];
     const double pkm = q.pcur[cm];
     const double thk = (double)Th[t];

@@ -1037,6 +1037,7 @@ dev::WPArgs<T> split_wp_impl(const dev::FusedArgs<T>& a) {
   w.tw = a.tw;
   w.tth = a.tth;
   w.tpi = a.tpi;
+  w.tpw = a.tpw;
   w.qv = a.uv.qv;
   w.dtau = (double)a.uv.dtau;
   return w;
@@ -1269,6 +1270,7 @@ void Ctx<T>::acoustic_stage(const StateSet<T>& out, double dtau, int nsub) {
     a.tw = tw_sub_ ? tw_sub_ : tw_;
     a.tth = tth_;
     a.tpi = bdy_active_ ? tpi_ : nullptr;
+    a.tpw = bdy_wgt_;
     methods_[method_].run(a);
   }
   check_launch("acoustic_stage");
@@ -1361,6 +1363,7 @@ double Ctx<T>::kernel_bench(const std::string& name, int reps) {
     a.tw = tw_;
     a.tth = tth_;
     a.tpi = bdy_active_ ? tpi_ : nullptr;
+    a.tpw = bdy_wgt_;
     methods_[method_].run(a);
   } else if (name == "fill_ghosts") {
     fill_ghosts(sn);

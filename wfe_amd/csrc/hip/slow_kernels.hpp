@@ -536,7 +536,9 @@ __global__ void k_pd_apply(Grid g, Params p, Base<T> b, Metric<T> m, const T* __
 // ------------------------------------------------- boundary relaxation
 // Davies rim relaxation toward the time-interpolated targets
 // (k_bdy_relax kernels.cu:292-304): T += w(i,j) [(1-tf) lo + tf hi - f] for
-// u, v, theta', qv; the pi' tendency is set (it is zero otherwise).
+// u, v, theta', qv; the pi' tendency is set in the rim. Outside the rim
+// (w = 0) nothing is written: there the pi' tendency is zero by definition
+// and the substep kernels do not read it.
 template <class T>
 struct RelaxSet {
   const T* f[4];      // u v thp qv (estimate state)
@@ -557,10 +559,7 @@ __global__ void k_bdy_relax(Grid g, RelaxSet<T> R) {
   const int c = g.id(i, j, k);
   const T w = R.wgt[g.id2(i, j)];
   const T tf = *R.tf;
-  if (w <= (T)0) {
-    R.t[3][c] = (T)0;
-    return;
-  }
+  if (w <= (T)0) return;
   const int fi[4] = {0, 1, 2, 4};
   for (int q = 0; q < 4; ++q) {
     const int v = fi[q];
