@@ -173,15 +173,16 @@ __global__ void k_acou_uv(Grid g, Params p, Base<T> b, Metric<T> m, UVArgs<T> a)
   }
 }
 
-// LDS-tiled form of k_acou_uv: a 64 x 4 block of faces at one level stages
-// pi* (at k and, over terrain, k +- 1) and theta_v once per cell (block +
-// one-cell west/south halo), instead of recomputing them for every face that
-// touches the cell. The radiation edges and the periodic wrap u(nx) = u(0)
-// go through the direct UVDev path, which evaluates the same expressions.
-template <class T>
+// LDS-tiled form of k_acou_uv: a BX x BY block of faces (256 threads, BX =
+// 64 or 32) at one level stages pi* (at k and, over terrain, k +- 1) and
+// theta_v once per cell (block + one-cell west/south halo), instead of
+// recomputing them for every face that touches the cell. The radiation
+// edges and the periodic wrap u(nx) = u(0) go through the direct UVDev
+// path, which evaluates the same expressions.
+template <class T, int BX = 64>
 __global__ void __launch_bounds__(256) k_acou_uv_tile(Grid g, Params p, Base<T> b, Metric<T> m,
                                                       UVArgs<T> a) {
-  constexpr int BX = 64, BY = 4;
+  constexpr int BY = 256 / BX;
   __shared__ double sP[3][BY + 1][BX + 1];  // pi* at k-1, k, k+1
   __shared__ T sT[BY + 1][BX + 1];
   const int tx = threadIdx.x, ty = threadIdx.y;
@@ -455,8 +456,10 @@ inline constexpr int kTwistChunk = 4;
 inline constexpr int kTwistPad = 2 * kTwistChunk;
 
 // Tiled, k-parallel form of the same column solve. A workgroup owns TX
-// consecutive columns of one row j and NW thread rows; thread row r handles
-// the levels k = r, r + NW, ... . Phases, separated by barriers:
+// consecutive interior columns in row-major order (column q = j nx + i, so a
+// tile may continue on the next row and no lane idles at the end of a row)
+// and NW thread rows; thread row r handles the levels k = r, r + NW, ... .
+// Phases, separated by barriers:
 //   1. every thread assembles, for its levels, P_k, E_k, Av_k, G_k, RHS_k
 //      in registers (all loads of the column are issued here, in parallel
 //      over k) and publishes P_k, E_k, Av_k in the D, C, B rows of LDS;
@@ -745,9 +748,10 @@ __global__ void __launch_bounds__(TX * NW, WPE) k_acou_wpi_tile(Grid g, Params p
 
   const int lx = threadIdx.x, ry = threadIdx.y;
   const BlockId bid = xcd_block();
-  const int i = bid.x * TX + lx;
-  const int j = bid.y;
-  const bool col = i < g.nx;
+  const int q = bid.x * TX + lx;  // interior column j nx + i
+  const bool col = q < g.nx * g.ny;
+  const int j = col ? q / g.nx : 0;
+  const int i = col ? q - j * g.nx : 0;
   const double c1 = (1.0 + p.beta) * 0.5;
   const double c2 = (1.0 - p.beta) * 0.5;
   const double dtau = a.dtau;

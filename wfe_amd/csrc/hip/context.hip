@@ -349,6 +349,22 @@ class Ctx final : public CtxBase {
   void kessler_step(const StateSet<T>& s, double dt);
   T* field_ptr(int field);
   dim3 blk3() const { return dim3(64, 4, 1); }
+  void launch_uv_tile(const dev::UVArgs<T>& uv) {
+    const dim3 guv = lv_grid(g_.nx + 1, g_.ny + 1, g_.nz);
+    if (lv_bx_ == 32)
+      TIMED("acou_uv", (dev::k_acou_uv_tile<T, 32><<<guv, lv_block(), 0, stream_>>>(
+                           g_, p_, base_, met_, uv)));
+    else
+      TIMED("acou_uv", (dev::k_acou_uv_tile<T, 64><<<guv, lv_block(), 0, stream_>>>(
+                           g_, p_, base_, met_, uv)));
+  }
+  // Workgroups of the step's per-level kernels: 256 threads, lv_bx_ wide
+  // (64, or 32 where 32-wide tiles leave fewer idle lanes along x).
+  int lv_bx_ = 64;
+  dim3 lv_block() const { return dim3(lv_bx_, 256 / lv_bx_, 1); }
+  dim3 lv_grid(int ni, int nj, int nk) const {
+    return dim3(cdiv(ni, lv_bx_), cdiv(nj, 256 / lv_bx_), (unsigned)nk);
+  }
 
   dev::Grid g_{};
   dev::Params p_{};
@@ -540,6 +556,8 @@ Ctx<T>::Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& b, const w
     if (v != 1 && v != 2 && v != 4) throw ArgError("WFE_TENDENCY_LEVELS must be 1, 2 or 4");
     tend_levels_ = v;
   }
+  lv_bx_ = cdiv(gr.nx, 32) * 32 < cdiv(gr.nx, 64) * 64 ? 32 : 64;
+  tend_tx_ = lv_bx_;
   if (const char* e = std::getenv("WFE_TENDENCY_TILE")) {
     const int v = std::atoi(e);
     if (v != 64 && v != 32) throw ArgError("WFE_TENDENCY_TILE must be 64 or 32");
@@ -940,7 +958,8 @@ template <class T>
 void Ctx<T>::slow_tendencies(const StateSet<T>& s, double dt, const StateSet<T>* qout,
                              double dt_stage, bool fold_buoyancy) {
   {
-    dim3 grd(cdiv(g_.nx + 5, 64), cdiv(g_.ny + 5, 4), (unsigned)g_.nz);
+    // 64 x 4 whatever lv_bx_ (32-wide tiles measured slower for this kernel)
+    const dim3 grd(cdiv(g_.nx + 5, 64), cdiv(g_.ny + 5, 4), (unsigned)g_.nz);
     TIMED("massflux", dev::k_massflux<T><<<grd, blk3(), 0, stream_>>>(g_, base_, met_, s.u, s.v, s.w, mfx_, mfy_,
                                                     mfz_, div_));
   }
@@ -988,11 +1007,11 @@ void Ctx<T>::slow_tendencies(const StateSet<T>& s, double dt, const StateSet<T>*
       S.ratio[q3] = pratio_[q3];
       S.tq[q3] = tq_[q3];
     }
-    dim3 gi(cdiv(g_.nx, 64), cdiv(g_.ny, 4), (unsigned)g_.nz);
-    TIMED("pd_ratio", dev::k_pd_ratio<T><<<gi, blk3(), 0, stream_>>>(g_, base_, met_, (T)dt, mfx_,
-                                                                    mfy_, mfz_, S));
-    TIMED("pd_apply", dev::k_pd_apply<T><<<gi, blk3(), 0, stream_>>>(g_, p_, base_, met_, mfx_,
-                                                                    mfy_, mfz_, div_, S, qs));
+    const dim3 gi = lv_grid(g_.nx, g_.ny, g_.nz);
+    TIMED("pd_ratio", dev::k_pd_ratio<T><<<gi, lv_block(), 0, stream_>>>(
+                          g_, base_, met_, (T)dt, mfx_, mfy_, mfz_, S));
+    TIMED("pd_apply", dev::k_pd_apply<T><<<gi, lv_block(), 0, stream_>>>(
+                          g_, p_, base_, met_, mfx_, mfy_, mfz_, div_, S, qs));
   }
   check_launch("slow_tendencies");
 }
@@ -1016,8 +1035,8 @@ void Ctx<T>::bdy_relax(const StateSet<T>& s) {
   R.t[4] = tq_[0];
   R.wgt = bdy_wgt_;
   R.tf = tf_dev_;
-  dim3 grd(cdiv(g_.nx, 64), cdiv(g_.ny, 4), (unsigned)g_.nz);
-  TIMED("bdy_relax", dev::k_bdy_relax<T><<<grd, blk3(), 0, stream_>>>(g_, R));
+  const dim3 grd = lv_grid(g_.nx, g_.ny, g_.nz);
+  TIMED("bdy_relax", dev::k_bdy_relax<T><<<grd, lv_block(), 0, stream_>>>(g_, R));
   check_launch("bdy_relax");
 }
 
@@ -1063,11 +1082,9 @@ void Ctx<T>::add_split_tile(const char* name, size_t cap) {
       }
   }
   methods_.push_back({name, [this, lb](const dev::FusedArgs<T>& a) {
-                        const dim3 guv(cdiv(g_.nx + 1, 64), cdiv(g_.ny + 1, 4), (unsigned)g_.nz);
-                        TIMED("acou_uv", dev::k_acou_uv_tile<T><<<guv, blk3(), 0, stream_>>>(
-                                             g_, p_, base_, met_, a.uv));
+                        launch_uv_tile(a.uv);
                         const dev::WPArgs<T> w = split_wp_impl<T>(a);
-                        const dim3 gt(cdiv(g_.nx, TX), (unsigned)g_.ny);
+                        const dim3 gt(cdiv(g_.nx * g_.ny, TX));
                         const dim3 bt(TX, NW);
                         const bool moist = p_.moisture, tpi = w.tpi != nullptr;
                         if (moist && tpi)
@@ -1128,9 +1145,7 @@ void Ctx<T>::build_methods(const hipDeviceProp_t& prop) {
   const size_t cap = (size_t)prop.sharedMemPerBlock;
   // General split pair: one thread per column (any nz <= 319).
   methods_.push_back({"split-col", [this](const dev::FusedArgs<T>& a) {
-                        const dim3 guv(cdiv(g_.nx + 1, 64), cdiv(g_.ny + 1, 4), (unsigned)g_.nz);
-                        TIMED("acou_uv", dev::k_acou_uv_tile<T><<<guv, blk3(), 0, stream_>>>(
-                                             g_, p_, base_, met_, a.uv));
+                        launch_uv_tile(a.uv);
                         const dev::WPArgs<T> w = split_wp_impl<T>(a);
                         const dim3 gcol(cdiv(g_.nx, 64), (unsigned)g_.ny);
                         if (g_.nz + 1 <= 64)
@@ -1279,8 +1294,8 @@ void Ctx<T>::acoustic_stage(const StateSet<T>& out, double dtau, int nsub) {
 template <class T>
 void Ctx<T>::stage_moisture(const StateSet<T>& out, double dt_stage) {
   const StateSet<T>& sn = S_[cur_];
-  const dim3 grd(cdiv(g_.nx, 64), cdiv(g_.ny, 4), (unsigned)g_.nz);
-  TIMED("stage_q", dev::k_stage_q<T><<<grd, blk3(), 0, stream_>>>(g_, (T)dt_stage, sn.qv, sn.qc, sn.qr,
+  const dim3 grd = lv_grid(g_.nx, g_.ny, g_.nz);
+  TIMED("stage_q", dev::k_stage_q<T><<<grd, lv_block(), 0, stream_>>>(g_, (T)dt_stage, sn.qv, sn.qc, sn.qr,
                                                        tq_[0], tq_[1], tq_[2], out.qv, out.qc,
                                                        out.qr));
   check_launch("stage_moisture");
