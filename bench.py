@@ -166,14 +166,22 @@ def kernel_profile(case, precision: str, warmup: int, steps: int, solo_reps: int
     return rep, tune, solo
 
 
-def pmc_traffic(kernel: str, which: str):
+def kernel_symbol(kernel: str, method: str | None) -> str:
+    """Kernel symbol of a timing name; the fused substep is k_acou_tile2d
+    for the 2-D tile methods and k_acou_row for the row methods."""
+    if kernel == "acoustic_fused" and method and method.startswith("tile2d"):
+        return "k_acou_tile2d"
+    return KERNEL_SYMBOL[kernel]
+
+
+def pmc_traffic(kernel: str, which: str, method: str | None = None):
     """HBM-side bytes per launch of `kernel` (mixed build) from the committed
     rocprofv3 PMC summary of the workload, or None."""
     path = PMC_SUMMARY[which]
     if not path.exists():
         return None
     d = json.loads(path.read_text())
-    sym = KERNEL_SYMBOL.get(kernel)
+    sym = kernel_symbol(kernel, method) if kernel in KERNEL_SYMBOL else None
     cands = [(v.get("calls", 0), v) for n, v in d.get("kernels", {}).items()
              if sym and n.startswith(sym + "<float")]
     if not cands:
@@ -184,7 +192,7 @@ def pmc_traffic(kernel: str, which: str):
     return v["fetch_bytes"] + v["write_bytes"]
 
 
-def substep_roofline(rep, solo, N: int, steps: int, which: str):
+def substep_roofline(rep, solo, N: int, steps: int, which: str, method: str | None = None):
     """Roofline of the acoustic substep, the step's dominant work: the
     substep's algorithmic bytes per interior point (moist, mixed: 124 B) x N
     over the average duration of one substep's launches inside the timed
@@ -204,13 +212,13 @@ def substep_roofline(rep, solo, N: int, steps: int, which: str):
         return None
     solo_us = sum(solo[k] for k in kernels)
     alg = SUBSTEP_BYTES["mixed"] * N
-    traffic = [pmc_traffic(k, which) for k in kernels]
+    traffic = [pmc_traffic(k, which, method) for k in kernels]
     traffic = None if any(t is None for t in traffic) else sum(traffic)
     avg_us = sum(1e3 * rep[k][1] / rep[k][0] for k in kernels)
     ach = alg / (avg_us * 1e-6)
     return {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": ach / HBM_PEAK, "traffic": traffic,
-            "kernel": " + ".join(KERNEL_SYMBOL[k] for k in kernels),
+            "kernel": " + ".join(kernel_symbol(k, method) for k in kernels),
             "avg_launch_us": avg_us, "solo_launch_us": solo_us,
             "achieved_solo": alg / (solo_us * 1e-6) / 1e9,
             "launches_per_step": rep[kernels[0]][0] / steps,
@@ -320,7 +328,8 @@ def supercell_benchmark(steps: int, warmup: int, error_steps: int, profile_steps
             "rel_err_vs_fp64": errs, "error_growth": growth,
             "step_roofline_frac": ALG_BYTES_STEP["supercell"]["mixed"] * N * steps / t_m
             / HBM_PEAK,
-            "substep_roofline": substep_roofline(rep, solo, N, profile_steps, "supercell"),
+            "substep_roofline": substep_roofline(rep, solo, N, profile_steps, "supercell",
+                                                 tune.get("chosen")),
             "acoustic_method": tune.get("chosen"),
             "kernels_ms": {k: {"launches": v[0], "total_ms": v[1], "avg_us": 1e3 * v[1] / v[0]}
                            for k, v in sorted(rep.items(), key=lambda kv: -kv[1][1])}}
@@ -556,7 +565,8 @@ def main() -> int:
         del ctl, gi_d
         log(f"[bench] kernel profile: {args.profile_steps} steps")
         rep, tune, solo = kernel_profile(case, "mixed", 5, args.profile_steps, args.solo_reps)
-        roof = substep_roofline(rep, solo, N, args.profile_steps, "regional")
+        roof = substep_roofline(rep, solo, N, args.profile_steps, "regional",
+                                tune.get("chosen"))
         cpu = None
         if not args.no_cpu:
             log(f"[bench] cpu baseline: oracle for ~{args.cpu_budget:.0f} s, then "

@@ -279,7 +279,9 @@ def test_adjoint_gradient_against_finite_differences_of_the_oracle():
 SLOT_LIMITS = [("fused-64x14", 42), ("fused-64x10x4", 40), ("fused-64x16", 48),
                ("fused-64x8", 48), ("fused-32x16", 48), ("fused-32x8", 48),
                ("fused-32x16w4", 48), ("fused-32x16x4", 64), ("split-64x16", 48),
-               ("split-32x16x4", 64), ("split-32x16w4", 48)]
+               ("split-32x16x4", 64), ("split-32x16w4", 48),
+               # FP64 LDS of the 2-D tiles: 16 x 4 takes at most 47 levels, 8 x 8 all 48
+               ("tile2d-16x4", 47), ("tile2d-8x8", 48)]
 
 
 @pytest.mark.parametrize("shape,nz", SLOT_LIMITS, ids=[f"{s}-nz{n}" for s, n in SLOT_LIMITS])

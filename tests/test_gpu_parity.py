@@ -255,15 +255,16 @@ FUSED_SHAPES = ["fused-64x16", "fused-64x8", "fused-32x16", "fused-32x8", "fused
                 "fused-32x16w4", "fused-64x14", "fused-64x10x4"]
 SPLIT_SHAPES = ["split-32x8", "split-32x16", "split-64x8", "split-64x16", "split-16x16",
                 "split-32x16x4", "split-32x16w4", "split-16x16w4", "split-col"]
+TILE2D_SHAPES = ["tile2d-16x4", "tile2d-8x8"]
 
 
-@pytest.mark.parametrize("shape", FUSED_SHAPES + SPLIT_SHAPES)
+@pytest.mark.parametrize("shape", FUSED_SHAPES + SPLIT_SHAPES + TILE2D_SHAPES)
 def test_fused_acoustic_shapes_match_oracle(monkeypatch, shape):
     """Every tile shape of the fused substep (and the split two-kernel path)
     is in FP64 parity with the oracle over full steps, open and periodic."""
     monkeypatch.setenv("WFE_ACOUSTIC_KERNEL", shape)
     # ragged against every tile; the second shape spans several row blocks in x
-    dims = [(21, 13), (70, 5)] if shape.startswith("fused") else [(21, 13)]
+    dims = [(21, 13), (70, 5)] if shape.startswith(("fused", "tile2d")) else [(21, 13)]
     for bc, (nx, ny) in [(b, d) for d in dims for b in ("open", "periodic")]:
         c = kitchen_sink(bc_x=bc, bc_y=bc, nx=nx, ny=ny)
         st = perturbed_state(c, 31)
@@ -282,7 +283,7 @@ def test_fused_shapes_agree_to_rounding(monkeypatch):
     c = kitchen_sink(nx=21, ny=13)
     st = perturbed_state(c, 32)
     outs = {}
-    for shape in FUSED_SHAPES + SPLIT_SHAPES:
+    for shape in FUSED_SHAPES + SPLIT_SHAPES + TILE2D_SHAPES:
         monkeypatch.setenv("WFE_ACOUSTIC_KERNEL", shape)
         from wfe_amd import Integrator
         gi = Integrator.from_case(c, "mixed")
@@ -465,3 +466,41 @@ def test_uniform_columns_match_per_column_reads(monkeypatch, method):
     gi = Integrator.from_case(kitchen_sink(), "mixed")
     gi.step(flat.scalar("dt"))
     assert not gi.tuning()["uniform_columns"]
+
+
+def test_tuned_shapes_leave_the_trajectory_unchanged(monkeypatch):
+    """The first step times the tendency (levels x tile) and Kessler tile
+    candidates (the Kessler ones on a copy of the state with a scratch rain
+    field); the steps then equal, bit for bit, steps with the chosen shapes
+    pinned through the environment, rain included."""
+    from wfe_amd import Integrator
+
+    c = kitchen_sink(nx=40, ny=18, nz=20)
+    st = perturbed_state(c, 23)
+    dt = c.scalar("dt")
+    for k in ("WFE_TENDENCY_LEVELS", "WFE_TENDENCY_TILE", "WFE_KESSLER_TILE"):
+        monkeypatch.delenv(k, raising=False)
+    gi = Integrator.from_case(c, "mixed")
+    for f, a in st.items():
+        gi.set(f, a)
+    for s in range(3):
+        gi.step(dt)
+    tun = gi.tuning()
+    assert tun["tendencies"] in tun["tendency_candidates_ms"], tun
+    assert len(tun["tendency_candidates_ms"]) == 6
+    assert tun["kessler"] in tun["kessler_candidates_ms"], tun
+    auto = {f: gi.get(f) for f in STATE + ("rain",)}
+    lv, tx = tun["tendencies"].split("x")
+    monkeypatch.setenv("WFE_TENDENCY_LEVELS", lv)
+    monkeypatch.setenv("WFE_TENDENCY_TILE", tx)
+    monkeypatch.setenv("WFE_KESSLER_TILE", tun["kessler"])
+    monkeypatch.setenv("WFE_ACOUSTIC_KERNEL", tun["chosen"])
+    gp = Integrator.from_case(c, "mixed")
+    for f, a in st.items():
+        gp.set(f, a)
+    for s in range(3):
+        gp.step(dt)
+    ptun = gp.tuning()
+    assert not ptun["tendency_candidates_ms"] and not ptun["kessler_candidates_ms"]
+    for f in auto:
+        assert np.array_equal(auto[f], gp.get(f)), f

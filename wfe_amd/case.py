@@ -210,7 +210,8 @@ class Integrator:
     def kernel_times(self) -> dict:
         """Launch count and total ms per kernel (tuning entries removed)."""
         return {k: v for k, v in self.kernel_report().items()
-                if v[0] > 0 and not k.startswith(("tuned:", "cand:", "uniform_columns:"))}
+                if v[0] > 0 and not k.startswith(("tuned:", "cand:", "uniform_columns:", "ttuned:",
+                                                  "tcand:", "ktuned:", "kcand:"))}
 
     def kernel_bench(self, kernel: str, reps: int = 200) -> float:
         """Average launch time [us] of one kernel of the step, launched reps
@@ -222,12 +223,19 @@ class Integrator:
 
     def tuning(self) -> dict:
         """The acoustic-substep shape the context chose and every candidate's
-        average time [ms] from its tuning pass."""
+        average time [ms] from its tuning pass; likewise the tendency kernel's
+        (levels per thread)x(tile width) and the Kessler tile."""
         rep = self.kernel_report()
         chosen = next((k[6:] for k in rep if k.startswith("tuned:")), None)
         return {"chosen": chosen,
                 "candidates_ms": {k[5:]: v[1] for k, v in rep.items() if k.startswith("cand:")},
-                "uniform_columns": any(k == "uniform_columns:1" for k in rep)}
+                "uniform_columns": any(k == "uniform_columns:1" for k in rep),
+                "tendencies": next((k[7:] for k in rep if k.startswith("ttuned:")), None),
+                "tendency_candidates_ms": {k[6:]: v[1] for k, v in rep.items()
+                                           if k.startswith("tcand:")},
+                "kessler": next((k[7:] for k in rep if k.startswith("ktuned:")), None),
+                "kessler_candidates_ms": {k[6:]: v[1] for k, v in rep.items()
+                                          if k.startswith("kcand:")}}
 
     # -- phase-level entry points (kernels.hpp free functions)
     def apply_bcs(self) -> None:

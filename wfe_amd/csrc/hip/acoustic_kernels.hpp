@@ -735,23 +735,21 @@ __device__ __forceinline__ void twist_back(const double* sA, const double* sC, d
   }
 }
 
-template <class T, int TX, int NW, int KPT, bool MOIST, bool TPI, int WPE = 1>
-__global__ void __launch_bounds__(TX * NW, WPE) k_acou_wpi_tile(Grid g, Params p, Base<T> b,
-                                                                Metric<T> m, Coef<T> cf,
-                                                                WPArgs<T> a) {
-  extern __shared__ double lds[];
+// Phases 1-4 of the split column kernel for the column (i, j) of lane lx
+// (col: the lane owns a column; otherwise i, j name any valid column and
+// nothing is stored), with u and v of the column read through `uv`:
+// u0(k), u1(k) the west and east faces, v0(k), v1(k) the south and north
+// faces at level k. `lds` holds the rows as k_acou_wpi_tile lays them out.
+template <class T, int TX, int NW, int KPT, bool MOIST, bool TPI, class UV>
+__device__ __forceinline__ void wpi_column(const Grid& g, const Params& p, const Base<T>& b,
+                                           const Metric<T>& m, const Coef<T>& cf,
+                                           const WPArgs<T>& a, double* lds, int lx, int ry,
+                                           bool col, int i, int j, const UV& uv) {
   const int L = g.nz + 1;
   double* sD = lds + kTwistPad * TX;  // P_k, then D_k -> ds_k -> W_k (k in [0, nz])
   double* sA = sD + L * TX;  // A_k            k in [1, nz)
   double* sB = sA + L * TX;  // B_k            k in [1, nz)
   double* sC = sB + L * TX;  // C_k -> cs_k    k in [1, nz)
-
-  const int lx = threadIdx.x, ry = threadIdx.y;
-  const BlockId bid = xcd_block();
-  const int q = bid.x * TX + lx;  // interior column j nx + i
-  const bool col = q < g.nx * g.ny;
-  const int j = col ? q / g.nx : 0;
-  const int i = col ? q - j * g.nx : 0;
   const double c1 = (1.0 + p.beta) * 0.5;
   const double c2 = (1.0 - p.beta) * 0.5;
   const double dtau = a.dtau;
@@ -781,10 +779,10 @@ __global__ void __launch_bounds__(TX * NW, WPE) k_acou_wpi_tile(Grid g, Params p
     if (k <= 0 || k >= g.nz || flat) return 0.0;
     const int c = c0 + k * PL;
     const double fac = 1.0 - (double)m.zeta_w[k + g.ng] * m.rzt;
-    const double uw = 0.25 * ((double)a.u[c - PL] + (double)a.u[c - PL + 1] + (double)a.u[c] +
-                              (double)a.u[c + 1]);
-    const double vw = 0.25 * ((double)a.v[c - PL] + (double)a.v[c - PL + NX] +
-                              (double)a.v[c] + (double)a.v[c + NX]);
+    const double uw = 0.25 * ((double)uv.u0(k - 1) + (double)uv.u1(k - 1) + (double)uv.u0(k) +
+                              (double)uv.u1(k));
+    const double vw = 0.25 * ((double)uv.v0(k - 1) + (double)uv.v1(k - 1) + (double)uv.v0(k) +
+                              (double)uv.v1(k));
     return (double)cf.aw[cb + k * PL] * (uw * hxc + vw * hyc) * fac;
   };
 
@@ -802,11 +800,11 @@ __global__ void __launch_bounds__(TX * NW, WPE) k_acou_wpi_tile(Grid g, Params p
     const int kk = kc + g.ng;
     const int cc = cb + kc * PL;  // base-state / coefficient index
     const double Kt = (double)cf.kt3[cc];
-    const double Dh = mc * (((double)cf.rtjx[cc + ox] * (double)a.u[c + 1] -
-                             (double)cf.rtjx[cc] * (double)a.u[c]) *
+    const double Dh = mc * (((double)cf.rtjx[cc + ox] * (double)uv.u1(kc) -
+                             (double)cf.rtjx[cc] * (double)uv.u0(kc)) *
                                 g.rdx +
-                            ((double)cf.rtjy[cc + oy] * (double)a.v[c + NX] -
-                             (double)cf.rtjy[cc] * (double)a.v[c]) *
+                            ((double)cf.rtjy[cc + oy] * (double)uv.v1(kc) -
+                             (double)cf.rtjy[cc] * (double)uv.v0(kc)) *
                                 g.rdy);
     const double rdzc = m.rdzc[kk];
     const T awk = kc >= 1 ? cf.aw[cc] : (T)0;
@@ -838,8 +836,8 @@ __global__ void __launch_bounds__(TX * NW, WPE) k_acou_wpi_tile(Grid g, Params p
     Rk[t] = kc >= 1 ? wk + dtau * (buoy + twk) - dtau * kCp * thvw * c2 * (pk - pkm) * rJdzw
                     : 0.0;
     if (k == 0) {  // diagnostic w(0) = u h_x + v h_y at the surface
-      const double us = 0.5 * ((double)a.u[c] + (double)a.u[c + 1]);
-      const double vs = 0.5 * ((double)a.v[c] + (double)a.v[c + NX]);
+      const double us = 0.5 * ((double)uv.u0(0) + (double)uv.u1(0));
+      const double vs = 0.5 * ((double)uv.v0(0) + (double)uv.v1(0));
       w0 = us * hxc + vs * hyc;
     }
     if (act) {
@@ -913,6 +911,218 @@ __global__ void __launch_bounds__(TX * NW, WPE) k_acou_wpi_tile(Grid g, Params p
       a.w_dst[c] = (T)Wk;
     }
   }
+}
+
+// u and v of one column read from the arrays (the split kernel).
+template <class T>
+struct UVFromArrays {
+  CArr<T> u, v;
+  int c0, PL, NX;
+  __device__ __forceinline__ T u0(int k) const { return u[c0 + k * PL]; }
+  __device__ __forceinline__ T u1(int k) const { return u[c0 + k * PL + 1]; }
+  __device__ __forceinline__ T v0(int k) const { return v[c0 + k * PL]; }
+  __device__ __forceinline__ T v1(int k) const { return v[c0 + k * PL + NX]; }
+};
+
+template <class T, int TX, int NW, int KPT, bool MOIST, bool TPI, int WPE = 1>
+__global__ void __launch_bounds__(TX * NW, WPE) k_acou_wpi_tile(Grid g, Params p, Base<T> b,
+                                                                Metric<T> m, Coef<T> cf,
+                                                                WPArgs<T> a) {
+  extern __shared__ double lds[];
+  const int lx = threadIdx.x, ry = threadIdx.y;
+  const BlockId bid = xcd_block();
+  const int q = bid.x * TX + lx;  // interior column j nx + i
+  const bool col = q < g.nx * g.ny;
+  const int j = col ? q / g.nx : 0;
+  const int i = col ? q - j * g.nx : 0;
+  const UVFromArrays<T> uv{a.u, a.v, g.id(i, j, 0), g.plane, g.NX};
+  wpi_column<T, TX, NW, KPT, MOIST, TPI>(g, p, b, m, cf, a, lds, lx, ry, col, i, j, uv);
+}
+
+// ---------------------------------------------------------------------------
+// Fused acoustic substep on 2-D tiles of columns, for grids whose columns
+// differ (terrain). A workgroup owns CX x CY columns (CX CY = 64, one lane
+// each) and all their levels (NW thread rows, KPT levels each):
+//   0a. pi* (levels -1 .. nz) and theta_v (levels 0 .. nz-1) of the cells
+//       i0-1 .. i0+CX, j0-1 .. j0+CY that lie below i = nx, j = ny (the
+//       west/south halo may be a ghost cell) are staged in LDS once;
+//   0b. u on the faces i0 .. i0+CX of rows j0 .. j0+CY-1 and v on the faces
+//       j0 .. j0+CY of columns i0 .. i0+CX-1, at every level, from the
+//       staged values with the expressions of k_acou_uv_tile (radiation
+//       edges and the periodic wraps through the direct path), kept in LDS
+//       and written out for the faces the workgroup owns (the west and
+//       south faces of its columns, and faces nx and ny);
+//   1-4. the column phases of the split kernel (wpi_column) with u and v
+//       from LDS; the rows of the solve reuse the LDS of step 0a.
+// Against the split pair this saves the second read of u, v, theta', qv and
+// pi' and one launch per substep; the east and north halo faces are formed
+// by both workgroups that touch them.
+template <int CX, int CY>
+struct Tile2D {
+  static constexpr int TX = 64, PX = CX + 2, PY = CY + 2, NP = PX * PY;
+  // doubles of the region shared by the staging (0a) and the solve rows
+  template <class T>
+  __host__ __device__ static size_t region_a(int nz) {
+    const size_t rows = (size_t)(4 * (nz + 1) + 2 * kTwistPad) * TX;
+    const size_t stage = (size_t)NP * (nz + 2) + ((size_t)NP * nz * sizeof(T) + 7) / 8;
+    return rows > stage ? rows : stage;
+  }
+  template <class T>
+  static size_t lds_bytes(int nz) {
+    return region_a<T>(nz) * sizeof(double) +
+           ((size_t)nz * CY * (CX + 1) + (size_t)nz * (CY + 1) * CX) * sizeof(T);
+  }
+};
+
+// u, v of one column from the LDS face tiles of k_acou_tile2d.
+template <class T, int CX, int CY>
+struct UVFromTile {
+  const T* sU;  // [k][cy][0 .. CX]
+  const T* sV;  // [k][0 .. CY][cx]
+  int cx, cy;
+  __device__ __forceinline__ T u0(int k) const { return sU[(k * CY + cy) * (CX + 1) + cx]; }
+  __device__ __forceinline__ T u1(int k) const { return sU[(k * CY + cy) * (CX + 1) + cx + 1]; }
+  __device__ __forceinline__ T v0(int k) const { return sV[(k * (CY + 1) + cy) * CX + cx]; }
+  __device__ __forceinline__ T v1(int k) const { return sV[(k * (CY + 1) + cy + 1) * CX + cx]; }
+};
+
+template <class T, int CX, int CY, int NW, int KPT, bool MOIST, bool TPI>
+__global__ void __launch_bounds__(64 * NW) k_acou_tile2d(Grid g, Params p, Base<T> b,
+                                                          Metric<T> m, Coef<T> cf, UVArgs<T> a,
+                                                          WPArgs<T> w) {
+  static_assert(CX * CY == 64, "one lane per column of the tile");
+  using Sh = Tile2D<CX, CY>;
+  constexpr int PX = Sh::PX, NP = Sh::NP, NT = 64 * NW;
+  extern __shared__ double lds[];
+  const int nz = g.nz;
+  double* sPs = lds;                                        // pi* [k + 1][py][px]
+  T* sTv = reinterpret_cast<T*>(lds + (size_t)NP * (nz + 2));  // theta_v [k][py][px]
+  T* sU = reinterpret_cast<T*>(lds + Sh::template region_a<T>(nz));
+  T* sV = sU + (size_t)nz * CY * (CX + 1);
+  const int tid = threadIdx.y * 64 + threadIdx.x;
+  const BlockId bid = xcd_block();
+  const int i0 = bid.x * CX, j0 = bid.y * CY;
+  const UVDev<T> d{g, p, b, m, a};
+
+  // ---- 0a: pi* and theta_v of the tile's cells and their west/south halo
+  for (int e = tid; e < NP * (nz + 2); e += NT) {
+    const int lev = e / NP, r2 = e - lev * NP;
+    const int py = r2 / PX, px = r2 - py * PX;
+    const int ii = i0 - 1 + px, jj = j0 - 1 + py, k = lev - 1;
+    if (ii <= g.nx - 1 && jj <= g.ny - 1) {
+      sPs[e] = d.pstar(ii, jj, k);
+      if (k >= 0 && k < nz) sTv[k * NP + r2] = d.thv(ii, jj, k);
+    }
+  }
+  __syncthreads();
+  auto P = [&](int ii, int jj, int k) { return sPs[((k + 1) * Sh::PY + (jj - j0 + 1)) * PX + (ii - i0 + 1)]; };
+  auto TV = [&](int ii, int jj, int k) { return sTv[(k * Sh::PY + (jj - j0 + 1)) * PX + (ii - i0 + 1)]; };
+  const T dtau = a.dtau;
+
+  // ---- 0b: u faces (CX + 1 per row, CY rows) and v faces (CX per row,
+  // CY + 1 rows) at every level
+  for (int e = tid; e < nz * CY * (CX + 1); e += NT) {
+    const int k = e / (CY * (CX + 1)), r2 = e - k * (CY * (CX + 1));
+    const int fy = r2 / (CX + 1), fx = r2 - fy * (CX + 1);
+    const int i = i0 + fx, j = j0 + fy;
+    if (i > g.nx || j >= g.ny) continue;
+    const int kk = k + g.ng;
+    const int c = g.id(i, j, k);
+    const int c2 = g.id2(i, j);
+    const T usrc = a.u_src[c];
+    T un;
+    if (i < g.nx && !(p.bcx && i == 0)) {
+      const T thbu = (T)0.5 * (TV(i - 1, j, k) + TV(i, j, k));
+      const T mu = (T)0.5 * (m.mapf[c2 - 1] + m.mapf[c2]);
+      double grad = (P(i, j, k) - P(i - 1, j, k)) * g.rdx;
+      const T hxu = m.hxu[c2];
+      if (hxu != (T)0) {
+        const T zxu = hxu * ((T)1 - m.zeta_c[kk] / m.zt);
+        const T dzc2 = m.zeta_c[kk + 1] - m.zeta_c[kk - 1];
+        const T ju = (T)0.5 * (m.jac[c2 - 1] + m.jac[c2]);
+        const double dpdz = 0.5 *
+                            ((P(i - 1, j, k + 1) - P(i - 1, j, k - 1)) +
+                             (P(i, j, k + 1) - P(i, j, k - 1))) /
+                            (double)dzc2;
+        grad -= (double)(zxu / ju) * dpdz;
+      }
+      un = usrc + dtau * (-(T)kCp * thbu * mu * (T)grad + a.tu[c]);
+    } else if (!p.bcx) {
+      un = d.u_step(0, j, k);
+    } else if (i == 0) {
+      if (usrc >= (T)0) {
+        un = b.ub[kk];
+      } else {
+        const T cb = usrc - (T)p.cstar;
+        un = usrc - dtau * cb * (d.u_step(1, j, k) - usrc) / (T)g.dx;
+      }
+    } else {
+      if (usrc <= (T)0) {
+        un = b.ub[kk];
+      } else {
+        const T cb = usrc + (T)p.cstar;
+        un = usrc - dtau * cb * (usrc - d.u_step(g.nx - 1, j, k)) / (T)g.dx;
+      }
+    }
+    sU[e] = un;
+    if (fx < CX || i == g.nx) a.u_dst[c] = un;
+  }
+  for (int e = tid; e < nz * (CY + 1) * CX; e += NT) {
+    const int k = e / ((CY + 1) * CX), r2 = e - k * ((CY + 1) * CX);
+    const int fy = r2 / CX, fx = r2 - fy * CX;
+    const int i = i0 + fx, j = j0 + fy;
+    if (i >= g.nx || j > g.ny) continue;
+    const int kk = k + g.ng;
+    const int c = g.id(i, j, k);
+    const int c2 = g.id2(i, j);
+    const T vsrc = a.v_src[c];
+    T vn;
+    if (j < g.ny && !(p.bcy && j == 0)) {
+      const T thbv = (T)0.5 * (TV(i, j - 1, k) + TV(i, j, k));
+      const T mv = (T)0.5 * (m.mapf[c2 - g.NX] + m.mapf[c2]);
+      double grad = (P(i, j, k) - P(i, j - 1, k)) * g.rdy;
+      const T hyv = m.hyv[c2];
+      if (hyv != (T)0) {
+        const T zyv = hyv * ((T)1 - m.zeta_c[kk] / m.zt);
+        const T dzc2 = m.zeta_c[kk + 1] - m.zeta_c[kk - 1];
+        const T jv = (T)0.5 * (m.jac[c2 - g.NX] + m.jac[c2]);
+        const double dpdz = 0.5 *
+                            ((P(i, j - 1, k + 1) - P(i, j - 1, k - 1)) +
+                             (P(i, j, k + 1) - P(i, j, k - 1))) /
+                            (double)dzc2;
+        grad -= (double)(zyv / jv) * dpdz;
+      }
+      vn = vsrc + dtau * (-(T)kCp * thbv * mv * (T)grad + a.tv[c]);
+    } else if (!p.bcy) {
+      vn = d.v_step(i, 0, k);
+    } else if (j == 0) {
+      if (vsrc >= (T)0) {
+        vn = b.vb[kk];
+      } else {
+        const T cb = vsrc - (T)p.cstar;
+        vn = vsrc - dtau * cb * (d.v_step(i, 1, k) - vsrc) / (T)g.dy;
+      }
+    } else {
+      if (vsrc <= (T)0) {
+        vn = b.vb[kk];
+      } else {
+        const T cb = vsrc + (T)p.cstar;
+        vn = vsrc - dtau * cb * (vsrc - d.v_step(i, g.ny - 1, k)) / (T)g.dy;
+      }
+    }
+    sV[e] = vn;
+    if (fy < CY || j == g.ny) a.v_dst[c] = vn;
+  }
+  __syncthreads();
+
+  // ---- 1-4: the column solve, u and v from the face tiles
+  const int lx = threadIdx.x, ry = threadIdx.y;
+  const int cx = lx % CX, cy = lx / CX;
+  const int i = i0 + cx, j = j0 + cy;
+  const bool col = i < g.nx && j < g.ny;
+  const UVFromTile<T, CX, CY> uv{sU, sV, cx, cy};
+  wpi_column<T, 64, NW, KPT, MOIST, TPI>(g, p, b, m, cf, w, lds, lx, ry, col, col ? i : 0,
+                                         col ? j : 0, uv);
 }
 
 // ---------------------------------------------------------------------------

@@ -317,7 +317,10 @@ class Ctx final : public CtxBase {
   }
   double kernel_bench(const std::string& name, int reps) override;
   hipStream_t stream() const override { return stream_; }
-  bool ready_for_capture() const override { return tuned_ && !timer_.on; }
+  bool ready_for_capture() const override {
+    const bool kessler = p_.moisture && p_.microphysics && g_.nz <= 48;
+    return tuned_ && tend_tuned_ && (kes_tuned_ || !kessler) && !timer_.on;
+  }
   int parity() const override { return cur_; }
   void flip_parity() override { cur_ ^= 1; }
   unsigned generation() const override { return gen_; }
@@ -332,6 +335,8 @@ class Ctx final : public CtxBase {
     if (tuned_) r += "tuned:" + methods_[method_].name + " 0 0\n";
     r += std::string("uniform_columns:") + (p_.hom ? "1" : "0") + " 0 0\n";
     r += tune_report_;
+    r += tend_report_;
+    r += kes_report_;
     return r;
   }
 
@@ -349,6 +354,28 @@ class Ctx final : public CtxBase {
   void kessler_step(const StateSet<T>& s, double dt);
   T* field_ptr(int field);
   dim3 blk3() const { return dim3(64, 4, 1); }
+  // Average time [ms] of `f` (a launch sequence on the stream) over 5 runs
+  // after 2 warm-up runs, with the per-kernel timer paused.
+  template <class F>
+  float time_launches(F&& f) {
+    hipEvent_t e0, e1;
+    WFE_HIP(hipEventCreate(&e0));
+    WFE_HIP(hipEventCreate(&e1));
+    const bool was_on = timer_.on;
+    timer_.on = false;
+    for (int r = 0; r < 2; ++r) f();
+    WFE_HIP(hipEventRecord(e0, stream_));
+    for (int r = 0; r < 5; ++r) f();
+    WFE_HIP(hipEventRecord(e1, stream_));
+    WFE_HIP(hipEventSynchronize(e1));
+    timer_.on = was_on;
+    float ms = 0;
+    WFE_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    check_launch("tuning");
+    return ms / 5;
+  }
   void launch_uv_tile(const dev::UVArgs<T>& uv) {
     const dim3 guv = lv_grid(g_.nx + 1, g_.ny + 1, g_.nz);
     if (lv_bx_ == 32)
@@ -403,6 +430,11 @@ class Ctx final : public CtxBase {
   // x thread rows). 32x16 (3 levels per thread, 480 workgroups on the
   // 120x120x40 supercell) measured 18.6 us against 21.8 us for 64x8.
   std::string kes_tile_ = "32x16";
+  // Tendency (levels per thread, tile width) and Kessler tile shapes are
+  // timed on the first step unless pinned through the environment.
+  bool tend_tuned_ = false, kes_tuned_ = false;
+  std::string tend_report_, kes_report_;
+  T* kes_rain_scratch_ = nullptr;
   T* tq_[3]{};
   T *mfx_ = nullptr, *mfy_ = nullptr, *mfz_ = nullptr, *div_ = nullptr;
   T* pratio_[3]{};
@@ -467,6 +499,8 @@ class Ctx final : public CtxBase {
   void add_split_tile(const char* name, size_t cap);
   template <int TX, int NW, int KPT, int WPE = 1>
   void add_row(const char* name, size_t cap);
+  template <int CX, int CY, int NW, int KPT>
+  void add_tile2d(const char* name, size_t cap);
   KernelTimer timer_;
   SoloBench solo_;
 };
@@ -555,6 +589,7 @@ Ctx<T>::Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& b, const w
     const int v = std::atoi(e);
     if (v != 1 && v != 2 && v != 4) throw ArgError("WFE_TENDENCY_LEVELS must be 1, 2 or 4");
     tend_levels_ = v;
+    tend_tuned_ = true;
   }
   lv_bx_ = cdiv(gr.nx, 32) * 32 < cdiv(gr.nx, 64) * 64 ? 32 : 64;
   tend_tx_ = lv_bx_;
@@ -562,6 +597,7 @@ Ctx<T>::Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& b, const w
     const int v = std::atoi(e);
     if (v != 64 && v != 32) throw ArgError("WFE_TENDENCY_TILE must be 64 or 32");
     tend_tx_ = v;
+    tend_tuned_ = true;
   }
   if (const char* e = std::getenv("WFE_KESSLER_TILE")) {
     const std::string v = e;
@@ -569,6 +605,7 @@ Ctx<T>::Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& b, const w
         v != "32x24" && v != "16x48")
       throw ArgError("WFE_KESSLER_TILE must be 64x8, 32x8, 32x16, 16x16, 64x16, 32x24 or 16x48");
     kes_tile_ = v;
+    kes_tuned_ = true;
   }
   WFE_HIP(hipStreamCreate(&stream_));
   const size_t n = npts_;
@@ -993,12 +1030,41 @@ void Ctx<T>::slow_tendencies(const StateSet<T>& s, double dt, const StateSet<T>*
               div_, tu_, tv_, tw_, tth_, tq0, tq1, tq2, sn.qv, sn.qc, sn.qr, do_q, (T)dt_stage,
               stage_on, buoy_on)));
   };
-  if (tend_levels_ == 4)
-    launch(std::integral_constant<int, 4>{});
-  else if (tend_levels_ == 2)
-    launch(std::integral_constant<int, 2>{});
-  else
-    launch(std::integral_constant<int, 1>{});
+  auto launch_cur = [&] {
+    if (tend_levels_ == 4)
+      launch(std::integral_constant<int, 4>{});
+    else if (tend_levels_ == 2)
+      launch(std::integral_constant<int, 2>{});
+    else
+      launch(std::integral_constant<int, 1>{});
+  };
+  if (!tend_tuned_) {
+    // The launch is a pure function of the stage's inputs, so repeating it
+    // for timing leaves the step's results unchanged.
+    tend_report_.clear();
+    float best = 1e30f;
+    int best_l = tend_levels_, best_t = tend_tx_;
+    for (const int lv : {1, 2, 4})
+      for (const int tx : {64, 32}) {
+        tend_levels_ = lv;
+        tend_tx_ = tx;
+        const float ms = time_launches(launch_cur);
+        char buf[96];
+        std::snprintf(buf, sizeof(buf), "tcand:%dx%d 1 %.6f\n", lv, tx, ms);
+        tend_report_ += buf;
+        if (ms < best) {
+          best = ms;
+          best_l = lv;
+          best_t = tx;
+        }
+      }
+    tend_levels_ = best_l;
+    tend_tx_ = best_t;
+    tend_report_ += "ttuned:" + std::to_string(best_l) + "x" + std::to_string(best_t) + " 0 0\n";
+    tend_tuned_ = true;
+    ++gen_;
+  }
+  launch_cur();
   if (p_.moisture && p_.pd_moist) {
     dev::PDSet<T> S{};
     const T* q[3] = {s.qv, s.qc, s.qr};
@@ -1141,6 +1207,44 @@ void Ctx<T>::add_row(const char* name, size_t cap) {
 }
 
 template <class T>
+template <int CX, int CY, int NW, int KPT>
+void Ctx<T>::add_tile2d(const char* name, size_t cap) {
+  using Sh = dev::Tile2D<CX, CY>;
+  if (g_.nz > NW * KPT) return;
+  const size_t lb = Sh::template lds_bytes<T>(g_.nz);
+  const void* fns[4] = {reinterpret_cast<const void*>(&dev::k_acou_tile2d<T, CX, CY, NW, KPT, false, false>),
+                        reinterpret_cast<const void*>(&dev::k_acou_tile2d<T, CX, CY, NW, KPT, false, true>),
+                        reinterpret_cast<const void*>(&dev::k_acou_tile2d<T, CX, CY, NW, KPT, true, false>),
+                        reinterpret_cast<const void*>(&dev::k_acou_tile2d<T, CX, CY, NW, KPT, true, true>)};
+  if (lb > 65536) {
+    if (lb > cap) return;
+    for (const void* f : fns)
+      if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb) != hipSuccess) {
+        (void)hipGetLastError();
+        return;
+      }
+  }
+  methods_.push_back({name, [this, lb](const dev::FusedArgs<T>& a) {
+                        const dev::WPArgs<T> w = split_wp_impl<T>(a);
+                        const dim3 gt(cdiv(g_.nx, CX), cdiv(g_.ny, CY));
+                        const dim3 bt(64, NW);
+                        const bool moist = p_.moisture, tpi = a.tpi != nullptr;
+                        if (moist && tpi)
+                          TIMED("acoustic_fused", (dev::k_acou_tile2d<T, CX, CY, NW, KPT, true, true>
+                                                   <<<gt, bt, lb, stream_>>>(g_, p_, base_, met_, coef_, a.uv, w)));
+                        else if (moist)
+                          TIMED("acoustic_fused", (dev::k_acou_tile2d<T, CX, CY, NW, KPT, true, false>
+                                                   <<<gt, bt, lb, stream_>>>(g_, p_, base_, met_, coef_, a.uv, w)));
+                        else if (tpi)
+                          TIMED("acoustic_fused", (dev::k_acou_tile2d<T, CX, CY, NW, KPT, false, true>
+                                                   <<<gt, bt, lb, stream_>>>(g_, p_, base_, met_, coef_, a.uv, w)));
+                        else
+                          TIMED("acoustic_fused", (dev::k_acou_tile2d<T, CX, CY, NW, KPT, false, false>
+                                                   <<<gt, bt, lb, stream_>>>(g_, p_, base_, met_, coef_, a.uv, w)));
+                      }});
+}
+
+template <class T>
 void Ctx<T>::build_methods(const hipDeviceProp_t& prop) {
   const size_t cap = (size_t)prop.sharedMemPerBlock;
   // General split pair: one thread per column (any nz <= 319).
@@ -1173,6 +1277,9 @@ void Ctx<T>::build_methods(const hipDeviceProp_t& prop) {
   add_row<32, 8, 6>("fused-32x8", cap);
   add_row<32, 16, 4>("fused-32x16x4", cap);  // columns up to 64 levels
   add_row<32, 16, 3, 4>("fused-32x16w4", cap);  // two blocks per CU
+  // 2-D column tiles (terrain): 16 x 4 and 8 x 8 columns per workgroup
+  add_tile2d<16, 4, 16, 3>("tile2d-16x4", cap);
+  add_tile2d<8, 8, 16, 3>("tile2d-8x8", cap);
   // Selection: WFE_ACOUSTIC_KERNEL = exact method name, or a family prefix
   // ("split", "fused"); unset or "auto" = all. WFE_WPI_KERNEL=column keeps
   // only the one-thread-per-column pair.
@@ -1303,7 +1410,7 @@ void Ctx<T>::stage_moisture(const StateSet<T>& out, double dt_stage) {
 
 template <class T>
 void Ctx<T>::kessler_step(const StateSet<T>& s, double dt) {
-  dev::KesArgs<T> a{s.thp, s.pip, s.qv, s.qc, s.qr, rain_, (T)dt};
+  dev::KesArgs<T> a{s.thp, s.pip, s.qv, s.qc, s.qr, rain_, (T)dt};  // the tuner swaps it
   const dim3 gcol(cdiv(g_.nx, 64), (unsigned)g_.ny);
   // Tile shapes: TX columns, NW thread rows, KPT = 48 / NW levels per thread.
   auto tile = [&](auto tx, auto nw) {
@@ -1317,7 +1424,7 @@ void Ctx<T>::kessler_step(const StateSet<T>& s, double dt) {
   using I16 = std::integral_constant<int, 16>;
   using I32 = std::integral_constant<int, 32>;
   using I64 = std::integral_constant<int, 64>;
-  if (g_.nz <= 48) {
+  auto run_tile = [&] {
     if (kes_tile_ == "32x8")
       tile(I32{}, I8{});
     else if (kes_tile_ == "32x16")
@@ -1334,6 +1441,38 @@ void Ctx<T>::kessler_step(const StateSet<T>& s, double dt) {
       tile(I64{}, I8{});
     else
       tile(I32{}, I16{});
+  };
+  if (g_.nz <= 48) {
+    if (!kes_tuned_) {
+      // Timed on a copy of the state (the stage estimates are free after
+      // the three stages) with a scratch rain field, then run on the state.
+      const StateSet<T>& sc = SE_[0];
+      for (const auto& f : {std::make_pair(sc.thp, s.thp), std::make_pair(sc.qv, s.qv),
+                            std::make_pair(sc.qc, s.qc), std::make_pair(sc.qr, s.qr)})
+        WFE_HIP(hipMemcpyAsync(f.first, f.second, npts_ * sizeof(T), hipMemcpyDeviceToDevice,
+                               stream_));
+      if (!kes_rain_scratch_) kes_rain_scratch_ = arena_.alloc<T>(n2_);
+      const dev::KesArgs<T> real = a;
+      a = dev::KesArgs<T>{sc.thp, s.pip, sc.qv, sc.qc, sc.qr, kes_rain_scratch_, (T)dt};
+      kes_report_.clear();
+      float best = 1e30f;
+      std::string best_t = kes_tile_;
+      for (const char* t : {"32x16", "16x16", "32x8", "64x8", "64x16", "32x24", "16x48"}) {
+        kes_tile_ = t;
+        const float ms = time_launches(run_tile);
+        kes_report_ += std::string("kcand:") + t + " 1 " + std::to_string(ms) + "\n";
+        if (ms < best) {
+          best = ms;
+          best_t = t;
+        }
+      }
+      kes_tile_ = best_t;
+      kes_report_ += "ktuned:" + best_t + " 0 0\n";
+      kes_tuned_ = true;
+      ++gen_;
+      a = real;
+    }
+    run_tile();
   } else if (g_.nz + 1 <= 64)
     TIMED("kessler", dev::k_kessler_col<T, 64><<<gcol, dim3(64, 1), 0, stream_>>>(g_, base_, met_, a));
   else
@@ -1406,9 +1545,10 @@ void Ctx<T>::step(double dt, double t) {
     dev::k_set_scalar<double><<<1, 64, 0, stream_>>>(utc_dev_, utc);
     check_launch("set_utc");
   }
-  // The first step tunes the substep method (synchronous timing), so it and
-  // any step under per-kernel timing run eagerly.
-  if (!use_graph_ || !tuned_ || timer_.on) {
+  // The first step tunes the substep method and the tendency and Kessler
+  // shapes (synchronous timing), so it and any step under per-kernel timing
+  // run eagerly.
+  if (!use_graph_ || !ready_for_capture()) {
     step_launches(dt);
     return;
   }
