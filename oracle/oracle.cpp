@@ -929,6 +929,111 @@ struct orc_model {
     for (int k = nz - 2; k >= 0; --k) phi[k] = (D[k] - C[k] * phi[k + 1]) / B[k];
   }
 
+  // Louis (1979) bulk exchange coefficients for momentum and heat at
+  // height z1 over roughness z0 (surface.cu:140-159).
+  static void louis(double Rib, double z1, double z0, double* Cd, double* Ch) {
+    double a2 = kKappa / std::log(z1 / z0);
+    a2 *= a2;
+    double Fm, Fh;
+    if (Rib < 0) {
+      const double sq = std::sqrt(-Rib * z1 / z0);
+      Fm = 1 - 9.4 * Rib / (1 + 7.4 * 9.4 * a2 * sq);
+      Fh = 1 - 9.4 * Rib / (1 + 5.3 * 9.4 * a2 * sq);
+    } else {
+      const double d = 1 + 4.7 * Rib;
+      Fm = Fh = 1 / (d * d);
+    }
+    *Cd = a2 * Fm;
+    *Ch = a2 * Fh;
+  }
+  // 10 m wind and 2 m potential temperature from the lowest level (z1, V1,
+  // th1) and the surface (ths) through the Businger-Dyer profiles
+  // (surface.cu:166-184); z0h = z0 / 10.
+  static void sfc_diag(double V1, double Rib, double z1, double z0, double ths, double th1,
+                       double* u10, double* th2) {
+    const double z0h = z0 * 0.1;
+    const double lnz1z0 = std::log(z1 / z0);
+    double zeta1 = (Rib >= 0) ? Rib * lnz1z0 / (1 - 5 * std::fmin(Rib, 0.19)) : Rib * lnz1z0;
+    zeta1 = std::fmax(-8.0, std::fmin(2.0, zeta1));
+    const double zol = zeta1 / z1;
+    const double den_m = lnz1z0 - psi_m(zeta1) + psi_m(z0 * zol);
+    const double num_m = std::log(10 / z0) - psi_m(10 * zol) + psi_m(z0 * zol);
+    const double r10 = std::fmax(0.0, std::fmin(1.2, num_m / std::fmax(den_m, 0.1)));
+    *u10 = V1 * r10;
+    const double den_h = std::log(z1 / z0h) - psi_h(zeta1) + psi_h(z0h * zol);
+    const double num_h = std::log(2 / z0h) - psi_h(2 * zol) + psi_h(z0h * zol);
+    const double r2 = std::fmax(0.0, std::fmin(1.2, num_h / std::fmax(den_h, 0.1)));
+    *th2 = ths + (th1 - ths) * r2;
+  }
+  // Cosine of the solar zenith angle, clipped at 0 (surface.cu:188-196).
+  static double solar_cosz(int doy, double utc, double lat, double lon) {
+    const double decl = 0.40928 * std::sin(6.2832 * (284 + doy) / 365.0);
+    const double latr = lat * 0.0174533;
+    const double hsol = utc + lon / 15;
+    const double H = 0.2618 * (hsol - 12);
+    const double cosz = std::sin(latr) * std::sin(decl) + std::cos(latr) * std::cos(decl) * std::cos(H);
+    return cosz < 0 ? 0 : cosz;
+  }
+  // Grey long-wave fluxes of a column of nz layers with temperatures T and
+  // emissivities eps over a surface at Ts (emissivity 0.97): Fdn, Fup at the
+  // nz + 1 interfaces (surface.cu:200-216).
+  static void lw_fluxes(int nz, const double* T, const double* eps, double Ts, double* Fup,
+                        double* Fdn) {
+    Fdn[nz] = 0;
+    for (int k = nz - 1; k >= 0; --k)
+      Fdn[k] = Fdn[k + 1] * (1 - eps[k]) + eps[k] * kSigma * T[k] * T[k] * T[k] * T[k];
+    Fup[0] = 0.97 * kSigma * Ts * Ts * Ts * Ts + 0.03 * Fdn[0];
+    for (int k = 0; k < nz; ++k)
+      Fup[k + 1] = Fup[k] * (1 - eps[k]) + eps[k] * kSigma * T[k] * T[k] * T[k] * T[k];
+  }
+  // Short-wave absorption of a column (vapour path du, liquid path dl per
+  // layer) under the sun at cosz with cloud albedo from the liquid water
+  // path lwp: swabs per layer, returns the flux reaching the surface
+  // (surface.cu:218-232).
+  static double sw_column(int nz, double cosz, double lwp, const double* du, const double* dl,
+                          double* swabs) {
+    const double mu = std::fmax(cosz, 0.05);
+    const double cldalb = lwp / (lwp + 60);
+    double Sdn = kS0 * cosz * 0.95 * (1 - cldalb);
+    for (int k = nz - 1; k >= 0; --k) {
+      const double tau = std::exp(-(0.008 * du[k] + 0.15 * dl[k]) / mu);
+      swabs[k] = Sdn * (1 - tau);
+      Sdn *= tau;
+    }
+    return Sdn;
+  }
+  // One implicit step of the four-layer soil (layer depths 0.1, 0.3, 0.6,
+  // 1.0 m) with surface flux G into layer 0 and a deep temperature below
+  // layer 3; water content w sets the heat capacity and conductivity
+  // (surface.cu:247-289). Tso is updated in place and clipped to 200-350 K.
+  static void soil_step(double dt, double w, double G, double tdeep, double* Tso) {
+    static const double kDz[4] = {0.1, 0.3, 0.6, 1.0};
+    const double Cso = soil_C(w), Kso = soil_K(w);
+    double A[4], B[4], C[4], D[4];
+    for (int L = 0; L < 4; ++L) {
+      const double cap = Cso * kDz[L] / dt;
+      const double ku = (L > 0) ? Kso / (0.5 * (kDz[L - 1] + kDz[L])) : 0.0;
+      const double kl = (L < 3) ? Kso / (0.5 * (kDz[L] + kDz[L + 1])) : Kso / kDz[L];
+      A[L] = -ku;
+      C[L] = (L < 3) ? -kl : 0.0;
+      B[L] = cap + ku + kl;
+      D[L] = cap * Tso[L];
+    }
+    D[0] += G;
+    D[3] += (Kso / kDz[3]) * tdeep;
+    for (int L = 1; L < 4; ++L) {
+      const double w2 = A[L] / B[L - 1];
+      B[L] -= w2 * C[L - 1];
+      D[L] -= w2 * D[L - 1];
+    }
+    Tso[3] = D[3] / B[3];
+    for (int L = 2; L >= 0; --L) Tso[L] = (D[L] - C[L] * Tso[L + 1]) / B[L];
+    for (int L = 0; L < 4; ++L) {
+      if (Tso[L] < 200) Tso[L] = 200;
+      if (Tso[L] > 350) Tso[L] = 350;
+    }
+  }
+
   // k_sfc_scalar (surface.cu:87-387) for column (i, j) of state s.
   void sfc_scalar_column(int i, int j, double dt, double utc, State& s) {
     const int nz = g.nz;
@@ -975,72 +1080,31 @@ struct orc_model {
     const double qs_s = qsat_tetens(p1, Ts);
     const double thvs = ths * (1 + kEps61 * (beta * qs_s + (1 - beta) * qvcol[0]));
     const double Rib = kGrav * z1 * (thv[0] - thvs) / (thvs * V1 * V1);
-    double a2 = kKappa / std::log(z1 / z0);
-    a2 *= a2;
-    double Fm, Fh;
-    if (Rib < 0) {
-      const double sq = std::sqrt(-Rib * z1 / z0);
-      Fm = 1 - 9.4 * Rib / (1 + 7.4 * 9.4 * a2 * sq);
-      Fh = 1 - 9.4 * Rib / (1 + 5.3 * 9.4 * a2 * sq);
-    } else {
-      const double d = 1 + 4.7 * Rib;
-      Fm = Fh = 1 / (d * d);
-    }
-    const double Cd = a2 * Fm, Ch = a2 * Fh;
+    double Cd, Ch;
+    louis(Rib, z1, z0, &Cd, &Ch);
     cdv[c2] = Cd * V1;
     // 2 m and 10 m diagnostics (:166-184)
-    const double z0h = z0 * 0.1;
-    const double lnz1z0 = std::log(z1 / z0);
-    double zeta1 = (Rib >= 0) ? Rib * lnz1z0 / (1 - 5 * std::fmin(Rib, 0.19)) : Rib * lnz1z0;
-    zeta1 = std::fmax(-8.0, std::fmin(2.0, zeta1));
-    const double zol = zeta1 / z1;
-    const double den_m = lnz1z0 - psi_m(zeta1) + psi_m(z0 * zol);
-    const double num_m = std::log(10 / z0) - psi_m(10 * zol) + psi_m(z0 * zol);
-    const double r10 = std::fmax(0.0, std::fmin(1.2, num_m / std::fmax(den_m, 0.1)));
-    u10[c2] = V1 * r10;
-    const double den_h = std::log(z1 / z0h) - psi_h(zeta1) + psi_h(z0h * zol);
-    const double num_h = std::log(2 / z0h) - psi_h(2 * zol) + psi_h(z0h * zol);
-    const double r2 = std::fmax(0.0, std::fmin(1.2, num_h / std::fmax(den_h, 0.1)));
-    const double th2 = ths + (thcol[0] - ths) * r2;
+    double th2;
+    sfc_diag(V1, Rib, z1, z0, ths, thcol[0], &u10[c2], &th2);
     t2m[c2] = th2 * pi1;
     double shf = Ch * V1 * (ths - thcol[0]);
     double qflx = Ch * V1 * beta * (qs_s - qvcol[0]);
     if (!onland && qflx < 0) qflx = 0;
     // radiation (:188-244)
-    const double decl = 0.40928 * std::sin(6.2832 * (284 + phys_doy) / 365.0);
-    const double latr = lat[c2] * 0.0174533;
-    const double hsol = utc + lon[c2] / 15;
-    const double H = 0.2618 * (hsol - 12);
-    double cosz = std::sin(latr) * std::sin(decl) + std::cos(latr) * std::cos(decl) * std::cos(H);
-    if (cosz < 0) cosz = 0;
-    std::vector<double> Tlev(nz), epslw(nz), Fup(nz + 1), Fdn(nz + 1), radht(nz), swabs(nz);
+    const double cosz = solar_cosz(phys_doy, utc, lat[c2], lon[c2]);
+    std::vector<double> Tlev(nz), epslw(nz), Fup(nz + 1), Fdn(nz + 1), radht(nz), swabs(nz),
+        du(nz), dl(nz);
     for (int k = 0; k < nz; ++k) {
       const long c = g.id(i, j, k);
       const double pik = pib[c] + s.pip[c];
       Tlev[k] = thcol[k] * pik;
-      const double du = rho[k] * qvcol[k] * dzc[k];
-      const double dl = rho[k] * (s.qc[c] > 0 ? s.qc[c] : 0.0) * dzc[k] * 1000;
-      epslw[k] = 1 - std::exp(-(0.22 * du + 0.30 * dl));
+      du[k] = rho[k] * qvcol[k] * dzc[k];
+      dl[k] = rho[k] * (s.qc[c] > 0 ? s.qc[c] : 0.0) * dzc[k] * 1000;
+      epslw[k] = 1 - std::exp(-(0.22 * du[k] + 0.30 * dl[k]));
     }
-    Fdn[nz] = 0;
-    for (int k = nz - 1; k >= 0; --k)
-      Fdn[k] = Fdn[k + 1] * (1 - epslw[k]) + epslw[k] * kSigma * Tlev[k] * Tlev[k] * Tlev[k] * Tlev[k];
+    lw_fluxes(nz, Tlev.data(), epslw.data(), Ts, Fup.data(), Fdn.data());
     const double lwd = Fdn[0];
-    Fup[0] = 0.97 * kSigma * Ts * Ts * Ts * Ts + 0.03 * Fdn[0];
-    for (int k = 0; k < nz; ++k)
-      Fup[k + 1] = Fup[k] * (1 - epslw[k]) + epslw[k] * kSigma * Tlev[k] * Tlev[k] * Tlev[k] * Tlev[k];
-    const double mu = std::fmax(cosz, 0.05);
-    const double cldalb = lwp / (lwp + 60);
-    double Sdn = kS0 * cosz * 0.95 * (1 - cldalb);
-    for (int k = nz - 1; k >= 0; --k) {
-      const long c = g.id(i, j, k);
-      const double du = rho[k] * qvcol[k] * dzc[k];
-      const double dl = rho[k] * (s.qc[c] > 0 ? s.qc[c] : 0.0) * dzc[k] * 1000;
-      const double tau = std::exp(-(0.008 * du + 0.15 * dl) / mu);
-      const double ab = Sdn * (1 - tau);
-      swabs[k] = ab;
-      Sdn *= tau;
-    }
+    const double Sdn = sw_column(nz, cosz, lwp, du.data(), dl.data(), swabs.data());
     const double swn = Sdn * (1 - alb);
     for (int k = 0; k < nz; ++k) {
       const long c = g.id(i, j, k);
@@ -1053,34 +1117,11 @@ struct orc_model {
     const double lwu = 0.97 * kSigma * Ts * Ts * Ts * Ts;
     const double G = swn + 0.97 * lwd - lwu - rho[0] * kCp * shf * pi1 - rho[0] * kLv * qflx;
     if (onland) {
-      static const double kDz[4] = {0.1, 0.3, 0.6, 1.0};
-      const double Cso = soil_C(soilw[c2]), Kso = soil_K(soilw[c2]);
       const long n2 = g.n2();
-      double Tso[4], A[4], B[4], C[4], D[4];
+      double Tso[4];
       for (int L = 0; L < 4; ++L) Tso[L] = soilt[L * n2 + c2];
-      for (int L = 0; L < 4; ++L) {
-        const double cap = Cso * kDz[L] / dt;
-        const double ku = (L > 0) ? Kso / (0.5 * (kDz[L - 1] + kDz[L])) : 0.0;
-        const double kl = (L < 3) ? Kso / (0.5 * (kDz[L] + kDz[L + 1])) : Kso / kDz[L];
-        A[L] = -ku;
-        C[L] = (L < 3) ? -kl : 0.0;
-        B[L] = cap + ku + kl;
-        D[L] = cap * Tso[L];
-      }
-      D[0] += G;
-      D[3] += (Kso / kDz[3]) * tdeep[c2];
-      for (int L = 1; L < 4; ++L) {
-        const double w = A[L] / B[L - 1];
-        B[L] -= w * C[L - 1];
-        D[L] -= w * D[L - 1];
-      }
-      Tso[3] = D[3] / B[3];
-      for (int L = 2; L >= 0; --L) Tso[L] = (D[L] - C[L] * Tso[L + 1]) / B[L];
-      for (int L = 0; L < 4; ++L) {
-        if (Tso[L] < 200) Tso[L] = 200;
-        if (Tso[L] > 350) Tso[L] = 350;
-        soilt[L * n2 + c2] = Tso[L];
-      }
+      soil_step(dt, soilw[c2], G, tdeep[c2], Tso);
+      for (int L = 0; L < 4; ++L) soilt[L * n2 + c2] = Tso[L];
       tsk[c2] = Tso[0];
     }
     // diffusivity (:292-305)
@@ -1450,6 +1491,31 @@ void orc_thomas(int n, const double* a, const double* b, const double* c, const 
 }
 
 double orc_qsat_w(double p, double T) { return qsat_tetens(p, T); }
+double orc_sfc_cosz(int doy, double utc, double lat, double lon) {
+  return orc_model::solar_cosz(doy, utc, lat, lon);
+}
+void orc_sfc_louis(double Rib, double z1, double z0, double* cd_ch) {
+  orc_model::louis(Rib, z1, z0, &cd_ch[0], &cd_ch[1]);
+}
+void orc_sfc_diag(double V1, double Rib, double z1, double z0, double ths, double th1,
+                  double* u10_th2) {
+  orc_model::sfc_diag(V1, Rib, z1, z0, ths, th1, &u10_th2[0], &u10_th2[1]);
+}
+void orc_sfc_lw(int nz, const double* T, const double* eps, double Ts, double* Fup,
+                double* Fdn) {
+  orc_model::lw_fluxes(nz, T, eps, Ts, Fup, Fdn);
+}
+double orc_sfc_sw(int nz, double cosz, double lwp, const double* du, const double* dl,
+                  double* swabs) {
+  return orc_model::sw_column(nz, cosz, lwp, du, dl, swabs);
+}
+void orc_sfc_soil(double dt, double w, double G, double tdeep, double* Tso) {
+  orc_model::soil_step(dt, w, G, tdeep, Tso);
+}
+double orc_sfc_soil_heat(double w, int which) {
+  return which == 0 ? orc_model::soil_C(w) : orc_model::soil_K(w);
+}
+
 double orc_sfc_psi(int heat, double zeta) {
   return heat ? orc_model::psi_h(zeta) : orc_model::psi_m(zeta);
 }

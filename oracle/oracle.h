@@ -26,12 +26,21 @@
  * reference fixture):
  *  - the values of CUDA's device exp/pow/log/sqrt inside Kessler, Tetens and
  *    the surface physics (host std:: math here, OCML on the GPU);
- *  - in the surface physics (surface.cu:87-387): solar geometry and the
- *    broadband radiation column, the four-layer soil, the Louis (1979) bulk
- *    exchange coefficients, the nonlocal boundary-layer height and
- *    counter-gradient term, and the 2 m / 10 m diagnostics. Pinned: the
- *    Businger-Dyer functions (against their defining integrals) and the
- *    implicit column diffusion (against a banded library solve);
+ *  - in the surface physics (surface.cu:87-387), the scheme's empirical
+ *    constants, which nothing outside the reference fixes: the Louis
+ *    stability coefficients (9.4, 7.4, 5.3, 4.7), the grey absorption
+ *    coefficients (0.22, 0.30 long-wave; 0.008, 0.15 short-wave), the 0.95
+ *    clear-sky factor and the lwp / (lwp + 60) cloud albedo, the soil heat
+ *    capacity and conductivity fits, the land/sea roughness and albedo, the
+ *    local-K mixing length and Ri functions, and the nonlocal boundary-layer
+ *    scheme (Ri_c = 0.25, the 6.8 excess, the K profile). Pinned: the
+ *    Businger-Dyer functions (defining integrals), the column diffusion and
+ *    the soil step (banded library solves, heat conservation), the solar
+ *    geometry (declination at the equinoxes and solstices, 15 deg per hour),
+ *    the long-wave fluxes (transparent, black and isothermal columns), the
+ *    short-wave column (conservation, Beer's law, slant path, solar
+ *    constant), the Louis neutral limit and the neutral log-law 10 m / 2 m
+ *    diagnostics;
  *  - the seeded ensemble perturbation of the initial state
  *    (main.cpp:105-144), which draws from libstdc++'s
  *    std::normal_distribution<float> (implementation-defined).
@@ -108,6 +117,21 @@ void orc_sfc_diffuse(int nz, double dt, const double* rho, const double* rhow, c
                      const double* dzw, const double* Kw, double src0, double* phi,
                      const double* fcg);
 double orc_qsat_i(double p, double T);
+/* Surface-physics pieces (surface.cu): cosine of the solar zenith angle;
+ * Louis drag/heat coefficients into cd_ch[2]; 10 m wind and 2 m potential
+ * temperature into u10_th2[2]; grey long-wave interface fluxes (nz + 1);
+ * short-wave layer absorption (returns the surface flux); one implicit step
+ * of the four soil layers (Tso[4] in place); soil heat capacity (which = 0)
+ * or conductivity (which = 1) at water content w. */
+double orc_sfc_cosz(int doy, double utc, double lat, double lon);
+void orc_sfc_louis(double Rib, double z1, double z0, double* cd_ch);
+void orc_sfc_diag(double V1, double Rib, double z1, double z0, double ths, double th1,
+                  double* u10_th2);
+void orc_sfc_lw(int nz, const double* T, const double* eps, double Ts, double* Fup, double* Fdn);
+double orc_sfc_sw(int nz, double cosz, double lwp, const double* du, const double* dl,
+                  double* swabs);
+void orc_sfc_soil(double dt, double w, double G, double tdeep, double* Tso);
+double orc_sfc_soil_heat(double w, int which);
 
 #ifdef __cplusplus
 }

@@ -46,6 +46,16 @@ def olib() -> C.CDLL:
         L.orc_sfc_psi.restype = C.c_double
         L.orc_sfc_psi.argtypes = [C.c_int, C.c_double]
         L.orc_sfc_diffuse.argtypes = [C.c_int, C.c_double] + [_DP] * 5 + [C.c_double, _DP, _DP]
+        L.orc_sfc_cosz.restype = C.c_double
+        L.orc_sfc_cosz.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
+        L.orc_sfc_louis.argtypes = [C.c_double] * 3 + [_DP]
+        L.orc_sfc_diag.argtypes = [C.c_double] * 6 + [_DP]
+        L.orc_sfc_lw.argtypes = [C.c_int, _DP, _DP, C.c_double, _DP, _DP]
+        L.orc_sfc_sw.restype = C.c_double
+        L.orc_sfc_sw.argtypes = [C.c_int, C.c_double, C.c_double, _DP, _DP, _DP]
+        L.orc_sfc_soil.argtypes = [C.c_double] * 4 + [_DP]
+        L.orc_sfc_soil_heat.restype = C.c_double
+        L.orc_sfc_soil_heat.argtypes = [C.c_double, C.c_int]
         L.orc_tendencies.argtypes = [C.c_void_p, C.c_double]
         L.orc_acoustic.argtypes = [C.c_void_p, C.c_double, C.c_int]
         L.orc_kessler.argtypes = [C.c_void_p, C.c_double]

@@ -45,6 +45,17 @@ MUTATIONS = {
                            "if (zeta >= 0) return -4 * std::fmin(zeta, 2.0);\n    const double x = "
                            "std::pow(1 - 16 * zeta, 0.25);\n    return 2 * std::log((1 + x * x)", 0),
     "diffusion_upper_conductance": ("      B[k] = m + a + c;", "      B[k] = m + a + 0.5 * c;", 0),
+    "declination_day_offset": ("(284 + doy) / 365.0", "(280 + doy) / 365.0", 0),
+    "hour_angle_rate": ("0.2618 * (hsol - 12)", "0.25 * (hsol - 12)", 0),
+    "solar_time_longitude": ("utc + lon / 15", "utc + lon / 12", 0),
+    "lw_surface_emissivity": ("Fup[0] = 0.97 * kSigma", "Fup[0] = 0.95 * kSigma", 0),
+    "sw_beam_path": ("std::exp(-(0.008 * du[k] + 0.15 * dl[k]) / mu)",
+                     "std::exp(-(0.008 * du[k] + 0.15 * dl[k]) / std::sqrt(mu))", 0),
+    "log_law_10m": ("std::log(10 / z0)", "std::log(9 / z0)", 0),
+    "heat_roughness": ("const double z0h = z0 * 0.1;", "const double z0h = z0 * 0.2;", 0),
+    "soil_layer_depth": ("{0.1, 0.3, 0.6, 1.0}", "{0.1, 0.3, 0.5, 1.0}", 0),
+    "louis_neutral_kappa": ("double a2 = kKappa / std::log(z1 / z0);",
+                            "double a2 = 0.41 / std::log(z1 / z0);", 0),
 }
 
 
