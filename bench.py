@@ -522,6 +522,8 @@ def main() -> int:
                     help="last checkpoint of the supercell error-growth study (600 = 1 h)")
     ap.add_argument("--mixed-only", action="store_true",
                     help="only the timed mixed-precision headline steps (profiling runs)")
+    ap.add_argument("--fp64-only", action="store_true",
+                    help="only the timed FP64 steps of the headline case (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -544,6 +546,12 @@ def main() -> int:
         case = Case(write_regional(tmp, W + K))
         g = case.grid
         N = g.n_interior
+        if args.fp64_only:
+            log(f"[bench] fp64: {W} warmup + {K} timed steps on {g.nx}x{g.ny}x{g.nz}")
+            gi_d, t_d = timed_run(case, "fp64", W, K)
+            print(json.dumps({"metric": "regional fp64 (profiling run)", "value": N * K / t_d,
+                              "ms_per_step": 1e3 * t_d / K, "copy_probe_GBps": copy_bw}))
+            return 0
         log(f"[bench] mixed: {W} warmup + {K} timed steps on {g.nx}x{g.ny}x{g.nz}")
         gi_m, t_m = timed_run(case, "mixed", W, K)
         log(f"[bench] mixed {t_m:.3f} s -> {1e3 * t_m / K:.3f} ms/step")

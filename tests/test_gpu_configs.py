@@ -134,6 +134,30 @@ def test_regional_boundary_files_and_time_interpolation_match_the_oracle(tmp_pat
         assert np.array_equal(c.array(f"bdy_hi_{f}"), oc.array(f"bdy_hi_{f}"))
 
 
+def test_context_created_after_others_crossed_boundary_times_starts_from_the_first_files(
+        tmp_path):
+    """Contexts of one case run one after another (as the benchmark runs its
+    mixed and FP64 passes): a context created once another has crossed
+    boundary times still starts from the first two boundary files, so it
+    steps exactly like a context created before any stepping."""
+    from wfe_amd import Case, Integrator
+    from wfe_amd.lam_case import write_lam_case
+    ini = write_lam_case(tmp_path / "lam", nx=32, ny=24, nz=16, dx=12000.0, top=12000.0,
+                         seed=6, hours=3, bdy_interval_h=1, bdy_shift_cells_per_h=3.0)
+    c = Case(ini)
+    dt = c.scalar("dt")
+    early = Integrator.from_case(c, "fp64")
+    first = Integrator.from_case(c, "fp64")
+    for s in range(130):  # past the 1 h and 2 h files
+        first.step(dt, s * dt)
+    late = Integrator.from_case(c, "fp64")
+    for s in range(70):
+        early.step(dt, s * dt)
+        late.step(dt, s * dt)
+    for f in STATE:
+        assert np.array_equal(early.get(f), late.get(f)), f
+
+
 def test_cfg4_uneven_ensemble_members_match_their_oracles(tmp_path):
     """Seeded regional members at two grid sizes and time steps (12 km, 60 s
     and 3 km, 15 s) with member theta' noise, advanced together by one graph

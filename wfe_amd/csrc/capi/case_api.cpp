@@ -193,10 +193,29 @@ int wfe_case_create_ctx(const wfe_case* cs, int precision, wfe_ctx** out) {
     rc = wfe_upload(*out, fields[q], c.ic[q].data(), c.ic[q].size());
   if (rc == WFE_OK && c.bdy.active) {
     rc = wfe_set_bdy_weights(*out, c.bdy.wgt.data(), c.bdy.wgt.size());
+    // A new context starts at the first two boundary times. The host pair
+    // follows the context advanced last, so once any context of this case has
+    // crossed a boundary time the first pair is read again from its files.
+    std::array<std::vector<double>, 5> lo0, hi0;
+    const std::array<std::vector<double>, 5>* lo = &c.bdy.lo;
+    const std::array<std::vector<double>, 5>* hi = &c.bdy.hi;
+    if (c.bdy.idx_hi != 1) {
+      bool ok = false;
+      rc = guarded_case([&] {
+        ok = c.bdy.load_into(c.g, c.input, c.base.thb, c.base.pib, 0, lo0) &&
+             c.bdy.load_into(c.g, c.input, c.base.thb, c.base.pib, std::min(1, c.input.n_bdy - 1),
+                             hi0);
+      });
+      if (rc == WFE_OK && !ok) {
+        wfe::set_last_error("cannot read the first boundary files");
+        rc = WFE_E_IO;
+      }
+      lo = &lo0;
+      hi = &hi0;
+    }
     for (int v = 0; v < 5 && rc == WFE_OK; ++v) {
-      rc = wfe_set_bdy_target(*out, 0, v, c.bdy.lo[v].data(), c.bdy.lo[v].size());
-      if (rc == WFE_OK)
-        rc = wfe_set_bdy_target(*out, 1, v, c.bdy.hi[v].data(), c.bdy.hi[v].size());
+      rc = wfe_set_bdy_target(*out, 0, v, (*lo)[v].data(), (*lo)[v].size());
+      if (rc == WFE_OK) rc = wfe_set_bdy_target(*out, 1, v, (*hi)[v].data(), (*hi)[v].size());
     }
   }
   if (rc == WFE_OK && c.sfc.on) {

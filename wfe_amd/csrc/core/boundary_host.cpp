@@ -39,7 +39,7 @@ std::vector<double> davies_weights(const GDims& g, const DynParams& dp, int widt
 
 bool BoundaryHost::load_into(const GDims& g, const InputData& in,
                              const std::vector<double>& thb3, const std::vector<double>& pib3,
-                             int idx, std::array<std::vector<double>, 5>& dst) {
+                             int idx, std::array<std::vector<double>, 5>& dst) const {
   std::vector<double> raw[5];
   if (!in.load_bdy(g, idx, raw)) return false;
   dst = perturbation_state(g, thb3, pib3, raw);

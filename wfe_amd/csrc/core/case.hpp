@@ -34,7 +34,7 @@ struct BoundaryHost {
   // Targets of boundary file idx into dst; false on a file error.
   bool load_into(const GDims& g, const InputData& in, const std::vector<double>& thb3,
                  const std::vector<double>& pib3, int idx,
-                 std::array<std::vector<double>, 5>& dst);
+                 std::array<std::vector<double>, 5>& dst) const;
 };
 
 // A fully assembled case: everything src/main.cpp:248-383 builds before the
