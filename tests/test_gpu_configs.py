@@ -158,6 +158,28 @@ def test_context_created_after_others_crossed_boundary_times_starts_from_the_fir
         assert np.array_equal(early.get(f), late.get(f)), f
 
 
+@pytest.mark.parametrize("extra", [{"bdy_width": 7}, {"bdy_width": 4, "nudge_tau": 5400}],
+                         ids=["rim", "nudged"])
+def test_relaxation_rim_and_interior_nudging_match_the_oracle(tmp_path, extra):
+    """Davies relaxation over the rim only (launched over the rim columns) and
+    with interior nudging (weights everywhere, launched over the grid), FP64
+    against the oracle over boundary-file crossings."""
+    from wfe_amd import Case, Integrator
+    from wfe_amd.lam_case import write_lam_case
+    ini = write_lam_case(tmp_path / "lam", nx=36, ny=26, nz=16, dx=12000.0, top=12000.0,
+                         seed=12, hours=2, bdy_interval_h=1, extra=extra)
+    c, oc = Case(ini), OracleCase(ini)
+    dt = c.scalar("dt")
+    om = oc.model()
+    gd = Integrator.from_case(c, "fp64")
+    for s in range(70):
+        for m in (om, gd):
+            m.step(dt, s * dt)
+    e = _errs(c.grid, gd, om, STATE)
+    print(extra, {k: f"{v:.2e}" for k, v in e.items()}, flush=True)
+    assert all(v <= 1e-10 for v in e.values()), e
+
+
 def test_cfg4_uneven_ensemble_members_match_their_oracles(tmp_path):
     """Seeded regional members at two grid sizes and time steps (12 km, 60 s
     and 3 km, 15 s) with member theta' noise, advanced together by one graph

@@ -442,6 +442,7 @@ class Ctx final : public CtxBase {
   bool bdy_active_ = false;
   T *bdy_lo_[5]{}, *bdy_hi_[5]{};
   T* bdy_wgt_ = nullptr;
+  int bdy_rim_ = 0;  // relaxation launched over this rim only (0: whole grid)
   double tfrac_ = 0;
   unsigned* d_absmax_ = nullptr;
   // Surface/PBL column physics after every step (set_physics): FP64 fields
@@ -941,6 +942,14 @@ void Ctx<T>::set_bdy_weights(const double* w, size_t n) {
   }
   const std::vector<T> tmp = to_dev<T>(w, false);
   WFE_HIP(hipMemcpy(bdy_wgt_, tmp.data(), n2_ * sizeof(T), hipMemcpyHostToDevice));
+  // Rim width: every interior column with a positive weight lies within
+  // bdy_rim_ cells of an edge (0: relax over the whole grid).
+  int rim = 0;
+  for (int j = 0; j < g_.ny; ++j)
+    for (int i = 0; i < g_.nx; ++i)
+      if (w[(size_t)(j + g_.ng) * hNX_ + (i + g_.ng)] > 0.0)
+        rim = std::max(rim, 1 + std::min(std::min(i, g_.nx - 1 - i), std::min(j, g_.ny - 1 - j)));
+  bdy_rim_ = 2 * rim < std::min(g_.nx, g_.ny) ? rim : 0;
   bdy_active_ = true;
   ++gen_;
 }
@@ -1101,8 +1110,14 @@ void Ctx<T>::bdy_relax(const StateSet<T>& s) {
   R.t[4] = tq_[0];
   R.wgt = bdy_wgt_;
   R.tf = tf_dev_;
-  const dim3 grd = lv_grid(g_.nx, g_.ny, g_.nz);
-  TIMED("bdy_relax", dev::k_bdy_relax<T><<<grd, lv_block(), 0, stream_>>>(g_, R));
+  if (bdy_rim_ > 0) {
+    const int ncol = 2 * bdy_rim_ * g_.nx + 2 * bdy_rim_ * (g_.ny - 2 * bdy_rim_);
+    TIMED("bdy_relax", dev::k_bdy_relax_rim<T><<<dim3(cdiv(ncol, 256), (unsigned)g_.nz), 256, 0,
+                                                 stream_>>>(g_, R, bdy_rim_));
+  } else {
+    const dim3 grd = lv_grid(g_.nx, g_.ny, g_.nz);
+    TIMED("bdy_relax", dev::k_bdy_relax<T><<<grd, lv_block(), 0, stream_>>>(g_, R));
+  }
   check_launch("bdy_relax");
 }
 

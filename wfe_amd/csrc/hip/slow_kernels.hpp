@@ -551,11 +551,8 @@ struct RelaxSet {
 };
 
 template <class T>
-__global__ void k_bdy_relax(Grid g, RelaxSet<T> R) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = blockIdx.z;
-  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+__device__ __forceinline__ void bdy_relax_cell(const Grid& g, const RelaxSet<T>& R, int i, int j,
+                                               int k) {
   const int c = g.id(i, j, k);
   const T w = R.wgt[g.id2(i, j)];
   const T tf = *R.tf;
@@ -566,6 +563,42 @@ __global__ void k_bdy_relax(Grid g, RelaxSet<T> R) {
     R.t[v][c] += w * (((T)1 - tf) * R.lo[v][c] + tf * R.hi[v][c] - R.f[q][c]);
   }
   R.t[3][c] = w * (((T)1 - tf) * R.lo[3][c] + tf * R.hi[3][c] - (T)R.pip[c]);
+}
+
+template <class T>
+__global__ void k_bdy_relax(Grid g, RelaxSet<T> R) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+  bdy_relax_cell(g, R, i, j, k);
+}
+
+// The same over the rim only: the columns within rw cells of an edge (every
+// column with a positive weight is), enumerated as the south and north
+// strips of rw full rows, then the west and east strips of rw columns
+// between them; blockIdx.y is the level.
+template <class T>
+__global__ void k_bdy_relax_rim(Grid g, RelaxSet<T> R, int rw) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  const int strip = rw * g.nx, side = rw * (g.ny - 2 * rw);
+  if (q >= 2 * strip + 2 * side) return;
+  int i, j;
+  if (q < 2 * strip) {
+    const int r = q / g.nx;
+    i = q - r * g.nx;
+    j = r < rw ? r : g.ny - 2 * rw + r;
+  } else {
+    q -= 2 * strip;
+    const int east = q >= side;
+    if (east) q -= side;
+    const int r = q / rw;
+    j = rw + r;
+    const int ii = q - r * rw;
+    i = east ? g.nx - rw + ii : ii;
+  }
+  bdy_relax_cell(g, R, i, j, k);
 }
 
 // One device scalar set in stream order (kept out of kernel arguments so a
