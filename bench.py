@@ -392,7 +392,10 @@ def adjoint_benchmark(nsteps: int, ckpt: int):
     the mean lowest-level kinetic energy in a seeded 20 x 20 km box with
     respect to the initial (u, v, w, theta', pi') through nsteps steps with a
     checkpoint every ckpt steps; one-step dot-product test; finite-difference
-    check of the gradient along a seeded direction for eps = 1e-3 .. 1e-7."""
+    check of the gradient along a seeded direction for eps = 1e-3 .. 1e-7.
+    Then the FP32 production gradient (mixed context: FP32 forward and
+    trajectory, FP64 adjoint) from the same initial state, timed and compared
+    with the FP64 one field by field (relative L2 over the interior)."""
     from wfe_amd import Case, Integrator
     from wfe_amd.adjoint import FIELDS5, ad_step, box_weights, ke_gradient, tl_step
 
@@ -446,6 +449,36 @@ def adjoint_benchmark(nsteps: int, ckpt: int):
         jp = J_of({f: x0[f] + eps * d[f] for f in FIELDS5})
         jm = J_of({f: x0[f] - eps * d[f] for f in FIELDS5})
         fd[f"{eps:.0e}"] = abs((jp - jm) / (2 * eps) - gd) / abs(gd)
+
+    # FP32 production gradient against the FP64 check
+    g32 = Integrator.from_case(case, "mixed")
+    for f in FIELDS5:
+        g32.set(f, x0[f])
+    dy32 = tl_step(g32, dt, dx)
+    xb32 = ad_step(g32, dt, yb)
+    lhs32 = sum(float(np.dot(dy32[f][mask], yb[f][mask])) for f in FIELDS5)
+    rhs32 = sum(float(np.dot(dx[f][mask], xb32[f][mask])) for f in FIELDS5)
+    g32.sync()
+    t0 = time.perf_counter()
+    J32, grad32 = ke_gradient(g32, dt, nsteps, ckpt, w)
+    t_adj32 = time.perf_counter() - t0
+    gm = Integrator.from_case(case, "mixed")
+    for f in FIELDS5:
+        gm.set(f, x0[f])
+    gm.sync()
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        gm.step(dt)
+    gm.sync()
+    t_fwd32 = time.perf_counter() - t0
+    gerr = {f: float(np.linalg.norm(grad32[f][mask] - grad[f][mask]) /
+                     np.linalg.norm(grad[f][mask])) for f in FIELDS5}
+    fp32 = {"J": J32, "J_rel_vs_fp64": abs(J32 - J) / abs(J),
+            "dot_product_rel": abs(lhs32 - rhs32) / abs(lhs32),
+            "gradient_rel_l2_vs_fp64": gerr,
+            "adjoint_s": t_adj32, "forward_s": t_fwd32,
+            "adjoint_over_forward": t_adj32 / t_fwd32,
+            "speedup_vs_fp64_adjoint": t_adj / t_adj32}
     return {"workload": "dry 120x120x40 (supercell levels), isentropic, periodic, FP64; "
                         f"J = mean lowest-level KE in box i {i0}..{i0 + 20}, j {j0}..{j0 + 20}",
             "steps": nsteps, "checkpoint_every": ckpt,
@@ -453,7 +486,8 @@ def adjoint_benchmark(nsteps: int, ckpt: int):
             "dot_product_rel": abs(lhs - rhs) / abs(lhs),
             "gradient_fd_rel_err": fd, "gradient_dot_direction": gd,
             "adjoint_s": t_adj, "forward_s": t_fwd,
-            "adjoint_over_forward": t_adj / t_fwd}
+            "adjoint_over_forward": t_adj / t_fwd,
+            "fp32_production": fp32}
 
 
 def cpu_model() -> str:

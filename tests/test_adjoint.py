@@ -1,6 +1,8 @@
-"""Tangent-linear and adjoint of the dry large step (FP64): dot-product
-tests, tangent-linear against finite differences of the forward step, and
-the kinetic-energy gradient against finite differences of the objective."""
+"""Tangent-linear and adjoint of the dry large step: dot-product tests,
+tangent-linear against finite differences of the forward step, and the
+kinetic-energy gradient against finite differences of the objective (FP64
+contexts); dot products and the gradient about the FP32 trajectory of a mixed
+context against the FP64 check."""
 import numpy as np
 import pytest
 
@@ -33,9 +35,9 @@ def random_state(g, rng, scale):
 SCALE = {"u": 1.0, "v": 1.0, "w": 0.3, "thp": 0.5, "pip": 1e-4}
 
 
-def make(c, seed):
+def make(c, seed, precision="fp64"):
     from wfe_amd import Integrator
-    gi = Integrator.from_case(c, "fp64")
+    gi = Integrator.from_case(c, precision)
     for f, a in perturbed_state(c, seed).items():
         gi.set(f, a)
     return gi
@@ -45,12 +47,14 @@ def dot(a, b, mask):
     return sum(float(np.dot(a[f][mask], b[f][mask])) for f in FIELDS5)
 
 
-@pytest.mark.parametrize("seed", [1, 2])
-def test_dot_product_one_step(seed):
+@pytest.mark.parametrize("seed,precision", [(1, "fp64"), (2, "fp64"), (1, "mixed")])
+def test_dot_product_one_step(seed, precision):
+    """<M dx, yb> = <dx, M^T yb>. In a mixed context the trajectory is FP32 and
+    the TL/AD arithmetic FP64, so the identity holds to FP64 rounding too."""
     from wfe_amd.adjoint import ad_step, tl_step
     c = dry_case()
     g = c.grid
-    gi = make(c, seed)
+    gi = make(c, seed, precision)
     rng = np.random.default_rng(100 + seed)
     mask = interior_mask(g)
     dx = random_state(g, rng, SCALE)
@@ -61,6 +65,25 @@ def test_dot_product_one_step(seed):
     lhs, rhs = dot(dy, yb, mask), dot(dx, xb, mask)
     print("dot product", lhs, rhs, abs(lhs - rhs) / abs(lhs))
     assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_tangent_linear_after_an_adjoint_call_is_unchanged(precision):
+    """The TL and AD passes share their perturbation buffers; cells the TL
+    reads but does not write (rigid-lid w levels, substep ghost rings) must
+    not carry what the adjoint pass left there."""
+    from wfe_amd.adjoint import ad_step, tl_step
+    c = dry_case()
+    g = c.grid
+    gi = make(c, 12, precision)
+    rng = np.random.default_rng(112)
+    dx, yb = random_state(g, rng, SCALE), random_state(g, rng, SCALE)
+    dt = c.scalar("dt")
+    before = tl_step(gi, dt, dx)
+    ad_step(gi, dt, yb)
+    after = tl_step(gi, dt, dx)
+    for f in FIELDS5:
+        assert np.array_equal(before[f], after[f]), f
 
 
 def test_tangent_linear_matches_finite_differences():
@@ -156,16 +179,56 @@ def test_torch_autograd_function():
         np.testing.assert_allclose(t.grad.numpy(), 2.0 * grad[f], rtol=1e-9, atol=1e-12 * scale)
 
 
+def test_mixed_trajectory_tangent_linear_tracks_the_fp64_one():
+    """The TL about a mixed context's FP32 trajectory against the TL about the
+    FP64 trajectory from the same initial state: they differ by the
+    trajectory's FP32 storage error only."""
+    from wfe_amd.adjoint import tl_step
+    c = dry_case()
+    g = c.grid
+    g64, g32 = make(c, 3), make(c, 3, "mixed")
+    rng = np.random.default_rng(17)
+    dx = random_state(g, rng, SCALE)
+    dt = c.scalar("dt")
+    d64, d32 = tl_step(g64, dt, dx), tl_step(g32, dt, dx)
+    mask = interior_mask(g)
+    errs = {f: float(np.linalg.norm(d32[f][mask] - d64[f][mask]) /
+                     np.linalg.norm(d64[f][mask])) for f in FIELDS5}
+    print("TL mixed vs fp64", {f: f"{e:.2e}" for f, e in errs.items()})
+    assert max(errs.values()) < 1e-4, errs
+
+
+def test_mixed_trajectory_gradient_tracks_the_fp64_gradient():
+    """FP32 production gradient (mixed context: FP32 forward and trajectory,
+    FP64 adjoint) against the FP64 check over 6 steps, checkpoint every 4."""
+    from wfe_amd.adjoint import box_weights, ke_gradient
+    c = dry_case()
+    g = c.grid
+    dt = c.scalar("dt")
+    w = box_weights(g, 6, 14, 5, 13)
+    g64, g32 = make(c, 4), make(c, 4, "mixed")
+    x32 = {f: g32.get(f) for f in FIELDS5}
+    J64, gr64 = ke_gradient(g64, dt, 6, 4, w)
+    J32, gr32 = ke_gradient(g32, dt, 6, 4, w)
+    for f in FIELDS5:  # the mixed context is left at its initial state too
+        np.testing.assert_array_equal(g32.get(f), x32[f])
+    mask = interior_mask(g)
+    errs = {f: float(np.linalg.norm(gr32[f][mask] - gr64[f][mask]) /
+                     np.linalg.norm(gr64[f][mask])) for f in FIELDS5}
+    print("J", J64, J32, abs(J32 - J64) / J64, "gradient mixed vs fp64",
+          {f: f"{e:.2e}" for f, e in errs.items()})
+    assert abs(J32 - J64) <= 1e-5 * abs(J64)
+    assert max(errs.values()) < 1e-4, errs
+
+
 def test_scope_is_enforced():
     from wfe_amd import Integrator, WfeError
     from wfe_amd.adjoint import tl_step
-    c = dry_case(bc_x="open")
-    gi = Integrator.from_case(c, "fp64")
-    with pytest.raises(WfeError):
-        tl_step(gi, c.scalar("dt"), {})
-    gm = Integrator.from_case(dry_case(), "mixed")
-    with pytest.raises(WfeError):
-        tl_step(gm, 2.0, {})
+    for precision in ("fp64", "mixed"):
+        c = dry_case(bc_x="open")
+        gi = Integrator.from_case(c, precision)
+        with pytest.raises(WfeError, match="lateral boundaries"):
+            tl_step(gi, c.scalar("dt"), {})
 
 
 def test_scope_is_checked_again_after_relaxation_is_switched_on():

@@ -105,6 +105,14 @@ def test_cfg3_regional_all_1440_steps_mixed_with_boundary_files(tmp_path):
     # the product's boundary manager advanced through the files (the targets
     # it holds now differ from the ones it started with)
     assert not np.array_equal(c.array("bdy_hi_thp"), lo0)
+    # 24 h against the FP64 path on the same case: the north-star bound of
+    # 1e-4 (relative L2 over the interior) on u, v, w and theta'
+    gd = Integrator.from_case(c, "fp64")
+    for s in range(1440):
+        gd.step(dt, s * dt)
+    e = _errs(g, gi, gd, ("u", "v", "w", "thp", "pip", "qv"))
+    print("cfg3 mixed vs fp64 after 24 h", {k: f"{v:.2e}" for k, v in e.items()}, flush=True)
+    assert all(e[f] <= 1e-4 for f in ("u", "v", "w", "thp")), e
 
 
 def test_regional_boundary_files_and_time_interpolation_match_the_oracle(tmp_path):
@@ -222,7 +230,8 @@ def _box(g, rng, size_cells):
 
 def test_cfg5_adjoint_at_the_supercell_shape():
     """Dot-product test and finite differences of the gradient at 120 x 120 x
-    40 with the supercell's levels: 20 steps with a checkpoint every 5."""
+    40 with the supercell's levels: 20 steps with a checkpoint every 5; then
+    the FP32 production gradient against the FP64 one."""
     from wfe_amd import Case, Integrator
     from wfe_amd.adjoint import FIELDS5, ad_step, ke_gradient, tl_step
     c = Case(CASES / "cfg5_adjoint_dry.ini")
@@ -272,6 +281,26 @@ def test_cfg5_adjoint_at_the_supercell_shape():
           J_of({f: x0[f] - eps * d[f] for f in FIELDS5})) / (2 * eps)
     print("cfg5 gradient vs FD", gd, fd, abs(fd - gd) / abs(gd), flush=True)
     assert abs(fd - gd) <= 1e-5 * abs(gd)
+    # FP32 production (mixed context: FP32 forward and trajectory, FP64
+    # adjoint) against this FP64 check from the same initial state
+    g32 = Integrator.from_case(c, "mixed")
+    for f in FIELDS5:
+        g32.set(f, x0[f])
+    # dot product about the FP32 trajectory: the stretched levels' dzeta is
+    # not representable in FP32, which TL and AD must treat alike
+    dy = tl_step(g32, dt, dx)
+    xb = ad_step(g32, dt, yb)
+    lhs = sum(float(np.dot(dy[f][mask], yb[f][mask])) for f in FIELDS5)
+    rhs = sum(float(np.dot(dx[f][mask], xb[f][mask])) for f in FIELDS5)
+    print("cfg5 FP32-trajectory dot product", lhs, rhs, abs(lhs - rhs) / abs(lhs), flush=True)
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+    J32, grad32 = ke_gradient(g32, dt, nsteps, 5, w)
+    errs = {f: float(np.linalg.norm(grad32[f][mask] - grad[f][mask]) /
+                     np.linalg.norm(grad[f][mask])) for f in FIELDS5}
+    print("cfg5 FP32 production vs FP64: J", abs(J32 - J) / abs(J), "gradient",
+          {f: f"{e:.2e}" for f, e in errs.items()}, flush=True)
+    assert abs(J32 - J) <= 1e-5 * abs(J)
+    assert max(errs.values()) <= 1e-3, errs
 
 
 def test_adjoint_gradient_against_finite_differences_of_the_oracle():

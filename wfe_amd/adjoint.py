@@ -1,10 +1,12 @@
 """Tangent-linear and adjoint of the dry large step through the C ABI, and a
 torch.autograd.Function for the lowest-level kinetic-energy objective.
 
-Scope (enforced by the library): plain FP64 contexts of dry cases with
+Scope (enforced by the library): FP64 or mixed contexts of dry cases with
 periodic lateral boundaries, flat terrain, no Coriolis, no diffusion, no w
 damping and no boundary relaxation. Fields are (u, v, w, thp, pip) in the
-padded layout of the case's grid.
+padded layout of the case's grid. In a mixed context the linearisation is
+about the FP32 trajectory the context itself computes; perturbations,
+adjoint variables and the objective are FP64 in both kinds of context.
 """
 from __future__ import annotations
 
@@ -74,7 +76,7 @@ def box_weights(grid, i0: int, i1: int, j0: int, j1: int) -> np.ndarray:
 def kinetic_energy_function():
     """torch.autograd.Function mapping the initial (u, v, w, thp, pip)
     tensors (padded layout, float64) to the mean lowest-level kinetic energy
-    J over `nsteps` steps of an FP64 integrator; backward returns dJ/dx0
+    J over `nsteps` steps of an integrator; backward returns dJ/dx0
     from the hand-written adjoint with checkpointing."""
     import torch
 

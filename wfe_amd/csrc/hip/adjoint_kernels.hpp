@@ -10,10 +10,22 @@
 //
 // Every AD kernel is the statement-by-statement transpose of its TL kernel;
 // contributions to shared inputs are scattered with FP64 atomics.
+//
+// The trajectory may be stored in FP32 (a mixed-precision forward): its
+// arrays are read through Traj<S>, which widens every element to FP64 at the
+// load, so all arithmetic -- trajectory averages and coefficients included --
+// is FP64 and TL and AD use bit-identical coefficients.
 #include "hip/device_common.hpp"
 #include "hip/slow_kernels.hpp"
 
 namespace wfe::dev {
+
+// Read-only trajectory array of storage type S, read as FP64.
+template <class S>
+struct Traj {
+  const S* p;
+  __device__ __forceinline__ double operator[](int i) const { return (double)p[i]; }
+};
 
 // 5th-order upwind interface weights on q[-2..3] for the branch of vel
 // (iface5: centred minus / plus the upwind term).
@@ -36,18 +48,21 @@ __device__ __forceinline__ void iface5_weights(double vel, double w[6]) {
   }
 }
 
-__device__ __forceinline__ double iface_at(const double* q, int c, int s, double vel) {
-  return iface5(q[c - 3 * s], q[c - 2 * s], q[c - s], q[c], q[c + s], q[c + 2 * s], vel);
+template <class Q>
+__device__ __forceinline__ double iface_at(const Q& q, int c, int s, double vel) {
+  return iface5<double>(q[c - 3 * s], q[c - 2 * s], q[c - s], q[c], q[c + s], q[c + 2 * s], vel);
 }
 
 // TL of F = f iface5(q, f) through the face whose upper cell is c.
-__device__ __forceinline__ double tl_face(const double* q, const double* dq, int c, int s,
-                                          double f, double df) {
+template <class Q>
+__device__ __forceinline__ double tl_face(const Q& q, const double* dq, int c, int s, double f,
+                                          double df) {
   return df * iface_at(q, c, s, f) + f * iface_at(dq, c, s, f);
 }
 
 // AD of the same: scatters Fb f w_n into qb, returns the adjoint of f.
-__device__ __forceinline__ double ad_face(const double* q, double* qb, int c, int s, double f,
+template <class Q>
+__device__ __forceinline__ double ad_face(const Q& q, double* qb, int c, int s, double f,
                                           double Fb) {
   double w[6];
   iface5_weights(f, w);
@@ -109,17 +124,20 @@ __global__ void k_ad_massflux(Grid g, Metric<double> m, double* mfxb, double* mf
   const int k = blockIdx.z;
   if (i > g.nx + 2 || j > g.ny + 2 || k >= g.nz) return;
   const int c = g.id(i, j, k);
-  // divergence -> fluxes (div = (fx(i+1)-fx(i))/dx + (fy(j+1)-fy(j))/dy + (fz(k+1)-fz(k))/dzc)
+  // divergence -> fluxes: div = (fx(i+1)-fx(i)) rdx + (fy(j+1)-fy(j)) rdy +
+  // (fz(k+1)-fz(k)) rdzc with the same FP64 reciprocals as k_massflux (the
+  // TL), so the pair stays an exact transpose when dzeta_c is not
+  // representable in the trajectory's storage type
   if (i >= -1 && i <= g.nx && j >= -1 && j <= g.ny) {
     const double d = divb[c];
     if (d != 0.0) {
-      atomicAdd(&mfxb[c + 1], d / g.dx);
-      atomicAdd(&mfxb[c], -d / g.dx);
-      atomicAdd(&mfyb[c + g.NX], d / g.dy);
-      atomicAdd(&mfyb[c], -d / g.dy);
-      const double dzc = m.dzeta_c[k + g.ng];
-      if (k + 1 < g.nz) atomicAdd(&mfzb[c + g.plane], d / dzc);
-      if (k >= 1) atomicAdd(&mfzb[c], -d / dzc);
+      const double dx = d * g.rdx, dy = d * g.rdy, dz = d * m.rdzc[k + g.ng];
+      atomicAdd(&mfxb[c + 1], dx);
+      atomicAdd(&mfxb[c], -dx);
+      atomicAdd(&mfyb[c + g.NX], dy);
+      atomicAdd(&mfyb[c], -dy);
+      if (k + 1 < g.nz) atomicAdd(&mfzb[c + g.plane], dz);
+      if (k >= 1) atomicAdd(&mfzb[c], -dz);
     }
   }
 }
@@ -144,8 +162,9 @@ __global__ void k_ad_massflux_uvw(Grid g, Base<double> b, Metric<double> m, doub
 }
 
 // ------------------------------------------------------------ tendencies
-struct TanTraj {  // trajectory of one stage
-  const double *u, *v, *w, *th, *mfx, *mfy, *mfz, *div;
+template <class S>
+struct TanTraj {  // trajectory of one stage (storage type of the forward)
+  Traj<S> u, v, w, th, mfx, mfy, mfz, div;
 };
 struct TanPert {  // perturbation (TL) or adjoint (AD) arrays of one stage
   double *u, *v, *w, *th, *mfx, *mfy, *mfz, *div;
@@ -154,7 +173,8 @@ struct TanTend {  // tendency perturbations (TL out) or adjoints (AD in)
   double *tu, *tv, *tw, *tth;
 };
 
-__global__ void k_tl_tend(Grid g, Base<double> b, Metric<double> m, TanTraj X, TanPert D,
+template <class S>
+__global__ void k_tl_tend(Grid g, Base<double> b, Metric<double> m, TanTraj<S> X, TanPert D,
                           TanTend out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -228,7 +248,8 @@ __global__ void k_tl_tend(Grid g, Base<double> b, Metric<double> m, TanTraj X, T
   }
 }
 
-__global__ void k_ad_tend(Grid g, Base<double> b, Metric<double> m, TanTraj X, TanPert A,
+template <class S>
+__global__ void k_ad_tend(Grid g, Base<double> b, Metric<double> m, TanTraj<S> X, TanPert A,
                           TanTend in) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -345,18 +366,21 @@ __global__ void k_ad_tend(Grid g, Base<double> b, Metric<double> m, TanTraj X, T
 // with the substep ghost rule: outside the interior, pi' is read from the
 // mapped interior of the current substep and pi'prev from the time-n ghost
 // cell; theta' outside the interior from the time-n ghost cell.
+template <class S>
 struct UVTraj {
-  const double *pcur, *pprev, *pn, *thc, *thn;
+  const double *pcur, *pprev, *pn;
+  Traj<S> thc, thn;
 };
 struct UVTan {  // TL: perturbations in, du/dv out; AD: adjoints
   double *u_src, *v_src, *u_dst, *v_dst, *pcur, *pprev, *pn, *thc, *thn, *tu, *tv;
 };
 
+template <class S>
 struct UVLin {
   const Grid& g;
   const Params& p;
   const Base<double>& b;
-  const UVTraj& X;
+  const UVTraj<S>& X;
   __device__ bool inside(int ii, int jj) const {
     return (unsigned)ii < (unsigned)g.nx && (unsigned)jj < (unsigned)g.ny;
   }
@@ -371,7 +395,7 @@ struct UVLin {
   }
   __device__ double thv(int ii, int jj, int k) const {
     const int c = g.id(ii, jj, k);
-    return b.thb[c] + (inside(ii, jj) ? X.thc[c] : X.thn[c]);
+    return (double)b.thb[c] + (double)(inside(ii, jj) ? X.thc[c] : X.thn[c]);
   }
   __device__ double dpstar(const UVTan& D, int ii, int jj, int k) const {
     const int c = g.id(ii, jj, k);
@@ -399,13 +423,14 @@ struct UVLin {
   }
 };
 
-__global__ void k_tl_uv(Grid g, Params p, Base<double> b, Metric<double> m, UVTraj X, UVTan D,
+template <class S>
+__global__ void k_tl_uv(Grid g, Params p, Base<double> b, Metric<double> m, UVTraj<S> X, UVTan D,
                         double dtau) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   const int k = blockIdx.z;
   if (i > g.nx || j > g.ny || k >= g.nz) return;
-  const UVLin L{g, p, b, X};
+  const UVLin<S> L{g, p, b, X};
   const int c = g.id(i, j, k);
   if (j < g.ny) {  // u face i (i = nx is the image of face 0)
     const int iu = i < g.nx ? i : 0;
@@ -429,13 +454,14 @@ __global__ void k_tl_uv(Grid g, Params p, Base<double> b, Metric<double> m, UVTr
   }
 }
 
-__global__ void k_ad_uv(Grid g, Params p, Base<double> b, Metric<double> m, UVTraj X, UVTan A,
+template <class S>
+__global__ void k_ad_uv(Grid g, Params p, Base<double> b, Metric<double> m, UVTraj<S> X, UVTan A,
                         double dtau) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   const int k = blockIdx.z;
   if (i > g.nx || j > g.ny || k >= g.nz) return;
-  const UVLin L{g, p, b, X};
+  const UVLin<S> L{g, p, b, X};
   const int c = g.id(i, j, k);
   if (j < g.ny) {
     const double ob = A.u_dst[c];
@@ -481,9 +507,12 @@ __global__ void k_ad_uv(Grid g, Params p, Base<double> b, Metric<double> m, UVTr
 // One thread per column, levels in local arrays (nz < MAXL). The forward
 // column (kernels.cu:531-645 with flat terrain and no moisture) is
 // recomputed from the trajectory, then linearised.
+template <class S>
 struct WPTraj {
-  const double *u, *v;  // after this substep's horizontal step
-  const double *w, *pip, *th;
+  Traj<S> u, v;  // after this substep's horizontal step
+  Traj<S> w;
+  const double* pip;
+  Traj<S> th;
 };
 struct WPTan {  // TL: perturbations; AD: adjoints
   double *u, *v, *w, *pip, *th, *tw, *tth;  // inputs
@@ -497,28 +526,31 @@ struct WPCol {
 };
 
 // Forward column: assembly, rows and the Thomas solve.
-template <int MAXL>
-__device__ void wp_forward(const Grid& g, const Params& p, const Base<double>& b,
-                           const Metric<double>& m, const Coef<double>& cf, const WPTraj& X,
+template <class S, int MAXL>
+__device__ void wp_forward(const Grid& g, const Params& p, const Base<double>& b, const Metric<double>& m,
+                           const Coef<double>& cf, const WPTraj<S>& X,
                            int c0, double mc, double dtau, WPCol<MAXL>& s) {
   const int NX = g.NX, PL = g.plane, nz = g.nz;
   const double c1 = (1.0 + p.beta) * 0.5, c2 = (1.0 - p.beta) * 0.5;
-  for (int k = 0; k <= nz; ++k) s.Av[k] = (k <= 0 || k >= nz) ? 0.0 : cf.aw[c0 + k * PL];
+  for (int k = 0; k <= nz; ++k) s.Av[k] = (k <= 0 || k >= nz) ? 0.0 : (double)cf.aw[c0 + k * PL];
   for (int k = 0; k < nz; ++k) {
     const int c = c0 + k * PL, kk = k + g.ng;
     const double Kt = cf.kt3[c];
-    const double Dh = mc *
-                      ((cf.rtjx[c + 1] * X.u[c + 1] - cf.rtjx[c] * X.u[c]) * g.rdx +
-                       (cf.rtjy[c + NX] * X.v[c + NX] - cf.rtjy[c] * X.v[c]) * g.rdy);
+    const double Dh = mc * (((double)cf.rtjx[c + 1] * (double)X.u[c + 1] -
+                             (double)cf.rtjx[c] * (double)X.u[c]) * g.rdx +
+                            ((double)cf.rtjy[c + NX] * (double)X.v[c + NX] -
+                             (double)cf.rtjy[c] * (double)X.v[c]) * g.rdy);
     const double rdzc = m.rdzc[kk];
-    s.P[k] = X.pip[c] - dtau * Kt * (Dh + c2 * (s.Av[k + 1] * X.w[c + PL] - s.Av[k] * X.w[c]) * rdzc);
+    s.P[k] = X.pip[c] - dtau * Kt *
+                            (Dh + c2 * (s.Av[k + 1] * (double)X.w[c + PL] - s.Av[k] * (double)X.w[c]) *
+                                      rdzc);
     s.E[k] = dtau * Kt * c1 * rdzc;
     if (k >= 1) {
       const double rJdzw = m.rdzw[kk];  // J = 1
-      const double thvw = b.thbw[c] + 0.5 * (X.th[c - PL] + X.th[c]);
+      const double thvw = (double)b.thbw[c] + 0.5 * ((double)X.th[c - PL] + (double)X.th[c]);
       s.G[k] = dtau * kCp * thvw * c1 * rJdzw;
-      const double buoy = kGrav * 0.5 * (X.th[c - PL] + X.th[c]) / b.thbw[c];
-      s.R[k] = X.w[c] + dtau * buoy - dtau * kCp * thvw * c2 * (X.pip[c] - X.pip[c - PL]) * rJdzw;
+      const double buoy = kGrav * 0.5 * ((double)X.th[c - PL] + (double)X.th[c]) / (double)b.thbw[c];
+      s.R[k] = (double)X.w[c] + dtau * buoy - dtau * kCp * thvw * c2 * (X.pip[c] - X.pip[c - PL]) * rJdzw;
     }
   }
   for (int k = 1; k < nz; ++k) {
@@ -544,10 +576,10 @@ __device__ void thomas(int nz, const double* A, const double* B, const double* C
   W[0] = 0.0;
 }
 
-template <int MAXL>
-__global__ void __launch_bounds__(64) k_tl_wpi(Grid g, Params p, Base<double> b,
-                                               Metric<double> m, Coef<double> cf, WPTraj X,
-                                               const double* tw, WPTan D, double dtau) {
+template <class S, int MAXL>
+__global__ void __launch_bounds__(64) k_tl_wpi(Grid g, Params p, Base<double> b, Metric<double> m,
+                                               Coef<double> cf, WPTraj<S> X, const S* tw, WPTan D,
+                                               double dtau) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
   if (i >= g.nx || j >= g.ny) return;
@@ -556,8 +588,8 @@ __global__ void __launch_bounds__(64) k_tl_wpi(Grid g, Params p, Base<double> b,
   const double c1 = (1.0 + p.beta) * 0.5, c2 = (1.0 - p.beta) * 0.5;
   WPCol<MAXL> s;
   const double mc = m.mapf[g.id2(i, j)];
-  wp_forward<MAXL>(g, p, b, m, cf, X, c0, mc, dtau, s);
-  for (int k = 1; k < nz; ++k) s.D[k] += dtau * tw[c0 + k * PL];
+  wp_forward<S, MAXL>(g, p, b, m, cf, X, c0, mc, dtau, s);
+  for (int k = 1; k < nz; ++k) s.D[k] += dtau * (double)tw[c0 + k * PL];
   thomas<MAXL>(nz, s.A, s.B, s.C, s.D, s.cs, s.ds, s.W);
   // perturbations of P, G, R and the rows
   double dP[MAXL], dG[MAXL], dR[MAXL], dD[MAXL], dW[MAXL + 1];
@@ -573,7 +605,7 @@ __global__ void __launch_bounds__(64) k_tl_wpi(Grid g, Params p, Base<double> b,
     dR[k] = 0.0;
     if (k >= 1) {
       const double rJdzw = m.rdzw[kk];
-      const double thvw = b.thbw[c] + 0.5 * (X.th[c - PL] + X.th[c]);
+      const double thvw = (double)b.thbw[c] + 0.5 * ((double)X.th[c - PL] + (double)X.th[c]);
       const double dthvw = 0.5 * (D.th[c - PL] + D.th[c]);
       dG[k] = dtau * kCp * dthvw * c1 * rJdzw;
       const double dbuoy = kGrav * 0.5 * (D.th[c - PL] + D.th[c]) / b.thbw[c];
@@ -600,10 +632,10 @@ __global__ void __launch_bounds__(64) k_tl_wpi(Grid g, Params p, Base<double> b,
   }
 }
 
-template <int MAXL>
-__global__ void __launch_bounds__(64) k_ad_wpi(Grid g, Params p, Base<double> b,
-                                               Metric<double> m, Coef<double> cf, WPTraj X,
-                                               const double* tw, WPTan A, double dtau) {
+template <class S, int MAXL>
+__global__ void __launch_bounds__(64) k_ad_wpi(Grid g, Params p, Base<double> b, Metric<double> m,
+                                               Coef<double> cf, WPTraj<S> X, const S* tw, WPTan A,
+                                               double dtau) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
   if (i >= g.nx || j >= g.ny) return;
@@ -612,8 +644,8 @@ __global__ void __launch_bounds__(64) k_ad_wpi(Grid g, Params p, Base<double> b,
   const double c1 = (1.0 + p.beta) * 0.5, c2 = (1.0 - p.beta) * 0.5;
   WPCol<MAXL> s;
   const double mc = m.mapf[g.id2(i, j)];
-  wp_forward<MAXL>(g, p, b, m, cf, X, c0, mc, dtau, s);
-  for (int k = 1; k < nz; ++k) s.D[k] += dtau * tw[c0 + k * PL];
+  wp_forward<S, MAXL>(g, p, b, m, cf, X, c0, mc, dtau, s);
+  for (int k = 1; k < nz; ++k) s.D[k] += dtau * (double)tw[c0 + k * PL];
   thomas<MAXL>(nz, s.A, s.B, s.C, s.D, s.cs, s.ds, s.W);
   // outputs -> W, P
   double Wb[MAXL + 1], Pb[MAXL], Gb[MAXL], Rb[MAXL];
@@ -675,7 +707,7 @@ __global__ void __launch_bounds__(64) k_ad_wpi(Grid g, Params p, Base<double> b,
     atomicAdd(&A.v[c], -Dhb * cf.rtjy[c] * g.rdy);
     if (k >= 1) {
       const double rJdzw = m.rdzw[kk];
-      const double thvw = b.thbw[c] + 0.5 * (X.th[c - PL] + X.th[c]);
+      const double thvw = (double)b.thbw[c] + 0.5 * ((double)X.th[c - PL] + (double)X.th[c]);
       const double dp = X.pip[c] - X.pip[c - PL];
       // G = dtau cp thvw c1 rJdzw; R = w + dtau g 0.5 (thm + thk)/thbw - dtau cp thvw c2 dp rJdzw
       const double thvwb = Gb[k] * dtau * kCp * c1 * rJdzw - Rb[k] * dtau * kCp * c2 * dp * rJdzw;
@@ -698,7 +730,8 @@ __global__ void k_axpy(int n, double a, const double* __restrict__ x, double* y)
 }
 
 // J += scale sum_{i,j} wgt(i,j) 0.5 (u^2 + v^2) at the lowest level.
-__global__ void k_ke_objective(Grid g, const double* __restrict__ u, const double* __restrict__ v,
+template <class S>
+__global__ void k_ke_objective(Grid g, const S* __restrict__ u, const S* __restrict__ v,
                                const double* __restrict__ wgt, double scale, double* J) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -706,11 +739,13 @@ __global__ void k_ke_objective(Grid g, const double* __restrict__ u, const doubl
   const double w = wgt[g.id2(i, j)];
   if (w == 0.0) return;
   const int c = g.id(i, j, 0);
-  atomicAdd(J, scale * w * 0.5 * (u[c] * u[c] + v[c] * v[c]));
+  const double uc = u[c], vc = v[c];
+  atomicAdd(J, scale * w * 0.5 * (uc * uc + vc * vc));
 }
 
 // Its gradient: ub += scale wgt u, vb += scale wgt v at the lowest level.
-__global__ void k_ke_gradient(Grid g, const double* __restrict__ u, const double* __restrict__ v,
+template <class S>
+__global__ void k_ke_gradient(Grid g, const S* __restrict__ u, const S* __restrict__ v,
                               const double* __restrict__ wgt, double scale, double* ub,
                               double* vb) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -719,8 +754,8 @@ __global__ void k_ke_gradient(Grid g, const double* __restrict__ u, const double
   const double w = wgt[g.id2(i, j)];
   if (w == 0.0) return;
   const int c = g.id(i, j, 0);
-  ub[c] += scale * w * u[c];
-  vb[c] += scale * w * v[c];
+  ub[c] += scale * w * (double)u[c];
+  vb[c] += scale * w * (double)v[c];
 }
 
 }  // namespace wfe::dev

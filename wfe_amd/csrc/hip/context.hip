@@ -260,11 +260,12 @@ struct CtxBase {
   virtual void enqueue_step(double dt) = 0;
 };
 
+template <class T>
 class Adjoint;
 
 template <class T>
 class Ctx final : public CtxBase {
-  friend class Adjoint;
+  friend class Adjoint<T>;
  public:
   Ctx(const wfe_grid& gr, const wfe_params& hp, const wfe_base& base, const wfe_metric& met);
   ~Ctx() override {
@@ -1717,22 +1718,46 @@ void Ctx<T>::acoustic_substeps(double dtau, int nsub) {
 }
 
 // ------------------------------------------------------------------ adjoint
-// Tangent-linear and adjoint of the dry large step (FP64 contexts): the
-// forward step is re-run with every stage estimate, tendency, mass flux and
-// substep output kept (the trajectory), then the TL kernels run forward or
-// the AD kernels backward through the same sequence. Gradients over many
-// steps keep the state every `ckpt` steps and recompute each segment.
-class Adjoint {
+// Tangent-linear and adjoint of the dry large step: the forward step is
+// re-run with every stage estimate, tendency, mass flux and substep output
+// kept (the trajectory), then the TL kernels run forward or the AD kernels
+// backward through the same sequence. Gradients over many steps keep the
+// state every `ckpt` steps and recompute each segment.
+//
+// The trajectory is held in the context's own storage type T (FP32 fields
+// and FP64 pi' in a mixed context, FP64 throughout in an FP64 one), so the
+// linearisation is about the trajectory the context itself computes; the
+// perturbations, adjoint variables, their accumulations and the objective
+// are FP64 in both.
+class AdjointBase {
  public:
-  struct Set5 {
-    double *u = nullptr, *v = nullptr, *w = nullptr, *th = nullptr, *pip = nullptr;
+  virtual ~AdjointBase() = default;
+  virtual void tl_step_host(double dt, const double* const* dx, double* const* dy) = 0;
+  virtual void ad_step_host(double dt, const double* const* yb, double* const* xb) = 0;
+  virtual double gradient(double dt, int nsteps, int ckpt, const double* wgt2d,
+                          double* const* grad) = 0;
+};
+
+template <class T>
+class Adjoint final : public AdjointBase {
+ public:
+  // Fields u, v, w, theta', pi' in storage type U (pi' always FP64).
+  template <class U>
+  struct Set5T {
+    U *u = nullptr, *v = nullptr, *w = nullptr, *th = nullptr;
+    double* pip = nullptr;
   };
-  struct Set4 {
-    double *a = nullptr, *b = nullptr, *c = nullptr, *d = nullptr;
+  using Set5 = Set5T<double>;  // perturbations / adjoint variables
+  using TSet = Set5T<T>;       // trajectory / model states
+  template <class U>
+  struct Set4T {
+    U *a = nullptr, *b = nullptr, *c = nullptr, *d = nullptr;
   };
+  using Set4 = Set4T<double>;
+  using TSet4 = Set4T<T>;
   static constexpr int kMaxSub = 16;
 
-  explicit Adjoint(Ctx<double>& f) : F(f) {
+  explicit Adjoint(Ctx<T>& f) : F(f) {
     check_scope();
     const auto& hp = F.hp_;
     ns_ = std::max(1, hp.acoustic_ns);
@@ -1740,18 +1765,20 @@ class Adjoint {
     nsub_[1] = std::max(1, ns_ / 2);
     nsub_[2] = ns_;
     const size_t n = F.npts_;
-    auto set5 = [&](Set5& s) {
-      s.u = A.alloc<double>(n);
-      s.v = A.alloc<double>(n);
-      s.w = A.alloc<double>(n);
-      s.th = A.alloc<double>(n);
+    auto set5 = [&](auto& s) {
+      using U = std::remove_pointer_t<decltype(s.u)>;
+      s.u = A.alloc<U>(n);
+      s.v = A.alloc<U>(n);
+      s.w = A.alloc<U>(n);
+      s.th = A.alloc<U>(n);
       s.pip = A.alloc<double>(n);
     };
-    auto set4 = [&](Set4& s) {
-      s.a = A.alloc<double>(n);
-      s.b = A.alloc<double>(n);
-      s.c = A.alloc<double>(n);
-      s.d = A.alloc<double>(n);
+    auto set4 = [&](auto& s) {
+      using U = std::remove_pointer_t<decltype(s.a)>;
+      s.a = A.alloc<U>(n);
+      s.b = A.alloc<U>(n);
+      s.c = A.alloc<U>(n);
+      s.d = A.alloc<U>(n);
     };
     for (int m = 0; m < 3; ++m) {
       set5(est_[m]);
@@ -1768,9 +1795,10 @@ class Adjoint {
     set5(x0_);
     set5(xb_);
     set5(tmp_);
-    zero_q_ = A.alloc<double>(n);
+    zero_q_ = A.alloc<T>(n);
     jsum_ = A.alloc<double>(1);
     wgt_ = A.alloc<double>(F.n2_);
+    widen_constants();
   }
 
   // The linearised model covers the dry periodic flat step only. The
@@ -1789,9 +1817,9 @@ class Adjoint {
     else if (F.g_.nz + 1 > 64) why = "more than 63 levels";
     else if (std::max(1, hp.acoustic_ns) > kMaxSub) why = "more than 16 acoustic substeps";
     if (why.empty()) {
-      std::vector<double> fc(F.n2_);
-      WFE_HIP(hipMemcpy(fc.data(), F.met_.fcor, F.n2_ * sizeof(double), hipMemcpyDeviceToHost));
-      for (double x : fc)
+      std::vector<T> fc(F.n2_);
+      WFE_HIP(hipMemcpy(fc.data(), F.met_.fcor, F.n2_ * sizeof(T), hipMemcpyDeviceToHost));
+      for (T x : fc)
         if (x != 0) why = "Coriolis is on";
     }
     if (!why.empty()) throw std::runtime_error("adjoint: scope is the dry periodic flat step (" + why + ")");
@@ -1799,10 +1827,10 @@ class Adjoint {
 
   // Forward step of F from its current state, keeping the trajectory.
   void record_step(double dt) {
-    Ctx<double>& f = F;
+    Ctx<T>& f = F;
     hipStream_t st = f.stream_;
     if (!f.tuned_) f.tune_acoustic(dt / ns_);
-    const StateSet<double> sn = f.S_[f.cur_];
+    const StateSet<T> sn = f.S_[f.cur_];
     f.fill_ghosts(sn);
     copy5(est_[0], as5(sn), st);
     const double sdt[3] = {dt / 3, dt / 2, dt};
@@ -1822,9 +1850,9 @@ class Adjoint {
       copy(T_[m].d, f.tth_, st);
       const double dtau = sdt[m] / nsub_[m];
       for (int s = 0; s < nsub_[m]; ++s) {
-        const Set5 src = s == 0 ? as5(sn) : sub_[m][s - 1];
-        const Set5& dst = sub_[m][s];
-        dev::FusedArgs<double> a{};
+        const TSet src = s == 0 ? as5(sn) : sub_[m][s - 1];
+        const TSet& dst = sub_[m][s];
+        dev::FusedArgs<T> a{};
         a.uv.u_src = src.u;
         a.uv.v_src = src.v;
         a.uv.u_dst = dst.u;
@@ -1837,7 +1865,7 @@ class Adjoint {
         a.uv.qv = zero_q_;
         a.uv.tu = T_[m].a;
         a.uv.tv = T_[m].b;
-        a.uv.dtau = dtau;
+        a.uv.dtau = (T)dtau;
         a.w_src = src.w;
         a.w_dst = dst.w;
         a.pip_dst = dst.pip;
@@ -1855,9 +1883,13 @@ class Adjoint {
 
   // dy = TL(dx) around the recorded step (device buffers).
   void tl(double dt, const Set5& dx, const Set5& dy) {
-    Ctx<double>& f = F;
+    Ctx<T>& f = F;
     hipStream_t st = f.stream_;
     const double sdt[3] = {dt / 3, dt / 2, dt};
+    // Cells the TL kernels read but never write (the rigid-lid w levels,
+    // ghost rings of the substep outputs) must hold zero, not what an earlier
+    // adjoint pass left in the shared buffers.
+    clear_perturbations();
     copy5(dest_[0], dx, st);
     fill5(dest_[0]);
     for (int m = 0; m < 3; ++m) {
@@ -1867,26 +1899,26 @@ class Adjoint {
       }
       {
         dim3 grd(cdiv(f.g_.nx + 5, 64), cdiv(f.g_.ny + 5, 4), (unsigned)f.g_.nz);
-        dev::k_massflux<double><<<grd, dim3(64, 4), 0, st>>>(f.g_, f.base_, f.met_, dest_[m].u,
-                                                             dest_[m].v, dest_[m].w, dmf_[m].a,
-                                                             dmf_[m].b, dmf_[m].c, dmf_[m].d);
+        dev::k_massflux<double><<<grd, dim3(64, 4), 0, st>>>(
+            f.g_, b64_, m64_, dest_[m].u, dest_[m].v, dest_[m].w, dmf_[m].a, dmf_[m].b,
+            dmf_[m].c, dmf_[m].d);
       }
-      dev::k_tl_tend<<<grid3(), dim3(64, 4), 0, st>>>(f.g_, f.base_, f.met_, traj(m), pert(m),
+      dev::k_tl_tend<<<grid3(), dim3(64, 4), 0, st>>>(f.g_, b64_, m64_, traj(m), pert(m),
                                                        tend(dT_[m]));
       const double dtau = sdt[m] / nsub_[m];
       for (int s = 0; s < nsub_[m]; ++s) {
-        const Set5 xs = s == 0 ? est_[0] : sub_[m][s - 1];
+        const TSet xs = s == 0 ? est_[0] : sub_[m][s - 1];
         const Set5 ds = s == 0 ? dest_[0] : dsub_[m][s - 1];
-        dev::UVTraj X{xs.pip, s <= 1 ? est_[0].pip : sub_[m][s - 2].pip, est_[0].pip, xs.th,
+        dev::UVTraj<T> X{xs.pip, s <= 1 ? est_[0].pip : sub_[m][s - 2].pip, est_[0].pip, xs.th,
                       est_[0].th};
         dev::UVTan D{ds.u, ds.v, dsub_[m][s].u, dsub_[m][s].v, ds.pip,
                      s <= 1 ? dest_[0].pip : dsub_[m][s - 2].pip, dest_[0].pip, ds.th,
                      dest_[0].th, dT_[m].a, dT_[m].b};
-        dev::k_tl_uv<<<grid_uv(), dim3(64, 4), 0, st>>>(f.g_, f.p_, f.base_, f.met_, X, D, dtau);
-        dev::WPTraj W{sub_[m][s].u, sub_[m][s].v, xs.w, xs.pip, xs.th};
+        dev::k_tl_uv<<<grid_uv(), dim3(64, 4), 0, st>>>(f.g_, f.p_, b64_, m64_, X, D, dtau);
+        dev::WPTraj<T> W{sub_[m][s].u, sub_[m][s].v, xs.w, xs.pip, xs.th};
         dev::WPTan DW{dsub_[m][s].u, dsub_[m][s].v, ds.w, ds.pip, ds.th, dT_[m].c, dT_[m].d,
                       dsub_[m][s].w, dsub_[m][s].pip, dsub_[m][s].th};
-        dev::k_tl_wpi<64><<<grid_col(), 64, 0, st>>>(f.g_, f.p_, f.base_, f.met_, f.coef_, W,
+        dev::k_tl_wpi<T, 64><<<grid_col(), 64, 0, st>>>(f.g_, f.p_, b64_, m64_, c64_, W,
                                                        T_[m].c, DW, dtau);
       }
     }
@@ -1896,40 +1928,35 @@ class Adjoint {
 
   // xb = AD(yb) around the recorded step (device buffers; yb is read only).
   void ad(double dt, const Set5& yb, const Set5& xb) {
-    Ctx<double>& f = F;
+    Ctx<T>& f = F;
     hipStream_t st = f.stream_;
     const double sdt[3] = {dt / 3, dt / 2, dt};
-    for (int m = 0; m < 3; ++m) {
-      zero5(dest_[m]);
-      zero4(dmf_[m]);
-      zero4(dT_[m]);
-      for (int s = 0; s < nsub_[m]; ++s) zero5(dsub_[m][s]);
-    }
+    clear_perturbations();
     copy5(dsub_[2][nsub_[2] - 1], yb, st);
     for (int m = 2; m >= 0; --m) {
       const double dtau = sdt[m] / nsub_[m];
       for (int s = nsub_[m] - 1; s >= 0; --s) {
-        const Set5 xs = s == 0 ? est_[0] : sub_[m][s - 1];
+        const TSet xs = s == 0 ? est_[0] : sub_[m][s - 1];
         const Set5 ds = s == 0 ? dest_[0] : dsub_[m][s - 1];
-        dev::WPTraj W{sub_[m][s].u, sub_[m][s].v, xs.w, xs.pip, xs.th};
+        dev::WPTraj<T> W{sub_[m][s].u, sub_[m][s].v, xs.w, xs.pip, xs.th};
         dev::WPTan AW{dsub_[m][s].u, dsub_[m][s].v, ds.w, ds.pip, ds.th, dT_[m].c, dT_[m].d,
                       dsub_[m][s].w, dsub_[m][s].pip, dsub_[m][s].th};
-        dev::k_ad_wpi<64><<<grid_col(), 64, 0, st>>>(f.g_, f.p_, f.base_, f.met_, f.coef_, W,
+        dev::k_ad_wpi<T, 64><<<grid_col(), 64, 0, st>>>(f.g_, f.p_, b64_, m64_, c64_, W,
                                                        T_[m].c, AW, dtau);
-        dev::UVTraj X{xs.pip, s <= 1 ? est_[0].pip : sub_[m][s - 2].pip, est_[0].pip, xs.th,
+        dev::UVTraj<T> X{xs.pip, s <= 1 ? est_[0].pip : sub_[m][s - 2].pip, est_[0].pip, xs.th,
                       est_[0].th};
         dev::UVTan AU{ds.u, ds.v, dsub_[m][s].u, dsub_[m][s].v, ds.pip,
                       s <= 1 ? dest_[0].pip : dsub_[m][s - 2].pip, dest_[0].pip, ds.th,
                       dest_[0].th, dT_[m].a, dT_[m].b};
-        dev::k_ad_uv<<<grid_uv(), dim3(64, 4), 0, st>>>(f.g_, f.p_, f.base_, f.met_, X, AU, dtau);
+        dev::k_ad_uv<<<grid_uv(), dim3(64, 4), 0, st>>>(f.g_, f.p_, b64_, m64_, X, AU, dtau);
       }
-      dev::k_ad_tend<<<grid3(), dim3(64, 4), 0, st>>>(f.g_, f.base_, f.met_, traj(m), pert(m),
+      dev::k_ad_tend<<<grid3(), dim3(64, 4), 0, st>>>(f.g_, b64_, m64_, traj(m), pert(m),
                                                        tend(dT_[m]));
       {
         dim3 grd(cdiv(f.g_.nx + 5, 64), cdiv(f.g_.ny + 5, 4), (unsigned)f.g_.nz);
-        dev::k_ad_massflux<<<grd, dim3(64, 4), 0, st>>>(f.g_, f.met_, dmf_[m].a, dmf_[m].b,
+        dev::k_ad_massflux<<<grd, dim3(64, 4), 0, st>>>(f.g_, m64_, dmf_[m].a, dmf_[m].b,
                                                         dmf_[m].c, dmf_[m].d);
-        dev::k_ad_massflux_uvw<<<grd, dim3(64, 4), 0, st>>>(f.g_, f.base_, f.met_, dest_[m].u,
+        dev::k_ad_massflux_uvw<<<grd, dim3(64, 4), 0, st>>>(f.g_, b64_, m64_, dest_[m].u,
                                                             dest_[m].v, dest_[m].w, dmf_[m].a,
                                                             dmf_[m].b, dmf_[m].c);
       }
@@ -1948,7 +1975,7 @@ class Adjoint {
   // The host entries leave the context as they found it: the saved state
   // goes back into the same state set, and the set that is current (which
   // record_step flips) is restored, so ensembles see the parity unchanged.
-  void tl_step_host(double dt, const double* const* dx, double* const* dy) {
+  void tl_step_host(double dt, const double* const* dx, double* const* dy) override {
     check_scope();
     const int par = F.cur_;
     save_state(x0_);
@@ -1959,7 +1986,7 @@ class Adjoint {
     F.cur_ = par;
     restore_state(x0_);
   }
-  void ad_step_host(double dt, const double* const* yb, double* const* xb) {
+  void ad_step_host(double dt, const double* const* yb, double* const* xb) override {
     check_scope();
     const int par = F.cur_;
     save_state(x0_);
@@ -1973,36 +2000,30 @@ class Adjoint {
 
   // J = (1/N) sum_{n=1..N} sum_{i,j} wgt(i,j) 0.5 (u^2 + v^2) at the lowest
   // level, and its gradient with respect to the initial state.
-  double gradient(double dt, int nsteps, int ckpt, const double* wgt2d, double* const* grad) {
+  double gradient(double dt, int nsteps, int ckpt, const double* wgt2d,
+                  double* const* grad) override {
     if (nsteps < 1 || ckpt < 1) throw ArgError("adjoint gradient: nsteps, ckpt must be >= 1");
     check_scope();
-    Ctx<double>& f = F;
+    Ctx<T>& f = F;
     hipStream_t st = f.stream_;
     const int par = f.cur_;
     {
-      const std::vector<double> w = f.to_dev<double>(wgt2d, false);
+      const std::vector<double> w = f.template to_dev<double>(wgt2d, false);
       WFE_HIP(hipMemcpy(wgt_, w.data(), f.n2_ * sizeof(double), hipMemcpyHostToDevice));
     }
     const double scale = 1.0 / nsteps;
     DevArena B;  // checkpoints and segment states, freed on return
     save_state(x0_);
     const int nck = (nsteps + ckpt - 1) / ckpt;
-    std::vector<Set5> cks(nck);
-    for (auto& c : cks) {
-      c.u = B.alloc<double>(f.npts_);
-      c.v = B.alloc<double>(f.npts_);
-      c.w = B.alloc<double>(f.npts_);
-      c.th = B.alloc<double>(f.npts_);
-      c.pip = B.alloc<double>(f.npts_);
-    }
-    std::vector<Set5> seg(ckpt);
-    for (auto& c : seg) {
-      c.u = B.alloc<double>(f.npts_);
-      c.v = B.alloc<double>(f.npts_);
-      c.w = B.alloc<double>(f.npts_);
-      c.th = B.alloc<double>(f.npts_);
-      c.pip = B.alloc<double>(f.npts_);
-    }
+    std::vector<TSet> cks(nck), seg(ckpt);
+    for (std::vector<TSet>* v : {&cks, &seg})
+      for (TSet& c : *v) {
+        c.u = B.alloc<T>(f.npts_);
+        c.v = B.alloc<T>(f.npts_);
+        c.w = B.alloc<T>(f.npts_);
+        c.th = B.alloc<T>(f.npts_);
+        c.pip = B.alloc<double>(f.npts_);
+      }
     // forward sweep: checkpoints and J
     WFE_HIP(hipMemsetAsync(jsum_, 0, sizeof(double), st));
     for (int n = 0; n < nsteps; ++n) {
@@ -2038,9 +2059,63 @@ class Adjoint {
   }
 
  private:
-  static Set5 as5(const StateSet<double>& s) { return Set5{s.u, s.v, s.w, s.thp, s.pip}; }
-  StateSet<double> est_set(int m, const StateSet<double>& sn) const {
-    StateSet<double> e = sn;
+  // Base state, metric and acoustic coefficients for the TL/AD kernels in
+  // FP64: the context's own in an FP64 context; in a mixed one, its FP32
+  // values widened (the values its forward step used).
+  void widen_constants() {
+    if constexpr (std::is_same_v<T, double>) {
+      b64_ = F.base_;
+      m64_ = F.met_;
+      c64_ = F.coef_;
+    } else {
+      const size_t n3 = F.npts_, n2 = F.n2_, n1 = (size_t)F.g_.NZ;
+      auto wide = [&](const T* d, size_t n) -> const double* {
+        std::vector<T> h(n);
+        WFE_HIP(hipMemcpy(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost));
+        const std::vector<double> w(h.begin(), h.end());
+        double* o = A.alloc<double>(n);
+        WFE_HIP(hipMemcpy(o, w.data(), n * sizeof(double), hipMemcpyHostToDevice));
+        return o;
+      };
+      const auto& b = F.base_;
+      b64_.thb = wide(b.thb, n3);
+      b64_.rhob = wide(b.rhob, n3);
+      b64_.dthbdz = wide(b.dthbdz, n3);
+      b64_.thbw = wide(b.thbw, n3);
+      b64_.rhobw = wide(b.rhobw, n3);
+      b64_.qvb = wide(b.qvb, n3);
+      b64_.ub = wide(b.ub, n1);
+      b64_.vb = wide(b.vb, n1);
+      b64_.pib = b.pib;
+      b64_.hom = b.hom;
+      const auto& m = F.met_;
+      m64_.zeta_w = wide(m.zeta_w, n1);
+      m64_.zeta_c = wide(m.zeta_c, n1);
+      m64_.dzeta_c = wide(m.dzeta_c, n1);
+      m64_.dzeta_w = wide(m.dzeta_w, n1);
+      m64_.hx = wide(m.hx, n2);
+      m64_.hy = wide(m.hy, n2);
+      m64_.hxu = wide(m.hxu, n2);
+      m64_.hyv = wide(m.hyv, n2);
+      m64_.jac = wide(m.jac, n2);
+      m64_.fcor = wide(m.fcor, n2);
+      m64_.mapf = wide(m.mapf, n2);
+      m64_.zt = m.zt;
+      m64_.rdzc = m.rdzc;
+      m64_.rdzw = m.rdzw;
+      m64_.rzt = m.rzt;
+      m64_.ray_c = wide(m.ray_c, n1);
+      m64_.ray_w = wide(m.ray_w, n1);
+      const auto& cf = F.coef_;
+      c64_.rtjx = wide(cf.rtjx, n3);
+      c64_.rtjy = wide(cf.rtjy, n3);
+      c64_.kt3 = wide(cf.kt3, n3);
+      c64_.aw = wide(cf.aw, n3);
+    }
+  }
+  static TSet as5(const StateSet<T>& s) { return TSet{s.u, s.v, s.w, s.thp, s.pip}; }
+  StateSet<T> est_set(int m, const StateSet<T>& sn) const {
+    StateSet<T> e = sn;
     e.u = est_[m].u;
     e.v = est_[m].v;
     e.w = est_[m].w;
@@ -2048,8 +2123,8 @@ class Adjoint {
     e.pip = est_[m].pip;
     return e;
   }
-  dev::TanTraj traj(int m) const {
-    return dev::TanTraj{est_[m].u, est_[m].v, est_[m].w, est_[m].th,
+  dev::TanTraj<T> traj(int m) const {
+    return dev::TanTraj<T>{est_[m].u, est_[m].v, est_[m].w, est_[m].th,
                         mf_[m].a, mf_[m].b, mf_[m].c, mf_[m].d};
   }
   dev::TanPert pert(int m) const {
@@ -2063,10 +2138,12 @@ class Adjoint {
   }
   dim3 grid_col() const { return dim3(cdiv(F.g_.nx, 64), (unsigned)F.g_.ny); }
 
-  void copy(double* d, const double* s, hipStream_t st) {
-    WFE_HIP(hipMemcpyAsync(d, s, F.npts_ * sizeof(double), hipMemcpyDeviceToDevice, st));
+  template <class U>
+  void copy(U* d, const U* s, hipStream_t st) {
+    WFE_HIP(hipMemcpyAsync(d, s, F.npts_ * sizeof(U), hipMemcpyDeviceToDevice, st));
   }
-  void copy5(const Set5& d, const Set5& s, hipStream_t st) {
+  template <class U>
+  void copy5(const Set5T<U>& d, const Set5T<U>& s, hipStream_t st) {
     copy(d.u, s.u, st);
     copy(d.v, s.v, st);
     copy(d.w, s.w, st);
@@ -2075,6 +2152,14 @@ class Adjoint {
   }
   void zero(double* d) {
     WFE_HIP(hipMemsetAsync(d, 0, F.npts_ * sizeof(double), F.stream_));
+  }
+  void clear_perturbations() {
+    for (int m = 0; m < 3; ++m) {
+      zero5(dest_[m]);
+      zero4(dmf_[m]);
+      zero4(dT_[m]);
+      for (int s = 0; s < nsub_[m]; ++s) zero5(dsub_[m][s]);
+    }
   }
   void zero5(const Set5& s) {
     zero(s.u);
@@ -2132,21 +2217,21 @@ class Adjoint {
     for (int q = 0; q < 5; ++q)
       dev::k_axpy<<<cdiv(n, 256), 256, 0, F.stream_>>>(n, 1.0, ss[q], dd[q]);
   }
-  void objective(const Set5& x, double scale) {
-    dev::k_ke_objective<<<dim3(cdiv(F.g_.nx, 64), cdiv(F.g_.ny, 4)), dim3(64, 4), 0,
+  void objective(const TSet& x, double scale) {
+    dev::k_ke_objective<T><<<dim3(cdiv(F.g_.nx, 64), cdiv(F.g_.ny, 4)), dim3(64, 4), 0,
                           F.stream_>>>(F.g_, x.u, x.v, wgt_, scale, jsum_);
   }
-  void objective_grad(const Set5& x, const Set5& xb, double scale) {
-    dev::k_ke_gradient<<<dim3(cdiv(F.g_.nx, 64), cdiv(F.g_.ny, 4)), dim3(64, 4), 0,
+  void objective_grad(const TSet& x, const Set5& xb, double scale) {
+    dev::k_ke_gradient<T><<<dim3(cdiv(F.g_.nx, 64), cdiv(F.g_.ny, 4)), dim3(64, 4), 0,
                          F.stream_>>>(F.g_, x.u, x.v, wgt_, scale, xb.u, xb.v);
   }
-  void save_state(const Set5& d) { copy5(d, as5(F.S_[F.cur_]), F.stream_); }
-  void restore_state(const Set5& s) { copy5(as5(F.S_[F.cur_]), s, F.stream_); }
+  void save_state(const TSet& d) { copy5(d, as5(F.S_[F.cur_]), F.stream_); }
+  void restore_state(const TSet& s) { copy5(as5(F.S_[F.cur_]), s, F.stream_); }
   void upload5(const Set5& d, const double* const* h) {
     double* dd[5] = {d.u, d.v, d.w, d.th, d.pip};
     WFE_HIP(hipStreamSynchronize(F.stream_));
     for (int q = 0; q < 5; ++q) {
-      const std::vector<double> t = F.to_dev<double>(h[q], true);
+      const std::vector<double> t = F.template to_dev<double>(h[q], true);
       WFE_HIP(hipMemcpy(dd[q], t.data(), F.npts_ * sizeof(double), hipMemcpyHostToDevice));
     }
   }
@@ -2160,13 +2245,18 @@ class Adjoint {
     }
   }
 
-  Ctx<double>& F;
+  Ctx<T>& F;
   DevArena A;
+  dev::Base<double> b64_{};
+  dev::Metric<double> m64_{};
+  dev::Coef<double> c64_{};
   int ns_ = 1, nsub_[3] = {1, 1, 1};
-  Set5 est_[3], dest_[3], sub_[3][kMaxSub], dsub_[3][kMaxSub];
-  Set4 mf_[3], dmf_[3], T_[3], dT_[3];
-  Set5 x0_, xb_, tmp_;
-  double* zero_q_ = nullptr;
+  TSet est_[3], sub_[3][kMaxSub], x0_;
+  Set5 dest_[3], dsub_[3][kMaxSub];
+  TSet4 mf_[3], T_[3];
+  Set4 dmf_[3], dT_[3];
+  Set5 xb_, tmp_;
+  T* zero_q_ = nullptr;
   double* jsum_ = nullptr;
   double* wgt_ = nullptr;
 };
@@ -2178,7 +2268,7 @@ class Adjoint {
 struct wfe_ctx {
   std::unique_ptr<wfe::CtxBase> impl;
   int precision = 0;
-  std::unique_ptr<wfe::Adjoint> adj;  // created on first use (FP64 contexts)
+  std::unique_ptr<wfe::AdjointBase> adj;  // created on first use
 };
 
 namespace {
@@ -2218,11 +2308,15 @@ void check_n2(wfe_ctx* c, size_t n) {
   if (n != c->impl->host_n2()) throw wfe::ArgError("2-D array element count mismatch");
 }
 
-wfe::Adjoint& adjoint_of(wfe_ctx* ctx) {
-  if (ctx->precision != WFE_PREC_FP64)
-    throw wfe::ArgError("adjoint: needs a plain FP64 context (WFE_PREC_FP64)");
-  if (!ctx->adj)
-    ctx->adj = std::make_unique<wfe::Adjoint>(*static_cast<wfe::Ctx<double>*>(ctx->impl.get()));
+wfe::AdjointBase& adjoint_of(wfe_ctx* ctx) {
+  if (!ctx->adj) {
+    if (ctx->precision == WFE_PREC_FP64)
+      ctx->adj = std::make_unique<wfe::Adjoint<double>>(
+          *static_cast<wfe::Ctx<double>*>(ctx->impl.get()));
+    else
+      ctx->adj = std::make_unique<wfe::Adjoint<float>>(
+          *static_cast<wfe::Ctx<float>*>(ctx->impl.get()));
+  }
   return *ctx->adj;
 }
 
@@ -2298,7 +2392,7 @@ int wfe_tl_step(wfe_ctx* ctx, double dt, const double* const* dx, double* const*
     if (!dx || !dy) throw wfe::ArgError("wfe_tl_step: null field arrays");
     for (int q = 0; q < 5; ++q)
       if (!dx[q] || !dy[q]) throw wfe::ArgError("wfe_tl_step: null field array");
-    wfe::Adjoint& a = adjoint_of(ctx);
+    wfe::AdjointBase& a = adjoint_of(ctx);
     check_n3(ctx, n);
     a.tl_step_host(dt, dx, dy);
   });
@@ -2310,7 +2404,7 @@ int wfe_ad_step(wfe_ctx* ctx, double dt, const double* const* yb, double* const*
     if (!yb || !xb) throw wfe::ArgError("wfe_ad_step: null field arrays");
     for (int q = 0; q < 5; ++q)
       if (!yb[q] || !xb[q]) throw wfe::ArgError("wfe_ad_step: null field array");
-    wfe::Adjoint& a = adjoint_of(ctx);
+    wfe::AdjointBase& a = adjoint_of(ctx);
     check_n3(ctx, n);
     a.ad_step_host(dt, yb, xb);
   });
@@ -2323,7 +2417,7 @@ int wfe_adj_gradient(wfe_ctx* ctx, double dt, int nsteps, int ckpt_every, const 
     if (!wgt2d || !J || !grad) throw wfe::ArgError("wfe_adj_gradient: null argument");
     for (int q = 0; q < 5; ++q)
       if (!grad[q]) throw wfe::ArgError("wfe_adj_gradient: null gradient array");
-    wfe::Adjoint& a = adjoint_of(ctx);
+    wfe::AdjointBase& a = adjoint_of(ctx);
     check_n3(ctx, n);
     check_n2(ctx, n2);
     *J = a.gradient(dt, nsteps, ckpt_every, wgt2d, grad);
