@@ -387,19 +387,24 @@ def ensemble_benchmark(steps: int):
         shutil.rmtree(d, ignore_errors=True)
 
 
-def adjoint_benchmark(nsteps: int, ckpt: int):
-    """Dry supercell-shape case (cases/cfg5_adjoint_dry.ini, FP64): gradient of
-    the mean lowest-level kinetic energy in a seeded 20 x 20 km box with
-    respect to the initial (u, v, w, theta', pi') through nsteps steps with a
-    checkpoint every ckpt steps; one-step dot-product test; finite-difference
-    check of the gradient along a seeded direction for eps = 1e-3 .. 1e-7.
-    Then the FP32 production gradient (mixed context: FP32 forward and
-    trajectory, FP64 adjoint) from the same initial state, timed and compared
-    with the FP64 one field by field (relative L2 over the interior)."""
+def adjoint_benchmark(nsteps: int, ckpt: int, moist: bool = False):
+    """Supercell-shape case, dry (cases/cfg5_adjoint_dry.ini) or on the WK82
+    moist sounding without microphysics (cases/cfg5_adjoint_moist.ini, q_v
+    linearised): gradient of the mean lowest-level kinetic energy in a
+    seeded 20 x 20 km box with respect to the initial (u, v, w, theta', pi'
+    and q_v) through nsteps steps with a checkpoint every ckpt steps, in an
+    FP64 context; one-step dot-product test; finite-difference check of the
+    gradient along a seeded direction (and, moist, along its q_v part alone)
+    for eps = 1e-3 .. 1e-7. Then the FP32 production gradient (mixed
+    context: FP32 forward and trajectory, FP64 adjoint) from the same
+    initial state, timed and compared with the FP64 one field by field
+    (relative L2 over the interior)."""
     from wfe_amd import Case, Integrator
-    from wfe_amd.adjoint import FIELDS5, ad_step, box_weights, ke_gradient, tl_step
+    from wfe_amd.adjoint import FIELDS, FIELDS5, ad_step, box_weights, ke_gradient, tl_step
 
-    case = Case(ROOT / "cases" / "cfg5_adjoint_dry.ini")
+    fields = FIELDS if moist else FIELDS5
+    name = "cfg5_adjoint_moist.ini" if moist else "cfg5_adjoint_dry.ini"
+    case = Case(ROOT / "cases" / name)
     g = case.grid
     dt = case.scalar("dt")
     rng = np.random.default_rng(20261202)
@@ -409,28 +414,32 @@ def adjoint_benchmark(nsteps: int, ckpt: int):
     mask = np.zeros((g.NZ, g.NY, g.NX), dtype=bool)
     mask[g.ng:g.ng + g.nz, g.ng:g.ng + g.ny, g.ng:g.ng + g.nx] = True
     mask = mask.ravel()
-    scale = {"u": 1.0, "v": 1.0, "w": 0.3, "thp": 0.5, "pip": 1e-4}
+    scale = {"u": 1.0, "v": 1.0, "w": 0.3, "thp": 0.5, "pip": 1e-4, "qv": 1e-3}
 
     def rand():
-        return {f: np.where(mask, rng.standard_normal(g.npts) * scale[f], 0.0) for f in FIELDS5}
+        return {f: np.where(mask, rng.standard_normal(g.npts) * scale[f], 0.0) for f in fields}
+
+    def dot(a, b):
+        return sum(float(np.dot(a[f][mask], b[f][mask])) for f in fields)
 
     dx, yb = rand(), rand()
     dy = tl_step(gi, dt, dx)
     xb = ad_step(gi, dt, yb)
-    lhs = sum(float(np.dot(dy[f][mask], yb[f][mask])) for f in FIELDS5)
-    rhs = sum(float(np.dot(dx[f][mask], xb[f][mask])) for f in FIELDS5)
+    lhs, rhs = dot(dy, yb), dot(dx, xb)
 
     gi.sync()
     t0 = time.perf_counter()
     J, grad = ke_gradient(gi, dt, nsteps, ckpt, w)
     t_adj = time.perf_counter() - t0
-    x0 = {f: gi.get(f) for f in FIELDS5}
+    x0 = {f: gi.get(f) for f in fields}
     d = rand()
-    gd = sum(float(np.dot(grad[f][mask], d[f][mask])) for f in FIELDS5)
+    dirs = {"all": d}
+    if moist:
+        dirs["qv"] = {f: (d[f] if f == "qv" else np.zeros(g.npts)) for f in fields}
 
     def J_of(xs):
         gp = Integrator.from_case(case, "fp64")
-        for f in FIELDS5:
+        for f in fields:
             gp.set(f, xs[f])
         tot = 0.0
         ww = w.reshape(g.NY, g.NX)
@@ -444,26 +453,29 @@ def adjoint_benchmark(nsteps: int, ckpt: int):
     t0 = time.perf_counter()
     J_fwd = J_of(x0)
     t_fwd = time.perf_counter() - t0
-    fd = {}
-    for eps in (1e-3, 1e-4, 1e-5, 1e-6, 1e-7):
-        jp = J_of({f: x0[f] + eps * d[f] for f in FIELDS5})
-        jm = J_of({f: x0[f] - eps * d[f] for f in FIELDS5})
-        fd[f"{eps:.0e}"] = abs((jp - jm) / (2 * eps) - gd) / abs(gd)
+    fd, gdir = {}, {}
+    for dname, dd in dirs.items():
+        gd = dot(grad, dd)
+        gdir[dname] = gd
+        fd[dname] = {}
+        for eps in (1e-3, 1e-4, 1e-5, 1e-6, 1e-7):
+            jp = J_of({f: x0[f] + eps * dd[f] for f in fields})
+            jm = J_of({f: x0[f] - eps * dd[f] for f in fields})
+            fd[dname][f"{eps:.0e}"] = abs((jp - jm) / (2 * eps) - gd) / abs(gd)
 
     # FP32 production gradient against the FP64 check
     g32 = Integrator.from_case(case, "mixed")
-    for f in FIELDS5:
+    for f in fields:
         g32.set(f, x0[f])
     dy32 = tl_step(g32, dt, dx)
     xb32 = ad_step(g32, dt, yb)
-    lhs32 = sum(float(np.dot(dy32[f][mask], yb[f][mask])) for f in FIELDS5)
-    rhs32 = sum(float(np.dot(dx[f][mask], xb32[f][mask])) for f in FIELDS5)
+    lhs32, rhs32 = dot(dy32, yb), dot(dx, xb32)
     g32.sync()
     t0 = time.perf_counter()
     J32, grad32 = ke_gradient(g32, dt, nsteps, ckpt, w)
     t_adj32 = time.perf_counter() - t0
     gm = Integrator.from_case(case, "mixed")
-    for f in FIELDS5:
+    for f in fields:
         gm.set(f, x0[f])
     gm.sync()
     t0 = time.perf_counter()
@@ -472,22 +484,30 @@ def adjoint_benchmark(nsteps: int, ckpt: int):
     gm.sync()
     t_fwd32 = time.perf_counter() - t0
     gerr = {f: float(np.linalg.norm(grad32[f][mask] - grad[f][mask]) /
-                     np.linalg.norm(grad[f][mask])) for f in FIELDS5}
+                     np.linalg.norm(grad[f][mask])) for f in fields}
     fp32 = {"J": J32, "J_rel_vs_fp64": abs(J32 - J) / abs(J),
             "dot_product_rel": abs(lhs32 - rhs32) / abs(lhs32),
             "gradient_rel_l2_vs_fp64": gerr,
             "adjoint_s": t_adj32, "forward_s": t_fwd32,
             "adjoint_over_forward": t_adj32 / t_fwd32,
             "speedup_vs_fp64_adjoint": t_adj / t_adj32}
-    return {"workload": "dry 120x120x40 (supercell levels), isentropic, periodic, FP64; "
-                        f"J = mean lowest-level KE in box i {i0}..{i0 + 20}, j {j0}..{j0 + 20}",
-            "steps": nsteps, "checkpoint_every": ckpt,
-            "J": J, "J_forward_check_rel": abs(J - J_fwd) / abs(J_fwd),
-            "dot_product_rel": abs(lhs - rhs) / abs(lhs),
-            "gradient_fd_rel_err": fd, "gradient_dot_direction": gd,
-            "adjoint_s": t_adj, "forward_s": t_fwd,
-            "adjoint_over_forward": t_adj / t_fwd,
-            "fp32_production": fp32}
+    kind = ("WK82 moist sounding, no microphysics, q_v linearised" if moist
+            else "isentropic")
+    out = {"workload": f"{'moist' if moist else 'dry'} 120x120x40 (supercell levels), {kind}, "
+                       f"periodic, FP64; J = mean lowest-level KE in box i {i0}..{i0 + 20}, "
+                       f"j {j0}..{j0 + 20}",
+           "steps": nsteps, "checkpoint_every": ckpt,
+           "J": J, "J_forward_check_rel": abs(J - J_fwd) / abs(J_fwd),
+           "dot_product_rel": abs(lhs - rhs) / abs(lhs),
+           "gradient_fd_rel_err": fd["all"], "gradient_dot_direction": gdir["all"],
+           "adjoint_s": t_adj, "forward_s": t_fwd,
+           "adjoint_over_forward": t_adj / t_fwd,
+           "fp32_production": fp32}
+    if moist:
+        out["qv_gradient_fd_rel_err"] = fd["qv"]
+        out["qv_gradient_dot_direction"] = gdir["qv"]
+        out["gradient_rms"] = {f: float(np.sqrt(np.mean(grad[f][mask] ** 2))) for f in fields}
+    return out
 
 
 def cpu_model() -> str:
@@ -629,6 +649,8 @@ def main() -> int:
             f"{ens['sequential_ms_per_round']:.2f} ms/round")
         log(f"[bench] adjoint: dry 120x120x40, {args.adjoint_steps} steps")
         adj = adjoint_benchmark(args.adjoint_steps, 50)
+        log(f"[bench] adjoint: moist 120x120x40 (q_v), {args.adjoint_steps} steps")
+        adj["moist"] = adjoint_benchmark(args.adjoint_steps, 50, moist=True)
 
     alg = ALG_BYTES_STEP["regional"]
     out = {

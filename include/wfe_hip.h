@@ -138,10 +138,11 @@ typedef struct {
 } wfe_sfc;
 int wfe_set_physics(wfe_ctx* ctx, const wfe_sfc* sfc, size_t n2);
 
-/* ---- tangent-linear and adjoint of the dry large step ----
- * No counterpart in the reference. Scope: FP64 or mixed contexts of dry cases
- * with periodic lateral boundaries, flat terrain, no Coriolis, no diffusion,
- * no w damping and no boundary relaxation (WFE_E_CONFIG otherwise); nz <= 63,
+/* ---- tangent-linear and adjoint of the large step ----
+ * No counterpart in the reference. Scope: FP64 or mixed contexts with
+ * periodic lateral boundaries, flat terrain, no Coriolis, no diffusion, no w
+ * damping, no boundary relaxation and no microphysics (WFE_E_CONFIG
+ * otherwise); with moisture on, q_v is linearised (the _q entries); nz <= 63,
  * acoustic_ns <= 16. In a mixed context the linearisation is about the FP32
  * trajectory the context computes (FP32 production); perturbations, adjoint
  * variables and J are FP64 in both. Fields are (u, v, w, theta', pi') as five FP64 host
@@ -161,6 +162,21 @@ int wfe_tl_step(wfe_ctx* ctx, double dt, const double* const* dx, double* const*
 int wfe_ad_step(wfe_ctx* ctx, double dt, const double* const* yb, double* const* xb, size_t n);
 int wfe_adj_gradient(wfe_ctx* ctx, double dt, int nsteps, int ckpt_every, const double* wgt2d,
                      size_t n2, double* J, double* const* grad, size_t n);
+/* The same with q_v as a sixth field (u, v, w, theta', pi', q_v): with
+ * moisture on (and microphysics off, flux-form transport without the
+ * positive-definite limiter, q_c = q_r = 0) q_v is transported, adds
+ * g 0.61 q_v' to the w buoyancy and scales theta_v by (1 + 0.61 q_v), and is
+ * linearised with the rest; without moisture the q_v input is ignored and
+ * its output is zero. The five-field entries above are these with q_v
+ * perturbations zero and the q_v output dropped. No counterpart in the
+ * reference. */
+int wfe_tl_step_q(wfe_ctx* ctx, double dt, const double* const* dx, double* const* dy,
+                  size_t n);
+int wfe_ad_step_q(wfe_ctx* ctx, double dt, const double* const* yb, double* const* xb,
+                  size_t n);
+int wfe_adj_gradient_q(wfe_ctx* ctx, double dt, int nsteps, int ckpt_every,
+                       const double* wgt2d, size_t n2, double* J, double* const* grad,
+                       size_t n);
 
 /* ---- ensembles: members of any grid size advancing together ----
  * No counterpart in the reference (which runs one case per process). An

@@ -276,3 +276,161 @@ def test_adjoint_entries_leave_state_set_and_ensembles_intact():
         np.testing.assert_array_equal(gb.get(f), ref.get(f))
     del ens
     assert isinstance(ga, Integrator)
+
+
+# ---------------------------------------------------------------- moist (q_v)
+FIELDS6 = FIELDS5 + ("qv",)
+SCALE6 = dict(SCALE, qv=1e-3)
+
+
+def moist_case(**over):
+    """The small periodic case on the WK82 moist sounding, microphysics off:
+    q_v is transported and acts through buoyancy and theta_v."""
+    kw = dict(profile="wk82", microphysics="off")
+    kw.update(over)
+    return dry_case(**kw)
+
+
+def make_moist(c, seed, precision="fp64"):
+    from wfe_amd import Integrator
+    gi = Integrator.from_case(c, precision)
+    for f, a in perturbed_state(c, seed).items():
+        if f not in ("qc", "qr"):  # q_c = q_r = 0: the linearised scope
+            gi.set(f, a)
+    return gi
+
+
+def random6(g, rng):
+    mask = interior_mask(g)
+    return {f: np.where(mask, rng.standard_normal(g.npts) * SCALE6[f], 0.0) for f in FIELDS6}
+
+
+def dot6(a, b, mask):
+    return sum(float(np.dot(a[f][mask], b[f][mask])) for f in FIELDS6)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_moist_dot_product_one_step(precision):
+    from wfe_amd.adjoint import ad_step, tl_step
+    c = moist_case()
+    assert c.params.moisture and not c.params.microphysics
+    g = c.grid
+    gi = make_moist(c, 21, precision)
+    rng = np.random.default_rng(121)
+    mask = interior_mask(g)
+    dx, yb = random6(g, rng), random6(g, rng)
+    dt = c.scalar("dt")
+    dy = tl_step(gi, dt, dx)
+    xb = ad_step(gi, dt, yb)
+    lhs, rhs = dot6(dy, yb, mask), dot6(dx, xb, mask)
+    # the q_v block on its own: q_v in, q_v out
+    qq = tl_step(gi, dt, {"qv": dx["qv"]})["qv"]
+    qb = ad_step(gi, dt, {"qv": yb["qv"]})["qv"]
+    lq, rq = float(np.dot(qq[mask], yb["qv"][mask])), float(np.dot(dx["qv"][mask], qb[mask]))
+    print("moist dot product", lhs, rhs, abs(lhs - rhs) / abs(lhs), "qv block", lq, rq)
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+    assert abs(lq - rq) <= 1e-10 * abs(lq)
+    assert np.abs(dy["qv"][mask]).max() > 0 and np.abs(xb["qv"][mask]).max() > 0
+
+
+def test_moist_tangent_linear_matches_finite_differences():
+    from wfe_amd import Integrator
+    from wfe_amd.adjoint import tl_step
+    c = moist_case()
+    g = c.grid
+    gi = make_moist(c, 22)
+    x = {f: gi.get(f) for f in FIELDS6}
+    rng = np.random.default_rng(122)
+    dx = random6(g, rng)
+    dt = c.scalar("dt")
+    dy = tl_step(gi, dt, dx)
+    mask = interior_mask(g)
+    eps = 1e-6
+    outs = []
+    for sgn in (1, -1):
+        gp = Integrator.from_case(c, "fp64")
+        for f in FIELDS6:
+            gp.set(f, x[f] + sgn * eps * dx[f])
+        gp.step(dt)
+        outs.append({f: gp.get(f) for f in FIELDS6})
+    for f in FIELDS6:
+        fd = (outs[0][f] - outs[1][f])[mask] / (2 * eps)
+        err = np.linalg.norm(fd - dy[f][mask]) / np.linalg.norm(dy[f][mask])
+        print(f, err)
+        assert err < 1e-5, (f, err)
+
+
+def test_moist_gradient_matches_finite_differences():
+    """dJ/d(u, v, w, theta', pi', q_v) along a full direction and along a
+    q_v-only one, against centred differences of the forward objective."""
+    from wfe_amd import Integrator
+    from wfe_amd.adjoint import box_weights, ke_gradient
+    c = moist_case()
+    g = c.grid
+    gi = make_moist(c, 23)
+    x = {f: gi.get(f) for f in FIELDS6}
+    dt = c.scalar("dt")
+    w = box_weights(g, 6, 14, 5, 13)
+    nsteps, ckpt = 6, 4
+    J, grad = ke_gradient(gi, dt, nsteps, ckpt, w)
+    for f in FIELDS6:
+        np.testing.assert_array_equal(gi.get(f), x[f])
+    mask = interior_mask(g)
+    rng = np.random.default_rng(123)
+    d_all = random6(g, rng)
+    d_q = {f: (d_all[f] if f == "qv" else np.zeros(g.npts)) for f in FIELDS6}
+
+    def J_of(xs):
+        gp = Integrator.from_case(c, "fp64")
+        for f in FIELDS6:
+            gp.set(f, xs[f])
+        tot, ww = 0.0, w.reshape(g.NY, g.NX)
+        for _ in range(nsteps):
+            gp.step(dt)
+            u = gp.get("u").reshape(g.NZ, g.NY, g.NX)[g.ng]
+            v = gp.get("v").reshape(g.NZ, g.NY, g.NX)[g.ng]
+            tot += float((ww * 0.5 * (u * u + v * v)).sum()) / nsteps
+        return tot
+
+    assert J == pytest.approx(J_of(x), rel=1e-12)
+    for name, d in (("all", d_all), ("qv", d_q)):
+        gd = sum(float(np.dot(grad[f][mask], d[f][mask])) for f in FIELDS6)
+        errs = []
+        for eps in (1e-3, 1e-4, 1e-5, 1e-6):
+            fd = (J_of({f: x[f] + eps * d[f] for f in FIELDS6}) -
+                  J_of({f: x[f] - eps * d[f] for f in FIELDS6})) / (2 * eps)
+            errs.append(abs(fd - gd) / abs(gd))
+        print("moist gradient vs FD", name, gd, errs)
+        assert gd != 0.0
+        assert min(errs) < 1e-6, (name, errs)
+
+
+def test_moist_scope_is_enforced():
+    from wfe_amd import Integrator, WfeError
+    from wfe_amd.adjoint import tl_step
+    for over, msg in (({"microphysics": "on"}, "microphysics"),
+                      ({"pd_moist": "on"}, "positive-definite")):
+        c = moist_case(**over)
+        gi = Integrator.from_case(c, "fp64")
+        with pytest.raises(WfeError, match=msg):
+            tl_step(gi, c.scalar("dt"), {})
+    c = moist_case()
+    gi = make_moist(c, 24)
+    tl_step(gi, c.scalar("dt"), {})
+    qc = gi.get("qc")
+    qc[len(qc) // 2] = 1e-4
+    gi.set("qc", qc)
+    with pytest.raises(WfeError, match="q_c or q_r"):
+        tl_step(gi, c.scalar("dt"), {})
+
+
+def test_dry_contexts_give_a_zero_qv_output():
+    from wfe_amd.adjoint import ad_step, box_weights, ke_gradient, tl_step
+    c = dry_case()
+    g = c.grid
+    gi = make(c, 25)
+    rng = np.random.default_rng(125)
+    d = random6(g, rng)
+    assert not np.any(tl_step(gi, c.scalar("dt"), d)["qv"])
+    assert not np.any(ad_step(gi, c.scalar("dt"), d)["qv"])
+    assert not np.any(ke_gradient(gi, c.scalar("dt"), 2, 1, box_weights(g, 4, 20, 4, 16))[1]["qv"])

@@ -303,22 +303,29 @@ def test_cfg5_adjoint_at_the_supercell_shape():
     assert max(errs.values()) <= 1e-3, errs
 
 
-def test_adjoint_gradient_against_finite_differences_of_the_oracle():
+@pytest.mark.parametrize("moist", [False, True], ids=["dry", "moist"])
+def test_adjoint_gradient_against_finite_differences_of_the_oracle(moist):
     """Reference gradient: the objective evaluated by the FP64 oracle (its own
     case assembly and step) at x0 +- eps d, against the hand-written adjoint's
-    gradient along d, on a small dry periodic grid."""
+    gradient along d, on a small periodic grid; dry, and on the WK82 moist
+    sounding without microphysics with q_v in the direction (and along q_v
+    alone)."""
     from wfe_amd import Case, Integrator
-    from wfe_amd.adjoint import FIELDS5, box_weights, ke_gradient
+    from wfe_amd.adjoint import FIELDS, FIELDS5, box_weights, ke_gradient
     over = dict(nx=24, ny=20, nz=16, dx=400, dy=400, dz=300, bubble_xc=4800, bubble_yc=4000,
                 bubble_zc=1500, bubble_xr=1500, bubble_yr=1500, bubble_zr=1200, wind_u=6,
                 rayleigh_zd=3000, rayleigh_alpha=0.02, dt=2.0, acoustic_ns=4)
+    if moist:
+        over.update(profile="wk82", microphysics="off")
+    fields = FIELDS if moist else FIELDS5
     ini = CASES / "warm_bubble.ini"
     c = Case(ini, **over)
+    assert bool(c.params.moisture) == moist
     g = c.grid
     dt = c.scalar("dt")
     st = perturbed_state(c, 3)
     gi = Integrator.from_case(c, "fp64")
-    for f in FIELDS5:
+    for f in fields:
         gi.set(f, st[f])
     w = box_weights(g, 6, 14, 5, 13)
     nsteps = 6
@@ -327,14 +334,13 @@ def test_adjoint_gradient_against_finite_differences_of_the_oracle():
     mask = np.zeros((g.NZ, g.NY, g.NX), dtype=bool)
     mask[g.ng:g.ng + g.nz, g.ng:g.ng + g.ny, g.ng:g.ng + g.nx] = True
     mask = mask.ravel()
-    scale = {"u": 1.0, "v": 1.0, "w": 0.3, "thp": 0.5, "pip": 1e-4}
-    d = {f: np.where(mask, rng.standard_normal(g.npts) * scale[f], 0.0) for f in FIELDS5}
-    gd = sum(float(np.dot(grad[f][mask], d[f][mask])) for f in FIELDS5)
+    scale = {"u": 1.0, "v": 1.0, "w": 0.3, "thp": 0.5, "pip": 1e-4, "qv": 1e-3}
+    d = {f: np.where(mask, rng.standard_normal(g.npts) * scale[f], 0.0) for f in fields}
 
-    def J_oracle(eps):
+    def J_oracle(eps, dd):
         om = OracleCase(ini, **over).model()
-        for f in FIELDS5:
-            om.set(f, st[f] + eps * d[f])
+        for f in fields:
+            om.set(f, st[f] + eps * dd[f])
         tot, ww = 0.0, w.reshape(g.NY, g.NX)
         for s in range(nsteps):
             om.step(dt, s * dt)
@@ -343,12 +349,83 @@ def test_adjoint_gradient_against_finite_differences_of_the_oracle():
             tot += float((ww * 0.5 * (u * u + v * v)).sum()) / nsteps
         return tot
 
-    assert J == pytest.approx(J_oracle(0.0), rel=1e-12)
-    for eps, tol in ((1e-4, 1e-4), (1e-6, 1e-6)):
-        fd = (J_oracle(eps) - J_oracle(-eps)) / (2 * eps)
-        print(f"oracle FD eps={eps:g}: {fd!r} adjoint {gd!r} rel {abs(fd - gd) / abs(gd):.2e}",
-              flush=True)
-        assert abs(fd - gd) <= tol * abs(gd)
+    assert J == pytest.approx(J_oracle(0.0, d), rel=1e-12)
+    dirs = [("all", d)]
+    if moist:
+        dirs.append(("qv", {f: (d[f] if f == "qv" else np.zeros(g.npts)) for f in fields}))
+    for name, dd in dirs:
+        gd = sum(float(np.dot(grad[f][mask], dd[f][mask])) for f in fields)
+        assert gd != 0.0
+        for eps, tol in ((1e-4, 1e-4), (1e-6, 1e-6)):
+            fd = (J_oracle(eps, dd) - J_oracle(-eps, dd)) / (2 * eps)
+            print(f"oracle FD {name} eps={eps:g}: {fd!r} adjoint {gd!r} "
+                  f"rel {abs(fd - gd) / abs(gd):.2e}", flush=True)
+            assert abs(fd - gd) <= tol * abs(gd)
+
+
+def test_cfg5_moist_adjoint_at_the_supercell_shape():
+    """The moist cfg 5 case (WK82 sounding, no microphysics) at 120 x 120 x
+    40: six-field dot products in FP64 and about the FP32 trajectory; the
+    gradient over 20 steps (checkpoint every 5) along a q_v-only direction
+    against finite differences; the FP32 production gradient against it."""
+    from wfe_amd import Case, Integrator
+    from wfe_amd.adjoint import FIELDS, ad_step, ke_gradient, tl_step
+    c = Case(CASES / "cfg5_adjoint_moist.ini")
+    g = c.grid
+    assert (g.nx, g.ny, g.nz) == (120, 120, 40) and c.params.moisture
+    dt = c.scalar("dt")
+    rng = np.random.default_rng(20261203)
+    mask = np.zeros((g.NZ, g.NY, g.NX), dtype=bool)
+    mask[g.ng:g.ng + g.nz, g.ng:g.ng + g.ny, g.ng:g.ng + g.nx] = True
+    mask = mask.ravel()
+    scale = {"u": 1.0, "v": 1.0, "w": 0.3, "thp": 0.5, "pip": 1e-4, "qv": 1e-3}
+
+    def rand():
+        return {f: np.where(mask, rng.standard_normal(g.npts) * scale[f], 0.0) for f in FIELDS}
+
+    dx, yb = rand(), rand()
+    ctx = {p: Integrator.from_case(c, p) for p in ("fp64", "mixed")}
+    for p, gi in ctx.items():
+        dy = tl_step(gi, dt, dx)
+        xb = ad_step(gi, dt, yb)
+        lhs = sum(float(np.dot(dy[f][mask], yb[f][mask])) for f in FIELDS)
+        rhs = sum(float(np.dot(dx[f][mask], xb[f][mask])) for f in FIELDS)
+        print(f"cfg5 moist {p} dot product", lhs, rhs, abs(lhs - rhs) / abs(lhs), flush=True)
+        assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+    w = _box(g, rng, 20)
+    nsteps = 20
+    gi = ctx["fp64"]
+    x0 = {f: gi.get(f) for f in FIELDS}
+    J, grad = ke_gradient(gi, dt, nsteps, 5, w)
+    d = rand()
+    dq = {f: (d[f] if f == "qv" else np.zeros(g.npts)) for f in FIELDS}
+    gq = float(np.dot(grad["qv"][mask], d["qv"][mask]))
+
+    def J_of(xs):
+        gp = Integrator.from_case(c, "fp64")
+        for f in FIELDS:
+            gp.set(f, xs[f])
+        tot, ww = 0.0, w.reshape(g.NY, g.NX)
+        for _ in range(nsteps):
+            gp.step(dt)
+            u = gp.get("u").reshape(g.NZ, g.NY, g.NX)[g.ng]
+            v = gp.get("v").reshape(g.NZ, g.NY, g.NX)[g.ng]
+            tot += float((ww * 0.5 * (u * u + v * v)).sum()) / nsteps
+        return tot
+
+    assert J == pytest.approx(J_of(x0), rel=1e-12)
+    eps = 1e-5
+    fd = (J_of({f: x0[f] + eps * dq[f] for f in FIELDS}) -
+          J_of({f: x0[f] - eps * dq[f] for f in FIELDS})) / (2 * eps)
+    print("cfg5 moist q_v gradient vs FD", gq, fd, abs(fd - gq) / abs(gq), flush=True)
+    assert gq != 0.0 and abs(fd - gq) <= 1e-5 * abs(gq)
+    J32, grad32 = ke_gradient(ctx["mixed"], dt, nsteps, 5, w)
+    errs = {f: float(np.linalg.norm(grad32[f][mask] - grad[f][mask]) /
+                     np.linalg.norm(grad[f][mask])) for f in FIELDS}
+    print("cfg5 moist FP32 production vs FP64: J", abs(J32 - J) / abs(J), "gradient",
+          {f: f"{e:.2e}" for f, e in errs.items()}, flush=True)
+    assert abs(J32 - J) <= 1e-5 * abs(J)
+    assert max(errs.values()) <= 1e-3, errs
 
 
 SLOT_LIMITS = [("fused-64x14", 42), ("fused-64x10x4", 40), ("fused-64x16", 48),

@@ -14,7 +14,8 @@ from oracle_lib import BASE, METRIC, PARAMS, STATE, OracleCase
 from wfe_amd import Case
 from wfe_amd.lam_case import write_lam_case
 
-IDEALIZED = ("cfg1_dry_bubble", "cfg2_supercell", "cfg5_adjoint_dry", "warm_bubble", "schaer",
+IDEALIZED = ("cfg1_dry_bubble", "cfg2_supercell", "cfg5_adjoint_dry", "cfg5_adjoint_moist",
+             "warm_bubble", "schaer",
              "schaer_rest", "straka", "moist_blob", "bubble_outflow", "wk82_supercell")
 BDY = ("bdy_wgt",) + tuple(f"bdy_{s}_{v}" for s in ("lo", "hi")
                            for v in ("u", "v", "thp", "pip", "qv"))
@@ -42,6 +43,7 @@ def _compare(prod: Case, orc: OracleCase, file_mode: bool):
 def test_idealized_case_files(name):
     ini = CASES / f"{name}.ini"
     over = {"nx": 40, "ny": 24} if name in ("cfg2_supercell", "cfg5_adjoint_dry",
+                                            "cfg5_adjoint_moist",
                                             "wk82_supercell") else {}
     _compare(Case(ini, **over), OracleCase(ini, **over), False)
 

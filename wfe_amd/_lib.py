@@ -110,6 +110,12 @@ SIGNATURES = [
     ("wfe_ad_step", C.c_int, [C.c_void_p, C.c_double, C.POINTER(_DP), C.POINTER(_DP), C.c_size_t]),
     ("wfe_adj_gradient", C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_int, _DP, C.c_size_t,
                                    C.POINTER(C.c_double), C.POINTER(_DP), C.c_size_t]),
+    ("wfe_tl_step_q", C.c_int, [C.c_void_p, C.c_double, C.POINTER(_DP), C.POINTER(_DP),
+                                C.c_size_t]),
+    ("wfe_ad_step_q", C.c_int, [C.c_void_p, C.c_double, C.POINTER(_DP), C.POINTER(_DP),
+                                C.c_size_t]),
+    ("wfe_adj_gradient_q", C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_int, _DP, C.c_size_t,
+                                     C.POINTER(C.c_double), C.POINTER(_DP), C.c_size_t]),
     ("wfe_ensemble_create", C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]),
     ("wfe_ensemble_destroy", C.c_int, [C.c_void_p]),
     ("wfe_ensemble_step", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),

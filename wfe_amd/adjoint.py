@@ -1,10 +1,12 @@
 """Tangent-linear and adjoint of the dry large step through the C ABI, and a
 torch.autograd.Function for the lowest-level kinetic-energy objective.
 
-Scope (enforced by the library): FP64 or mixed contexts of dry cases with
-periodic lateral boundaries, flat terrain, no Coriolis, no diffusion, no w
-damping and no boundary relaxation. Fields are (u, v, w, thp, pip) in the
-padded layout of the case's grid. In a mixed context the linearisation is
+Scope (enforced by the library): FP64 or mixed contexts with periodic
+lateral boundaries, flat terrain, no Coriolis, no diffusion, no w damping,
+no boundary relaxation and no microphysics; with moisture on, q_v is a
+linearised field (flux-form transport without the positive-definite
+limiter, q_c = q_r = 0). Fields are (u, v, w, thp, pip, qv) in the padded
+layout of the case's grid; qv is zero on output for a dry case. In a mixed context the linearisation is
 about the FP32 trajectory the context itself computes; perturbations,
 adjoint variables and the objective are FP64 in both kinds of context.
 """
@@ -17,17 +19,18 @@ import numpy as np
 from ._lib import check, lib
 
 FIELDS5 = ("u", "v", "w", "thp", "pip")
+FIELDS = FIELDS5 + ("qv",)
 _DP = C.POINTER(C.c_double)
 
 
 def _ptrs(arrays):
-    return (_DP * 5)(*[a.ctypes.data_as(_DP) for a in arrays])
+    return (_DP * len(arrays))(*[a.ctypes.data_as(_DP) for a in arrays])
 
 
 def _prep(gi, d: dict):
     n = gi.grid.npts
     out = []
-    for f in FIELDS5:
+    for f in FIELDS:
         a = np.ascontiguousarray(d.get(f, np.zeros(n)), dtype=np.float64)
         if a.size != n:
             raise ValueError(f"{f}: expected {n} elements, got {a.size}")
@@ -38,17 +41,17 @@ def _prep(gi, d: dict):
 def tl_step(gi, dt: float, dx: dict) -> dict:
     """dy = M'(x) dx for one large step around the integrator's state."""
     xin = _prep(gi, dx)
-    out = [np.zeros(gi.grid.npts) for _ in FIELDS5]
-    check(lib().wfe_tl_step(gi._h, dt, _ptrs(xin), _ptrs(out), gi.grid.npts), "wfe_tl_step")
-    return dict(zip(FIELDS5, out))
+    out = [np.zeros(gi.grid.npts) for _ in FIELDS]
+    check(lib().wfe_tl_step_q(gi._h, dt, _ptrs(xin), _ptrs(out), gi.grid.npts), "wfe_tl_step_q")
+    return dict(zip(FIELDS, out))
 
 
 def ad_step(gi, dt: float, yb: dict) -> dict:
     """xb = M'(x)^T yb for one large step around the integrator's state."""
     yin = _prep(gi, yb)
-    out = [np.zeros(gi.grid.npts) for _ in FIELDS5]
-    check(lib().wfe_ad_step(gi._h, dt, _ptrs(yin), _ptrs(out), gi.grid.npts), "wfe_ad_step")
-    return dict(zip(FIELDS5, out))
+    out = [np.zeros(gi.grid.npts) for _ in FIELDS]
+    check(lib().wfe_ad_step_q(gi._h, dt, _ptrs(yin), _ptrs(out), gi.grid.npts), "wfe_ad_step_q")
+    return dict(zip(FIELDS, out))
 
 
 def ke_gradient(gi, dt: float, nsteps: int, ckpt_every: int, wgt2d: np.ndarray):
@@ -58,12 +61,12 @@ def ke_gradient(gi, dt: float, nsteps: int, ckpt_every: int, wgt2d: np.ndarray):
     w = np.ascontiguousarray(wgt2d, dtype=np.float64)
     if w.size != g.NX * g.NY:
         raise ValueError("wgt2d must be the padded NX*NY plane")
-    grads = [np.zeros(g.npts) for _ in FIELDS5]
+    grads = [np.zeros(g.npts) for _ in FIELDS]
     J = C.c_double()
-    check(lib().wfe_adj_gradient(gi._h, dt, int(nsteps), int(ckpt_every),
-                                 w.ctypes.data_as(_DP), w.size, C.byref(J), _ptrs(grads),
-                                 g.npts), "wfe_adj_gradient")
-    return J.value, dict(zip(FIELDS5, grads))
+    check(lib().wfe_adj_gradient_q(gi._h, dt, int(nsteps), int(ckpt_every),
+                                   w.ctypes.data_as(_DP), w.size, C.byref(J), _ptrs(grads),
+                                   g.npts), "wfe_adj_gradient_q")
+    return J.value, dict(zip(FIELDS, grads))
 
 
 def box_weights(grid, i0: int, i1: int, j0: int, j1: int) -> np.ndarray:
@@ -74,20 +77,24 @@ def box_weights(grid, i0: int, i1: int, j0: int, j1: int) -> np.ndarray:
 
 
 def kinetic_energy_function():
-    """torch.autograd.Function mapping the initial (u, v, w, thp, pip)
-    tensors (padded layout, float64) to the mean lowest-level kinetic energy
-    J over `nsteps` steps of an integrator; backward returns dJ/dx0
-    from the hand-written adjoint with checkpointing."""
+    """torch.autograd.Function mapping the initial (u, v, w, thp, pip) tensors,
+    and qv as a sixth when given (padded layout, float64), to the mean
+    lowest-level kinetic energy J over `nsteps` steps of an integrator;
+    backward returns dJ/dx0 from the hand-written adjoint with
+    checkpointing."""
     import torch
 
     class KineticEnergy(torch.autograd.Function):
         @staticmethod
-        def forward(ctx, gi, dt, nsteps, ckpt_every, wgt2d, u, v, w, thp, pip):
-            for name, t in zip(FIELDS5, (u, v, w, thp, pip)):
+        def forward(ctx, gi, dt, nsteps, ckpt_every, wgt2d, *xs):
+            if len(xs) not in (5, 6):
+                raise ValueError("expected (u, v, w, thp, pip) or (u, v, w, thp, pip, qv)")
+            names = FIELDS[:len(xs)]
+            for name, t in zip(names, xs):
                 gi.set(name, t.detach().cpu().numpy().astype(np.float64))
             J, grads = ke_gradient(gi, dt, nsteps, ckpt_every, wgt2d)
-            ctx.grads = [torch.from_numpy(grads[f]).to(u.device) for f in FIELDS5]
-            return torch.tensor(J, dtype=torch.float64, device=u.device)
+            ctx.grads = [torch.from_numpy(grads[f]).to(xs[0].device) for f in names]
+            return torch.tensor(J, dtype=torch.float64, device=xs[0].device)
 
         @staticmethod
         def backward(ctx, gJ):

@@ -6,7 +6,12 @@
 // stored trajectory, with the upwind branch of every interface value frozen
 // at the trajectory's mass flux. Scope: periodic lateral boundaries, flat
 // terrain (J = 1, no cross-metric terms), map factor 1, no Coriolis, no
-// diffusion, no vertical-Courant damping, no moisture.
+// diffusion, no vertical-Courant damping. With moisture on (no
+// microphysics, flux-form transport without the positive-definite limiter,
+// q_c = q_r = 0), q_v is a sixth linearised field: its transport and stage
+// update, the moist buoyancy in T_w and the virtual-temperature factor
+// (1 + 0.61 q_v) of the horizontal pressure gradient and of the column's
+// theta_v at w levels.
 //
 // Every AD kernel is the statement-by-statement transpose of its TL kernel;
 // contributions to shared inputs are scattered with FP64 atomics.
@@ -78,9 +83,10 @@ __device__ __forceinline__ double ad_face(const Q& q, double* qb, int c, int s, 
 // forward fill to the perturbation (k_fill_all) and the AD moves each ghost
 // cell's adjoint onto the interior cells it was copied from.
 struct AdFill {
-  double* f[5];
-  int kind[5];
-  int xlast[5], ylast[5];
+  double* f[6];
+  int kind[6];
+  int xlast[6], ylast[6];
+  int n;
 };
 
 __global__ void __launch_bounds__(256) k_ad_fill(Grid g, int bcx, int bcy, AdFill A) {
@@ -89,7 +95,8 @@ __global__ void __launch_bounds__(256) k_ad_fill(Grid g, int bcx, int bcy, AdFil
   const GhostCell q = ghost_cell(g, idx);
   const int c = g.id(q.i, q.j, q.k);
 #pragma unroll
-  for (int fi = 0; fi < 5; ++fi) {
+  for (int fi = 0; fi < 6; ++fi) {
+    if (fi >= A.n) break;
     double* f = A.f[fi];
     const double gb = f[c];
     f[c] = 0.0;
@@ -165,12 +172,19 @@ __global__ void k_ad_massflux_uvw(Grid g, Base<double> b, Metric<double> m, doub
 template <class S>
 struct TanTraj {  // trajectory of one stage (storage type of the forward)
   Traj<S> u, v, w, th, mfx, mfy, mfz, div;
+  Traj<S> q;  // the stage estimate's q_v (moist)
 };
 struct TanPert {  // perturbation (TL) or adjoint (AD) arrays of one stage
   double *u, *v, *w, *th, *mfx, *mfy, *mfz, *div;
+  double* q;
 };
 struct TanTend {  // tendency perturbations (TL out) or adjoints (AD in)
   double *tu, *tv, *tw, *tth;
+  // moist: q_v at time n (TL in / AD accumulated; ghost cells filled) and
+  // the stage's q_v = q_v,n + dts T_q (TL out / AD in)
+  double *qn, *qout;
+  double dts;
+  int moist;
 };
 
 template <class S>
@@ -234,7 +248,10 @@ __global__ void k_tl_tend(Grid g, Base<double> b, Metric<double> m, TanTraj<S> X
                        (tl_face(X.w, D.w, c + PL, PL, fzT, dfzT) - tl_face(X.w, D.w, c, PL, fzB, dfzB)) / dzw;
     const double dv = 0.5 * (X.div[c - PL] + X.div[c]), ddv = 0.5 * (D.div[c - PL] + D.div[c]);
     const double rw = b.rhobw[c] * m.jac[c2];
-    out.tw[c] = (-dfd + D.w[c] * dv + X.w[c] * ddv) / rw - m.ray_w[kk] * D.w[c];
+    double t = (-dfd + D.w[c] * dv + X.w[c] * ddv) / rw - m.ray_w[kk] * D.w[c];
+    // moist buoyancy g 0.61 (q_v at the w level - base) from time-n q_v
+    if (out.moist) t += kGrav * kEps61 * 0.5 * (out.qn[c - PL] + out.qn[c]);
+    out.tw[c] = t;
   }
   {  // theta'
     const double dfd =
@@ -245,6 +262,17 @@ __global__ void k_tl_tend(Grid g, Base<double> b, Metric<double> m, TanTraj<S> X
         (tl_face(X.th, D.th, c + PL, PL, X.mfz[c + PL], D.mfz[c + PL]) -
          tl_face(X.th, D.th, c, PL, X.mfz[c], D.mfz[c])) / dzc;
     out.tth[c] = (-dfd + D.th[c] * X.div[c] + X.th[c] * D.div[c]) / rt - m.ray_c[kk] * D.th[c];
+  }
+  if (out.moist) {  // q_v: flux-form transport, then the stage update
+    const double dfd =
+        (tl_face(X.q, D.q, c + 1, 1, X.mfx[c + 1], D.mfx[c + 1]) -
+         tl_face(X.q, D.q, c, 1, X.mfx[c], D.mfx[c])) / dx +
+        (tl_face(X.q, D.q, c + NX, NX, X.mfy[c + NX], D.mfy[c + NX]) -
+         tl_face(X.q, D.q, c, NX, X.mfy[c], D.mfy[c])) / dy +
+        (tl_face(X.q, D.q, c + PL, PL, X.mfz[c + PL], D.mfz[c + PL]) -
+         tl_face(X.q, D.q, c, PL, X.mfz[c], D.mfz[c])) / dzc;
+    const double tq = (-dfd + D.q[c] * X.div[c] + X.q[c] * D.div[c]) / rt;
+    out.qout[c] = out.qn[c] + out.dts * tq;
   }
 }
 
@@ -322,6 +350,11 @@ __global__ void k_ad_tend(Grid g, Base<double> b, Metric<double> m, TanTraj<S> X
     if (T != 0.0) {
       const double dzw = m.dzeta_w[kk];
       const double rw = b.rhobw[c] * m.jac[c2];
+      if (in.moist) {
+        const double qb = kGrav * kEps61 * 0.5 * T;
+        atomicAdd(&in.qn[c - PL], qb);
+        atomicAdd(&in.qn[c], qb);
+      }
       const double a = T / rw;
       const double fdb = -a;
       const double dv = 0.5 * (X.div[c - PL] + X.div[c]);
@@ -358,6 +391,22 @@ __global__ void k_ad_tend(Grid g, Base<double> b, Metric<double> m, TanTraj<S> X
       atomicAdd(&A.mfz[c], ad_face(X.th, A.th, c, PL, X.mfz[c], -fdb / dzc));
     }
   }
+  if (in.moist) {  // q_v stage update and transport
+    const double Qb = in.qout[c];
+    if (Qb != 0.0) {
+      atomicAdd(&in.qn[c], Qb);
+      const double a = in.dts * Qb / rt;
+      const double fdb = -a;
+      atomicAdd(&A.q[c], a * X.div[c]);
+      atomicAdd(&A.div[c], a * X.q[c]);
+      atomicAdd(&A.mfx[c + 1], ad_face(X.q, A.q, c + 1, 1, X.mfx[c + 1], fdb / dx));
+      atomicAdd(&A.mfx[c], ad_face(X.q, A.q, c, 1, X.mfx[c], -fdb / dx));
+      atomicAdd(&A.mfy[c + NX], ad_face(X.q, A.q, c + NX, NX, X.mfy[c + NX], fdb / dy));
+      atomicAdd(&A.mfy[c], ad_face(X.q, A.q, c, NX, X.mfy[c], -fdb / dy));
+      atomicAdd(&A.mfz[c + PL], ad_face(X.q, A.q, c + PL, PL, X.mfz[c + PL], fdb / dzc));
+      atomicAdd(&A.mfz[c], ad_face(X.q, A.q, c, PL, X.mfz[c], -fdb / dzc));
+    }
+  }
 }
 
 // ------------------------------------------------- horizontal acoustic step
@@ -370,9 +419,11 @@ template <class S>
 struct UVTraj {
   const double *pcur, *pprev, *pn;
   Traj<S> thc, thn;
+  Traj<S> qn;  // time-n q_v, ghost cells filled (read when p.moisture)
 };
 struct UVTan {  // TL: perturbations in, du/dv out; AD: adjoints
   double *u_src, *v_src, *u_dst, *v_dst, *pcur, *pprev, *pn, *thc, *thn, *tu, *tv;
+  double* qn;
 };
 
 template <class S>
@@ -393,9 +444,14 @@ struct UVLin {
     const double pc = X.pcur[mapped(ii, jj, k)];
     return pc + p.smdiv * (pc - X.pn[c]);
   }
-  __device__ double thv(int ii, int jj, int k) const {
+  // theta_v = (theta_bar + theta') (1 + 0.61 q_v,n), the factor when moist
+  __device__ double dry(int ii, int jj, int k) const {
     const int c = g.id(ii, jj, k);
-    return (double)b.thb[c] + (double)(inside(ii, jj) ? X.thc[c] : X.thn[c]);
+    return b.thb[c] + (inside(ii, jj) ? X.thc[c] : X.thn[c]);
+  }
+  __device__ double fq(int c) const { return p.moisture ? 1.0 + kEps61 * X.qn[c] : 1.0; }
+  __device__ double thv(int ii, int jj, int k) const {
+    return dry(ii, jj, k) * fq(g.id(ii, jj, k));
   }
   __device__ double dpstar(const UVTan& D, int ii, int jj, int k) const {
     const int c = g.id(ii, jj, k);
@@ -405,7 +461,8 @@ struct UVLin {
   }
   __device__ double dthv(const UVTan& D, int ii, int jj, int k) const {
     const int c = g.id(ii, jj, k);
-    return inside(ii, jj) ? D.thc[c] : D.thn[c];
+    const double dt = (inside(ii, jj) ? D.thc[c] : D.thn[c]) * fq(c);
+    return p.moisture ? dt + dry(ii, jj, k) * kEps61 * D.qn[c] : dt;
   }
   __device__ void ad_pstar(const UVTan& A, int ii, int jj, int k, double pb) const {
     const int c = g.id(ii, jj, k);
@@ -419,7 +476,8 @@ struct UVLin {
   }
   __device__ void ad_thv(const UVTan& A, int ii, int jj, int k, double tb) const {
     const int c = g.id(ii, jj, k);
-    atomicAdd(inside(ii, jj) ? &A.thc[c] : &A.thn[c], tb);
+    atomicAdd(inside(ii, jj) ? &A.thc[c] : &A.thn[c], tb * fq(c));
+    if (p.moisture) atomicAdd(&A.qn[c], tb * dry(ii, jj, k) * kEps61);
   }
 };
 
@@ -513,11 +571,24 @@ struct WPTraj {
   Traj<S> w;
   const double* pip;
   Traj<S> th;
+  Traj<S> qn;  // time-n q_v (read when p.moisture)
 };
 struct WPTan {  // TL: perturbations; AD: adjoints
   double *u, *v, *w, *pip, *th, *tw, *tth;  // inputs
   double *w_out, *pip_out, *th_out;         // outputs
+  double* qn;                               // time-n q_v (input)
 };
+
+// theta_v at the w level below cell c: its dry part and the moist factor.
+template <class S>
+__device__ __forceinline__ double thvw_dry(const Base<double>& b, const WPTraj<S>& X, int c,
+                                           int PL) {
+  return b.thbw[c] + 0.5 * (X.th[c - PL] + X.th[c]);
+}
+template <class S>
+__device__ __forceinline__ double thvw_fq(const Params& p, const WPTraj<S>& X, int c, int PL) {
+  return p.moisture ? 1.0 + kEps61 * 0.5 * (X.qn[c - PL] + X.qn[c]) : 1.0;
+}
 
 template <int MAXL>
 struct WPCol {
@@ -547,7 +618,7 @@ __device__ void wp_forward(const Grid& g, const Params& p, const Base<double>& b
     s.E[k] = dtau * Kt * c1 * rdzc;
     if (k >= 1) {
       const double rJdzw = m.rdzw[kk];  // J = 1
-      const double thvw = (double)b.thbw[c] + 0.5 * ((double)X.th[c - PL] + (double)X.th[c]);
+      const double thvw = thvw_dry(b, X, c, PL) * thvw_fq(p, X, c, PL);
       s.G[k] = dtau * kCp * thvw * c1 * rJdzw;
       const double buoy = kGrav * 0.5 * ((double)X.th[c - PL] + (double)X.th[c]) / (double)b.thbw[c];
       s.R[k] = (double)X.w[c] + dtau * buoy - dtau * kCp * thvw * c2 * (X.pip[c] - X.pip[c - PL]) * rJdzw;
@@ -605,8 +676,10 @@ __global__ void __launch_bounds__(64) k_tl_wpi(Grid g, Params p, Base<double> b,
     dR[k] = 0.0;
     if (k >= 1) {
       const double rJdzw = m.rdzw[kk];
-      const double thvw = (double)b.thbw[c] + 0.5 * ((double)X.th[c - PL] + (double)X.th[c]);
-      const double dthvw = 0.5 * (D.th[c - PL] + D.th[c]);
+      const double th_d = thvw_dry(b, X, c, PL), fq = thvw_fq(p, X, c, PL);
+      const double thvw = th_d * fq;
+      double dthvw = 0.5 * (D.th[c - PL] + D.th[c]) * fq;
+      if (p.moisture) dthvw += th_d * kEps61 * 0.5 * (D.qn[c - PL] + D.qn[c]);
       dG[k] = dtau * kCp * dthvw * c1 * rJdzw;
       const double dbuoy = kGrav * 0.5 * (D.th[c - PL] + D.th[c]) / b.thbw[c];
       const double dp = X.pip[c] - X.pip[c - PL], ddp = D.pip[c] - D.pip[c - PL];
@@ -707,11 +780,18 @@ __global__ void __launch_bounds__(64) k_ad_wpi(Grid g, Params p, Base<double> b,
     atomicAdd(&A.v[c], -Dhb * cf.rtjy[c] * g.rdy);
     if (k >= 1) {
       const double rJdzw = m.rdzw[kk];
-      const double thvw = (double)b.thbw[c] + 0.5 * ((double)X.th[c - PL] + (double)X.th[c]);
+      const double th_d = thvw_dry(b, X, c, PL), fq = thvw_fq(p, X, c, PL);
+      const double thvw = th_d * fq;
       const double dp = X.pip[c] - X.pip[c - PL];
-      // G = dtau cp thvw c1 rJdzw; R = w + dtau g 0.5 (thm + thk)/thbw - dtau cp thvw c2 dp rJdzw
+      // G = dtau cp thvw c1 rJdzw; R = w + dtau g 0.5 (thm + thk)/thbw - dtau cp thvw c2 dp rJdzw;
+      // thvw = (thbw + 0.5 (thm + thk)) (1 + 0.61 0.5 (qm + qk))
       const double thvwb = Gb[k] * dtau * kCp * c1 * rJdzw - Rb[k] * dtau * kCp * c2 * dp * rJdzw;
-      const double thsum_b = 0.5 * thvwb + Rb[k] * dtau * kGrav * 0.5 / b.thbw[c];
+      if (p.moisture) {
+        const double qb = th_d * kEps61 * 0.5 * thvwb;
+        atomicAdd(&A.qn[c - PL], qb);
+        atomicAdd(&A.qn[c], qb);
+      }
+      const double thsum_b = 0.5 * thvwb * fq + Rb[k] * dtau * kGrav * 0.5 / b.thbw[c];
       A.th[c - PL] += thsum_b;
       A.th[c] += thsum_b;
       A.w[c] += Rb[k];

@@ -1732,20 +1732,24 @@ void Ctx<T>::acoustic_substeps(double dtau, int nsub) {
 class AdjointBase {
  public:
   virtual ~AdjointBase() = default;
-  virtual void tl_step_host(double dt, const double* const* dx, double* const* dy) = 0;
-  virtual void ad_step_host(double dt, const double* const* yb, double* const* xb) = 0;
+  // Host arrays: nf = 5 (u, v, w, theta', pi') or 6 (and q_v); the q_v
+  // entry is ignored on input and zero on output when moisture is off.
+  virtual void tl_step_host(double dt, const double* const* dx, double* const* dy, int nf) = 0;
+  virtual void ad_step_host(double dt, const double* const* yb, double* const* xb, int nf) = 0;
   virtual double gradient(double dt, int nsteps, int ckpt, const double* wgt2d,
-                          double* const* grad) = 0;
+                          double* const* grad, int nf) = 0;
 };
 
 template <class T>
 class Adjoint final : public AdjointBase {
  public:
-  // Fields u, v, w, theta', pi' in storage type U (pi' always FP64).
+  // Fields u, v, w, theta', pi' in storage type U (pi' always FP64), and
+  // q_v when moisture is on (null otherwise).
   template <class U>
   struct Set5T {
     U *u = nullptr, *v = nullptr, *w = nullptr, *th = nullptr;
     double* pip = nullptr;
+    U* q = nullptr;
   };
   using Set5 = Set5T<double>;  // perturbations / adjoint variables
   using TSet = Set5T<T>;       // trajectory / model states
@@ -1760,6 +1764,7 @@ class Adjoint final : public AdjointBase {
   explicit Adjoint(Ctx<T>& f) : F(f) {
     check_scope();
     const auto& hp = F.hp_;
+    moist_ = hp.moisture != 0;
     ns_ = std::max(1, hp.acoustic_ns);
     nsub_[0] = 1;
     nsub_[1] = std::max(1, ns_ / 2);
@@ -1772,6 +1777,7 @@ class Adjoint final : public AdjointBase {
       s.w = A.alloc<U>(n);
       s.th = A.alloc<U>(n);
       s.pip = A.alloc<double>(n);
+      if (moist_) s.q = A.alloc<U>(n);
     };
     auto set4 = [&](auto& s) {
       using U = std::remove_pointer_t<decltype(s.a)>;
@@ -1796,18 +1802,25 @@ class Adjoint final : public AdjointBase {
     set5(xb_);
     set5(tmp_);
     zero_q_ = A.alloc<T>(n);
+    if (moist_) {
+      qc_scr_ = A.alloc<T>(n);
+      qr_scr_ = A.alloc<T>(n);
+    }
     jsum_ = A.alloc<double>(1);
     wgt_ = A.alloc<double>(F.n2_);
     widen_constants();
   }
 
-  // The linearised model covers the dry periodic flat step only. The
-  // context can change after the adjoint was set up (boundary relaxation is
-  // switched on by wfe_set_bdy_weights), so every entry checks again.
+  // The linearised model covers the periodic flat step, dry or with q_v
+  // transported and acting through buoyancy and theta_v but no
+  // microphysics. The context can change after the adjoint was set up
+  // (boundary relaxation is switched on by wfe_set_bdy_weights, the state
+  // is overwritten), so every entry checks again.
   void check_scope() const {
     const auto& hp = F.hp_;
     std::string why;
-    if (hp.moisture) why = "moisture is on";
+    if (hp.moisture && hp.microphysics) why = "microphysics is on";
+    else if (hp.moisture && hp.pd_moist) why = "positive-definite moisture transport is on";
     else if (hp.bc_x_open || hp.bc_y_open) why = "lateral boundaries are not periodic";
     else if (F.p_.terrain) why = "terrain slopes are non-zero";
     else if (hp.diff_K != 0) why = "diffusion is on";
@@ -1822,7 +1835,19 @@ class Adjoint final : public AdjointBase {
       for (T x : fc)
         if (x != 0) why = "Coriolis is on";
     }
-    if (!why.empty()) throw std::runtime_error("adjoint: scope is the dry periodic flat step (" + why + ")");
+    if (why.empty() && hp.moisture) {
+      // q_c and q_r are not linearised: without microphysics they stay
+      // zero when they start at zero
+      std::vector<T> q(F.npts_);
+      for (const T* d : {F.S_[F.cur_].qc, F.S_[F.cur_].qr}) {
+        WFE_HIP(hipMemcpy(q.data(), d, F.npts_ * sizeof(T), hipMemcpyDeviceToHost));
+        for (T x : q)
+          if (x != 0) why = "q_c or q_r is non-zero";
+      }
+    }
+    if (!why.empty())
+      throw std::runtime_error("adjoint: scope is the periodic flat step without microphysics (" +
+                               why + ")");
   }
 
   // Forward step of F from its current state, keeping the trajectory.
@@ -1839,7 +1864,19 @@ class Adjoint final : public AdjointBase {
         copy5(est_[m], sub_[m - 1][nsub_[m - 1] - 1], st);
         f.fill_ghosts(est_set(m, sn));
       }
-      f.slow_tendencies(est_set(m, sn), dt);
+      if (moist_) {
+        // q_v of the stage = q_v,n + dts T_q(estimate), written where T_q is
+        // formed (as the forward step does) into the stage's output set;
+        // q_c, q_r (zero) go to scratch, or into the new state at the end
+        StateSet<T> qo{};
+        const StateSet<T>& nxt = f.S_[f.cur_ ^ 1];
+        qo.qv = sub_[m][nsub_[m] - 1].q;
+        qo.qc = m == 2 ? nxt.qc : qc_scr_;
+        qo.qr = m == 2 ? nxt.qr : qr_scr_;
+        f.slow_tendencies(est_set(m, sn), dt, &qo, sdt[m], true);
+      } else {
+        f.slow_tendencies(est_set(m, sn), dt);
+      }
       copy(mf_[m].a, f.mfx_, st);
       copy(mf_[m].b, f.mfy_, st);
       copy(mf_[m].c, f.mfz_, st);
@@ -1862,7 +1899,7 @@ class Adjoint final : public AdjointBase {
         a.uv.pn = sn.pip;
         a.uv.thp_cur = src.th;
         a.uv.thp_n = sn.thp;
-        a.uv.qv = zero_q_;
+        a.uv.qv = moist_ ? sn.qv : zero_q_;  // time-n q_v
         a.uv.tu = T_[m].a;
         a.uv.tv = T_[m].b;
         a.uv.dtau = (T)dtau;
@@ -1904,20 +1941,20 @@ class Adjoint final : public AdjointBase {
             dmf_[m].c, dmf_[m].d);
       }
       dev::k_tl_tend<<<grid3(), dim3(64, 4), 0, st>>>(f.g_, b64_, m64_, traj(m), pert(m),
-                                                       tend(dT_[m]));
+                                                       tend(m, sdt[m]));
       const double dtau = sdt[m] / nsub_[m];
       for (int s = 0; s < nsub_[m]; ++s) {
         const TSet xs = s == 0 ? est_[0] : sub_[m][s - 1];
         const Set5 ds = s == 0 ? dest_[0] : dsub_[m][s - 1];
         dev::UVTraj<T> X{xs.pip, s <= 1 ? est_[0].pip : sub_[m][s - 2].pip, est_[0].pip, xs.th,
-                      est_[0].th};
+                         est_[0].th, est_[0].q};
         dev::UVTan D{ds.u, ds.v, dsub_[m][s].u, dsub_[m][s].v, ds.pip,
                      s <= 1 ? dest_[0].pip : dsub_[m][s - 2].pip, dest_[0].pip, ds.th,
-                     dest_[0].th, dT_[m].a, dT_[m].b};
+                     dest_[0].th, dT_[m].a, dT_[m].b, dest_[0].q};
         dev::k_tl_uv<<<grid_uv(), dim3(64, 4), 0, st>>>(f.g_, f.p_, b64_, m64_, X, D, dtau);
-        dev::WPTraj<T> W{sub_[m][s].u, sub_[m][s].v, xs.w, xs.pip, xs.th};
+        dev::WPTraj<T> W{sub_[m][s].u, sub_[m][s].v, xs.w, xs.pip, xs.th, est_[0].q};
         dev::WPTan DW{dsub_[m][s].u, dsub_[m][s].v, ds.w, ds.pip, ds.th, dT_[m].c, dT_[m].d,
-                      dsub_[m][s].w, dsub_[m][s].pip, dsub_[m][s].th};
+                      dsub_[m][s].w, dsub_[m][s].pip, dsub_[m][s].th, dest_[0].q};
         dev::k_tl_wpi<T, 64><<<grid_col(), 64, 0, st>>>(f.g_, f.p_, b64_, m64_, c64_, W,
                                                        T_[m].c, DW, dtau);
       }
@@ -1938,20 +1975,20 @@ class Adjoint final : public AdjointBase {
       for (int s = nsub_[m] - 1; s >= 0; --s) {
         const TSet xs = s == 0 ? est_[0] : sub_[m][s - 1];
         const Set5 ds = s == 0 ? dest_[0] : dsub_[m][s - 1];
-        dev::WPTraj<T> W{sub_[m][s].u, sub_[m][s].v, xs.w, xs.pip, xs.th};
+        dev::WPTraj<T> W{sub_[m][s].u, sub_[m][s].v, xs.w, xs.pip, xs.th, est_[0].q};
         dev::WPTan AW{dsub_[m][s].u, dsub_[m][s].v, ds.w, ds.pip, ds.th, dT_[m].c, dT_[m].d,
-                      dsub_[m][s].w, dsub_[m][s].pip, dsub_[m][s].th};
+                      dsub_[m][s].w, dsub_[m][s].pip, dsub_[m][s].th, dest_[0].q};
         dev::k_ad_wpi<T, 64><<<grid_col(), 64, 0, st>>>(f.g_, f.p_, b64_, m64_, c64_, W,
                                                        T_[m].c, AW, dtau);
         dev::UVTraj<T> X{xs.pip, s <= 1 ? est_[0].pip : sub_[m][s - 2].pip, est_[0].pip, xs.th,
-                      est_[0].th};
+                         est_[0].th, est_[0].q};
         dev::UVTan AU{ds.u, ds.v, dsub_[m][s].u, dsub_[m][s].v, ds.pip,
                       s <= 1 ? dest_[0].pip : dsub_[m][s - 2].pip, dest_[0].pip, ds.th,
-                      dest_[0].th, dT_[m].a, dT_[m].b};
+                      dest_[0].th, dT_[m].a, dT_[m].b, dest_[0].q};
         dev::k_ad_uv<<<grid_uv(), dim3(64, 4), 0, st>>>(f.g_, f.p_, b64_, m64_, X, AU, dtau);
       }
       dev::k_ad_tend<<<grid3(), dim3(64, 4), 0, st>>>(f.g_, b64_, m64_, traj(m), pert(m),
-                                                       tend(dT_[m]));
+                                                       tend(m, sdt[m]));
       {
         dim3 grd(cdiv(f.g_.nx + 5, 64), cdiv(f.g_.ny + 5, 4), (unsigned)f.g_.nz);
         dev::k_ad_massflux<<<grd, dim3(64, 4), 0, st>>>(f.g_, m64_, dmf_[m].a, dmf_[m].b,
@@ -1975,33 +2012,33 @@ class Adjoint final : public AdjointBase {
   // The host entries leave the context as they found it: the saved state
   // goes back into the same state set, and the set that is current (which
   // record_step flips) is restored, so ensembles see the parity unchanged.
-  void tl_step_host(double dt, const double* const* dx, double* const* dy) override {
+  void tl_step_host(double dt, const double* const* dx, double* const* dy, int nf) override {
     check_scope();
     const int par = F.cur_;
     save_state(x0_);
     record_step(dt);
-    upload5(tmp_, dx);
+    upload5(tmp_, dx, nf);
     tl(dt, tmp_, xb_);
-    download5(xb_, dy);
+    download5(xb_, dy, nf);
     F.cur_ = par;
     restore_state(x0_);
   }
-  void ad_step_host(double dt, const double* const* yb, double* const* xb) override {
+  void ad_step_host(double dt, const double* const* yb, double* const* xb, int nf) override {
     check_scope();
     const int par = F.cur_;
     save_state(x0_);
     record_step(dt);
-    upload5(tmp_, yb);
+    upload5(tmp_, yb, nf);
     ad(dt, tmp_, xb_);
-    download5(xb_, xb);
+    download5(xb_, xb, nf);
     F.cur_ = par;
     restore_state(x0_);
   }
 
   // J = (1/N) sum_{n=1..N} sum_{i,j} wgt(i,j) 0.5 (u^2 + v^2) at the lowest
   // level, and its gradient with respect to the initial state.
-  double gradient(double dt, int nsteps, int ckpt, const double* wgt2d,
-                  double* const* grad) override {
+  double gradient(double dt, int nsteps, int ckpt, const double* wgt2d, double* const* grad,
+                  int nf) override {
     if (nsteps < 1 || ckpt < 1) throw ArgError("adjoint gradient: nsteps, ckpt must be >= 1");
     check_scope();
     Ctx<T>& f = F;
@@ -2023,6 +2060,7 @@ class Adjoint final : public AdjointBase {
         c.w = B.alloc<T>(f.npts_);
         c.th = B.alloc<T>(f.npts_);
         c.pip = B.alloc<double>(f.npts_);
+        if (moist_) c.q = B.alloc<T>(f.npts_);
       }
     // forward sweep: checkpoints and J
     WFE_HIP(hipMemsetAsync(jsum_, 0, sizeof(double), st));
@@ -2051,7 +2089,7 @@ class Adjoint final : public AdjointBase {
         ad(dt, tmp_, xb_);
       }
     }
-    download5(xb_, grad);
+    download5(xb_, grad, nf);
     f.cur_ = par;
     restore_state(x0_);
     WFE_HIP(hipStreamSynchronize(st));
@@ -2113,7 +2151,9 @@ class Adjoint final : public AdjointBase {
       c64_.aw = wide(cf.aw, n3);
     }
   }
-  static TSet as5(const StateSet<T>& s) { return TSet{s.u, s.v, s.w, s.thp, s.pip}; }
+  TSet as5(const StateSet<T>& s) const {
+    return TSet{s.u, s.v, s.w, s.thp, s.pip, moist_ ? s.qv : nullptr};
+  }
   StateSet<T> est_set(int m, const StateSet<T>& sn) const {
     StateSet<T> e = sn;
     e.u = est_[m].u;
@@ -2121,17 +2161,24 @@ class Adjoint final : public AdjointBase {
     e.w = est_[m].w;
     e.thp = est_[m].th;
     e.pip = est_[m].pip;
+    if (moist_) e.qv = est_[m].q;
     return e;
   }
   dev::TanTraj<T> traj(int m) const {
-    return dev::TanTraj<T>{est_[m].u, est_[m].v, est_[m].w, est_[m].th,
-                        mf_[m].a, mf_[m].b, mf_[m].c, mf_[m].d};
+    return dev::TanTraj<T>{est_[m].u, est_[m].v, est_[m].w, est_[m].th, mf_[m].a,
+                           mf_[m].b,  mf_[m].c,  mf_[m].d,  est_[m].q};
   }
   dev::TanPert pert(int m) const {
     return dev::TanPert{dest_[m].u, dest_[m].v, dest_[m].w, dest_[m].th,
-                        dmf_[m].a, dmf_[m].b, dmf_[m].c, dmf_[m].d};
+                        dmf_[m].a,  dmf_[m].b,  dmf_[m].c,  dmf_[m].d, dest_[m].q};
   }
-  static dev::TanTend tend(const Set4& t) { return dev::TanTend{t.a, t.b, t.c, t.d}; }
+  // Tendencies of stage m; moist: time-n q_v and the stage's q_v output
+  // (held in the stage's last substep set).
+  dev::TanTend tend(int m, double dts) const {
+    const Set4& t = dT_[m];
+    return dev::TanTend{t.a, t.b, t.c, t.d, dest_[0].q, dsub_[m][nsub_[m] - 1].q, dts,
+                        moist_ ? 1 : 0};
+  }
   dim3 grid3() const { return dim3(cdiv(F.g_.nx, 64), cdiv(F.g_.ny, 4), (unsigned)F.g_.nz); }
   dim3 grid_uv() const {
     return dim3(cdiv(F.g_.nx + 1, 64), cdiv(F.g_.ny + 1, 4), (unsigned)F.g_.nz);
@@ -2149,6 +2196,7 @@ class Adjoint final : public AdjointBase {
     copy(d.w, s.w, st);
     copy(d.th, s.th, st);
     copy(d.pip, s.pip, st);
+    if (moist_) copy(d.q, s.q, st);
   }
   void zero(double* d) {
     WFE_HIP(hipMemsetAsync(d, 0, F.npts_ * sizeof(double), F.stream_));
@@ -2167,6 +2215,7 @@ class Adjoint final : public AdjointBase {
     zero(s.w);
     zero(s.th);
     zero(s.pip);
+    if (moist_) zero(s.q);
   }
   void zero4(const Set4& s) {
     zero(s.a);
@@ -2177,18 +2226,18 @@ class Adjoint final : public AdjointBase {
   dev::FillList<double> fill_list(const Set5& s) const {
     dev::FillList<double> L{};
     const int nx = F.g_.nx, ny = F.g_.ny;
-    double* f[5] = {s.u, s.v, s.w, s.th, s.pip};
-    const int kind[5] = {dev::kFillScalar, dev::kFillScalar, dev::kFillW, dev::kFillScalar,
-                         dev::kFillScalar};
-    const int xl[5] = {nx, nx - 1, nx - 1, nx - 1, nx - 1};
-    const int yl[5] = {ny - 1, ny, ny - 1, ny - 1, ny - 1};
-    for (int q = 0; q < 5; ++q) {
+    double* f[6] = {s.u, s.v, s.w, s.th, s.pip, s.q};
+    const int kind[6] = {dev::kFillScalar, dev::kFillScalar, dev::kFillW, dev::kFillScalar,
+                         dev::kFillScalar, dev::kFillScalar};
+    const int xl[6] = {nx, nx - 1, nx - 1, nx - 1, nx - 1, nx - 1};
+    const int yl[6] = {ny - 1, ny, ny - 1, ny - 1, ny - 1, ny - 1};
+    L.n = moist_ ? 6 : 5;
+    for (int q = 0; q < L.n; ++q) {
       L.f[q] = f[q];
       L.kind[q] = kind[q];
       L.xlast[q] = xl[q];
       L.ylast[q] = yl[q];
     }
-    L.n = 5;
     return L;
   }
   void fill5(const Set5& s) {
@@ -2201,7 +2250,8 @@ class Adjoint final : public AdjointBase {
   void ad_fill5(const Set5& s) {
     const dev::FillList<double> L = fill_list(s);
     dev::AdFill A{};
-    for (int q = 0; q < 5; ++q) {
+    A.n = L.n;
+    for (int q = 0; q < L.n; ++q) {
       A.f[q] = L.f[q];
       A.kind[q] = L.kind[q];
       A.xlast[q] = L.xlast[q];
@@ -2212,9 +2262,9 @@ class Adjoint final : public AdjointBase {
   }
   void add5(const Set5& d, const Set5& s) {
     const int n = (int)F.npts_;
-    double* dd[5] = {d.u, d.v, d.w, d.th, d.pip};
-    const double* ss[5] = {s.u, s.v, s.w, s.th, s.pip};
-    for (int q = 0; q < 5; ++q)
+    double* dd[6] = {d.u, d.v, d.w, d.th, d.pip, d.q};
+    const double* ss[6] = {s.u, s.v, s.w, s.th, s.pip, s.q};
+    for (int q = 0; q < (moist_ ? 6 : 5); ++q)
       dev::k_axpy<<<cdiv(n, 256), 256, 0, F.stream_>>>(n, 1.0, ss[q], dd[q]);
   }
   void objective(const TSet& x, double scale) {
@@ -2227,19 +2277,30 @@ class Adjoint final : public AdjointBase {
   }
   void save_state(const TSet& d) { copy5(d, as5(F.S_[F.cur_]), F.stream_); }
   void restore_state(const TSet& s) { copy5(as5(F.S_[F.cur_]), s, F.stream_); }
-  void upload5(const Set5& d, const double* const* h) {
-    double* dd[5] = {d.u, d.v, d.w, d.th, d.pip};
+  // nf host arrays in; q_v (the sixth) is used when moist, else ignored; a
+  // moist set given five arrays gets a zero q_v.
+  void upload5(const Set5& d, const double* const* h, int nf) {
+    double* dd[6] = {d.u, d.v, d.w, d.th, d.pip, d.q};
     WFE_HIP(hipStreamSynchronize(F.stream_));
-    for (int q = 0; q < 5; ++q) {
+    for (int q = 0; q < (moist_ ? 6 : 5); ++q) {
+      if (q >= nf) {
+        WFE_HIP(hipMemset(dd[q], 0, F.npts_ * sizeof(double)));
+        continue;
+      }
       const std::vector<double> t = F.template to_dev<double>(h[q], true);
       WFE_HIP(hipMemcpy(dd[q], t.data(), F.npts_ * sizeof(double), hipMemcpyHostToDevice));
     }
   }
-  void download5(const Set5& s, double* const* h) {
-    const double* ss[5] = {s.u, s.v, s.w, s.th, s.pip};
+  // nf host arrays out; a sixth without moisture is all zero.
+  void download5(const Set5& s, double* const* h, int nf) {
+    const double* ss[6] = {s.u, s.v, s.w, s.th, s.pip, s.q};
     WFE_HIP(hipStreamSynchronize(F.stream_));
     std::vector<double> t(F.npts_);
-    for (int q = 0; q < 5; ++q) {
+    for (int q = 0; q < nf; ++q) {
+      if (q == 5 && !moist_) {
+        std::fill(h[q], h[q] + F.host_n3(), 0.0);
+        continue;
+      }
       WFE_HIP(hipMemcpy(t.data(), ss[q], F.npts_ * sizeof(double), hipMemcpyDeviceToHost));
       F.to_host(t, h[q], true);
     }
@@ -2256,7 +2317,9 @@ class Adjoint final : public AdjointBase {
   TSet4 mf_[3], T_[3];
   Set4 dmf_[3], dT_[3];
   Set5 xb_, tmp_;
+  bool moist_ = false;
   T* zero_q_ = nullptr;
+  T *qc_scr_ = nullptr, *qr_scr_ = nullptr;
   double* jsum_ = nullptr;
   double* wgt_ = nullptr;
 };
@@ -2318,6 +2381,50 @@ wfe::AdjointBase& adjoint_of(wfe_ctx* ctx) {
           *static_cast<wfe::Ctx<float>*>(ctx->impl.get()));
   }
   return *ctx->adj;
+}
+
+}  // namespace
+
+namespace {
+
+int adj_tl(wfe_ctx* ctx, double dt, const double* const* dx, double* const* dy, size_t n,
+           int nf, const char* who) {
+  if (int e = need_ctx(ctx)) return e;
+  return guarded([&] {
+    if (!dx || !dy) throw wfe::ArgError(std::string(who) + ": null field arrays");
+    for (int q = 0; q < nf; ++q)
+      if (!dx[q] || !dy[q]) throw wfe::ArgError(std::string(who) + ": null field array");
+    wfe::AdjointBase& a = adjoint_of(ctx);
+    check_n3(ctx, n);
+    a.tl_step_host(dt, dx, dy, nf);
+  });
+}
+
+int adj_ad(wfe_ctx* ctx, double dt, const double* const* yb, double* const* xb, size_t n,
+           int nf, const char* who) {
+  if (int e = need_ctx(ctx)) return e;
+  return guarded([&] {
+    if (!yb || !xb) throw wfe::ArgError(std::string(who) + ": null field arrays");
+    for (int q = 0; q < nf; ++q)
+      if (!yb[q] || !xb[q]) throw wfe::ArgError(std::string(who) + ": null field array");
+    wfe::AdjointBase& a = adjoint_of(ctx);
+    check_n3(ctx, n);
+    a.ad_step_host(dt, yb, xb, nf);
+  });
+}
+
+int adj_gradient(wfe_ctx* ctx, double dt, int nsteps, int ckpt_every, const double* wgt2d,
+                 size_t n2, double* J, double* const* grad, size_t n, int nf, const char* who) {
+  if (int e = need_ctx(ctx)) return e;
+  return guarded([&] {
+    if (!wgt2d || !J || !grad) throw wfe::ArgError(std::string(who) + ": null argument");
+    for (int q = 0; q < nf; ++q)
+      if (!grad[q]) throw wfe::ArgError(std::string(who) + ": null gradient array");
+    wfe::AdjointBase& a = adjoint_of(ctx);
+    check_n3(ctx, n);
+    check_n2(ctx, n2);
+    *J = a.gradient(dt, nsteps, ckpt_every, wgt2d, grad, nf);
+  });
 }
 
 }  // namespace
@@ -2387,41 +2494,33 @@ int wfe_step(wfe_ctx* ctx, double dt, double t) {
 
 // ------------------------------------------------------------ adjoint
 int wfe_tl_step(wfe_ctx* ctx, double dt, const double* const* dx, double* const* dy, size_t n) {
-  if (int e = need_ctx(ctx)) return e;
-  return guarded([&] {
-    if (!dx || !dy) throw wfe::ArgError("wfe_tl_step: null field arrays");
-    for (int q = 0; q < 5; ++q)
-      if (!dx[q] || !dy[q]) throw wfe::ArgError("wfe_tl_step: null field array");
-    wfe::AdjointBase& a = adjoint_of(ctx);
-    check_n3(ctx, n);
-    a.tl_step_host(dt, dx, dy);
-  });
+  return adj_tl(ctx, dt, dx, dy, n, 5, "wfe_tl_step");
 }
 
 int wfe_ad_step(wfe_ctx* ctx, double dt, const double* const* yb, double* const* xb, size_t n) {
-  if (int e = need_ctx(ctx)) return e;
-  return guarded([&] {
-    if (!yb || !xb) throw wfe::ArgError("wfe_ad_step: null field arrays");
-    for (int q = 0; q < 5; ++q)
-      if (!yb[q] || !xb[q]) throw wfe::ArgError("wfe_ad_step: null field array");
-    wfe::AdjointBase& a = adjoint_of(ctx);
-    check_n3(ctx, n);
-    a.ad_step_host(dt, yb, xb);
-  });
+  return adj_ad(ctx, dt, yb, xb, n, 5, "wfe_ad_step");
 }
 
 int wfe_adj_gradient(wfe_ctx* ctx, double dt, int nsteps, int ckpt_every, const double* wgt2d,
                      size_t n2, double* J, double* const* grad, size_t n) {
-  if (int e = need_ctx(ctx)) return e;
-  return guarded([&] {
-    if (!wgt2d || !J || !grad) throw wfe::ArgError("wfe_adj_gradient: null argument");
-    for (int q = 0; q < 5; ++q)
-      if (!grad[q]) throw wfe::ArgError("wfe_adj_gradient: null gradient array");
-    wfe::AdjointBase& a = adjoint_of(ctx);
-    check_n3(ctx, n);
-    check_n2(ctx, n2);
-    *J = a.gradient(dt, nsteps, ckpt_every, wgt2d, grad);
-  });
+  return adj_gradient(ctx, dt, nsteps, ckpt_every, wgt2d, n2, J, grad, n, 5, "wfe_adj_gradient");
+}
+
+int wfe_tl_step_q(wfe_ctx* ctx, double dt, const double* const* dx, double* const* dy,
+                  size_t n) {
+  return adj_tl(ctx, dt, dx, dy, n, 6, "wfe_tl_step_q");
+}
+
+int wfe_ad_step_q(wfe_ctx* ctx, double dt, const double* const* yb, double* const* xb,
+                  size_t n) {
+  return adj_ad(ctx, dt, yb, xb, n, 6, "wfe_ad_step_q");
+}
+
+int wfe_adj_gradient_q(wfe_ctx* ctx, double dt, int nsteps, int ckpt_every,
+                       const double* wgt2d, size_t n2, double* J, double* const* grad,
+                       size_t n) {
+  return adj_gradient(ctx, dt, nsteps, ckpt_every, wgt2d, n2, J, grad, n, 6,
+                      "wfe_adj_gradient_q");
 }
 
 // ------------------------------------------------------------ ensembles
