@@ -140,12 +140,13 @@ int wfe_set_physics(wfe_ctx* ctx, const wfe_sfc* sfc, size_t n2);
 
 /* ---- tangent-linear and adjoint of the large step ----
  * No counterpart in the reference. Scope: FP64 or mixed contexts with
- * periodic lateral boundaries, flat terrain, no Coriolis, no diffusion, no w
- * damping, no boundary relaxation and no microphysics (WFE_E_CONFIG
- * otherwise); with moisture on, q_v is linearised (the _q entries); nz <= 63,
- * acoustic_ns <= 16. In a mixed context the linearisation is about the FP32
- * trajectory the context computes (FP32 production); perturbations, adjoint
- * variables and J are FP64 in both. Fields are (u, v, w, theta', pi') as five FP64 host
+ * periodic lateral boundaries, flat terrain, no w damping, no boundary
+ * relaxation and no microphysics (WFE_E_CONFIG otherwise); Coriolis and
+ * constant-K diffusion are included; with moisture on, q_v is linearised
+ * (the _q entries); nz <= 63, acoustic_ns <= 16. In a mixed context the
+ * linearisation is about the FP32 trajectory the context computes (FP32
+ * production); perturbations, adjoint variables and J are FP64 in both.
+ * Fields are (u, v, w, theta', pi') as five FP64 host
  * arrays in the padded reference layout (n = padded 3-D element count);
  * ghost values of inputs are ignored (they are filled from the interior).
  * The upwind branch of every interface value is taken from the trajectory.

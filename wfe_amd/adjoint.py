@@ -1,9 +1,10 @@
-"""Tangent-linear and adjoint of the dry large step through the C ABI, and a
+"""Tangent-linear and adjoint of the large step through the C ABI, and a
 torch.autograd.Function for the lowest-level kinetic-energy objective.
 
 Scope (enforced by the library): FP64 or mixed contexts with periodic
-lateral boundaries, flat terrain, no Coriolis, no diffusion, no w damping,
-no boundary relaxation and no microphysics; with moisture on, q_v is a
+lateral boundaries, flat terrain, no w damping, no boundary relaxation and
+no microphysics; Coriolis and constant-K diffusion are included; with
+moisture on, q_v is a
 linearised field (flux-form transport without the positive-definite
 limiter, q_c = q_r = 0). Fields are (u, v, w, thp, pip, qv) in the padded
 layout of the case's grid; qv is zero on output for a dry case. In a mixed context the linearisation is

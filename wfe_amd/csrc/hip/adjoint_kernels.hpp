@@ -5,8 +5,9 @@
 // implicit w-pi' column solve. They linearise the forward kernels around a
 // stored trajectory, with the upwind branch of every interface value frozen
 // at the trajectory's mass flux. Scope: periodic lateral boundaries, flat
-// terrain (J = 1, no cross-metric terms), map factor 1, no Coriolis, no
-// diffusion, no vertical-Courant damping. With moisture on (no
+// terrain (J = 1, no cross-metric terms), no vertical-Courant damping;
+// Coriolis (any f(x, y)) and constant-K diffusion are linear and enter the
+// TL as they are, the AD as their transposes. With moisture on (no
 // microphysics, flux-form transport without the positive-definite limiter,
 // q_c = q_r = 0), q_v is a sixth linearised field: its transport and stage
 // update, the moist buoyancy in T_w and the virtual-temperature factor
@@ -185,7 +186,28 @@ struct TanTend {  // tendency perturbations (TL out) or adjoints (AD in)
   double *qn, *qout;
   double dts;
   int moist;
+  double K;  // diffusion coefficient (0: off)
 };
+
+// K times the 7-point Laplacian of the perturbation f at c (k_diffuse,
+// flat terrain: 1/(dzeta J)^2 = rdz^2), and its transpose: T scattered onto
+// the seven points.
+__device__ __forceinline__ double tl_lap(const Grid& g, const double* f, int c, double rdz2) {
+  const double cc = f[c];
+  return (f[c + 1] - 2.0 * cc + f[c - 1]) * (g.rdx * g.rdx) +
+         (f[c + g.NX] - 2.0 * cc + f[c - g.NX]) * (g.rdy * g.rdy) +
+         (f[c + g.plane] - 2.0 * cc + f[c - g.plane]) * rdz2;
+}
+__device__ __forceinline__ void ad_lap(const Grid& g, double* fb, int c, double rdz2, double T) {
+  const double ax = T * (g.rdx * g.rdx), ay = T * (g.rdy * g.rdy), az = T * rdz2;
+  atomicAdd(&fb[c], -2.0 * (ax + ay + az));
+  atomicAdd(&fb[c + 1], ax);
+  atomicAdd(&fb[c - 1], ax);
+  atomicAdd(&fb[c + g.NX], ay);
+  atomicAdd(&fb[c - g.NX], ay);
+  atomicAdd(&fb[c + g.plane], az);
+  atomicAdd(&fb[c - g.plane], az);
+}
 
 template <class S>
 __global__ void k_tl_tend(Grid g, Base<double> b, Metric<double> m, TanTraj<S> X, TanPert D,
@@ -215,7 +237,11 @@ __global__ void k_tl_tend(Grid g, Base<double> b, Metric<double> m, TanTraj<S> X
                        (tl_face(X.u, D.u, c + PL, PL, fzT, dfzT) - tl_face(X.u, D.u, c, PL, fzB, dfzB)) / dzc;
     const double dv = 0.5 * (X.div[c - 1] + X.div[c]), ddv = 0.5 * (D.div[c - 1] + D.div[c]);
     const double ru = 0.5 * (b.rhob[c - 1] * m.jac[c2 - 1] + rt);
-    out.tu[c] = (-dfd + D.u[c] * dv + X.u[c] * ddv) / ru - m.ray_c[kk] * D.u[c];
+    const double fc = 0.5 * (m.fcor[c2 - 1] + m.fcor[c2]);
+    const double dvavg = 0.25 * (D.v[c - 1] + D.v[c - 1 + NX] + D.v[c] + D.v[c + NX]);
+    double t = (-dfd + D.u[c] * dv + X.u[c] * ddv) / ru + fc * dvavg - m.ray_c[kk] * D.u[c];
+    if (out.K > 0.0) t += out.K * tl_lap(g, D.u, c, m.rdzc[kk] * m.rdzc[kk]);
+    out.tu[c] = t;
   }
   {  // v
     const double fcB = 0.5 * (X.mfy[c - NX] + X.mfy[c]), dfcB = 0.5 * (D.mfy[c - NX] + D.mfy[c]);
@@ -231,7 +257,11 @@ __global__ void k_tl_tend(Grid g, Base<double> b, Metric<double> m, TanTraj<S> X
                        (tl_face(X.v, D.v, c + PL, PL, fzT, dfzT) - tl_face(X.v, D.v, c, PL, fzB, dfzB)) / dzc;
     const double dv = 0.5 * (X.div[c - NX] + X.div[c]), ddv = 0.5 * (D.div[c - NX] + D.div[c]);
     const double rv = 0.5 * (b.rhob[c - NX] * m.jac[c2 - NX] + rt);
-    out.tv[c] = (-dfd + D.v[c] * dv + X.v[c] * ddv) / rv - m.ray_c[kk] * D.v[c];
+    const double fc = 0.5 * (m.fcor[c2 - NX] + m.fcor[c2]);
+    const double duavg = 0.25 * (D.u[c - NX] + D.u[c + 1 - NX] + D.u[c] + D.u[c + 1]);
+    double t = (-dfd + D.v[c] * dv + X.v[c] * ddv) / rv - fc * duavg - m.ray_c[kk] * D.v[c];
+    if (out.K > 0.0) t += out.K * tl_lap(g, D.v, c, m.rdzc[kk] * m.rdzc[kk]);
+    out.tv[c] = t;
   }
   if (k >= 1) {  // w
     const double dzw = m.dzeta_w[kk];
@@ -249,6 +279,7 @@ __global__ void k_tl_tend(Grid g, Base<double> b, Metric<double> m, TanTraj<S> X
     const double dv = 0.5 * (X.div[c - PL] + X.div[c]), ddv = 0.5 * (D.div[c - PL] + D.div[c]);
     const double rw = b.rhobw[c] * m.jac[c2];
     double t = (-dfd + D.w[c] * dv + X.w[c] * ddv) / rw - m.ray_w[kk] * D.w[c];
+    if (out.K > 0.0) t += out.K * tl_lap(g, D.w, c, m.rdzw[kk] * m.rdzw[kk]);
     // moist buoyancy g 0.61 (q_v at the w level - base) from time-n q_v
     if (out.moist) t += kGrav * kEps61 * 0.5 * (out.qn[c - PL] + out.qn[c]);
     out.tw[c] = t;
@@ -261,7 +292,9 @@ __global__ void k_tl_tend(Grid g, Base<double> b, Metric<double> m, TanTraj<S> X
          tl_face(X.th, D.th, c, NX, X.mfy[c], D.mfy[c])) / dy +
         (tl_face(X.th, D.th, c + PL, PL, X.mfz[c + PL], D.mfz[c + PL]) -
          tl_face(X.th, D.th, c, PL, X.mfz[c], D.mfz[c])) / dzc;
-    out.tth[c] = (-dfd + D.th[c] * X.div[c] + X.th[c] * D.div[c]) / rt - m.ray_c[kk] * D.th[c];
+    double t = (-dfd + D.th[c] * X.div[c] + X.th[c] * D.div[c]) / rt - m.ray_c[kk] * D.th[c];
+    if (out.K > 0.0) t += out.K * tl_lap(g, D.th, c, m.rdzc[kk] * m.rdzc[kk]);
+    out.tth[c] = t;
   }
   if (out.moist) {  // q_v: flux-form transport, then the stage update
     const double dfd =
@@ -299,6 +332,12 @@ __global__ void k_ad_tend(Grid g, Base<double> b, Metric<double> m, TanTraj<S> X
     const double T = in.tu[c];
     if (T != 0.0) {
       const double ru = 0.5 * (b.rhob[c - 1] * m.jac[c2 - 1] + rt);
+      const double cb = 0.25 * 0.5 * (m.fcor[c2 - 1] + m.fcor[c2]) * T;  // Coriolis
+      atomicAdd(&A.v[c - 1], cb);
+      atomicAdd(&A.v[c - 1 + NX], cb);
+      atomicAdd(&A.v[c], cb);
+      atomicAdd(&A.v[c + NX], cb);
+      if (in.K > 0.0) ad_lap(g, A.u, c, m.rdzc[kk] * m.rdzc[kk], in.K * T);
       const double a = T / ru;
       const double fdb = -a;
       const double dv = 0.5 * (X.div[c - 1] + X.div[c]);
@@ -324,6 +363,12 @@ __global__ void k_ad_tend(Grid g, Base<double> b, Metric<double> m, TanTraj<S> X
     const double T = in.tv[c];
     if (T != 0.0) {
       const double rv = 0.5 * (b.rhob[c - NX] * m.jac[c2 - NX] + rt);
+      const double cb = -0.25 * 0.5 * (m.fcor[c2 - NX] + m.fcor[c2]) * T;  // Coriolis
+      atomicAdd(&A.u[c - NX], cb);
+      atomicAdd(&A.u[c + 1 - NX], cb);
+      atomicAdd(&A.u[c], cb);
+      atomicAdd(&A.u[c + 1], cb);
+      if (in.K > 0.0) ad_lap(g, A.v, c, m.rdzc[kk] * m.rdzc[kk], in.K * T);
       const double a = T / rv;
       const double fdb = -a;
       const double dv = 0.5 * (X.div[c - NX] + X.div[c]);
@@ -350,6 +395,7 @@ __global__ void k_ad_tend(Grid g, Base<double> b, Metric<double> m, TanTraj<S> X
     if (T != 0.0) {
       const double dzw = m.dzeta_w[kk];
       const double rw = b.rhobw[c] * m.jac[c2];
+      if (in.K > 0.0) ad_lap(g, A.w, c, m.rdzw[kk] * m.rdzw[kk], in.K * T);
       if (in.moist) {
         const double qb = kGrav * kEps61 * 0.5 * T;
         atomicAdd(&in.qn[c - PL], qb);
@@ -379,6 +425,7 @@ __global__ void k_ad_tend(Grid g, Base<double> b, Metric<double> m, TanTraj<S> X
   {  // theta'
     const double T = in.tth[c];
     if (T != 0.0) {
+      if (in.K > 0.0) ad_lap(g, A.th, c, m.rdzc[kk] * m.rdzc[kk], in.K * T);
       const double a = T / rt;
       const double fdb = -a;
       atomicAdd(&A.th[c], a * X.div[c] - m.ray_c[kk] * T);

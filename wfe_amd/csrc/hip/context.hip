@@ -1823,18 +1823,12 @@ class Adjoint final : public AdjointBase {
     else if (hp.moisture && hp.pd_moist) why = "positive-definite moisture transport is on";
     else if (hp.bc_x_open || hp.bc_y_open) why = "lateral boundaries are not periodic";
     else if (F.p_.terrain) why = "terrain slopes are non-zero";
-    else if (hp.diff_K != 0) why = "diffusion is on";
+    else if (hp.diff_K < 0) why = "the diffusion coefficient is negative";
     else if (hp.w_damping) why = "w damping is on";
     else if (F.bdy_active_) why = "boundary relaxation is on";
     else if (F.phys_on_) why = "the surface physics is on";
     else if (F.g_.nz + 1 > 64) why = "more than 63 levels";
     else if (std::max(1, hp.acoustic_ns) > kMaxSub) why = "more than 16 acoustic substeps";
-    if (why.empty()) {
-      std::vector<T> fc(F.n2_);
-      WFE_HIP(hipMemcpy(fc.data(), F.met_.fcor, F.n2_ * sizeof(T), hipMemcpyDeviceToHost));
-      for (T x : fc)
-        if (x != 0) why = "Coriolis is on";
-    }
     if (why.empty() && hp.moisture) {
       // q_c and q_r are not linearised: without microphysics they stay
       // zero when they start at zero
@@ -2177,7 +2171,7 @@ class Adjoint final : public AdjointBase {
   dev::TanTend tend(int m, double dts) const {
     const Set4& t = dT_[m];
     return dev::TanTend{t.a, t.b, t.c, t.d, dest_[0].q, dsub_[m][nsub_[m] - 1].q, dts,
-                        moist_ ? 1 : 0};
+                        moist_ ? 1 : 0, F.hp_.diff_K};
   }
   dim3 grid3() const { return dim3(cdiv(F.g_.nx, 64), cdiv(F.g_.ny, 4), (unsigned)F.g_.nz); }
   dim3 grid_uv() const {
