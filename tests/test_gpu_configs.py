@@ -7,13 +7,16 @@ reader and boundary manager are inside the comparison too.
   cfg 2  moist supercell 120 x 120 x 40, mixed, all 1200 steps: finite,
          |w| below the guard, moisture positive; FP64 vs mixed error;
   cfg 3  regional 330 x 220 x 45, mixed, all 1440 steps (24 h) with boundary
-         files every 3 h: finite, |w| guard, PD positivity; a small regional
+         files every 3 h: finite, |w| guard, PD positivity, and within 1e-4
+         of the FP64 path on u, v, w, theta' after 24 h; a small regional
          case crossing two boundary files against the oracle;
   cfg 4  an uneven ensemble of seeded regional members at two grid sizes,
          advanced together, each member against its own oracle;
-  cfg 5  adjoint at the 120 x 120 x 40 supercell shape (dot product, finite
-         differences) and a reference gradient from finite differences of
-         the FP64 oracle on a small grid.
+  cfg 5  adjoint at the 120 x 120 x 40 supercell shape, dry and with q_v
+         (dot products in FP64 and about the FP32 trajectory, finite
+         differences, FP32 production against the FP64 check), and reference
+         gradients from finite differences of the FP64 oracle on a small grid
+         (dry, moist, moist with Coriolis and diffusion).
 """
 from __future__ import annotations
 
