@@ -140,8 +140,8 @@ int wfe_set_physics(wfe_ctx* ctx, const wfe_sfc* sfc, size_t n2);
 
 /* ---- tangent-linear and adjoint of the large step ----
  * No counterpart in the reference. Scope: FP64 or mixed contexts with
- * periodic lateral boundaries, flat terrain, no w damping, no boundary
- * relaxation and no microphysics (WFE_E_CONFIG otherwise); Coriolis and
+ * periodic lateral boundaries, no w damping, no boundary relaxation and no
+ * microphysics (WFE_E_CONFIG otherwise); terrain, map factors, Coriolis and
  * constant-K diffusion are included; with moisture on, q_v is linearised
  * (the _q entries); nz <= 63, acoustic_ns <= 16. In a mixed context the
  * linearisation is about the FP32 trajectory the context computes (FP32

@@ -473,3 +473,87 @@ def test_coriolis_and_diffusion_dot_product_and_finite_differences(moist):
         err = np.linalg.norm(fd - dy[f][mask]) / np.linalg.norm(dy[f][mask])
         print(f, err)
         assert err < 1e-6, (f, err)
+
+
+# ------------------------------------------------------ terrain, map factors
+TERRAIN = dict(terrain="gauss", mtn_h0=400, mtn_a=2500, mtn_xc=4800, mtn_yc=4000)
+
+
+def _dot_and_fd(c, gi, flds, fresh, seed, tol_fd=1e-7):
+    """Dot product of the TL and AD around gi's state, and the TL against
+    centred differences of the forward step (fresh() makes a new context of
+    the same model)."""
+    from wfe_amd.adjoint import ad_step, tl_step
+    g = c.grid
+    rng = np.random.default_rng(seed)
+    mask = interior_mask(g)
+    dx, yb = random6(g, rng), random6(g, rng)
+    dt = c.scalar("dt")
+    dy = tl_step(gi, dt, dx)
+    xb = ad_step(gi, dt, yb)
+    lhs = sum(float(np.dot(dy[f][mask], yb[f][mask])) for f in flds)
+    rhs = sum(float(np.dot(dx[f][mask], xb[f][mask])) for f in flds)
+    print("dot product", lhs, rhs, abs(lhs - rhs) / abs(lhs))
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+    x = {f: gi.get(f) for f in flds}
+    # Centred differences cross the kinks of the upwind selection (where a
+    # trajectory flux is within eps of zero) at the larger eps and lose
+    # digits to round-off at the smaller: the best of three is kept per field.
+    errs = {f: [] for f in flds}
+    for eps in (1e-5, 1e-6, 1e-7):
+        outs = []
+        for sgn in (1, -1):
+            gp = fresh()
+            for f in flds:
+                gp.set(f, x[f] + sgn * eps * dx[f])
+            gp.step(dt)
+            outs.append({f: gp.get(f) for f in flds})
+        for f in flds:
+            fd = (outs[0][f] - outs[1][f])[mask] / (2 * eps)
+            errs[f].append(np.linalg.norm(fd - dy[f][mask]) / np.linalg.norm(dy[f][mask]))
+    for f in flds:
+        print(f, [f"{e:.1e}" for e in errs[f]])
+        assert min(errs[f]) < tol_fd, (f, errs[f])
+
+
+@pytest.mark.parametrize("moist", [False, True], ids=["dry", "moist"])
+def test_terrain_dot_product_and_finite_differences(moist):
+    """A Gaussian hill (400 m, 2.5 km half-width) under periodic boundaries,
+    with Coriolis and diffusion: Jacobian, cross-metric mass flux, the
+    terrain pi* gradient, S_v and the surface w in the linearised step."""
+    from wfe_amd import Integrator
+    kw = dict(TERRAIN, coriolis_f=1e-3, diff_K=15.0)
+    c = moist_case(**kw) if moist else dry_case(**kw)
+    hx = c.array("hx")
+    assert np.abs(hx).max() > 0.05  # slopes of several percent
+    gi = make_moist(c, 41) if moist else make(c, 41)
+    flds = FIELDS6 if moist else FIELDS5
+    _dot_and_fd(c, gi, flds, lambda: Integrator.from_case(c, "fp64"), 141)
+
+
+def test_map_factors_dot_product_and_finite_differences():
+    """Non-uniform map factors (0.97-1.03) on the periodic hill: the map
+    factor scales the horizontal divergence D_h of the column's pi' equation
+    and the pressure gradient of the horizontal step, and nothing else."""
+    from wfe_amd import Integrator
+    from wfe_amd._lib import BASE_NAMES, METRIC_NAMES
+    c = dry_case(**TERRAIN)
+    g = c.grid
+    arrays = {}
+    for n in BASE_NAMES + METRIC_NAMES:
+        try:
+            arrays[n] = c.array(n)
+        except KeyError:
+            pass
+    jj, ii = np.meshgrid(np.arange(g.NY), np.arange(g.NX), indexing="ij")
+    arrays["mapf"] = (1.0 + 0.03 * np.sin(2 * np.pi * ii / g.nx) *
+                      np.cos(2 * np.pi * jj / g.ny)).ravel()
+
+    def fresh():
+        return Integrator.from_arrays(g, c.params, arrays, c.scalar("zt"), "fp64")
+
+    gi = fresh()
+    for f, a in perturbed_state(c, 42).items():
+        if f in FIELDS5:
+            gi.set(f, a)
+    _dot_and_fd(c, gi, FIELDS5, fresh, 142)

@@ -306,15 +306,18 @@ def test_cfg5_adjoint_at_the_supercell_shape():
     assert max(errs.values()) <= 1e-3, errs
 
 
-@pytest.mark.parametrize("moist,extra", [(False, {}), (True, {}),
-                                         (True, {"coriolis_f": 1e-3, "diff_K": 15.0})],
-                         ids=["dry", "moist", "moist-coriolis-diffusion"])
+@pytest.mark.parametrize("moist,extra", [
+    (False, {}), (True, {}), (True, {"coriolis_f": 1e-3, "diff_K": 15.0}),
+    (True, {"coriolis_f": 1e-3, "diff_K": 15.0, "terrain": "gauss", "mtn_h0": 400,
+            "mtn_a": 2500, "mtn_xc": 4800, "mtn_yc": 4000})],
+    ids=["dry", "moist", "moist-coriolis-diffusion", "moist-terrain"])
 def test_adjoint_gradient_against_finite_differences_of_the_oracle(moist, extra):
     """Reference gradient: the objective evaluated by the FP64 oracle (its own
     case assembly and step) at x0 +- eps d, against the hand-written adjoint's
     gradient along d, on a small periodic grid; dry, on the WK82 moist
     sounding without microphysics with q_v in the direction (and along q_v
-    alone), and the moist case with Coriolis and diffusion."""
+    alone), the moist case with Coriolis and diffusion, and the same over a
+    Gaussian hill."""
     from wfe_amd import Case, Integrator
     from wfe_amd.adjoint import FIELDS, FIELDS5, box_weights, ke_gradient
     over = dict(nx=24, ny=20, nz=16, dx=400, dy=400, dz=300, bubble_xc=4800, bubble_yc=4000,

@@ -2,8 +2,9 @@
 torch.autograd.Function for the lowest-level kinetic-energy objective.
 
 Scope (enforced by the library): FP64 or mixed contexts with periodic
-lateral boundaries, flat terrain, no w damping, no boundary relaxation and
-no microphysics; Coriolis and constant-K diffusion are included; with
+lateral boundaries, no w damping, no boundary relaxation and no
+microphysics; terrain, map factors, Coriolis and constant-K diffusion are
+included; with
 moisture on, q_v is a
 linearised field (flux-form transport without the positive-definite
 limiter, q_c = q_r = 0). Fields are (u, v, w, thp, pip, qv) in the padded

@@ -4,10 +4,13 @@
 // the Rayleigh layer, the horizontal acoustic step and the vertically
 // implicit w-pi' column solve. They linearise the forward kernels around a
 // stored trajectory, with the upwind branch of every interface value frozen
-// at the trajectory's mass flux. Scope: periodic lateral boundaries, flat
-// terrain (J = 1, no cross-metric terms), no vertical-Courant damping;
-// Coriolis (any f(x, y)) and constant-K diffusion are linear and enter the
-// TL as they are, the AD as their transposes. With moisture on (no
+// at the trajectory's mass flux. Scope: periodic lateral boundaries, no
+// vertical-Courant damping. Terrain enters through the Jacobian J, the
+// cross-metric vertical mass flux, the terrain pressure-gradient term of the
+// horizontal acoustic step, S_v in the column's pi' equation and the
+// diagnostic surface w(0) = u h_x + v h_y. Coriolis (any f(x, y)) and
+// constant-K diffusion are linear and enter the TL as they are, the AD as
+// their transposes. With moisture on (no
 // microphysics, flux-form transport without the positive-definite limiter,
 // q_c = q_r = 0), q_v is a sixth linearised field: its transport and stage
 // update, the moist buoyancy in T_w and the virtual-temperature factor
@@ -151,7 +154,9 @@ __global__ void k_ad_massflux(Grid g, Metric<double> m, double* mfxb, double* mf
 }
 
 // Fluxes -> velocities, after k_ad_massflux has gathered every divergence
-// contribution (one thread per flux point; flat terrain, J = 1).
+// contribution (one thread per flux point). The vertical flux
+// rho_w (w - (u^xz h_x + v^yz h_y)(1 - zeta_w / zt)) scatters onto the four
+// u and four v faces around the w point, so u and v accumulate atomically.
 __global__ void k_ad_massflux_uvw(Grid g, Base<double> b, Metric<double> m, double* ub,
                                   double* vb, double* wb, const double* mfxb,
                                   const double* mfyb, const double* mfzb) {
@@ -164,9 +169,25 @@ __global__ void k_ad_massflux_uvw(Grid g, Base<double> b, Metric<double> m, doub
   const double rj = b.rhob[c] * m.jac[c2];
   const double rjx = 0.5 * (b.rhob[c - 1] * m.jac[c2 - 1] + rj);
   const double rjy = 0.5 * (b.rhob[c - g.NX] * m.jac[c2 - g.NX] + rj);
-  ub[c] += rjx * mfxb[c];
-  vb[c] += rjy * mfyb[c];
-  if (k >= 1 && i <= g.nx + 1 && j <= g.ny + 1) wb[c] += b.rhobw[c] * mfzb[c];
+  atomicAdd(&ub[c], rjx * mfxb[c]);
+  atomicAdd(&vb[c], rjy * mfyb[c]);
+  if (k >= 1 && i <= g.nx + 1 && j <= g.ny + 1) {
+    const double fb = mfzb[c];
+    wb[c] += b.rhobw[c] * fb;
+    const double hx = m.hx[c2], hy = m.hy[c2];
+    if (fb != 0.0 && (hx != 0.0 || hy != 0.0)) {
+      const double cb = -0.25 * b.rhobw[c] * fb * (1.0 - m.zeta_w[k + g.ng] * m.rzt);
+      const int cm = c - g.plane;
+      atomicAdd(&ub[cm], cb * hx);
+      atomicAdd(&ub[cm + 1], cb * hx);
+      atomicAdd(&ub[c], cb * hx);
+      atomicAdd(&ub[c + 1], cb * hx);
+      atomicAdd(&vb[cm], cb * hy);
+      atomicAdd(&vb[cm + g.NX], cb * hy);
+      atomicAdd(&vb[c], cb * hy);
+      atomicAdd(&vb[c + g.NX], cb * hy);
+    }
+  }
 }
 
 // ------------------------------------------------------------ tendencies
@@ -223,6 +244,7 @@ __global__ void k_tl_tend(Grid g, Base<double> b, Metric<double> m, TanTraj<S> X
   const double dx = g.dx, dy = g.dy;
   const double dzc = m.dzeta_c[kk];
   const double rt = b.rhob[c] * m.jac[c2];
+  const double rj2 = 1.0 / (m.jac[c2] * m.jac[c2]);  // diffusion: 1 / (dzeta J)^2
   {  // u
     const double fcL = 0.5 * (X.mfx[c - 1] + X.mfx[c]), dfcL = 0.5 * (D.mfx[c - 1] + D.mfx[c]);
     const double fcR = 0.5 * (X.mfx[c] + X.mfx[c + 1]), dfcR = 0.5 * (D.mfx[c] + D.mfx[c + 1]);
@@ -240,7 +262,7 @@ __global__ void k_tl_tend(Grid g, Base<double> b, Metric<double> m, TanTraj<S> X
     const double fc = 0.5 * (m.fcor[c2 - 1] + m.fcor[c2]);
     const double dvavg = 0.25 * (D.v[c - 1] + D.v[c - 1 + NX] + D.v[c] + D.v[c + NX]);
     double t = (-dfd + D.u[c] * dv + X.u[c] * ddv) / ru + fc * dvavg - m.ray_c[kk] * D.u[c];
-    if (out.K > 0.0) t += out.K * tl_lap(g, D.u, c, m.rdzc[kk] * m.rdzc[kk]);
+    if (out.K > 0.0) t += out.K * tl_lap(g, D.u, c, m.rdzc[kk] * m.rdzc[kk] * rj2);
     out.tu[c] = t;
   }
   {  // v
@@ -260,7 +282,7 @@ __global__ void k_tl_tend(Grid g, Base<double> b, Metric<double> m, TanTraj<S> X
     const double fc = 0.5 * (m.fcor[c2 - NX] + m.fcor[c2]);
     const double duavg = 0.25 * (D.u[c - NX] + D.u[c + 1 - NX] + D.u[c] + D.u[c + 1]);
     double t = (-dfd + D.v[c] * dv + X.v[c] * ddv) / rv - fc * duavg - m.ray_c[kk] * D.v[c];
-    if (out.K > 0.0) t += out.K * tl_lap(g, D.v, c, m.rdzc[kk] * m.rdzc[kk]);
+    if (out.K > 0.0) t += out.K * tl_lap(g, D.v, c, m.rdzc[kk] * m.rdzc[kk] * rj2);
     out.tv[c] = t;
   }
   if (k >= 1) {  // w
@@ -279,7 +301,7 @@ __global__ void k_tl_tend(Grid g, Base<double> b, Metric<double> m, TanTraj<S> X
     const double dv = 0.5 * (X.div[c - PL] + X.div[c]), ddv = 0.5 * (D.div[c - PL] + D.div[c]);
     const double rw = b.rhobw[c] * m.jac[c2];
     double t = (-dfd + D.w[c] * dv + X.w[c] * ddv) / rw - m.ray_w[kk] * D.w[c];
-    if (out.K > 0.0) t += out.K * tl_lap(g, D.w, c, m.rdzw[kk] * m.rdzw[kk]);
+    if (out.K > 0.0) t += out.K * tl_lap(g, D.w, c, m.rdzw[kk] * m.rdzw[kk] * rj2);
     // moist buoyancy g 0.61 (q_v at the w level - base) from time-n q_v
     if (out.moist) t += kGrav * kEps61 * 0.5 * (out.qn[c - PL] + out.qn[c]);
     out.tw[c] = t;
@@ -293,7 +315,7 @@ __global__ void k_tl_tend(Grid g, Base<double> b, Metric<double> m, TanTraj<S> X
         (tl_face(X.th, D.th, c + PL, PL, X.mfz[c + PL], D.mfz[c + PL]) -
          tl_face(X.th, D.th, c, PL, X.mfz[c], D.mfz[c])) / dzc;
     double t = (-dfd + D.th[c] * X.div[c] + X.th[c] * D.div[c]) / rt - m.ray_c[kk] * D.th[c];
-    if (out.K > 0.0) t += out.K * tl_lap(g, D.th, c, m.rdzc[kk] * m.rdzc[kk]);
+    if (out.K > 0.0) t += out.K * tl_lap(g, D.th, c, m.rdzc[kk] * m.rdzc[kk] * rj2);
     out.tth[c] = t;
   }
   if (out.moist) {  // q_v: flux-form transport, then the stage update
@@ -323,6 +345,7 @@ __global__ void k_ad_tend(Grid g, Base<double> b, Metric<double> m, TanTraj<S> X
   const double dx = g.dx, dy = g.dy;
   const double dzc = m.dzeta_c[kk];
   const double rt = b.rhob[c] * m.jac[c2];
+  const double rj2 = 1.0 / (m.jac[c2] * m.jac[c2]);
   // Adjoint of a face-averaged flux f = 0.5 (mf[a] + mf[b]).
   auto avg_b = [](double* mfb, int a, int bb, double fb) {
     atomicAdd(&mfb[a], 0.5 * fb);
@@ -337,7 +360,7 @@ __global__ void k_ad_tend(Grid g, Base<double> b, Metric<double> m, TanTraj<S> X
       atomicAdd(&A.v[c - 1 + NX], cb);
       atomicAdd(&A.v[c], cb);
       atomicAdd(&A.v[c + NX], cb);
-      if (in.K > 0.0) ad_lap(g, A.u, c, m.rdzc[kk] * m.rdzc[kk], in.K * T);
+      if (in.K > 0.0) ad_lap(g, A.u, c, m.rdzc[kk] * m.rdzc[kk] * rj2, in.K * T);
       const double a = T / ru;
       const double fdb = -a;
       const double dv = 0.5 * (X.div[c - 1] + X.div[c]);
@@ -368,7 +391,7 @@ __global__ void k_ad_tend(Grid g, Base<double> b, Metric<double> m, TanTraj<S> X
       atomicAdd(&A.u[c + 1 - NX], cb);
       atomicAdd(&A.u[c], cb);
       atomicAdd(&A.u[c + 1], cb);
-      if (in.K > 0.0) ad_lap(g, A.v, c, m.rdzc[kk] * m.rdzc[kk], in.K * T);
+      if (in.K > 0.0) ad_lap(g, A.v, c, m.rdzc[kk] * m.rdzc[kk] * rj2, in.K * T);
       const double a = T / rv;
       const double fdb = -a;
       const double dv = 0.5 * (X.div[c - NX] + X.div[c]);
@@ -395,7 +418,7 @@ __global__ void k_ad_tend(Grid g, Base<double> b, Metric<double> m, TanTraj<S> X
     if (T != 0.0) {
       const double dzw = m.dzeta_w[kk];
       const double rw = b.rhobw[c] * m.jac[c2];
-      if (in.K > 0.0) ad_lap(g, A.w, c, m.rdzw[kk] * m.rdzw[kk], in.K * T);
+      if (in.K > 0.0) ad_lap(g, A.w, c, m.rdzw[kk] * m.rdzw[kk] * rj2, in.K * T);
       if (in.moist) {
         const double qb = kGrav * kEps61 * 0.5 * T;
         atomicAdd(&in.qn[c - PL], qb);
@@ -425,7 +448,7 @@ __global__ void k_ad_tend(Grid g, Base<double> b, Metric<double> m, TanTraj<S> X
   {  // theta'
     const double T = in.tth[c];
     if (T != 0.0) {
-      if (in.K > 0.0) ad_lap(g, A.th, c, m.rdzc[kk] * m.rdzc[kk], in.K * T);
+      if (in.K > 0.0) ad_lap(g, A.th, c, m.rdzc[kk] * m.rdzc[kk] * rj2, in.K * T);
       const double a = T / rt;
       const double fdb = -a;
       atomicAdd(&A.th[c], a * X.div[c] - m.ray_c[kk] * T);
@@ -482,12 +505,18 @@ struct UVLin {
   __device__ bool inside(int ii, int jj) const {
     return (unsigned)ii < (unsigned)g.nx && (unsigned)jj < (unsigned)g.ny;
   }
-  __device__ int mapped(int ii, int jj, int k) const {
-    return g.id(map_lat(ii, g.nx, g.nx - 1, p.bcx), map_lat(jj, g.ny, g.ny - 1, p.bcy), k);
+  __device__ bool inside3(int ii, int jj, int kz) const {
+    return inside(ii, jj) && (unsigned)kz < (unsigned)g.nz;
+  }
+  // the interior point a stencil point outside reads pi' from (lateral map,
+  // vertical clamp), as UVDev::pstar
+  __device__ int mapped(int ii, int jj, int kz) const {
+    const int km = kz < 0 ? 0 : (kz > g.nz - 1 ? g.nz - 1 : kz);
+    return g.id(map_lat(ii, g.nx, g.nx - 1, p.bcx), map_lat(jj, g.ny, g.ny - 1, p.bcy), km);
   }
   __device__ double pstar(int ii, int jj, int k) const {
     const int c = g.id(ii, jj, k);
-    if (inside(ii, jj)) return X.pcur[c] + p.smdiv * (X.pcur[c] - X.pprev[c]);
+    if (inside3(ii, jj, k)) return X.pcur[c] + p.smdiv * (X.pcur[c] - X.pprev[c]);
     const double pc = X.pcur[mapped(ii, jj, k)];
     return pc + p.smdiv * (pc - X.pn[c]);
   }
@@ -502,7 +531,7 @@ struct UVLin {
   }
   __device__ double dpstar(const UVTan& D, int ii, int jj, int k) const {
     const int c = g.id(ii, jj, k);
-    if (inside(ii, jj)) return D.pcur[c] + p.smdiv * (D.pcur[c] - D.pprev[c]);
+    if (inside3(ii, jj, k)) return D.pcur[c] + p.smdiv * (D.pcur[c] - D.pprev[c]);
     const double pc = D.pcur[mapped(ii, jj, k)];
     return pc + p.smdiv * (pc - D.pn[c]);
   }
@@ -513,13 +542,42 @@ struct UVLin {
   }
   __device__ void ad_pstar(const UVTan& A, int ii, int jj, int k, double pb) const {
     const int c = g.id(ii, jj, k);
-    if (inside(ii, jj)) {
+    if (inside3(ii, jj, k)) {
       atomicAdd(&A.pcur[c], (1.0 + p.smdiv) * pb);
       atomicAdd(&A.pprev[c], -p.smdiv * pb);
     } else {
       atomicAdd(&A.pcur[mapped(ii, jj, k)], (1.0 + p.smdiv) * pb);
       atomicAdd(&A.pn[c], -p.smdiv * pb);
     }
+  }
+  // Terrain term of the pi* gradient at a u face (x) or v face (y): the
+  // coefficient (z_s / J_face) / (2 dzeta_2) of the centred vertical
+  // differences of pi* in the two cells beside the face (zero on flat
+  // ground), and the differences themselves.
+  __device__ double tcoef(const Metric<double>& m, int c2, int k, int dir) const {
+    const double h = dir == 0 ? m.hxu[c2] : m.hyv[c2];
+    if (h == 0.0) return 0.0;
+    const int kk = k + g.ng;
+    const double zs = h * (1.0 - m.zeta_c[kk] / m.zt);
+    const double dz2 = m.zeta_c[kk + 1] - m.zeta_c[kk - 1];
+    const int o = dir == 0 ? 1 : g.NX;
+    const double jf = 0.5 * (m.jac[c2 - o] + m.jac[c2]);
+    return (zs / jf) * 0.5 / dz2;
+  }
+  __device__ double tdiff(int i1, int j1, int i2, int j2, int k) const {
+    return (pstar(i1, j1, k + 1) - pstar(i1, j1, k - 1)) +
+           (pstar(i2, j2, k + 1) - pstar(i2, j2, k - 1));
+  }
+  __device__ double dtdiff(const UVTan& D, int i1, int j1, int i2, int j2, int k) const {
+    return (dpstar(D, i1, j1, k + 1) - dpstar(D, i1, j1, k - 1)) +
+           (dpstar(D, i2, j2, k + 1) - dpstar(D, i2, j2, k - 1));
+  }
+  __device__ void ad_tdiff(const UVTan& A, int i1, int j1, int i2, int j2, int k,
+                           double b) const {
+    ad_pstar(A, i1, j1, k + 1, b);
+    ad_pstar(A, i1, j1, k - 1, -b);
+    ad_pstar(A, i2, j2, k + 1, b);
+    ad_pstar(A, i2, j2, k - 1, -b);
   }
   __device__ void ad_thv(const UVTan& A, int ii, int jj, int k, double tb) const {
     const int c = g.id(ii, jj, k);
@@ -543,8 +601,13 @@ __global__ void k_tl_uv(Grid g, Params p, Base<double> b, Metric<double> m, UVTr
     const double mu = 0.5 * (m.mapf[c2 - 1] + m.mapf[c2]);
     const double thu = 0.5 * (L.thv(iu - 1, j, k) + L.thv(iu, j, k));
     const double dthu = 0.5 * (L.dthv(D, iu - 1, j, k) + L.dthv(D, iu, j, k));
-    const double grad = (L.pstar(iu, j, k) - L.pstar(iu - 1, j, k)) * g.rdx;
-    const double dgrad = (L.dpstar(D, iu, j, k) - L.dpstar(D, iu - 1, j, k)) * g.rdx;
+    double grad = (L.pstar(iu, j, k) - L.pstar(iu - 1, j, k)) * g.rdx;
+    double dgrad = (L.dpstar(D, iu, j, k) - L.dpstar(D, iu - 1, j, k)) * g.rdx;
+    const double tc = L.tcoef(m, c2, k, 0);
+    if (tc != 0.0) {
+      grad -= tc * L.tdiff(iu - 1, j, iu, j, k);
+      dgrad -= tc * L.dtdiff(D, iu - 1, j, iu, j, k);
+    }
     D.u_dst[c] = D.u_src[cu] + dtau * (-kCp * mu * (dthu * grad + thu * dgrad) + D.tu[cu]);
   }
   if (i < g.nx) {  // v face j (j = ny is the image of face 0)
@@ -553,8 +616,13 @@ __global__ void k_tl_uv(Grid g, Params p, Base<double> b, Metric<double> m, UVTr
     const double mv = 0.5 * (m.mapf[c2 - g.NX] + m.mapf[c2]);
     const double thv_ = 0.5 * (L.thv(i, jv - 1, k) + L.thv(i, jv, k));
     const double dthv_ = 0.5 * (L.dthv(D, i, jv - 1, k) + L.dthv(D, i, jv, k));
-    const double grad = (L.pstar(i, jv, k) - L.pstar(i, jv - 1, k)) * g.rdy;
-    const double dgrad = (L.dpstar(D, i, jv, k) - L.dpstar(D, i, jv - 1, k)) * g.rdy;
+    double grad = (L.pstar(i, jv, k) - L.pstar(i, jv - 1, k)) * g.rdy;
+    double dgrad = (L.dpstar(D, i, jv, k) - L.dpstar(D, i, jv - 1, k)) * g.rdy;
+    const double tc = L.tcoef(m, c2, k, 1);
+    if (tc != 0.0) {
+      grad -= tc * L.tdiff(i, jv - 1, i, jv, k);
+      dgrad -= tc * L.dtdiff(D, i, jv - 1, i, jv, k);
+    }
     D.v_dst[c] = D.v_src[cv] + dtau * (-kCp * mv * (dthv_ * grad + thv_ * dgrad) + D.tv[cv]);
   }
 }
@@ -575,7 +643,9 @@ __global__ void k_ad_uv(Grid g, Params p, Base<double> b, Metric<double> m, UVTr
       const int cu = g.id(iu, j, k), c2 = g.id2(iu, j);
       const double mu = 0.5 * (m.mapf[c2 - 1] + m.mapf[c2]);
       const double thu = 0.5 * (L.thv(iu - 1, j, k) + L.thv(iu, j, k));
-      const double grad = (L.pstar(iu, j, k) - L.pstar(iu - 1, j, k)) * g.rdx;
+      double grad = (L.pstar(iu, j, k) - L.pstar(iu - 1, j, k)) * g.rdx;
+      const double tc = L.tcoef(m, c2, k, 0);
+      if (tc != 0.0) grad -= tc * L.tdiff(iu - 1, j, iu, j, k);
       atomicAdd(&A.u_src[cu], ob);
       atomicAdd(&A.tu[cu], dtau * ob);
       const double x = -dtau * kCp * mu * ob;
@@ -585,6 +655,7 @@ __global__ void k_ad_uv(Grid g, Params p, Base<double> b, Metric<double> m, UVTr
       const double gb = x * thu * g.rdx;
       L.ad_pstar(A, iu, j, k, gb);
       L.ad_pstar(A, iu - 1, j, k, -gb);
+      if (tc != 0.0) L.ad_tdiff(A, iu - 1, j, iu, j, k, -tc * x * thu);
     }
   }
   if (i < g.nx) {
@@ -594,7 +665,9 @@ __global__ void k_ad_uv(Grid g, Params p, Base<double> b, Metric<double> m, UVTr
       const int cv = g.id(i, jv, k), c2 = g.id2(i, jv);
       const double mv = 0.5 * (m.mapf[c2 - g.NX] + m.mapf[c2]);
       const double thv_ = 0.5 * (L.thv(i, jv - 1, k) + L.thv(i, jv, k));
-      const double grad = (L.pstar(i, jv, k) - L.pstar(i, jv - 1, k)) * g.rdy;
+      double grad = (L.pstar(i, jv, k) - L.pstar(i, jv - 1, k)) * g.rdy;
+      const double tc = L.tcoef(m, c2, k, 1);
+      if (tc != 0.0) grad -= tc * L.tdiff(i, jv - 1, i, jv, k);
       atomicAdd(&A.v_src[cv], ob);
       atomicAdd(&A.tv[cv], dtau * ob);
       const double x = -dtau * kCp * mv * ob;
@@ -604,6 +677,7 @@ __global__ void k_ad_uv(Grid g, Params p, Base<double> b, Metric<double> m, UVTr
       const double gb = x * thv_ * g.rdy;
       L.ad_pstar(A, i, jv, k, gb);
       L.ad_pstar(A, i, jv - 1, k, -gb);
+      if (tc != 0.0) L.ad_tdiff(A, i, jv - 1, i, jv, k, -tc * x * thv_);
     }
   }
 }
@@ -643,13 +717,38 @@ struct WPCol {
   double A[MAXL], B[MAXL], C[MAXL], D[MAXL], cs[MAXL], ds[MAXL];
 };
 
+// Cross-metric vertical flux S_v = A_v (u^xz h_x + v^yz h_y)(1 - zeta_w / zt)
+// at w level k of a column (zero at k = 0, nz and on flat ground), from the
+// horizontal velocities after the substep's horizontal step; linear in
+// (u, v), so the same function gives its TL on perturbations.
+template <class V>
+__device__ __forceinline__ double sv_at(const Grid& g, const Metric<double>& m,
+                                        const Coef<double>& cf, const V& u, const V& v, int c0,
+                                        int k, double hx, double hy) {
+  if (k <= 0 || k >= g.nz || (hx == 0.0 && hy == 0.0)) return 0.0;
+  const int PL = g.plane, NX = g.NX, c = c0 + k * PL;
+  const double fac = 1.0 - m.zeta_w[k + g.ng] / m.zt;
+  const double uw = 0.25 * (u[c - PL] + u[c - PL + 1] + u[c] + u[c + 1]);
+  const double vw = 0.25 * (v[c - PL] + v[c - PL + NX] + v[c] + v[c + NX]);
+  return cf.aw[c] * (uw * hx + vw * hy) * fac;
+}
+
+// Surface w(0) = u^x h_x + v^y h_y of a column (diagnostic; zero on flat
+// ground).
+template <class V>
+__device__ __forceinline__ double w_surface(const Grid& g, const V& u, const V& v, int c0,
+                                            double hx, double hy) {
+  return 0.5 * (u[c0] + u[c0 + 1]) * hx + 0.5 * (v[c0] + v[c0 + g.NX]) * hy;
+}
+
 // Forward column: assembly, rows and the Thomas solve.
 template <class S, int MAXL>
 __device__ void wp_forward(const Grid& g, const Params& p, const Base<double>& b, const Metric<double>& m,
                            const Coef<double>& cf, const WPTraj<S>& X,
-                           int c0, double mc, double dtau, WPCol<MAXL>& s) {
+                           int c0, int c2d, double mc, double dtau, WPCol<MAXL>& s) {
   const int NX = g.NX, PL = g.plane, nz = g.nz;
   const double c1 = (1.0 + p.beta) * 0.5, c2 = (1.0 - p.beta) * 0.5;
+  const double J = m.jac[c2d], hx = m.hx[c2d], hy = m.hy[c2d];
   for (int k = 0; k <= nz; ++k) s.Av[k] = (k <= 0 || k >= nz) ? 0.0 : (double)cf.aw[c0 + k * PL];
   for (int k = 0; k < nz; ++k) {
     const int c = c0 + k * PL, kk = k + g.ng;
@@ -659,12 +758,15 @@ __device__ void wp_forward(const Grid& g, const Params& p, const Base<double>& b
                             ((double)cf.rtjy[c + NX] * (double)X.v[c + NX] -
                              (double)cf.rtjy[c] * (double)X.v[c]) * g.rdy);
     const double rdzc = m.rdzc[kk];
+    const double dsv = sv_at(g, m, cf, X.u, X.v, c0, k + 1, hx, hy) -
+                       sv_at(g, m, cf, X.u, X.v, c0, k, hx, hy);
     s.P[k] = X.pip[c] - dtau * Kt *
-                            (Dh + c2 * (s.Av[k + 1] * (double)X.w[c + PL] - s.Av[k] * (double)X.w[c]) *
-                                      rdzc);
+                            (Dh - dsv * rdzc +
+                             c2 * (s.Av[k + 1] * (double)X.w[c + PL] - s.Av[k] * (double)X.w[c]) *
+                                 rdzc);
     s.E[k] = dtau * Kt * c1 * rdzc;
     if (k >= 1) {
-      const double rJdzw = m.rdzw[kk];  // J = 1
+      const double rJdzw = m.rdzw[kk] / J;
       const double thvw = thvw_dry(b, X, c, PL) * thvw_fq(p, X, c, PL);
       s.G[k] = dtau * kCp * thvw * c1 * rJdzw;
       const double buoy = kGrav * 0.5 * ((double)X.th[c - PL] + (double)X.th[c]) / (double)b.thbw[c];
@@ -705,10 +807,12 @@ __global__ void __launch_bounds__(64) k_tl_wpi(Grid g, Params p, Base<double> b,
   const int c0 = g.id(i, j, 0);
   const double c1 = (1.0 + p.beta) * 0.5, c2 = (1.0 - p.beta) * 0.5;
   WPCol<MAXL> s;
-  const double mc = m.mapf[g.id2(i, j)];
-  wp_forward<S, MAXL>(g, p, b, m, cf, X, c0, mc, dtau, s);
+  const int c2d = g.id2(i, j);
+  const double mc = m.mapf[c2d], J = m.jac[c2d], hx = m.hx[c2d], hy = m.hy[c2d];
+  wp_forward<S, MAXL>(g, p, b, m, cf, X, c0, c2d, mc, dtau, s);
   for (int k = 1; k < nz; ++k) s.D[k] += dtau * (double)tw[c0 + k * PL];
   thomas<MAXL>(nz, s.A, s.B, s.C, s.D, s.cs, s.ds, s.W);
+  s.W[0] = w_surface(g, X.u, X.v, c0, hx, hy);
   // perturbations of P, G, R and the rows
   double dP[MAXL], dG[MAXL], dR[MAXL], dD[MAXL], dW[MAXL + 1];
   for (int k = 0; k < nz; ++k) {
@@ -718,11 +822,15 @@ __global__ void __launch_bounds__(64) k_tl_wpi(Grid g, Params p, Base<double> b,
                        ((cf.rtjx[c + 1] * D.u[c + 1] - cf.rtjx[c] * D.u[c]) * g.rdx +
                         (cf.rtjy[c + NX] * D.v[c + NX] - cf.rtjy[c] * D.v[c]) * g.rdy);
     const double rdzc = m.rdzc[kk];
-    dP[k] = D.pip[c] - dtau * Kt * (dDh + c2 * (s.Av[k + 1] * D.w[c + PL] - s.Av[k] * D.w[c]) * rdzc);
+    const double ddsv = sv_at(g, m, cf, D.u, D.v, c0, k + 1, hx, hy) -
+                        sv_at(g, m, cf, D.u, D.v, c0, k, hx, hy);
+    dP[k] = D.pip[c] - dtau * Kt *
+                           (dDh - ddsv * rdzc +
+                            c2 * (s.Av[k + 1] * D.w[c + PL] - s.Av[k] * D.w[c]) * rdzc);
     dG[k] = 0.0;
     dR[k] = 0.0;
     if (k >= 1) {
-      const double rJdzw = m.rdzw[kk];
+      const double rJdzw = m.rdzw[kk] / J;
       const double th_d = thvw_dry(b, X, c, PL), fq = thvw_fq(p, X, c, PL);
       const double thvw = th_d * fq;
       double dthvw = 0.5 * (D.th[c - PL] + D.th[c]) * fq;
@@ -744,6 +852,7 @@ __global__ void __launch_bounds__(64) k_tl_wpi(Grid g, Params p, Base<double> b,
   }
   double cs2[MAXL], ds2[MAXL];
   thomas<MAXL>(nz, s.A, s.B, s.C, dD, cs2, ds2, dW);
+  dW[0] = w_surface(g, D.u, D.v, c0, hx, hy);
   for (int k = 0; k < nz; ++k) {
     const int c = c0 + k * PL;
     D.pip_out[c] = dP[k] - s.E[k] * (s.Av[k + 1] * dW[k + 1] - s.Av[k] * dW[k]);
@@ -763,10 +872,12 @@ __global__ void __launch_bounds__(64) k_ad_wpi(Grid g, Params p, Base<double> b,
   const int c0 = g.id(i, j, 0);
   const double c1 = (1.0 + p.beta) * 0.5, c2 = (1.0 - p.beta) * 0.5;
   WPCol<MAXL> s;
-  const double mc = m.mapf[g.id2(i, j)];
-  wp_forward<S, MAXL>(g, p, b, m, cf, X, c0, mc, dtau, s);
+  const int c2d = g.id2(i, j);
+  const double mc = m.mapf[c2d], J = m.jac[c2d], hx = m.hx[c2d], hy = m.hy[c2d];
+  wp_forward<S, MAXL>(g, p, b, m, cf, X, c0, c2d, mc, dtau, s);
   for (int k = 1; k < nz; ++k) s.D[k] += dtau * (double)tw[c0 + k * PL];
   thomas<MAXL>(nz, s.A, s.B, s.C, s.D, s.cs, s.ds, s.W);
+  s.W[0] = w_surface(g, X.u, X.v, c0, hx, hy);
   // outputs -> W, P
   double Wb[MAXL + 1], Pb[MAXL], Gb[MAXL], Rb[MAXL];
   for (int k = 0; k <= nz; ++k) Wb[k] = 0.0;
@@ -800,6 +911,13 @@ __global__ void __launch_bounds__(64) k_ad_wpi(Grid g, Params p, Base<double> b,
     ds2[k] = (k == 1 ? Wb[k] : Wb[k] - sub * ds2[k - 1]) / mm;
   }
   for (int k = nz - 1; k >= 1; --k) lam[k] = ds2[k] - (k + 1 < nz ? cs2[k] * lam[k + 1] : 0.0);
+  // the surface w(0) = u^x h_x + v^y h_y
+  if (Wb[0] != 0.0 && (hx != 0.0 || hy != 0.0)) {
+    atomicAdd(&A.u[c0], 0.5 * hx * Wb[0]);
+    atomicAdd(&A.u[c0 + 1], 0.5 * hx * Wb[0]);
+    atomicAdd(&A.v[c0], 0.5 * hy * Wb[0]);
+    atomicAdd(&A.v[c0 + NX], 0.5 * hy * Wb[0]);
+  }
   // rows -> G, R, P (Db = lam; Ab = -lam W_{k-1}, Bb = -lam W_k, Cb = -lam W_{k+1})
   for (int k = 1; k < nz; ++k) {
     const double Db = lam[k];
@@ -812,21 +930,27 @@ __global__ void __launch_bounds__(64) k_ad_wpi(Grid g, Params p, Base<double> b,
     Pb[k - 1] += Db * s.G[k];
   }
   // G, R, P -> inputs
+  double Svb[MAXL + 1];
+  for (int k = 0; k <= nz; ++k) Svb[k] = 0.0;
   for (int k = 0; k < nz; ++k) {
     const int c = c0 + k * PL, kk = k + g.ng;
     const double Kt = cf.kt3[c];
     const double rdzc = m.rdzc[kk];
-    // P_k = pip - dtau Kt (Dh + c2 (Av_{k+1} w_{k+1} - Av_k w_k) rdzc)
+    // P_k = pip - dtau Kt (Dh - (Sv_{k+1} - Sv_k) rdzc + c2 (Av_{k+1} w_{k+1} - Av_k w_k) rdzc),
+    // Dh = m (...): the map factor multiplies Dh alone
     A.pip[c] += Pb[k];
-    const double Dhb = -dtau * Kt * Pb[k] * mc;
-    A.w[c + PL] += Dhb * c2 * s.Av[k + 1] * rdzc;
-    A.w[c] += -Dhb * c2 * s.Av[k] * rdzc;
+    const double Qb = -dtau * Kt * Pb[k];
+    const double Dhb = Qb * mc;
+    A.w[c + PL] += Qb * c2 * s.Av[k + 1] * rdzc;
+    A.w[c] += -Qb * c2 * s.Av[k] * rdzc;
+    Svb[k + 1] -= Qb * rdzc;
+    Svb[k] += Qb * rdzc;
     atomicAdd(&A.u[c + 1], Dhb * cf.rtjx[c + 1] * g.rdx);
     atomicAdd(&A.u[c], -Dhb * cf.rtjx[c] * g.rdx);
     atomicAdd(&A.v[c + NX], Dhb * cf.rtjy[c + NX] * g.rdy);
     atomicAdd(&A.v[c], -Dhb * cf.rtjy[c] * g.rdy);
     if (k >= 1) {
-      const double rJdzw = m.rdzw[kk];
+      const double rJdzw = m.rdzw[kk] / J;
       const double th_d = thvw_dry(b, X, c, PL), fq = thvw_fq(p, X, c, PL);
       const double thvw = th_d * fq;
       const double dp = X.pip[c] - X.pip[c - PL];
@@ -845,6 +969,22 @@ __global__ void __launch_bounds__(64) k_ad_wpi(Grid g, Params p, Base<double> b,
       const double dpb = -Rb[k] * dtau * kCp * thvw * c2 * rJdzw;
       A.pip[c] += dpb;
       A.pip[c - PL] -= dpb;
+    }
+  }
+  // S_v -> the u, v faces around each w level
+  if (hx != 0.0 || hy != 0.0) {
+    for (int k = 1; k < nz; ++k) {
+      if (Svb[k] == 0.0) continue;
+      const int c = c0 + k * PL;
+      const double f = 0.25 * Svb[k] * cf.aw[c] * (1.0 - m.zeta_w[k + g.ng] / m.zt);
+      atomicAdd(&A.u[c - PL], f * hx);
+      atomicAdd(&A.u[c - PL + 1], f * hx);
+      atomicAdd(&A.u[c], f * hx);
+      atomicAdd(&A.u[c + 1], f * hx);
+      atomicAdd(&A.v[c - PL], f * hy);
+      atomicAdd(&A.v[c - PL + NX], f * hy);
+      atomicAdd(&A.v[c], f * hy);
+      atomicAdd(&A.v[c + NX], f * hy);
     }
   }
 }

@@ -1822,7 +1822,6 @@ class Adjoint final : public AdjointBase {
     if (hp.moisture && hp.microphysics) why = "microphysics is on";
     else if (hp.moisture && hp.pd_moist) why = "positive-definite moisture transport is on";
     else if (hp.bc_x_open || hp.bc_y_open) why = "lateral boundaries are not periodic";
-    else if (F.p_.terrain) why = "terrain slopes are non-zero";
     else if (hp.diff_K < 0) why = "the diffusion coefficient is negative";
     else if (hp.w_damping) why = "w damping is on";
     else if (F.bdy_active_) why = "boundary relaxation is on";
