@@ -139,10 +139,12 @@ typedef struct {
 int wfe_set_physics(wfe_ctx* ctx, const wfe_sfc* sfc, size_t n2);
 
 /* ---- tangent-linear and adjoint of the large step ----
- * No counterpart in the reference. Scope: FP64 or mixed contexts with
- * periodic lateral boundaries, no w damping, no boundary relaxation and no
- * microphysics (WFE_E_CONFIG otherwise); terrain, map factors, Coriolis and
- * constant-K diffusion are included; with moisture on, q_v is linearised
+ * No counterpart in the reference. Scope: FP64 or mixed contexts without
+ * w damping, PD transport, microphysics or surface physics (WFE_E_CONFIG
+ * otherwise); terrain, map factors, open boundaries (radiation, branch
+ * frozen at the trajectory), Davies relaxation (targets and time fraction
+ * held at the context's current values over a gradient's steps), Coriolis
+ * and constant-K diffusion are included; with moisture on, q_v is linearised
  * (the _q entries); nz <= 63, acoustic_ns <= 16. In a mixed context the
  * linearisation is about the FP32 trajectory the context computes (FP32
  * production); perturbations, adjoint variables and J are FP64 in both.

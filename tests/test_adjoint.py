@@ -225,28 +225,66 @@ def test_scope_is_enforced():
     from wfe_amd import Integrator, WfeError
     from wfe_amd.adjoint import tl_step
     for precision in ("fp64", "mixed"):
-        c = dry_case(bc_x="open")
+        c = dry_case(w_damping="on")
         gi = Integrator.from_case(c, precision)
-        with pytest.raises(WfeError, match="lateral boundaries"):
+        with pytest.raises(WfeError, match="w damping"):
             tl_step(gi, c.scalar("dt"), {})
 
 
-def test_scope_is_checked_again_after_relaxation_is_switched_on():
-    """wfe_set_bdy_weights after a first adjoint call changes the model the
-    step advances; the adjoint entries must refuse it rather than keep
-    returning derivatives of the unrelaxed step."""
+def _relax(gi, c, seed, rim=4, w0=0.02, interior=0.0):
+    """Davies weights (w0 at the edge, falling linearly over rim cells, plus
+    a uniform interior nudging weight) and seeded targets for u, v, theta',
+    pi' and q_v, time fraction 0.3."""
     import ctypes as C
-    from wfe_amd import WfeError
     from wfe_amd._lib import lib
-    from wfe_amd.adjoint import tl_step
-    c = dry_case()
     g = c.grid
-    gi = make(c, 6)
-    tl_step(gi, c.scalar("dt"), {})  # in scope: sets the adjoint up
-    w = np.full(g.NX * g.NY, 1e-3)
-    assert lib().wfe_set_bdy_weights(gi._h, w.ctypes.data_as(C.POINTER(C.c_double)), w.size) == 0
-    with pytest.raises(WfeError, match=r"\(-3\).*boundary relaxation"):
-        tl_step(gi, c.scalar("dt"), {})
+    w = np.zeros((g.NY, g.NX))
+    for j in range(g.ny):
+        for i in range(g.nx):
+            d = min(i, j, g.nx - 1 - i, g.ny - 1 - j)
+            w[g.ng + j, g.ng + i] = (w0 * (rim - d) / rim if d < rim else 0.0) + interior
+    w = w.ravel()
+    DP = C.POINTER(C.c_double)
+    L = lib()
+    assert L.wfe_set_bdy_weights(gi._h, w.ctypes.data_as(DP), w.size) == 0
+    rng = np.random.default_rng(seed)
+    for slot in (0, 1):
+        for var, f in enumerate(("u", "v", "thp", "pip", "qv")):
+            t = gi.get(f) + rng.standard_normal(g.npts) * SCALE6[f]
+            assert L.wfe_set_bdy_target(gi._h, slot, var, t.ctypes.data_as(DP), t.size) == 0
+    assert L.wfe_set_bdy_tfrac(gi._h, 0.3) == 0
+    return w
+
+
+@pytest.mark.parametrize("moist,interior", [(False, 0.0), (True, 0.0), (True, 1e-3)],
+                         ids=["dry-rim", "moist-rim", "moist-nudged"])
+def test_relaxation_enters_the_linearisation(moist, interior):
+    """Relaxation switched on after the adjoint was set up (wfe_set_bdy_weights)
+    is part of the next linearised step: rim weights over open boundaries,
+    and with interior nudging everywhere."""
+    from wfe_amd import Integrator
+    from wfe_amd.adjoint import tl_step
+    kw = dict(TERRAIN, bc_x="open", bc_y="open")
+    c = moist_case(**kw) if moist else dry_case(**kw)
+    gi = make_moist(c, 61) if moist else make(c, 61)
+    flds = FIELDS6 if moist else FIELDS5
+    dt = c.scalar("dt")
+    rng = np.random.default_rng(161)
+    dx = random6(c.grid, rng)
+    before = tl_step(gi, dt, dx)
+    _relax(gi, c, 7, interior=interior)
+    after = tl_step(gi, dt, dx)
+    assert any(not np.array_equal(before[f], after[f]) for f in flds)
+
+    def fresh():
+        gp = Integrator.from_case(c, "fp64")
+        # same state-independent setup: weights, targets from the same seed
+        for f in flds:
+            gp.set(f, gi.get(f))
+        _relax(gp, c, 7, interior=interior)
+        return gp
+
+    _dot_and_fd(c, gi, flds, fresh, 162)
 
 
 def test_adjoint_entries_leave_state_set_and_ensembles_intact():
@@ -557,3 +595,18 @@ def test_map_factors_dot_product_and_finite_differences():
         if f in FIELDS5:
             gi.set(f, a)
     _dot_and_fd(c, gi, FIELDS5, fresh, 142)
+
+
+@pytest.mark.parametrize("moist", [False, True], ids=["dry", "moist"])
+def test_open_boundaries_dot_product_and_finite_differences(moist):
+    """Open x and y (radiation on the edge faces: base wind on inflow,
+    Klemp-Wilhelmson on outflow, the branch frozen at the trajectory's edge
+    velocity; zero-gradient ghosts) over the hill, with Coriolis and
+    diffusion."""
+    from wfe_amd import Integrator
+    kw = dict(TERRAIN, coriolis_f=1e-3, diff_K=15.0, bc_x="open", bc_y="open")
+    c = moist_case(**kw) if moist else dry_case(**kw)
+    assert c.params.bc_x_open and c.params.bc_y_open
+    gi = make_moist(c, 51) if moist else make(c, 51)
+    flds = FIELDS6 if moist else FIELDS5
+    _dot_and_fd(c, gi, flds, lambda: Integrator.from_case(c, "fp64"), 151)

@@ -309,15 +309,20 @@ def test_cfg5_adjoint_at_the_supercell_shape():
 @pytest.mark.parametrize("moist,extra", [
     (False, {}), (True, {}), (True, {"coriolis_f": 1e-3, "diff_K": 15.0}),
     (True, {"coriolis_f": 1e-3, "diff_K": 15.0, "terrain": "gauss", "mtn_h0": 400,
-            "mtn_a": 2500, "mtn_xc": 4800, "mtn_yc": 4000})],
-    ids=["dry", "moist", "moist-coriolis-diffusion", "moist-terrain"])
+            "mtn_a": 2500, "mtn_xc": 4800, "mtn_yc": 4000}),
+    (True, {"coriolis_f": 1e-3, "diff_K": 15.0, "terrain": "gauss", "mtn_h0": 400,
+            "mtn_a": 2500, "mtn_xc": 4800, "mtn_yc": 4000, "bc_x": "open"})],
+    ids=["dry", "moist", "moist-coriolis-diffusion", "moist-terrain", "moist-terrain-open-x"])
 def test_adjoint_gradient_against_finite_differences_of_the_oracle(moist, extra):
     """Reference gradient: the objective evaluated by the FP64 oracle (its own
     case assembly and step) at x0 +- eps d, against the hand-written adjoint's
     gradient along d, on a small periodic grid; dry, on the WK82 moist
     sounding without microphysics with q_v in the direction (and along q_v
-    alone), the moist case with Coriolis and diffusion, and the same over a
-    Gaussian hill."""
+    alone), the moist case with Coriolis and diffusion, the same over a
+    Gaussian hill, and that with open x boundaries. (Open edges are taken
+    where the edge velocity stays away from zero: at zero the radiation
+    condition switches between inflow and outflow, a jump that centred
+    differences over six steps would straddle.)"""
     from wfe_amd import Case, Integrator
     from wfe_amd.adjoint import FIELDS, FIELDS5, box_weights, ke_gradient
     over = dict(nx=24, ny=20, nz=16, dx=400, dy=400, dz=300, bubble_xc=4800, bubble_yc=4000,
@@ -370,6 +375,60 @@ def test_adjoint_gradient_against_finite_differences_of_the_oracle(moist, extra)
             print(f"oracle FD {name} eps={eps:g}: {fd!r} adjoint {gd!r} "
                   f"rel {abs(fd - gd) / abs(gd):.2e}", flush=True)
             assert abs(fd - gd) <= tol * abs(gd)
+
+
+def test_regional_adjoint_against_finite_differences_of_the_oracle(tmp_path):
+    """A small regional case in file mode (Lambert map factors, terrain, open
+    boundaries, 5-point Davies relaxation toward one boundary file, WK82-like
+    moisture from the file) with PD transport and microphysics off: the
+    adjoint's gradient along a full direction and along q_v alone against
+    centred differences of the FP64 oracle's objective over 4 steps."""
+    from wfe_amd import Case, Integrator
+    from wfe_amd.adjoint import FIELDS, box_weights, ke_gradient
+    from wfe_amd.lam_case import write_lam_case
+    ini = write_lam_case(tmp_path / "lam", nx=24, ny=18, nz=16, dx=12000.0, top=12000.0,
+                         seed=7, hours=0,
+                         extra={"pd_moist": "off", "microphysics": "off", "w_damping": "off"})
+    c = Case(ini)
+    g = c.grid
+    assert c.params.bc_x_open and c.params.bc_y_open and c.params.moisture
+    assert np.abs(c.array("hx")).max() > 0 and np.ptp(c.array("mapf")) > 1e-3
+    dt = c.scalar("dt")
+    gi = Integrator.from_case(c, "fp64")
+    x0 = {f: gi.get(f) for f in FIELDS}
+    w = box_weights(g, 6, 18, 5, 13)
+    nsteps = 4
+    J, grad = ke_gradient(gi, dt, nsteps, 2, w)
+    mask = np.zeros((g.NZ, g.NY, g.NX), dtype=bool)
+    mask[g.ng:g.ng + g.nz, g.ng:g.ng + g.ny, g.ng:g.ng + g.nx] = True
+    mask = mask.ravel()
+    rng = np.random.default_rng(17)
+    scale = {"u": 1.0, "v": 1.0, "w": 0.3, "thp": 0.5, "pip": 1e-4, "qv": 1e-3}
+    d = {f: np.where(mask, rng.standard_normal(g.npts) * scale[f], 0.0) for f in FIELDS}
+
+    def J_oracle(eps, dd):
+        om = OracleCase(ini).model()
+        for f in FIELDS:
+            om.set(f, x0[f] + eps * dd[f])
+        tot, ww = 0.0, w.reshape(g.NY, g.NX)
+        for s in range(nsteps):
+            om.step(dt, s * dt)
+            u = om.get("u").reshape(g.NZ, g.NY, g.NX)[g.ng]
+            v = om.get("v").reshape(g.NZ, g.NY, g.NX)[g.ng]
+            tot += float((ww * 0.5 * (u * u + v * v)).sum()) / nsteps
+        return tot
+
+    assert J == pytest.approx(J_oracle(0.0, d), rel=1e-12)
+    dq = {f: (d[f] if f == "qv" else np.zeros(g.npts)) for f in FIELDS}
+    for name, dd in (("all", d), ("qv", dq)):
+        gd = sum(float(np.dot(grad[f][mask], dd[f][mask])) for f in FIELDS)
+        errs = []
+        for eps in (1e-5, 1e-6, 1e-7):
+            fd = (J_oracle(eps, dd) - J_oracle(-eps, dd)) / (2 * eps)
+            errs.append(abs(fd - gd) / abs(gd))
+        print(f"regional oracle FD {name}: adjoint {gd!r}, rel err by eps 1e-5..1e-7",
+              [f"{e:.2e}" for e in errs], flush=True)
+        assert gd != 0.0 and min(errs) < 1e-6, (name, errs)
 
 
 def test_cfg5_moist_adjoint_at_the_supercell_shape():

@@ -1,10 +1,10 @@
 """Tangent-linear and adjoint of the large step through the C ABI, and a
 torch.autograd.Function for the lowest-level kinetic-energy objective.
 
-Scope (enforced by the library): FP64 or mixed contexts with periodic
-lateral boundaries, no w damping, no boundary relaxation and no
-microphysics; terrain, map factors, Coriolis and constant-K diffusion are
-included; with
+Scope (enforced by the library): FP64 or mixed contexts without w damping,
+PD transport, microphysics or surface physics; terrain, map factors, open
+boundaries, Davies relaxation (targets held at the context's current ones),
+Coriolis and constant-K diffusion are included; with
 moisture on, q_v is a
 linearised field (flux-form transport without the positive-definite
 limiter, q_c = q_r = 0). Fields are (u, v, w, thp, pip, qv) in the padded

@@ -10,7 +10,10 @@
 // horizontal acoustic step, S_v in the column's pi' equation and the
 // diagnostic surface w(0) = u h_x + v h_y. Coriolis (any f(x, y)) and
 // constant-K diffusion are linear and enter the TL as they are, the AD as
-// their transposes. With moisture on (no
+// their transposes. Open lateral boundaries take the radiation condition on
+// their edge faces, with the inflow/outflow branch frozen at the trajectory;
+// Davies relaxation toward fixed targets adds -w df to the tendencies and
+// sets the pi' tendency to -w dpi' in the rim. With moisture on (no
 // microphysics, flux-form transport without the positive-definite limiter,
 // q_c = q_r = 0), q_v is a sixth linearised field: its transport and stage
 // update, the moist buoyancy in T_w and the virtual-temperature factor
@@ -102,13 +105,22 @@ __global__ void __launch_bounds__(256) k_ad_fill(Grid g, int bcx, int bcy, AdFil
   for (int fi = 0; fi < 6; ++fi) {
     if (fi >= A.n) break;
     double* f = A.f[fi];
+    const bool wf = A.kind[fi] == kFillW;
+    if (wf && q.k == g.nz) {  // the rigid lid w(nz) = 0 is not a variable
+      f[c] = 0.0;
+      continue;
+    }
+    const int im = map_lat(q.i, g.nx, A.xlast[fi], bcx);
+    const int jm = map_lat(q.j, g.ny, A.ylast[fi], bcy);
+    // A ring cell the fill maps onto itself (u at i = nx, v at j = ny with
+    // open boundaries: the last face is a prognostic face) keeps its
+    // adjoint; other ring cells add into it meanwhile.
+    const bool kin = q.k >= 0 && q.k < g.nz;  // vertical ghosts fold onto other levels
+    if (kin && im == q.i && jm == q.j) continue;
     const double gb = f[c];
     f[c] = 0.0;
     if (gb == 0.0) continue;
-    const int im = map_lat(q.i, g.nx, A.xlast[fi], bcx);
-    const int jm = map_lat(q.j, g.ny, A.ylast[fi], bcy);
-    if (A.kind[fi] == kFillW) {
-      if (q.k == g.nz) continue;
+    if (wf) {
       if (q.k < 0) {
         atomicAdd(&f[g.id(im, jm, 0)], 2.0 * gb);
         atomicAdd(&f[g.id(im, jm, -q.k)], -gb);
@@ -199,6 +211,7 @@ struct TanTraj {  // trajectory of one stage (storage type of the forward)
 struct TanPert {  // perturbation (TL) or adjoint (AD) arrays of one stage
   double *u, *v, *w, *th, *mfx, *mfy, *mfz, *div;
   double* q;
+  double* pip;
 };
 struct TanTend {  // tendency perturbations (TL out) or adjoints (AD in)
   double *tu, *tv, *tw, *tth;
@@ -207,8 +220,45 @@ struct TanTend {  // tendency perturbations (TL out) or adjoints (AD in)
   double *qn, *qout;
   double dts;
   int moist;
-  double K;  // diffusion coefficient (0: off)
+  double K;     // diffusion coefficient (0: off)
+  double* tpi;  // pi' tendency (relaxation only; rim columns)
 };
+
+// Davies relaxation (k_bdy_relax, kernels.cu:292-304) of the stage
+// estimate toward fixed targets: T_f += w (target - f) for u, v, theta' and
+// (through the stage update q = q_n + dts T_q) q_v, T_pi' = w (target - pi')
+// where w > 0. TL: -w df; AD: the transposes, one cell per thread.
+template <class S>
+__global__ void k_tl_relax(Grid g, Traj<S> wgt, TanPert D, TanTend out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+  const double w = wgt[g.id2(i, j)];
+  if (w <= 0.0) return;
+  const int c = g.id(i, j, k);
+  out.tu[c] -= w * D.u[c];
+  out.tv[c] -= w * D.v[c];
+  out.tth[c] -= w * D.th[c];
+  out.tpi[c] = -w * D.pip[c];
+  if (out.moist) out.qout[c] -= out.dts * w * D.q[c];
+}
+
+template <class S>
+__global__ void k_ad_relax(Grid g, Traj<S> wgt, TanPert A, TanTend in) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+  const double w = wgt[g.id2(i, j)];
+  if (w <= 0.0) return;
+  const int c = g.id(i, j, k);
+  A.u[c] -= w * in.tu[c];
+  A.v[c] -= w * in.tv[c];
+  A.th[c] -= w * in.tth[c];
+  A.pip[c] -= w * in.tpi[c];
+  if (in.moist) A.q[c] -= in.dts * w * in.qout[c];
+}
 
 // K times the 7-point Laplacian of the perturbation f at c (k_diffuse,
 // flat terrain: 1/(dzeta J)^2 = rdz^2), and its transpose: T scattered onto
@@ -480,16 +530,23 @@ __global__ void k_ad_tend(Grid g, Base<double> b, Metric<double> m, TanTraj<S> X
 }
 
 // ------------------------------------------------- horizontal acoustic step
-// u_new = u + dtau (-cp thv_u grad pi* + Tu) on faces i = 0..nx-1 (u(nx) is
-// the periodic image of u(0)), same for v. pi* = pi' + smdiv (pi' - pi'prev)
-// with the substep ghost rule: outside the interior, pi' is read from the
-// mapped interior of the current substep and pi'prev from the time-n ghost
-// cell; theta' outside the interior from the time-n ghost cell.
+// u_new = u + dtau (-cp thv_u m_u grad pi* + Tu) on faces i = 0..nx (with
+// periodic x, u(nx) is the image of u(0)), the same for v. pi* = pi' +
+// smdiv (pi' - pi'prev) with the substep ghost rule: outside the interior,
+// pi' is read from the mapped interior of the current substep and pi'prev
+// from the time-n ghost cell; theta' outside the interior from the time-n
+// ghost cell. Over terrain the gradient has the term -(z_x / J) dpi*/dzeta.
+// On open edges the boundary face takes the Klemp-Wilhelmson radiation
+// value (k_bc_rad_u_x / k_bc_rad_v_y, kernels.cu:649-688): the base wind on
+// inflow; on outflow u - dtau (u - c*) (u_step(1) - u) / dx at i = 0 and
+// u - dtau (u + c*) (u - u_step(nx-1)) / dx at i = nx. The inflow/outflow
+// branch is frozen at the trajectory's edge velocity.
 template <class S>
 struct UVTraj {
   const double *pcur, *pprev, *pn;
   Traj<S> thc, thn;
-  Traj<S> qn;  // time-n q_v, ghost cells filled (read when p.moisture)
+  Traj<S> qn;          // time-n q_v, ghost cells filled (read when p.moisture)
+  Traj<S> us, vs, tu, tv;  // u, v entering the substep and their tendencies
 };
 struct UVTan {  // TL: perturbations in, du/dv out; AD: adjoints
   double *u_src, *v_src, *u_dst, *v_dst, *pcur, *pprev, *pn, *thc, *thn, *tu, *tv;
@@ -501,7 +558,9 @@ struct UVLin {
   const Grid& g;
   const Params& p;
   const Base<double>& b;
+  const Metric<double>& m;
   const UVTraj<S>& X;
+  double dtau;
   __device__ bool inside(int ii, int jj) const {
     return (unsigned)ii < (unsigned)g.nx && (unsigned)jj < (unsigned)g.ny;
   }
@@ -550,11 +609,16 @@ struct UVLin {
       atomicAdd(&A.pn[c], -p.smdiv * pb);
     }
   }
-  // Terrain term of the pi* gradient at a u face (x) or v face (y): the
-  // coefficient (z_s / J_face) / (2 dzeta_2) of the centred vertical
+  __device__ void ad_thv(const UVTan& A, int ii, int jj, int k, double tb) const {
+    const int c = g.id(ii, jj, k);
+    atomicAdd(inside(ii, jj) ? &A.thc[c] : &A.thn[c], tb * fq(c));
+    if (p.moisture) atomicAdd(&A.qn[c], tb * dry(ii, jj, k) * kEps61);
+  }
+  // Terrain term of the pi* gradient at a u face (dir 0) or v face (dir 1):
+  // the coefficient (z_s / J_face) / (2 dzeta_2) of the centred vertical
   // differences of pi* in the two cells beside the face (zero on flat
-  // ground), and the differences themselves.
-  __device__ double tcoef(const Metric<double>& m, int c2, int k, int dir) const {
+  // ground).
+  __device__ double tcoef(int c2, int k, int dir) const {
     const double h = dir == 0 ? m.hxu[c2] : m.hyv[c2];
     if (h == 0.0) return 0.0;
     const int kk = k + g.ng;
@@ -564,27 +628,87 @@ struct UVLin {
     const double jf = 0.5 * (m.jac[c2 - o] + m.jac[c2]);
     return (zs / jf) * 0.5 / dz2;
   }
-  __device__ double tdiff(int i1, int j1, int i2, int j2, int k) const {
-    return (pstar(i1, j1, k + 1) - pstar(i1, j1, k - 1)) +
-           (pstar(i2, j2, k + 1) - pstar(i2, j2, k - 1));
+  // Face (i, j, k) in direction dir: the cells beside it are (i-1, j) and
+  // (i, j) for u, (i, j-1) and (i, j) for v.
+  struct Face {
+    int c, c2, i0, j0, i1, j1;
+    double mf, rd;
+  };
+  __device__ Face face(int i, int j, int k, int dir) const {
+    Face f;
+    f.c = g.id(i, j, k);
+    f.c2 = g.id2(i, j);
+    f.i1 = i;
+    f.j1 = j;
+    f.i0 = dir == 0 ? i - 1 : i;
+    f.j0 = dir == 0 ? j : j - 1;
+    const int o = dir == 0 ? 1 : g.NX;
+    f.mf = 0.5 * (m.mapf[f.c2 - o] + m.mapf[f.c2]);
+    f.rd = dir == 0 ? g.rdx : g.rdy;
+    return f;
   }
-  __device__ double dtdiff(const UVTan& D, int i1, int j1, int i2, int j2, int k) const {
-    return (dpstar(D, i1, j1, k + 1) - dpstar(D, i1, j1, k - 1)) +
-           (dpstar(D, i2, j2, k + 1) - dpstar(D, i2, j2, k - 1));
+  __device__ double tdiff(const Face& f, int k) const {
+    return (pstar(f.i0, f.j0, k + 1) - pstar(f.i0, f.j0, k - 1)) +
+           (pstar(f.i1, f.j1, k + 1) - pstar(f.i1, f.j1, k - 1));
   }
-  __device__ void ad_tdiff(const UVTan& A, int i1, int j1, int i2, int j2, int k,
-                           double b) const {
-    ad_pstar(A, i1, j1, k + 1, b);
-    ad_pstar(A, i1, j1, k - 1, -b);
-    ad_pstar(A, i2, j2, k + 1, b);
-    ad_pstar(A, i2, j2, k - 1, -b);
+  __device__ double grad(const Face& f, int k, double tc) const {
+    double gr = (pstar(f.i1, f.j1, k) - pstar(f.i0, f.j0, k)) * f.rd;
+    if (tc != 0.0) gr -= tc * tdiff(f, k);
+    return gr;
   }
-  __device__ void ad_thv(const UVTan& A, int ii, int jj, int k, double tb) const {
-    const int c = g.id(ii, jj, k);
-    atomicAdd(inside(ii, jj) ? &A.thc[c] : &A.thn[c], tb * fq(c));
-    if (p.moisture) atomicAdd(&A.qn[c], tb * dry(ii, jj, k) * kEps61);
+  // Interior step of the face: trajectory value, TL, AD (adjoint ob of the
+  // new value scattered onto the inputs).
+  __device__ double step(int i, int j, int k, int dir) const {
+    const Face f = face(i, j, k, dir);
+    const double th = 0.5 * (thv(f.i0, f.j0, k) + thv(f.i1, f.j1, k));
+    const double src = dir == 0 ? X.us[f.c] : X.vs[f.c];
+    const double t = dir == 0 ? X.tu[f.c] : X.tv[f.c];
+    return src + dtau * (-kCp * th * f.mf * grad(f, k, tcoef(f.c2, k, dir)) + t);
+  }
+  __device__ double dstep(const UVTan& D, int i, int j, int k, int dir) const {
+    const Face f = face(i, j, k, dir);
+    const double tc = tcoef(f.c2, k, dir);
+    const double th = 0.5 * (thv(f.i0, f.j0, k) + thv(f.i1, f.j1, k));
+    const double dth = 0.5 * (dthv(D, f.i0, f.j0, k) + dthv(D, f.i1, f.j1, k));
+    double dgr = (dpstar(D, f.i1, f.j1, k) - dpstar(D, f.i0, f.j0, k)) * f.rd;
+    if (tc != 0.0)
+      dgr -= tc * ((dpstar(D, f.i0, f.j0, k + 1) - dpstar(D, f.i0, f.j0, k - 1)) +
+                   (dpstar(D, f.i1, f.j1, k + 1) - dpstar(D, f.i1, f.j1, k - 1)));
+    const double src = dir == 0 ? D.u_src[f.c] : D.v_src[f.c];
+    const double t = dir == 0 ? D.tu[f.c] : D.tv[f.c];
+    return src + dtau * (-kCp * f.mf * (dth * grad(f, k, tc) + th * dgr) + t);
+  }
+  __device__ void ad_step(const UVTan& A, int i, int j, int k, int dir, double ob) const {
+    const Face f = face(i, j, k, dir);
+    const double tc = tcoef(f.c2, k, dir);
+    const double th = 0.5 * (thv(f.i0, f.j0, k) + thv(f.i1, f.j1, k));
+    atomicAdd(dir == 0 ? &A.u_src[f.c] : &A.v_src[f.c], ob);
+    atomicAdd(dir == 0 ? &A.tu[f.c] : &A.tv[f.c], dtau * ob);
+    const double x = -dtau * kCp * f.mf * ob;
+    const double thb = 0.5 * x * grad(f, k, tc);
+    ad_thv(A, f.i0, f.j0, k, thb);
+    ad_thv(A, f.i1, f.j1, k, thb);
+    const double gb = x * th;
+    ad_pstar(A, f.i1, f.j1, k, gb * f.rd);
+    ad_pstar(A, f.i0, f.j0, k, -gb * f.rd);
+    if (tc != 0.0) {
+      const double tb = -tc * gb;
+      ad_pstar(A, f.i0, f.j0, k + 1, tb);
+      ad_pstar(A, f.i0, f.j0, k - 1, -tb);
+      ad_pstar(A, f.i1, f.j1, k + 1, tb);
+      ad_pstar(A, f.i1, f.j1, k - 1, -tb);
+    }
   }
 };
+
+// Which formula face n (0..nn) of a direction takes: 0 the interior step,
+// 1 the periodic image of face 0, 2 the west/south radiation edge, 3 the
+// east/north one.
+__device__ __forceinline__ int face_kind(int n, int nn, int open) {
+  if (n < nn && !(open && n == 0)) return 0;
+  if (!open) return 1;
+  return n == 0 ? 2 : 3;
+}
 
 template <class S>
 __global__ void k_tl_uv(Grid g, Params p, Base<double> b, Metric<double> m, UVTraj<S> X, UVTan D,
@@ -593,37 +717,36 @@ __global__ void k_tl_uv(Grid g, Params p, Base<double> b, Metric<double> m, UVTr
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   const int k = blockIdx.z;
   if (i > g.nx || j > g.ny || k >= g.nz) return;
-  const UVLin<S> L{g, p, b, X};
+  const UVLin<S> L{g, p, b, m, X, dtau};
   const int c = g.id(i, j, k);
-  if (j < g.ny) {  // u face i (i = nx is the image of face 0)
-    const int iu = i < g.nx ? i : 0;
-    const int cu = g.id(iu, j, k), c2 = g.id2(iu, j);
-    const double mu = 0.5 * (m.mapf[c2 - 1] + m.mapf[c2]);
-    const double thu = 0.5 * (L.thv(iu - 1, j, k) + L.thv(iu, j, k));
-    const double dthu = 0.5 * (L.dthv(D, iu - 1, j, k) + L.dthv(D, iu, j, k));
-    double grad = (L.pstar(iu, j, k) - L.pstar(iu - 1, j, k)) * g.rdx;
-    double dgrad = (L.dpstar(D, iu, j, k) - L.dpstar(D, iu - 1, j, k)) * g.rdx;
-    const double tc = L.tcoef(m, c2, k, 0);
-    if (tc != 0.0) {
-      grad -= tc * L.tdiff(iu - 1, j, iu, j, k);
-      dgrad -= tc * L.dtdiff(D, iu - 1, j, iu, j, k);
+  for (int dir = 0; dir < 2; ++dir) {
+    if (dir == 0 ? j >= g.ny : i >= g.nx) continue;
+    const int n = dir == 0 ? i : j, nn = dir == 0 ? g.nx : g.ny;
+    const int open = dir == 0 ? p.bcx : p.bcy;
+    const double* src = dir == 0 ? D.u_src : D.v_src;
+    double d;
+    switch (face_kind(n, nn, open)) {
+      case 0: d = L.dstep(D, i, j, k, dir); break;
+      case 1: d = dir == 0 ? L.dstep(D, 0, j, k, 0) : L.dstep(D, i, 0, k, 1); break;
+      case 2: {  // west/south edge: outflow when the edge velocity is negative
+        const double u0 = dir == 0 ? X.us[c] : X.vs[c];
+        if (u0 >= 0.0) { d = 0.0; break; }
+        const double cb = u0 - p.cstar, h = dir == 0 ? g.dx : g.dy;
+        const int i1 = dir == 0 ? 1 : i, j1 = dir == 0 ? j : 1;
+        const double U1 = L.step(i1, j1, k, dir), dU1 = L.dstep(D, i1, j1, k, dir);
+        d = src[c] - dtau * (src[c] * (U1 - u0) + cb * (dU1 - src[c])) / h;
+        break;
+      }
+      default: {  // east/north edge: outflow when the edge velocity is positive
+        const double u1 = dir == 0 ? X.us[c] : X.vs[c];
+        if (u1 <= 0.0) { d = 0.0; break; }
+        const double cb = u1 + p.cstar, h = dir == 0 ? g.dx : g.dy;
+        const int in = dir == 0 ? nn - 1 : i, jn = dir == 0 ? j : nn - 1;
+        const double Un = L.step(in, jn, k, dir), dUn = L.dstep(D, in, jn, k, dir);
+        d = src[c] - dtau * (src[c] * (u1 - Un) + cb * (src[c] - dUn)) / h;
+      }
     }
-    D.u_dst[c] = D.u_src[cu] + dtau * (-kCp * mu * (dthu * grad + thu * dgrad) + D.tu[cu]);
-  }
-  if (i < g.nx) {  // v face j (j = ny is the image of face 0)
-    const int jv = j < g.ny ? j : 0;
-    const int cv = g.id(i, jv, k), c2 = g.id2(i, jv);
-    const double mv = 0.5 * (m.mapf[c2 - g.NX] + m.mapf[c2]);
-    const double thv_ = 0.5 * (L.thv(i, jv - 1, k) + L.thv(i, jv, k));
-    const double dthv_ = 0.5 * (L.dthv(D, i, jv - 1, k) + L.dthv(D, i, jv, k));
-    double grad = (L.pstar(i, jv, k) - L.pstar(i, jv - 1, k)) * g.rdy;
-    double dgrad = (L.dpstar(D, i, jv, k) - L.dpstar(D, i, jv - 1, k)) * g.rdy;
-    const double tc = L.tcoef(m, c2, k, 1);
-    if (tc != 0.0) {
-      grad -= tc * L.tdiff(i, jv - 1, i, jv, k);
-      dgrad -= tc * L.dtdiff(D, i, jv - 1, i, jv, k);
-    }
-    D.v_dst[c] = D.v_src[cv] + dtau * (-kCp * mv * (dthv_ * grad + thv_ * dgrad) + D.tv[cv]);
+    (dir == 0 ? D.u_dst : D.v_dst)[c] = d;
   }
 }
 
@@ -634,50 +757,42 @@ __global__ void k_ad_uv(Grid g, Params p, Base<double> b, Metric<double> m, UVTr
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   const int k = blockIdx.z;
   if (i > g.nx || j > g.ny || k >= g.nz) return;
-  const UVLin<S> L{g, p, b, X};
+  const UVLin<S> L{g, p, b, m, X, dtau};
   const int c = g.id(i, j, k);
-  if (j < g.ny) {
-    const double ob = A.u_dst[c];
-    if (ob != 0.0) {
-      const int iu = i < g.nx ? i : 0;
-      const int cu = g.id(iu, j, k), c2 = g.id2(iu, j);
-      const double mu = 0.5 * (m.mapf[c2 - 1] + m.mapf[c2]);
-      const double thu = 0.5 * (L.thv(iu - 1, j, k) + L.thv(iu, j, k));
-      double grad = (L.pstar(iu, j, k) - L.pstar(iu - 1, j, k)) * g.rdx;
-      const double tc = L.tcoef(m, c2, k, 0);
-      if (tc != 0.0) grad -= tc * L.tdiff(iu - 1, j, iu, j, k);
-      atomicAdd(&A.u_src[cu], ob);
-      atomicAdd(&A.tu[cu], dtau * ob);
-      const double x = -dtau * kCp * mu * ob;
-      const double thb_ = 0.5 * x * grad;
-      L.ad_thv(A, iu - 1, j, k, thb_);
-      L.ad_thv(A, iu, j, k, thb_);
-      const double gb = x * thu * g.rdx;
-      L.ad_pstar(A, iu, j, k, gb);
-      L.ad_pstar(A, iu - 1, j, k, -gb);
-      if (tc != 0.0) L.ad_tdiff(A, iu - 1, j, iu, j, k, -tc * x * thu);
-    }
-  }
-  if (i < g.nx) {
-    const double ob = A.v_dst[c];
-    if (ob != 0.0) {
-      const int jv = j < g.ny ? j : 0;
-      const int cv = g.id(i, jv, k), c2 = g.id2(i, jv);
-      const double mv = 0.5 * (m.mapf[c2 - g.NX] + m.mapf[c2]);
-      const double thv_ = 0.5 * (L.thv(i, jv - 1, k) + L.thv(i, jv, k));
-      double grad = (L.pstar(i, jv, k) - L.pstar(i, jv - 1, k)) * g.rdy;
-      const double tc = L.tcoef(m, c2, k, 1);
-      if (tc != 0.0) grad -= tc * L.tdiff(i, jv - 1, i, jv, k);
-      atomicAdd(&A.v_src[cv], ob);
-      atomicAdd(&A.tv[cv], dtau * ob);
-      const double x = -dtau * kCp * mv * ob;
-      const double thb_ = 0.5 * x * grad;
-      L.ad_thv(A, i, jv - 1, k, thb_);
-      L.ad_thv(A, i, jv, k, thb_);
-      const double gb = x * thv_ * g.rdy;
-      L.ad_pstar(A, i, jv, k, gb);
-      L.ad_pstar(A, i, jv - 1, k, -gb);
-      if (tc != 0.0) L.ad_tdiff(A, i, jv - 1, i, jv, k, -tc * x * thv_);
+  for (int dir = 0; dir < 2; ++dir) {
+    if (dir == 0 ? j >= g.ny : i >= g.nx) continue;
+    const double ob = (dir == 0 ? A.u_dst : A.v_dst)[c];
+    if (ob == 0.0) continue;
+    const int n = dir == 0 ? i : j, nn = dir == 0 ? g.nx : g.ny;
+    const int open = dir == 0 ? p.bcx : p.bcy;
+    double* srcb = dir == 0 ? A.u_src : A.v_src;
+    switch (face_kind(n, nn, open)) {
+      case 0: L.ad_step(A, i, j, k, dir, ob); break;
+      case 1:
+        if (dir == 0) L.ad_step(A, 0, j, k, 0, ob);
+        else L.ad_step(A, i, 0, k, 1, ob);
+        break;
+      case 2: {
+        const double u0 = dir == 0 ? X.us[c] : X.vs[c];
+        if (u0 >= 0.0) break;
+        const double cb = u0 - p.cstar, h = dir == 0 ? g.dx : g.dy;
+        const int i1 = dir == 0 ? 1 : i, j1 = dir == 0 ? j : 1;
+        const double U1 = L.step(i1, j1, k, dir);
+        // d = src (1 - dtau (U1 - u0) / h + dtau cb / h) - dtau cb / h dU1
+        atomicAdd(&srcb[c], ob * (1.0 - dtau * (U1 - u0) / h + dtau * cb / h));
+        L.ad_step(A, i1, j1, k, dir, -ob * dtau * cb / h);
+        break;
+      }
+      default: {
+        const double u1 = dir == 0 ? X.us[c] : X.vs[c];
+        if (u1 <= 0.0) break;
+        const double cb = u1 + p.cstar, h = dir == 0 ? g.dx : g.dy;
+        const int in = dir == 0 ? nn - 1 : i, jn = dir == 0 ? j : nn - 1;
+        const double Un = L.step(in, jn, k, dir);
+        // d = src (1 - dtau (u1 - Un) / h - dtau cb / h) + dtau cb / h dUn
+        atomicAdd(&srcb[c], ob * (1.0 - dtau * (u1 - Un) / h - dtau * cb / h));
+        L.ad_step(A, in, jn, k, dir, ob * dtau * cb / h);
+      }
     }
   }
 }
@@ -693,11 +808,15 @@ struct WPTraj {
   const double* pip;
   Traj<S> th;
   Traj<S> qn;  // time-n q_v (read when p.moisture)
+  // relaxation pi' tendency and the rim weight (tpw.p null: no relaxation);
+  // the tendency enters P in columns with a positive weight
+  Traj<S> tpi, tpw;
 };
 struct WPTan {  // TL: perturbations; AD: adjoints
   double *u, *v, *w, *pip, *th, *tw, *tth;  // inputs
   double *w_out, *pip_out, *th_out;         // outputs
   double* qn;                               // time-n q_v (input)
+  double* tpi;                              // relaxation pi' tendency (input)
 };
 
 // theta_v at the w level below cell c: its dry part and the moist factor.
@@ -764,6 +883,7 @@ __device__ void wp_forward(const Grid& g, const Params& p, const Base<double>& b
                             (Dh - dsv * rdzc +
                              c2 * (s.Av[k + 1] * (double)X.w[c + PL] - s.Av[k] * (double)X.w[c]) *
                                  rdzc);
+    if (X.tpw.p && X.tpw[c2d] > 0.0) s.P[k] += dtau * X.tpi[c];
     s.E[k] = dtau * Kt * c1 * rdzc;
     if (k >= 1) {
       const double rJdzw = m.rdzw[kk] / J;
@@ -827,6 +947,7 @@ __global__ void __launch_bounds__(64) k_tl_wpi(Grid g, Params p, Base<double> b,
     dP[k] = D.pip[c] - dtau * Kt *
                            (dDh - ddsv * rdzc +
                             c2 * (s.Av[k + 1] * D.w[c + PL] - s.Av[k] * D.w[c]) * rdzc);
+    if (X.tpw.p && X.tpw[c2d] > 0.0) dP[k] += dtau * D.tpi[c];
     dG[k] = 0.0;
     dR[k] = 0.0;
     if (k >= 1) {
@@ -939,6 +1060,7 @@ __global__ void __launch_bounds__(64) k_ad_wpi(Grid g, Params p, Base<double> b,
     // P_k = pip - dtau Kt (Dh - (Sv_{k+1} - Sv_k) rdzc + c2 (Av_{k+1} w_{k+1} - Av_k w_k) rdzc),
     // Dh = m (...): the map factor multiplies Dh alone
     A.pip[c] += Pb[k];
+    if (X.tpw.p && X.tpw[c2d] > 0.0) A.tpi[c] += dtau * Pb[k];
     const double Qb = -dtau * Kt * Pb[k];
     const double Dhb = Qb * mc;
     A.w[c + PL] += Qb * c2 * s.Av[k + 1] * rdzc;

@@ -1802,6 +1802,10 @@ class Adjoint final : public AdjointBase {
     set5(xb_);
     set5(tmp_);
     zero_q_ = A.alloc<T>(n);
+    for (int m = 0; m < 3; ++m) {
+      tpi_tr_[m] = A.alloc<T>(n);
+      dtpi_[m] = A.alloc<double>(n);
+    }
     if (moist_) {
       qc_scr_ = A.alloc<T>(n);
       qr_scr_ = A.alloc<T>(n);
@@ -1811,20 +1815,18 @@ class Adjoint final : public AdjointBase {
     widen_constants();
   }
 
-  // The linearised model covers the periodic flat step, dry or with q_v
-  // transported and acting through buoyancy and theta_v but no
-  // microphysics. The context can change after the adjoint was set up
-  // (boundary relaxation is switched on by wfe_set_bdy_weights, the state
-  // is overwritten), so every entry checks again.
+  // The linearised model covers the step without microphysics, w damping,
+  // PD transport or surface physics: dry, or with q_v transported and acting
+  // through buoyancy and theta_v. The context can change after the adjoint
+  // was set up (relaxation switched on by wfe_set_bdy_weights, the state
+  // overwritten), so every entry checks again and follows the context.
   void check_scope() const {
     const auto& hp = F.hp_;
     std::string why;
     if (hp.moisture && hp.microphysics) why = "microphysics is on";
     else if (hp.moisture && hp.pd_moist) why = "positive-definite moisture transport is on";
-    else if (hp.bc_x_open || hp.bc_y_open) why = "lateral boundaries are not periodic";
     else if (hp.diff_K < 0) why = "the diffusion coefficient is negative";
     else if (hp.w_damping) why = "w damping is on";
-    else if (F.bdy_active_) why = "boundary relaxation is on";
     else if (F.phys_on_) why = "the surface physics is on";
     else if (F.g_.nz + 1 > 64) why = "more than 63 levels";
     else if (std::max(1, hp.acoustic_ns) > kMaxSub) why = "more than 16 acoustic substeps";
@@ -1839,8 +1841,7 @@ class Adjoint final : public AdjointBase {
       }
     }
     if (!why.empty())
-      throw std::runtime_error("adjoint: scope is the periodic flat step without microphysics (" +
-                               why + ")");
+      throw std::runtime_error("adjoint: the step is outside the linearised model (" + why + ")");
   }
 
   // Forward step of F from its current state, keeping the trajectory.
@@ -1857,18 +1858,23 @@ class Adjoint final : public AdjointBase {
         copy5(est_[m], sub_[m - 1][nsub_[m - 1] - 1], st);
         f.fill_ghosts(est_set(m, sn));
       }
+      // q_v of the stage = q_v,n + dts T_q(estimate), into the stage's
+      // output set; q_c, q_r (zero) go to scratch, or into the new state at
+      // the end. Without relaxation the update is formed where T_q is (as
+      // the forward step does); with it, after the relaxation term.
+      const bool bdy = f.bdy_active_;
+      StateSet<T> qo{};
       if (moist_) {
-        // q_v of the stage = q_v,n + dts T_q(estimate), written where T_q is
-        // formed (as the forward step does) into the stage's output set;
-        // q_c, q_r (zero) go to scratch, or into the new state at the end
-        StateSet<T> qo{};
         const StateSet<T>& nxt = f.S_[f.cur_ ^ 1];
         qo.qv = sub_[m][nsub_[m] - 1].q;
         qo.qc = m == 2 ? nxt.qc : qc_scr_;
         qo.qr = m == 2 ? nxt.qr : qr_scr_;
-        f.slow_tendencies(est_set(m, sn), dt, &qo, sdt[m], true);
-      } else {
-        f.slow_tendencies(est_set(m, sn), dt);
+      }
+      f.slow_tendencies(est_set(m, sn), dt, moist_ && !bdy ? &qo : nullptr, sdt[m], true);
+      if (bdy) {
+        f.bdy_relax(est_set(m, sn));
+        copy(tpi_tr_[m], f.tpi_, st);
+        if (moist_) f.stage_moisture(qo, sdt[m]);
       }
       copy(mf_[m].a, f.mfx_, st);
       copy(mf_[m].b, f.mfy_, st);
@@ -1902,7 +1908,8 @@ class Adjoint final : public AdjointBase {
         a.thp_dst = dst.th;
         a.tw = T_[m].c;
         a.tth = T_[m].d;
-        a.tpi = nullptr;
+        a.tpi = bdy ? tpi_tr_[m] : nullptr;
+        a.tpw = f.bdy_wgt_;
         f.methods_[f.method_].run(a);
       }
     }
@@ -1935,19 +1942,23 @@ class Adjoint final : public AdjointBase {
       }
       dev::k_tl_tend<<<grid3(), dim3(64, 4), 0, st>>>(f.g_, b64_, m64_, traj(m), pert(m),
                                                        tend(m, sdt[m]));
+      if (f.bdy_active_)
+        dev::k_tl_relax<T><<<grid3(), dim3(64, 4), 0, st>>>(f.g_, dev::Traj<T>{f.bdy_wgt_},
+                                                            pert(m), tend(m, sdt[m]));
       const double dtau = sdt[m] / nsub_[m];
       for (int s = 0; s < nsub_[m]; ++s) {
         const TSet xs = s == 0 ? est_[0] : sub_[m][s - 1];
         const Set5 ds = s == 0 ? dest_[0] : dsub_[m][s - 1];
         dev::UVTraj<T> X{xs.pip, s <= 1 ? est_[0].pip : sub_[m][s - 2].pip, est_[0].pip, xs.th,
-                         est_[0].th, est_[0].q};
+                         est_[0].th, est_[0].q, xs.u, xs.v, T_[m].a, T_[m].b};
         dev::UVTan D{ds.u, ds.v, dsub_[m][s].u, dsub_[m][s].v, ds.pip,
                      s <= 1 ? dest_[0].pip : dsub_[m][s - 2].pip, dest_[0].pip, ds.th,
                      dest_[0].th, dT_[m].a, dT_[m].b, dest_[0].q};
         dev::k_tl_uv<<<grid_uv(), dim3(64, 4), 0, st>>>(f.g_, f.p_, b64_, m64_, X, D, dtau);
-        dev::WPTraj<T> W{sub_[m][s].u, sub_[m][s].v, xs.w, xs.pip, xs.th, est_[0].q};
+        dev::WPTraj<T> W{sub_[m][s].u, sub_[m][s].v, xs.w,       xs.pip,
+                         xs.th,       est_[0].q,    tpi_tr_[m], f.bdy_active_ ? f.bdy_wgt_ : nullptr};
         dev::WPTan DW{dsub_[m][s].u, dsub_[m][s].v, ds.w, ds.pip, ds.th, dT_[m].c, dT_[m].d,
-                      dsub_[m][s].w, dsub_[m][s].pip, dsub_[m][s].th, dest_[0].q};
+                      dsub_[m][s].w, dsub_[m][s].pip, dsub_[m][s].th, dest_[0].q, dtpi_[m]};
         dev::k_tl_wpi<T, 64><<<grid_col(), 64, 0, st>>>(f.g_, f.p_, b64_, m64_, c64_, W,
                                                        T_[m].c, DW, dtau);
       }
@@ -1968,13 +1979,14 @@ class Adjoint final : public AdjointBase {
       for (int s = nsub_[m] - 1; s >= 0; --s) {
         const TSet xs = s == 0 ? est_[0] : sub_[m][s - 1];
         const Set5 ds = s == 0 ? dest_[0] : dsub_[m][s - 1];
-        dev::WPTraj<T> W{sub_[m][s].u, sub_[m][s].v, xs.w, xs.pip, xs.th, est_[0].q};
+        dev::WPTraj<T> W{sub_[m][s].u, sub_[m][s].v, xs.w,       xs.pip,
+                         xs.th,       est_[0].q,    tpi_tr_[m], f.bdy_active_ ? f.bdy_wgt_ : nullptr};
         dev::WPTan AW{dsub_[m][s].u, dsub_[m][s].v, ds.w, ds.pip, ds.th, dT_[m].c, dT_[m].d,
-                      dsub_[m][s].w, dsub_[m][s].pip, dsub_[m][s].th, dest_[0].q};
+                      dsub_[m][s].w, dsub_[m][s].pip, dsub_[m][s].th, dest_[0].q, dtpi_[m]};
         dev::k_ad_wpi<T, 64><<<grid_col(), 64, 0, st>>>(f.g_, f.p_, b64_, m64_, c64_, W,
                                                        T_[m].c, AW, dtau);
         dev::UVTraj<T> X{xs.pip, s <= 1 ? est_[0].pip : sub_[m][s - 2].pip, est_[0].pip, xs.th,
-                         est_[0].th, est_[0].q};
+                         est_[0].th, est_[0].q, xs.u, xs.v, T_[m].a, T_[m].b};
         dev::UVTan AU{ds.u, ds.v, dsub_[m][s].u, dsub_[m][s].v, ds.pip,
                       s <= 1 ? dest_[0].pip : dsub_[m][s - 2].pip, dest_[0].pip, ds.th,
                       dest_[0].th, dT_[m].a, dT_[m].b, dest_[0].q};
@@ -1982,6 +1994,9 @@ class Adjoint final : public AdjointBase {
       }
       dev::k_ad_tend<<<grid3(), dim3(64, 4), 0, st>>>(f.g_, b64_, m64_, traj(m), pert(m),
                                                        tend(m, sdt[m]));
+      if (f.bdy_active_)
+        dev::k_ad_relax<T><<<grid3(), dim3(64, 4), 0, st>>>(f.g_, dev::Traj<T>{f.bdy_wgt_},
+                                                            pert(m), tend(m, sdt[m]));
       {
         dim3 grd(cdiv(f.g_.nx + 5, 64), cdiv(f.g_.ny + 5, 4), (unsigned)f.g_.nz);
         dev::k_ad_massflux<<<grd, dim3(64, 4), 0, st>>>(f.g_, m64_, dmf_[m].a, dmf_[m].b,
@@ -2162,15 +2177,15 @@ class Adjoint final : public AdjointBase {
                            mf_[m].b,  mf_[m].c,  mf_[m].d,  est_[m].q};
   }
   dev::TanPert pert(int m) const {
-    return dev::TanPert{dest_[m].u, dest_[m].v, dest_[m].w, dest_[m].th,
-                        dmf_[m].a,  dmf_[m].b,  dmf_[m].c,  dmf_[m].d, dest_[m].q};
+    return dev::TanPert{dest_[m].u, dest_[m].v, dest_[m].w, dest_[m].th, dmf_[m].a,
+                        dmf_[m].b,  dmf_[m].c,  dmf_[m].d,  dest_[m].q,  dest_[m].pip};
   }
   // Tendencies of stage m; moist: time-n q_v and the stage's q_v output
   // (held in the stage's last substep set).
   dev::TanTend tend(int m, double dts) const {
     const Set4& t = dT_[m];
     return dev::TanTend{t.a, t.b, t.c, t.d, dest_[0].q, dsub_[m][nsub_[m] - 1].q, dts,
-                        moist_ ? 1 : 0, F.hp_.diff_K};
+                        moist_ ? 1 : 0, F.hp_.diff_K, dtpi_[m]};
   }
   dim3 grid3() const { return dim3(cdiv(F.g_.nx, 64), cdiv(F.g_.ny, 4), (unsigned)F.g_.nz); }
   dim3 grid_uv() const {
@@ -2196,6 +2211,7 @@ class Adjoint final : public AdjointBase {
   }
   void clear_perturbations() {
     for (int m = 0; m < 3; ++m) {
+      zero(dtpi_[m]);
       zero5(dest_[m]);
       zero4(dmf_[m]);
       zero4(dT_[m]);
@@ -2312,6 +2328,8 @@ class Adjoint final : public AdjointBase {
   Set5 xb_, tmp_;
   bool moist_ = false;
   T* zero_q_ = nullptr;
+  T* tpi_tr_[3] = {};         // relaxation pi' tendency of each stage
+  double* dtpi_[3] = {};      // its perturbation / adjoint
   T *qc_scr_ = nullptr, *qr_scr_ = nullptr;
   double* jsum_ = nullptr;
   double* wgt_ = nullptr;
