@@ -140,8 +140,9 @@ int wfe_set_physics(wfe_ctx* ctx, const wfe_sfc* sfc, size_t n2);
 
 /* ---- tangent-linear and adjoint of the large step ----
  * No counterpart in the reference. Scope: FP64 or mixed contexts without
- * w damping, PD transport, microphysics or surface physics (WFE_E_CONFIG
- * otherwise); terrain, map factors, open boundaries (radiation, branch
+ * w damping, microphysics or surface physics (WFE_E_CONFIG otherwise);
+ * terrain, map factors, PD q_v transport (limiter branches frozen at the
+ * trajectory), open boundaries (radiation, branch
  * frozen at the trajectory), Davies relaxation (targets and time fraction
  * held at the context's current values over a gradient's steps), Coriolis
  * and constant-K diffusion are included; with moisture on, q_v is linearised
@@ -166,8 +167,8 @@ int wfe_ad_step(wfe_ctx* ctx, double dt, const double* const* yb, double* const*
 int wfe_adj_gradient(wfe_ctx* ctx, double dt, int nsteps, int ckpt_every, const double* wgt2d,
                      size_t n2, double* J, double* const* grad, size_t n);
 /* The same with q_v as a sixth field (u, v, w, theta', pi', q_v): with
- * moisture on (and microphysics off, flux-form transport without the
- * positive-definite limiter, q_c = q_r = 0) q_v is transported, adds
+ * moisture on (microphysics off, q_c = q_r = 0) q_v is transported (flux
+ * form, or positive-definite with the limiter's branches frozen), adds
  * g 0.61 q_v' to the w buoyancy and scales theta_v by (1 + 0.61 q_v), and is
  * linearised with the rest; without moisture the q_v input is ignored and
  * its output is zero. The five-field entries above are these with q_v

@@ -447,7 +447,7 @@ def test_moist_scope_is_enforced():
     from wfe_amd import Integrator, WfeError
     from wfe_amd.adjoint import tl_step
     for over, msg in (({"microphysics": "on"}, "microphysics"),
-                      ({"pd_moist": "on"}, "positive-definite")):
+                      ({"w_damping": "on"}, "w damping")):
         c = moist_case(**over)
         gi = Integrator.from_case(c, "fp64")
         with pytest.raises(WfeError, match=msg):
@@ -610,3 +610,36 @@ def test_open_boundaries_dot_product_and_finite_differences(moist):
     gi = make_moist(c, 51) if moist else make(c, 51)
     flds = FIELDS6 if moist else FIELDS5
     _dot_and_fd(c, gi, flds, lambda: Integrator.from_case(c, "fp64"), 151)
+
+
+@pytest.mark.parametrize("setup", ["periodic", "open-relaxed"])
+def test_pd_transport_dot_product_and_finite_differences(setup):
+    """Positive-definite q_v transport (the limiter's branches frozen at the
+    trajectory): six-field dot product and the TL against centred
+    differences, periodic, and with open boundaries, terrain and
+    relaxation."""
+    from wfe_amd import Integrator
+    kw = dict(pd_moist="on")
+    if setup != "periodic":
+        kw.update(TERRAIN, bc_x="open", bc_y="open")
+    c = moist_case(**kw)
+    assert c.params.pd_moist
+    gi = make_moist(c, 71)
+    # q_v near zero somewhere, so the limiter acts (R < 1) in some cells
+    qv = gi.get("qv")
+    g = c.grid
+    q3 = g.as3d(qv)
+    q3[g.ng + g.nz - 3:g.ng + g.nz, :, :] *= 1e-3
+    gi.set("qv", qv)
+    if setup != "periodic":  # targets from the state the forward runs start from
+        _relax(gi, c, 9)
+
+    def fresh():
+        gp = Integrator.from_case(c, "fp64")
+        if setup != "periodic":
+            for f in FIELDS6:
+                gp.set(f, gi.get(f))
+            _relax(gp, c, 9)
+        return gp
+
+    _dot_and_fd(c, gi, FIELDS6, fresh, 171)

@@ -2,13 +2,12 @@
 torch.autograd.Function for the lowest-level kinetic-energy objective.
 
 Scope (enforced by the library): FP64 or mixed contexts without w damping,
-PD transport, microphysics or surface physics; terrain, map factors, open
-boundaries, Davies relaxation (targets held at the context's current ones),
-Coriolis and constant-K diffusion are included; with
-moisture on, q_v is a
-linearised field (flux-form transport without the positive-definite
-limiter, q_c = q_r = 0). Fields are (u, v, w, thp, pip, qv) in the padded
-layout of the case's grid; qv is zero on output for a dry case. In a mixed context the linearisation is
+microphysics or surface physics. Terrain, map factors, open boundaries,
+Davies relaxation (targets held at the context's current ones), Coriolis and
+constant-K diffusion are included. With moisture on, q_v is a linearised
+field, by flux-form or positive-definite transport, with q_c = q_r = 0.
+Fields are (u, v, w, thp, pip, qv) in the padded layout of the case's grid;
+qv is zero on output for a dry case. In a mixed context the linearisation is
 about the FP32 trajectory the context itself computes; perturbations,
 adjoint variables and the objective are FP64 in both kinds of context.
 """

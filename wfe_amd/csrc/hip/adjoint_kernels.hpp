@@ -222,7 +222,177 @@ struct TanTend {  // tendency perturbations (TL out) or adjoints (AD in)
   int moist;
   double K;     // diffusion coefficient (0: off)
   double* tpi;  // pi' tendency (relaxation only; rim columns)
+  int pd;       // q_v transported by the PD kernels below (k_*_tend skip it)
 };
+
+// Positive-definite q_v transport (k_pd_ratio / k_pd_apply,
+// kernels.cu:187-264): face fluxes F = mf iface5(q, mf); the limiter ratio
+// R = min(1, rt max(0, q) / (dt out)) with out the sum of the outgoing
+// fluxes max(0, +-F)/h of the cell; the renormalised flux F R(donor cell,
+// upwind of F); T_q = (-div(F R) + q div(mf)) / rt and the stage q =
+// q_n + dts T_q. Every branch (flux sign, outgoing or not, q > 0, R < 1,
+// out above its floor) is frozen at the trajectory; R's branch is read from
+// the recorded ratio field.
+template <class S>
+struct PDLin {
+  const Grid& g;
+  const TanTraj<S>& X;
+  __device__ int stride(int dir) const { return dir == 0 ? 1 : (dir == 1 ? g.NX : g.plane); }
+  __device__ double mf(int dir, int c) const {
+    return dir == 0 ? X.mfx[c] : (dir == 1 ? X.mfy[c] : X.mfz[c]);
+  }
+  __device__ double flux(int dir, int c) const {
+    const double f = mf(dir, c);
+    return f * iface_at(X.q, c, stride(dir), f);
+  }
+  __device__ double dflux(const TanPert& D, int dir, int c) const {
+    const double* dm = dir == 0 ? D.mfx : (dir == 1 ? D.mfy : D.mfz);
+    return tl_face(X.q, D.q, c, stride(dir), mf(dir, c), dm[c]);
+  }
+  __device__ void ad_flux(const TanPert& A, int dir, int c, double Fb) const {
+    double* mb = dir == 0 ? A.mfx : (dir == 1 ? A.mfy : A.mfz);
+    atomicAdd(&mb[c], ad_face(X.q, A.q, c, stride(dir), mf(dir, c), Fb));
+  }
+};
+
+// The recorded ratio of a donor cell (ii, jj, kz): lateral map and vertical
+// clamp of the ratio field, as k_pd_apply reads it.
+__device__ __forceinline__ int pd_ridx(const Grid& g, const Params& p, int ii, int jj, int kz) {
+  ii = map_lat(ii, g.nx, g.nx - 1, p.bcx);
+  jj = map_lat(jj, g.ny, g.ny - 1, p.bcy);
+  kz = kz < 0 ? 0 : (kz > g.nz - 1 ? g.nz - 1 : kz);
+  return g.id(ii, jj, kz);
+}
+
+template <class S>
+__global__ void k_tl_pd_ratio(Grid g, Base<double> b, Metric<double> m, double dt,
+                              TanTraj<S> X, TanPert D, Traj<S> R, double* dR) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+  const int c = g.id(i, j, k);
+  const PDLin<S> P{g, X};
+  const double h[3] = {g.dx, g.dy, m.dzeta_c[k + g.ng]};
+  double out = 0.0, dout = 0.0;
+  for (int dir = 0; dir < 3; ++dir) {
+    const int st = P.stride(dir);
+    const double Fh = P.flux(dir, c + st), Fl = P.flux(dir, c);
+    if (Fh > 0.0) {
+      out += Fh / h[dir];
+      dout += P.dflux(D, dir, c + st) / h[dir];
+    }
+    if (Fl < 0.0) {
+      out -= Fl / h[dir];
+      dout -= P.dflux(D, dir, c) / h[dir];
+    }
+  }
+  double d = 0.0;
+  if (dt * out > 1e-20 && R[c] < 1.0) {
+    const double rt = b.rhob[c] * m.jac[g.id2(i, j)];
+    const double q = X.q[c];
+    const double avail = rt * (q > 0.0 ? q : 0.0), davail = q > 0.0 ? rt * D.q[c] : 0.0;
+    const double r = avail / (dt * out);
+    d = (davail - r * dt * dout) / (dt * out);
+  }
+  dR[c] = d;
+}
+
+template <class S>
+__global__ void k_tl_pd_apply(Grid g, Params p, Base<double> b, Metric<double> m, TanTraj<S> X,
+                              TanPert D, Traj<S> R, const double* dR, TanTend out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+  const int c = g.id(i, j, k);
+  const PDLin<S> P{g, X};
+  const double rd[3] = {g.rdx, g.rdy, m.rdzc[k + g.ng]};
+  double dfdiv = 0.0;
+  for (int dir = 0; dir < 3; ++dir) {
+    for (int side = 0; side < 2; ++side) {  // west/south/bottom face, then the opposite
+      const int fi = i + (dir == 0 ? side : 0), fj = j + (dir == 1 ? side : 0),
+                fk = k + (dir == 2 ? side : 0);
+      const int cf = g.id(fi, fj, fk);
+      const double F = P.flux(dir, cf);
+      const int di = fi - (dir == 0 && F >= 0.0), dj = fj - (dir == 1 && F >= 0.0),
+                dk = fk - (dir == 2 && F >= 0.0);
+      const int r = pd_ridx(g, p, di, dj, dk);
+      const double ds = P.dflux(D, dir, cf) * R[r] + F * dR[r];
+      dfdiv += (side ? ds : -ds) * rd[dir];
+    }
+  }
+  const double rt = b.rhob[c] * m.jac[g.id2(i, j)];
+  const double dtq = (-dfdiv + D.q[c] * X.div[c] + X.q[c] * D.div[c]) / rt;
+  out.qout[c] = out.qn[c] + out.dts * dtq;
+}
+
+template <class S>
+__global__ void k_ad_pd_apply(Grid g, Params p, Base<double> b, Metric<double> m, TanTraj<S> X,
+                              TanPert A, Traj<S> R, double* Rb, TanTend in) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+  const int c = g.id(i, j, k);
+  const double Qb = in.qout[c];
+  if (Qb == 0.0) return;
+  atomicAdd(&in.qn[c], Qb);
+  const PDLin<S> P{g, X};
+  const double rt = b.rhob[c] * m.jac[g.id2(i, j)];
+  const double a = in.dts * Qb / rt;
+  atomicAdd(&A.q[c], a * X.div[c]);
+  atomicAdd(&A.div[c], a * X.q[c]);
+  const double rd[3] = {g.rdx, g.rdy, m.rdzc[k + g.ng]};
+  for (int dir = 0; dir < 3; ++dir) {
+    for (int side = 0; side < 2; ++side) {
+      const int fi = i + (dir == 0 ? side : 0), fj = j + (dir == 1 ? side : 0),
+                fk = k + (dir == 2 ? side : 0);
+      const int cf = g.id(fi, fj, fk);
+      const double F = P.flux(dir, cf);
+      const int di = fi - (dir == 0 && F >= 0.0), dj = fj - (dir == 1 && F >= 0.0),
+                dk = fk - (dir == 2 && F >= 0.0);
+      const int r = pd_ridx(g, p, di, dj, dk);
+      const double sb = (side ? -a : a) * rd[dir];  // fdiv enters T_q with a minus sign
+      atomicAdd(&Rb[r], sb * F);
+      P.ad_flux(A, dir, cf, sb * R[r]);
+    }
+  }
+}
+
+template <class S>
+__global__ void k_ad_pd_ratio(Grid g, Base<double> b, Metric<double> m, double dt,
+                              TanTraj<S> X, TanPert A, Traj<S> R, const double* Rb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+  const int c = g.id(i, j, k);
+  const double rb = Rb[c];
+  if (rb == 0.0 || !(R[c] < 1.0)) return;
+  const PDLin<S> P{g, X};
+  const double h[3] = {g.dx, g.dy, m.dzeta_c[k + g.ng]};
+  double out = 0.0;
+  for (int dir = 0; dir < 3; ++dir) {
+    const int st = P.stride(dir);
+    const double Fh = P.flux(dir, c + st), Fl = P.flux(dir, c);
+    if (Fh > 0.0) out += Fh / h[dir];
+    if (Fl < 0.0) out -= Fl / h[dir];
+  }
+  if (!(dt * out > 1e-20)) return;
+  const double rt = b.rhob[c] * m.jac[g.id2(i, j)];
+  const double q = X.q[c];
+  const double avail = rt * (q > 0.0 ? q : 0.0);
+  const double r = avail / (dt * out);
+  // R = avail / (dt out): avail_b = rb / (dt out), out_b = -rb r / out
+  if (q > 0.0) atomicAdd(&A.q[c], rt * rb / (dt * out));
+  const double ob = -rb * r / out;
+  for (int dir = 0; dir < 3; ++dir) {
+    const int st = P.stride(dir);
+    if (P.flux(dir, c + st) > 0.0) P.ad_flux(A, dir, c + st, ob / h[dir]);
+    if (P.flux(dir, c) < 0.0) P.ad_flux(A, dir, c, -ob / h[dir]);
+  }
+}
 
 // Davies relaxation (k_bdy_relax, kernels.cu:292-304) of the stage
 // estimate toward fixed targets: T_f += w (target - f) for u, v, theta' and
@@ -368,7 +538,7 @@ __global__ void k_tl_tend(Grid g, Base<double> b, Metric<double> m, TanTraj<S> X
     if (out.K > 0.0) t += out.K * tl_lap(g, D.th, c, m.rdzc[kk] * m.rdzc[kk] * rj2);
     out.tth[c] = t;
   }
-  if (out.moist) {  // q_v: flux-form transport, then the stage update
+  if (out.moist && !out.pd) {  // q_v: flux-form transport, then the stage update
     const double dfd =
         (tl_face(X.q, D.q, c + 1, 1, X.mfx[c + 1], D.mfx[c + 1]) -
          tl_face(X.q, D.q, c, 1, X.mfx[c], D.mfx[c])) / dx +
@@ -511,7 +681,7 @@ __global__ void k_ad_tend(Grid g, Base<double> b, Metric<double> m, TanTraj<S> X
       atomicAdd(&A.mfz[c], ad_face(X.th, A.th, c, PL, X.mfz[c], -fdb / dzc));
     }
   }
-  if (in.moist) {  // q_v stage update and transport
+  if (in.moist && !in.pd) {  // q_v stage update and transport
     const double Qb = in.qout[c];
     if (Qb != 0.0) {
       atomicAdd(&in.qn[c], Qb);

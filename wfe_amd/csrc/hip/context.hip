@@ -1802,9 +1802,14 @@ class Adjoint final : public AdjointBase {
     set5(xb_);
     set5(tmp_);
     zero_q_ = A.alloc<T>(n);
+    pd_ = moist_ && hp.pd_moist;
     for (int m = 0; m < 3; ++m) {
       tpi_tr_[m] = A.alloc<T>(n);
       dtpi_[m] = A.alloc<double>(n);
+      if (pd_) {
+        ratio_tr_[m] = A.alloc<T>(n);
+        dratio_[m] = A.alloc<double>(n);
+      }
     }
     if (moist_) {
       qc_scr_ = A.alloc<T>(n);
@@ -1824,7 +1829,6 @@ class Adjoint final : public AdjointBase {
     const auto& hp = F.hp_;
     std::string why;
     if (hp.moisture && hp.microphysics) why = "microphysics is on";
-    else if (hp.moisture && hp.pd_moist) why = "positive-definite moisture transport is on";
     else if (hp.diff_K < 0) why = "the diffusion coefficient is negative";
     else if (hp.w_damping) why = "w damping is on";
     else if (F.phys_on_) why = "the surface physics is on";
@@ -1871,6 +1875,7 @@ class Adjoint final : public AdjointBase {
         qo.qr = m == 2 ? nxt.qr : qr_scr_;
       }
       f.slow_tendencies(est_set(m, sn), dt, moist_ && !bdy ? &qo : nullptr, sdt[m], true);
+      if (pd_) copy(ratio_tr_[m], f.pratio_[0], st);
       if (bdy) {
         f.bdy_relax(est_set(m, sn));
         copy(tpi_tr_[m], f.tpi_, st);
@@ -1942,6 +1947,13 @@ class Adjoint final : public AdjointBase {
       }
       dev::k_tl_tend<<<grid3(), dim3(64, 4), 0, st>>>(f.g_, b64_, m64_, traj(m), pert(m),
                                                        tend(m, sdt[m]));
+      if (pd_) {
+        dev::k_tl_pd_ratio<T><<<grid3(), dim3(64, 4), 0, st>>>(
+            f.g_, b64_, m64_, dt, traj(m), pert(m), dev::Traj<T>{ratio_tr_[m]}, dratio_[m]);
+        dev::k_tl_pd_apply<T><<<grid3(), dim3(64, 4), 0, st>>>(
+            f.g_, f.p_, b64_, m64_, traj(m), pert(m), dev::Traj<T>{ratio_tr_[m]}, dratio_[m],
+            tend(m, sdt[m]));
+      }
       if (f.bdy_active_)
         dev::k_tl_relax<T><<<grid3(), dim3(64, 4), 0, st>>>(f.g_, dev::Traj<T>{f.bdy_wgt_},
                                                             pert(m), tend(m, sdt[m]));
@@ -1997,6 +2009,13 @@ class Adjoint final : public AdjointBase {
       if (f.bdy_active_)
         dev::k_ad_relax<T><<<grid3(), dim3(64, 4), 0, st>>>(f.g_, dev::Traj<T>{f.bdy_wgt_},
                                                             pert(m), tend(m, sdt[m]));
+      if (pd_) {
+        dev::k_ad_pd_apply<T><<<grid3(), dim3(64, 4), 0, st>>>(
+            f.g_, f.p_, b64_, m64_, traj(m), pert(m), dev::Traj<T>{ratio_tr_[m]}, dratio_[m],
+            tend(m, sdt[m]));
+        dev::k_ad_pd_ratio<T><<<grid3(), dim3(64, 4), 0, st>>>(
+            f.g_, b64_, m64_, dt, traj(m), pert(m), dev::Traj<T>{ratio_tr_[m]}, dratio_[m]);
+      }
       {
         dim3 grd(cdiv(f.g_.nx + 5, 64), cdiv(f.g_.ny + 5, 4), (unsigned)f.g_.nz);
         dev::k_ad_massflux<<<grd, dim3(64, 4), 0, st>>>(f.g_, m64_, dmf_[m].a, dmf_[m].b,
@@ -2185,7 +2204,7 @@ class Adjoint final : public AdjointBase {
   dev::TanTend tend(int m, double dts) const {
     const Set4& t = dT_[m];
     return dev::TanTend{t.a, t.b, t.c, t.d, dest_[0].q, dsub_[m][nsub_[m] - 1].q, dts,
-                        moist_ ? 1 : 0, F.hp_.diff_K, dtpi_[m]};
+                        moist_ ? 1 : 0, F.hp_.diff_K, dtpi_[m], pd_ ? 1 : 0};
   }
   dim3 grid3() const { return dim3(cdiv(F.g_.nx, 64), cdiv(F.g_.ny, 4), (unsigned)F.g_.nz); }
   dim3 grid_uv() const {
@@ -2212,6 +2231,7 @@ class Adjoint final : public AdjointBase {
   void clear_perturbations() {
     for (int m = 0; m < 3; ++m) {
       zero(dtpi_[m]);
+      if (pd_) zero(dratio_[m]);
       zero5(dest_[m]);
       zero4(dmf_[m]);
       zero4(dT_[m]);
@@ -2330,6 +2350,9 @@ class Adjoint final : public AdjointBase {
   T* zero_q_ = nullptr;
   T* tpi_tr_[3] = {};         // relaxation pi' tendency of each stage
   double* dtpi_[3] = {};      // its perturbation / adjoint
+  bool pd_ = false;           // q_v transported by the PD scheme
+  T* ratio_tr_[3] = {};       // q_v limiter ratio of each stage
+  double* dratio_[3] = {};    // its perturbation / adjoint
   T *qc_scr_ = nullptr, *qr_scr_ = nullptr;
   double* jsum_ = nullptr;
   double* wgt_ = nullptr;
