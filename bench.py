@@ -510,6 +510,106 @@ def adjoint_benchmark(nsteps: int, ckpt: int, moist: bool = False):
     return out
 
 
+def regional_adjoint_benchmark(nsteps: int, ckpt: int):
+    """The regional 330 x 220 x 45 case (Lambert map factors, terrain, open
+    boundaries, Davies relaxation toward one boundary file, PD moisture) with
+    microphysics and w damping off, FP64: one-step dot product, the gradient
+    of the mean lowest-level kinetic energy in a seeded 20 x 20 cell box over
+    nsteps steps (checkpoint every ckpt), a centred-difference check along a
+    seeded direction (eps 1e-6), and the FP32-production gradient. The case's
+    stratosphere is dry (q_v = 0 exactly on about 28% of the points), where
+    the PD limiter sits on its kink at q_v = 0 and the linearisation takes the
+    forward's branch (zero slope); centred differences there average the two
+    sides, so the direction's q_v part and the q_v comparison are also given
+    over the points with q_v > 0."""
+    import shutil
+    import tempfile
+
+    from wfe_amd import Case, Integrator
+    from wfe_amd.adjoint import FIELDS, ad_step, box_weights, ke_gradient, tl_step
+    from wfe_amd.lam_case import write_lam_case
+
+    d = tempfile.mkdtemp(prefix="wfe_radj_")
+    try:
+        ini = write_lam_case(f"{d}/lam", hours=0, extra={"microphysics": "off",
+                                                         "w_damping": "off"})
+        case = Case(ini)
+        g = case.grid
+        dt = case.scalar("dt")
+        rng = np.random.default_rng(20261203)
+        i0, j0 = (int(x) for x in rng.integers(30, min(g.nx, g.ny) - 50, size=2))
+        w = box_weights(g, i0, i0 + 20, j0, j0 + 20)
+        mask = np.zeros((g.NZ, g.NY, g.NX), dtype=bool)
+        mask[g.ng:g.ng + g.nz, g.ng:g.ng + g.ny, g.ng:g.ng + g.nx] = True
+        mask = mask.ravel()
+        scale = {"u": 1.0, "v": 1.0, "w": 0.3, "thp": 0.5, "pip": 1e-4, "qv": 1e-3}
+
+        def rand():
+            return {f: np.where(mask, rng.standard_normal(g.npts) * scale[f], 0.0)
+                    for f in FIELDS}
+
+        def dot(a, b):
+            return sum(float(np.dot(a[f][mask], b[f][mask])) for f in FIELDS)
+
+        gi = Integrator.from_case(case, "fp64")
+        dx, yb = rand(), rand()
+        lhs, rhs = dot(tl_step(gi, dt, dx), yb), dot(dx, ad_step(gi, dt, yb))
+        gi.sync()
+        t0 = time.perf_counter()
+        J, grad = ke_gradient(gi, dt, nsteps, ckpt, w)
+        t_adj = time.perf_counter() - t0
+        x0 = {f: gi.get(f) for f in FIELDS}
+        wet = x0["qv"] > 0.0
+        dd = rand()
+        dd["qv"] = np.where(wet, dd["qv"], 0.0)
+        gd = dot(grad, dd)
+
+        def J_of(xs):
+            gp = Integrator.from_case(case, "fp64")
+            for f in FIELDS:
+                gp.set(f, xs[f])
+            tot, ww = 0.0, w.reshape(g.NY, g.NX)
+            for s in range(nsteps):
+                gp.step(dt, s * dt)
+                u = gp.get("u").reshape(g.NZ, g.NY, g.NX)[g.ng]
+                v = gp.get("v").reshape(g.NZ, g.NY, g.NX)[g.ng]
+                tot += float((ww * 0.5 * (u * u + v * v)).sum()) / nsteps
+            return tot
+
+        t0 = time.perf_counter()
+        J_fwd = J_of(x0)
+        t_fwd = time.perf_counter() - t0
+        eps = 1e-6
+        fd = (J_of({f: x0[f] + eps * dd[f] for f in FIELDS}) -
+              J_of({f: x0[f] - eps * dd[f] for f in FIELDS})) / (2 * eps)
+        g32 = Integrator.from_case(case, "mixed")
+        for f in FIELDS:
+            g32.set(f, x0[f])
+        g32.sync()
+        t0 = time.perf_counter()
+        J32, grad32 = ke_gradient(g32, dt, nsteps, ckpt, w)
+        t_adj32 = time.perf_counter() - t0
+        gerr = {f: float(np.linalg.norm(grad32[f][mask] - grad[f][mask]) /
+                         np.linalg.norm(grad[f][mask])) for f in FIELDS}
+        mw = mask & wet
+        gerr["qv_where_qv_positive"] = float(np.linalg.norm(grad32["qv"][mw] - grad["qv"][mw]) /
+                                             np.linalg.norm(grad["qv"][mw]))
+        return {"workload": f"regional 330x220x45 (terrain, Lambert map factors, open "
+                            f"boundaries, Davies relaxation, PD moisture; microphysics and "
+                            f"w damping off), FP64; J = mean lowest-level KE in box "
+                            f"i {i0}..{i0 + 20}, j {j0}..{j0 + 20}",
+                "steps": nsteps, "checkpoint_every": ckpt, "J": J,
+                "J_forward_check_rel": abs(J - J_fwd) / abs(J_fwd),
+                "dot_product_rel": abs(lhs - rhs) / abs(lhs),
+                "dry_fraction": float(1.0 - wet[mask].mean()),
+                "gradient_fd_rel_err_eps_1e-6": abs(fd - gd) / abs(gd),
+                "adjoint_s": t_adj, "forward_s": t_fwd, "adjoint_over_forward": t_adj / t_fwd,
+                "fp32_production": {"adjoint_s": t_adj32, "J_rel_vs_fp64": abs(J32 - J) / abs(J),
+                                    "gradient_rel_l2_vs_fp64": gerr}}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def cpu_model() -> str:
     try:
         for line in Path("/proc/cpuinfo").read_text().splitlines():
@@ -651,6 +751,8 @@ def main() -> int:
         adj = adjoint_benchmark(args.adjoint_steps, 50)
         log(f"[bench] adjoint: moist 120x120x40 (q_v), {args.adjoint_steps} steps")
         adj["moist"] = adjoint_benchmark(args.adjoint_steps, 50, moist=True)
+        log("[bench] adjoint: regional 330x220x45, 60 steps")
+        adj["regional"] = regional_adjoint_benchmark(60, 20)
 
     alg = ALG_BYTES_STEP["regional"]
     out = {
