@@ -140,8 +140,8 @@ int wfe_set_physics(wfe_ctx* ctx, const wfe_sfc* sfc, size_t n2);
 
 /* ---- tangent-linear and adjoint of the large step ----
  * No counterpart in the reference. Scope: FP64 or mixed contexts without
- * w damping, microphysics or surface physics (WFE_E_CONFIG otherwise);
- * terrain, map factors, PD q_v transport (limiter branches frozen at the
+ * microphysics or surface physics (WFE_E_CONFIG otherwise); terrain, map
+ * factors, w damping, PD q_v transport (limiter branches frozen at the
  * trajectory), open boundaries (radiation, branch
  * frozen at the trajectory), Davies relaxation (targets and time fraction
  * held at the context's current values over a gradient's steps), Coriolis

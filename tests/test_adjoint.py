@@ -225,9 +225,9 @@ def test_scope_is_enforced():
     from wfe_amd import Integrator, WfeError
     from wfe_amd.adjoint import tl_step
     for precision in ("fp64", "mixed"):
-        c = dry_case(w_damping="on")
+        c = dry_case(acoustic_ns=17)
         gi = Integrator.from_case(c, precision)
-        with pytest.raises(WfeError, match="w damping"):
+        with pytest.raises(WfeError, match="more than 16 acoustic substeps"):
             tl_step(gi, c.scalar("dt"), {})
 
 
@@ -447,7 +447,7 @@ def test_moist_scope_is_enforced():
     from wfe_amd import Integrator, WfeError
     from wfe_amd.adjoint import tl_step
     for over, msg in (({"microphysics": "on"}, "microphysics"),
-                      ({"w_damping": "on"}, "w damping")):
+                      ({"acoustic_ns": 17}, "more than 16 acoustic substeps")):
         c = moist_case(**over)
         gi = Integrator.from_case(c, "fp64")
         with pytest.raises(WfeError, match=msg):
@@ -643,3 +643,25 @@ def test_pd_transport_dot_product_and_finite_differences(setup):
         return gp
 
     _dot_and_fd(c, gi, FIELDS6, fresh, 171)
+
+
+def test_w_damping_dot_product_and_finite_differences():
+    """Vertical-Courant damping of w on its active branch: a thin-layer grid
+    (dz = 60 m) with a few strong updrafts, so that C = |w| dt / dz exceeds 1
+    in some cells."""
+    from wfe_amd import Integrator
+    c = moist_case(w_damping="on", dz=60, rayleigh_zd=700, bubble_zc=400, bubble_zr=300)
+    g = c.grid
+    gi = make_moist(c, 81)
+    w = gi.get("w")
+    w3 = g.as3d(w)
+    w3[g.ng + 3:g.ng + 8, g.ng + 8:g.ng + 11, g.ng + 10:g.ng + 13] += 45.0
+    gi.set("w", w)
+    dt = c.scalar("dt")
+    assert (np.abs(w3[g.ng + 1:g.ng + g.nz]) * dt / 60.0 > 1.0).sum() >= 20
+
+    def fresh():
+        gp = Integrator.from_case(c, "fp64")
+        return gp
+
+    _dot_and_fd(c, gi, FIELDS6, fresh, 181)

@@ -1,8 +1,8 @@
 """Tangent-linear and adjoint of the large step through the C ABI, and a
 torch.autograd.Function for the lowest-level kinetic-energy objective.
 
-Scope (enforced by the library): FP64 or mixed contexts without w damping,
-microphysics or surface physics. Terrain, map factors, open boundaries,
+Scope (enforced by the library): FP64 or mixed contexts without
+microphysics or surface physics. Terrain, map factors, w damping, open boundaries,
 Davies relaxation (targets held at the context's current ones), Coriolis and
 constant-K diffusion are included. With moisture on, q_v is a linearised
 field, by flux-form or positive-definite transport, with q_c = q_r = 0.

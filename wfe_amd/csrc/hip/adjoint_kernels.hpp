@@ -1,11 +1,10 @@
 #pragma once
-// Tangent-linear (TL) and adjoint (AD) kernels of the dry large step in FP64:
+// Tangent-linear (TL) and adjoint (AD) kernels of the large step in FP64:
 // ghost fill, mass fluxes and divergence, flux-form advection tendencies with
 // the Rayleigh layer, the horizontal acoustic step and the vertically
 // implicit w-pi' column solve. They linearise the forward kernels around a
 // stored trajectory, with the upwind branch of every interface value frozen
-// at the trajectory's mass flux. Scope: periodic lateral boundaries, no
-// vertical-Courant damping. Terrain enters through the Jacobian J, the
+// at the trajectory's mass flux. Terrain enters through the Jacobian J, the
 // cross-metric vertical mass flux, the terrain pressure-gradient term of the
 // horizontal acoustic step, S_v in the column's pi' equation and the
 // diagnostic surface w(0) = u h_x + v h_y. Coriolis (any f(x, y)) and
@@ -13,10 +12,11 @@
 // their transposes. Open lateral boundaries take the radiation condition on
 // their edge faces, with the inflow/outflow branch frozen at the trajectory;
 // Davies relaxation toward fixed targets adds -w df to the tendencies and
-// sets the pi' tendency to -w dpi' in the rim. With moisture on (no
-// microphysics, flux-form transport without the positive-definite limiter,
-// q_c = q_r = 0), q_v is a sixth linearised field: its transport and stage
-// update, the moist buoyancy in T_w and the virtual-temperature factor
+// sets the pi' tendency to -w dpi' in the rim; the vertical-Courant damping
+// of w enters on its active branch. With moisture on (no microphysics,
+// q_c = q_r = 0), q_v is a sixth linearised field: its transport (flux form,
+// or positive-definite with the limiter's branches frozen) and stage update,
+// the moist buoyancy in T_w and the virtual-temperature factor
 // (1 + 0.61 q_v) of the horizontal pressure gradient and of the column's
 // theta_v at w levels.
 //
@@ -223,7 +223,16 @@ struct TanTend {  // tendency perturbations (TL out) or adjoints (AD in)
   double K;     // diffusion coefficient (0: off)
   double* tpi;  // pi' tendency (relaxation only; rim columns)
   int pd;       // q_v transported by the PD kernels below (k_*_tend skip it)
+  // vertical-Courant damping of w (k_tendencies): where C = |w| dt / (dzeta J)
+  // exceeds 1, T_w gets -0.3 (C - 1) w / dt; its slope -0.3 (2 C - 1) / dt
+  double dtl;   // the large step dt
+  int wdamp;
 };
+
+__device__ __forceinline__ double wdamp_slope(double wk, double a, double dtl) {
+  const double cr = fabs(wk) * a;  // a = dt / (dzeta_w J)
+  return cr > 1.0 ? -0.3 * (2.0 * cr - 1.0) / dtl : 0.0;
+}
 
 // Positive-definite q_v transport (k_pd_ratio / k_pd_apply,
 // kernels.cu:187-264): face fluxes F = mf iface5(q, mf); the limiter ratio
@@ -521,6 +530,7 @@ __global__ void k_tl_tend(Grid g, Base<double> b, Metric<double> m, TanTraj<S> X
     const double dv = 0.5 * (X.div[c - PL] + X.div[c]), ddv = 0.5 * (D.div[c - PL] + D.div[c]);
     const double rw = b.rhobw[c] * m.jac[c2];
     double t = (-dfd + D.w[c] * dv + X.w[c] * ddv) / rw - m.ray_w[kk] * D.w[c];
+    if (out.wdamp) t += wdamp_slope(X.w[c], out.dtl * m.rdzw[kk] / m.jac[c2], out.dtl) * D.w[c];
     if (out.K > 0.0) t += out.K * tl_lap(g, D.w, c, m.rdzw[kk] * m.rdzw[kk] * rj2);
     // moist buoyancy g 0.61 (q_v at the w level - base) from time-n q_v
     if (out.moist) t += kGrav * kEps61 * 0.5 * (out.qn[c - PL] + out.qn[c]);
@@ -639,6 +649,8 @@ __global__ void k_ad_tend(Grid g, Base<double> b, Metric<double> m, TanTraj<S> X
       const double dzw = m.dzeta_w[kk];
       const double rw = b.rhobw[c] * m.jac[c2];
       if (in.K > 0.0) ad_lap(g, A.w, c, m.rdzw[kk] * m.rdzw[kk] * rj2, in.K * T);
+      if (in.wdamp)
+        atomicAdd(&A.w[c], wdamp_slope(X.w[c], in.dtl * m.rdzw[kk] / m.jac[c2], in.dtl) * T);
       if (in.moist) {
         const double qb = kGrav * kEps61 * 0.5 * T;
         atomicAdd(&in.qn[c - PL], qb);

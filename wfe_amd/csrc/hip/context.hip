@@ -1820,9 +1820,9 @@ class Adjoint final : public AdjointBase {
     widen_constants();
   }
 
-  // The linearised model covers the step without microphysics, w damping,
-  // PD transport or surface physics: dry, or with q_v transported and acting
-  // through buoyancy and theta_v. The context can change after the adjoint
+  // The linearised model covers the step without microphysics or surface
+  // physics: dry, or with q_v transported and acting through buoyancy and
+  // theta_v. The context can change after the adjoint
   // was set up (relaxation switched on by wfe_set_bdy_weights, the state
   // overwritten), so every entry checks again and follows the context.
   void check_scope() const {
@@ -1830,7 +1830,6 @@ class Adjoint final : public AdjointBase {
     std::string why;
     if (hp.moisture && hp.microphysics) why = "microphysics is on";
     else if (hp.diff_K < 0) why = "the diffusion coefficient is negative";
-    else if (hp.w_damping) why = "w damping is on";
     else if (F.phys_on_) why = "the surface physics is on";
     else if (F.g_.nz + 1 > 64) why = "more than 63 levels";
     else if (std::max(1, hp.acoustic_ns) > kMaxSub) why = "more than 16 acoustic substeps";
@@ -1926,6 +1925,7 @@ class Adjoint final : public AdjointBase {
   // dy = TL(dx) around the recorded step (device buffers).
   void tl(double dt, const Set5& dx, const Set5& dy) {
     Ctx<T>& f = F;
+    dt_ = dt;
     hipStream_t st = f.stream_;
     const double sdt[3] = {dt / 3, dt / 2, dt};
     // Cells the TL kernels read but never write (the rigid-lid w levels,
@@ -1982,6 +1982,7 @@ class Adjoint final : public AdjointBase {
   // xb = AD(yb) around the recorded step (device buffers; yb is read only).
   void ad(double dt, const Set5& yb, const Set5& xb) {
     Ctx<T>& f = F;
+    dt_ = dt;
     hipStream_t st = f.stream_;
     const double sdt[3] = {dt / 3, dt / 2, dt};
     clear_perturbations();
@@ -2204,7 +2205,8 @@ class Adjoint final : public AdjointBase {
   dev::TanTend tend(int m, double dts) const {
     const Set4& t = dT_[m];
     return dev::TanTend{t.a, t.b, t.c, t.d, dest_[0].q, dsub_[m][nsub_[m] - 1].q, dts,
-                        moist_ ? 1 : 0, F.hp_.diff_K, dtpi_[m], pd_ ? 1 : 0};
+                        moist_ ? 1 : 0, F.hp_.diff_K, dtpi_[m], pd_ ? 1 : 0, dt_,
+                        F.hp_.w_damping ? 1 : 0};
   }
   dim3 grid3() const { return dim3(cdiv(F.g_.nx, 64), cdiv(F.g_.ny, 4), (unsigned)F.g_.nz); }
   dim3 grid_uv() const {
@@ -2351,6 +2353,7 @@ class Adjoint final : public AdjointBase {
   T* tpi_tr_[3] = {};         // relaxation pi' tendency of each stage
   double* dtpi_[3] = {};      // its perturbation / adjoint
   bool pd_ = false;           // q_v transported by the PD scheme
+  double dt_ = 0.0;           // large step of the current TL / AD pass
   T* ratio_tr_[3] = {};       // q_v limiter ratio of each stage
   double* dratio_[3] = {};    // its perturbation / adjoint
   T *qc_scr_ = nullptr, *qr_scr_ = nullptr;
