@@ -512,8 +512,8 @@ def adjoint_benchmark(nsteps: int, ckpt: int, moist: bool = False):
 
 def regional_adjoint_benchmark(nsteps: int, ckpt: int):
     """The regional 330 x 220 x 45 case (Lambert map factors, terrain, open
-    boundaries, Davies relaxation toward one boundary file, PD moisture) with
-    microphysics and w damping off, FP64: one-step dot product, the gradient
+    boundaries, Davies relaxation toward one boundary file, PD moisture, w
+    damping) with microphysics off, FP64: one-step dot product, the gradient
     of the mean lowest-level kinetic energy in a seeded 20 x 20 cell box over
     nsteps steps (checkpoint every ckpt), a centred-difference check along a
     seeded direction (eps 1e-6), and the FP32-production gradient. The case's
@@ -531,8 +531,7 @@ def regional_adjoint_benchmark(nsteps: int, ckpt: int):
 
     d = tempfile.mkdtemp(prefix="wfe_radj_")
     try:
-        ini = write_lam_case(f"{d}/lam", hours=0, extra={"microphysics": "off",
-                                                         "w_damping": "off"})
+        ini = write_lam_case(f"{d}/lam", hours=0, extra={"microphysics": "off"})
         case = Case(ini)
         g = case.grid
         dt = case.scalar("dt")
@@ -595,8 +594,8 @@ def regional_adjoint_benchmark(nsteps: int, ckpt: int):
         gerr["qv_where_qv_positive"] = float(np.linalg.norm(grad32["qv"][mw] - grad["qv"][mw]) /
                                              np.linalg.norm(grad["qv"][mw]))
         return {"workload": f"regional 330x220x45 (terrain, Lambert map factors, open "
-                            f"boundaries, Davies relaxation, PD moisture; microphysics and "
-                            f"w damping off), FP64; J = mean lowest-level KE in box "
+                            f"boundaries, Davies relaxation, PD moisture, w damping; "
+                            f"microphysics off), FP64; J = mean lowest-level KE in box "
                             f"i {i0}..{i0 + 20}, j {j0}..{j0 + 20}",
                 "steps": nsteps, "checkpoint_every": ckpt, "J": J,
                 "J_forward_check_rel": abs(J - J_fwd) / abs(J_fwd),

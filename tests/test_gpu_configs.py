@@ -380,7 +380,7 @@ def test_adjoint_gradient_against_finite_differences_of_the_oracle(moist, extra)
 @pytest.mark.parametrize("pd", ["off", "on"], ids=["flux-form-q", "pd-q"])
 def test_regional_adjoint_against_finite_differences_of_the_oracle(tmp_path, pd):
     """A small regional case in file mode (Lambert map factors, terrain, open
-    boundaries, 5-point Davies relaxation toward one boundary file, WK82-like
+    boundaries, 5-point Davies relaxation toward one boundary file, w damping,
     moisture from the file), microphysics off, q_v by flux-form or
     positive-definite transport: the adjoint's gradient along a full
     direction and along q_v alone against centred differences of the FP64
@@ -390,7 +390,7 @@ def test_regional_adjoint_against_finite_differences_of_the_oracle(tmp_path, pd)
     from wfe_amd.lam_case import write_lam_case
     ini = write_lam_case(tmp_path / "lam", nx=24, ny=18, nz=16, dx=12000.0, top=12000.0,
                          seed=7, hours=0,
-                         extra={"pd_moist": pd, "microphysics": "off", "w_damping": "off"})
+                         extra={"pd_moist": pd, "microphysics": "off"})
     c = Case(ini)
     g = c.grid
     assert c.params.bc_x_open and c.params.bc_y_open and c.params.moisture
