@@ -665,3 +665,25 @@ def test_w_damping_dot_product_and_finite_differences():
         return gp
 
     _dot_and_fd(c, gi, FIELDS6, fresh, 181)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_everything_in_scope_dot_product(precision):
+    """All linearised terms at once -- terrain, open edges, relaxation, PD
+    q_v, w damping, Coriolis and diffusion -- in FP64 and about a mixed
+    context's FP32 trajectory (widened constants for every term)."""
+    from wfe_amd.adjoint import ad_step, tl_step
+    c = moist_case(**TERRAIN, bc_x="open", bc_y="open", pd_moist="on", w_damping="on",
+                   coriolis_f=1e-3, diff_K=15.0)
+    g = c.grid
+    gi = make_moist(c, 91, precision)
+    _relax(gi, c, 11, interior=1e-4)
+    rng = np.random.default_rng(191)
+    mask = interior_mask(g)
+    dx, yb = random6(g, rng), random6(g, rng)
+    dt = c.scalar("dt")
+    dy = tl_step(gi, dt, dx)
+    xb = ad_step(gi, dt, yb)
+    lhs, rhs = dot6(dy, yb, mask), dot6(dx, xb, mask)
+    print(precision, "dot product", lhs, rhs, abs(lhs - rhs) / abs(lhs))
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
