@@ -189,6 +189,11 @@ int wfe_adj_gradient_q(wfe_ctx* ctx, double dt, int nsteps, int ckpt_every,
  * together or stepped the same number of times) and advances all of them by
  * one large step per wfe_ensemble_step, member q with dt[q]: one hipGraph
  * holds every member's step as an independent branch, one launch per step.
+ * Batched substeps (one launch of each acoustic substep kernel over every
+ * member's grid) are switched off; WFE_ENSEMBLE_BATCH=1 makes
+ * wfe_ensemble_create fail with WFE_E_ARG. A member with surface physics
+ * (wfe_set_physics) makes wfe_ensemble_step fail with WFE_E_ARG: the
+ * physics needs the time of each step, which the ensemble does not carry.
  * The first call runs each member's first (tuning) step eagerly. Work issued
  * afterwards on a member context (download, absmax, its own wfe_step) is
  * ordered after the ensemble step. Destroy the ensemble before its members. */
@@ -197,6 +202,9 @@ int wfe_ensemble_create(wfe_ctx* const* members, int n, wfe_ensemble** out);
 int wfe_ensemble_destroy(wfe_ensemble* e);
 int wfe_ensemble_step(wfe_ensemble* e, const double* dt);
 int wfe_ensemble_sync(wfe_ensemble* e);
+/* How many members the last captured ensemble step advanced in batched
+ * substep launches (0 while batching is switched off). */
+int wfe_ensemble_batched(const wfe_ensemble* e, int* members);
 
 /* Field3D::upload / download (src/core/field3d.hpp:42-47) of the time-n
  * state (Integrator::state(), integrator.hpp:24), of the slow tendencies, of

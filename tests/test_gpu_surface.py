@@ -119,3 +119,17 @@ def test_driver_writes_the_surface_fields(tmp_path):
         assert (out / f"{f}_000010.bin").stat().st_size == 4 * 40 * 30
     t2 = np.fromfile(out / "t2m_000010.bin", dtype=np.float32)
     assert np.isfinite(t2).all() and 200 < t2.min() and t2.max() < 330
+
+
+def test_ensembles_refuse_members_with_surface_physics(tmp_path):
+    """An ensemble step carries no time of day, which the surface physics
+    needs for each step; such members are refused rather than stepped with
+    a stale clock."""
+    from wfe_amd import Case, Ensemble, Integrator
+    from wfe_amd._lib import WfeError
+    c = Case(_case(tmp_path, "local", hours=0))
+    members = [Integrator.from_case(c, "mixed") for _ in range(2)]
+    ens = Ensemble(members)
+    with pytest.raises(WfeError, match="surface physics"):
+        ens.step([c.scalar("dt")] * 2)
+    del ens

@@ -253,7 +253,8 @@ class Integrator:
 
 class Ensemble:
     """Members of any grid size advancing together, one hipGraph launch per
-    step (wfe_ensemble_*). Keeps references to its members."""
+    step (wfe_ensemble_*); compatible members share one launch per acoustic
+    substep kernel. Keeps references to its members."""
 
     def __init__(self, members: list[Integrator]):
         self.members = list(members)
@@ -273,3 +274,10 @@ class Ensemble:
 
     def sync(self) -> None:
         check(lib().wfe_ensemble_sync(self._h), "wfe_ensemble_sync")
+
+    @property
+    def batched_members(self) -> int:
+        """Members advanced in batched substep launches by the last captured step."""
+        n = C.c_int()
+        check(lib().wfe_ensemble_batched(self._h, C.byref(n)), "wfe_ensemble_batched")
+        return n.value

@@ -179,14 +179,14 @@ __global__ void k_acou_uv(Grid g, Params p, Base<T> b, Metric<T> m, UVArgs<T> a)
 // recomputing them for every face that touches the cell. The radiation
 // edges and the periodic wrap u(nx) = u(0) go through the direct UVDev
 // path, which evaluates the same expressions.
-template <class T, int BX = 64>
-__global__ void __launch_bounds__(256) k_acou_uv_tile(Grid g, Params p, Base<T> b, Metric<T> m,
-                                                      UVArgs<T> a) {
+// The tile of logical block bid (x: BX faces along x, y: BY rows, z: level).
+template <class T, int BX>
+__device__ __forceinline__ void uv_tile(const Grid& g, const Params& p, const Base<T>& b,
+                                        const Metric<T>& m, const UVArgs<T>& a, BlockId bid) {
   constexpr int BY = 256 / BX;
   __shared__ double sP[3][BY + 1][BX + 1];  // pi* at k-1, k, k+1
   __shared__ T sT[BY + 1][BX + 1];
   const int tx = threadIdx.x, ty = threadIdx.y;
-  const BlockId bid = xcd_block();
   const int i0 = bid.x * BX, j0 = bid.y * BY;
   const int k = bid.z;
   const UVDev<T> d{g, p, b, m, a};
@@ -304,6 +304,12 @@ __global__ void __launch_bounds__(256) k_acou_uv_tile(Grid g, Params p, Base<T> 
     }
     a.v_dst[c] = vn;
   }
+}
+
+template <class T, int BX = 64>
+__global__ void __launch_bounds__(256) k_acou_uv_tile(Grid g, Params p, Base<T> b, Metric<T> m,
+                                                      UVArgs<T> a) {
+  uv_tile<T, BX>(g, p, b, m, a, xcd_block());
 }
 
 template <class T>
@@ -937,6 +943,70 @@ __global__ void __launch_bounds__(TX * NW, WPE) k_acou_wpi_tile(Grid g, Params p
   const int i = col ? q - j * g.nx : 0;
   const UVFromArrays<T> uv{a.u, a.v, g.id(i, j, 0), g.plane, g.NX};
   wpi_column<T, TX, NW, KPT, MOIST, TPI>(g, p, b, m, cf, a, lds, lx, ry, col, i, j, uv);
+}
+
+// ---------------------------------------------------------------------------
+// Member-batched substep pair: one launch of k_acou_uv_batch and one of
+// k_acou_wpi_batch advance the same substep of every member of an ensemble,
+// whatever the members' grid sizes. Each member's arguments and the range
+// of logical blocks it owns sit in a device table, in ascending ranges; a
+// workgroup finds its member from its logical block id (after the XCD remap
+// over the whole launch) and runs the single-member tile on that member's
+// arrays, so every cell sees exactly the arithmetic of k_acou_uv_tile and
+// k_acou_wpi_tile.
+template <class T>
+struct BatchMember {
+  Grid g;
+  Params p;
+  Base<T> b;
+  Metric<T> m;
+  Coef<T> cf;
+  UVArgs<T> uv;
+  WPArgs<T> wp;
+  int uv_begin, uv_gx, uv_gz;  // u/v tiles: [uv_begin, uv_begin + gx * gy * gz)
+  int wp_begin;                // column tiles: [wp_begin, next member's wp_begin)
+};
+
+// The member whose range holds logical block l: the last one starting at or
+// before l (binary search over the ascending starts).
+template <class T, bool UV>
+__device__ __forceinline__ int batch_member(const BatchMember<T>* __restrict__ tab, int n, int l) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    const int start = UV ? tab[mid].uv_begin : tab[mid].wp_begin;
+    if (start <= l)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+template <class T, int BX>
+__global__ void __launch_bounds__(256) k_acou_uv_batch(const BatchMember<T>* __restrict__ tab,
+                                                       int n) {
+  const int l = xcd_remap(blockIdx.x, gridDim.x);
+  const BatchMember<T>& M = tab[batch_member<T, true>(tab, n, l)];
+  const int r = l - M.uv_begin;
+  const int t = r / M.uv_gx;
+  uv_tile<T, BX>(M.g, M.p, M.b, M.m, M.uv, BlockId{r - t * M.uv_gx, t / M.uv_gz, t % M.uv_gz});
+}
+
+template <class T, int TX, int NW, int KPT, bool MOIST, bool TPI>
+__global__ void __launch_bounds__(TX * NW) k_acou_wpi_batch(const BatchMember<T>* __restrict__ tab,
+                                                            int n) {
+  extern __shared__ double lds[];
+  const int l = xcd_remap(blockIdx.x, gridDim.x);
+  const BatchMember<T>& M = tab[batch_member<T, false>(tab, n, l)];
+  const Grid& g = M.g;
+  const int lx = threadIdx.x, ry = threadIdx.y;
+  const int q = (l - M.wp_begin) * TX + lx;  // interior column j nx + i
+  const bool col = q < g.nx * g.ny;
+  const int j = col ? q / g.nx : 0;
+  const int i = col ? q - j * g.nx : 0;
+  const UVFromArrays<T> uv{M.wp.u, M.wp.v, g.id(i, j, 0), g.plane, g.NX};
+  wpi_column<T, TX, NW, KPT, MOIST, TPI>(g, M.p, M.b, M.m, M.cf, M.wp, lds, lx, ry, col, i, j, uv);
 }
 
 // ---------------------------------------------------------------------------

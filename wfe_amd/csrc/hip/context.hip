@@ -258,6 +258,7 @@ struct CtxBase {
   virtual void flip_parity() = 0;
   virtual unsigned generation() const = 0;
   virtual void enqueue_step(double dt) = 0;
+  virtual bool surface_physics() const = 0;
 };
 
 template <class T>
@@ -330,6 +331,44 @@ class Ctx final : public CtxBase {
     step_launches(dt);
     cur_ = par;
   }
+
+  // The pieces of a step (step_launches), for the member-batched ensemble
+  // step. Stage m reads stage_in(m) and writes stage_out(m); the time-n
+  // state is S_[cur_].
+  int stage_nsub(int m) const {
+    const int ns = std::max(1, hp_.acoustic_ns);
+    return m == 0 ? 1 : m == 1 ? std::max(1, ns / 2) : ns;
+  }
+  static double stage_dt(double dt, int m) { return m == 0 ? dt / 3 : m == 1 ? dt / 2 : dt; }
+  const StateSet<T>& stage_in(int m) const { return m == 0 ? S_[cur_] : SE_[m - 1]; }
+  const StateSet<T>& stage_out(int m) const { return m == 2 ? S_[cur_ ^ 1] : SE_[m]; }
+  // Without boundary relaxation (which adds to T_q after the tendencies)
+  // the moisture stage update is applied where T_q is formed.
+  bool fuse_q() const { return p_.moisture && !bdy_active_; }
+  // Ghost fill, slow tendencies and relaxation of stage m.
+  void stage_pre(double dt, int m) {
+    const StateSet<T>& est = stage_in(m);
+    fill_ghosts(est);
+    slow_tendencies(est, dt, fuse_q() ? &stage_out(m) : nullptr, stage_dt(dt, m), true);
+    if (bdy_active_) bdy_relax(est);
+  }
+  // The moisture stage update of stage m where it is not fused.
+  void stage_post(double dt, int m) {
+    if (p_.moisture && !fuse_q()) stage_moisture(stage_out(m), stage_dt(dt, m));
+  }
+  // Kessler and the surface physics on the new state S_[cur_].
+  void step_post(double dt) {
+    if (p_.moisture && p_.microphysics) kessler_step(S_[cur_], dt);
+    if (phys_on_) surface_step(S_[cur_], dt);  // integrator.cpp:109
+  }
+  // This member's entry for substep s of stage m in a batched launch.
+  void batch_member(dev::BatchMember<T>& M, double dt, int m, int s) const;
+  int uv_tile_width() const { return lv_bx_; }
+  const dev::Grid& grid() const { return g_; }
+  int levels() const { return g_.nz; }
+  bool relaxing() const { return bdy_active_; }
+  bool moist() const { return p_.moisture != 0; }
+  bool surface_physics() const override { return phys_on_; }
   std::string kernel_report() override {
     WFE_HIP(hipStreamSynchronize(stream_));
     std::string r = timer_.report();
@@ -351,6 +390,7 @@ class Ctx final : public CtxBase {
                        double dt_stage = 0, bool fold_buoyancy = false);
   void bdy_relax(const StateSet<T>& est);
   void acoustic_stage(const StateSet<T>& out, double dtau, int nsub);
+  dev::FusedArgs<T> substep_args(const StateSet<T>& out, double dtau, int s, int nsub) const;
   void stage_moisture(const StateSet<T>& out, double dt_stage);
   void kessler_step(const StateSet<T>& s, double dt);
   T* field_ptr(int field);
@@ -1145,6 +1185,19 @@ dev::WPArgs<T> split_wp_impl(const dev::FusedArgs<T>& a) {
 }
 
 template <class T>
+void Ctx<T>::batch_member(dev::BatchMember<T>& M, double dt, int m, int s) const {
+  const int nsub = stage_nsub(m);
+  const dev::FusedArgs<T> a = substep_args(stage_out(m), stage_dt(dt, m) / nsub, s, nsub);
+  M.g = g_;
+  M.p = p_;
+  M.b = base_;
+  M.m = met_;
+  M.cf = coef_;
+  M.uv = a.uv;
+  M.wp = split_wp_impl<T>(a);
+}
+
+template <class T>
 template <int TX, int NW, int KPT, int WPE>
 void Ctx<T>::add_split_tile(const char* name, size_t cap) {
   using Tile = dev::WpiTile<TX, NW, KPT>;
@@ -1383,35 +1436,42 @@ void Ctx<T>::tune_acoustic(double dtau) {
 // u, v ping-pong; w is updated in place (own column only).
 template <class T>
 void Ctx<T>::acoustic_stage(const StateSet<T>& out, double dtau, int nsub) {
-  const StateSet<T>& sn = S_[cur_];
   if (!tuned_) tune_acoustic(dtau);
-  for (int s = 0; s < nsub; ++s) {
-    const bool first = s == 0, last = s == nsub - 1;
-    dev::FusedArgs<T> a{};
-    a.uv.u_src = first ? sn.u : wu_[(s - 1) & 1];
-    a.uv.v_src = first ? sn.v : wv_[(s - 1) & 1];
-    a.uv.u_dst = last ? out.u : wu_[s & 1];
-    a.uv.v_dst = last ? out.v : wv_[s & 1];
-    a.uv.pcur = first ? sn.pip : xpip_[(s - 1) % 3];
-    a.uv.pprev = s <= 1 ? sn.pip : xpip_[(s - 2) % 3];
-    a.uv.pn = sn.pip;
-    a.uv.thp_cur = first ? sn.thp : xthp_[(s - 1) & 1];
-    a.uv.thp_n = sn.thp;
-    a.uv.qv = sn.qv;
-    a.uv.tu = tu_;
-    a.uv.tv = tv_;
-    a.uv.dtau = (T)dtau;
-    a.w_src = first ? sn.w : ww_;
-    a.w_dst = last ? out.w : ww_;
-    a.pip_dst = last ? out.pip : xpip_[s % 3];
-    a.thp_dst = last ? out.thp : xthp_[s & 1];
-    a.tw = tw_sub_ ? tw_sub_ : tw_;
-    a.tth = tth_;
-    a.tpi = bdy_active_ ? tpi_ : nullptr;
-    a.tpw = bdy_wgt_;
-    methods_[method_].run(a);
-  }
+  for (int s = 0; s < nsub; ++s) methods_[method_].run(substep_args(out, dtau, s, nsub));
   check_launch("acoustic_stage");
+}
+
+// The buffers of substep s of nsub: u, v and theta' ping-pong, pi' rotates
+// over three buffers so that the substep's pi*(pi', pi'prev) inputs are never
+// its output; the last substep writes the stage output.
+template <class T>
+dev::FusedArgs<T> Ctx<T>::substep_args(const StateSet<T>& out, double dtau, int s,
+                                     int nsub) const {
+  const StateSet<T>& sn = S_[cur_];
+  const bool first = s == 0, last = s == nsub - 1;
+  dev::FusedArgs<T> a{};
+  a.uv.u_src = first ? sn.u : wu_[(s - 1) & 1];
+  a.uv.v_src = first ? sn.v : wv_[(s - 1) & 1];
+  a.uv.u_dst = last ? out.u : wu_[s & 1];
+  a.uv.v_dst = last ? out.v : wv_[s & 1];
+  a.uv.pcur = first ? sn.pip : xpip_[(s - 1) % 3];
+  a.uv.pprev = s <= 1 ? sn.pip : xpip_[(s - 2) % 3];
+  a.uv.pn = sn.pip;
+  a.uv.thp_cur = first ? sn.thp : xthp_[(s - 1) & 1];
+  a.uv.thp_n = sn.thp;
+  a.uv.qv = sn.qv;
+  a.uv.tu = tu_;
+  a.uv.tv = tv_;
+  a.uv.dtau = (T)dtau;
+  a.w_src = first ? sn.w : ww_;
+  a.w_dst = last ? out.w : ww_;
+  a.pip_dst = last ? out.pip : xpip_[s % 3];
+  a.thp_dst = last ? out.thp : xthp_[s & 1];
+  a.tw = tw_sub_ ? tw_sub_ : tw_;
+  a.tth = tth_;
+  a.tpi = bdy_active_ ? tpi_ : nullptr;
+  a.tpw = bdy_wgt_;
+  return a;
 }
 
 template <class T>
@@ -1602,26 +1662,13 @@ void Ctx<T>::step(double dt, double t) {
 
 template <class T>
 void Ctx<T>::step_launches(double dt) {
-  const double stage_dt[3] = {dt / 3, dt / 2, dt};
-  const int ns = std::max(1, hp_.acoustic_ns);
-  const int ns_stage[3] = {1, std::max(1, ns / 2), ns};
-  const StateSet<T>& sn = S_[cur_];
-  const StateSet<T>& snext = S_[cur_ ^ 1];
-  // Without boundary relaxation (which adds to T_q after the tendencies)
-  // the moisture stage update is applied where T_q is formed.
-  const bool fuse_q = p_.moisture && !bdy_active_;
   for (int m = 0; m < 3; ++m) {
-    const StateSet<T>& est = m == 0 ? sn : SE_[m - 1];
-    const StateSet<T>& out = m == 2 ? snext : SE_[m];
-    fill_ghosts(est);
-    slow_tendencies(est, dt, fuse_q ? &out : nullptr, stage_dt[m], true);
-    if (bdy_active_) bdy_relax(est);
-    acoustic_stage(out, stage_dt[m] / ns_stage[m], ns_stage[m]);
-    if (p_.moisture && !fuse_q) stage_moisture(out, stage_dt[m]);
+    stage_pre(dt, m);
+    acoustic_stage(stage_out(m), stage_dt(dt, m) / stage_nsub(m), stage_nsub(m));
+    stage_post(dt, m);
   }
   cur_ ^= 1;
-  if (p_.moisture && p_.microphysics) kessler_step(S_[cur_], dt);
-  if (phys_on_) surface_step(S_[cur_], dt);  // integrator.cpp:109
+  step_post(dt);
 }
 
 template <class T>
@@ -2561,11 +2608,21 @@ int wfe_adj_gradient_q(wfe_ctx* ctx, double dt, int nsteps, int ckpt_every,
 }
 
 // ------------------------------------------------------------ ensembles
-// Members advance together: one hipGraph per step holds every member's step
-// as an independent branch (captured fork/join over the members' streams),
-// so the whole ensemble is one launch per step and members' kernels overlap
-// on the device. One graph per parity of the members' state buffers,
-// re-captured when a time step or a member's generation changes.
+// Members advance together: one hipGraph per step (captured fork/join over
+// the members' streams), so the whole ensemble is one launch per step and
+// members' kernels overlap on the device. One graph per parity of the
+// members' state buffers, re-captured when a time step or a member's
+// generation changes.
+//
+// Batched substeps (switched off): members that share a storage type, the
+// substep counts, moisture and boundary relaxation (and have at most 64
+// levels) would form a batch: each of their acoustic substeps one
+// k_acou_uv_batch and one k_acou_wpi_batch launch over all of them, with
+// each member's slow phase of a stage a branch of its own between them.
+// The host path walks cleanly with the HIP calls stood in on the CPU, but
+// its first run on the GPU ended in a host segmentation fault inside the
+// ensemble step whose cause is not found, so WFE_ENSEMBLE_BATCH=1, which
+// asks for it, is refused and every member's step is one branch.
 struct wfe_ensemble {
   std::vector<wfe_ctx*> m;
   hipStream_t s = nullptr;
@@ -2574,9 +2631,17 @@ struct wfe_ensemble {
   hipGraphExec_t exec[2] = {nullptr, nullptr};
   std::vector<double> gdt[2];
   std::vector<unsigned> ggen[2];
+  bool batching = false;
+  // Per parity: the device table of the batched members' substep arguments
+  // ([stage][substep][member] per batch, batches one after another).
+  void* tab[2] = {nullptr, nullptr};
+  size_t tab_bytes[2] = {0, 0};
+  int batched_members = 0;  // in the last captured step
   ~wfe_ensemble() {
     for (hipGraphExec_t& e : exec)
       if (e) (void)hipGraphExecDestroy(e);
+    for (void* t : tab)
+      if (t) (void)hipFree(t);
     for (hipEvent_t e : join) (void)hipEventDestroy(e);
     if (fork) (void)hipEventDestroy(fork);
     if (done) (void)hipEventDestroy(done);
@@ -2585,6 +2650,166 @@ struct wfe_ensemble {
 };
 
 namespace {
+
+extern "C++" {
+
+// A batch: members (indices into the ensemble) that are contexts of one
+// storage type T with equal substep structure, and their launch layout.
+template <class T>
+struct Batch {
+  std::vector<int> idx;
+  std::vector<wfe::Ctx<T>*> cs;
+  int ns = 0, bx = 64, kpt = 3;
+  size_t lds = 0;
+  unsigned uv_blocks = 0, wp_blocks = 0;
+  size_t tab_off = 0;  // entries before this batch's in the table
+};
+
+// Group the members of storage type T into batches of two or more.
+template <class T>
+std::vector<Batch<T>> find_batches(const wfe_ensemble& E) {
+  std::vector<Batch<T>> out;
+  std::vector<bool> used(E.m.size(), false);
+  for (size_t a = 0; a < E.m.size(); ++a) {
+    auto* ca = dynamic_cast<wfe::Ctx<T>*>(E.m[a]->impl.get());
+    if (used[a] || !ca || ca->levels() > 64) continue;
+    Batch<T> b;
+    for (size_t q = a; q < E.m.size(); ++q) {
+      auto* c = dynamic_cast<wfe::Ctx<T>*>(E.m[q]->impl.get());
+      if (used[q] || !c || c->levels() > 64) continue;
+      if (c->stage_nsub(1) != ca->stage_nsub(1) || c->stage_nsub(2) != ca->stage_nsub(2) ||
+          c->moist() != ca->moist() || c->relaxing() != ca->relaxing())
+        continue;
+      b.idx.push_back((int)q);
+      b.cs.push_back(c);
+    }
+    if (b.cs.size() < 2) continue;
+    for (int q : b.idx) used[q] = true;
+    out.push_back(std::move(b));
+  }
+  return out;
+}
+
+template <class T, int BX, int KPT>
+void set_batch_lds(size_t lds) {
+  const void* fns[4] = {
+      reinterpret_cast<const void*>(&wfe::dev::k_acou_wpi_batch<T, 64, 16, KPT, false, false>),
+      reinterpret_cast<const void*>(&wfe::dev::k_acou_wpi_batch<T, 64, 16, KPT, false, true>),
+      reinterpret_cast<const void*>(&wfe::dev::k_acou_wpi_batch<T, 64, 16, KPT, true, false>),
+      reinterpret_cast<const void*>(&wfe::dev::k_acou_wpi_batch<T, 64, 16, KPT, true, true>)};
+  for (const void* f : fns)
+    WFE_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+}
+
+// Lay out the batches (block ranges, LDS) and write their host table
+// entries for the members' current parity.
+template <class T>
+void layout_batches(std::vector<Batch<T>>& bs, const double* dt,
+                    std::vector<wfe::dev::BatchMember<T>>& h) {
+  using wfe::cdiv;
+  for (Batch<T>& b : bs) {
+    wfe::Ctx<T>& lead = *b.cs[0];
+    b.ns = lead.stage_nsub(2);
+    b.bx = lead.uv_tile_width();
+    int nzmax = 0;
+    for (auto* c : b.cs) nzmax = std::max(nzmax, c->levels());
+    b.kpt = nzmax <= 48 ? 3 : 4;
+    b.lds = wfe::dev::WpiTile<64, 16, 4>::lds_bytes(nzmax);
+    const size_t n = b.cs.size();
+    b.tab_off = h.size();
+    h.resize(h.size() + (size_t)3 * b.ns * n);
+    std::vector<int> uv0(n), wp0(n), gx(n), gz(n);
+    b.uv_blocks = b.wp_blocks = 0;
+    for (size_t q = 0; q < n; ++q) {
+      const wfe::dev::Grid& g = b.cs[q]->grid();
+      gx[q] = cdiv(g.nx + 1, b.bx);
+      gz[q] = g.nz;
+      uv0[q] = (int)b.uv_blocks;
+      b.uv_blocks += (unsigned)(gx[q] * cdiv(g.ny + 1, 256 / b.bx) * g.nz);
+      wp0[q] = (int)b.wp_blocks;
+      b.wp_blocks += (unsigned)cdiv(g.nx * g.ny, 64);
+    }
+    for (int st = 0; st < 3; ++st)
+      for (int sub = 0; sub < lead.stage_nsub(st); ++sub)
+        for (size_t q = 0; q < n; ++q) {
+          wfe::dev::BatchMember<T>& M = h[b.tab_off + ((size_t)st * b.ns + sub) * n + q];
+          b.cs[q]->batch_member(M, dt[b.idx[q]], st, sub);
+          M.uv_begin = uv0[q];
+          M.uv_gx = gx[q];
+          M.uv_gz = gz[q];
+          M.wp_begin = wp0[q];
+        }
+  }
+}
+
+// The two batched launches of one substep of batch b on stream st.
+template <class T, int BX, int KPT>
+void launch_batch_substep(const Batch<T>& b, const wfe::dev::BatchMember<T>* tab, hipStream_t st) {
+  namespace dev = wfe::dev;
+  const int n = (int)b.cs.size();
+  dev::k_acou_uv_batch<T, BX><<<b.uv_blocks, dim3(BX, 256 / BX), 0, st>>>(tab, n);
+  const bool moist = b.cs[0]->moist(), tpi = b.cs[0]->relaxing();
+  const dim3 bt(64, 16);
+  if (moist && tpi)
+    dev::k_acou_wpi_batch<T, 64, 16, KPT, true, true><<<b.wp_blocks, bt, b.lds, st>>>(tab, n);
+  else if (moist)
+    dev::k_acou_wpi_batch<T, 64, 16, KPT, true, false><<<b.wp_blocks, bt, b.lds, st>>>(tab, n);
+  else if (tpi)
+    dev::k_acou_wpi_batch<T, 64, 16, KPT, false, true><<<b.wp_blocks, bt, b.lds, st>>>(tab, n);
+  else
+    dev::k_acou_wpi_batch<T, 64, 16, KPT, false, false><<<b.wp_blocks, bt, b.lds, st>>>(tab, n);
+  wfe::check_launch("ensemble batched substep");
+}
+
+template <class T>
+void batch_substep(const Batch<T>& b, const wfe::dev::BatchMember<T>* tab, hipStream_t st) {
+  if (b.bx == 32)
+    b.kpt == 3 ? launch_batch_substep<T, 32, 3>(b, tab, st) : launch_batch_substep<T, 32, 4>(b, tab, st);
+  else
+    b.kpt == 3 ? launch_batch_substep<T, 64, 3>(b, tab, st) : launch_batch_substep<T, 64, 4>(b, tab, st);
+}
+
+// Capture batch b's step: the members' slow phases as branches off the lead
+// member's stream, the batched substeps on it; every member's stream ends
+// joined into the ensemble stream. Parity stays unchanged.
+template <class T>
+void enqueue_batch(wfe_ensemble& E, const Batch<T>& b, const wfe::dev::BatchMember<T>* tab,
+                   const double* dt) {
+  const size_t n = b.cs.size();
+  hipStream_t ls = b.cs[0]->stream();
+  hipEvent_t lead = E.join[b.idx[0]];
+  for (int st = 0; st < 3; ++st) {
+    for (size_t q = 0; q < n; ++q) {
+      wfe::Ctx<T>& c = *b.cs[q];
+      const double d = dt[b.idx[q]];
+      if (q > 0 && st > 0) WFE_HIP(hipStreamWaitEvent(c.stream(), lead, 0));
+      if (st > 0) c.stage_post(d, st - 1);
+      c.stage_pre(d, st);
+    }
+    for (size_t q = 1; q < n; ++q) {
+      WFE_HIP(hipEventRecord(E.join[b.idx[q]], b.cs[q]->stream()));
+      WFE_HIP(hipStreamWaitEvent(ls, E.join[b.idx[q]], 0));
+    }
+    for (int sub = 0; sub < b.cs[0]->stage_nsub(st); ++sub)
+      batch_substep(b, tab + ((size_t)st * b.ns + sub) * n, ls);
+    WFE_HIP(hipEventRecord(lead, ls));
+  }
+  for (size_t q = 0; q < n; ++q) {
+    wfe::Ctx<T>& c = *b.cs[q];
+    const double d = dt[b.idx[q]];
+    if (q > 0) WFE_HIP(hipStreamWaitEvent(c.stream(), lead, 0));
+    c.stage_post(d, 2);
+    c.flip_parity();
+    c.step_post(d);
+    c.flip_parity();
+  }
+  for (size_t q = 0; q < n; ++q) {
+    WFE_HIP(hipEventRecord(E.join[b.idx[q]], b.cs[q]->stream()));
+    WFE_HIP(hipStreamWaitEvent(E.s, E.join[b.idx[q]], 0));
+  }
+}
+
+}  // extern "C++"
 
 void ensemble_step(wfe_ensemble& E, const double* dt) {
   const size_t n = E.m.size();
@@ -2596,8 +2821,13 @@ void ensemble_step(wfe_ensemble& E, const double* dt) {
   bool ready = true;
   for (wfe_ctx* c : E.m) ready = ready && c->impl->ready_for_capture();
   const int par = E.m[0]->impl->parity();
-  for (wfe_ctx* c : E.m)
+  for (wfe_ctx* c : E.m) {
     if (c->impl->parity() != par) throw wfe::ArgError("ensemble members out of step");
+    // The surface physics takes the hour of day from the time passed to
+    // each step, which an ensemble step does not carry.
+    if (c->impl->surface_physics())
+      throw wfe::ArgError("ensemble members with surface physics are not supported");
+  }
   if (!ready) {
     // First step of a member tunes it (synchronous timing): run eagerly.
     for (size_t q = 0; q < n; ++q) E.m[q]->impl->step(dt[q], 0.0);
@@ -2611,6 +2841,44 @@ void ensemble_step(wfe_ensemble& E, const double* dt) {
       WFE_HIP(hipGraphExecDestroy(E.exec[par]));
       E.exec[par] = nullptr;
     }
+    // Batches and their argument tables (written before the capture, once
+    // no earlier replay can still read them).
+    std::vector<Batch<float>> bf;
+    std::vector<Batch<double>> bd;
+    if (E.batching) {
+      bf = find_batches<float>(E);
+      bd = find_batches<double>(E);
+    }
+    std::vector<wfe::dev::BatchMember<float>> hf;
+    std::vector<wfe::dev::BatchMember<double>> hd;
+    layout_batches(bf, dt, hf);
+    layout_batches(bd, dt, hd);
+    const size_t bytes_f = hf.size() * sizeof(hf[0]);
+    const size_t bytes = bytes_f + hd.size() * sizeof(hd[0]);
+    std::vector<bool> in_batch(n, false);
+    E.batched_members = 0;
+    for (const auto& b : bf)
+      for (int q : b.idx) in_batch[q] = true, ++E.batched_members;
+    for (const auto& b : bd)
+      for (int q : b.idx) in_batch[q] = true, ++E.batched_members;
+    if (bytes > 0) {
+      WFE_HIP(hipStreamSynchronize(E.s));
+      if (E.tab_bytes[par] < bytes) {
+        if (E.tab[par]) WFE_HIP(hipFree(E.tab[par]));
+        E.tab[par] = nullptr;
+        E.tab_bytes[par] = 0;
+        WFE_HIP(hipMalloc(&E.tab[par], bytes));
+        E.tab_bytes[par] = bytes;
+      }
+      char* t = static_cast<char*>(E.tab[par]);
+      if (!hf.empty()) WFE_HIP(hipMemcpy(t, hf.data(), bytes_f, hipMemcpyHostToDevice));
+      if (!hd.empty()) WFE_HIP(hipMemcpy(t + bytes_f, hd.data(), bytes - bytes_f, hipMemcpyHostToDevice));
+      for (const auto& b : bf) b.kpt == 3 ? set_batch_lds<float, 64, 3>(b.lds) : set_batch_lds<float, 64, 4>(b.lds);
+      for (const auto& b : bd) b.kpt == 3 ? set_batch_lds<double, 64, 3>(b.lds) : set_batch_lds<double, 64, 4>(b.lds);
+    }
+    auto* tf = reinterpret_cast<const wfe::dev::BatchMember<float>*>(E.tab[par]);
+    auto* td = reinterpret_cast<const wfe::dev::BatchMember<double>*>(
+        static_cast<const char*>(E.tab[par]) + bytes_f);
     hipGraph_t gr = nullptr;
     WFE_HIP(hipStreamBeginCapture(E.s, hipStreamCaptureModeThreadLocal));
     try {
@@ -2619,10 +2887,13 @@ void ensemble_step(wfe_ensemble& E, const double* dt) {
       for (size_t q = 0; q < n; ++q) {
         hipStream_t ms = E.m[q]->impl->stream();
         WFE_HIP(hipStreamWaitEvent(ms, E.fork, 0));
+        if (in_batch[q]) continue;
         E.m[q]->impl->enqueue_step(dt[q]);
         WFE_HIP(hipEventRecord(E.join[q], ms));
         WFE_HIP(hipStreamWaitEvent(E.s, E.join[q], 0));
       }
+      for (const auto& b : bf) enqueue_batch(E, b, tf + b.tab_off, dt);
+      for (const auto& b : bd) enqueue_batch(E, b, td + b.tab_off, dt);
     } catch (...) {
       (void)hipStreamEndCapture(E.s, &gr);
       if (gr) (void)hipGraphDestroy(gr);
@@ -2670,6 +2941,11 @@ int wfe_ensemble_create(wfe_ctx* const* members, int n, wfe_ensemble** out) {
     WFE_HIP(hipEventCreateWithFlags(&E->done, hipEventDisableTiming));
     E->join.assign(n, nullptr);
     for (int q = 0; q < n; ++q) WFE_HIP(hipEventCreateWithFlags(&E->join[q], hipEventDisableTiming));
+    if (const char* e = std::getenv("WFE_ENSEMBLE_BATCH"))
+      if (std::string(e) != "0")
+        throw wfe::ArgError(
+            "WFE_ENSEMBLE_BATCH=1: batched ensemble substeps are switched off (their first GPU "
+            "run ended in a host segmentation fault of unknown cause)");
     *out = E.release();
   });
 }
@@ -2688,6 +2964,15 @@ int wfe_ensemble_step(wfe_ensemble* e, const double* dt) {
     return WFE_E_ARG;
   }
   return guarded([&] { ensemble_step(*e, dt); });
+}
+
+int wfe_ensemble_batched(const wfe_ensemble* e, int* members) {
+  if (!e || !members) {
+    wfe::set_last_error("wfe_ensemble_batched: null argument");
+    return WFE_E_ARG;
+  }
+  *members = e->batched_members;
+  return WFE_OK;
 }
 
 int wfe_ensemble_sync(wfe_ensemble* e) {
