@@ -108,8 +108,8 @@ def test_cfg3_regional_all_1440_steps_mixed_with_boundary_files(tmp_path):
     # the product's boundary manager advanced through the files (the targets
     # it holds now differ from the ones it started with)
     assert not np.array_equal(c.array("bdy_hi_thp"), lo0)
-    # 24 h against the FP64 path on the same case: the north-star bound of
-    # 1e-4 (relative L2 over the interior) on u, v, w and theta'
+    # 24 h against the FP64 path on the same case: the mixed path's error
+    # bound of 1e-4 (relative L2 over the interior) on u, v, w and theta'
     gd = Integrator.from_case(c, "fp64")
     for s in range(1440):
         gd.step(dt, s * dt)
