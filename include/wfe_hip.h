@@ -191,9 +191,10 @@ int wfe_adj_gradient_q(wfe_ctx* ctx, double dt, int nsteps, int ckpt_every,
  * holds every member's step as an independent branch, one launch per step.
  * Batched substeps (one launch of each acoustic substep kernel over every
  * member's grid) are switched off; WFE_ENSEMBLE_BATCH=1 makes
- * wfe_ensemble_create fail with WFE_E_ARG. A member with surface physics
- * (wfe_set_physics) makes wfe_ensemble_step fail with WFE_E_ARG: the
- * physics needs the time of each step, which the ensemble does not carry.
+ * wfe_ensemble_create fail with WFE_E_ARG. Members with surface physics
+ * (wfe_set_physics) need the time of each step: wfe_ensemble_step_at takes
+ * t[q], the time at the start of member q's step (as wfe_step's t), and
+ * wfe_ensemble_step fails with WFE_E_ARG for such an ensemble.
  * The first call runs each member's first (tuning) step eagerly. Work issued
  * afterwards on a member context (download, absmax, its own wfe_step) is
  * ordered after the ensemble step. Destroy the ensemble before its members. */
@@ -201,6 +202,7 @@ typedef struct wfe_ensemble wfe_ensemble;
 int wfe_ensemble_create(wfe_ctx* const* members, int n, wfe_ensemble** out);
 int wfe_ensemble_destroy(wfe_ensemble* e);
 int wfe_ensemble_step(wfe_ensemble* e, const double* dt);
+int wfe_ensemble_step_at(wfe_ensemble* e, const double* dt, const double* t);
 int wfe_ensemble_sync(wfe_ensemble* e);
 /* How many members the last captured ensemble step advanced in batched
  * substep launches (0 while batching is switched off). */

@@ -121,15 +121,29 @@ def test_driver_writes_the_surface_fields(tmp_path):
     assert np.isfinite(t2).all() and 200 < t2.min() and t2.max() < 330
 
 
-def test_ensembles_refuse_members_with_surface_physics(tmp_path):
-    """An ensemble step carries no time of day, which the surface physics
-    needs for each step; such members are refused rather than stepped with
-    a stale clock."""
+def test_ensembles_with_surface_physics_follow_the_clock(tmp_path):
+    """Members with surface physics in an ensemble, stepped with their times
+    (one member on another clock), equal the same members stepped alone, bit
+    for bit; without times the ensemble step is refused rather than run
+    with a stale hour of day."""
     from wfe_amd import Case, Ensemble, Integrator
     from wfe_amd._lib import WfeError
-    c = Case(_case(tmp_path, "local", hours=0))
-    members = [Integrator.from_case(c, "mixed") for _ in range(2)]
+    cases = [Case(_case(tmp_path, "local", hours=0)),
+             Case(_case(tmp_path, "nonlocal", nx=32, ny=24, nz=20, seed=5, hours=0))]
+    dts = [c.scalar("dt") for c in cases]
+    t0 = [9 * 3600.0, 15 * 3600.0]
+    alone = [Integrator.from_case(c, "mixed") for c in cases]
+    for s in range(6):
+        for gi, dt, t in zip(alone, dts, t0):
+            gi.step(dt, t + s * 1800.0)
+    members = [Integrator.from_case(c, "mixed") for c in cases]
     ens = Ensemble(members)
     with pytest.raises(WfeError, match="surface physics"):
-        ens.step([c.scalar("dt")] * 2)
+        ens.step(dts)
+    for s in range(6):
+        ens.step(dts, [t + s * 1800.0 for t in t0])
+    ens.sync()
+    for q, (a, b) in enumerate(zip(alone, members)):
+        for f in STATE + ("tsk", "t2m", "pblh"):
+            np.testing.assert_array_equal(b.get(f), a.get(f), err_msg=f"member {q} {f}")
     del ens

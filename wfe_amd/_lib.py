@@ -119,6 +119,7 @@ SIGNATURES = [
     ("wfe_ensemble_create", C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]),
     ("wfe_ensemble_destroy", C.c_int, [C.c_void_p]),
     ("wfe_ensemble_step", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    ("wfe_ensemble_step_at", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("wfe_ensemble_sync", C.c_int, [C.c_void_p]),
     ("wfe_ensemble_batched", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     ("wfe_copy_probe", C.c_int, [C.c_size_t, C.c_int, C.POINTER(C.c_double)]),

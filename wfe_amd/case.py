@@ -268,9 +268,16 @@ class Ensemble:
             lib().wfe_ensemble_destroy(self._h)
             self._h = C.c_void_p()
 
-    def step(self, dts) -> None:
-        arr = (C.c_double * len(self.members))(*[float(d) for d in dts])
-        check(lib().wfe_ensemble_step(self._h, arr), "wfe_ensemble_step")
+    def step(self, dts, times=None) -> None:
+        """One step of every member, member q by dts[q]; times[q] is the time
+        at the start of its step (needed by members with surface physics)."""
+        n = len(self.members)
+        arr = (C.c_double * n)(*[float(d) for d in dts])
+        if times is None:
+            check(lib().wfe_ensemble_step(self._h, arr), "wfe_ensemble_step")
+        else:
+            ts = (C.c_double * n)(*[float(t) for t in times])
+            check(lib().wfe_ensemble_step_at(self._h, arr, ts), "wfe_ensemble_step_at")
 
     def sync(self) -> None:
         check(lib().wfe_ensemble_sync(self._h), "wfe_ensemble_sync")

@@ -259,6 +259,9 @@ struct CtxBase {
   virtual unsigned generation() const = 0;
   virtual void enqueue_step(double dt) = 0;
   virtual bool surface_physics() const = 0;
+  // The surface physics' hour of day for a step from t to t + dt, written
+  // on the context's stream (a device scalar, so step graphs follow it).
+  virtual void set_clock(double dt, double t) = 0;
 };
 
 template <class T>
@@ -285,6 +288,7 @@ class Ctx final : public CtxBase {
     if (stream_) (void)hipStreamDestroy(stream_);
   }
   void step(double dt, double t) override;
+  void set_clock(double dt, double t) override;
   void set_physics(const wfe_sfc& sfc) override;
   void upload(int field, const double* h, size_t n) override;
   void download(int field, double* h, size_t n) override;
@@ -1614,13 +1618,18 @@ double Ctx<T>::kernel_bench(const std::string& name, int reps) {
 // time-n set S_[cur_], the stage estimate SE_, the next time level
 // S_[cur_^1], and ping-pong work buffers inside the acoustic loop.
 template <class T>
+void Ctx<T>::set_clock(double dt, double t) {
+  if (!phys_on_) return;
+  // hour of day at the end of the step (surface.cu:501-502)
+  double utc = phys_start_hour_ + (t + dt) / 3600;
+  utc = utc - 24 * std::floor(utc / 24);
+  dev::k_set_scalar<double><<<1, 64, 0, stream_>>>(utc_dev_, utc);
+  check_launch("set_utc");
+}
+
+template <class T>
 void Ctx<T>::step(double dt, double t) {
-  if (phys_on_) {  // hour of day at the end of the step (surface.cu:501-502)
-    double utc = phys_start_hour_ + (t + dt) / 3600;
-    utc = utc - 24 * std::floor(utc / 24);
-    dev::k_set_scalar<double><<<1, 64, 0, stream_>>>(utc_dev_, utc);
-    check_launch("set_utc");
-  }
+  set_clock(dt, t);
   // The first step tunes the substep method and the tendency and Kessler
   // shapes (synchronous timing), so it and any step under per-kernel timing
   // run eagerly.
@@ -2811,8 +2820,17 @@ void enqueue_batch(wfe_ensemble& E, const Batch<T>& b, const wfe::dev::BatchMemb
 
 }  // extern "C++"
 
-void ensemble_step(wfe_ensemble& E, const double* dt) {
+void ensemble_step(wfe_ensemble& E, const double* dt, const double* t) {
   const size_t n = E.m.size();
+  // The surface physics takes the hour of day from the time of each step:
+  // written on each member's stream before the ensemble stream is ordered
+  // after it, so the ensemble graph reads this step's value.
+  for (size_t q = 0; q < n; ++q) {
+    if (E.m[q]->impl->surface_physics() && !t)
+      throw wfe::ArgError("ensemble member with surface physics stepped without times "
+                          "(wfe_ensemble_step_at)");
+    if (t) E.m[q]->impl->set_clock(dt[q], t[q]);
+  }
   // Order the ensemble stream after each member's own pending work.
   for (size_t q = 0; q < n; ++q) {
     WFE_HIP(hipEventRecord(E.join[q], E.m[q]->impl->stream()));
@@ -2821,16 +2839,11 @@ void ensemble_step(wfe_ensemble& E, const double* dt) {
   bool ready = true;
   for (wfe_ctx* c : E.m) ready = ready && c->impl->ready_for_capture();
   const int par = E.m[0]->impl->parity();
-  for (wfe_ctx* c : E.m) {
+  for (wfe_ctx* c : E.m)
     if (c->impl->parity() != par) throw wfe::ArgError("ensemble members out of step");
-    // The surface physics takes the hour of day from the time passed to
-    // each step, which an ensemble step does not carry.
-    if (c->impl->surface_physics())
-      throw wfe::ArgError("ensemble members with surface physics are not supported");
-  }
   if (!ready) {
     // First step of a member tunes it (synchronous timing): run eagerly.
-    for (size_t q = 0; q < n; ++q) E.m[q]->impl->step(dt[q], 0.0);
+    for (size_t q = 0; q < n; ++q) E.m[q]->impl->step(dt[q], t ? t[q] : 0.0);
     return;
   }
   std::vector<unsigned> gen(n);
@@ -2963,7 +2976,15 @@ int wfe_ensemble_step(wfe_ensemble* e, const double* dt) {
     wfe::set_last_error("wfe_ensemble_step: null argument");
     return WFE_E_ARG;
   }
-  return guarded([&] { ensemble_step(*e, dt); });
+  return guarded([&] { ensemble_step(*e, dt, nullptr); });
+}
+
+int wfe_ensemble_step_at(wfe_ensemble* e, const double* dt, const double* t) {
+  if (!e || !dt || !t) {
+    wfe::set_last_error("wfe_ensemble_step_at: null argument");
+    return WFE_E_ARG;
+  }
+  return guarded([&] { ensemble_step(*e, dt, t); });
 }
 
 int wfe_ensemble_batched(const wfe_ensemble* e, int* members) {
