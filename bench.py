@@ -719,8 +719,8 @@ def main() -> int:
                     help="only the timed FP64 steps of the headline case (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is described, not measured)"}))
+        print(json.dumps({"error": "the benchmark runs on one GPU; --gpus must be 1 "
+                                   "(the multi-GPU sharding is a design, not built)"}))
         return 2
 
     import shutil
