@@ -92,3 +92,13 @@ def oracle_and_gpu(c: Case, precision: str, state: dict | None = None):
             om.set(f, a)
             gi.set(f, a)
     return om, gi
+
+
+def pin_shapes(monkeypatch) -> None:
+    """Contexts compared bit for bit run the same kernel shapes: each context
+    otherwise tunes its own on its first step by timing, and the substep
+    variants differ in rounding."""
+    monkeypatch.setenv("WFE_ACOUSTIC_KERNEL", "split-64x16")
+    monkeypatch.setenv("WFE_TENDENCY_LEVELS", "2")
+    monkeypatch.setenv("WFE_TENDENCY_TILE", "64")
+    monkeypatch.setenv("WFE_KESSLER_TILE", "32x16")

@@ -6,7 +6,7 @@ context against the FP64 check."""
 import numpy as np
 import pytest
 
-from helpers import case, perturbed_state
+from helpers import case, perturbed_state, pin_shapes
 
 pytestmark = pytest.mark.gpu
 
@@ -287,11 +287,12 @@ def test_relaxation_enters_the_linearisation(moist, interior):
     _dot_and_fd(c, gi, flds, fresh, 162)
 
 
-def test_adjoint_entries_leave_state_set_and_ensembles_intact():
+def test_adjoint_entries_leave_state_set_and_ensembles_intact(monkeypatch):
     """tl_step / ad_step / the gradient run forward steps internally; the
     context must come back in the same state set with the same values, so an
     ensemble built with a context that was never differentiated still steps
     in lock-step with it, and both match a fresh context stepped alone."""
+    pin_shapes(monkeypatch)
     from wfe_amd import Ensemble, Integrator
     from wfe_amd.adjoint import ad_step, box_weights, ke_gradient, tl_step
     c = dry_case()

@@ -23,7 +23,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from helpers import CASES, kitchen_sink, oracle_and_gpu, perturbed_state, rel_err
+from helpers import CASES, kitchen_sink, oracle_and_gpu, perturbed_state, pin_shapes, rel_err
 from oracle_lib import OracleCase
 
 pytestmark = pytest.mark.gpu
@@ -146,11 +146,12 @@ def test_regional_boundary_files_and_time_interpolation_match_the_oracle(tmp_pat
 
 
 def test_context_created_after_others_crossed_boundary_times_starts_from_the_first_files(
-        tmp_path):
+        tmp_path, monkeypatch):
     """Contexts of one case run one after another (as the benchmark runs its
     mixed and FP64 passes): a context created once another has crossed
     boundary times still starts from the first two boundary files, so it
     steps exactly like a context created before any stepping."""
+    pin_shapes(monkeypatch)
     from wfe_amd import Case, Integrator
     from wfe_amd.lam_case import write_lam_case
     ini = write_lam_case(tmp_path / "lam", nx=32, ny=24, nz=16, dx=12000.0, top=12000.0,

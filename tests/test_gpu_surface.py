@@ -10,7 +10,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-from helpers import rel_err
+from helpers import pin_shapes, rel_err
 from oracle_lib import OracleCase
 
 pytestmark = pytest.mark.gpu
@@ -90,6 +90,7 @@ def test_surface_physics_under_step_graphs_matches_eager_steps(tmp_path, monkeyp
     """The hour of day is a device scalar written before each step, so the
     replayed step graph follows the clock."""
     from wfe_amd import Case, Integrator
+    pin_shapes(monkeypatch)
     ini = _case(tmp_path, "nonlocal")
     c = Case(ini)
     dt = c.scalar("dt")
@@ -121,11 +122,12 @@ def test_driver_writes_the_surface_fields(tmp_path):
     assert np.isfinite(t2).all() and 200 < t2.min() and t2.max() < 330
 
 
-def test_ensembles_with_surface_physics_follow_the_clock(tmp_path):
+def test_ensembles_with_surface_physics_follow_the_clock(tmp_path, monkeypatch):
     """Members with surface physics in an ensemble, stepped with their times
     (one member on another clock), equal the same members stepped alone, bit
     for bit; without times the ensemble step is refused rather than run
     with a stale hour of day."""
+    pin_shapes(monkeypatch)
     from wfe_amd import Case, Ensemble, Integrator
     from wfe_amd._lib import WfeError
     cases = [Case(_case(tmp_path, "local", hours=0)),

@@ -307,7 +307,7 @@ __device__ __forceinline__ void uv_tile(const Grid& g, const Params& p, const Ba
 }
 
 template <class T, int BX = 64>
-__global__ void __launch_bounds__(256) k_acou_uv_tile(Grid g, Params p, Base<T> b, Metric<T> m,
+__global__ void __launch_bounds__(256, sizeof(T) == 4 ? 8 : 1) k_acou_uv_tile(Grid g, Params p, Base<T> b, Metric<T> m,
                                                       UVArgs<T> a) {
   uv_tile<T, BX>(g, p, b, m, a, xcd_block());
 }
